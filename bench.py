@@ -1,0 +1,348 @@
+#!/usr/bin/env python3
+"""Benchmark of the sieve + SW hot path on one MI355X.
+
+Workload (the headline step): the batched 2-bit Hamming sieve with
+certification over a 100 Mbp synthetic genome (80% uniform ACGT, 20% repeat
+families of 300 bp - 6 kbp copies at 85-99% identity), 2,000,000 reads of
+150 bp at 1% substitution, 16 candidate windows per read (true locus, then
+copies of the same repeat family, then random loci), window = L + 4w = 278
+(w = 32), mismatch limit 8.  One step = pack the 2M ASCII reads into bit
+planes on the device + one sieve launch over the 32M (read, window) jobs.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0).  Only --gpus 1 is supported.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "kira-ls-aligner_amd")
+
+# MI355X peaks: HBM3E 8 TB/s (published); int32 VALU = 256 CU x 64 lanes x 2.4 GHz.
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+def load_pkg():
+    name = "kira_ls_aligner_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+# --------------------------------------------------------------- workloads
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def make_genome(rng, total=100_000_000, repeat_frac=0.20):
+    """80% uniform ACGT + 20% repeat-family copies.  Returns (seq uint8, copies)
+    with copies = int64 array [n, 3] of (start, length, family)."""
+    target = int(total * repeat_frac)
+    copies_seq, fam_of = [], []
+    placed = 0
+    fam = 0
+    while placed < target:
+        clen = int(rng.integers(300, 6001))
+        cons = rng.integers(0, 4, clen, dtype=np.uint8)
+        for _ in range(int(rng.integers(2, 11))):
+            ident = rng.uniform(0.85, 0.99)
+            c = cons.copy()
+            sub = rng.random(clen) < (1 - ident)
+            c[sub] = (c[sub] + rng.integers(1, 4, int(sub.sum()), dtype=np.uint8)) % 4
+            # 1-bp indels at a tenth of the substitution rate, applied in one pass
+            n_indel = int(rng.poisson(clen * (1 - ident) * 0.1))
+            if n_indel:
+                where = np.unique(rng.integers(0, clen, n_indel))
+                is_del = rng.random(len(where)) < 0.5
+                keep = np.ones(clen, dtype=bool)
+                keep[where[is_del]] = False
+                ins_at = where[~is_del]
+                c = np.insert(c, ins_at, rng.integers(0, 4, len(ins_at), dtype=np.uint8))
+                keep = np.insert(keep, ins_at, True)
+                c = c[keep]
+            copies_seq.append(c)
+            fam_of.append(fam)
+            placed += len(c)
+            if placed >= target:
+                break
+        fam += 1
+    n_bg = total - placed
+    cuts = np.sort(rng.integers(0, n_bg, len(copies_seq)))
+    bg = rng.integers(0, 4, n_bg, dtype=np.uint8)
+    order = rng.permutation(len(copies_seq))
+    parts, meta, prev, pos = [], [], 0, 0
+    for k, ci in enumerate(order):
+        seg = bg[prev:cuts[k]]
+        parts.append(seg)
+        pos += len(seg)
+        parts.append(copies_seq[ci])
+        meta.append((pos, len(copies_seq[ci]), fam_of[ci]))
+        pos += len(copies_seq[ci])
+        prev = cuts[k]
+    parts.append(bg[prev:])
+    codes = np.concatenate(parts)
+    return ACGT[codes], np.array(meta, dtype=np.int64)
+
+
+def make_sieve_workload(n_reads=2_000_000, read_len=150, windows=16, band=32, sub_rate=0.01,
+                        genome_len=100_000_000, seed=2):
+    rng = np.random.default_rng(seed)
+    genome, copies = make_genome(rng, genome_len)
+    G = len(genome)
+    flank = 2 * band
+    W = read_len + 4 * band
+    pos = rng.integers(flank, G - read_len - flank - 1, n_reads)
+    reads = np.empty((n_reads, read_len), dtype=np.uint8)
+    chunk = 250_000
+    for s in range(0, n_reads, chunk):
+        e = min(n_reads, s + chunk)
+        idx = pos[s:e, None] + np.arange(read_len)[None, :]
+        r = genome[idx]
+        m = rng.integers(0, 10000, r.shape, dtype=np.uint16) < int(sub_rate * 10000)
+        codes = np.searchsorted(ACGT, r)
+        codes[m] = (codes[m] + rng.integers(1, 4, int(m.sum()), dtype=np.uint8)) % 4
+        reads[s:e] = ACGT[codes]
+    # windows: true locus, then copies of the same repeat family, then random loci
+    win = rng.integers(0, G - W, (n_reads, windows)).astype(np.int64)
+    win[:, 0] = pos - flank
+    starts = copies[:, 0]
+    ci = np.searchsorted(starts, pos, side="right") - 1
+    inside = (ci >= 0) & (pos + read_len <= starts[np.maximum(ci, 0)] + copies[np.maximum(ci, 0), 1])
+    fam_members = {}
+    for k, f in enumerate(copies[:, 2]):
+        fam_members.setdefault(int(f), []).append(k)
+    for r in np.nonzero(inside)[0]:
+        c = ci[r]
+        off = pos[r] - starts[c]
+        slot = 1
+        for other in fam_members[int(copies[c, 2])]:
+            if other == c or slot >= windows:
+                continue
+            s0 = starts[other] + off - flank
+            if 0 <= s0 <= G - W:
+                win[r, slot] = s0
+                slot += 1
+    return dict(genome=genome, reads=reads, win_start=win.reshape(-1),
+                read_len=read_len, W=W, windows=windows, n_reads=n_reads,
+                repeat_reads=int(inside.sum()))
+
+
+# --------------------------------------------------------------- CPU baseline
+_LUT = np.zeros(256, dtype=np.uint8)
+for _c, _v in zip(b"ACGTacgt", [0, 1, 2, 3, 0, 1, 2, 3]):
+    _LUT[_c] = _v
+
+
+def pack_rows(rows):
+    """Vectorised PackedDna::pack of equal-length rows (bitpacked.rs:57-86)."""
+    n, L = rows.shape
+    nb = (L + 3) // 4
+    codes = np.zeros((n, nb * 4), dtype=np.uint8)
+    codes[:, :L] = _LUT[rows]
+    c = codes.reshape(n, nb, 4)
+    return (c[:, :, 0] | (c[:, :, 1] << 2) | (c[:, :, 2] << 4) | (c[:, :, 3] << 6)).astype(np.uint8)
+
+
+def pre_shift_rows(bits):
+    """Vectorised pre_shift_into (bitpacked.rs:95-119): [n, 4, nb]."""
+    n, nb = bits.shape
+    out = np.empty((n, 4, nb), dtype=np.uint8)
+    out[:, 0] = bits
+    nxt = np.zeros_like(bits)
+    nxt[:, :-1] = bits[:, 1:]
+    for p in range(1, 4):
+        out[:, p] = (bits >> (2 * p)) | ((nxt << (8 - 2 * p)) & 0xFF)
+    return out
+
+
+def cpu_baseline_sieve(work, n_sample_reads, threads, limit):
+    """The reference CPU scan (mismatch_count_u64 over every shift,
+    bitpacked.rs:154-270) restated in C (the oracle), pthreads over jobs."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    L, W, K = work["read_len"], work["W"], work["windows"]
+    reads = work["reads"][:n_sample_reads]
+    n = n_sample_reads * K
+    starts = work["win_start"][:n]
+    texts = work["genome"][starts[:, None] + np.arange(W)[None, :]]
+    rb = pack_rows(reads)
+    tb = pre_shift_rows(pack_rows(texts))
+    nb = tb.shape[2]
+    lay = dict(n=n)
+    rp = np.zeros((n, O.READ_BYTES_MAX), dtype=np.uint8)
+    rp[:, :rb.shape[1]] = np.repeat(rb, K, axis=0)
+    lay["reads"] = rp.reshape(-1)
+    lay["read_lens"] = np.full(n, L, dtype=np.int32)
+    lay["ref_lens"] = np.full(n, W, dtype=np.int32)
+    lay["limits"] = np.full(n, limit, dtype=np.uint32)
+    lay["ref_phase_bytes"] = np.full(n, nb, dtype=np.int64)
+    lay["ref_offsets"] = (np.arange(n, dtype=np.int64) * nb)
+    lay["ref_flat"] = np.ascontiguousarray(tb.transpose(1, 0, 2)).reshape(-1)
+    lay["ref_bytes_per_phase"] = n * nb
+    O.backend_run_batch_layout(lay, threads=threads)  # warm-up
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        shifts, best, second = O.backend_run_batch_layout(lay, threads=threads)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), (shifts, best, second)
+
+
+# --------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--reads", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=131_072)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build targets one GPU; --gpus must be 1 "
+              "(the path shards with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    pkg = load_pkg()
+    if not pkg.is_available():
+        raise SystemExit("no GPU visible to libkira_hip")
+    t_gen = time.time()
+    work = make_sieve_workload(n_reads=args.reads)
+    t_gen = time.time() - t_gen
+    print(f"[bench] workload generated in {t_gen:.1f}s: {work['n_reads']} reads, "
+          f"{work['repeat_reads']} in repeats", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    L, W, K, n_reads = work["read_len"], work["W"], work["windows"], work["n_reads"]
+    n_jobs = n_reads * K
+    limit = 8
+    ctx = pkg.Context(0)
+    genome = pkg.Genome(ctx, work["genome"].tobytes())
+    stride = 160
+    rd = np.zeros((n_reads, stride), dtype=np.uint8)
+    rd[:, :L] = work["reads"]
+    reads = torch.from_numpy(rd).to(dev)
+    lens = torch.full((n_reads,), L, dtype=torch.int32, device=dev)
+    win_start = torch.from_numpy(work["win_start"]).to(dev)
+    win_len = torch.full((n_jobs,), W, dtype=torch.int32, device=dev)
+    planes = torch.empty((n_reads, 24), dtype=torch.int32, device=dev)
+    valid = torch.empty(n_reads, dtype=torch.uint8, device=dev)
+    out = pkg.SieveOutputs(*(torch.empty(n_jobs, dtype=torch.int32, device=dev)
+                             for _ in range(3)),
+                           torch.empty(n_jobs, dtype=torch.uint8, device=dev))
+    stream = torch.cuda.current_stream(dev)
+    cert_max = 4  # spectral_max_mismatches(150) = max(150*3/100, 2)
+
+    def step(ev=None):
+        pkg.sieve_genome(ctx, genome, reads, lens, win_start, win_len, jobs_per_read=K,
+                         limit_all=limit, cert_max_mism=cert_max, max_read_len=L, planes=planes,
+                         valid=valid, out=out, stream=stream)
+
+    def step_timed(evs):
+        # pack reads, then the sieve launch bracketed by events on the same stream
+        pkg.sieve_genome(ctx, genome, reads, lens, win_start[:0], win_len[:0],
+                         jobs_per_read=K, limit_all=limit, max_read_len=L, planes=planes,
+                         valid=valid, out=out, stream=stream)
+        evs[0].record(stream)
+        pkg.sieve_genome(ctx, genome, reads, lens, win_start, win_len, jobs_per_read=K,
+                         limit_all=limit, cert_max_mism=cert_max, max_read_len=L,
+                         planes=planes, valid=valid, out=out, stream=stream, skip_pack=True)
+        evs[1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kern_ms = []
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step_timed(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    ms_per_step = wall * 1000 / args.steps
+    kernel_ms = float(np.mean(kern_ms))
+    print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, sieve kernel "
+          f"{kernel_ms:.3f} ms", file=sys.stderr, flush=True)
+
+    flags = out.flags.cpu().numpy()
+    shifts = out.shift.cpu().numpy()
+    best = out.best.cpu().numpy()
+    second = out.second.cpu().numpy()
+    accepted = int(((flags & 2) != 0).sum())
+    certified = int(((flags & 4) != 0).sum())
+    true_hit = float((shifts.reshape(n_reads, K)[:, 0] == 2 * 32).mean())
+
+    # parity spot check of the timed output against the CPU oracle
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        ns = min(args.cpu_sample_reads, n_reads)
+        t_cpu, (cs, cb, csec) = cpu_baseline_sieve(work, ns, threads, limit)
+        m = ns * K
+        parity = bool(np.array_equal(cs, shifts[:m]) and np.array_equal(cb, best[:m]) and
+                      np.array_equal(csec, second[:m]))
+        cpu = {"value": ns / t_cpu, "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} reads x {K} windows = {m} scans of 150 bp vs 278 bp "
+                         f"(mismatch_count_u64 + best/second, packing excluded), "
+                         f"median of 3"}
+        print(f"[bench] cpu baseline {ns / t_cpu:.0f} reads/s on {threads} threads; "
+              f"parity={parity}", file=sys.stderr, flush=True)
+
+    n_shifts = W - L + 1
+    words = (L + 31) // 32
+    bytes_per_launch = n_reads * ((L + 3) // 4) + n_jobs * (((W + 3) // 4) + 12)
+    ops_per_launch = n_jobs * n_shifts * words * 4
+    gbs = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    tops = ops_per_launch / (kernel_ms * 1e-3) / 1e12
+    line = {
+        "metric": "reads/s + SW GCUPS on 1 MI355X vs host CPU; p50/p99 batch latency",
+        "value": n_reads / (ms_per_step * 1e-3),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "batched 2-bit sieve + certification: 100 Mbp genome (20% "
+                               "repeat families), 2M x 150 bp reads at 1% subs, 16 windows "
+                               "per read, W=278, limit 8",
+                   "reads": n_reads, "jobs": n_jobs, "window": W, "read_len": L,
+                   "parallelism": "1 GPU"},
+        "jobs_per_s": n_jobs / (ms_per_step * 1e-3),
+        "base_compares_per_s": n_jobs * n_shifts * L / (ms_per_step * 1e-3),
+        "sieve_kernel_ms": kernel_ms,
+        "accepted_jobs": accepted,
+        "certified_jobs": certified,
+        "true_locus_recall": true_hit,
+        "parity_vs_oracle_sample": parity,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+        "roofline_valu": {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
+                          "unit": "Tops/s", "frac": tops / VALU_PEAK_TOPS,
+                          "ops_per_launch": ops_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

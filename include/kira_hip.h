@@ -1,0 +1,146 @@
+/*
+ * kira_hip.h — C ABI of the MI355X (gfx950) sieve + Smith-Waterman backend
+ * for kira-ls-aligner.  Plain pointers and sizes only; no exceptions cross
+ * this boundary.  Every call returns a kira_status; the message of the last
+ * failure on a context is available from kira_ctx_last_error().
+ *
+ * Threading: a context owns one HIP stream and its device buffers and must
+ * be driven by one host thread at a time (the reference's dispatcher owns
+ * its CudaBackend on a single worker thread, src/cuda/dispatcher.rs:28-48).
+ * Host arrays are caller-owned; device memory is library-owned unless a
+ * *_dev entry point documents otherwise.
+ */
+#ifndef KIRA_HIP_H
+#define KIRA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors CudaError (src/cuda/mod.rs:17-33). */
+typedef enum {
+    KIRA_OK = 0,
+    KIRA_ERR_NOT_BUILT = 1,
+    KIRA_ERR_DRIVER = 2,
+    KIRA_ERR_RUNTIME = 3,
+    KIRA_ERR_NO_DEVICE = 4,
+    KIRA_ERR_READ_TOO_LONG = 5,
+    KIRA_ERR_INVALID_ARGUMENT = 6,
+    KIRA_ERR_OUT_OF_MEMORY = 7
+} kira_status;
+
+/* READ_BYTES_MAX — src/cuda/mod.rs:37: packed bytes of the longest read. */
+#define KIRA_READ_BYTES_MAX 64
+
+typedef struct kira_ctx kira_ctx;
+typedef struct kira_genome kira_genome;
+
+/* ------------------------------------------------------------------ context */
+
+/* is_available() — src/cuda/mod.rs:40-49 / backend.rs:38-40. */
+int kira_is_available(void);
+
+/* CudaBackend::new() — src/cuda/backend.rs:63-93. */
+kira_status kira_ctx_create(int device, kira_ctx **out);
+void kira_ctx_destroy(kira_ctx *ctx);
+const char *kira_ctx_last_error(const kira_ctx *ctx);
+/* HIP stream (hipStream_t) the context launches on.  set_stream makes the
+ * context launch on a caller-owned stream instead (NULL restores its own). */
+void *kira_ctx_stream(kira_ctx *ctx);
+void kira_ctx_set_stream(kira_ctx *ctx, void *stream);
+
+/* CudaBackend::release_buffers() — src/cuda/backend.rs:252-261. */
+void kira_release_buffers(kira_ctx *ctx);
+
+/* ------------------------------------------------------------------ sieve */
+
+/*
+ * CudaBackend::run_batch — src/cuda/backend.rs:96-249, host arrays in the
+ * reference's SoA launch layout (src/cuda/spectral.cu:23-39).
+ *   reads_packed      n x read_bytes_max bytes, 2-bit packed (A=00 C=01 G=10 T=11,
+ *                     base k of byte i at bits 2k..2k+1)
+ *   read_packed_len   packed byte length of each read (the Rust Vec length);
+ *                     any value > KIRA_READ_BYTES_MAX fails the whole batch
+ *                     with KIRA_ERR_READ_TOO_LONG, like the reference
+ *   read_nucs         nucleotides per read
+ *   ref_shifted_flat  4 bit-phases concatenated phase-major; only phase 0
+ *                     is read (the kernel shifts on the fly)
+ *   ref_offsets       byte offset of each job's window inside one phase
+ *   ref_nucs          nucleotides per window
+ *   max_mismatches    accept threshold; the limit used is min(it, read_nucs)
+ * Outputs (n and 2n int32): out_shifts[i] = best shift or -1 if rejected;
+ * out_mismatches[i] = best count or INT32_MAX if rejected;
+ * out_mismatches[n+i] = second-best count at a distinct shift or INT32_MAX.
+ * read_nucs == 0 or ref_nucs < read_nucs gives (-1, INT32_MAX, INT32_MAX).
+ * n == 0 returns KIRA_OK without touching the device.
+ */
+kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_packed,
+                                 int32_t read_bytes_max, const int32_t *read_packed_len,
+                                 const int32_t *read_nucs, const uint8_t *ref_shifted_flat,
+                                 int64_t ref_bytes_per_phase, const int64_t *ref_offsets,
+                                 const int32_t *ref_nucs, const uint32_t *max_mismatches,
+                                 int32_t *out_shifts, int32_t *out_mismatches);
+
+/*
+ * Resident packed reference (no counterpart in the reference, which re-packs
+ * and re-uploads every window per batch, src/pipeline/stage4_alignment.rs:1280-1303).
+ * The ASCII sequence is uploaded once and packed on the device into two bit
+ * planes (low/high code bit, 32 bases per uint32) plus a non-ACGT bitmap.
+ */
+kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
+                               kira_genome **out);
+void kira_genome_destroy(kira_genome *g);
+int64_t kira_genome_len(const kira_genome *g);
+
+/*
+ * Read batch packed on the device from ASCII.  reads: n x stride bytes,
+ * read_lens[i] <= 256 bases.  planes_out (device, n x 24 uint32: 8 low-plane
+ * words, 8 high-plane words, 8 lower-case flag words) and valid_out (device, n bytes, 1 iff every
+ * base is A/C/G/T in either case — PackedDna::valid, bitpacked.rs:29-33).
+ * All pointers are device pointers; runs on the context stream.
+ */
+kira_status kira_reads_pack_dev(kira_ctx *ctx, int32_t n, const uint8_t *reads, int32_t stride,
+                                const int32_t *read_lens, uint32_t *planes_out,
+                                uint8_t *valid_out);
+
+/* Batched sieve job description.  All pointers are DEVICE pointers. */
+typedef struct {
+    int32_t n_jobs;
+    int32_t max_read_len;        /* longest read of the batch (<= 256 bases); sets
+                                    the kernel's read-word count.  A longer read
+                                    makes its jobs undispatched (flags 0). */
+    const uint32_t *read_planes; /* n_reads x 24, from kira_reads_pack_dev */
+    const int32_t *read_lens;    /* per read */
+    const uint8_t *read_valid;   /* per read, may be NULL (all valid) */
+    const int32_t *job_read;     /* per job read index; NULL -> job / jobs_per_read */
+    int32_t jobs_per_read;
+    const int64_t *win_start;    /* per job, base offset into the genome */
+    const int32_t *win_len;      /* per job window length in bases */
+    const int32_t *limit;        /* per job accept limit; NULL -> limit_all */
+    int32_t limit_all;
+    /* outputs */
+    int32_t *out_shift;  /* best shift or -1 (rejected / skipped) */
+    int32_t *out_best;   /* best count or INT32_MAX */
+    int32_t *out_second; /* second-best count or INT32_MAX */
+    uint8_t *out_flags;  /* may be NULL: bit0 dispatched (read and window all
+                            ACGT, S9), bit1 best <= limit, bit2 certified */
+    /* certification (spectral_hit_is_certified, src/alignment/mod.rs:688-762):
+       cert_max_mism < 0 disables it */
+    int32_t cert_max_mism;
+    int32_t cert_strict;
+} kira_sieve_batch;
+
+/* One launch of the sieve over a batch against a resident genome, on the
+ * context stream (asynchronous). */
+kira_status kira_sieve_genome_dev(kira_ctx *ctx, const kira_genome *g,
+                                  const kira_sieve_batch *batch);
+
+/* Synchronise the context stream. */
+kira_status kira_ctx_sync(kira_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

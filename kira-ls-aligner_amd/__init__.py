@@ -1,0 +1,382 @@
+"""Python host mirror of kira-ls-aligner's GPU backend API over libkira_hip.so.
+
+The shapes follow the reference's `crate::cuda` module (src/cuda/mod.rs,
+backend.rs, dispatcher.rs): `is_available()`, `HipBackend` (CudaBackend),
+`HipJob` / `HipResult` (CudaJob / CudaResult), `HipError` (CudaError) and the
+`dispatcher` start / dispatch / is_active / stop functions.  Everything runs
+through the C ABI declared in include/kira_hip.h; there is no CPU fallback:
+a missing library or device raises.
+
+The directory name is not a valid Python identifier, so it is loaded with
+importlib from this file's path (see bench.py / tests/kira_pkg.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import queue
+import threading
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkira_hip.so")
+READ_BYTES_MAX = 64  # src/cuda/mod.rs:37
+I32_MAX = 2 ** 31 - 1
+
+KIRA_OK = 0
+_STATUS = {1: "NotBuilt", 2: "Driver", 3: "Runtime", 4: "NoDevice", 5: "ReadTooLong",
+           6: "InvalidArgument", 7: "OutOfMemory"}
+
+
+class HipError(RuntimeError):
+    """CudaError — src/cuda/mod.rs:17-33 (kind is the variant name)."""
+
+    def __init__(self, kind: str, msg: str = ""):
+        super().__init__(f"{kind}: {msg}" if msg else kind)
+        self.kind = kind
+
+
+class SieveBatch(C.Structure):
+    """kira_sieve_batch (include/kira_hip.h); every pointer is a device pointer."""
+
+    _fields_ = [("n_jobs", C.c_int32), ("max_read_len", C.c_int32),
+                ("read_planes", C.c_void_p), ("read_lens", C.c_void_p),
+                ("read_valid", C.c_void_p), ("job_read", C.c_void_p),
+                ("jobs_per_read", C.c_int32), ("win_start", C.c_void_p),
+                ("win_len", C.c_void_p), ("limit", C.c_void_p), ("limit_all", C.c_int32),
+                ("out_shift", C.c_void_p), ("out_best", C.c_void_p),
+                ("out_second", C.c_void_p), ("out_flags", C.c_void_p),
+                ("cert_max_mism", C.c_int32), ("cert_strict", C.c_int32)]
+
+
+# Every symbol include/kira_hip.h declares, with (restype, argtypes).
+_P = C.c_void_p
+SIGNATURES = {
+    "kira_is_available": (C.c_int, []),
+    "kira_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "kira_ctx_destroy": (None, [_P]),
+    "kira_ctx_last_error": (C.c_char_p, [_P]),
+    "kira_ctx_stream": (C.c_void_p, [_P]),
+    "kira_ctx_set_stream": (None, [_P, _P]),
+    "kira_release_buffers": (None, [_P]),
+    "kira_sieve_run_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int64, _P,
+                                       _P, _P, _P, _P]),
+    "kira_genome_create": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_void_p)]),
+    "kira_genome_destroy": (None, [_P]),
+    "kira_genome_len": (C.c_int64, [_P]),
+    "kira_reads_pack_dev": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P]),
+    "kira_sieve_genome_dev": (C.c_int, [_P, _P, C.POINTER(SieveBatch)]),
+    "kira_ctx_sync": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libkira_hip.so or raise (never falls back to a CPU path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError("NotBuilt", f"{LIB_PATH} missing; build with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def is_available() -> bool:
+    """is_available() — src/cuda/mod.rs:40-49."""
+    try:
+        return bool(lib().kira_is_available())
+    except (HipError, OSError):
+        return False
+
+
+def _np_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class HipJob:
+    """CudaJob — src/cuda/backend.rs:13-26."""
+
+    read_packed: bytes
+    read_nucs: int
+    ref_shifted: Sequence[bytes]
+    ref_nucs: int
+    max_mismatches: int
+
+
+@dataclass
+class HipResult:
+    """CudaResult — src/cuda/backend.rs:28-35."""
+
+    shift: int
+    mismatches: int
+    second_mismatches: Optional[int]
+
+
+class Context:
+    """One kira_ctx (device + stream + buffers)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().kira_ctx_create(device, C.byref(h))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "Driver"), "kira_ctx_create failed")
+        self.h = h
+
+    def check(self, st: int, what: str):
+        if st != KIRA_OK:
+            msg = lib().kira_ctx_last_error(self.h)
+            raise HipError(_STATUS.get(st, "Runtime"), f"{what}: {msg.decode() if msg else ''}")
+
+    def set_stream(self, stream_handle: int):
+        lib().kira_ctx_set_stream(self.h, C.c_void_p(stream_handle))
+
+    def sync(self):
+        self.check(lib().kira_ctx_sync(self.h), "sync")
+
+    def release_buffers(self):
+        lib().kira_release_buffers(self.h)
+
+    def close(self):
+        if self.h:
+            lib().kira_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipBackend:
+    """CudaBackend — src/cuda/backend.rs:42-262, on libkira_hip."""
+
+    def __init__(self, device: int = 0):
+        self.ctx = Context(device)
+
+    def run_batch(self, jobs: Sequence[HipJob]) -> List[HipResult]:
+        """run_batch — src/cuda/backend.rs:96-249 (same errors, same result order)."""
+        n = len(jobs)
+        if n == 0:
+            return []
+        for j in jobs:
+            if len(j.read_packed) > READ_BYTES_MAX:
+                raise HipError("ReadTooLong", f"{len(j.read_packed)} > {READ_BYTES_MAX}")
+            l0 = len(j.ref_shifted[0])
+            if any(len(p) != l0 for p in j.ref_shifted):
+                raise HipError("Runtime", "ref_shifted phases have different lengths")
+        reads = np.zeros(n * READ_BYTES_MAX, dtype=np.uint8)
+        plen = np.empty(n, dtype=np.int32)
+        nucs = np.empty(n, dtype=np.int32)
+        ref_nucs = np.empty(n, dtype=np.int32)
+        limits = np.empty(n, dtype=np.uint32)
+        offsets = np.empty(n, dtype=np.int64)
+        sizes = np.array([len(j.ref_shifted[0]) for j in jobs], dtype=np.int64)
+        total = int(sizes.sum())
+        flat = np.zeros(4 * total + 8, dtype=np.uint8)
+        acc = 0
+        for i, j in enumerate(jobs):
+            rb = np.frombuffer(bytes(j.read_packed), dtype=np.uint8)
+            reads[i * READ_BYTES_MAX:i * READ_BYTES_MAX + len(rb)] = rb
+            plen[i] = len(rb)
+            nucs[i] = j.read_nucs
+            ref_nucs[i] = j.ref_nucs
+            limits[i] = min(j.max_mismatches, 2 ** 32 - 1)
+            for p in range(4):
+                ph = np.frombuffer(bytes(j.ref_shifted[p]), dtype=np.uint8)
+                flat[p * total + acc:p * total + acc + len(ph)] = ph
+            offsets[i] = acc
+            acc += int(sizes[i])
+        out_s = np.empty(n, dtype=np.int32)
+        out_m = np.empty(2 * n, dtype=np.int32)
+        st = lib().kira_sieve_run_batch(self.ctx.h, n, _np_ptr(reads), READ_BYTES_MAX,
+                                        _np_ptr(plen), _np_ptr(nucs), _np_ptr(flat), total,
+                                        _np_ptr(offsets), _np_ptr(ref_nucs), _np_ptr(limits),
+                                        _np_ptr(out_s), _np_ptr(out_m))
+        self.ctx.check(st, "run_batch")
+        return [HipResult(int(out_s[i]), int(out_m[i]),
+                          None if out_m[n + i] == I32_MAX else int(out_m[n + i]))
+                for i in range(n)]
+
+    def release_buffers(self):
+        self.ctx.release_buffers()
+
+
+class Genome:
+    """A packed reference resident on the device (kira_genome_create)."""
+
+    def __init__(self, ctx: Context, seq: bytes):
+        self.ctx = ctx
+        buf = np.frombuffer(seq, dtype=np.uint8)
+        h = C.c_void_p()
+        ctx.check(lib().kira_genome_create(ctx.h, _np_ptr(buf), len(seq), C.byref(h)),
+                  "genome_create")
+        self.h = h
+        self.length = len(seq)
+
+    def close(self):
+        if self.h:
+            lib().kira_genome_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_reads_dev(ctx: Context, n: int, reads_ptr: int, stride: int, lens_ptr: int,
+                   planes_ptr: int, valid_ptr: int):
+    """kira_reads_pack_dev on device pointers (ints)."""
+    ctx.check(lib().kira_reads_pack_dev(ctx.h, n, C.c_void_p(reads_ptr), stride,
+                                        C.c_void_p(lens_ptr), C.c_void_p(planes_ptr),
+                                        C.c_void_p(valid_ptr) if valid_ptr else None),
+              "reads_pack")
+
+
+def sieve_genome_dev(ctx: Context, genome: Genome, batch: SieveBatch):
+    ctx.check(lib().kira_sieve_genome_dev(ctx.h, genome.h, C.byref(batch)), "sieve_genome")
+
+
+# ------------------------------------------------------------ dispatcher
+class _Dispatcher:
+    """dispatcher — src/cuda/dispatcher.rs:13-104: one worker thread owns the
+    backend; callers block on a private reply queue; any error -> None."""
+
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.tx: Optional[queue.Queue] = None
+        self.thread: Optional[threading.Thread] = None
+
+    def start(self, device: int = 0):
+        with self.lock:
+            if self.tx is not None:
+                return
+            tx: queue.Queue = queue.Queue()
+
+            def worker():
+                try:
+                    backend = HipBackend(device)
+                except HipError as e:
+                    err = HipError("Driver", f"GPU worker thread is dead: {e}")
+                    while True:
+                        item = tx.get()
+                        if item is None:
+                            return
+                        item[1].put(err)
+                while True:
+                    item = tx.get()
+                    if item is None:
+                        backend.release_buffers()
+                        return
+                    jobs, reply = item
+                    try:
+                        reply.put(backend.run_batch(jobs))
+                    except HipError as e:
+                        reply.put(e)
+
+            self.tx = tx
+            self.thread = threading.Thread(target=worker, name="kira-hip-dispatch", daemon=True)
+            self.thread.start()
+
+    def dispatch(self, jobs: List[HipJob]) -> Optional[List[HipResult]]:
+        if not jobs:
+            return []
+        with self.lock:
+            if self.tx is None:
+                return None
+            reply: queue.Queue = queue.Queue()
+            self.tx.put((jobs, reply))
+        res = reply.get()
+        return None if isinstance(res, HipError) else res
+
+    def is_active(self) -> bool:
+        with self.lock:
+            return self.tx is not None
+
+    def stop(self):
+        with self.lock:
+            tx, th = self.tx, self.thread
+            self.tx, self.thread = None, None
+        if tx is not None:
+            tx.put(None)
+            th.join()
+
+
+dispatcher = _Dispatcher()
+
+
+# ------------------------------------------------------------ torch plumbing
+def _dev_ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+class SieveOutputs:
+    def __init__(self, shift, best, second, flags):
+        self.shift, self.best, self.second, self.flags = shift, best, second, flags
+
+
+def sieve_genome(ctx: Context, genome: Genome, reads, read_lens, win_start, win_len, *,
+                 job_read=None, jobs_per_read: int = 1, limit=None, limit_all: int = 0,
+                 cert_max_mism: int = -1, cert_strict: bool = False, max_read_len=None,
+                 planes=None, valid=None, out: Optional[SieveOutputs] = None,
+                 stream=None, skip_pack: bool = False) -> SieveOutputs:
+    """Batched sieve + certification against a resident genome.
+
+    reads: uint8 CUDA tensor [n_reads, stride] (ASCII); read_lens int32 [n_reads];
+    win_start int64 [n_jobs]; win_len int32 [n_jobs]; job_read int32 [n_jobs] or
+    None (job // jobs_per_read); limit int32 [n_jobs] or None (limit_all).
+    Launches on `stream` (a torch.cuda.Stream, default: the current stream).
+    skip_pack reuses `planes`/`valid` packed by an earlier call."""
+    import torch
+    dev = reads.device
+    n_reads, stride = reads.shape
+    n_jobs = int(win_start.shape[0])
+    if stream is None:
+        stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    if planes is None:
+        planes = torch.empty((n_reads, 24), dtype=torch.int32, device=dev)
+    if valid is None:
+        valid = torch.empty(n_reads, dtype=torch.uint8, device=dev)
+    if not skip_pack:
+        pack_reads_dev(ctx, n_reads, _dev_ptr(reads), stride, _dev_ptr(read_lens),
+                       _dev_ptr(planes), _dev_ptr(valid))
+    if out is None:
+        out = SieveOutputs(torch.empty(n_jobs, dtype=torch.int32, device=dev),
+                           torch.empty(n_jobs, dtype=torch.int32, device=dev),
+                           torch.empty(n_jobs, dtype=torch.int32, device=dev),
+                           torch.empty(n_jobs, dtype=torch.uint8, device=dev))
+    if max_read_len is None:
+        max_read_len = int(read_lens.max().item()) if n_reads else 1
+    b = SieveBatch()
+    b.n_jobs = n_jobs
+    b.max_read_len = int(max_read_len)
+    b.read_planes = _dev_ptr(planes)
+    b.read_lens = _dev_ptr(read_lens)
+    b.read_valid = _dev_ptr(valid)
+    b.job_read = _dev_ptr(job_read) if job_read is not None else None
+    b.jobs_per_read = jobs_per_read
+    b.win_start = _dev_ptr(win_start)
+    b.win_len = _dev_ptr(win_len)
+    b.limit = _dev_ptr(limit) if limit is not None else None
+    b.limit_all = limit_all
+    b.out_shift = _dev_ptr(out.shift)
+    b.out_best = _dev_ptr(out.best)
+    b.out_second = _dev_ptr(out.second)
+    b.out_flags = _dev_ptr(out.flags)
+    b.cert_max_mism = cert_max_mism
+    b.cert_strict = int(cert_strict)
+    sieve_genome_dev(ctx, genome, b)
+    return out

@@ -1,0 +1,73 @@
+// Internal declarations shared by the HIP translation units of libkira_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "kira_hip.h"
+
+namespace kira {
+
+// Genome / window text as bit planes: base p lives at bit (p & 31) of word
+// (p >> 5) + kPadFront of each plane.  lo/hi = the two bits of the 2-bit code
+// (A=00 C=01 G=10 T=11, bitpacked.rs:12-20); inv = not A/C/G/T; lc = lower
+// case A/C/G/T (only byte-level certification needs it).
+constexpr int kPadFront = 2;
+constexpr int kPadBack = 16;
+
+struct Planes {
+    uint32_t *lo = nullptr, *hi = nullptr, *inv = nullptr, *lc = nullptr;
+    int64_t n_bases = 0;
+    int64_t n_words = 0;  // words per plane including padding
+};
+
+// Read planes: 24 words per read — lo[8], hi[8], lc[8] (reads <= 256 bases).
+constexpr int kReadWords = 24;
+constexpr int kMaxReadBases = 256;
+
+struct SieveLaunch {
+    int32_t n_jobs;
+    const uint32_t *read_planes;
+    const int32_t *read_lens;
+    const uint8_t *read_valid;
+    const int32_t *job_read;
+    int32_t jobs_per_read;
+    const int64_t *win_start;
+    const int32_t *win_len;
+    const int32_t *limit;
+    int32_t limit_all;
+    int32_t *out_shift, *out_best, *out_second;
+    uint8_t *out_flags;
+    int32_t cert_max_mism, cert_strict;
+    const uint32_t *g_lo, *g_hi, *g_inv, *g_lc;
+    int64_t g_bases;
+};
+
+hipError_t launch_pack_ascii_planes(const uint8_t *seq, int64_t n, Planes p, hipStream_t s);
+hipError_t launch_pack_2bit_planes(const uint8_t *packed, int64_t n_bytes, Planes p,
+                                   hipStream_t s);
+hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stride,
+                                   const int32_t *lens, uint32_t *planes, uint8_t *valid,
+                                   hipStream_t s);
+hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stride,
+                                  const int32_t *lens, uint32_t *planes, hipStream_t s);
+hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
+
+}  // namespace kira
+
+struct kira_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;      // stream launches go to
+    hipStream_t own_stream = nullptr;  // the context's own stream
+    std::string err;
+    // geometric-growth device scratch for the reference-layout batch entry
+    void *buf[16] = {};
+    size_t cap[16] = {};
+};
+
+struct kira_genome {
+    kira_ctx *ctx = nullptr;
+    kira::Planes planes;
+};

@@ -1,0 +1,466 @@
+// Spectral Sieve on gfx950: exhaustive multi-shift 2-bit Hamming with best and
+// second-best shift (src/alignment/bitpacked.rs:154-199, src/cuda/spectral.cu:136-213)
+// plus the ungapped certification of the best hit (src/alignment/mod.rs:688-762).
+//
+// Layout: reads and window text are bit planes (32 bases per uint32, one plane
+// per code bit), so one 32-base compare is xor, xor, or, popcount.  One lane
+// evaluates one (read, window) job.  Shifts are swept in tiles of 128: the
+// tile's window words are aligned once, then for each in-word offset b the
+// window is funnel-shifted once and compared at the four word offsets a of
+// the tile (shift = 32a + b), so the shift costs amortise over four compares.
+// Best/second are kept as packed ranks (mismatches << 16 | shift-in-tile) so
+// ties resolve to the lowest shift exactly as the reference's strict-< scan.
+#include "internal.hpp"
+
+namespace kira {
+namespace {
+
+__device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+}
+
+// t | (a ^ b) in one v_bitop3_b32 (truth table over S0=0xF0, S1=0xCC, S2=0xAA).
+__device__ __forceinline__ uint32_t or_xor(uint32_t t, uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_bitop3_b32(t, a, b, 0xF6);
+}
+
+// Median of three (v_med3_u32).
+__device__ __forceinline__ uint32_t med3u(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t lo = x < y ? x : y, hi = x > y ? x : y;
+    uint32_t t = lo > z ? lo : z;
+    return t < hi ? t : hi;
+}
+
+// Base code of an ASCII byte and its flags (bitpacked.rs:12-20, 88-91).
+__device__ __forceinline__ void classify(uint8_t c, uint32_t &lo, uint32_t &hi, uint32_t &inv,
+                                         uint32_t &lc) {
+    uint8_t u = c & 0xDF;  // ASCII upper
+    bool acgt = (u == 'A' || u == 'C' || u == 'G' || u == 'T');
+    bool letter = acgt && (c >= 'a');
+    uint32_t code = (u == 'C') ? 1u : (u == 'G') ? 2u : (u == 'T') ? 3u : 0u;
+    if (!acgt) code = 0;
+    lo = code & 1u;
+    hi = code >> 1;
+    inv = acgt ? 0u : 1u;
+    lc = letter ? 1u : 0u;
+}
+
+// One thread per 32-base word of an ASCII sequence.
+__global__ void pack_ascii_planes_kernel(const uint8_t *__restrict__ seq, int64_t n,
+                                         Planes p) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= p.n_words) return;
+    int64_t base = (w - kPadFront) * 32;
+    uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
+    if (w >= kPadFront) {
+        for (int k = 0; k < 32; k++) {
+            int64_t q = base + k;
+            if (q >= n) break;
+            uint32_t a, b, c, d;
+            classify(seq[q], a, b, c, d);
+            lo |= a << k;
+            hi |= b << k;
+            inv |= c << k;
+            lc |= d << k;
+        }
+    }
+    p.lo[w] = lo;
+    p.hi[w] = hi;
+    if (p.inv) p.inv[w] = inv;
+    if (p.lc) p.lc[w] = lc;
+}
+
+// Split the even/odd bits of a 2-bit-interleaved word (16 bases) into two
+// 16-bit planes.
+__device__ __forceinline__ uint32_t even_bits(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t load_u32_bytes(const uint8_t *p, int64_t avail) {
+    uint32_t v = 0;
+    for (int k = 0; k < 4; k++)
+        if (k < avail) v |= (uint32_t)p[k] << (8 * k);
+    return v;
+}
+
+// One thread per 32-base word of a 2-bit packed byte stream (phase 0 of the
+// reference's pre-shifted windows).
+__global__ void pack_2bit_planes_kernel(const uint8_t *__restrict__ packed, int64_t n_bytes,
+                                        Planes p) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= p.n_words) return;
+    uint32_t lo = 0, hi = 0;
+    if (w >= kPadFront) {
+        int64_t b = (w - kPadFront) * 8;
+        if (b < n_bytes) {
+            uint32_t x0 = load_u32_bytes(packed + b, n_bytes - b);
+            uint32_t x1 = b + 4 < n_bytes ? load_u32_bytes(packed + b + 4, n_bytes - b - 4) : 0;
+            lo = even_bits(x0) | (even_bits(x1) << 16);
+            hi = even_bits(x0 >> 1) | (even_bits(x1 >> 1) << 16);
+        }
+    }
+    p.lo[w] = lo;
+    p.hi[w] = hi;
+    if (p.inv) p.inv[w] = 0;
+    if (p.lc) p.lc[w] = 0;
+}
+
+// One thread per (read, plane word).
+__global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ reads,
+                                        int32_t stride, const int32_t *__restrict__ lens,
+                                        uint32_t *__restrict__ planes,
+                                        uint8_t *__restrict__ valid) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * 8) return;
+    int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
+    int32_t L = lens[r];
+    uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
+    const uint8_t *src = reads + (int64_t)r * stride;
+    for (int b = 0; b < 32; b++) {
+        int32_t q = k * 32 + b;
+        if (q >= L) break;
+        uint32_t a0, b0, c0, d0;
+        classify(src[q], a0, b0, c0, d0);
+        lo |= a0 << b;
+        hi |= b0 << b;
+        inv |= c0 << b;
+        lc |= d0 << b;
+    }
+    uint32_t *dst = planes + (int64_t)r * kReadWords;
+    dst[k] = lo;
+    dst[8 + k] = hi;
+    dst[16 + k] = lc;
+    if (valid) {
+        // 8 consecutive lanes hold one read; reduce their inv flags.
+        uint32_t f = inv != 0 ? 1u : 0u;
+        f |= __shfl_xor(f, 1, 8);
+        f |= __shfl_xor(f, 2, 8);
+        f |= __shfl_xor(f, 4, 8);
+        if (k == 0) valid[r] = (L <= kMaxReadBases && f == 0) ? 1 : 0;
+    }
+}
+
+// Reads already 2-bit packed (reference layout): one thread per (read, word).
+__global__ void pack_reads_2bit_kernel(int32_t n, const uint8_t *__restrict__ packed,
+                                       int32_t stride, const int32_t *__restrict__ lens,
+                                       uint32_t *__restrict__ planes) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * 8) return;
+    int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
+    int32_t L = lens[r];
+    int32_t nbytes = (L + 3) / 4;
+    const uint8_t *src = packed + (int64_t)r * stride;
+    int64_t b = (int64_t)k * 8;
+    uint32_t lo = 0, hi = 0;
+    if (b < nbytes) {
+        uint32_t x0 = load_u32_bytes(src + b, nbytes - b);
+        uint32_t x1 = b + 4 < nbytes ? load_u32_bytes(src + b + 4, nbytes - b - 4) : 0;
+        lo = even_bits(x0) | (even_bits(x1) << 16);
+        hi = even_bits(x0 >> 1) | (even_bits(x1 >> 1) << 16);
+    }
+    // bases at or past L are padding: keep them zero
+    int32_t keep = L - k * 32;
+    uint32_t m = keep >= 32 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << keep) - 1u));
+    uint32_t *dst = planes + (int64_t)r * kReadWords;
+    dst[k] = lo & m;
+    dst[8 + k] = hi & m;
+    dst[16 + k] = 0;
+}
+
+__device__ __forceinline__ uint32_t range_mask(int k, int lo, int hi) {
+    // bits [lo, hi) of the 32-bit word k of a bit vector
+    int a = lo - 32 * k, b = hi - 32 * k;
+    a = a < 0 ? 0 : (a > 32 ? 32 : a);
+    b = b < 0 ? 0 : (b > 32 ? 32 : b);
+    if (b <= a) return 0u;
+    uint32_t hm = b == 32 ? 0xFFFFFFFFu : ((1u << b) - 1u);
+    uint32_t lm = a == 32 ? 0xFFFFFFFFu : ((1u << a) - 1u);
+    return hm & ~lm;
+}
+
+// Window text words starting at absolute base position p (p >= -64).
+template <int NW>
+__device__ __forceinline__ void window_words(const uint32_t *__restrict__ plane, int64_t p,
+                                             uint32_t (&out)[NW]) {
+    int64_t q = p + (int64_t)kPadFront * 32;
+    int64_t w0 = q >> 5;
+    uint32_t s = (uint32_t)(q & 31);
+    uint32_t prev = plane[w0];
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        uint32_t nx = plane[w0 + k + 1];
+        out[k] = funnel(nx, prev, s);
+        prev = nx;
+    }
+}
+
+// Byte mismatches between the read and the window at relative offset d from
+// the best shift, restricted to read positions [lo, hi).
+template <int RW>
+__device__ __forceinline__ int masked_mismatch(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
+                                               const uint32_t (&rc)[RW], const SieveLaunch &a,
+                                               int64_t pos, int lo, int hi) {
+    uint32_t wl[RW], wh[RW], wc[RW] = {};
+    window_words<RW>(a.g_lo, pos, wl);
+    window_words<RW>(a.g_hi, pos, wh);
+    if (a.g_lc) window_words<RW>(a.g_lc, pos, wc);
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < RW; k++) {
+        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (a.g_lc ? (rc[k] ^ wc[k]) : rc[k]);
+        m += __popc(d & range_mask(k, lo, hi));
+    }
+    return m;
+}
+
+// spectral_hit_is_certified (src/alignment/mod.rs:688-762) on bit planes.
+// Bytes are A/C/G/T in either case here (the job was dispatched), so byte
+// inequality = code inequality or case inequality.
+template <int RW>
+__device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
+                        const uint32_t (&rc)[RW], const SieveLaunch &a, int64_t s0, int L,
+                        int shift, uint32_t best, uint32_t second) {
+    if ((int)best > a.cert_max_mism) return false;
+    if (second != 0xFFFFFFFFu && second <= best + 1) return false;
+    if (best == 0) return true;
+    if (best > 2) return false;
+    int64_t at = s0 + shift;
+    uint32_t wl[RW], wh[RW], wc[RW] = {};
+    window_words<RW>(a.g_lo, at, wl);
+    window_words<RW>(a.g_hi, at, wh);
+    if (a.g_lc) window_words<RW>(a.g_lc, at, wc);
+    int n = 0, p0 = -1, p1 = -1;
+#pragma unroll
+    for (int k = 0; k < RW; k++) {
+        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (a.g_lc ? (rc[k] ^ wc[k]) : rc[k]);
+        d &= range_mask(k, 0, L);
+        while (d) {
+            int b = __builtin_ctz(d);
+            if (n == 0) p0 = 32 * k + b;
+            else if (n == 1) p1 = 32 * k + b;
+            n++;
+            d &= d - 1;
+        }
+    }
+    if (n != (int)best) return false;
+    int edge = L < 8 ? L : 8;
+    if (p0 < edge || p0 + edge >= L) return false;
+    if (n == 2) {
+        if (p1 < edge || p1 + edge >= L) return false;
+        if (p1 - p0 <= 4) return false;
+    }
+    int p = p0;
+    int suffix = L - (p + 1);
+    int min_suffix = a.cert_strict ? 4 : 8;
+    if (suffix >= min_suffix) {
+        // read[i] vs ref[i-1] for i in [p+1, L); read[i] vs ref[i+1] for i in [p, L-1)
+        if (masked_mismatch<RW>(rl, rh, rc, a, at - 1, p + 1, L) <= 1) return false;
+        if (masked_mismatch<RW>(rl, rh, rc, a, at + 1, p, L - 1) <= 1) return false;
+        if (a.cert_strict && suffix >= 6) {
+            if (masked_mismatch<RW>(rl, rh, rc, a, at - 2, p + 2, L) <= 1) return false;
+            if (masked_mismatch<RW>(rl, rh, rc, a, at + 2, p, L - 2) <= 1) return false;
+        }
+    }
+    return true;
+}
+
+// Scan one tile of up to 128 shifts starting at T0.  FULL: all 128 valid.
+template <int RW, bool FULL, bool MASKALL>
+__device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
+                                          const uint32_t (&mk)[RW], const uint32_t *g_lo,
+                                          const uint32_t *g_hi, int64_t pos, uint32_t rem,
+                                          uint32_t &tb, uint32_t &ts) {
+    constexpr int NW = RW + 4;
+    uint32_t wl[NW], wh[NW];
+    window_words<NW>(g_lo, pos, wl);
+    window_words<NW>(g_hi, pos, wh);
+    uint32_t nb = FULL ? 32u : (rem < 32u ? rem : 32u);
+    for (uint32_t b = 0; b < nb; b++) {
+        uint32_t xl[RW + 3], xh[RW + 3];
+#pragma unroll
+        for (int k = 0; k < RW + 3; k++) {
+            xl[k] = funnel(wl[k + 1], wl[k], b);
+            xh[k] = funnel(wh[k + 1], wh[k], b);
+        }
+#pragma unroll
+        for (int aa = 0; aa < 4; aa++) {
+            uint32_t m = 0;
+#pragma unroll
+            for (int k = 0; k < RW; k++) {
+                // (rl ^ xl) | (rh ^ xh) as xor + one 3-input bit op
+                uint32_t d = or_xor(rl[k] ^ xl[aa + k], rh[k], xh[aa + k]);
+                if (MASKALL || k == RW - 1) d &= mk[k];
+                m += __popc(d);
+            }
+            uint32_t tl = (uint32_t)aa * 32u + b;
+            uint32_t rank = (m << 16) | tl;
+            if (!FULL && tl >= rem) rank = 0xFFFFFFFFu;
+            // tb <= ts always, so min(ts, max(tb, rank)) is the median of the three
+            ts = med3u(tb, rank, ts);
+            tb = tb < rank ? tb : rank;
+        }
+    }
+}
+
+template <int RW, bool MASKALL>
+__device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t (&rl)[RW],
+                                           const uint32_t (&rh)[RW], const uint32_t (&mk)[RW],
+                                           int64_t s0, uint32_t nt, uint32_t &gbm,
+                                           uint32_t &gbt, uint32_t &gsm) {
+    gbm = 0xFFFFFFFFu;
+    gbt = 0;
+    gsm = 0xFFFFFFFFu;
+    for (uint32_t T0 = 0; T0 < nt; T0 += 128) {
+        uint32_t tb = 0xFFFFFFFFu, ts = 0xFFFFFFFFu;
+        uint32_t rem = nt - T0;
+        if (rem >= 128)
+            scan_tile<RW, true, MASKALL>(rl, rh, mk, a.g_lo, a.g_hi, s0 + T0, rem, tb, ts);
+        else
+            scan_tile<RW, false, MASKALL>(rl, rh, mk, a.g_lo, a.g_hi, s0 + T0, rem, tb, ts);
+        uint32_t tbm = tb >> 16;
+        uint32_t tsm = ts == 0xFFFFFFFFu ? 0xFFFFFFFFu : (ts >> 16);
+        uint32_t hi = gbm > tbm ? gbm : tbm;
+        uint32_t ns = gsm < hi ? gsm : hi;
+        ns = ns < tsm ? ns : tsm;
+        if (tbm < gbm) {
+            gbm = tbm;
+            gbt = T0 + (tb & 0xFFFFu);
+        }
+        gsm = ns;
+    }
+}
+
+template <int RW>
+__global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.n_jobs) return;
+    int32_t r = a.job_read ? a.job_read[j] : (int32_t)(j / a.jobs_per_read);
+    int32_t L = a.read_lens[r];
+    int32_t W = a.win_len[j];
+    int64_t s0 = a.win_start[j];
+    int32_t lim = a.limit ? a.limit[j] : a.limit_all;
+
+    // S9: skip (never dispatch) jobs whose read or window is not all ACGT, and
+    // windows that leave the text.
+    bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
+              s0 + W <= a.g_bases && L <= 32 * RW;
+    if (ok && a.g_inv && W > 0) {
+        int64_t q0 = s0 + (int64_t)kPadFront * 32, q1 = q0 + W;
+        uint32_t any = 0;
+        for (int64_t w = q0 >> 5; w <= ((q1 - 1) >> 5); w++) {
+            uint32_t v = a.g_inv[w];
+            int lo = (int)(w == (q0 >> 5) ? (q0 & 31) : 0);
+            int hi = (int)(w == ((q1 - 1) >> 5) ? (((q1 - 1) & 31) + 1) : 32);
+            uint32_t m = (hi == 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+            any |= v & m;
+        }
+        ok = any == 0;
+    }
+    if (!ok) {
+        a.out_shift[j] = -1;
+        a.out_best[j] = INT32_MAX;
+        a.out_second[j] = INT32_MAX;
+        if (a.out_flags) a.out_flags[j] = 0;
+        return;
+    }
+    if (L == 0 || W < L) {
+        a.out_shift[j] = -1;
+        a.out_best[j] = INT32_MAX;
+        a.out_second[j] = INT32_MAX;
+        if (a.out_flags) a.out_flags[j] = 1;
+        return;
+    }
+    const uint32_t *rp = a.read_planes + (int64_t)r * kReadWords;
+    uint32_t rl[RW], rh[RW], mk[RW];
+#pragma unroll
+    for (int k = 0; k < RW; k++) {
+        rl[k] = rp[k];
+        rh[k] = rp[8 + k];
+        int keep = L - 32 * k;
+        mk[k] = keep >= 32 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << keep) - 1u));
+    }
+    uint32_t nt = (uint32_t)(W - L + 1);
+    uint32_t gbm, gbt, gsm;
+    (void)uniform;
+    if (L > 32 * (RW - 1))
+        sieve_core<RW, false>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
+    else
+        sieve_core<RW, true>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
+
+    bool acc = (int64_t)gbm <= (int64_t)lim;
+    a.out_shift[j] = acc ? (int32_t)gbt : -1;
+    a.out_best[j] = acc ? (int32_t)gbm : INT32_MAX;
+    a.out_second[j] = gsm == 0xFFFFFFFFu ? INT32_MAX : (int32_t)gsm;
+    if (a.out_flags) {
+        uint8_t f = 1;
+        if (acc) {
+            f |= 2;
+            if (a.cert_max_mism >= 0) {
+                uint32_t rc[RW];
+#pragma unroll
+                for (int k = 0; k < RW; k++) rc[k] = rp[16 + k];
+                if (certify<RW>(rl, rh, rc, a, s0, L, (int)gbt, gbm, gsm)) f |= 4;
+            }
+        }
+        a.out_flags[j] = f;
+    }
+}
+
+}  // namespace
+
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_pack_ascii_planes(const uint8_t *seq, int64_t n, Planes p, hipStream_t s) {
+    if (p.n_words == 0) return hipSuccess;
+    pack_ascii_planes_kernel<<<grid_for(p.n_words, 256), 256, 0, s>>>(seq, n, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_2bit_planes(const uint8_t *packed, int64_t n_bytes, Planes p,
+                                   hipStream_t s) {
+    if (p.n_words == 0) return hipSuccess;
+    pack_2bit_planes_kernel<<<grid_for(p.n_words, 256), 256, 0, s>>>(packed, n_bytes, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stride,
+                                   const int32_t *lens, uint32_t *planes, uint8_t *valid,
+                                   hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(n, reads, stride, lens,
+                                                                          planes, valid);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stride,
+                                  const int32_t *lens, uint32_t *planes, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    pack_reads_2bit_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(n, packed, stride, lens,
+                                                                         planes);
+    return hipGetLastError();
+}
+
+hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s) {
+    if (a.n_jobs == 0) return hipSuccess;
+    unsigned g = grid_for(a.n_jobs, 256);
+    int u = uniform ? 1 : 0;
+    switch (read_words) {
+    case 1: sieve_kernel<1><<<g, 256, 0, s>>>(a, u); break;
+    case 2: sieve_kernel<2><<<g, 256, 0, s>>>(a, u); break;
+    case 3: sieve_kernel<3><<<g, 256, 0, s>>>(a, u); break;
+    case 4: sieve_kernel<4><<<g, 256, 0, s>>>(a, u); break;
+    case 5: sieve_kernel<5><<<g, 256, 0, s>>>(a, u); break;
+    case 6: sieve_kernel<6><<<g, 256, 0, s>>>(a, u); break;
+    case 7: sieve_kernel<7><<<g, 256, 0, s>>>(a, u); break;
+    case 8: sieve_kernel<8><<<g, 256, 0, s>>>(a, u); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace kira
