@@ -140,6 +140,84 @@ kira_status kira_sieve_genome_dev(kira_ctx *ctx, const kira_genome *g,
 /* Synchronise the context stream. */
 kira_status kira_ctx_sync(kira_ctx *ctx);
 
+/* ------------------------------------------------------------------ banded SW */
+
+/* AlignmentConfig — src/alignment/mod.rs:26-35. */
+typedef struct {
+    int32_t match_score, mismatch, gap_open, gap_extend, bandwidth, xdrop, clip_penalty;
+} kira_sw_cfg;
+
+/* Knobs the reference reads once from the environment:
+ * adaptive  KIRA_ADAPTIVE_BAND (src/alignment/mod.rs:1267), reference default 1
+ * max_drift KIRA_ADAPTIVE_MAX_DRIFT (mod.rs:1319), default 16
+ * dp_xdrop  KIRA_DP_XDROP (mod.rs:1284), -1 = use cfg.xdrop */
+typedef struct {
+    int32_t adaptive, max_drift, dp_xdrop;
+} kira_sw_knobs;
+
+/* SwPublicResult — src/alignment/mod.rs:904-915.  The CIGAR (soft clips not
+ * included) is n_cigar words at cigar_arena[cigar_off], each len << 4 | op
+ * with op M=0 I=1 D=2 N=3 S=4. */
+typedef struct {
+    uint32_t ref_start, ref_end;
+    int32_t read_start, read_end, score, early_abort, n_cigar, reserved;
+    int64_t cigar_off;
+} kira_sw_result;
+
+/*
+ * banded_sw_public (src/alignment/mod.rs:891-920, core banded_sw_with
+ * :1370-1650) over a batch, host arrays in and out.  Job k aligns
+ * reads[read_off[k] .. +read_len[k]] against refs[ref_off[k] .. +ref_len[k]]
+ * with band centre offset offset[k], band band[k] (NULL: cfg->bandwidth) and
+ * abort_score[k] (<= INT32_MIN/8 disables the abort bound).  Bands up to 128
+ * are supported; wider bands return KIRA_ERR_INVALID_ARGUMENT.  cigar_arena
+ * receives every job's CIGAR back to back; arena_cap words must suffice
+ * (3 * sum(read_len) + 2n always does).
+ */
+kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
+                                 int64_t reads_bytes, const int64_t *read_off,
+                                 const int32_t *read_len, const uint8_t *refs,
+                                 int64_t refs_bytes, const int64_t *ref_off,
+                                 const int32_t *ref_len, const int32_t *offset,
+                                 const int32_t *band, const int32_t *abort_score,
+                                 const kira_sw_cfg *cfg, const kira_sw_knobs *knobs,
+                                 kira_sw_result *out, uint32_t *cigar_arena, int64_t arena_cap);
+
+/*
+ * Device-resident form for batches that repeat a shape: a plan fixes the
+ * job count, read lengths and bands (it sorts jobs into 64-job packets by
+ * band class and length and sizes the trace slabs); run_dev then only
+ * launches kernels on the context stream.  All kira_sw_jobs_dev pointers are
+ * device pointers; refs must have at least 1024 readable bytes before and
+ * after every window.  res: n x 8 int32 (score, read_end, ref_end,
+ * early_abort, ref_start, read_start, n_cigar, 0); cigar: the plan's
+ * cigar_words, job k's ops at word kira_banded_sw_plan_cigar_off(plan)[k].
+ */
+typedef struct kira_sw_plan kira_sw_plan;
+typedef struct {
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *refs;
+    const int64_t *ref_off;
+    const int32_t *ref_len;
+    const int32_t *offset;
+    const int32_t *band;
+    const int32_t *abort_score;
+    int32_t *res;
+    uint32_t *cigar;
+} kira_sw_jobs_dev;
+
+kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *read_len,
+                                       const int32_t *band, const kira_sw_cfg *cfg,
+                                       kira_sw_plan **out);
+void kira_banded_sw_plan_destroy(kira_sw_plan *plan);
+int64_t kira_banded_sw_plan_cigar_words(const kira_sw_plan *plan);
+const int64_t *kira_banded_sw_plan_cigar_off(const kira_sw_plan *plan);
+kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
+                                   const kira_sw_jobs_dev *jobs, const kira_sw_cfg *cfg,
+                                   const kira_sw_knobs *knobs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,31 @@ class SieveBatch(C.Structure):
                 ("cert_max_mism", C.c_int32), ("cert_strict", C.c_int32)]
 
 
+class SwCfg(C.Structure):
+    """kira_sw_cfg = AlignmentConfig (src/alignment/mod.rs:26-35)."""
+
+    _fields_ = [(n, C.c_int32) for n in ("match_score", "mismatch", "gap_open", "gap_extend",
+                                         "bandwidth", "xdrop", "clip_penalty")]
+
+
+class SwKnobs(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("adaptive", "max_drift", "dp_xdrop")]
+
+
+class SwResult(C.Structure):
+    """kira_sw_result = SwPublicResult (src/alignment/mod.rs:904-915)."""
+
+    _fields_ = [("ref_start", C.c_uint32), ("ref_end", C.c_uint32), ("read_start", C.c_int32),
+                ("read_end", C.c_int32), ("score", C.c_int32), ("early_abort", C.c_int32),
+                ("n_cigar", C.c_int32), ("reserved", C.c_int32), ("cigar_off", C.c_int64)]
+
+
+class SwJobsDev(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
+                                          "ref_len", "offset", "band", "abort_score", "res",
+                                          "cigar")]
+
+
 # Every symbol include/kira_hip.h declares, with (restype, argtypes).
 _P = C.c_void_p
 SIGNATURES = {
@@ -70,6 +95,16 @@ SIGNATURES = {
     "kira_reads_pack_dev": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P]),
     "kira_sieve_genome_dev": (C.c_int, [_P, _P, C.POINTER(SieveBatch)]),
     "kira_ctx_sync": (C.c_int, [_P]),
+    "kira_banded_sw_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P,
+                                       _P, _P, _P, _P, C.POINTER(SwCfg), C.POINTER(SwKnobs),
+                                       _P, _P, C.c_int64]),
+    "kira_banded_sw_plan_create": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(SwCfg),
+                                             C.POINTER(C.c_void_p)]),
+    "kira_banded_sw_plan_destroy": (None, [_P]),
+    "kira_banded_sw_plan_cigar_words": (C.c_int64, [_P]),
+    "kira_banded_sw_plan_cigar_off": (C.POINTER(C.c_int64), [_P]),
+    "kira_banded_sw_run_dev": (C.c_int, [_P, _P, C.POINTER(SwJobsDev), C.POINTER(SwCfg),
+                                         C.POINTER(SwKnobs)]),
 }
 
 _lib = None
@@ -380,3 +415,58 @@ def sieve_genome(ctx: Context, genome: Genome, reads, read_lens, win_start, win_
     b.cert_strict = int(cert_strict)
     sieve_genome_dev(ctx, genome, b)
     return out
+
+
+# ------------------------------------------------------------ banded SW
+ABORT_NONE = -(2 ** 31) // 8
+
+
+def sw_cfg(match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth=32, xdrop=100,
+           clip_penalty=5) -> SwCfg:
+    return SwCfg(match_score, mismatch, gap_open, gap_extend, bandwidth, xdrop, clip_penalty)
+
+
+def sw_knobs(adaptive=True, max_drift=16, dp_xdrop=-1) -> SwKnobs:
+    return SwKnobs(int(adaptive), max_drift, dp_xdrop)
+
+
+def banded_sw_batch(ctx: Context, jobs, cfg: SwCfg, knobs: Optional[SwKnobs] = None):
+    """banded_sw_public (src/alignment/mod.rs:891-920) over a batch.
+
+    jobs: sequence of (read: bytes, ref: bytes, offset: int, abort_score: int,
+    band: Optional[int]).  Returns dicts with ref_start, ref_end, read_start,
+    read_end, score, early_abort and cigar (list of len << 4 | op words)."""
+    knobs = knobs or sw_knobs()
+    n = len(jobs)
+    if n == 0:
+        return []
+    reads = b"".join(j[0] for j in jobs)
+    refs = b"".join(j[1] for j in jobs)
+    rl = np.array([len(j[0]) for j in jobs], dtype=np.int32)
+    fl = np.array([len(j[1]) for j in jobs], dtype=np.int32)
+    ro = np.concatenate([[0], np.cumsum(rl[:-1], dtype=np.int64)]).astype(np.int64)
+    fo = np.concatenate([[0], np.cumsum(fl[:-1], dtype=np.int64)]).astype(np.int64)
+    off = np.array([j[2] for j in jobs], dtype=np.int32)
+    ab = np.array([j[3] for j in jobs], dtype=np.int32)
+    bands = [j[4] if len(j) > 4 else None for j in jobs]
+    band = None
+    if any(b is not None for b in bands):
+        band = np.array([cfg.bandwidth if b is None else b for b in bands], dtype=np.int32)
+    rbuf = np.frombuffer(reads, dtype=np.uint8) if reads else np.zeros(1, np.uint8)
+    fbuf = np.frombuffer(refs, dtype=np.uint8) if refs else np.zeros(1, np.uint8)
+    out = (SwResult * n)()
+    cap = int(3 * rl.sum() + 2 * n + 2)
+    arena = np.zeros(cap, dtype=np.uint32)
+    st = lib().kira_banded_sw_batch(ctx.h, n, _np_ptr(rbuf), len(reads), _np_ptr(ro), _np_ptr(rl),
+                                    _np_ptr(fbuf), len(refs), _np_ptr(fo), _np_ptr(fl),
+                                    _np_ptr(off), _np_ptr(band) if band is not None else None,
+                                    _np_ptr(ab), C.byref(cfg), C.byref(knobs), out,
+                                    _np_ptr(arena), cap)
+    ctx.check(st, "banded_sw_batch")
+    res = []
+    for k in range(n):
+        o = out[k]
+        res.append(dict(ref_start=o.ref_start, ref_end=o.ref_end, read_start=o.read_start,
+                        read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
+                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+    return res

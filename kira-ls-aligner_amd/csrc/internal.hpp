@@ -55,6 +55,45 @@ hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stri
                                   const int32_t *lens, uint32_t *planes, hipStream_t s);
 hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
 
+// Banded SW: results per job, kSwResWords int32 each:
+// score, read_end, ref_end, early_abort, ref_start, read_start, n_cigar, 0.
+constexpr int kSwResWords = 8;
+// Reference arena padding (bytes) so band rows that poke past a window's
+// ends stay inside the allocation.
+constexpr int64_t kRefPad = 1024;
+
+struct SwLaunch {
+    int32_t match_score, mismatch, go, gap_extend, clip_penalty;
+    int32_t bandwidth, adaptive, max_drift, dp_xdrop;
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *refs;
+    const int64_t *ref_off;
+    const int32_t *ref_len;
+    const int32_t *offset;
+    const int32_t *band;  // per job, may be null (bandwidth)
+    const int32_t *abort_score;
+    // work packets of 64 jobs (one wave each), pulled through *counter
+    const int32_t *packet_jobs;
+    const int32_t *packet_rows;
+    const int32_t *packet_rowdw;
+    const int64_t *packet_trace_off;
+    const int64_t *packet_meta_off;
+    int32_t n_packets;
+    int32_t *counter;
+    uint32_t *trace;
+    int2 *row_meta;
+    int32_t *res;
+    const int32_t *job_slot;
+    uint32_t *cigar;
+    const int64_t *cigar_off;
+    const int32_t *cigar_cap;
+};
+
+hipError_t launch_banded_sw(const SwLaunch &a, int C, int n_waves, hipStream_t s);
+hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s);
+
 }  // namespace kira
 
 struct kira_ctx {

@@ -1,0 +1,351 @@
+// Banded affine-gap Smith-Waterman with the reference's adaptive band
+// (banded_sw_with, src/alignment/mod.rs:1370-1650) on gfx950.
+//
+// One lane runs one job row by row (row i+1's band centre is the arg-max of
+// row i, so rows are inherently sequential).  The band lives in registers in
+// diagonal coordinates: position q of row i is column j = c_i - w + q.  With
+// that indexing the up neighbour of (i, j) is position q+1 and the diagonal
+// neighbour position q of the previous row, both static register indices;
+// when the centre moves by other than one column per row the arrays are
+// shifted by the difference first.  Cells outside the band or the window
+// hold pads chosen so the recurrences reproduce the reference's "absent"
+// values (i32::MIN/4) exactly.  Each cell writes a 4-bit trace code
+// (direction | E-extend << 2 | F-extend << 3); a packet of 64 jobs shares a
+// slab laid out [row][word][lane] so every trace store is one coalesced
+// 256-byte write.  A second kernel walks the traces back into CIGARs.
+#include "internal.hpp"
+
+namespace kira {
+namespace {
+
+constexpr int32_t kMin4 = INT32_MIN / 4;
+constexpr int32_t kMin8 = INT32_MIN / 8;
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
+    int64_t s = (int64_t)a + b;
+    return s > INT32_MAX ? INT32_MAX : (s < INT32_MIN ? INT32_MIN : (int32_t)s);
+}
+
+struct Row {
+    int32_t rb;   // row best (0 if no positive cell)
+    int32_t rbq;  // its position
+};
+
+// One row of the band.  MASK: some positions fall outside the band or the
+// window and must hold pads.
+template <int C, bool MASK>
+__device__ __forceinline__ Row band_row(int32_t (&H)[C], int32_t (&E)[C], const uint8_t *ref,
+                                        int32_t W, int32_t base, int32_t js, int32_t je,
+                                        uint8_t qb, const SwLaunch &a, uint32_t *trace_row,
+                                        int lane) {
+    const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
+    const int32_t padH = kMin4 + go, padE = kMin4 + ge;
+    int32_t hl = padH, fl = padE;  // left neighbour: absent at the band start
+    Row r{0, -1};
+    uint32_t word = 0;
+    // Interior rows fetch the band's reference bytes (column j = base + q is
+    // ref[j - 1]) as aligned dwords, funnel-shifted once per row; the arena is
+    // padded so the <= 8 bytes read past a window's ends stay allocated.
+    constexpr int NW = (C + 3) / 4 + 1;
+    const uint8_t *rp = ref + (base - 1);
+    const uintptr_t ra = (uintptr_t)rp;
+    const uint32_t *wp = (const uint32_t *)(ra & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(ra & 3) * 8u;
+    uint32_t rw[NW] = {};
+    if (!MASK) {
+        uint32_t prev = wp[0];
+#pragma unroll
+        for (int k = 0; k < NW - 1; k++) {
+            uint32_t nx = wp[k + 1];
+            rw[k] = __builtin_amdgcn_alignbit(nx, prev, sh);
+            prev = nx;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+        int32_t d = H[q];
+        int32_t uh = q + 1 < C ? H[q + 1] : padH;
+        int32_t ue = q + 1 < C ? E[q + 1] : padE;
+        bool valid = true;
+        if (MASK) valid = (base + q >= js) && (base + q <= je);
+        // edge rows may reach past the window (and the arena): guarded loads
+        uint8_t rb = MASK ? (valid ? rp[q] : (uint8_t)0) : (uint8_t)(rw[q >> 2] >> (8 * (q & 3)));
+        int32_t s = (qb == rb) ? A : -B;
+        int32_t hm = imax(d, 0) + s;
+        int32_t efh = uh - go, efe = ue - ge;
+        int32_t e = imax(efh, efe);
+        int32_t ffh = hl - go, fff = fl - ge;
+        int32_t f = imax(ffh, fff);
+        int32_t h = imax(imax(hm, e), imax(f, 0));
+        uint32_t dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
+        uint32_t code = dir | (efe > efh ? 4u : 0u) | (fff > ffh ? 8u : 0u);
+        if (MASK && !valid) {
+            h = padH;
+            e = padE;
+            f = padE;
+            code = 0;
+        }
+        H[q] = h;
+        E[q] = e;
+        hl = h;
+        fl = f;
+        if (h > r.rb) {
+            r.rb = h;
+            r.rbq = q;
+        }
+        word |= code << (4 * (q & 7));
+        if ((q & 7) == 7 || q == C - 1) {
+            trace_row[(q >> 3) * 64 + lane] = word;
+            word = 0;
+        }
+    }
+    return r;
+}
+
+// Shift the band arrays so that position q holds what was at q + delta.
+template <int C>
+__device__ __forceinline__ void shift_band(int32_t (&H)[C], int32_t (&E)[C], int32_t delta,
+                                           int32_t padH, int32_t padE) {
+    if (delta >= C || delta <= -C) {
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+            H[q] = padH;
+            E[q] = padE;
+        }
+        return;
+    }
+    while (delta > 0) {
+#pragma unroll
+        for (int q = 0; q < C - 1; q++) {
+            H[q] = H[q + 1];
+            E[q] = E[q + 1];
+        }
+        H[C - 1] = padH;
+        E[C - 1] = padE;
+        delta--;
+    }
+    while (delta < 0) {
+#pragma unroll
+        for (int q = C - 1; q > 0; q--) {
+            H[q] = H[q - 1];
+            E[q] = E[q - 1];
+        }
+        H[0] = padH;
+        E[0] = padE;
+        delta++;
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
+    const int lane = threadIdx.x;
+    for (;;) {
+        int32_t p = 0;
+        if (lane == 0) p = atomicAdd(a.counter, 1);
+        p = __shfl(p, 0);
+        if (p >= a.n_packets) return;
+        const int32_t job = a.packet_jobs[(int64_t)p * 64 + lane];
+        const int32_t rows = a.packet_rows[p];
+        const int32_t rowdw = (C + 7) / 8;
+        uint32_t *slab = a.trace + a.packet_trace_off[p];
+        int2 *meta = a.row_meta + a.packet_meta_off[p];
+        if (job < 0) continue;
+        const int32_t L = a.read_len[job], W = a.ref_len[job];
+        const int32_t off = a.offset[job];
+        const int32_t w = imax(a.band ? a.band[job] : a.bandwidth, 1);
+        const int32_t abort_score = a.abort_score[job];
+        const uint8_t *read = a.reads + a.read_off[job];
+        const uint8_t *ref = a.refs + a.ref_off[job];
+        const int32_t go = a.go, ge = a.gap_extend;
+        const int32_t padH = kMin4 + go, padE = kMin4 + ge;
+        const int32_t D = a.adaptive ? a.max_drift : 0;
+        const int32_t xdrop = a.dp_xdrop;
+
+        int32_t H[C], E[C];
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+            H[q] = padH;
+            E[q] = padE;
+        }
+        int32_t ac = 1 + off;
+        int32_t prev_c = 0;
+        bool prev_empty = true;
+        int32_t best = 0, bi = 0, bj = 0, bq = 0, bqj = 0;
+        int32_t early = 0;
+        int32_t last_row = 0;
+        for (int32_t i = 1; i <= L && i <= rows; i++) {
+            last_row = i;
+            const int32_t sc = i + off;
+            const int32_t c = a.adaptive ? imin(imax(ac, sc - D), sc + D) : sc;
+            const int32_t base = c - w;
+            const int32_t js = imax(base, 1), je = imin(c + w, W);
+            uint32_t *trow = slab + (int64_t)(i - 1) * rowdw * 64;
+            if (js > je) {
+                // empty row: the reference clears the previous row and skips
+                // the centre update and the stop checks (mod.rs:1436-1443)
+#pragma unroll
+                for (int q = 0; q < C; q++) {
+                    H[q] = padH;
+                    E[q] = padE;
+                }
+                prev_empty = true;
+                meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, 1 | (0 << 16));
+                continue;
+            }
+            if (prev_empty) {
+#pragma unroll
+                for (int q = 0; q < C; q++) {
+                    H[q] = padH;
+                    E[q] = padE;
+                }
+            } else {
+                int32_t delta = c - prev_c - 1;
+                if (delta != 0) shift_band<C>(H, E, delta, padH, padE);
+            }
+            prev_c = c;
+            prev_empty = false;
+            const uint8_t qb = read[i - 1];
+            Row r;
+            if (js == base && je == base + C - 1 && 2 * w + 1 == C)
+                r = band_row<C, false>(H, E, ref, W, base, js, je, qb, a, trow, lane);
+            else
+                r = band_row<C, true>(H, E, ref, W, base, js, je, qb, a, trow, lane);
+            meta[(int64_t)(i - 1) * 64 + lane] =
+                make_int2(base, (js - base) | ((je - base) << 16));
+            const int32_t rbj = r.rbq >= 0 ? base + r.rbq : -1;
+            if (r.rb > best) {
+                best = r.rb;
+                bi = i;
+                bj = rbj;
+            }
+            if (i == L && r.rb > 0) {
+                bq = r.rb;
+                bqj = rbj;
+            }
+            if (a.adaptive) {
+                int32_t raw = rbj >= 0 ? rbj + 1 : ac + 1;
+                int32_t sn = i + 1 + off;
+                ac = imin(imax(raw, sn - D), sn + D);
+            }
+            if (xdrop > 0 && best - r.rb > xdrop) {
+                early = 1;
+                break;
+            }
+            if (abort_score > kMin8) {
+                int32_t remaining = L - i;
+                if (best + remaining * a.match_score < abort_score) {
+                    early = 1;
+                    break;
+                }
+            }
+        }
+        (void)last_row;
+        if (bq > 0 && sat_add(bq, a.clip_penalty) > best) {
+            best = bq;
+            bi = L;
+            bj = bqj;
+        }
+        int32_t *res = a.res + (int64_t)job * kSwResWords;
+        res[0] = best;
+        res[1] = bi;  // read_end
+        res[2] = bj;  // ref_end
+        res[3] = early;
+    }
+}
+
+// Traceback (mod.rs:1581-1626) over the slab written by banded_sw_kernel.
+__global__ void banded_traceback_kernel(SwLaunch a, int32_t n_jobs) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_jobs) return;
+    const int32_t job = (int32_t)t;
+    const int32_t pl = a.job_slot[job];  // packet * 64 + lane
+    const int32_t p = pl >> 6, lane = pl & 63;
+    const int32_t rowdw = a.packet_rowdw[p];
+    const uint32_t *slab = a.trace + a.packet_trace_off[p];
+    const int2 *meta = a.row_meta + a.packet_meta_off[p];
+    int32_t *res = a.res + (int64_t)job * kSwResWords;
+    int32_t i = res[1], j = res[2];
+    uint32_t *cig = a.cigar + a.cigar_off[job];
+    const int32_t cap = a.cigar_cap[job];
+    int32_t n = 0;
+    int32_t last_op = -1;
+    int state = 0;
+    while (i > 0 && j > 0) {
+        int2 m = meta[(int64_t)(i - 1) * 64 + lane];
+        int32_t base = m.x, js = base + (m.y & 0xFFFF), je = base + (m.y >> 16);
+        if (j < js || j > je) break;
+        int32_t q = j - base;
+        uint32_t tr = (slab[((int64_t)(i - 1) * rowdw + (q >> 3)) * 64 + lane] >> (4 * (q & 7))) & 15u;
+        int op = -1;
+        if (state == 0) {
+            uint32_t d = tr & 3u;
+            if (d == 0) break;
+            if (d == 1) {
+                op = 0;
+                i--;
+                j--;
+            } else if (d == 2) {
+                state = 1;
+            } else {
+                state = 2;
+            }
+        } else if (state == 1) {
+            op = 1;
+            i--;
+            state = (tr & 4u) ? 1 : 0;
+        } else {
+            op = 2;
+            j--;
+            state = (tr & 8u) ? 2 : 0;
+        }
+        if (op >= 0) {
+            if (op == last_op) {
+                if (n - 1 < cap) cig[n - 1] += 16u;
+            } else {
+                if (n < cap) cig[n] = (1u << 4) | (uint32_t)op;
+                n++;
+                last_op = op;
+            }
+        }
+    }
+    // ops were produced end-to-start: reverse in place
+    int32_t m = n < cap ? n : cap;
+    for (int32_t x = 0, y = m - 1; x < y; x++, y--) {
+        uint32_t tmp = cig[x];
+        cig[x] = cig[y];
+        cig[y] = tmp;
+    }
+    res[4] = j;  // ref_start
+    res[5] = i;  // read_start
+    res[6] = n <= cap ? n : -1;
+}
+
+__global__ void zero_counter_kernel(int32_t *c) { *c = 0; }
+
+}  // namespace
+
+hipError_t launch_banded_sw(const SwLaunch &a, int C, int n_waves, hipStream_t s) {
+    zero_counter_kernel<<<1, 1, 0, s>>>(a.counter);
+    if (a.n_packets > 0) {
+        switch (C) {
+        case 9: banded_sw_kernel<9><<<n_waves, 64, 0, s>>>(a); break;
+        case 17: banded_sw_kernel<17><<<n_waves, 64, 0, s>>>(a); break;
+        case 33: banded_sw_kernel<33><<<n_waves, 64, 0, s>>>(a); break;
+        case 65: banded_sw_kernel<65><<<n_waves, 64, 0, s>>>(a); break;
+        case 129: banded_sw_kernel<129><<<n_waves, 64, 0, s>>>(a); break;
+        case 257: banded_sw_kernel<257><<<n_waves, 64, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s) {
+    if (n_jobs == 0) return hipSuccess;
+    banded_traceback_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
+    return hipGetLastError();
+}
+
+}  // namespace kira

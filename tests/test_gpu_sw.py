@@ -1,0 +1,112 @@
+"""GPU parity of the adaptive banded SW (banded_sw_with) vs the CPU oracle."""
+import random
+
+import pytest
+
+from oracle import oracle as O
+from tests.kira_pkg import load
+from tests.refgen import gen_ref
+
+pytestmark = pytest.mark.gpu
+NEG = -(2 ** 31) // 4
+ABORT_NONE = -(2 ** 31) // 8
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    p = load()
+    assert p.is_available(), "no GPU visible"
+    return p
+
+
+@pytest.fixture(scope="module")
+def ctx(pkg):
+    c = pkg.Context(0)
+    yield c
+    c.close()
+
+
+def mutate(rng, s, sub, indel):
+    out = bytearray()
+    for b in s:
+        r = rng.random()
+        if r < indel / 2:
+            continue  # deletion
+        if r < indel:
+            out.append(rng.choice(b"ACGT"))  # insertion before the base
+        out.append(rng.choice(b"ACGT".replace(bytes([b]), b"")) if rng.random() < sub else b)
+    return bytes(out)
+
+
+def make_jobs(seed, n, bands=(16, 32, 64, 128), lmin=20, lmax=300):
+    rng = random.Random(seed)
+    jobs = []
+    for k in range(n):
+        w = rng.choice(bands)
+        L = rng.randrange(lmin, lmax + 1)
+        flank = max(2 * w, 50)
+        g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank + 20))
+        src = g[flank:flank + L]
+        read = mutate(rng, src, rng.uniform(0.01, 0.05), 0.005)[:lmax] or b"A"
+        ref = g
+        off = flank + rng.randrange(-3, 4)
+        abort = rng.choice([ABORT_NONE, NEG, len(read) // 2, len(read) + 5, 0])
+        jobs.append((read, ref, off, abort, w))
+    return jobs
+
+
+def check(pkg, ctx, jobs, c, kn_gpu, kn_or):
+    got = pkg.banded_sw_batch(ctx, jobs, c, kn_gpu)
+    for k, (read, ref, off, abort, w) in enumerate(jobs):
+        cc = O.cfg(c.match_score, c.mismatch, c.gap_open, c.gap_extend, w, c.xdrop,
+                   c.clip_penalty)
+        want = O.banded_sw(read, ref, off, cc, abort, kn_or)
+        assert got[k] == want, (k, len(read), len(ref), off, abort, w)
+
+
+@pytest.mark.parametrize("adaptive", [True, False])
+def test_random_jobs_match_oracle(pkg, ctx, adaptive):
+    jobs = make_jobs(1 if adaptive else 2, 400)
+    c = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
+    check(pkg, ctx, jobs, c, pkg.sw_knobs(adaptive, 16, -1), O.knobs(adaptive, 16, -1))
+
+
+def test_edges_offsets_and_odd_bands(pkg, ctx):
+    """Bands off the class sizes, centres far off the window, tiny windows,
+    empty rows, x-drop aborts and a large match score."""
+    rng = random.Random(5)
+    jobs = []
+    for k in range(300):
+        w = rng.choice([1, 2, 3, 5, 7, 8, 11, 20, 33, 40, 100])
+        L = rng.randrange(1, 120)
+        W = rng.randrange(1, 200)
+        read = bytes(rng.choice(b"ACGTN") for _ in range(L))
+        ref = bytes(rng.choice(b"ACGT") for _ in range(W))
+        if W > L and rng.random() < 0.5:
+            st = rng.randrange(W - L)
+            ref = ref[:st] + read + ref[st + L:]
+            ref = ref[:W]
+        off = rng.randrange(-L - 5, W + 5)
+        abort = rng.choice([ABORT_NONE, NEG, 5, L])
+        jobs.append((read, ref, off, abort, w))
+    for cfgv in [(1, 4, 6, 1, 32, 100, 5), (2, 3, 0, 2, 32, 0, 0), (200, 4, 6, 1, 32, 30, 5)]:
+        c = pkg.sw_cfg(*cfgv)
+        for adaptive in (True, False):
+            drift = rng.choice([0, 4, 16])
+            check(pkg, ctx, jobs, c, pkg.sw_knobs(adaptive, drift, -1),
+                  O.knobs(adaptive, drift, -1))
+
+
+def test_reference_known_answers(pkg, ctx):
+    """src/alignment/mod.rs:2368-2417 and :2592-2635 on the GPU."""
+    ref = gen_ref(120)
+    read = ref[0:30] + ref[33:60] + ref[63:120]
+    c = pkg.sw_cfg(1, 4, 6, 1, 4, 1000, 5)
+    ad, = pkg.banded_sw_batch(ctx, [(read, ref, 0, ABORT_NONE)], c, pkg.sw_knobs(True))
+    assert sum(o >> 4 for o in ad["cigar"] if o & 15 == 2) == 6
+    assert ad["read_end"] - ad["read_start"] >= 113
+    ref = gen_ref(90)
+    read = ref[:40] + b"TTGCA" + ref[40:]
+    r, = pkg.banded_sw_batch(ctx, [(read, ref, 0, ABORT_NONE)], pkg.sw_cfg(1, 4, 6, 1, 20, 1000, 5))
+    want = O.banded_sw(read, ref, 0, O.cfg(1, 4, 6, 1, 20, 1000, 5), ABORT_NONE, O.knobs())
+    assert r == want
