@@ -116,6 +116,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HipError("NotBuilt", f"{LIB_PATH} missing; build with __graft_entry__.build()")
+        # torch ships its own libamdhip64.so.7.  Loading it first makes this
+        # library bind to the same HIP runtime (same soname), so device
+        # pointers and streams from torch are valid here and torch still sees
+        # the GPU after this library initialises HIP.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
