@@ -243,7 +243,9 @@ def main():
     out = pkg.SieveOutputs(*(torch.empty(n_jobs, dtype=torch.int32, device=dev)
                              for _ in range(3)),
                            torch.empty(n_jobs, dtype=torch.uint8, device=dev))
-    stream = torch.cuda.current_stream(dev)
+    # one explicit stream for every launch and event of the timed region
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
     cert_max = 4  # spectral_max_mismatches(150) = max(150*3/100, 2)
 
     def step(ev=None):

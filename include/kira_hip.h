@@ -47,9 +47,11 @@ kira_status kira_ctx_create(int device, kira_ctx **out);
 void kira_ctx_destroy(kira_ctx *ctx);
 const char *kira_ctx_last_error(const kira_ctx *ctx);
 /* HIP stream (hipStream_t) the context launches on.  set_stream makes the
- * context launch on a caller-owned stream instead (NULL restores its own). */
+ * context launch on a caller-owned stream (NULL = the legacy default
+ * stream); use_own_stream switches back to the context's own stream. */
 void *kira_ctx_stream(kira_ctx *ctx);
 void kira_ctx_set_stream(kira_ctx *ctx, void *stream);
+void kira_ctx_use_own_stream(kira_ctx *ctx);
 
 /* CudaBackend::release_buffers() — src/cuda/backend.rs:252-261. */
 void kira_release_buffers(kira_ctx *ctx);

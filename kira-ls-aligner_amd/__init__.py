@@ -86,6 +86,7 @@ SIGNATURES = {
     "kira_ctx_last_error": (C.c_char_p, [_P]),
     "kira_ctx_stream": (C.c_void_p, [_P]),
     "kira_ctx_set_stream": (None, [_P, _P]),
+    "kira_ctx_use_own_stream": (None, [_P]),
     "kira_release_buffers": (None, [_P]),
     "kira_sieve_run_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int64, _P,
                                        _P, _P, _P, _P]),

@@ -55,8 +55,8 @@ hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stri
                                   const int32_t *lens, uint32_t *planes, hipStream_t s);
 hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
 
-// Banded SW: results per job, kSwResWords int32 each:
-// score, read_end, ref_end, early_abort, ref_start, read_start, n_cigar, 0.
+// Banded SW: results per job, kSwResWords int32 each: score, read_end,
+// ref_end, early_abort, ref_start, read_start, n_cigar, int16-overflow flag.
 constexpr int kSwResWords = 8;
 // Reference arena padding (bytes) so band rows that poke past a window's
 // ends stay inside the allocation.
@@ -91,7 +91,7 @@ struct SwLaunch {
     const int32_t *cigar_cap;
 };
 
-hipError_t launch_banded_sw(const SwLaunch &a, int C, int n_waves, hipStream_t s);
+hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
 hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s);
 
 }  // namespace kira

@@ -124,7 +124,12 @@ void *kira_ctx_stream(kira_ctx *ctx) { return ctx ? (void *)ctx->stream : nullpt
 
 void kira_ctx_set_stream(kira_ctx *ctx, void *stream) {
     if (!ctx) return;
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    ctx->stream = (hipStream_t)stream;
+}
+
+void kira_ctx_use_own_stream(kira_ctx *ctx) {
+    if (!ctx) return;
+    ctx->stream = ctx->own_stream;
 }
 
 kira_status kira_ctx_sync(kira_ctx *ctx) {

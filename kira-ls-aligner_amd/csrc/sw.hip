@@ -21,9 +21,22 @@ namespace {
 constexpr int32_t kMin4 = INT32_MIN / 4;
 constexpr int32_t kMin8 = INT32_MIN / 8;
 
-__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
-__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
-__device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
+// The per-job logic is __host__ __device__ so the same code can be built and
+// bounds-checked on the CPU.
+__host__ __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);  // v_alignbit_b32
+}
+// Keeps the scheduler from hoisting every cell's (row-independent) E terms
+// ahead of the row, which multiplies live registers.
+__host__ __device__ __forceinline__ void cell_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+__host__ __device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
     int64_t s = (int64_t)a + b;
     return s > INT32_MAX ? INT32_MAX : (s < INT32_MIN ? INT32_MIN : (int32_t)s);
 }
@@ -33,16 +46,61 @@ struct Row {
     int32_t rbq;  // its position
 };
 
+// Band storage for the previous row.  PACK keeps H (low) and E (high) as
+// saturating int16 halves of one register, halving the register footprint;
+// the plan routes a job there only when its scores provably fit
+// (H <= L * match < 32767), so no value is ever clamped.  Pads mark cells
+// outside the band: with them the recurrences give the reference's
+// "absent" results (e/f = i32::MIN/4, diagonal H = 0, no extend bits).
+template <int C, bool PACK>
+struct Band {
+    int32_t H[PACK ? 1 : C], E[PACK ? 1 : C];
+    uint32_t P[PACK ? C : 1];
+    int32_t padH, padE;  // stored pads
+    __host__ __device__ __forceinline__ int32_t h(int q) const {
+        if constexpr (PACK) return (int32_t)(int16_t)(P[q] & 0xFFFFu);
+        else return H[q];
+    }
+    __host__ __device__ __forceinline__ int32_t e(int q) const {
+        if constexpr (PACK) return ((int32_t)P[q]) >> 16;
+        else return E[q];
+    }
+    __host__ __device__ __forceinline__ void set(int q, int32_t hv, int32_t ev) {
+        if constexpr (PACK) P[q] = pack16(hv, ev);
+        else {
+            H[q] = hv;
+            E[q] = ev;
+        }
+    }
+    __host__ __device__ __forceinline__ void move(int dst, int src) {
+        if constexpr (PACK) P[dst] = P[src];
+        else {
+            H[dst] = H[src];
+            E[dst] = E[src];
+        }
+    }
+    __host__ __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int q = 0; q < C; q++) set(q, padH, padE);
+    }
+    static __host__ __device__ __forceinline__ uint32_t pack16(int32_t hv, int32_t ev) {
+        int32_t lo = hv < -32768 ? -32768 : (hv > 32767 ? 32767 : hv);
+        int32_t hi = ev < -32768 ? -32768 : (ev > 32767 ? 32767 : ev);
+        return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);  // v_cvt_pk_i16_i32
+    }
+};
+
 // One row of the band.  MASK: some positions fall outside the band or the
 // window and must hold pads.
-template <int C, bool MASK>
-__device__ __forceinline__ Row band_row(int32_t (&H)[C], int32_t (&E)[C], const uint8_t *ref,
-                                        int32_t W, int32_t base, int32_t js, int32_t je,
-                                        uint8_t qb, const SwLaunch &a, uint32_t *trace_row,
-                                        int lane) {
+template <int C, bool PACK, bool MASK>
+__host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_t *ref,
+                                                 int32_t base, int32_t js, int32_t je, uint8_t qb,
+                                                 const SwLaunch &a, uint32_t *trace_row,
+                                                 int lane) {
     const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
-    const int32_t padH = kMin4 + go, padE = kMin4 + ge;
-    int32_t hl = padH, fl = padE;  // left neighbour: absent at the band start
+    // the F chain lives in registers only: exact i32 absent values
+    const int32_t fpadH = kMin4 + go, fpadF = kMin4 + ge;
+    int32_t hl = fpadH, fl = fpadF;
     Row r{0, -1};
     uint32_t word = 0;
     // Interior rows fetch the band's reference bytes (column j = base + q is
@@ -50,24 +108,24 @@ __device__ __forceinline__ Row band_row(int32_t (&H)[C], int32_t (&E)[C], const 
     // padded so the <= 8 bytes read past a window's ends stay allocated.
     constexpr int NW = (C + 3) / 4 + 1;
     const uint8_t *rp = ref + (base - 1);
-    const uintptr_t ra = (uintptr_t)rp;
-    const uint32_t *wp = (const uint32_t *)(ra & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(ra & 3) * 8u;
     uint32_t rw[NW] = {};
     if (!MASK) {
+        const uintptr_t ra = (uintptr_t)rp;
+        const uint32_t *wp = (const uint32_t *)(ra & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)(ra & 3) * 8u;
         uint32_t prev = wp[0];
 #pragma unroll
         for (int k = 0; k < NW - 1; k++) {
             uint32_t nx = wp[k + 1];
-            rw[k] = __builtin_amdgcn_alignbit(nx, prev, sh);
+            rw[k] = funnel(nx, prev, sh);
             prev = nx;
         }
     }
 #pragma unroll
     for (int q = 0; q < C; q++) {
-        int32_t d = H[q];
-        int32_t uh = q + 1 < C ? H[q + 1] : padH;
-        int32_t ue = q + 1 < C ? E[q + 1] : padE;
+        int32_t d = bd.h(q);
+        int32_t uh = q + 1 < C ? bd.h(q + 1) : bd.padH;
+        int32_t ue = q + 1 < C ? bd.e(q + 1) : bd.padE;
         bool valid = true;
         if (MASK) valid = (base + q >= js) && (base + q <= je);
         // edge rows may reach past the window (and the arena): guarded loads
@@ -82,63 +140,151 @@ __device__ __forceinline__ Row band_row(int32_t (&H)[C], int32_t (&E)[C], const 
         uint32_t dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
         uint32_t code = dir | (efe > efh ? 4u : 0u) | (fff > ffh ? 8u : 0u);
         if (MASK && !valid) {
-            h = padH;
-            e = padE;
-            f = padE;
+            bd.set(q, bd.padH, bd.padE);
+            hl = fpadH;
+            fl = fpadF;
             code = 0;
-        }
-        H[q] = h;
-        E[q] = e;
-        hl = h;
-        fl = f;
-        if (h > r.rb) {
-            r.rb = h;
-            r.rbq = q;
+        } else {
+            bd.set(q, h, e);
+            hl = h;
+            fl = f;
+            if (h > r.rb) {
+                r.rb = h;
+                r.rbq = q;
+            }
         }
         word |= code << (4 * (q & 7));
         if ((q & 7) == 7 || q == C - 1) {
             trace_row[(q >> 3) * 64 + lane] = word;
             word = 0;
         }
+        cell_fence();
     }
     return r;
 }
 
-// Shift the band arrays so that position q holds what was at q + delta.
-template <int C>
-__device__ __forceinline__ void shift_band(int32_t (&H)[C], int32_t (&E)[C], int32_t delta,
-                                           int32_t padH, int32_t padE) {
+// Shift the band so that position q holds what was at q + delta.
+template <int C, bool PACK>
+__host__ __device__ __forceinline__ void shift_band(Band<C, PACK> &bd, int32_t delta) {
     if (delta >= C || delta <= -C) {
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-            H[q] = padH;
-            E[q] = padE;
-        }
+        bd.clear();
         return;
     }
     while (delta > 0) {
 #pragma unroll
-        for (int q = 0; q < C - 1; q++) {
-            H[q] = H[q + 1];
-            E[q] = E[q + 1];
-        }
-        H[C - 1] = padH;
-        E[C - 1] = padE;
+        for (int q = 0; q < C - 1; q++) bd.move(q, q + 1);
+        bd.set(C - 1, bd.padH, bd.padE);
         delta--;
     }
     while (delta < 0) {
 #pragma unroll
-        for (int q = C - 1; q > 0; q--) {
-            H[q] = H[q - 1];
-            E[q] = E[q - 1];
-        }
-        H[0] = padH;
-        E[0] = padE;
+        for (int q = C - 1; q > 0; q--) bd.move(q, q - 1);
+        bd.set(0, bd.padH, bd.padE);
         delta++;
     }
 }
 
-template <int C>
+// One job's rows (banded_sw_with, mod.rs:1370-1579); rows <= the packet's
+// row count so the slab holds every row written.
+template <int C, bool PACK>
+__host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_t rows,
+                                uint32_t *slab, int2 *meta) {
+    const int32_t rowdw = (C + 7) / 8;
+    const int32_t L = a.read_len[job], W = a.ref_len[job];
+    const int32_t off = a.offset[job];
+    const int32_t w = imax(a.band ? a.band[job] : a.bandwidth, 1);
+    const int32_t abort_score = a.abort_score[job];
+    const uint8_t *read = a.reads + a.read_off[job];
+    const uint8_t *ref = a.refs + a.ref_off[job];
+    const int32_t D = a.adaptive ? a.max_drift : 0;
+    const int32_t xdrop = a.dp_xdrop;
+
+    Band<C, PACK> bd;
+    if constexpr (PACK) {
+        bd.padH = -16384;
+        bd.padE = -16384 - (a.go - a.gap_extend);
+    } else {
+        bd.padH = kMin4 + a.go;
+        bd.padE = kMin4 + a.gap_extend;
+    }
+    bd.clear();
+    int32_t ac = 1 + off;
+    int32_t prev_c = 0;
+    bool prev_empty = true;
+    int32_t best = 0, bi = 0, bj = 0, bq = 0, bqj = 0;
+    int32_t early = 0;
+    for (int32_t i = 1; i <= L && i <= rows; i++) {
+        const int32_t sc = i + off;
+        const int32_t c = a.adaptive ? imin(imax(ac, sc - D), sc + D) : sc;
+        const int32_t base = c - w;
+        const int32_t js = imax(base, 1), je = imin(c + w, W);
+        uint32_t *trow = slab + (int64_t)(i - 1) * rowdw * 64;
+        if (js > je) {
+            // empty row: the reference clears the previous row and skips the
+            // centre update and the stop checks (mod.rs:1436-1443)
+            bd.clear();
+            prev_empty = true;
+            meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, 1);
+            continue;
+        }
+        if (prev_empty) {
+            bd.clear();
+        } else {
+            int32_t delta = c - prev_c - 1;
+            if (delta != 0) shift_band<C, PACK>(bd, delta);
+        }
+        prev_c = c;
+        prev_empty = false;
+        const uint8_t qb = read[i - 1];
+        Row r;
+        if (js == base && je == base + C - 1)
+            r = band_row<C, PACK, false>(bd, ref, base, js, je, qb, a, trow, lane);
+        else
+            r = band_row<C, PACK, true>(bd, ref, base, js, je, qb, a, trow, lane);
+        meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, (js - base) | ((je - base) << 16));
+        const int32_t rbj = r.rbq >= 0 ? base + r.rbq : -1;
+        if (r.rb > best) {
+            best = r.rb;
+            bi = i;
+            bj = rbj;
+        }
+        if (i == L && r.rb > 0) {
+            bq = r.rb;
+            bqj = rbj;
+        }
+        if (a.adaptive) {
+            int32_t raw = rbj >= 0 ? rbj + 1 : ac + 1;
+            int32_t sn = i + 1 + off;
+            ac = imin(imax(raw, sn - D), sn + D);
+        }
+        if (xdrop > 0 && best - r.rb > xdrop) {
+            early = 1;
+            break;
+        }
+        if (abort_score > kMin8) {
+            int32_t remaining = L - i;
+            if (best + remaining * a.match_score < abort_score) {
+                early = 1;
+                break;
+            }
+        }
+    }
+    const int32_t max_h = best;
+    if (bq > 0 && sat_add(bq, a.clip_penalty) > best) {
+        best = bq;
+        bi = L;
+        bj = bqj;
+    }
+    int32_t *res = a.res + (int64_t)job * kSwResWords;
+    res[0] = best;
+    res[1] = bi;  // read_end
+    res[2] = bj;  // ref_end
+    res[3] = early;
+    // a packed band must never have clamped a score (the plan's bound)
+    res[7] = (PACK && max_h > 32767) ? 1 : 0;
+}
+
+template <int C, bool PACK>
 __global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
     const int lane = threadIdx.x;
     for (;;) {
@@ -147,119 +293,14 @@ __global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
         p = __shfl(p, 0);
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + lane];
-        const int32_t rows = a.packet_rows[p];
-        const int32_t rowdw = (C + 7) / 8;
-        uint32_t *slab = a.trace + a.packet_trace_off[p];
-        int2 *meta = a.row_meta + a.packet_meta_off[p];
-        if (job < 0) continue;
-        const int32_t L = a.read_len[job], W = a.ref_len[job];
-        const int32_t off = a.offset[job];
-        const int32_t w = imax(a.band ? a.band[job] : a.bandwidth, 1);
-        const int32_t abort_score = a.abort_score[job];
-        const uint8_t *read = a.reads + a.read_off[job];
-        const uint8_t *ref = a.refs + a.ref_off[job];
-        const int32_t go = a.go, ge = a.gap_extend;
-        const int32_t padH = kMin4 + go, padE = kMin4 + ge;
-        const int32_t D = a.adaptive ? a.max_drift : 0;
-        const int32_t xdrop = a.dp_xdrop;
-
-        int32_t H[C], E[C];
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-            H[q] = padH;
-            E[q] = padE;
-        }
-        int32_t ac = 1 + off;
-        int32_t prev_c = 0;
-        bool prev_empty = true;
-        int32_t best = 0, bi = 0, bj = 0, bq = 0, bqj = 0;
-        int32_t early = 0;
-        int32_t last_row = 0;
-        for (int32_t i = 1; i <= L && i <= rows; i++) {
-            last_row = i;
-            const int32_t sc = i + off;
-            const int32_t c = a.adaptive ? imin(imax(ac, sc - D), sc + D) : sc;
-            const int32_t base = c - w;
-            const int32_t js = imax(base, 1), je = imin(c + w, W);
-            uint32_t *trow = slab + (int64_t)(i - 1) * rowdw * 64;
-            if (js > je) {
-                // empty row: the reference clears the previous row and skips
-                // the centre update and the stop checks (mod.rs:1436-1443)
-#pragma unroll
-                for (int q = 0; q < C; q++) {
-                    H[q] = padH;
-                    E[q] = padE;
-                }
-                prev_empty = true;
-                meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, 1 | (0 << 16));
-                continue;
-            }
-            if (prev_empty) {
-#pragma unroll
-                for (int q = 0; q < C; q++) {
-                    H[q] = padH;
-                    E[q] = padE;
-                }
-            } else {
-                int32_t delta = c - prev_c - 1;
-                if (delta != 0) shift_band<C>(H, E, delta, padH, padE);
-            }
-            prev_c = c;
-            prev_empty = false;
-            const uint8_t qb = read[i - 1];
-            Row r;
-            if (js == base && je == base + C - 1 && 2 * w + 1 == C)
-                r = band_row<C, false>(H, E, ref, W, base, js, je, qb, a, trow, lane);
-            else
-                r = band_row<C, true>(H, E, ref, W, base, js, je, qb, a, trow, lane);
-            meta[(int64_t)(i - 1) * 64 + lane] =
-                make_int2(base, (js - base) | ((je - base) << 16));
-            const int32_t rbj = r.rbq >= 0 ? base + r.rbq : -1;
-            if (r.rb > best) {
-                best = r.rb;
-                bi = i;
-                bj = rbj;
-            }
-            if (i == L && r.rb > 0) {
-                bq = r.rb;
-                bqj = rbj;
-            }
-            if (a.adaptive) {
-                int32_t raw = rbj >= 0 ? rbj + 1 : ac + 1;
-                int32_t sn = i + 1 + off;
-                ac = imin(imax(raw, sn - D), sn + D);
-            }
-            if (xdrop > 0 && best - r.rb > xdrop) {
-                early = 1;
-                break;
-            }
-            if (abort_score > kMin8) {
-                int32_t remaining = L - i;
-                if (best + remaining * a.match_score < abort_score) {
-                    early = 1;
-                    break;
-                }
-            }
-        }
-        (void)last_row;
-        if (bq > 0 && sat_add(bq, a.clip_penalty) > best) {
-            best = bq;
-            bi = L;
-            bj = bqj;
-        }
-        int32_t *res = a.res + (int64_t)job * kSwResWords;
-        res[0] = best;
-        res[1] = bi;  // read_end
-        res[2] = bj;  // ref_end
-        res[3] = early;
+        if (job >= 0)
+            sw_job<C, PACK>(a, job, lane, a.packet_rows[p], a.trace + a.packet_trace_off[p],
+                      a.row_meta + a.packet_meta_off[p]);
     }
 }
 
 // Traceback (mod.rs:1581-1626) over the slab written by banded_sw_kernel.
-__global__ void banded_traceback_kernel(SwLaunch a, int32_t n_jobs) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_jobs) return;
-    const int32_t job = (int32_t)t;
+__host__ __device__ void sw_traceback_job(const SwLaunch &a, int32_t job) {
     const int32_t pl = a.job_slot[job];  // packet * 64 + lane
     const int32_t p = pl >> 6, lane = pl & 63;
     const int32_t rowdw = a.packet_rowdw[p];
@@ -322,22 +363,35 @@ __global__ void banded_traceback_kernel(SwLaunch a, int32_t n_jobs) {
     res[6] = n <= cap ? n : -1;
 }
 
+__global__ void banded_traceback_kernel(SwLaunch a, int32_t n_jobs) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_jobs) sw_traceback_job(a, (int32_t)t);
+}
+
 __global__ void zero_counter_kernel(int32_t *c) { *c = 0; }
 
 }  // namespace
 
-hipError_t launch_banded_sw(const SwLaunch &a, int C, int n_waves, hipStream_t s) {
+hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s) {
     zero_counter_kernel<<<1, 1, 0, s>>>(a.counter);
     if (a.n_packets > 0) {
+#define KIRA_SW_CASE(CC)                                                        \
+    case CC:                                                                    \
+        if (pack)                                                               \
+            banded_sw_kernel<CC, true><<<n_waves, 64, 0, s>>>(a);               \
+        else                                                                    \
+            banded_sw_kernel<CC, false><<<n_waves, 64, 0, s>>>(a);              \
+        break;
         switch (C) {
-        case 9: banded_sw_kernel<9><<<n_waves, 64, 0, s>>>(a); break;
-        case 17: banded_sw_kernel<17><<<n_waves, 64, 0, s>>>(a); break;
-        case 33: banded_sw_kernel<33><<<n_waves, 64, 0, s>>>(a); break;
-        case 65: banded_sw_kernel<65><<<n_waves, 64, 0, s>>>(a); break;
-        case 129: banded_sw_kernel<129><<<n_waves, 64, 0, s>>>(a); break;
-        case 257: banded_sw_kernel<257><<<n_waves, 64, 0, s>>>(a); break;
+            KIRA_SW_CASE(9)
+            KIRA_SW_CASE(17)
+            KIRA_SW_CASE(33)
+            KIRA_SW_CASE(65)
+            KIRA_SW_CASE(129)
+            KIRA_SW_CASE(257)
         default: return hipErrorInvalidValue;
         }
+#undef KIRA_SW_CASE
     }
     return hipGetLastError();
 }

@@ -14,13 +14,15 @@
 
 namespace {
 
-// Band classes: a job with band w runs in the smallest C >= 2w + 1.
+// Band classes: a job with band w runs in the smallest C >= 2w + 1, with the
+// band stored as packed int16 when its scores provably fit, else as int32.
 constexpr int kClasses[] = {9, 17, 33, 65, 129, 257};
-constexpr int kNumClasses = 6;
+constexpr int kNumBands = 6;
+constexpr int kNumClasses = 2 * kNumBands;  // class = band * 2 + packed
 
 int class_of(int32_t w) {
     int need = 2 * w + 1;
-    for (int k = 0; k < kNumClasses; k++)
+    for (int k = 0; k < kNumBands; k++)
         if (kClasses[k] >= need) return k;
     return -1;
 }
@@ -79,11 +81,17 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
     *out = nullptr;
     std::vector<int> cls(n);
+    const int32_t go = sat_add(cfg->gap_open, cfg->gap_extend);
+    const bool costs_small = cfg->match_score >= 0 && cfg->mismatch >= 0 && cfg->gap_open >= 0 &&
+                             cfg->gap_extend >= 0 && go <= 16000;
     for (int32_t k = 0; k < n; k++) {
         int32_t w = std::max(band ? band[k] : cfg->bandwidth, 1);
-        cls[k] = class_of(w);
-        if (cls[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "band wider than 128");
+        int b = class_of(w);
+        if (b < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "band wider than 128");
         if (read_len[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "negative read length");
+        // every H is at most read_len * match: int16 storage cannot clamp
+        bool pack = costs_small && (int64_t)read_len[k] * cfg->match_score < 32767;
+        cls[k] = 2 * b + (pack ? 1 : 0);
     }
     // packets: jobs sorted by (band class, read length descending)
     std::vector<int32_t> order(n);
@@ -107,7 +115,7 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
         if (p->class_first[c] < 0) p->class_first[c] = pk;
         p->class_count[c]++;
         int32_t rows = read_len[order[s]];
-        int32_t rowdw = (kClasses[c] + 7) / 8;
+        int32_t rowdw = (kClasses[c / 2] + 7) / 8;
         for (int32_t l = 0; l < 64; l++) {
             int32_t job = s + l < e ? order[s + l] : -1;
             pj.push_back(job);
@@ -227,7 +235,7 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
         ac.n_packets = plan->class_count[c];
         ac.counter = plan->counter + c;
         int waves = std::min(plan->class_count[c], 256 * 16);
-        if ((e = kira::launch_banded_sw(ac, kClasses[c], waves, ctx->stream)))
+        if ((e = kira::launch_banded_sw(ac, kClasses[c / 2], (c & 1) != 0, waves, ctx->stream)))
             return fail(ctx, KIRA_ERR_RUNTIME, "banded SW launch", e);
     }
     a.packet_rowdw = plan->packet_rowdw;
@@ -323,6 +331,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
         o.n_cigar = r[6];
         o.reserved = 0;
         if (r[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
+        if (r[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
         if (pos + r[6] > arena_cap || (r[6] > 0 && !cigar_arena))
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "cigar arena too small");
         o.cigar_off = pos;

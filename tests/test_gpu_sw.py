@@ -56,7 +56,12 @@ def make_jobs(seed, n, bands=(16, 32, 64, 128), lmin=20, lmax=300):
 
 
 def check(pkg, ctx, jobs, c, kn_gpu, kn_or):
+    import time
+    t0 = time.time()
+    print(f"[sw] batch of {len(jobs)} jobs, bands {sorted(set(j[4] for j in jobs))}",
+          flush=True)
     got = pkg.banded_sw_batch(ctx, jobs, c, kn_gpu)
+    print(f"[sw] gpu done in {time.time() - t0:.2f}s", flush=True)
     for k, (read, ref, off, abort, w) in enumerate(jobs):
         cc = O.cfg(c.match_score, c.mismatch, c.gap_open, c.gap_extend, w, c.xdrop,
                    c.clip_penalty)
