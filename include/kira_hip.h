@@ -185,6 +185,44 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
                                  const kira_sw_cfg *cfg, const kira_sw_knobs *knobs,
                                  kira_sw_result *out, uint32_t *cigar_arena, int64_t arena_cap);
 
+/* BatchInput — src/alignment/mod.rs:48-56 (host pointers). */
+typedef struct {
+    const uint8_t *read;
+    uint32_t read_len;
+    const uint8_t *window;
+    uint32_t window_len;
+    uint32_t win_start;
+    uint32_t chain_ref_start, chain_ref_end, chain_read_start, chain_read_end;
+    int32_t abort_score;
+} kira_batch_input;
+
+/* SimdMode — src/simd/mod.rs:5-13. */
+typedef enum {
+    KIRA_SIMD_SCALAR = 0,
+    KIRA_SIMD_AVX2 = 1,
+    KIRA_SIMD_AVXVNNI = 2,
+    KIRA_SIMD_NEON = 3
+} kira_simd_mode;
+
+/*
+ * align_batch_simd's Smith-Waterman (src/alignment/mod.rs:1111-1149) for
+ * n_batches calls at once: batch b is inputs[batch_off[b] .. batch_off[b+1]),
+ * exactly the slice the reference passes to one align_batch_simd call (the
+ * lane-group size and the early stop depend on that grouping).  mode is the
+ * requested SimdMode and detected the host's (detect_cached); the same
+ * dispatch follows: mixed (read, window) lengths or the scalar mode run the
+ * adaptive banded SW per job with the clamped chain offset (mod.rs:60-72);
+ * AVX2 runs 16-lane full-window groups behind its int16 cost gate
+ * (:1854-1868), AVX-VNNI one int8 group when viable (sw_int8_vnni.rs:47-55),
+ * NEON 4-lane groups.  out[k] is the SwResult of inputs[k] (soft clips are
+ * added by build_alignment, :1200-1254).
+ */
+kira_status kira_align_batch_simd(kira_ctx *ctx, int32_t n_batches, const int32_t *batch_off,
+                                  const kira_batch_input *inputs, kira_simd_mode mode,
+                                  kira_simd_mode detected, const kira_sw_cfg *cfg,
+                                  const kira_sw_knobs *knobs, kira_sw_result *out,
+                                  uint32_t *cigar_arena, int64_t arena_cap);
+
 /*
  * Device-resident form for batches that repeat a shape: a plan fixes the
  * job count, read lengths and bands (it sorts jobs into 64-job packets by

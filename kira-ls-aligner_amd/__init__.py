@@ -71,6 +71,19 @@ class SwResult(C.Structure):
                 ("n_cigar", C.c_int32), ("reserved", C.c_int32), ("cigar_off", C.c_int64)]
 
 
+class BatchInput(C.Structure):
+    """kira_batch_input = BatchInput (src/alignment/mod.rs:48-56)."""
+
+    _fields_ = [("read", C.c_char_p), ("read_len", C.c_uint32), ("window", C.c_char_p),
+                ("window_len", C.c_uint32), ("win_start", C.c_uint32),
+                ("chain_ref_start", C.c_uint32), ("chain_ref_end", C.c_uint32),
+                ("chain_read_start", C.c_uint32), ("chain_read_end", C.c_uint32),
+                ("abort_score", C.c_int32)]
+
+
+SIMD_MODES = {"scalar": 0, "avx2": 1, "avxvnni": 2, "neon": 3}
+
+
 class SwJobsDev(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
                                           "ref_len", "offset", "band", "abort_score", "res",
@@ -99,6 +112,9 @@ SIGNATURES = {
     "kira_banded_sw_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P,
                                        _P, _P, _P, _P, C.POINTER(SwCfg), C.POINTER(SwKnobs),
                                        _P, _P, C.c_int64]),
+    "kira_align_batch_simd": (C.c_int, [_P, C.c_int32, _P, C.POINTER(BatchInput), C.c_int,
+                                        C.c_int, C.POINTER(SwCfg), C.POINTER(SwKnobs), _P, _P,
+                                        C.c_int64]),
     "kira_banded_sw_plan_create": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(SwCfg),
                                              C.POINTER(C.c_void_p)]),
     "kira_banded_sw_plan_destroy": (None, [_P]),
@@ -479,3 +495,41 @@ def banded_sw_batch(ctx: Context, jobs, cfg: SwCfg, knobs: Optional[SwKnobs] = N
                         read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
                         cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
     return res
+
+
+def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",
+                     detected: Optional[str] = None, knobs: Optional[SwKnobs] = None):
+    """align_batch_simd's SW results (src/alignment/mod.rs:1111-1149) for several
+    calls at once.  batches: list of lists of BatchInput-like tuples
+    (read, window, win_start, chain_ref_start, chain_ref_end, chain_read_start,
+    chain_read_end, abort_score).  Returns one list of result dicts per batch."""
+    knobs = knobs or sw_knobs()
+    detected = mode if detected is None else detected
+    flat = [j for b in batches for j in b]
+    off = np.zeros(len(batches) + 1, dtype=np.int32)
+    off[1:] = np.cumsum([len(b) for b in batches])
+    n = len(flat)
+    if n == 0:
+        return [[] for _ in batches]
+    arr = (BatchInput * n)()
+    keep = []
+    for k, j in enumerate(flat):
+        rb = C.create_string_buffer(bytes(j[0]), len(j[0]))
+        wb = C.create_string_buffer(bytes(j[1]), len(j[1]))
+        keep += [rb, wb]
+        arr[k] = BatchInput(C.cast(rb, C.c_char_p), len(j[0]), C.cast(wb, C.c_char_p), len(j[1]),
+                            *j[2:8])
+    out = (SwResult * n)()
+    cap = int(sum(3 * len(j[0]) + 2 for j in flat) + 2)
+    arena = np.zeros(cap, dtype=np.uint32)
+    st = lib().kira_align_batch_simd(ctx.h, len(batches), _np_ptr(off), arr, SIMD_MODES[mode],
+                                     SIMD_MODES[detected], C.byref(cfg), C.byref(knobs), out,
+                                     _np_ptr(arena), cap)
+    ctx.check(st, "align_batch_simd")
+    res = []
+    for k in range(n):
+        o = out[k]
+        res.append(dict(ref_start=o.ref_start, ref_end=o.ref_end, read_start=o.read_start,
+                        read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
+                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+    return [res[off[b]:off[b + 1]] for b in range(len(batches))]

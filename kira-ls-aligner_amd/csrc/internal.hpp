@@ -92,6 +92,39 @@ struct SwLaunch {
 };
 
 hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
+
+// Full-window lane-group SW (align_batch_simd's SIMD kernels).
+struct GroupLaunch {
+    int32_t match_score, mismatch, go, gap_extend, clip_penalty;
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *refs;
+    const int64_t *ref_off;
+    const int32_t *ref_len;
+    const int32_t *abort_score;
+    const int32_t *group_of;  // per job
+    const int32_t *neg;       // per group: row-0 E / column-0 F sentinel of the lane type
+    int32_t *group_stop;      // per group
+    const int32_t *packet_jobs;
+    const int32_t *packet_rows;
+    const int32_t *packet_words;
+    const int64_t *packet_trace_off;
+    const int64_t *packet_row_off;
+    int32_t n_packets;
+    int32_t *counter;
+    uint32_t *trace;
+    int2 *bnd;
+    int2 *rstat;
+    const int32_t *job_slot;
+    int32_t *res;
+    uint32_t *cigar;
+    const int64_t *cigar_off;
+    const int32_t *cigar_cap;
+};
+
+hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,
+                           hipStream_t s);
 hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s);
 
 }  // namespace kira
