@@ -127,7 +127,7 @@ typedef struct {
     int32_t *out_best;   /* best count or INT32_MAX */
     int32_t *out_second; /* second-best count or INT32_MAX */
     uint8_t *out_flags;  /* may be NULL: bit0 dispatched (read and window all
-                            ACGT, S9), bit1 best <= limit, bit2 certified */
+                            ACGT), bit1 best <= limit, bit2 certified */
     /* certification (spectral_hit_is_certified, src/alignment/mod.rs:688-762):
        cert_max_mism < 0 disables it */
     int32_t cert_max_mism;

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) 
     int64_t s0 = a.win_start[j];
     int32_t lim = a.limit ? a.limit[j] : a.limit_all;
 
-    // S9: skip (never dispatch) jobs whose read or window is not all ACGT, and
+    // Skip (never dispatch, stage4_alignment.rs:1284-1288) jobs whose read or window is not all ACGT, and
     // windows that leave the text.
     bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
               s0 + W <= a.g_bases && L <= 32 * RW;

@@ -5,16 +5,16 @@
  * it, always as the checker or the timed CPU baseline, never as the product.
  *
  * Parity pins (checked in tests/test_oracle_*.py):
- *   S1/S2  pack / pre-shift: reference known answers (tests/bitpacked_scan.rs:44-59,
+ *   pack / pre-shift: reference known answers (tests/bitpacked_scan.rs:44-59,
  *          tests/unit/alignment_bitpacked.rs:4-9,53-57) + closed form.
- *   S3/S4  brute-force Hamming over every shift (tests/bitpacked_scan.rs:13-33),
+ *   mismatch count / scan: brute-force Hamming over every shift (tests/bitpacked_scan.rs:13-33),
  *          global-best + tied-second known answers (tests/unit/alignment_bitpacked.rs:59-76).
- *   S5     the reference's GPU corpus (tests/cuda_spectral.rs:13-137) vs brute force.
- *   S7/S8  closed-form rules + hand cases; MD/NM known answers (md.rs:214-285).
- *   S10    tests/prefilter_accept.rs (5 gate outcomes), tests/ungapped_stats.rs.
- *   A3-A7  tests/alignment.rs known answers; full-matrix brute force with the
+ *   batch backend: the reference's GPU corpus (tests/cuda_spectral.rs:13-137) vs brute force.
+ *   certification: closed-form rules + hand cases; MD/NM known answers (md.rs:214-285).
+ *   prefilter: tests/prefilter_accept.rs (5 gate outcomes), tests/ungapped_stats.rs.
+ *   lane-group SW: tests/alignment.rs known answers; full-matrix brute force with the
  *          same tie rules on tiny inputs; CIGAR replay = score.
- *   A8     src/alignment/mod.rs:2368-2635 known answers; replay invariant.
+ *   banded SW: src/alignment/mod.rs:2368-2635 known answers; replay invariant.
  */
 #include "kira_oracle.h"
 
@@ -28,7 +28,7 @@
 #define I32_MIN_8 (INT32_MIN / 8)
 
 /* ======================================================================
- * S1 — encode_base / pack_into : src/alignment/bitpacked.rs:12-20, 57-91
+ * encode_base / pack_into : src/alignment/bitpacked.rs:12-20, 57-91
  * ====================================================================== */
 static inline uint8_t encode_base(uint8_t b) {
     switch (b) {
@@ -55,7 +55,7 @@ int ko_pack(const uint8_t *seq, size_t n, uint8_t *bits) {
     return valid;
 }
 
-/* S2 — shift_bits_right_into / pre_shift_into : bitpacked.rs:95-119 */
+/* shift_bits_right_into / pre_shift_into : bitpacked.rs:95-119 */
 void ko_shift_bits_right(const uint8_t *bits, size_t n, unsigned bit_offset, uint8_t *out) {
     if (bit_offset == 0) {
         memcpy(out, bits, n);
@@ -73,7 +73,7 @@ void ko_pre_shift(const uint8_t *bits, size_t n, uint8_t *out4) {
         ko_shift_bits_right(bits, n, phase * 2, out4 + phase * n);
 }
 
-/* S3 — mismatch_count_u64 : bitpacked.rs:238-270.  (The AVX2 and scalar
+/* mismatch_count_u64 : bitpacked.rs:238-270.  (The AVX2 and scalar
  * variants at :272-354 return the same count; only the <=64-byte u64 form is
  * ever reached by the reads this path handles.) */
 static inline uint32_t popc64(uint64_t x) { return (uint32_t)__builtin_popcountll(x); }
@@ -108,7 +108,7 @@ uint32_t ko_mismatch_count(const uint8_t *read, const uint8_t *ref, size_t n_byt
     return total;
 }
 
-/* S4 — scan_best_with_second_raw : bitpacked.rs:154-199 */
+/* scan_best_with_second_raw : bitpacked.rs:154-199 */
 int ko_scan_best_with_second(const uint8_t *read_bits, size_t read_bytes, size_t read_len,
                              int read_valid, const uint8_t *phases, size_t phase_len,
                              size_t ref_len, uint64_t *shift, uint64_t *best,
@@ -137,7 +137,7 @@ int ko_scan_best_with_second(const uint8_t *read_bits, size_t read_bytes, size_t
     return 1;
 }
 
-/* S5 — CudaBackend::run_batch (src/cuda/backend.rs:96-249) with the kernel's
+/* CudaBackend::run_batch (src/cuda/backend.rs:96-249) with the kernel's
  * result conventions (src/cuda/spectral.cu:149-212): limit =
  * min(max_mismatches, read_nucs); read_len == 0 or ref_len < read_len gives
  * (-1, INT32_MAX, INT32_MAX); a rejected best keeps its second. */
@@ -252,13 +252,13 @@ void ko_backend_run_batch_mt(int32_t n, const uint8_t *reads_packed, int32_t rea
     free(tk);
 }
 
-/* S6 — router::spectral_max_mismatches : src/alignment/router.rs:191-195 */
+/* router::spectral_max_mismatches : src/alignment/router.rs:191-195 */
 uint32_t ko_spectral_max_mismatches(uint32_t read_len, uint32_t pct, uint32_t floor_) {
     uint32_t v = (uint32_t)(((uint64_t)read_len * pct) / 100);
     return v > floor_ ? v : floor_;
 }
 
-/* S7 — spectral_hit_is_certified + bounded_hamming : src/alignment/mod.rs:688-776 */
+/* spectral_hit_is_certified + bounded_hamming : src/alignment/mod.rs:688-776 */
 static size_t bounded_hamming(const uint8_t *a, size_t alen, const uint8_t *b, size_t blen,
                               size_t limit) {
     if (alen != blen) return limit + 1;
@@ -439,7 +439,7 @@ size_t ko_cigar_string(const uint32_t *cigar, int32_t n, char *out, size_t cap) 
     return len;
 }
 
-/* S9 — build_simd_window : src/pipeline/stage4_alignment.rs:1426-1445 */
+/* build_simd_window : src/pipeline/stage4_alignment.rs:1426-1445 */
 int ko_build_simd_window(uint64_t ref_len, uint32_t chain_ref_start, uint32_t chain_read_start,
                          uint32_t read_len, int32_t bandwidth, uint32_t *win_start,
                          uint32_t *win_len) {
@@ -455,7 +455,7 @@ int ko_build_simd_window(uint64_t ref_len, uint32_t chain_ref_start, uint32_t ch
 }
 
 /* ======================================================================
- * S10 — prefilter : src/alignment/prefilter.rs:82-453
+ * prefilter : src/alignment/prefilter.rs:82-453
  * count_mismatches (kira-simd 0.1.0 via src/simd/mod.rs:56-58) = number of
  * unequal bytes of two equal-length slices.
  * ====================================================================== */
@@ -676,7 +676,7 @@ static inline int32_t clamp32(int32_t v, int32_t lo, int32_t hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-/* A8 — banded_sw_with : src/alignment/mod.rs:1370-1650 (prev_cell /
+/* banded_sw_with : src/alignment/mod.rs:1370-1650 (prev_cell /
  * prev_diag :1723-1754, push_cigar :1756-1764). */
 void ko_banded_sw(const uint8_t *read, size_t q_len, const uint8_t *ref, size_t r_len,
                   int32_t offset, const ko_cfg *cfg, int32_t abort_score, const ko_knobs *kn,
@@ -911,7 +911,7 @@ static inline int32_t lane_add(int32_t a, int32_t b, int bits) {
     return (int32_t)s;
 }
 
-/* A3/A4/A5/A7 — sw_batch_avx2_inner (mod.rs:1914-2135), sw_batch_int8_inner
+/* sw_batch_avx2_inner (mod.rs:1914-2135), sw_batch_int8_inner
  * (sw_int8_vnni.rs:141-371), sw_batch_neon (mod.rs:2139-2324): full-window
  * local affine SW over a lane group with the group-wide early stop. */
 void ko_sw_group(const ko_batch_input *in, int32_t n, const ko_cfg *cfg, int bits,

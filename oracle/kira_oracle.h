@@ -57,7 +57,7 @@ typedef struct {
     int32_t cigar_cap;
 } ko_sw_result;
 
-/* ---------------- sieve (S-rows) ---------------- */
+/* ---------------- sieve ---------------- */
 int ko_pack(const uint8_t *seq, size_t n, uint8_t *bits);
 void ko_shift_bits_right(const uint8_t *bits, size_t n, unsigned bit_offset, uint8_t *out);
 void ko_pre_shift(const uint8_t *bits, size_t n, uint8_t *out4);
@@ -88,7 +88,7 @@ int ko_build_simd_window(uint64_t ref_len, uint32_t chain_ref_start, uint32_t ch
                          uint32_t read_len, int32_t bandwidth, uint32_t *win_start,
                          uint32_t *win_len);
 
-/* ---------------- prefilter (S10) ---------------- */
+/* ---------------- ungapped prefilter ---------------- */
 typedef struct {
     uint32_t ref_start, ref_end, read_start, read_end;
 } ko_span;
@@ -119,7 +119,7 @@ void ko_prefilter_chain(const uint8_t *read, size_t read_len, const uint8_t *ref
                         int tail_ext, int32_t ungap_min_id /* -1 = unset */,
                         ko_prefilter_out *out, char *md, size_t md_cap);
 
-/* ---------------- Smith-Waterman (A-rows) ---------------- */
+/* ---------------- Smith-Waterman ---------------- */
 void ko_banded_sw(const uint8_t *read, size_t q_len, const uint8_t *ref, size_t r_len,
                   int32_t offset, const ko_cfg *cfg, int32_t abort_score, const ko_knobs *kn,
                   ko_sw_result *res);

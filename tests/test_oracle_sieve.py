@@ -1,4 +1,4 @@
-"""Pins the CPU oracle's sieve rows (S1-S9) against the reference's own known
+"""Pins the CPU oracle's sieve functions against the reference's own known
 answers and against brute force on the raw bytes."""
 import random
 
@@ -10,7 +10,7 @@ from tests.refgen import XorShift, cuda_corpus, naive_best_second, naive_mismatc
 I32MAX = 2 ** 31 - 1
 
 
-# --- S1/S2: tests/bitpacked_scan.rs:44-59, tests/unit/alignment_bitpacked.rs:4-57
+# --- pack / pre-shift: tests/bitpacked_scan.rs:44-59, tests/unit/alignment_bitpacked.rs:4-57
 def test_pack_roundtrip_short():
     bits, valid = O.pack(b"ACGTACGT")
     assert bits == bytes([0xE4, 0xE4]) and valid
@@ -56,7 +56,7 @@ def test_pre_shift_closed_form():
             assert len(ph[p]) == len(bits)
 
 
-# --- S3/S4
+# --- mismatch count and scan
 def test_scan_exact_match_at_zero():
     r = b"ACGTACGTACGT"
     assert O.scan_best_with_second(r, r)[:2] == (0, 0)
@@ -116,7 +116,7 @@ def test_mismatch_count_tail_masking():
         assert O.mismatch_count(pa, pb[:len(pa)], n) == naive_mismatches(a, b, 0)
 
 
-# --- S5: the reference's GPU corpus, tests/cuda_spectral.rs:20-137
+# --- batch backend: the reference's GPU corpus, tests/cuda_spectral.rs:20-137
 def test_backend_corpus_matches_brute_force():
     corpus = cuda_corpus(64)
     res = O.backend_run_batch(corpus)
@@ -143,7 +143,7 @@ def test_backend_sentinels():
     assert res[3] == (-1, I32MAX, 3)  # rejected best (3 > 1) keeps its second
 
 
-# --- S6/S7/S8/S9
+# --- limits, certification, tags, windows
 def test_spectral_max_mismatches():
     assert O.spectral_max_mismatches(150) == 4
     assert O.spectral_max_mismatches(50) == 2
