@@ -185,6 +185,60 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
                                  const kira_sw_cfg *cfg, const kira_sw_knobs *knobs,
                                  kira_sw_result *out, uint32_t *cigar_arena, int64_t arena_cap);
 
+/* ------------------------------------------------------------------ prefilter */
+
+/* AnchorSpan coordinates — src/alignment/mod.rs:38-46. */
+typedef struct {
+    uint32_t ref_start, ref_end, read_start, read_end;
+} kira_span;
+
+/* prefilter_chain arguments that are the same for a whole batch
+ * (src/alignment/prefilter.rs:93-109) plus its environment knobs:
+ * tail_ext KIRA_UNGAP_TAIL (:318-326, default 1), ungap_min_id
+ * KIRA_UNGAP_MINID (:331-338, -1 = unset).  The opt-in WFA gap probe
+ * (KIRA_UNGAP_GAP_CHECK, default off) is not supported. */
+typedef struct {
+    int32_t accept_enable, accept_only_top1, multi_alignments_enabled, tail_ext;
+    int32_t accept_span_slack, accept_max_mismatches, accept_require_score_margin;
+    int32_t long_read_threshold;
+    float accept_min_identity;
+    int32_t ungap_min_id;
+} kira_prefilter_params;
+
+/* PrefilterOutcome (prefilter.rs:62-80) with the accepted alignment of
+ * build_ungapped_alignment (:455-517).  result 0 Accept, 1 Reject,
+ * 2 Fallback; reason = PrefilterReason::idx() (:31-44), Other(..) as
+ * 9 + 16 * k with k = 1 read_len0, 2 non_monotonic, 3 anchor0, 4 span0,
+ * 5 neg_start, 6 span_oob, 7 margin_fail.  cigar: len << 4 | op (M0 S4);
+ * the MD string is host work (md.rs MdBuilder). */
+typedef struct {
+    int32_t result, reason, has_metrics;
+    uint16_t len, mism, matches, identity_x10000;
+    int32_t score;
+    uint64_t read_start, ref_start;
+    uint32_t aln_ref_start, aln_ref_end, aln_read_start, aln_read_end;
+    int32_t aln_score;
+    uint32_t aln_nm;
+    int32_t n_cigar;
+    uint32_t cigar[3];
+} kira_prefilter_out;
+
+/*
+ * prefilter_chain (src/alignment/prefilter.rs:93-295) for n chains, host
+ * arrays: chain k's read is reads[read_off[k] .. +read_len[k]], its contig
+ * refs[ref_off[k] .. +ref_len[k]], anchor spans[k]; is_top1 / score_margin
+ * may be NULL (true / INT32_MAX).  A span reaching past its read or contig
+ * is rejected with KIRA_ERR_INVALID_ARGUMENT (the reference would panic).
+ */
+kira_status kira_prefilter_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
+                                 int64_t reads_bytes, const int64_t *read_off,
+                                 const int32_t *read_len, const uint8_t *refs,
+                                 int64_t refs_bytes, const int64_t *ref_off,
+                                 const int64_t *ref_len, const kira_span *spans,
+                                 const uint8_t *is_top1, const int32_t *score_margin,
+                                 const kira_sw_cfg *cfg, const kira_prefilter_params *params,
+                                 kira_prefilter_out *out);
+
 /* BatchInput — src/alignment/mod.rs:48-56 (host pointers). */
 typedef struct {
     const uint8_t *read;

@@ -84,6 +84,46 @@ class BatchInput(C.Structure):
 SIMD_MODES = {"scalar": 0, "avx2": 1, "avxvnni": 2, "neon": 3}
 
 
+class Span(C.Structure):
+    """kira_span = AnchorSpan (src/alignment/mod.rs:38-46)."""
+
+    _fields_ = [("ref_start", C.c_uint32), ("ref_end", C.c_uint32), ("read_start", C.c_uint32),
+                ("read_end", C.c_uint32)]
+
+
+class PrefilterParams(C.Structure):
+    """kira_prefilter_params: prefilter_chain's per-batch arguments
+    (src/alignment/prefilter.rs:93-109) and KIRA_UNGAP_TAIL / KIRA_UNGAP_MINID."""
+
+    _fields_ = [(n, C.c_int32) for n in (
+        "accept_enable", "accept_only_top1", "multi_alignments_enabled", "tail_ext",
+        "accept_span_slack", "accept_max_mismatches", "accept_require_score_margin",
+        "long_read_threshold")] + [("accept_min_identity", C.c_float), ("ungap_min_id", C.c_int32)]
+
+
+class PrefilterOut(C.Structure):
+    """kira_prefilter_out = PrefilterOutcome (src/alignment/prefilter.rs:62-80)."""
+
+    _fields_ = [("result", C.c_int32), ("reason", C.c_int32), ("has_metrics", C.c_int32),
+                ("len", C.c_uint16), ("mism", C.c_uint16), ("matches", C.c_uint16),
+                ("identity_x10000", C.c_uint16), ("score", C.c_int32),
+                ("read_start", C.c_uint64), ("ref_start", C.c_uint64),
+                ("aln_ref_start", C.c_uint32), ("aln_ref_end", C.c_uint32),
+                ("aln_read_start", C.c_uint32), ("aln_read_end", C.c_uint32),
+                ("aln_score", C.c_int32), ("aln_nm", C.c_uint32), ("n_cigar", C.c_int32),
+                ("cigar", C.c_uint32 * 3)]
+
+
+# PrefilterResult, and PrefilterReason::idx() -> as_str() (src/alignment/
+# prefilter.rs:31-58); Other(..) carries the sub-reason as 9 + 16 * k.
+PREFILTER_RESULTS = ("accept", "reject", "fallback")
+PREFILTER_REASONS = ("accept", "disabled", "not_short_preset", "not_top1",
+                     "max_alignments_not_one", "span_too_small", "mism_too_high", "id_too_low",
+                     "indel_suspect")
+PREFILTER_OTHER = (None, "read_len0", "non_monotonic", "anchor0", "span0", "neg_start",
+                   "span_oob", "margin_fail")
+
+
 class SwJobsDev(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
                                           "ref_len", "offset", "band", "abort_score", "res",
@@ -115,6 +155,9 @@ SIGNATURES = {
     "kira_align_batch_simd": (C.c_int, [_P, C.c_int32, _P, C.POINTER(BatchInput), C.c_int,
                                         C.c_int, C.POINTER(SwCfg), C.POINTER(SwKnobs), _P, _P,
                                         C.c_int64]),
+    "kira_prefilter_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P,
+                                       _P, C.POINTER(Span), _P, _P, C.POINTER(SwCfg),
+                                       C.POINTER(PrefilterParams), C.POINTER(PrefilterOut)]),
     "kira_banded_sw_plan_create": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(SwCfg),
                                              C.POINTER(C.c_void_p)]),
     "kira_banded_sw_plan_destroy": (None, [_P]),
@@ -533,3 +576,61 @@ def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",
                         read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
                         cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
     return [res[off[b]:off[b + 1]] for b in range(len(batches))]
+
+
+# ------------------------------------------------------------ ungapped prefilter
+def prefilter_params(accept_enable=True, accept_only_top1=True, multi_alignments_enabled=False,
+                     tail_ext=True, accept_span_slack=5, accept_max_mismatches=2,
+                     accept_require_score_margin=0, long_read_threshold=300,
+                     accept_min_identity=98.0, ungap_min_id=-1) -> PrefilterParams:
+    return PrefilterParams(int(accept_enable), int(accept_only_top1),
+                           int(multi_alignments_enabled), int(tail_ext), accept_span_slack,
+                           accept_max_mismatches, accept_require_score_margin,
+                           long_read_threshold, accept_min_identity, ungap_min_id)
+
+
+def prefilter_batch(ctx: Context, chains, cfg: SwCfg, params: Optional[PrefilterParams] = None):
+    """prefilter_chain (src/alignment/prefilter.rs:93-295) for a batch of chains.
+
+    chains: sequence of (read: bytes, ref: bytes, span: (ref_start, ref_end,
+    read_start, read_end), is_top1: bool = True, score_margin: int = i32::MAX).
+    Returns dicts {result, reason, metrics?, alignment?}; the alignment's MD
+    string is left to the caller (md_from_cigar)."""
+    params = params or prefilter_params()
+    n = len(chains)
+    if n == 0:
+        return []
+    reads = b"".join(c[0] for c in chains)
+    refs = b"".join(c[1] for c in chains)
+    rl = np.array([len(c[0]) for c in chains], dtype=np.int32)
+    fl = np.array([len(c[1]) for c in chains], dtype=np.int64)
+    ro = np.concatenate([[0], np.cumsum(rl[:-1], dtype=np.int64)]).astype(np.int64)
+    fo = np.concatenate([[0], np.cumsum(fl[:-1])]).astype(np.int64)
+    spans = (Span * n)(*[Span(*c[2]) for c in chains])
+    top = np.array([int(c[3]) if len(c) > 3 else 1 for c in chains], dtype=np.uint8)
+    margin = np.array([c[4] if len(c) > 4 else I32_MAX for c in chains], dtype=np.int32)
+    rbuf = np.frombuffer(reads, dtype=np.uint8) if reads else np.zeros(1, np.uint8)
+    fbuf = np.frombuffer(refs, dtype=np.uint8) if refs else np.zeros(1, np.uint8)
+    out = (PrefilterOut * n)()
+    st = lib().kira_prefilter_batch(ctx.h, n, _np_ptr(rbuf), len(reads), _np_ptr(ro),
+                                    _np_ptr(rl), _np_ptr(fbuf), len(refs), _np_ptr(fo),
+                                    _np_ptr(fl), spans, _np_ptr(top), _np_ptr(margin),
+                                    C.byref(cfg), C.byref(params), out)
+    ctx.check(st, "prefilter_batch")
+    res = []
+    for k in range(n):
+        o = out[k]
+        r = o.reason
+        d = dict(result=PREFILTER_RESULTS[o.result],
+                 reason=PREFILTER_REASONS[r] if r < 9 else PREFILTER_OTHER[(r - 9) // 16])
+        if o.has_metrics:
+            d["metrics"] = dict(len=o.len, mism=o.mism, matches=o.matches,
+                                identity_x10000=o.identity_x10000, score=o.score,
+                                read_start=o.read_start, ref_start=o.ref_start)
+        if o.result == 0:
+            d["alignment"] = dict(ref_start=o.aln_ref_start, ref_end=o.aln_ref_end,
+                                  read_start=o.aln_read_start, read_end=o.aln_read_end,
+                                  score=o.aln_score, nm=o.aln_nm,
+                                  cigar=[int(o.cigar[i]) for i in range(o.n_cigar)])
+        res.append(d)
+    return res

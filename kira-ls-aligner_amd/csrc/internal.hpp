@@ -125,6 +125,25 @@ struct GroupLaunch {
 
 hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,
                            hipStream_t s);
+
+struct PrefilterLaunch {
+    int64_t n;
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *refs;
+    const int64_t *ref_off;
+    const int64_t *ref_len;
+    const kira_span *spans;
+    const uint8_t *is_top1;
+    const int32_t *score_margin;
+    int32_t match_score, mismatch, xdrop;
+    uint16_t id_floor;  // identity floor after KIRA_UNGAP_MINID
+    kira_prefilter_params p;
+    kira_prefilter_out *out;
+};
+
+hipError_t launch_prefilter(const PrefilterLaunch &a, hipStream_t s);
 hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s);
 
 }  // namespace kira
