@@ -140,6 +140,58 @@ def make_sieve_workload(n_reads=2_000_000, read_len=150, windows=16, band=32, su
                 repeat_reads=int(inside.sum()))
 
 
+def make_sw_workload(genome, n_pairs=1_000_000, seed=3, pad=1024, chunk=100_000):
+    """Uneven banded SW pairs: read length U{100..300}, band U{16,32,64,128},
+    window = clamp_window geometry (flank max(2w, 50) around the read's locus,
+    src/alignment/mod.rs:1766-1771), read drawn from the locus with a per-pair
+    substitution rate U[1%, 5%] and 0.5% indel events (half insertions, half
+    deletions, 1 bp).  Windows are slices of the resident genome: refs is the
+    genome with `pad` zero bytes on both sides and ref_off points into it."""
+    rng = np.random.default_rng(seed)
+    G = len(genome)
+    L = rng.integers(100, 301, n_pairs).astype(np.int64)
+    band = rng.choice(np.array([16, 32, 64, 128], dtype=np.int64), n_pairs)
+    flank = np.maximum(2 * band, 50)
+    sub = rng.uniform(0.01, 0.05, n_pairs)
+    pos = rng.integers(300, G - 700, n_pairs).astype(np.int64)
+    codes_g = np.searchsorted(ACGT, genome)  # genome is ACGT only
+    reads, read_len = [], np.empty(n_pairs, dtype=np.int32)
+    for s in range(0, n_pairs, chunk):
+        e = min(n_pairs, s + chunk)
+        n = e - s
+        slen = L[s:e] + 16
+        start = np.repeat(pos[s:e], slen)
+        base_off = np.arange(int(slen.sum())) - np.repeat(np.cumsum(slen) - slen, slen)
+        src = codes_g[start + base_off].astype(np.uint8)
+        tot = len(src)
+        u = rng.random(tot)
+        is_del = u < 0.0025
+        is_ins = (u >= 0.0025) & (u < 0.005)
+        subm = rng.random(tot) < np.repeat(sub[s:e], slen)
+        src[subm] = (src[subm] + rng.integers(1, 4, int(subm.sum()), dtype=np.uint8)) % 4
+        cnt = 1 - is_del.astype(np.int64) + is_ins.astype(np.int64)
+        out = np.repeat(src, cnt)
+        first = np.cumsum(cnt) - cnt
+        out[first[is_ins]] = rng.integers(0, 4, int(is_ins.sum()), dtype=np.uint8)
+        pair = np.repeat(np.repeat(np.arange(n), slen), cnt)
+        emitted = np.bincount(pair, minlength=n)
+        local = np.arange(len(out)) - np.repeat(np.cumsum(emitted) - emitted, emitted)
+        keep = local < np.repeat(L[s:e], emitted)
+        reads.append(ACGT[out[keep]])
+        read_len[s:e] = np.minimum(L[s:e], emitted)
+    reads = np.concatenate(reads)
+    read_off = np.concatenate([[0], np.cumsum(read_len[:-1], dtype=np.int64)])
+    win_start = pos - flank
+    win_end = np.minimum(pos + L + flank, G)
+    refs = np.zeros(G + 2 * pad, dtype=np.uint8)
+    refs[pad:pad + G] = genome
+    return dict(reads=reads, read_off=read_off.astype(np.int64), read_len=read_len,
+                refs=refs, ref_off=(win_start + pad).astype(np.int64),
+                ref_len=(win_end - win_start).astype(np.int32),
+                offset=(pos - win_start).astype(np.int32), band=band.astype(np.int32),
+                abort=np.full(n_pairs, -(2 ** 31) // 8, dtype=np.int32), n=n_pairs)
+
+
 # --------------------------------------------------------------- CPU baseline
 _LUT = np.zeros(256, dtype=np.uint8)
 for _c, _v in zip(b"ACGTacgt", [0, 1, 2, 3, 0, 1, 2, 3]):

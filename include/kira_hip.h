@@ -159,10 +159,12 @@ typedef struct {
 
 /* SwPublicResult — src/alignment/mod.rs:904-915.  The CIGAR (soft clips not
  * included) is n_cigar words at cigar_arena[cigar_off], each len << 4 | op
- * with op M=0 I=1 D=2 N=3 S=4. */
+ * with op M=0 I=1 D=2 N=3 S=4.  cells: DP cells the reference algorithm
+ * evaluates for the job (sum of band row widths over the rows it processes;
+ * rows x window for the full-window lane groups), the unit of GCUPS. */
 typedef struct {
     uint32_t ref_start, ref_end;
-    int32_t read_start, read_end, score, early_abort, n_cigar, reserved;
+    int32_t read_start, read_end, score, early_abort, n_cigar, cells;
     int64_t cigar_off;
 } kira_sw_result;
 
@@ -283,8 +285,9 @@ kira_status kira_align_batch_simd(kira_ctx *ctx, int32_t n_batches, const int32_
  * band class and length and sizes the trace slabs); run_dev then only
  * launches kernels on the context stream.  All kira_sw_jobs_dev pointers are
  * device pointers; refs must have at least 1024 readable bytes before and
- * after every window.  res: n x 8 int32 (score, read_end, ref_end,
- * early_abort, ref_start, read_start, n_cigar, 0); cigar: the plan's
+ * after every window.  res: n x 10 int32 (score, read_end, ref_end,
+ * early_abort, ref_start, read_start, n_cigar, int16 overflow flag (never set
+ * by a correct plan), cells, 0); cigar: the plan's
  * cigar_words, job k's ops at word kira_banded_sw_plan_cigar_off(plan)[k].
  */
 typedef struct kira_sw_plan kira_sw_plan;

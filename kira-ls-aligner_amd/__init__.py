@@ -68,7 +68,7 @@ class SwResult(C.Structure):
 
     _fields_ = [("ref_start", C.c_uint32), ("ref_end", C.c_uint32), ("read_start", C.c_int32),
                 ("read_end", C.c_int32), ("score", C.c_int32), ("early_abort", C.c_int32),
-                ("n_cigar", C.c_int32), ("reserved", C.c_int32), ("cigar_off", C.c_int64)]
+                ("n_cigar", C.c_int32), ("cells", C.c_int32), ("cigar_off", C.c_int64)]
 
 
 class BatchInput(C.Structure):
@@ -536,8 +536,60 @@ def banded_sw_batch(ctx: Context, jobs, cfg: SwCfg, knobs: Optional[SwKnobs] = N
         o = out[k]
         res.append(dict(ref_start=o.ref_start, ref_end=o.ref_end, read_start=o.read_start,
                         read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
-                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]],
+                        cells=int(o.cells)))
     return res
+
+
+class BandedSwPlan:
+    """kira_sw_plan: the packet schedule and trace slabs of one batch shape
+    (read lengths and bands); reusable for every batch of that shape."""
+
+    def __init__(self, ctx: Context, read_len: np.ndarray, band: Optional[np.ndarray],
+                 cfg: SwCfg):
+        rl = np.ascontiguousarray(read_len, dtype=np.int32)
+        bd = None if band is None else np.ascontiguousarray(band, dtype=np.int32)
+        h = C.c_void_p()
+        st = lib().kira_banded_sw_plan_create(ctx.h, len(rl), _np_ptr(rl),
+                                              None if bd is None else _np_ptr(bd),
+                                              C.byref(cfg), C.byref(h))
+        ctx.check(st, "banded_sw_plan_create")
+        self.h, self.n = h, len(rl)
+        self.cigar_words = int(lib().kira_banded_sw_plan_cigar_words(h))
+        ptr = lib().kira_banded_sw_plan_cigar_off(h)
+        self.cigar_off = (np.ctypeslib.as_array(ptr, shape=(self.n,)).copy() if self.n
+                          else np.zeros(0, dtype=np.int64))
+
+    def close(self):
+        if self.h:
+            lib().kira_banded_sw_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def banded_sw_run_dev(ctx: Context, plan: BandedSwPlan, *, reads, read_off, read_len, refs,
+                      ref_off, ref_len, offset, abort_score, res, cigar, cfg: SwCfg,
+                      band=None, knobs: Optional[SwKnobs] = None, stream=None):
+    """banded_sw_public over device-resident jobs (torch CUDA tensors): reads /
+    refs uint8, *_off int64, *_len / offset / band / abort_score int32, res int32
+    [n, 10], cigar int32 or uint32 [plan.cigar_words].  refs must hold 1024
+    readable bytes before and after every window.  Launches on `stream`
+    (default: the current stream) and does not synchronise."""
+    import torch
+    knobs = knobs or sw_knobs()
+    if stream is None:
+        stream = torch.cuda.current_stream(reads.device)
+    ctx.set_stream(stream.cuda_stream)
+    j = SwJobsDev(*[_dev_ptr(t) if t is not None else None for t in (
+        reads, read_off, read_len, refs, ref_off, ref_len, offset, band, abort_score, res,
+        cigar)])
+    ctx.check(lib().kira_banded_sw_run_dev(ctx.h, plan.h, C.byref(j), C.byref(cfg),
+                                           C.byref(knobs)), "banded_sw_run_dev")
 
 
 def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",
@@ -574,7 +626,8 @@ def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",
         o = out[k]
         res.append(dict(ref_start=o.ref_start, ref_end=o.ref_end, read_start=o.read_start,
                         read_end=o.read_end, score=o.score, early_abort=bool(o.early_abort),
-                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]],
+                        cells=int(o.cells)))
     return [res[off[b]:off[b + 1]] for b in range(len(batches))]
 
 

@@ -56,8 +56,9 @@ hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stri
 hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
 
 // Banded SW: results per job, kSwResWords int32 each: score, read_end,
-// ref_end, early_abort, ref_start, read_start, n_cigar, int16-overflow flag.
-constexpr int kSwResWords = 8;
+// ref_end, early_abort, ref_start, read_start, n_cigar, int16-overflow flag,
+// DP cells the reference algorithm evaluates for the job, 0.
+constexpr int kSwResWords = 10;
 // Reference arena padding (bytes) so band rows that poke past a window's
 // ends stay inside the allocation.
 constexpr int64_t kRefPad = 1024;

@@ -212,7 +212,7 @@ __host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_
     int32_t prev_c = 0;
     bool prev_empty = true;
     int32_t best = 0, bi = 0, bj = 0, bq = 0, bqj = 0;
-    int32_t early = 0;
+    int32_t early = 0, cells = 0;
     for (int32_t i = 1; i <= L && i <= rows; i++) {
         const int32_t sc = i + off;
         const int32_t c = a.adaptive ? imin(imax(ac, sc - D), sc + D) : sc;
@@ -235,6 +235,7 @@ __host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_
         }
         prev_c = c;
         prev_empty = false;
+        cells += je - js + 1;
         const uint8_t qb = read[i - 1];
         Row r;
         if (js == base && je == base + C - 1)
@@ -282,6 +283,8 @@ __host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_
     res[3] = early;
     // a packed band must never have clamped a score (the plan's bound)
     res[7] = (PACK && max_h > 32767) ? 1 : 0;
+    res[8] = cells;
+    res[9] = 0;
 }
 
 template <int C, bool PACK>

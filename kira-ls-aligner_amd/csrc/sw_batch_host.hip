@@ -299,7 +299,7 @@ extern "C" kira_status kira_align_batch_simd(kira_ctx *ctx, int32_t n_batches,
             o.ref_start = (uint32_t)r[4];
             o.read_start = r[5];
             o.n_cigar = r[6];
-            o.reserved = 0;
+            o.cells = r[8];
             cig[gj[t]].assign(hcig.begin() + coff[t], hcig.begin() + coff[t] + r[6]);
         }
     }

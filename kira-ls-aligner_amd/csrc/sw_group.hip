@@ -244,6 +244,8 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     res[5] = i;
     res[6] = n <= cap ? n : -1;
     res[7] = 0;
+    res[8] = S * a.ref_len[job];  // the reference evaluates S full rows
+    res[9] = 0;
 }
 
 __global__ void group_finish_kernel(GroupLaunch a, int32_t n) {

@@ -329,7 +329,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
         o.ref_start = (uint32_t)r[4];
         o.read_start = r[5];
         o.n_cigar = r[6];
-        o.reserved = 0;
+        o.cells = r[8];
         if (r[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
         if (r[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
         if (pos + r[6] > arena_cap || (r[6] > 0 && !cigar_arena))

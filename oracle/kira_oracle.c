@@ -692,6 +692,10 @@ void ko_banded_sw(const uint8_t *read, size_t q_len, const uint8_t *ref, size_t 
     uint8_t **trace = (uint8_t **)calloc(q_len + 1, sizeof(uint8_t *));
     size_t cap = (size_t)2 * (size_t)band + 2;
     if (cap > r_len + 1) cap = r_len + 1;
+    /* one trace slab for the call (the reference recycles per-thread row
+     * buffers, mod.rs:1387-1409); row i uses its first row_len[i] bytes */
+    uint8_t *trace_slab = (uint8_t *)malloc((q_len + 1) * cap);
+    int64_t cells = 0;
     int32_t *prev_h = (int32_t *)malloc(cap * sizeof(int32_t));
     int32_t *prev_e = (int32_t *)malloc(cap * sizeof(int32_t));
     int32_t *cur_h = (int32_t *)malloc(cap * sizeof(int32_t));
@@ -724,7 +728,8 @@ void ko_banded_sw(const uint8_t *read, size_t q_len, const uint8_t *ref, size_t 
         size_t rl = (size_t)(j_end - j_start + 1);
         row_starts[i] = j_start;
         row_len[i] = (int32_t)rl;
-        trace[i] = (uint8_t *)malloc(rl);
+        trace[i] = trace_slab + i * cap;
+        cells += (int64_t)rl;
         int32_t row_best = 0, row_best_j = -1;
         for (int32_t j = j_start; j <= j_end; j++) {
             size_t idx = (size_t)(j - j_start);
@@ -843,7 +848,8 @@ void ko_banded_sw(const uint8_t *read, size_t q_len, const uint8_t *ref, size_t 
     res->read_end = read_end;
     res->score = best_score;
     res->early_abort = early;
-    for (size_t k = 0; k <= q_len; k++) free(trace[k]);
+    res->cells = cells;
+    free(trace_slab);
     free(trace);
     free(row_starts);
     free(row_len);
@@ -958,6 +964,7 @@ void ko_sw_group(const ko_batch_input *in, int32_t n, const ko_cfg *cfg, int bit
     for (int k = 0; k < lanes; k++) {
         best[k] = bi[k] = bj[k] = bq[k] = bqj[k] = done[k] = sat[k] = 0;
     }
+    size_t rows_done = q;
     for (size_t i = 1; i <= q; i++) {
         for (int k = 0; k < lanes; k++) {
             cur_h[k] = 0;
@@ -1013,7 +1020,10 @@ void ko_sw_group(const ko_batch_input *in, int32_t n, const ko_cfg *cfg, int bit
             }
             if (!done[k]) all_done = 0;
         }
-        if (all_done) break;
+        if (all_done) {
+            rows_done = i;
+            break;
+        }
         int32_t *t;
         t = prev_h; prev_h = cur_h; cur_h = t;
         t = prev_e; prev_e = cur_e; cur_e = t;
@@ -1041,6 +1051,7 @@ void ko_sw_group(const ko_batch_input *in, int32_t n, const ko_cfg *cfg, int bit
         res[k].read_end = si;
         res[k].score = bs;
         res[k].early_abort = done[k];
+        res[k].cells = (int64_t)(rows_done * r);
     }
     free(trace);
     free(prev_h);
@@ -1136,10 +1147,11 @@ typedef struct {
     int32_t lo, hi;
     const uint8_t *reads, *refs;
     const int64_t *read_off, *ref_off;
-    const int32_t *read_len, *ref_len, *offsets, *abort_scores;
+    const int32_t *read_len, *ref_len, *offsets, *abort_scores, *bands;
     const ko_cfg *cfg;
     const ko_knobs *kn;
     int32_t *out_score, *out_end;
+    int64_t *out_cells;
 } sw_task;
 
 static void *sw_worker(void *arg) {
@@ -1150,10 +1162,13 @@ static void *sw_worker(void *arg) {
         memset(&r, 0, sizeof(r));
         r.cigar = cig;
         r.cigar_cap = 4096;
+        ko_cfg c = *t->cfg;
+        if (t->bands) c.bandwidth = t->bands[k];
         ko_banded_sw(t->reads + t->read_off[k], (size_t)t->read_len[k], t->refs + t->ref_off[k],
-                     (size_t)t->ref_len[k], t->offsets[k], t->cfg, t->abort_scores[k], t->kn, &r);
+                     (size_t)t->ref_len[k], t->offsets[k], &c, t->abort_scores[k], t->kn, &r);
         t->out_score[k] = r.score;
         t->out_end[k] = (int32_t)r.ref_end;
+        if (t->out_cells) t->out_cells[k] = r.cells;
     }
     return NULL;
 }
@@ -1161,8 +1176,9 @@ static void *sw_worker(void *arg) {
 void ko_banded_sw_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_off,
                            const int32_t *read_len, const uint8_t *refs, const int64_t *ref_off,
                            const int32_t *ref_len, const int32_t *offsets,
-                           const int32_t *abort_scores, const ko_cfg *cfg, const ko_knobs *kn,
-                           int32_t *out_score, int32_t *out_end, int threads) {
+                           const int32_t *abort_scores, const int32_t *bands,
+                           const ko_cfg *cfg, const ko_knobs *kn, int32_t *out_score,
+                           int32_t *out_end, int64_t *out_cells, int threads) {
     if (threads < 1) threads = 1;
     pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
     sw_task *tk = (sw_task *)calloc((size_t)threads, sizeof(sw_task));
@@ -1178,6 +1194,8 @@ void ko_banded_sw_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_
         t->ref_len = ref_len;
         t->offsets = offsets;
         t->abort_scores = abort_scores;
+        t->bands = bands;
+        t->out_cells = out_cells;
         t->cfg = cfg;
         t->kn = kn;
         t->out_score = out_score;

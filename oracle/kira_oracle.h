@@ -55,6 +55,7 @@ typedef struct {
     int32_t n_cigar;
     uint32_t *cigar; /* caller-owned, capacity cigar_cap */
     int32_t cigar_cap;
+    int64_t cells; /* banded SW: DP cells evaluated (sum of row widths) */
 } ko_sw_result;
 
 /* ---------------- sieve ---------------- */
@@ -161,12 +162,14 @@ uint32_t ko_nm_md_from_cigar(const uint8_t *read, size_t read_len, const uint8_t
                              const uint32_t *cigar, int32_t n_cigar, char *md, size_t md_cap);
 size_t ko_cigar_string(const uint32_t *cigar, int32_t n, char *out, size_t cap);
 
-/* CPU baselines, parallel over jobs with pthreads. */
+/* CPU baselines, parallel over jobs with pthreads.  bands (nullable) replaces
+ * cfg->bandwidth per job; out_cells (nullable) receives each job's cell count. */
 void ko_banded_sw_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_off,
                            const int32_t *read_len, const uint8_t *refs, const int64_t *ref_off,
                            const int32_t *ref_len, const int32_t *offsets,
-                           const int32_t *abort_scores, const ko_cfg *cfg, const ko_knobs *kn,
-                           int32_t *out_score, int32_t *out_end, int threads);
+                           const int32_t *abort_scores, const int32_t *bands, const ko_cfg *cfg,
+                           const ko_knobs *kn, int32_t *out_score, int32_t *out_end,
+                           int64_t *out_cells, int threads);
 
 #ifdef __cplusplus
 }

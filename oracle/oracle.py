@@ -36,7 +36,7 @@ class SwResult(C.Structure):
     _fields_ = [("ref_start", C.c_uint32), ("ref_end", C.c_uint32), ("read_start", C.c_int32),
                 ("read_end", C.c_int32), ("score", C.c_int32), ("early_abort", C.c_int32),
                 ("n_cigar", C.c_int32), ("cigar", C.POINTER(C.c_uint32)),
-                ("cigar_cap", C.c_int32)]
+                ("cigar_cap", C.c_int32), ("cells", C.c_int64)]
 
 
 class BatchInput(C.Structure):
@@ -130,8 +130,8 @@ def lib():
         L.ko_nm_md_from_cigar.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                           C.c_size_t, C.c_size_t, C.POINTER(C.c_uint32),
                                           C.c_int32, C.c_char_p, C.c_size_t]
-        L.ko_banded_sw_batch_mt.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, C.POINTER(Cfg),
-                                            C.POINTER(Knobs), P, P, C.c_int]
+        L.ko_banded_sw_batch_mt.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P,
+                                            C.POINTER(Cfg), C.POINTER(Knobs), P, P, P, C.c_int]
         _lib = L
     return _lib
 
@@ -320,7 +320,7 @@ def cigar_str(ops: Sequence[int]) -> str:
 def _sw_dict(r: SwResult):
     return dict(ref_start=r.ref_start, ref_end=r.ref_end, read_start=r.read_start,
                 read_end=r.read_end, score=r.score, early_abort=bool(r.early_abort),
-                cigar=[r.cigar[k] for k in range(r.n_cigar)])
+                cigar=[r.cigar[k] for k in range(r.n_cigar)], cells=int(r.cells))
 
 
 def _new_result(cap):
@@ -428,3 +428,26 @@ def nm_md_from_cigar(read: bytes, ref: bytes, read_start: int, ref_start: int, c
     nm = lib().ko_nm_md_from_cigar(read, len(read), ref, len(ref), read_start, ref_start, arr,
                                    len(ops), md, len(md))
     return nm, md.value.decode()
+
+
+def banded_sw_batch_mt(reads, read_off, read_len, refs, ref_off, ref_len, offsets, aborts,
+                       c: Cfg, kn: Knobs = None, bands=None, threads=1):
+    """banded_sw_with over many jobs on `threads` pthreads (flat uint8 buffers,
+    int64 offsets, int32 lengths).  Returns (score, ref_end, cells) arrays."""
+    kn = kn or knobs()
+    n = len(read_len)
+    score = np.zeros(n, dtype=np.int32)
+    end = np.zeros(n, dtype=np.int32)
+    cells = np.zeros(n, dtype=np.int64)
+    arrs = [np.ascontiguousarray(reads, dtype=np.uint8),
+            np.ascontiguousarray(read_off, dtype=np.int64),
+            np.ascontiguousarray(read_len, dtype=np.int32),
+            np.ascontiguousarray(refs, dtype=np.uint8),
+            np.ascontiguousarray(ref_off, dtype=np.int64),
+            np.ascontiguousarray(ref_len, dtype=np.int32),
+            np.ascontiguousarray(offsets, dtype=np.int32),
+            np.ascontiguousarray(aborts, dtype=np.int32)]
+    b = None if bands is None else np.ascontiguousarray(bands, dtype=np.int32)
+    lib().ko_banded_sw_batch_mt(n, *[_p(a) for a in arrs], None if b is None else _p(b),
+                                C.byref(c), C.byref(kn), _p(score), _p(end), _p(cells), threads)
+    return score, end, cells

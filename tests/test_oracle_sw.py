@@ -170,6 +170,10 @@ def test_gapless_adaptive_equals_fixed():
     read = ref[10:90]
     ad = O.banded_sw(read, ref, 10, _cfg(6), ABORT_NONE, O.knobs(adaptive=True))
     fx = O.banded_sw(read, ref, 10, _cfg(6), ABORT_NONE, O.knobs(adaptive=False))
+    # same alignment; the two bands cover different cells on some rows, so
+    # only the evaluated cell counts differ
+    ad.pop("cells")
+    fx.pop("cells")
     assert ad == fx and _del_len(ad["cigar"]) == 0 and ad["read_end"] - ad["read_start"] >= 79
 
 
