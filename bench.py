@@ -262,6 +262,9 @@ def main():
     ap.add_argument("--reads", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample-reads", type=int, default=131_072)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sw-pairs", type=int, default=1_000_000)
+    ap.add_argument("--sw-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; --gpus must be 1 "
@@ -395,7 +398,94 @@ def main():
                           "ops_per_launch": ops_per_launch},
         "cpu_baseline": cpu,
     }
+    if args.sw_pairs > 0:
+        line["sw"] = bench_sw(pkg, ctx, work["genome"], args, stream)
+        line["sw_gcups"] = line["sw"]["gcups"]
     print(json.dumps(line), flush=True)
+
+
+SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
+
+
+def bench_sw(pkg, ctx, genome, args, stream):
+    """Uneven banded SW (1M pairs, L U{100..300}, band U{16,32,64,128}, adaptive,
+    x-drop 100, no abort) in one device-resident batch; GCUPS over the cells
+    the reference algorithm evaluates."""
+    import torch
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    sw = make_sw_workload(genome, n_pairs=args.sw_pairs)
+    n = sw["n"]
+    print(f"[bench] SW workload: {n} pairs generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    cfg = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
+    kn = pkg.sw_knobs()
+    t0 = time.time()
+    plan = pkg.BandedSwPlan(ctx, sw["read_len"], sw["band"], cfg)
+    t_plan = time.time() - t0
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d = dict(reads=T(sw["reads"]), read_off=T(sw["read_off"]), read_len=T(sw["read_len"]),
+             refs=T(sw["refs"]), ref_off=T(sw["ref_off"]), ref_len=T(sw["ref_len"]),
+             offset=T(sw["offset"]), band=T(sw["band"]), abort_score=T(sw["abort"]))
+    res = torch.empty((n, 10), dtype=torch.int32, device=dev)
+    cig = torch.empty(max(plan.cigar_words, 1), dtype=torch.int32, device=dev)
+
+    def run():
+        pkg.banded_sw_run_dev(ctx, plan, res=res, cigar=cig, cfg=cfg, knobs=kn, stream=stream,
+                              **d)
+
+    run()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.sw_steps)]
+    t0 = time.perf_counter()
+    for k in range(args.sw_steps):
+        evs[k][0].record(stream)
+        run()
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000 / args.sw_steps
+    ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    r = res.cpu().numpy()
+    cells = int(r[:, 8].astype(np.int64).sum())
+    gcups = cells / (ms * 1e-3) / 1e9
+    print(f"[bench] SW: {ms:.2f} ms/batch ({wall_ms:.2f} wall), {cells / n:.0f} cells/pair, "
+          f"{gcups:.1f} GCUPS", file=sys.stderr, flush=True)
+    out = {"workload": "uneven banded SW: 1M pairs, L U{100..300}, band U{16,32,64,128}, "
+                       "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
+                       "routed per job",
+           "pairs": n, "cells": cells, "ms_per_batch": ms, "wall_ms_per_batch": wall_ms,
+           "gcups": gcups, "plan_ms": t_plan * 1000, "steps": args.sw_steps,
+           "roofline": {"bound": "valu", "achieved": cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12,
+                        "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                        "frac": cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                        "ops_per_cell": SW_OPS_PER_CELL},
+           "parity_vs_oracle_sample": None, "cpu_baseline": None}
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        m = min(args.cpu_sample_pairs, n)
+        threads = len(os.sched_getaffinity(0))
+        oc = O.cfg(1, 4, 6, 1, 32, 100, 5)
+        a = [sw["reads"], sw["read_off"][:m], sw["read_len"][:m], sw["refs"], sw["ref_off"][:m],
+             sw["ref_len"][:m], sw["offset"][:m], sw["abort"][:m]]
+        O.banded_sw_batch_mt(*a, oc, bands=sw["band"][:m], threads=threads)  # warm-up
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sc, en, cc = O.banded_sw_batch_mt(*a, oc, bands=sw["band"][:m], threads=threads)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        out["parity_vs_oracle_sample"] = bool(
+            np.array_equal(sc, r[:m, 0]) and np.array_equal(en, r[:m, 2]) and
+            np.array_equal(cc, r[:m, 8].astype(np.int64)))
+        out["cpu_baseline"] = {"value": float(cc.sum()) / t / 1e9, "unit": "GCUPS",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {m} pairs of the same batch (scalar "
+                                         f"banded_sw_with, pthreads over pairs), median of 3"}
+        print(f"[bench] SW cpu baseline {cc.sum() / t / 1e9:.2f} GCUPS on {threads} threads; "
+              f"parity={out['parity_vs_oracle_sample']}", file=sys.stderr, flush=True)
+    plan.close()
+    return out
 
 
 if __name__ == "__main__":
