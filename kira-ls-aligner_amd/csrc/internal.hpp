@@ -79,6 +79,7 @@ struct SwLaunch {
     const int32_t *packet_jobs;
     const int32_t *packet_rows;
     const int32_t *packet_rowdw;
+    const int32_t *packet_P;  // 0: lane-per-job packet; P: 16-lane jobs, P positions a lane
     const int64_t *packet_trace_off;
     const int64_t *packet_meta_off;
     int32_t n_packets;
@@ -93,6 +94,8 @@ struct SwLaunch {
 };
 
 hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
+// Wide bands: 16 lanes per job, P band positions per lane (P in 9, 17, 25, 33).
+hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int n_waves, hipStream_t s);
 
 // Full-window lane-group SW (align_batch_simd's SIMD kernels).
 struct GroupLaunch {

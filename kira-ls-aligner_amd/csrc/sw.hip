@@ -307,6 +307,7 @@ __host__ __device__ void sw_traceback_job(const SwLaunch &a, int32_t job) {
     const int32_t pl = a.job_slot[job];  // packet * 64 + lane
     const int32_t p = pl >> 6, lane = pl & 63;
     const int32_t rowdw = a.packet_rowdw[p];
+    const int32_t P = a.packet_P[p];
     const uint32_t *slab = a.trace + a.packet_trace_off[p];
     const int2 *meta = a.row_meta + a.packet_meta_off[p];
     int32_t *res = a.res + (int64_t)job * kSwResWords;
@@ -321,7 +322,12 @@ __host__ __device__ void sw_traceback_job(const SwLaunch &a, int32_t job) {
         int32_t base = m.x, js = base + (m.y & 0xFFFF), je = base + (m.y >> 16);
         if (j < js || j > je) break;
         int32_t q = j - base;
-        uint32_t tr = (slab[((int64_t)(i - 1) * rowdw + (q >> 3)) * 64 + lane] >> (4 * (q & 7))) & 15u;
+        int32_t col = lane;
+        if (P > 0) {  // 16-lane job: position q sits in lane q / P at q % P
+            col += q / P;
+            q -= (q / P) * P;
+        }
+        uint32_t tr = (slab[((int64_t)(i - 1) * rowdw + (q >> 3)) * 64 + col] >> (4 * (q & 7))) & 15u;
         int op = -1;
         if (state == 0) {
             uint32_t d = tr & 3u;

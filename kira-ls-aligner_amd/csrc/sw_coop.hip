@@ -1,0 +1,340 @@
+// Banded adaptive Smith-Waterman (banded_sw_with, src/alignment/mod.rs:1370-1650)
+// for wide bands: one DPP row of 16 lanes runs one job.
+//
+// A lane-per-job band of 2w + 1 > 129 cells no longer fits the register
+// file, so here the band of row i is spread over the 16 lanes of a row
+// group: lane g holds positions q = g*P .. g*P + P - 1 (diagonal
+// coordinates, column j = c_i - w + q, as in the lane-per-job kernel), and
+// C = 16 P >= 2w + 1 (positions past 2w are always outside the band).
+//
+// Per row every cell's diagonal / E term is local (the up neighbour of a
+// lane's last position is the next lane's first, one DPP move).  The
+// horizontal gap F is a max-plus prefix: with go = open + extend >= extend,
+//   F[q] = max(Hpre[q-1] - go, F[q-1] - ge),   Hpre = max(diag + s, E, 0)
+// equals the reference's F over the final H, so each lane runs its own
+// chain, a 4-step DPP scan carries the chains across lanes (decaying by
+// P * ge per lane), and each position takes max(local, carried - p * ge).
+// The row's first strict arg-max is a 4-step DPP rotate-reduce of
+// (score << 10 | 1023 - q).  Everything else (pads for absent cells, trace
+// codes, centre update, x-drop / abort stops, last-row clip rule) follows
+// the lane-per-job kernel, so both give identical results.
+//
+// Trace slab per packet of 4 jobs: [row][word][wave lane], word k of lane g
+// holding the 4-bit codes of positions g*P + 8k .. g*P + 8k + 7.
+#include "internal.hpp"
+
+namespace kira {
+namespace {
+
+constexpr int32_t kMin4 = INT32_MIN / 4;
+constexpr int32_t kMin8 = INT32_MIN / 8;
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+
+// DPP row controls (16-lane rows): row_shl:n, lane l reads lane l + n;
+// row_shr:n, lane l reads lane l - n; row_ror:n rotates within the row.
+// Lanes whose source falls outside the row keep `fill`.
+template <int N>
+__device__ __forceinline__ int32_t from_next(int32_t v, int32_t fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x100 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ int32_t from_prev(int32_t v, int32_t fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ int32_t rotate(int32_t v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+}
+
+template <int P>
+struct CoopBand {
+    int32_t H[P], E[P];
+    int32_t padH, padE;
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            H[p] = padH;
+            E[p] = padE;
+        }
+    }
+    // new[q] = old[q + delta] over the whole 16-lane band
+    __device__ __forceinline__ void shift(int32_t delta) {
+        if (delta >= 16 * P || delta <= -16 * P) {
+            clear();
+            return;
+        }
+        while (delta >= P) {
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                H[p] = from_next<1>(H[p], padH);
+                E[p] = from_next<1>(E[p], padE);
+            }
+            delta -= P;
+        }
+        while (delta <= -P) {
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                H[p] = from_prev<1>(H[p], padH);
+                E[p] = from_prev<1>(E[p], padE);
+            }
+            delta += P;
+        }
+        while (delta > 0) {
+            int32_t nh = from_next<1>(H[0], padH), ne = from_next<1>(E[0], padE);
+#pragma unroll
+            for (int p = 0; p < P - 1; p++) {
+                H[p] = H[p + 1];
+                E[p] = E[p + 1];
+            }
+            H[P - 1] = nh;
+            E[P - 1] = ne;
+            delta--;
+        }
+        while (delta < 0) {
+            int32_t nh = from_prev<1>(H[P - 1], padH), ne = from_prev<1>(E[P - 1], padE);
+#pragma unroll
+            for (int p = P - 1; p > 0; p--) {
+                H[p] = H[p - 1];
+                E[p] = E[p - 1];
+            }
+            H[0] = nh;
+            E[0] = ne;
+            delta++;
+        }
+    }
+};
+
+// One job on the 16 lanes of a DPP row; g = lane within the row, wl = wave
+// lane of g = 0 (the job's slot column in the packet's meta slab).
+template <int P>
+__device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t rows,
+                         uint32_t *slab, int2 *meta) {
+    constexpr int RW = (P + 7) / 8;
+    constexpr int NW = (P + 3) / 4 + 1;
+    const int32_t L = a.read_len[job], W = a.ref_len[job];
+    const int32_t off = a.offset[job];
+    const int32_t w = imax(a.band ? a.band[job] : a.bandwidth, 1);
+    const int32_t abort_score = a.abort_score[job];
+    const uint8_t *read = a.reads + a.read_off[job];
+    const uint8_t *ref = a.refs + a.ref_off[job];
+    const int32_t D = a.adaptive ? a.max_drift : 0;
+    const int32_t xdrop = a.dp_xdrop;
+    const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
+    const int32_t q0 = g * P;
+
+    CoopBand<P> bd;
+    bd.padH = kMin4 + go;
+    bd.padE = kMin4 + ge;
+    bd.clear();
+    int32_t ac = 1 + off, prev_c = 0;
+    bool prev_empty = true;
+    int32_t best = 0, bi = 0, bj = 0, bq = 0, bqj = 0, early = 0, cells = 0;
+    for (int32_t i = 1; i <= L && i <= rows; i++) {
+        const int32_t sc = i + off;
+        const int32_t c = a.adaptive ? imin(imax(ac, sc - D), sc + D) : sc;
+        const int32_t base = c - w;
+        const int32_t js = imax(base, 1), je = imin(c + w, W);
+        uint32_t *trow = slab + (int64_t)(i - 1) * RW * 64 + wl + g;
+        if (js > je) {
+            bd.clear();
+            prev_empty = true;
+            if (g == 0) meta[(int64_t)(i - 1) * 64 + wl] = make_int2(base, 1);
+            continue;
+        }
+        if (prev_empty) {
+            bd.clear();
+        } else {
+            int32_t delta = c - prev_c - 1;
+            if (delta != 0) bd.shift(delta);
+        }
+        prev_c = c;
+        prev_empty = false;
+        cells += je - js + 1;
+        const uint8_t qb = read[i - 1];
+        // reference bytes of this lane's columns base + q0 + p (ref[j - 1]);
+        // every window has 1024 readable bytes on both sides and a non-empty
+        // row keeps these columns within 530 of the window
+        const uintptr_t ra = (uintptr_t)(ref + (base - 1 + q0));
+        const uint32_t *wp = (const uint32_t *)(ra & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)(ra & 3) * 8u;
+        uint32_t rw[NW - 1];
+        {
+            uint32_t prev = wp[0];
+#pragma unroll
+            for (int k = 0; k < NW - 1; k++) {
+                uint32_t nx = wp[k + 1];
+                rw[k] = funnel(nx, prev, sh);
+                prev = nx;
+            }
+        }
+        // valid positions of this lane: p in [vlo, vhi]
+        const int32_t vlo = js - base - q0, vhi = je - base - q0;
+        const int32_t upH = from_next<1>(bd.H[0], bd.padH), upE = from_next<1>(bd.E[0], bd.padE);
+        int32_t hm[P], ev[P], hp[P];
+        uint32_t ebits = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int32_t d = bd.H[p];
+            const int32_t uh = p + 1 < P ? bd.H[p + 1] : upH;
+            const int32_t ue = p + 1 < P ? bd.E[p + 1] : upE;
+            const uint8_t rb = (uint8_t)(rw[p >> 2] >> (8 * (p & 3)));
+            const int32_t s = (qb == rb) ? A : -B;
+            hm[p] = imax(d, 0) + s;
+            const int32_t efh = uh - go, efe = ue - ge;
+            ev[p] = imax(efh, efe);
+            ebits |= (efe > efh ? 1u : 0u) << p;
+            const bool valid = p >= vlo && p <= vhi;
+            hp[p] = valid ? imax(imax(hm[p], ev[p]), 0) : bd.padH;
+        }
+        // F: local chains, then the carry across lanes
+        int32_t fl[P];
+        int32_t f = kMin4;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            fl[p] = f;
+            f = imax(hp[p] - go, f - ge);
+        }
+        const int32_t dP = P * ge;
+        int32_t x = from_prev<1>(f, kMin4);
+        x = imax(x, from_prev<1>(x, kMin4) - dP);
+        x = imax(x, from_prev<2>(x, kMin4) - 2 * dP);
+        x = imax(x, from_prev<4>(x, kMin4) - 4 * dP);
+        x = imax(x, from_prev<8>(x, kMin4) - 8 * dP);
+        // x = F at this lane's first position from earlier lanes
+        int32_t F[P], Hf[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            F[p] = imax(fl[p], x - p * ge);
+            Hf[p] = imax(hp[p], F[p]);
+        }
+        // F-extend bit of position p compares F and H of position p - 1
+        int32_t lastF = from_prev<1>(F[P - 1], kMin4), lastH = from_prev<1>(Hf[P - 1], bd.padH);
+        bool lastValid = (q0 - 1 >= js - base) && (q0 - 1 <= je - base);
+        int32_t rbest = 0, rbq = -1;
+        uint32_t word = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const bool valid = p >= vlo && p <= vhi;
+            const int32_t h = Hf[p];
+            uint32_t dir = h == 0 ? 0u : (h == hm[p] ? 1u : (h == ev[p] ? 2u : 3u));
+            uint32_t fb = (lastValid && (lastF - ge > lastH - go)) ? 8u : 0u;
+            uint32_t code = dir | (((ebits >> p) & 1u) << 2) | fb;
+            if (valid) {
+                if (h > rbest) {
+                    rbest = h;
+                    rbq = q0 + p;
+                }
+                bd.H[p] = h;
+                bd.E[p] = ev[p];
+            } else {
+                code = 0;
+                bd.H[p] = bd.padH;
+                bd.E[p] = bd.padE;
+            }
+            lastF = F[p];
+            lastH = h;
+            lastValid = valid;
+            word |= code << (4 * (p & 7));
+            if ((p & 7) == 7 || p == P - 1) {
+                trow[(p >> 3) * 64] = word;
+                word = 0;
+            }
+        }
+        // first strict row maximum over the 16 lanes
+        int32_t key = rbest > 0 ? (rbest << 10) | (1023 - rbq) : 0;
+        key = imax(key, rotate<8>(key));
+        key = imax(key, rotate<4>(key));
+        key = imax(key, rotate<2>(key));
+        key = imax(key, rotate<1>(key));
+        const int32_t rb_all = key >> 10;
+        const int32_t rbj = key > 0 ? base + (1023 - (key & 1023)) : -1;
+        if (g == 0)
+            meta[(int64_t)(i - 1) * 64 + wl] = make_int2(base, (js - base) | ((je - base) << 16));
+        if (rb_all > best) {
+            best = rb_all;
+            bi = i;
+            bj = rbj;
+        }
+        if (i == L && rb_all > 0) {
+            bq = rb_all;
+            bqj = rbj;
+        }
+        if (a.adaptive) {
+            int32_t raw = rbj >= 0 ? rbj + 1 : ac + 1;
+            int32_t sn = i + 1 + off;
+            ac = imin(imax(raw, sn - D), sn + D);
+        }
+        if (xdrop > 0 && best - rb_all > xdrop) {
+            early = 1;
+            break;
+        }
+        if (abort_score > kMin8) {
+            int32_t remaining = L - i;
+            if (best + remaining * A < abort_score) {
+                early = 1;
+                break;
+            }
+        }
+    }
+    if (bq > 0) {
+        int64_t sum = (int64_t)bq + a.clip_penalty;
+        int32_t sat = sum > INT32_MAX ? INT32_MAX : (sum < INT32_MIN ? INT32_MIN : (int32_t)sum);
+        if (sat > best) {
+            best = bq;
+            bi = L;
+            bj = bqj;
+        }
+    }
+    if (g == 0) {
+        int32_t *res = a.res + (int64_t)job * kSwResWords;
+        res[0] = best;
+        res[1] = bi;
+        res[2] = bj;
+        res[3] = early;
+        res[7] = 0;
+        res[8] = cells;
+        res[9] = 0;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
+    const int lane = threadIdx.x;
+    const int g = lane & 15, wl = lane & ~15;
+    for (;;) {
+        int32_t p = 0;
+        if (lane == 0) p = atomicAdd(a.counter, 1);
+        p = __shfl(p, 0);
+        if (p >= a.n_packets) return;
+        const int32_t job = a.packet_jobs[(int64_t)p * 64 + wl];
+        if (job >= 0)
+            coop_job<P>(a, job, g, wl, a.packet_rows[p], a.trace + a.packet_trace_off[p],
+                        a.row_meta + a.packet_meta_off[p]);
+    }
+}
+
+__global__ void zero_coop_counter_kernel(int32_t *c) { *c = 0; }
+
+}  // namespace
+
+hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int n_waves, hipStream_t s) {
+    zero_coop_counter_kernel<<<1, 1, 0, s>>>(a.counter);
+    if (a.n_packets > 0) {
+        switch (P) {
+        case 9: banded_sw_coop_kernel<9><<<n_waves, 64, 0, s>>>(a); break;
+        case 17: banded_sw_coop_kernel<17><<<n_waves, 64, 0, s>>>(a); break;
+        case 25: banded_sw_coop_kernel<25><<<n_waves, 64, 0, s>>>(a); break;
+        case 33: banded_sw_coop_kernel<33><<<n_waves, 64, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace kira

@@ -14,16 +14,36 @@
 
 namespace {
 
-// Band classes: a job with band w runs in the smallest C >= 2w + 1, with the
-// band stored as packed int16 when its scores provably fit, else as int32.
-constexpr int kClasses[] = {9, 17, 33, 65, 129, 257};
-constexpr int kNumBands = 6;
-constexpr int kNumClasses = 2 * kNumBands;  // class = band * 2 + packed
+// Job classes.  Lane-per-job: a job with band w runs in the smallest
+// C >= 2w + 1 of kLaneC, with the band stored as packed int16 when its
+// scores provably fit, else as int32 (class = band index * 2 + packed).
+// Bands above 64 (C = 257 would spill the register file) run 16 lanes per
+// job with P = kCoopP[k] positions a lane (16 P >= 2w + 1) when the costs
+// keep its int32 scan and arg-max key exact; otherwise C = 257 lane-per-job.
+constexpr int kLaneC[] = {9, 17, 33, 65, 129, 257};
+constexpr int kNumLane = 6;
+constexpr int kCoopP[] = {9, 17, 25, 33};
+constexpr int kNumCoop = 4;
+constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
+constexpr int kMaxBand = (16 * 33 - 1) / 2;  // 263
 
-int class_of(int32_t w) {
+bool class_is_coop(int c) { return c >= 2 * kNumLane; }
+int class_coop_p(int c) { return kCoopP[c - 2 * kNumLane]; }
+int class_jobs_per_packet(int c) { return class_is_coop(c) ? 4 : 64; }
+int class_rowdw(int c) {
+    return class_is_coop(c) ? (class_coop_p(c) + 7) / 8 : (kLaneC[c / 2] + 7) / 8;
+}
+
+int lane_class_of(int32_t w) {
     int need = 2 * w + 1;
-    for (int k = 0; k < kNumBands; k++)
-        if (kClasses[k] >= need) return k;
+    for (int k = 0; k < kNumLane; k++)
+        if (kLaneC[k] >= need) return k;
+    return -1;
+}
+
+int coop_class_of(int32_t w) {
+    for (int k = 0; k < kNumCoop; k++)
+        if (16 * kCoopP[k] >= 2 * w + 1) return 2 * kNumLane + k;
     return -1;
 }
 
@@ -57,6 +77,7 @@ struct kira_sw_plan {
     int32_t class_first[kNumClasses] = {};
     int32_t class_count[kNumClasses] = {};
     int32_t *packet_jobs = nullptr, *packet_rows = nullptr, *packet_rowdw = nullptr;
+    int32_t *packet_P = nullptr;
     int64_t *packet_trace_off = nullptr, *packet_meta_off = nullptr;
     int32_t *job_slot = nullptr, *cigar_cap = nullptr, *counter = nullptr;
     int64_t *cigar_off = nullptr;
@@ -65,7 +86,7 @@ struct kira_sw_plan {
     int64_t cigar_words = 0;
     std::vector<int64_t> host_cigar_off;
     void release() {
-        void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_trace_off,
+        void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_P, packet_trace_off,
                         packet_meta_off, job_slot, cigar_cap, counter, cigar_off, trace, meta};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -84,13 +105,28 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     const int32_t go = sat_add(cfg->gap_open, cfg->gap_extend);
     const bool costs_small = cfg->match_score >= 0 && cfg->mismatch >= 0 && cfg->gap_open >= 0 &&
                              cfg->gap_extend >= 0 && go <= 16000;
+    // 16-lane jobs: the F scan stays inside int32 and the arg-max key
+    // (score << 10 | position) needs score < 2^21
+    const int32_t kCoopCost = 1 << 20;
+    const bool coop_costs = cfg->match_score >= 0 && cfg->mismatch >= 0 && cfg->gap_open >= 0 &&
+                            cfg->gap_extend >= 0 && cfg->match_score <= kCoopCost &&
+                            cfg->mismatch <= kCoopCost && go <= kCoopCost;
     for (int32_t k = 0; k < n; k++) {
         int32_t w = std::max(band ? band[k] : cfg->bandwidth, 1);
-        int b = class_of(w);
-        if (b < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "band wider than 128");
         if (read_len[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "negative read length");
         // every H is at most read_len * match: int16 storage cannot clamp
         bool pack = costs_small && (int64_t)read_len[k] * cfg->match_score < 32767;
+        bool coop = coop_costs && (int64_t)read_len[k] * cfg->match_score < (1 << 21);
+        if (w > 64 && coop) {
+            cls[k] = coop_class_of(w);
+            if (cls[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "band wider than 263");
+            continue;
+        }
+        int b = lane_class_of(w);
+        if (b < 0)
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT,
+                        w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
+                                      : "band wider than 263");
         cls[k] = 2 * b + (pack ? 1 : 0);
     }
     // packets: jobs sorted by (band class, read length descending)
@@ -103,26 +139,28 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     kira_sw_plan *p = new (std::nothrow) kira_sw_plan();
     if (!p) return KIRA_ERR_OUT_OF_MEMORY;
     p->n = n;
-    std::vector<int32_t> pj, prows, prowdw, slot(n);
+    std::vector<int32_t> pj, prows, prowdw, pP, slot(n);
     std::vector<int64_t> ptoff, pmoff;
     int64_t toff = 0, moff = 0;
     for (int c = 0; c < kNumClasses; c++) p->class_first[c] = -1;
     for (int32_t s = 0; s < n;) {
         int c = cls[order[s]];
+        const int per = class_jobs_per_packet(c), stride = 64 / per;
         int32_t e = s;
-        while (e < n && e - s < 64 && cls[order[e]] == c) e++;
+        while (e < n && e - s < per && cls[order[e]] == c) e++;
         int32_t pk = (int32_t)prows.size();
         if (p->class_first[c] < 0) p->class_first[c] = pk;
         p->class_count[c]++;
         int32_t rows = read_len[order[s]];
-        int32_t rowdw = (kClasses[c / 2] + 7) / 8;
+        int32_t rowdw = class_rowdw(c);
         for (int32_t l = 0; l < 64; l++) {
-            int32_t job = s + l < e ? order[s + l] : -1;
+            int32_t job = (l % stride == 0 && s + l / stride < e) ? order[s + l / stride] : -1;
             pj.push_back(job);
             if (job >= 0) slot[job] = pk * 64 + l;
         }
         prows.push_back(rows);
         prowdw.push_back(rowdw);
+        pP.push_back(class_is_coop(c) ? class_coop_p(c) : 0);
         ptoff.push_back(toff);
         pmoff.push_back(moff);
         toff += (int64_t)rows * rowdw * 64;
@@ -144,6 +182,7 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     int32_t P = std::max(p->n_packets, 1);
     if (e || (e = dev_alloc(&p->packet_jobs, (size_t)P * 64)) ||
         (e = dev_alloc(&p->packet_rows, P)) || (e = dev_alloc(&p->packet_rowdw, P)) ||
+        (e = dev_alloc(&p->packet_P, P)) ||
         (e = dev_alloc(&p->packet_trace_off, P)) || (e = dev_alloc(&p->packet_meta_off, P)) ||
         (e = dev_alloc(&p->job_slot, n)) || (e = dev_alloc(&p->cigar_cap, n)) ||
         (e = dev_alloc(&p->cigar_off, n)) || (e = dev_alloc(&p->counter, kNumClasses)) ||
@@ -159,6 +198,7 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     if ((e = up(p->packet_jobs, pj.data(), pj.size() * 4)) ||
         (e = up(p->packet_rows, prows.data(), prows.size() * 4)) ||
         (e = up(p->packet_rowdw, prowdw.data(), prowdw.size() * 4)) ||
+        (e = up(p->packet_P, pP.data(), pP.size() * 4)) ||
         (e = up(p->packet_trace_off, ptoff.data(), ptoff.size() * 8)) ||
         (e = up(p->packet_meta_off, pmoff.data(), pmoff.size() * 8)) ||
         (e = up(p->job_slot, slot.data(), (size_t)n * 4)) ||
@@ -222,6 +262,7 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
     a.cigar = j->cigar;
     a.cigar_off = plan->cigar_off;
     a.cigar_cap = plan->cigar_cap;
+    a.packet_P = plan->packet_P;
     hipError_t e = hipSuccess;
     for (int c = 0; c < kNumClasses; c++) {
         if (plan->class_count[c] == 0) continue;
@@ -230,13 +271,19 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
         ac.packet_jobs = plan->packet_jobs + (int64_t)f * 64;
         ac.packet_rows = plan->packet_rows + f;
         ac.packet_rowdw = plan->packet_rowdw + f;
+        ac.packet_P = plan->packet_P + f;
         ac.packet_trace_off = plan->packet_trace_off + f;
         ac.packet_meta_off = plan->packet_meta_off + f;
         ac.n_packets = plan->class_count[c];
         ac.counter = plan->counter + c;
-        int waves = std::min(plan->class_count[c], 256 * 16);
-        if ((e = kira::launch_banded_sw(ac, kClasses[c / 2], (c & 1) != 0, waves, ctx->stream)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "banded SW launch", e);
+        if (class_is_coop(c)) {
+            int waves = std::min(plan->class_count[c], 256 * 20);
+            e = kira::launch_banded_sw_coop(ac, class_coop_p(c), waves, ctx->stream);
+        } else {
+            int waves = std::min(plan->class_count[c], 256 * 16);
+            e = kira::launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, ctx->stream);
+        }
+        if (e) return fail(ctx, KIRA_ERR_RUNTIME, "banded SW launch", e);
     }
     a.packet_rowdw = plan->packet_rowdw;
     a.packet_trace_off = plan->packet_trace_off;

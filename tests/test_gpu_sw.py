@@ -115,3 +115,38 @@ def test_reference_known_answers(pkg, ctx):
     r, = pkg.banded_sw_batch(ctx, [(read, ref, 0, ABORT_NONE)], pkg.sw_cfg(1, 4, 6, 1, 20, 1000, 5))
     want = O.banded_sw(read, ref, 0, O.cfg(1, 4, 6, 1, 20, 1000, 5), ABORT_NONE, O.knobs())
     assert r == want
+
+
+def test_wide_bands_and_long_windows(pkg, ctx):
+    """Bands 65..263 (16 lanes per job) on long windows at long-read error
+    rates (5% substitutions, 5% indels), next to narrow-band jobs."""
+    rng = random.Random(11)
+    jobs = []
+    for k in range(96):
+        w = rng.choice([65, 70, 71, 72, 100, 128, 129, 135, 136, 199, 200, 250, 263, 16, 40])
+        L = rng.randrange(50, 2500)
+        flank = max(2 * w, 50)
+        g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank + 20))
+        read = mutate(rng, g[flank:flank + L], 0.05, 0.05) or b"A"
+        off = flank + rng.randrange(-20, 21)
+        abort = rng.choice([ABORT_NONE, ABORT_NONE, len(read) // 3])
+        jobs.append((read, g, off, abort, w))
+    for adaptive in (True, False):
+        c = pkg.sw_cfg(1, 4, 6, 1, 32, rng.choice([100, 1000]), 5)
+        check(pkg, ctx, jobs, c, pkg.sw_knobs(adaptive, 16, -1), O.knobs(adaptive, 16, -1))
+
+
+def test_wide_band_limits(pkg, ctx):
+    ref = gen_ref(400)
+    read = ref[100:200]
+    with pytest.raises(pkg.HipError) as e:
+        pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 264)], pkg.sw_cfg())
+    assert e.value.kind == "InvalidArgument"
+    # scores past the 16-lane kernel's exact range: bands up to 128 still run
+    # (lane per job, int32), wider ones are refused
+    big = pkg.sw_cfg(30000, 4, 6, 1, 32, 100, 5)
+    with pytest.raises(pkg.HipError):
+        pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 200)], big)
+    r, = pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 100)], big)
+    want = O.banded_sw(read, ref, 100, O.cfg(30000, 4, 6, 1, 100, 100, 5), ABORT_NONE, O.knobs())
+    assert r == want
