@@ -177,7 +177,7 @@ __device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         const int32_t vlo = js - base - q0, vhi = je - base - q0;
         const int32_t upH = from_next<1>(bd.H[0], bd.padH), upE = from_next<1>(bd.E[0], bd.padE);
         int32_t hm[P], ev[P], hp[P];
-        uint32_t ebits = 0;
+        uint64_t ebits = 0;  // E-extend bit of each position (P may exceed 32)
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const int32_t d = bd.H[p];
@@ -188,7 +188,7 @@ __device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
             hm[p] = imax(d, 0) + s;
             const int32_t efh = uh - go, efe = ue - ge;
             ev[p] = imax(efh, efe);
-            ebits |= (efe > efh ? 1u : 0u) << p;
+            ebits |= (uint64_t)(efe > efh ? 1u : 0u) << p;
             const bool valid = p >= vlo && p <= vhi;
             hp[p] = valid ? imax(imax(hm[p], ev[p]), 0) : bd.padH;
         }
@@ -224,7 +224,7 @@ __device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
             const int32_t h = Hf[p];
             uint32_t dir = h == 0 ? 0u : (h == hm[p] ? 1u : (h == ev[p] ? 2u : 3u));
             uint32_t fb = (lastValid && (lastF - ge > lastH - go)) ? 8u : 0u;
-            uint32_t code = dir | (((ebits >> p) & 1u) << 2) | fb;
+            uint32_t code = dir | ((uint32_t)((ebits >> p) & 1u) << 2) | fb;
             if (valid) {
                 if (h > rbest) {
                     rbest = h;
