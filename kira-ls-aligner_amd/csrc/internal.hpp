@@ -79,7 +79,6 @@ struct SwLaunch {
     const int32_t *packet_jobs;
     const int32_t *packet_rows;
     const int32_t *packet_rowdw;
-    const int32_t *packet_P;  // 0: lane-per-job packet; P: 16-lane jobs, P positions a lane
     const int64_t *packet_trace_off;
     const int64_t *packet_meta_off;
     int32_t n_packets;
@@ -148,7 +147,6 @@ struct PrefilterLaunch {
 };
 
 hipError_t launch_prefilter(const PrefilterLaunch &a, hipStream_t s);
-hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s);
 
 }  // namespace kira
 

@@ -12,8 +12,10 @@
 // values (i32::MIN/4) exactly.  Each cell writes a 4-bit trace code
 // (direction | E-extend << 2 | F-extend << 3); a packet of 64 jobs shares a
 // slab laid out [row][word][lane] so every trace store is one coalesced
-// 256-byte write.  A second kernel walks the traces back into CIGARs.
+// 256-byte write.  After its last row the lane walks its trace back into
+// the CIGAR, overlapping that latency-bound walk with other waves' rows.
 #include "internal.hpp"
+#include "sw_traceback.hpp"
 
 namespace kira {
 namespace {
@@ -296,85 +298,14 @@ __global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
         p = __shfl(p, 0);
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + lane];
-        if (job >= 0)
-            sw_job<C, PACK>(a, job, lane, a.packet_rows[p], a.trace + a.packet_trace_off[p],
-                      a.row_meta + a.packet_meta_off[p]);
-    }
-}
-
-// Traceback (mod.rs:1581-1626) over the slab written by banded_sw_kernel.
-__host__ __device__ void sw_traceback_job(const SwLaunch &a, int32_t job) {
-    const int32_t pl = a.job_slot[job];  // packet * 64 + lane
-    const int32_t p = pl >> 6, lane = pl & 63;
-    const int32_t rowdw = a.packet_rowdw[p];
-    const int32_t P = a.packet_P[p];
-    const uint32_t *slab = a.trace + a.packet_trace_off[p];
-    const int2 *meta = a.row_meta + a.packet_meta_off[p];
-    int32_t *res = a.res + (int64_t)job * kSwResWords;
-    int32_t i = res[1], j = res[2];
-    uint32_t *cig = a.cigar + a.cigar_off[job];
-    const int32_t cap = a.cigar_cap[job];
-    int32_t n = 0;
-    int32_t last_op = -1;
-    int state = 0;
-    while (i > 0 && j > 0) {
-        int2 m = meta[(int64_t)(i - 1) * 64 + lane];
-        int32_t base = m.x, js = base + (m.y & 0xFFFF), je = base + (m.y >> 16);
-        if (j < js || j > je) break;
-        int32_t q = j - base;
-        int32_t col = lane;
-        if (P > 0) {  // 16-lane job: position q sits in lane q / P at q % P
-            col += q / P;
-            q -= (q / P) * P;
-        }
-        uint32_t tr = (slab[((int64_t)(i - 1) * rowdw + (q >> 3)) * 64 + col] >> (4 * (q & 7))) & 15u;
-        int op = -1;
-        if (state == 0) {
-            uint32_t d = tr & 3u;
-            if (d == 0) break;
-            if (d == 1) {
-                op = 0;
-                i--;
-                j--;
-            } else if (d == 2) {
-                state = 1;
-            } else {
-                state = 2;
-            }
-        } else if (state == 1) {
-            op = 1;
-            i--;
-            state = (tr & 4u) ? 1 : 0;
-        } else {
-            op = 2;
-            j--;
-            state = (tr & 8u) ? 2 : 0;
-        }
-        if (op >= 0) {
-            if (op == last_op) {
-                if (n - 1 < cap) cig[n - 1] += 16u;
-            } else {
-                if (n < cap) cig[n] = (1u << 4) | (uint32_t)op;
-                n++;
-                last_op = op;
-            }
+        if (job >= 0) {
+            uint32_t *slab = a.trace + a.packet_trace_off[p];
+            int2 *meta = a.row_meta + a.packet_meta_off[p];
+            sw_job<C, PACK>(a, job, lane, a.packet_rows[p], slab, meta);
+            // the lane reads back only the trace it wrote itself
+            sw_traceback_job(a, job, slab, meta, (C + 7) / 8, 0, lane);
         }
     }
-    // ops were produced end-to-start: reverse in place
-    int32_t m = n < cap ? n : cap;
-    for (int32_t x = 0, y = m - 1; x < y; x++, y--) {
-        uint32_t tmp = cig[x];
-        cig[x] = cig[y];
-        cig[y] = tmp;
-    }
-    res[4] = j;  // ref_start
-    res[5] = i;  // read_start
-    res[6] = n <= cap ? n : -1;
-}
-
-__global__ void banded_traceback_kernel(SwLaunch a, int32_t n_jobs) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n_jobs) sw_traceback_job(a, (int32_t)t);
 }
 
 __global__ void zero_counter_kernel(int32_t *c) { *c = 0; }
@@ -405,10 +336,5 @@ hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hi
     return hipGetLastError();
 }
 
-hipError_t launch_banded_traceback(const SwLaunch &a, int32_t n_jobs, hipStream_t s) {
-    if (n_jobs == 0) return hipSuccess;
-    banded_traceback_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
-    return hipGetLastError();
-}
 
 }  // namespace kira

@@ -22,6 +22,7 @@
 // Trace slab per packet of 4 jobs: [row][word][wave lane], word k of lane g
 // holding the 4-bit codes of positions g*P + 8k .. g*P + 8k + 7.
 #include "internal.hpp"
+#include "sw_traceback.hpp"
 
 namespace kira {
 namespace {
@@ -313,9 +314,17 @@ __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
         p = __shfl(p, 0);
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + wl];
-        if (job >= 0)
-            coop_job<P>(a, job, g, wl, a.packet_rows[p], a.trace + a.packet_trace_off[p],
-                        a.row_meta + a.packet_meta_off[p]);
+        if (job >= 0) {
+            uint32_t *slab = a.trace + a.packet_trace_off[p];
+            int2 *meta = a.row_meta + a.packet_meta_off[p];
+            coop_job<P>(a, job, g, wl, a.packet_rows[p], slab, meta);
+            // lane 0 of the row walks the trace its 15 neighbours helped write
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (g == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                sw_traceback_job(a, job, slab, meta, (P + 7) / 8, P, wl);
+            }
+        }
     }
 }
 

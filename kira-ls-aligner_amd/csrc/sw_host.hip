@@ -77,7 +77,6 @@ struct kira_sw_plan {
     int32_t class_first[kNumClasses] = {};
     int32_t class_count[kNumClasses] = {};
     int32_t *packet_jobs = nullptr, *packet_rows = nullptr, *packet_rowdw = nullptr;
-    int32_t *packet_P = nullptr;
     int64_t *packet_trace_off = nullptr, *packet_meta_off = nullptr;
     int32_t *job_slot = nullptr, *cigar_cap = nullptr, *counter = nullptr;
     int64_t *cigar_off = nullptr;
@@ -86,7 +85,7 @@ struct kira_sw_plan {
     int64_t cigar_words = 0;
     std::vector<int64_t> host_cigar_off;
     void release() {
-        void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_P, packet_trace_off,
+        void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_trace_off,
                         packet_meta_off, job_slot, cigar_cap, counter, cigar_off, trace, meta};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -139,7 +138,7 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     kira_sw_plan *p = new (std::nothrow) kira_sw_plan();
     if (!p) return KIRA_ERR_OUT_OF_MEMORY;
     p->n = n;
-    std::vector<int32_t> pj, prows, prowdw, pP, slot(n);
+    std::vector<int32_t> pj, prows, prowdw, slot(n);
     std::vector<int64_t> ptoff, pmoff;
     int64_t toff = 0, moff = 0;
     for (int c = 0; c < kNumClasses; c++) p->class_first[c] = -1;
@@ -160,7 +159,6 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
         }
         prows.push_back(rows);
         prowdw.push_back(rowdw);
-        pP.push_back(class_is_coop(c) ? class_coop_p(c) : 0);
         ptoff.push_back(toff);
         pmoff.push_back(moff);
         toff += (int64_t)rows * rowdw * 64;
@@ -182,7 +180,6 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     int32_t P = std::max(p->n_packets, 1);
     if (e || (e = dev_alloc(&p->packet_jobs, (size_t)P * 64)) ||
         (e = dev_alloc(&p->packet_rows, P)) || (e = dev_alloc(&p->packet_rowdw, P)) ||
-        (e = dev_alloc(&p->packet_P, P)) ||
         (e = dev_alloc(&p->packet_trace_off, P)) || (e = dev_alloc(&p->packet_meta_off, P)) ||
         (e = dev_alloc(&p->job_slot, n)) || (e = dev_alloc(&p->cigar_cap, n)) ||
         (e = dev_alloc(&p->cigar_off, n)) || (e = dev_alloc(&p->counter, kNumClasses)) ||
@@ -198,7 +195,6 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     if ((e = up(p->packet_jobs, pj.data(), pj.size() * 4)) ||
         (e = up(p->packet_rows, prows.data(), prows.size() * 4)) ||
         (e = up(p->packet_rowdw, prowdw.data(), prowdw.size() * 4)) ||
-        (e = up(p->packet_P, pP.data(), pP.size() * 4)) ||
         (e = up(p->packet_trace_off, ptoff.data(), ptoff.size() * 8)) ||
         (e = up(p->packet_meta_off, pmoff.data(), pmoff.size() * 8)) ||
         (e = up(p->job_slot, slot.data(), (size_t)n * 4)) ||
@@ -262,7 +258,6 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
     a.cigar = j->cigar;
     a.cigar_off = plan->cigar_off;
     a.cigar_cap = plan->cigar_cap;
-    a.packet_P = plan->packet_P;
     hipError_t e = hipSuccess;
     for (int c = 0; c < kNumClasses; c++) {
         if (plan->class_count[c] == 0) continue;
@@ -271,7 +266,6 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
         ac.packet_jobs = plan->packet_jobs + (int64_t)f * 64;
         ac.packet_rows = plan->packet_rows + f;
         ac.packet_rowdw = plan->packet_rowdw + f;
-        ac.packet_P = plan->packet_P + f;
         ac.packet_trace_off = plan->packet_trace_off + f;
         ac.packet_meta_off = plan->packet_meta_off + f;
         ac.n_packets = plan->class_count[c];
@@ -285,11 +279,6 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
         }
         if (e) return fail(ctx, KIRA_ERR_RUNTIME, "banded SW launch", e);
     }
-    a.packet_rowdw = plan->packet_rowdw;
-    a.packet_trace_off = plan->packet_trace_off;
-    a.packet_meta_off = plan->packet_meta_off;
-    if ((e = kira::launch_banded_traceback(a, plan->n, ctx->stream)))
-        return fail(ctx, KIRA_ERR_RUNTIME, "traceback launch", e);
     return KIRA_OK;
 }
 
