@@ -15,7 +15,6 @@
 // 256-byte write.  After its last row the lane walks its trace back into
 // the CIGAR, overlapping that latency-bound walk with other waves' rows.
 #include "internal.hpp"
-#include "sw_traceback.hpp"
 
 namespace kira {
 namespace {
@@ -189,7 +188,7 @@ __host__ __device__ __forceinline__ void shift_band(Band<C, PACK> &bd, int32_t d
 // One job's rows (banded_sw_with, mod.rs:1370-1579); rows <= the packet's
 // row count so the slab holds every row written.
 template <int C, bool PACK>
-__host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_t rows,
+__host__ __device__ int2 sw_job(const SwLaunch &a, int32_t job, int lane, int32_t rows,
                                 uint32_t *slab, int2 *meta) {
     const int32_t rowdw = (C + 7) / 8;
     const int32_t L = a.read_len[job], W = a.ref_len[job];
@@ -287,6 +286,110 @@ __host__ __device__ void sw_job(const SwLaunch &a, int32_t job, int lane, int32_
     res[7] = (PACK && max_h > 32767) ? 1 : 0;
     res[8] = cells;
     res[9] = 0;
+    return make_int2(bi, bj);
+}
+
+// Traceback (mod.rs:1581-1626) of a lane-per-job slab ([row][q / 8][lane]).
+// Each round prefetches the meta and trace codes of the next kTbDepth cells
+// on the current diagonal (cells (i - k, j - k)), so a run of diagonal
+// moves costs one pair of dependent loads per kTbDepth cells.
+constexpr int kTbDepth = 4;
+
+__host__ __device__ __forceinline__ void sw_traceback_job(const SwLaunch &a, int32_t job,
+                                                          int32_t bi, int32_t bj,
+                                                          const uint32_t *slab, const int2 *meta,
+                                                          int32_t rowdw, int32_t lane) {
+    uint32_t *cig = a.cigar + a.cigar_off[job];
+    const int32_t cap = a.cigar_cap[job];
+    int32_t i = bi, j = bj, n = 0, last_op = -1, run = 0, state = 0;
+    bool stop = false;
+    auto flush = [&]() {
+        if (last_op >= 0) {
+            if (n < cap) cig[n] = ((uint32_t)run << 4) | (uint32_t)last_op;
+            n++;
+        }
+    };
+    while (!stop && i > 0 && j > 0) {
+        uint32_t tw[kTbDepth];
+        bool ok[kTbDepth];
+#pragma unroll
+        for (int k = 0; k < kTbDepth; k++) {
+            const int32_t pi = i - k, pj = j - k;
+            tw[k] = 0;
+            ok[k] = false;
+            if (pi >= 1 && pj >= 1) {
+                const int2 m = meta[(int64_t)(pi - 1) * 64 + lane];
+                const int32_t base = m.x, js = base + (m.y & 0xFFFF), je = base + (m.y >> 16);
+                if (pj >= js && pj <= je) {
+                    const int32_t q = pj - base;
+                    tw[k] = (slab[((int64_t)(pi - 1) * rowdw + (q >> 3)) * 64 + lane] >>
+                             (4 * (q & 7))) & 15u;
+                    ok[k] = true;
+                }
+            }
+        }
+        const int32_t i0 = i, j0 = j;
+        for (;;) {
+            const int32_t k = i0 - i;
+            if (k >= kTbDepth || j0 - j != k || i <= 0 || j <= 0) break;
+            uint32_t tr = tw[0];
+            bool okk = ok[0];
+#pragma unroll
+            for (int x = 1; x < kTbDepth; x++)
+                if (k == x) {
+                    tr = tw[x];
+                    okk = ok[x];
+                }
+            if (!okk) {  // the cell lies outside its row's band
+                stop = true;
+                break;
+            }
+            int op = -1;
+            if (state == 0) {
+                const uint32_t d = tr & 3u;
+                if (d == 0) {
+                    stop = true;
+                    break;
+                }
+                if (d == 1) {
+                    op = 0;
+                    i--;
+                    j--;
+                } else {
+                    state = d == 2 ? 1 : 2;
+                }
+            } else if (state == 1) {
+                op = 1;
+                i--;
+                state = (tr & 4u) ? 1 : 0;
+            } else {
+                op = 2;
+                j--;
+                state = (tr & 8u) ? 2 : 0;
+            }
+            if (op >= 0) {
+                if (op == last_op) {
+                    run++;
+                } else {
+                    flush();
+                    last_op = op;
+                    run = 1;
+                }
+            }
+        }
+    }
+    flush();
+    // ops were produced end-to-start: reverse in place
+    const int32_t m = n < cap ? n : cap;
+    for (int32_t x = 0, y = m - 1; x < y; x++, y--) {
+        uint32_t t = cig[x];
+        cig[x] = cig[y];
+        cig[y] = t;
+    }
+    int32_t *res = a.res + (int64_t)job * kSwResWords;
+    res[4] = j;  // ref_start
+    res[5] = i;  // read_start
+    res[6] = n <= cap ? n : -1;
 }
 
 template <int C, bool PACK>
@@ -301,9 +404,9 @@ __global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
         if (job >= 0) {
             uint32_t *slab = a.trace + a.packet_trace_off[p];
             int2 *meta = a.row_meta + a.packet_meta_off[p];
-            sw_job<C, PACK>(a, job, lane, a.packet_rows[p], slab, meta);
+            const int2 end = sw_job<C, PACK>(a, job, lane, a.packet_rows[p], slab, meta);
             // the lane reads back only the trace it wrote itself
-            sw_traceback_job(a, job, slab, meta, (C + 7) / 8, 0, lane);
+            sw_traceback_job(a, job, end.x, end.y, slab, meta, (C + 7) / 8, lane);
         }
     }
 }

@@ -22,7 +22,6 @@
 // Trace slab per packet of 4 jobs: [row][word][wave lane], word k of lane g
 // holding the 4-bit codes of positions g*P + 8k .. g*P + 8k + 7.
 #include "internal.hpp"
-#include "sw_traceback.hpp"
 
 namespace kira {
 namespace {
@@ -114,7 +113,7 @@ struct CoopBand {
 // One job on the 16 lanes of a DPP row; g = lane within the row, wl = wave
 // lane of g = 0 (the job's slot column in the packet's meta slab).
 template <int P>
-__device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t rows,
+__device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t rows,
                          uint32_t *slab, int2 *meta) {
     constexpr int RW = (P + 7) / 8;
     constexpr int NW = (P + 3) / 4 + 1;
@@ -302,6 +301,101 @@ __device__ void coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         res[8] = cells;
         res[9] = 0;
     }
+    return make_int2(bi, bj);
+}
+
+// Traceback (mod.rs:1581-1626) by the job's 16 lanes together: each round
+// lane k fetches the trace code of cell (i - k, j - k), so a run of
+// diagonal moves costs one dependent meta + trace load per 16 cells
+// instead of per cell; the walk itself (the same state machine as the
+// lane-per-job traceback) runs redundantly in all 16 lanes and lane 0
+// writes the CIGAR.
+template <int P>
+__device__ void coop_traceback(const SwLaunch &a, int32_t job, int g, int wl, int2 end,
+                               const uint32_t *slab, const int2 *meta) {
+    constexpr int RW = (P + 7) / 8;
+    uint32_t *cig = a.cigar + a.cigar_off[job];
+    const int32_t cap = a.cigar_cap[job];
+    int32_t i = end.x, j = end.y, n = 0, last_op = -1, run = 0, state = 0;
+    bool stop = false;
+    auto flush = [&]() {
+        if (last_op >= 0) {
+            if (g == 0 && n < cap) cig[n] = ((uint32_t)run << 4) | (uint32_t)last_op;
+            n++;
+        }
+    };
+    while (!stop && i > 0 && j > 0) {
+        const int32_t pi = i - g, pj = j - g;
+        uint32_t tw = 0;
+        int32_t ok = 0;
+        if (pi >= 1 && pj >= 1) {
+            const int2 m = meta[(int64_t)(pi - 1) * 64 + wl];
+            const int32_t base = m.x, js = base + (m.y & 0xFFFF), je = base + (m.y >> 16);
+            if (pj >= js && pj <= je) {
+                const int32_t q = pj - base, lg = q / P, qq = q - lg * P;
+                tw = (slab[((int64_t)(pi - 1) * RW + (qq >> 3)) * 64 + wl + lg] >> (4 * (qq & 7))) &
+                     15u;
+                ok = 1;
+            }
+        }
+        const int32_t i0 = i, j0 = j;
+        for (;;) {
+            const int32_t k = i0 - i;
+            if (k >= 16 || j0 - j != k || i <= 0 || j <= 0) break;
+            const int32_t okk = __shfl(ok, wl + k);
+            const uint32_t tr = (uint32_t)__shfl((int32_t)tw, wl + k);
+            if (!okk) {  // the cell lies outside its row's band
+                stop = true;
+                break;
+            }
+            int op = -1;
+            if (state == 0) {
+                const uint32_t d = tr & 3u;
+                if (d == 0) {
+                    stop = true;
+                    break;
+                }
+                if (d == 1) {
+                    op = 0;
+                    i--;
+                    j--;
+                } else {
+                    state = d == 2 ? 1 : 2;
+                }
+            } else if (state == 1) {
+                op = 1;
+                i--;
+                state = (tr & 4u) ? 1 : 0;
+            } else {
+                op = 2;
+                j--;
+                state = (tr & 8u) ? 2 : 0;
+            }
+            if (op >= 0) {
+                if (op == last_op) {
+                    run++;
+                } else {
+                    flush();
+                    last_op = op;
+                    run = 1;
+                }
+            }
+        }
+    }
+    flush();
+    if (g == 0) {
+        // ops were produced end-to-start: reverse in place
+        const int32_t m = n < cap ? n : cap;
+        for (int32_t x = 0, y = m - 1; x < y; x++, y--) {
+            uint32_t t = cig[x];
+            cig[x] = cig[y];
+            cig[y] = t;
+        }
+        int32_t *res = a.res + (int64_t)job * kSwResWords;
+        res[4] = j;
+        res[5] = i;
+        res[6] = n <= cap ? n : -1;
+    }
 }
 
 template <int P>
@@ -317,13 +411,11 @@ __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
         if (job >= 0) {
             uint32_t *slab = a.trace + a.packet_trace_off[p];
             int2 *meta = a.row_meta + a.packet_meta_off[p];
-            coop_job<P>(a, job, g, wl, a.packet_rows[p], slab, meta);
-            // lane 0 of the row walks the trace its 15 neighbours helped write
+            const int2 end = coop_job<P>(a, job, g, wl, a.packet_rows[p], slab, meta);
+            // every lane reads trace words its neighbours wrote
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (g == 0) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                sw_traceback_job(a, job, slab, meta, (P + 7) / 8, P, wl);
-            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            coop_traceback<P>(a, job, g, wl, end, slab, meta);
         }
     }
 }
