@@ -192,6 +192,51 @@ def make_sw_workload(genome, n_pairs=1_000_000, seed=3, pad=1024, chunk=100_000)
                 abort=np.full(n_pairs, -(2 ** 31) // 8, dtype=np.int32), n=n_pairs)
 
 
+def make_long_workload(genome, n_windows=82_000, seed=4, pad=1024, chunk=8_000):
+    """Long-read chain-gap windows: source length U[500, 5000] bp, band
+    ceil(5% of the length) (25-250), query drawn from the source with 5%
+    substitutions, 2.5% insertions and 2.5% deletions (1 bp each); the window
+    is the source plus `band` bases on each side, offset = band.  82k windows
+    ~ 20k reads of median 10 kbp cut into chain gaps."""
+    rng = np.random.default_rng(seed)
+    G = len(genome)
+    S = rng.integers(500, 5001, n_windows).astype(np.int64)
+    band = np.ceil(0.05 * S).astype(np.int64)
+    pos = rng.integers(300, G - 5600, n_windows).astype(np.int64)
+    codes_g = np.searchsorted(ACGT, genome)
+    reads, read_len = [], np.empty(n_windows, dtype=np.int32)
+    for s in range(0, n_windows, chunk):
+        e = min(n_windows, s + chunk)
+        n = e - s
+        slen = S[s:e]
+        start = np.repeat(pos[s:e], slen)
+        base_off = np.arange(int(slen.sum())) - np.repeat(np.cumsum(slen) - slen, slen)
+        src = codes_g[start + base_off].astype(np.uint8)
+        tot = len(src)
+        u = rng.random(tot)
+        is_del = u < 0.025
+        is_ins = (u >= 0.025) & (u < 0.05)
+        subm = rng.random(tot) < 0.05
+        src[subm] = (src[subm] + rng.integers(1, 4, int(subm.sum()), dtype=np.uint8)) % 4
+        cnt = 1 - is_del.astype(np.int64) + is_ins.astype(np.int64)
+        out = np.repeat(src, cnt)
+        first = np.cumsum(cnt) - cnt
+        out[first[is_ins]] = rng.integers(0, 4, int(is_ins.sum()), dtype=np.uint8)
+        pair = np.repeat(np.repeat(np.arange(n), slen), cnt)
+        read_len[s:e] = np.bincount(pair, minlength=n)
+        reads.append(ACGT[out])
+    reads = np.concatenate(reads)
+    read_off = np.concatenate([[0], np.cumsum(read_len[:-1], dtype=np.int64)])
+    win_start = pos - band
+    refs = np.zeros(G + 2 * pad, dtype=np.uint8)
+    refs[pad:pad + G] = genome
+    return dict(reads=reads, read_off=read_off.astype(np.int64), read_len=read_len,
+                refs=refs, ref_off=(win_start + pad).astype(np.int64),
+                ref_len=(S + 2 * band).astype(np.int32), offset=band.astype(np.int32),
+                band=band.astype(np.int32),
+                abort=np.full(n_windows, -(2 ** 31) // 8, dtype=np.int32), n=n_windows)
+
+
 # --------------------------------------------------------------- CPU baseline
 _LUT = np.zeros(256, dtype=np.uint8)
 for _c, _v in zip(b"ACGTacgt", [0, 1, 2, 3, 0, 1, 2, 3]):
@@ -265,6 +310,7 @@ def main():
     ap.add_argument("--sw-pairs", type=int, default=1_000_000)
     ap.add_argument("--sw-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
+    ap.add_argument("--long-windows", type=int, default=82_000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; --gpus must be 1 "
@@ -407,17 +453,13 @@ def main():
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
 
 
-def bench_sw(pkg, ctx, genome, args, stream):
-    """Uneven banded SW (1M pairs, L U{100..300}, band U{16,32,64,128}, adaptive,
-    x-drop 100, no abort) in one device-resident batch; GCUPS over the cells
-    the reference algorithm evaluates."""
+def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label):
+    """One device-resident banded SW batch (plan + timed run_dev launches);
+    GCUPS over the cells the reference algorithm evaluates, oracle parity and
+    the oracle's CPU rate on the first `cpu_pairs` jobs."""
     import torch
     dev = torch.device("cuda:0")
-    t0 = time.time()
-    sw = make_sw_workload(genome, n_pairs=args.sw_pairs)
     n = sw["n"]
-    print(f"[bench] SW workload: {n} pairs generated in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
     cfg = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
     kn = pkg.sw_knobs()
     t0 = time.time()
@@ -436,34 +478,36 @@ def bench_sw(pkg, ctx, genome, args, stream):
 
     run()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.sw_steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
     t0 = time.perf_counter()
-    for k in range(args.sw_steps):
+    for k in range(steps):
         evs[k][0].record(stream)
         run()
         evs[k][1].record(stream)
     torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1000 / args.sw_steps
+    wall_ms = (time.perf_counter() - t0) * 1000 / steps
     ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     r = res.cpu().numpy()
     cells = int(r[:, 8].astype(np.int64).sum())
     gcups = cells / (ms * 1e-3) / 1e9
-    print(f"[bench] SW: {ms:.2f} ms/batch ({wall_ms:.2f} wall), {cells / n:.0f} cells/pair, "
-          f"{gcups:.1f} GCUPS", file=sys.stderr, flush=True)
-    out = {"workload": "uneven banded SW: 1M pairs, L U{100..300}, band U{16,32,64,128}, "
-                       "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
-                       "routed per job",
-           "pairs": n, "cells": cells, "ms_per_batch": ms, "wall_ms_per_batch": wall_ms,
-           "gcups": gcups, "plan_ms": t_plan * 1000, "steps": args.sw_steps,
-           "roofline": {"bound": "valu", "achieved": cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12,
-                        "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
-                        "frac": cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+    ops = cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12
+    print(f"[bench] {label}: {n} jobs, {ms:.2f} ms/batch ({wall_ms:.2f} wall), "
+          f"{cells / n:.0f} cells/job, {gcups:.1f} GCUPS, plan {plan.device_bytes / 1e9:.1f} GB",
+          file=sys.stderr, flush=True)
+    out = {"jobs": n, "cells": cells, "ms_per_batch": ms, "wall_ms_per_batch": wall_ms,
+           "gcups": gcups, "plan_ms": t_plan * 1000, "plan_device_gb": plan.device_bytes / 1e9,
+           "steps": steps,
+           "roofline": {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
+                        "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS,
                         "ops_per_cell": SW_OPS_PER_CELL},
            "parity_vs_oracle_sample": None, "cpu_baseline": None}
-    if not args.no_cpu:
+    del res, cig, d
+    plan.close()
+    torch.cuda.empty_cache()
+    if not args.no_cpu and cpu_pairs > 0:
         sys.path.insert(0, ROOT)
         from oracle import oracle as O
-        m = min(args.cpu_sample_pairs, n)
+        m = min(cpu_pairs, n)
         threads = len(os.sched_getaffinity(0))
         oc = O.cfg(1, 4, 6, 1, 32, 100, 5)
         a = [sw["reads"], sw["read_off"][:m], sw["read_len"][:m], sw["refs"], sw["ref_off"][:m],
@@ -480,11 +524,50 @@ def bench_sw(pkg, ctx, genome, args, stream):
             np.array_equal(cc, r[:m, 8].astype(np.int64)))
         out["cpu_baseline"] = {"value": float(cc.sum()) / t / 1e9, "unit": "GCUPS",
                                "cores": threads, "kind": "port",
-                               "sample": f"first {m} pairs of the same batch (scalar "
-                                         f"banded_sw_with, pthreads over pairs), median of 3"}
-        print(f"[bench] SW cpu baseline {cc.sum() / t / 1e9:.2f} GCUPS on {threads} threads; "
-              f"parity={out['parity_vs_oracle_sample']}", file=sys.stderr, flush=True)
-    plan.close()
+                               "sample": f"first {m} jobs of the same batch (scalar "
+                                         f"banded_sw_with, pthreads over jobs), median of 3"}
+        print(f"[bench] {label} cpu baseline {cc.sum() / t / 1e9:.2f} GCUPS on {threads} "
+              f"threads; parity={out['parity_vs_oracle_sample']}", file=sys.stderr, flush=True)
+    return out
+
+
+def concat_sw(a, b):
+    """One queue holding the jobs of a then b (both windows into the same refs)."""
+    out = {"n": a["n"] + b["n"], "refs": a["refs"]}
+    for k in ("read_len", "ref_off", "ref_len", "offset", "band", "abort"):
+        out[k] = np.concatenate([a[k], b[k]])
+    out["reads"] = np.concatenate([a["reads"], b["reads"]])
+    out["read_off"] = np.concatenate([a["read_off"], b["read_off"] + len(a["reads"])])
+    return out
+
+
+def bench_sw(pkg, ctx, genome, args, stream):
+    """Uneven banded SW (1M pairs, L U{100..300}, band U{16,32,64,128}, adaptive,
+    x-drop 100, no abort) in one device-resident batch; long-read chain-gap
+    windows (500-5000 bp, band 5%); and both in one queue."""
+    t0 = time.time()
+    sw = make_sw_workload(genome, n_pairs=args.sw_pairs)
+    print(f"[bench] SW workload: {sw['n']} pairs generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    out = run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, args.cpu_sample_pairs,
+                       "SW short pairs")
+    out["workload"] = ("uneven banded SW: 1M pairs, L U{100..300}, band U{16,32,64,128}, "
+                       "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
+                       "routed per job")
+    out["pairs"] = out["jobs"]
+    if args.long_windows > 0:
+        t0 = time.time()
+        lw = make_long_workload(genome, n_windows=args.long_windows)
+        print(f"[bench] long windows: {lw['n']} generated in {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+        lo = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 256, "SW long windows")
+        lo["workload"] = ("long-read chain-gap windows: 500-5000 bp, band ceil(5% len) "
+                          "(25-250), 5% subs, 2.5% ins, 2.5% del, exact int32 (int16 storage only where proven)")
+        out["long_windows"] = lo
+        mixed = concat_sw(sw, lw)
+        mx = run_sw_batch(pkg, ctx, mixed, args, stream, args.sw_steps, 0, "SW mixed queue")
+        mx["workload"] = "the short pairs and the long windows in one plan / one queue"
+        out["mixed_queue"] = mx
     return out
 
 

@@ -313,6 +313,9 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
                                        kira_sw_plan **out);
 void kira_banded_sw_plan_destroy(kira_sw_plan *plan);
 int64_t kira_banded_sw_plan_cigar_words(const kira_sw_plan *plan);
+/* Device memory the plan holds (trace and row-meta slabs, packet tables),
+ * for admission against an HBM budget. */
+int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan);
 const int64_t *kira_banded_sw_plan_cigar_off(const kira_sw_plan *plan);
 kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
                                    const kira_sw_jobs_dev *jobs, const kira_sw_cfg *cfg,

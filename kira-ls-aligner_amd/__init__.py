@@ -162,6 +162,7 @@ SIGNATURES = {
                                              C.POINTER(C.c_void_p)]),
     "kira_banded_sw_plan_destroy": (None, [_P]),
     "kira_banded_sw_plan_cigar_words": (C.c_int64, [_P]),
+    "kira_banded_sw_plan_device_bytes": (C.c_int64, [_P]),
     "kira_banded_sw_plan_cigar_off": (C.POINTER(C.c_int64), [_P]),
     "kira_banded_sw_run_dev": (C.c_int, [_P, _P, C.POINTER(SwJobsDev), C.POINTER(SwCfg),
                                          C.POINTER(SwKnobs)]),
@@ -556,6 +557,7 @@ class BandedSwPlan:
         ctx.check(st, "banded_sw_plan_create")
         self.h, self.n = h, len(rl)
         self.cigar_words = int(lib().kira_banded_sw_plan_cigar_words(h))
+        self.device_bytes = int(lib().kira_banded_sw_plan_device_bytes(h))
         ptr = lib().kira_banded_sw_plan_cigar_off(h)
         self.cigar_off = (np.ctypeslib.as_array(ptr, shape=(self.n,)).copy() if self.n
                           else np.zeros(0, dtype=np.int64))

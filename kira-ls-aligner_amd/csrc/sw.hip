@@ -29,13 +29,6 @@ __host__ __device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return 
 __host__ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);  // v_alignbit_b32
 }
-// Keeps the scheduler from hoisting every cell's (row-independent) E terms
-// ahead of the row, which multiplies live registers.
-__host__ __device__ __forceinline__ void cell_fence() {
-#if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_amdgcn_sched_barrier(0);
-#endif
-}
 
 __host__ __device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
     int64_t s = (int64_t)a + b;
@@ -84,10 +77,14 @@ struct Band {
 #pragma unroll
         for (int q = 0; q < C; q++) set(q, padH, padE);
     }
+    // No clamping: with the plan's bounds H lies in [0, 32767) or is the pad
+    // -16384, and E in [-16384 - go, 32767) with go <= 16000.
     static __host__ __device__ __forceinline__ uint32_t pack16(int32_t hv, int32_t ev) {
-        int32_t lo = hv < -32768 ? -32768 : (hv > 32767 ? 32767 : hv);
-        int32_t hi = ev < -32768 ? -32768 : (ev > 32767 ? 32767 : ev);
-        return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);  // v_cvt_pk_i16_i32
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_perm((uint32_t)ev, (uint32_t)hv, 0x05040100u);  // v_perm_b32
+#else
+        return ((uint32_t)hv & 0xFFFFu) | ((uint32_t)ev << 16);
+#endif
     }
 };
 
@@ -103,6 +100,9 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
     const int32_t fpadH = kMin4 + go, fpadF = kMin4 + ge;
     int32_t hl = fpadH, fl = fpadF;
     Row r{0, -1};
+    // packed bands keep the row's first strict maximum as one max over
+    // (h << 9 | 511 - q): h < 32767 so the key never overflows
+    int32_t key = 0;
     uint32_t word = 0;
     // Interior rows fetch the band's reference bytes (column j = base + q is
     // ref[j - 1]) as aligned dwords, funnel-shifted once per row; the arena is
@@ -149,7 +149,9 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
             bd.set(q, h, e);
             hl = h;
             fl = f;
-            if (h > r.rb) {
+            if constexpr (PACK) {
+                key = imax(key, (h << 9) | (511 - q));
+            } else if (h > r.rb) {
                 r.rb = h;
                 r.rbq = q;
             }
@@ -159,7 +161,10 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
             trace_row[(q >> 3) * 64 + lane] = word;
             word = 0;
         }
-        cell_fence();
+    }
+    if constexpr (PACK) {
+        r.rb = key >> 9;
+        r.rbq = r.rb > 0 ? 511 - (key & 511) : -1;
     }
     return r;
 }

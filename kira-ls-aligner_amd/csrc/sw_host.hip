@@ -83,6 +83,7 @@ struct kira_sw_plan {
     uint32_t *trace = nullptr;
     int2 *meta = nullptr;
     int64_t cigar_words = 0;
+    int64_t device_bytes = 0;
     std::vector<int64_t> host_cigar_off;
     void release() {
         void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_trace_off,
@@ -175,6 +176,9 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
         acc += ccap[k];
     }
     p->cigar_words = acc;
+    p->device_bytes = toff * 4 + moff * (int64_t)sizeof(int2) +
+                      (int64_t)std::max(p->n_packets, 1) * (64 * 4 + 4 * 2 + 8 * 2) +
+                      (int64_t)n * (4 + 4 + 8) + kNumClasses * 4;
     p->host_cigar_off = coff;
     hipError_t e = hipSetDevice(ctx->device);
     int32_t P = std::max(p->n_packets, 1);
@@ -216,6 +220,10 @@ void kira_banded_sw_plan_destroy(kira_sw_plan *plan) {
 
 int64_t kira_banded_sw_plan_cigar_words(const kira_sw_plan *plan) {
     return plan ? plan->cigar_words : 0;
+}
+
+int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan) {
+    return plan ? plan->device_bytes : 0;
 }
 
 const int64_t *kira_banded_sw_plan_cigar_off(const kira_sw_plan *plan) {
