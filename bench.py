@@ -311,6 +311,10 @@ def main():
     ap.add_argument("--sw-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
     ap.add_argument("--long-windows", type=int, default=82_000)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=int, default=96)
+    ap.add_argument("--serve-load", type=float, default=0.7)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; --gpus must be 1 "
@@ -444,10 +448,148 @@ def main():
                           "ops_per_launch": ops_per_launch},
         "cpu_baseline": cpu,
     }
+    lw = None
+    if args.long_windows > 0 and (args.sw_pairs > 0 or args.serve_requests > 0):
+        t0 = time.time()
+        lw = make_long_workload(work["genome"], n_windows=args.long_windows)
+        print(f"[bench] long windows: {lw['n']} generated in {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     if args.sw_pairs > 0:
-        line["sw"] = bench_sw(pkg, ctx, work["genome"], args, stream)
+        line["sw"] = bench_sw(pkg, ctx, work["genome"], args, stream, lw)
         line["sw_gcups"] = line["sw"]["gcups"]
+    if args.serve_requests > 0 and lw is not None:
+        del reads, lens, win_start, win_len, planes, valid, out
+        torch.cuda.empty_cache()
+        line["serving"] = bench_serving(pkg, ctx, genome, work, lw, args)
     print(json.dumps(line), flush=True)
+
+
+def make_requests(work, lw, n_req, seed=5, max_reads=50_000):
+    """Request stream: batch sizes U{1..50k} reads, 80% short (150 bp, 16
+    candidate windows each; one banded SW job (w = 32) on the true window,
+    dropped when that window is certified) and 20% long reads (each 4
+    chain-gap windows of the long-window set, one banded SW job per window,
+    band 5%)."""
+    rng = np.random.default_rng(seed)
+    L, W, K = work["read_len"], work["W"], work["windows"]
+    n_short_pool = work["n_reads"]
+    reqs = []
+    s_cur, l_cur = 0, 0
+    pad = 1024
+    for _ in range(n_req):
+        n = int(rng.integers(1, max_reads + 1))
+        ns = max(1, int(round(0.8 * n)))
+        nl = n - ns
+        idx = (s_cur + np.arange(ns)) % n_short_pool
+        s_cur += ns
+        short = work["reads"][idx]                        # [ns, L]
+        nwin = 4 * nl
+        widx = (l_cur + np.arange(nwin)) % lw["n"]
+        l_cur += nwin
+        lens_l = lw["read_len"][widx]
+        offs_l = lw["read_off"][widx]
+        long_reads = [lw["reads"][o:o + m] for o, m in zip(offs_l, lens_l)]
+        flat = np.concatenate([short.reshape(-1)] + long_reads) if nwin else short.reshape(-1)
+        lens = np.concatenate([np.full(ns, L, np.int32), lens_l.astype(np.int32)])
+        offs = np.concatenate([[0], np.cumsum(lens[:-1], dtype=np.int64)]).astype(np.int64)
+        sieve_read = np.repeat(np.arange(ns, dtype=np.int32), K)
+        win = work["win_start"].reshape(-1, K)[idx].reshape(-1)
+        sieve = (sieve_read, win, np.full(ns * K, W, np.int32))
+        true_w = win.reshape(ns, K)[:, 0]
+        sw_read = np.concatenate([np.arange(ns, dtype=np.int32),
+                                  ns + np.arange(nwin, dtype=np.int32)])
+        sw_start = np.concatenate([true_w, lw["ref_off"][widx] - pad]).astype(np.int64)
+        sw_len = np.concatenate([np.full(ns, W, np.int32), lw["ref_len"][widx]])
+        sw_off = np.concatenate([np.full(ns, 64, np.int32), lw["offset"][widx]])
+        sw_band = np.concatenate([np.full(ns, 32, np.int32), lw["band"][widx]])
+        sw_abort = np.full(ns + nwin, -(2 ** 31) // 8, np.int32)
+        sw_skip = np.concatenate([np.arange(ns, dtype=np.int32) * K,
+                                  np.full(nwin, -1, np.int32)])
+        reqs.append(dict(flat=flat, offs=offs, lens=lens, sieve=sieve,
+                         sw=(sw_read, sw_start, sw_len, sw_off, sw_band, sw_abort, sw_skip),
+                         n_reads=ns + nl, cells_hint=int(nwin)))
+    return reqs
+
+
+def serve_stream(pkg, ctx, genome, reqs, arrivals, *, slots, budget, graph):
+    """Submit each request at its arrival time (seconds from the start; None =
+    all at once) and wait for all; returns latencies (s) and the span."""
+    srv = pkg.Server(ctx, genome, slots=slots, hbm_budget_bytes=budget,
+                     host_staging_bytes=8 << 30, use_graph=graph,
+                     cfg=pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5), sieve_limit=8, cert_max_mism=4)
+    t0 = time.perf_counter()
+    tickets = []
+    for k, q in enumerate(reqs):
+        if arrivals is not None:
+            dt = t0 + arrivals[k] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        tickets.append(srv.submit(q["flat"], q["offs"], q["lens"], sieve=q["sieve"],
+                                  sw=q["sw"]))
+    res = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["done_s"] - r["submit_s"] for r in res])
+    span = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
+    skipped = int(sum(int(r["sw_skipped"].sum()) for r in res))
+    return lat, span, st, skipped, res
+
+
+def bench_serving(pkg, ctx, genome, work, lw, args):
+    """Poisson arrivals of read batches into the continuous-batching server;
+    p50 / p99 batch latency (submit -> results on the host) and reads/s,
+    against the plain form (one batch on the device at a time)."""
+    n_req = args.serve_requests
+    t0 = time.time()
+    reqs = make_requests(work, lw, n_req)
+    total_reads = sum(q["n_reads"] for q in reqs)
+    print(f"[bench] serving: {n_req} requests, {total_reads} reads generated in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    budget = args.serve_budget_gb << 30
+    # warm-up (first launches, pool first touch)
+    serve_stream(pkg, ctx, genome, reqs[:2], None, slots=4, budget=budget, graph=False)
+    lat, span, st, _, _ = serve_stream(pkg, ctx, genome, reqs, None, slots=args.serve_slots,
+                                       budget=budget, graph=False)
+    capacity = total_reads / span
+    print(f"[bench] serving capacity (all queued at once): {capacity:.0f} reads/s, "
+          f"peak in flight {st['peak_in_flight']}", file=sys.stderr, flush=True)
+    rate = args.serve_load * capacity / (total_reads / n_req)  # requests per second
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    arrivals -= arrivals[0]
+    out = {"workload": "Poisson arrivals of 1-50k-read batches (80% 150 bp short reads x 16 "
+                       "windows + banded SW on the true window unless certified; 20% long "
+                       "reads x 4 chain-gap windows, band 5%), resident genome, HBM-budgeted "
+                       "admission",
+           "requests": n_req, "reads": total_reads, "capacity_reads_per_s": capacity,
+           "load": args.serve_load, "arrival_rate_per_s": rate,
+           "hbm_budget_gb": args.serve_budget_gb, "slots": args.serve_slots}
+    modes = [("continuous", args.serve_slots, False), ("continuous_graph", args.serve_slots, True),
+             ("plain_one_batch_at_a_time", 1, False)]
+    results = {}
+    for name, slots, graph in modes:
+        lat, span, st, skipped, res = serve_stream(pkg, ctx, genome, reqs, arrivals,
+                                                   slots=slots, budget=budget, graph=graph)
+        results[name] = res
+        out[name] = {"p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "mean_ms": float(lat.mean() * 1e3), "reads_per_s": total_reads / span,
+                     "peak_in_flight": st["peak_in_flight"],
+                     "peak_device_gb": st["peak_device_bytes"] / 1e9,
+                     "sw_jobs_dropped_as_certified": skipped}
+        print(f"[bench] serving {name}: p50 {out[name]['p50_ms']:.1f} ms, p99 "
+              f"{out[name]['p99_ms']:.1f} ms, {out[name]['reads_per_s']:.0f} reads/s, peak in "
+              f"flight {st['peak_in_flight']}", file=sys.stderr, flush=True)
+    # the same request gives the same results in every mode
+    same = True
+    base = results["plain_one_batch_at_a_time"]
+    for name in ("continuous", "continuous_graph"):
+        for a, b in zip(base, results[name]):
+            for k in ("shift", "best", "second", "flags", "sw_skipped"):
+                same &= bool(np.array_equal(a[k], b[k]))
+            same &= a["sw"] == b["sw"]
+    out["results_identical_across_modes"] = same
+    return out
 
 
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
@@ -541,7 +683,7 @@ def concat_sw(a, b):
     return out
 
 
-def bench_sw(pkg, ctx, genome, args, stream):
+def bench_sw(pkg, ctx, genome, args, stream, lw):
     """Uneven banded SW (1M pairs, L U{100..300}, band U{16,32,64,128}, adaptive,
     x-drop 100, no abort) in one device-resident batch; long-read chain-gap
     windows (500-5000 bp, band 5%); and both in one queue."""
@@ -555,11 +697,7 @@ def bench_sw(pkg, ctx, genome, args, stream):
                        "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
                        "routed per job")
     out["pairs"] = out["jobs"]
-    if args.long_windows > 0:
-        t0 = time.time()
-        lw = make_long_workload(genome, n_windows=args.long_windows)
-        print(f"[bench] long windows: {lw['n']} generated in {time.time() - t0:.1f}s",
-              file=sys.stderr, flush=True)
+    if lw is not None:
         lo = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 256, "SW long windows")
         lo["workload"] = ("long-read chain-gap windows: 500-5000 bp, band ceil(5% len) "
                           "(25-250), 5% subs, 2.5% ins, 2.5% del, exact int32 (int16 storage only where proven)")

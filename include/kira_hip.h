@@ -95,6 +95,10 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
                                kira_genome **out);
 void kira_genome_destroy(kira_genome *g);
 int64_t kira_genome_len(const kira_genome *g);
+/* Device pointer to base 0 of the resident ASCII sequence, with 1024 zero
+ * bytes readable before it and after its end (the padding the banded SW
+ * device entry points need around every window). */
+const uint8_t *kira_genome_ascii_dev(const kira_genome *g);
 
 /*
  * Read batch packed on the device from ASCII.  reads: n x stride bytes,
@@ -320,6 +324,95 @@ const int64_t *kira_banded_sw_plan_cigar_off(const kira_sw_plan *plan);
 kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
                                    const kira_sw_jobs_dev *jobs, const kira_sw_cfg *cfg,
                                    const kira_sw_knobs *knobs);
+
+/* ------------------------------------------------------------------ serving */
+
+/*
+ * Continuous-batching server over a resident genome (the reference's
+ * gpu-server / dispatcher, src/cuda/server.rs:15-193, dispatcher.rs:21-104,
+ * which run one batch at a time).  kira_server_submit queues a request and
+ * returns at once; one worker thread admits queued requests first come first
+ * served into up to `slots` concurrent streams whenever a slot is free and
+ * the request's device memory fits in the hbm_budget_bytes pool (and its
+ * inputs and outputs in the pinned staging pool), so requests that arrive
+ * while others run overlap on the device.  Per request: one input copy,
+ * read packing, sieve + certification, banded SW (jobs whose linked sieve
+ * job is certified are dropped on the device), one output copy; with
+ * use_graph these are captured and launched as one hipGraph per request.
+ * A request's results do not depend on the other requests in flight.
+ * All request arrays are host arrays, copied at submit.
+ */
+typedef struct kira_server kira_server;
+
+typedef struct {
+    int32_t slots;               /* requests in flight at once, 1..16 */
+    int64_t hbm_budget_bytes;    /* device memory pool for in-flight requests */
+    int64_t host_staging_bytes;  /* pinned host staging pool */
+    int32_t use_graph;           /* one captured hipGraph per request */
+    kira_sw_cfg cfg;
+    kira_sw_knobs knobs;
+    int32_t sieve_limit;         /* accept limit of sieve jobs without their own */
+    int32_t cert_max_mism;       /* certification limit, < 0 disables it */
+    int32_t cert_strict;
+} kira_server_config;
+
+typedef struct {
+    /* reads: read k is reads[read_off[k] .. +read_len[k]] (ASCII); sieve
+       jobs use reads of at most 256 bases (longer ones are not dispatched) */
+    int32_t n_reads;
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    /* sieve jobs: read index, window start (genome coordinate) and length,
+       optional per-job limit */
+    int32_t n_sieve;
+    const int32_t *sieve_read;
+    const int64_t *sieve_win_start;
+    const int32_t *sieve_win_len;
+    const int32_t *sieve_limit;
+    /* banded SW jobs (banded_sw_public): read index, window in genome
+       coordinates, band centre offset, band (NULL: cfg.bandwidth), abort
+       score, and the sieve job whose certification drops the job (NULL or
+       -1: never dropped) */
+    int32_t n_sw;
+    const int32_t *sw_read;
+    const int64_t *sw_win_start;
+    const int32_t *sw_win_len;
+    const int32_t *sw_offset;
+    const int32_t *sw_band;
+    const int32_t *sw_abort;
+    const int32_t *sw_skip_sieve;
+} kira_request;
+
+/* Arrays owned by the server until kira_server_release(ticket). */
+typedef struct {
+    int32_t status; /* kira_status of the request */
+    int32_t n_sieve, n_sw;
+    const int32_t *shift, *best, *second; /* as kira_sieve_batch outputs */
+    const uint8_t *flags;
+    const kira_sw_result *sw;  /* cigar_off indexes `cigar` */
+    const uint8_t *sw_skipped; /* 1: dropped because its sieve job was certified */
+    const uint32_t *cigar;
+    int64_t cigar_words;
+    double submit_s, start_s, done_s; /* steady clock: queued, admitted, results ready */
+} kira_response;
+
+typedef struct {
+    int64_t submitted, admitted, completed;
+    int32_t peak_in_flight;
+    int64_t peak_device_bytes;
+} kira_server_stats;
+
+kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g,
+                               const kira_server_config *cfg, kira_server **out);
+kira_status kira_server_submit(kira_server *s, const kira_request *req, int64_t *ticket);
+/* Blocks until the request is done; out->status carries its own outcome. */
+kira_status kira_server_wait(kira_server *s, int64_t ticket, kira_response *out);
+void kira_server_release(kira_server *s, int64_t ticket);
+void kira_server_get_stats(kira_server *s, kira_server_stats *out);
+const char *kira_server_last_error(kira_server *s);
+/* Drains every queued request, then frees the server. */
+void kira_server_destroy(kira_server *s);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,40 @@ class PrefilterOut(C.Structure):
                 ("cigar", C.c_uint32 * 3)]
 
 
+class ServerConfig(C.Structure):
+    """kira_server_config (include/kira_hip.h)."""
+
+    _fields_ = [("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+                ("host_staging_bytes", C.c_int64), ("use_graph", C.c_int32), ("cfg", SwCfg),
+                ("knobs", SwKnobs), ("sieve_limit", C.c_int32), ("cert_max_mism", C.c_int32),
+                ("cert_strict", C.c_int32)]
+
+
+class Request(C.Structure):
+    """kira_request: one batch of reads with its sieve and banded SW jobs."""
+
+    _fields_ = [("n_reads", C.c_int32), ("reads", C.c_void_p), ("read_off", C.c_void_p),
+                ("read_len", C.c_void_p), ("n_sieve", C.c_int32), ("sieve_read", C.c_void_p),
+                ("sieve_win_start", C.c_void_p), ("sieve_win_len", C.c_void_p),
+                ("sieve_limit", C.c_void_p), ("n_sw", C.c_int32), ("sw_read", C.c_void_p),
+                ("sw_win_start", C.c_void_p), ("sw_win_len", C.c_void_p),
+                ("sw_offset", C.c_void_p), ("sw_band", C.c_void_p), ("sw_abort", C.c_void_p),
+                ("sw_skip_sieve", C.c_void_p)]
+
+
+class Response(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_sieve", C.c_int32), ("n_sw", C.c_int32),
+                ("shift", C.c_void_p), ("best", C.c_void_p), ("second", C.c_void_p),
+                ("flags", C.c_void_p), ("sw", C.POINTER(SwResult)), ("sw_skipped", C.c_void_p),
+                ("cigar", C.c_void_p), ("cigar_words", C.c_int64), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("peak_in_flight", C.c_int32), ("peak_device_bytes", C.c_int64)]
+
+
 # PrefilterResult, and PrefilterReason::idx() -> as_str() (src/alignment/
 # prefilter.rs:31-58); Other(..) carries the sub-reason as 9 + 16 * k.
 PREFILTER_RESULTS = ("accept", "reject", "fallback")
@@ -146,6 +180,7 @@ SIGNATURES = {
     "kira_genome_create": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_void_p)]),
     "kira_genome_destroy": (None, [_P]),
     "kira_genome_len": (C.c_int64, [_P]),
+    "kira_genome_ascii_dev": (C.c_void_p, [_P]),
     "kira_reads_pack_dev": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P]),
     "kira_sieve_genome_dev": (C.c_int, [_P, _P, C.POINTER(SieveBatch)]),
     "kira_ctx_sync": (C.c_int, [_P]),
@@ -158,6 +193,13 @@ SIGNATURES = {
     "kira_prefilter_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P,
                                        _P, C.POINTER(Span), _P, _P, C.POINTER(SwCfg),
                                        C.POINTER(PrefilterParams), C.POINTER(PrefilterOut)]),
+    "kira_server_create": (C.c_int, [_P, _P, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "kira_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "kira_server_wait": (C.c_int, [_P, C.c_int64, C.POINTER(Response)]),
+    "kira_server_release": (None, [_P, C.c_int64]),
+    "kira_server_get_stats": (None, [_P, C.POINTER(ServerStats)]),
+    "kira_server_last_error": (C.c_char_p, [_P]),
+    "kira_server_destroy": (None, [_P]),
     "kira_banded_sw_plan_create": (C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(SwCfg),
                                              C.POINTER(C.c_void_p)]),
     "kira_banded_sw_plan_destroy": (None, [_P]),
@@ -327,6 +369,9 @@ class Genome:
                   "genome_create")
         self.h = h
         self.length = len(seq)
+        # device address of base 0 of the resident ASCII copy (1024 readable
+        # zero bytes on each side)
+        self.ascii_ptr = int(lib().kira_genome_ascii_dev(h) or 0)
 
     def close(self):
         if self.h:
@@ -689,3 +734,122 @@ def prefilter_batch(ctx: Context, chains, cfg: SwCfg, params: Optional[Prefilter
                                   cigar=[int(o.cigar[i]) for i in range(o.n_cigar)])
         res.append(d)
     return res
+
+
+# ------------------------------------------------------------ serving
+def _arr(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+class Server:
+    """kira_server: continuous batching of read batches over a resident genome.
+
+    submit() copies a request and returns a ticket at once; wait(ticket)
+    blocks until its results are back on the host and returns them as numpy
+    arrays with the submit / admit / done times (steady clock, seconds)."""
+
+    def __init__(self, ctx: Context, genome: Genome, *, slots: int = 4,
+                 hbm_budget_bytes: int = 32 << 30, host_staging_bytes: int = 4 << 30,
+                 use_graph: bool = False, cfg: Optional[SwCfg] = None,
+                 knobs: Optional[SwKnobs] = None, sieve_limit: int = 8,
+                 cert_max_mism: int = -1, cert_strict: bool = False):
+        self.genome = genome  # keeps the genome alive
+        c = ServerConfig(slots, hbm_budget_bytes, host_staging_bytes, int(use_graph),
+                         cfg or sw_cfg(), knobs or sw_knobs(), sieve_limit, cert_max_mism,
+                         int(cert_strict))
+        h = C.c_void_p()
+        st = lib().kira_server_create(ctx.h, genome.h, C.byref(c), C.byref(h))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "Runtime"), "kira_server_create")
+        self.h = h
+
+    def submit(self, reads, read_off, read_len, *, sieve=None, sw=None) -> int:
+        """reads: flat uint8 (ASCII); read_off int64, read_len int32.
+        sieve: (read, win_start, win_len[, limit]) arrays; sw: (read,
+        win_start, win_len, offset, band, abort[, skip_sieve]) arrays."""
+        keep = []
+
+        def P(a, dt):
+            a = _arr(a, dt)
+            if a is None:
+                return None
+            keep.append(a)
+            return _np_ptr(a) if a.size else None
+
+        reads = np.ascontiguousarray(np.frombuffer(reads, np.uint8) if isinstance(reads, bytes)
+                                     else reads, dtype=np.uint8)
+        keep.append(reads)
+        r = Request()
+        r.n_reads = len(read_len)
+        r.reads = _np_ptr(reads) if reads.size else None
+        r.read_off = P(read_off, np.int64)
+        r.read_len = P(read_len, np.int32)
+        if sieve is not None:
+            r.n_sieve = len(sieve[0])
+            r.sieve_read = P(sieve[0], np.int32)
+            r.sieve_win_start = P(sieve[1], np.int64)
+            r.sieve_win_len = P(sieve[2], np.int32)
+            r.sieve_limit = P(sieve[3], np.int32) if len(sieve) > 3 else None
+        if sw is not None:
+            r.n_sw = len(sw[0])
+            r.sw_read = P(sw[0], np.int32)
+            r.sw_win_start = P(sw[1], np.int64)
+            r.sw_win_len = P(sw[2], np.int32)
+            r.sw_offset = P(sw[3], np.int32)
+            r.sw_band = P(sw[4], np.int32) if sw[4] is not None else None
+            r.sw_abort = P(sw[5], np.int32)
+            r.sw_skip_sieve = P(sw[6], np.int32) if len(sw) > 6 and sw[6] is not None else None
+        t = C.c_int64()
+        st = lib().kira_server_submit(self.h, C.byref(r), C.byref(t))
+        if st != KIRA_OK:
+            msg = lib().kira_server_last_error(self.h)
+            raise HipError(_STATUS.get(st, "Runtime"), msg.decode() if msg else "submit")
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        o = Response()
+        st = lib().kira_server_wait(self.h, ticket, C.byref(o))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "Runtime"), "wait")
+        try:
+            if o.status != KIRA_OK:
+                msg = lib().kira_server_last_error(self.h)
+                raise HipError(_STATUS.get(o.status, "Runtime"), msg.decode() if msg else "")
+
+            def view(ptr, n, dt):
+                if n == 0 or not ptr:
+                    return np.zeros(0, dtype=dt)
+                return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                             shape=(n,)).copy()
+
+            ns, nw = o.n_sieve, o.n_sw
+            cig = view(o.cigar, o.cigar_words, np.uint32)
+            sw = []
+            for k in range(nw):
+                x = o.sw[k]
+                sw.append(dict(ref_start=x.ref_start, ref_end=x.ref_end, read_start=x.read_start,
+                               read_end=x.read_end, score=x.score,
+                               early_abort=bool(x.early_abort), cells=int(x.cells),
+                               cigar=[int(v) for v in cig[x.cigar_off:x.cigar_off + x.n_cigar]]))
+            return dict(shift=view(o.shift, ns, np.int32), best=view(o.best, ns, np.int32),
+                        second=view(o.second, ns, np.int32), flags=view(o.flags, ns, np.uint8),
+                        sw=sw, sw_skipped=view(o.sw_skipped, nw, np.uint8),
+                        submit_s=o.submit_s, start_s=o.start_s, done_s=o.done_s)
+        finally:
+            lib().kira_server_release(self.h, ticket)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        lib().kira_server_get_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().kira_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

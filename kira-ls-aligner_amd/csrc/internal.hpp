@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "kira_hip.h"
 
@@ -48,9 +49,10 @@ struct SieveLaunch {
 hipError_t launch_pack_ascii_planes(const uint8_t *seq, int64_t n, Planes p, hipStream_t s);
 hipError_t launch_pack_2bit_planes(const uint8_t *packed, int64_t n_bytes, Planes p,
                                    hipStream_t s);
+// Read i starts at reads + (offs ? offs[i] : i * stride).
 hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stride,
-                                   const int32_t *lens, uint32_t *planes, uint8_t *valid,
-                                   hipStream_t s);
+                                   const int64_t *offs, const int32_t *lens, uint32_t *planes,
+                                   uint8_t *valid, hipStream_t s);
 hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stride,
                                   const int32_t *lens, uint32_t *planes, hipStream_t s);
 hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
@@ -90,11 +92,61 @@ struct SwLaunch {
     uint32_t *cigar;
     const int64_t *cigar_off;
     const int32_t *cigar_cap;
+    // optional: job k is dropped (result all zero, res[9] = 1) when
+    // skip_src[k] >= 0 and skip_flags[skip_src[k]] has the certified bit (4)
+    const uint8_t *skip_flags;
+    const int32_t *skip_src;
 };
+
+// Result of a job dropped through SwLaunch::skip_src.
+__host__ __device__ inline bool sw_job_skipped(const SwLaunch &a, int32_t job) {
+    if (!a.skip_src) return false;
+    const int32_t src = a.skip_src[job];
+    return src >= 0 && (a.skip_flags[src] & 4u) != 0;
+}
+__host__ __device__ inline void sw_write_skipped(const SwLaunch &a, int32_t job) {
+    int32_t *res = a.res + (int64_t)job * kSwResWords;
+    for (int k = 0; k < kSwResWords; k++) res[k] = 0;
+    res[9] = 1;
+}
 
 hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
 // Wide bands: 16 lanes per job, P band positions per lane (P in 9, 17, 25, 33).
 hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int n_waves, hipStream_t s);
+
+// Host-side packetisation of a banded SW batch (jobs sorted by band class
+// and read length into packets, trace / meta slab offsets, CIGAR arena).
+constexpr int kSwClasses = 16;
+struct SwPlanLayout {
+    int32_t n = 0, n_packets = 0;
+    int32_t class_first[kSwClasses] = {}, class_count[kSwClasses] = {};
+    std::vector<int32_t> packet_jobs, packet_rows, packet_rowdw, job_slot, cigar_cap;
+    std::vector<int64_t> packet_trace_off, packet_meta_off, cigar_off;
+    int64_t trace_words = 0, meta_rows = 0, cigar_words = 0;
+};
+// nullptr on success, else the reason the batch cannot run.
+const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *band,
+                           const kira_sw_cfg &cfg, SwPlanLayout &out);
+// The layout's tables as one block (host copy written by sw_plan_pack_tables,
+// device copy bound by sw_plan_bind; both 256-byte aligned sections).
+size_t sw_plan_table_bytes(const SwPlanLayout &L);
+void sw_plan_pack_tables(const SwPlanLayout &L, uint8_t *dst);
+// Device bytes of the trace and meta slabs plus the class counters.
+size_t sw_plan_slab_bytes(const SwPlanLayout &L);
+struct SwPlanDev {
+    const int32_t *packet_jobs, *packet_rows, *packet_rowdw, *job_slot, *cigar_cap;
+    const int64_t *packet_trace_off, *packet_meta_off, *cigar_off;
+    uint32_t *trace;
+    int2 *meta;
+    int32_t *counter;
+};
+SwPlanDev sw_plan_bind(const SwPlanLayout &L, const uint8_t *dev_tables, uint8_t *dev_slabs);
+// All launches of one batch on stream s (nullptr on success, else the error
+// and *e the HIP status).
+const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira_sw_jobs_dev &j,
+                           const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
+                           const uint8_t *skip_flags, const int32_t *skip_src, hipStream_t s,
+                           hipError_t *e);
 
 // Full-window lane-group SW (align_batch_simd's SIMD kernels).
 struct GroupLaunch {
@@ -163,4 +215,5 @@ struct kira_ctx {
 struct kira_genome {
     kira_ctx *ctx = nullptr;
     kira::Planes planes;
+    uint8_t *ascii = nullptr;  // base 0 of the ASCII copy; kRefPad zero bytes on each side
 };

@@ -254,28 +254,33 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
     void *mem = nullptr;
     if ((e = hipMalloc(&mem, (size_t)probe.n_words * 4 * 4)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome allocation", e);
+    // the ASCII sequence stays resident too (SW and prefilter windows read
+    // bytes), with kRefPad zero bytes on each side for band rows that poke
+    // past the contig ends
     uint8_t *d_seq = nullptr;
-    if ((e = hipMalloc(&d_seq, (size_t)std::max<int64_t>(len, 1)))) {
+    if ((e = hipMalloc(&d_seq, (size_t)len + 2 * kira::kRefPad))) {
         (void)hipFree(mem);
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome staging allocation", e);
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome allocation", e);
     }
     kira::Planes p = planes_in(mem, len, true);
-    if ((len > 0 && (e = hipMemcpyAsync(d_seq, seq, (size_t)len, hipMemcpyHostToDevice,
-                                        ctx->stream))) ||
-        (e = kira::launch_pack_ascii_planes(d_seq, len, p, ctx->stream)) ||
+    if ((e = hipMemsetAsync(d_seq, 0, (size_t)len + 2 * kira::kRefPad, ctx->stream)) ||
+        (len > 0 && (e = hipMemcpyAsync(d_seq + kira::kRefPad, seq, (size_t)len,
+                                        hipMemcpyHostToDevice, ctx->stream))) ||
+        (e = kira::launch_pack_ascii_planes(d_seq + kira::kRefPad, len, p, ctx->stream)) ||
         (e = hipStreamSynchronize(ctx->stream))) {
         (void)hipFree(mem);
         (void)hipFree(d_seq);
         return fail(ctx, KIRA_ERR_RUNTIME, "genome packing", e);
     }
-    (void)hipFree(d_seq);
     kira_genome *g = new (std::nothrow) kira_genome();
     if (!g) {
         (void)hipFree(mem);
+        (void)hipFree(d_seq);
         return KIRA_ERR_OUT_OF_MEMORY;
     }
     g->ctx = ctx;
     g->planes = p;
+    g->ascii = d_seq + kira::kRefPad;
     *out = g;
     return KIRA_OK;
 }
@@ -283,8 +288,11 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
 void kira_genome_destroy(kira_genome *g) {
     if (!g) return;
     if (g->planes.lo) (void)hipFree(g->planes.lo);
+    if (g->ascii) (void)hipFree(g->ascii - kira::kRefPad);
     delete g;
 }
+
+const uint8_t *kira_genome_ascii_dev(const kira_genome *g) { return g ? g->ascii : nullptr; }
 
 int64_t kira_genome_len(const kira_genome *g) { return g ? g->planes.n_bases : 0; }
 
@@ -293,8 +301,8 @@ kira_status kira_reads_pack_dev(kira_ctx *ctx, int32_t n, const uint8_t *reads, 
                                 uint8_t *valid_out) {
     if (!ctx || n < 0 || (n > 0 && (!reads || !read_lens || !planes_out)))
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
-    hipError_t e = kira::launch_pack_reads_ascii(n, reads, stride, read_lens, planes_out,
-                                                 valid_out, ctx->stream);
+    hipError_t e = kira::launch_pack_reads_ascii(n, reads, stride, nullptr, read_lens,
+                                                 planes_out, valid_out, ctx->stream);
     if (e != hipSuccess) return fail(ctx, KIRA_ERR_RUNTIME, "read pack launch", e);
     return KIRA_OK;
 }

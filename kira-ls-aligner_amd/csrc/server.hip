@@ -1,0 +1,655 @@
+// Serving loop: continuous batching of read batches against a resident
+// genome.  The reference's GPU server (src/cuda/server.rs:15-193) feeds one
+// CudaBackend through a single dispatcher thread (src/cuda/dispatcher.rs:21-104)
+// and runs one batch at a time; here one worker thread admits queued
+// requests into up to `slots` concurrent streams as soon as a slot and the
+// request's device memory (carved from one pool of hbm_budget_bytes) are
+// free, so batches that arrive while others run overlap on the device.
+//
+// A request is a batch of reads with sieve jobs (read, window) and banded SW
+// jobs (read, window, offset, band, abort, optional sieve job whose
+// certification drops it).  Per request, on its slot's stream: one
+// host-to-device copy of every input from pinned staging, read packing,
+// the sieve + certification, the banded SW kernels (certified jobs skipped
+// on the device), one device-to-host copy of every output, an event.  With
+// use_graph the whole sequence is captured once per request and launched as
+// a hipGraph.  Results of a request do not depend on what else is in flight:
+// each request owns its memory and every kernel reads only its own inputs
+// and the read-only genome.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double now_s() {
+    return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// First-fit range allocator over one preallocated block.
+class RangePool {
+    std::map<size_t, size_t> free_;  // offset -> size
+    size_t cap_ = 0, used_ = 0;
+
+   public:
+    void init(size_t cap) {
+        cap_ = cap;
+        used_ = 0;
+        free_.clear();
+        if (cap) free_[0] = cap;
+    }
+    size_t capacity() const { return cap_; }
+    size_t used() const { return used_; }
+    bool alloc(size_t n, size_t &off) {
+        n = align256(std::max<size_t>(n, 1));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            size_t rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[off + n] = rest;
+            used_ += n;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align256(std::max<size_t>(n, 1));
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+// Byte layout of one request's staging (host, pinned) and device blocks.
+struct Layout {
+    // input section (same offsets on host and device)
+    size_t reads, read_off, read_len, s_read, s_start, s_len, s_limit;
+    size_t w_roff, w_rlen, w_foff, w_flen, w_off, w_band, w_abort, w_skip, tables, in_bytes;
+    // output section (offsets relative to the output section)
+    size_t o_shift, o_best, o_second, o_flags, o_res, o_cigar, out_bytes;
+    // device-only scratch (relative to the scratch section)
+    size_t planes, valid, slabs, scratch_bytes;
+};
+
+struct Request {
+    int64_t ticket = 0;
+    double t_submit = 0, t_start = 0;
+    int32_t nr = 0, ns = 0, nw = 0, max_sieve_len = 0;
+    std::vector<uint8_t> reads;
+    std::vector<int64_t> read_off;
+    std::vector<int32_t> read_len;
+    std::vector<int32_t> s_read, s_len, s_limit;
+    std::vector<int64_t> s_start;
+    std::vector<int64_t> w_roff, w_foff;
+    std::vector<int32_t> w_rlen, w_flen, w_off, w_band, w_abort, w_skip;
+    kira::SwPlanLayout plan;
+    Layout lay{};
+    bool laid_out = false;
+    size_t dev_off = 0, dev_bytes = 0, host_off = 0, host_bytes = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
+struct Result {
+    int32_t status = KIRA_OK;
+    std::string err;
+    std::vector<int32_t> shift, best, second;
+    std::vector<uint8_t> flags, skipped;
+    std::vector<kira_sw_result> sw;
+    std::vector<uint32_t> cigar;
+    double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+}  // namespace
+
+struct kira_server {
+    int device = 0;
+    const kira_genome *genome = nullptr;
+    kira_server_config cfg{};
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
+    std::vector<std::unique_ptr<Request>> slot_req;  // null = slot free
+    uint8_t *dpool = nullptr, *hpool = nullptr;
+    RangePool dalloc, halloc;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::unique_ptr<Request>> pending;
+    std::unordered_map<int64_t, std::unique_ptr<Result>> results;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::string err;
+    kira_server_stats stats{};
+
+    void run();
+    bool layout(Request &r, std::string &why);
+    bool admit(std::unique_ptr<Request> &r, int slot, std::string &why);
+    hipError_t enqueue(Request &r, hipStream_t s);
+    void finish(int slot, hipError_t e);
+};
+
+bool kira_server::layout(Request &r, std::string &why) {
+    if (r.laid_out) return true;
+    if (const char *w = kira::sw_plan_layout(r.nw, r.w_rlen.data(), r.w_band.data(), cfg.cfg,
+                                             r.plan)) {
+        why = w;
+        return false;
+    }
+    Layout &L = r.lay;
+    size_t o = 0;
+    auto sec = [&](size_t bytes) {
+        size_t at = o;
+        o += align256(std::max<size_t>(bytes, 1));
+        return at;
+    };
+    L.reads = sec(r.reads.size());
+    L.read_off = sec((size_t)r.nr * 8);
+    L.read_len = sec((size_t)r.nr * 4);
+    L.s_read = sec((size_t)r.ns * 4);
+    L.s_start = sec((size_t)r.ns * 8);
+    L.s_len = sec((size_t)r.ns * 4);
+    L.s_limit = sec((size_t)r.ns * 4);
+    L.w_roff = sec((size_t)r.nw * 8);
+    L.w_rlen = sec((size_t)r.nw * 4);
+    L.w_foff = sec((size_t)r.nw * 8);
+    L.w_flen = sec((size_t)r.nw * 4);
+    L.w_off = sec((size_t)r.nw * 4);
+    L.w_band = sec((size_t)r.nw * 4);
+    L.w_abort = sec((size_t)r.nw * 4);
+    L.w_skip = sec((size_t)r.nw * 4);
+    L.tables = sec(kira::sw_plan_table_bytes(r.plan));
+    L.in_bytes = o;
+    o = 0;
+    L.o_shift = sec((size_t)r.ns * 4);
+    L.o_best = sec((size_t)r.ns * 4);
+    L.o_second = sec((size_t)r.ns * 4);
+    L.o_flags = sec((size_t)r.ns);
+    L.o_res = sec((size_t)r.nw * kira::kSwResWords * 4);
+    L.o_cigar = sec((size_t)r.plan.cigar_words * 4);
+    L.out_bytes = o;
+    o = 0;
+    L.planes = sec((size_t)r.nr * kira::kReadWords * 4);
+    L.valid = sec((size_t)r.nr);
+    L.slabs = sec(kira::sw_plan_slab_bytes(r.plan));
+    L.scratch_bytes = o;
+    r.dev_bytes = L.in_bytes + L.out_bytes + L.scratch_bytes;
+    r.host_bytes = L.in_bytes + L.out_bytes;
+    r.laid_out = true;
+    return true;
+}
+
+hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
+    const Layout &L = r.lay;
+    uint8_t *din = dpool + r.dev_off, *dout = din + L.in_bytes, *dscr = dout + L.out_bytes;
+    uint8_t *hin = hpool + r.host_off, *hout = hin + L.in_bytes;
+    hipError_t e = hipMemcpyAsync(din, hin, L.in_bytes, hipMemcpyHostToDevice, s);
+    if (e) return e;
+    uint32_t *planes = (uint32_t *)(dscr + L.planes);
+    uint8_t *valid = dscr + L.valid;
+    int32_t *o_shift = (int32_t *)(dout + L.o_shift), *o_best = (int32_t *)(dout + L.o_best);
+    int32_t *o_second = (int32_t *)(dout + L.o_second);
+    uint8_t *o_flags = dout + L.o_flags;
+    if (r.ns > 0) {
+        if ((e = kira::launch_pack_reads_ascii(r.nr, din + L.reads, 0,
+                                               (const int64_t *)(din + L.read_off),
+                                               (const int32_t *)(din + L.read_len), planes,
+                                               valid, s)))
+            return e;
+        kira::SieveLaunch a{};
+        a.n_jobs = r.ns;
+        a.read_planes = planes;
+        a.read_lens = (const int32_t *)(din + L.read_len);
+        a.read_valid = valid;
+        a.job_read = (const int32_t *)(din + L.s_read);
+        a.jobs_per_read = 1;
+        a.win_start = (const int64_t *)(din + L.s_start);
+        a.win_len = (const int32_t *)(din + L.s_len);
+        a.limit = (const int32_t *)(din + L.s_limit);
+        a.limit_all = cfg.sieve_limit;
+        a.out_shift = o_shift;
+        a.out_best = o_best;
+        a.out_second = o_second;
+        a.out_flags = o_flags;
+        a.cert_max_mism = cfg.cert_max_mism;
+        a.cert_strict = cfg.cert_strict;
+        const kira::Planes &gp = genome->planes;
+        a.g_lo = gp.lo;
+        a.g_hi = gp.hi;
+        a.g_inv = gp.inv;
+        a.g_lc = gp.lc;
+        a.g_bases = gp.n_bases;
+        int rw = std::max(1, std::min(8, (r.max_sieve_len + 31) / 32));
+        if ((e = kira::launch_sieve(a, rw, false, s))) return e;
+    }
+    if (r.nw > 0) {
+        kira::SwPlanDev d = kira::sw_plan_bind(r.plan, din + L.tables, dscr + L.slabs);
+        kira_sw_jobs_dev j{};
+        j.reads = din + L.reads;
+        j.read_off = (const int64_t *)(din + L.w_roff);
+        j.read_len = (const int32_t *)(din + L.w_rlen);
+        j.refs = genome->ascii;
+        j.ref_off = (const int64_t *)(din + L.w_foff);
+        j.ref_len = (const int32_t *)(din + L.w_flen);
+        j.offset = (const int32_t *)(din + L.w_off);
+        j.band = (const int32_t *)(din + L.w_band);
+        j.abort_score = (const int32_t *)(din + L.w_abort);
+        j.res = (int32_t *)(dout + L.o_res);
+        j.cigar = (uint32_t *)(dout + L.o_cigar);
+        const char *why = kira::sw_plan_launch(r.plan, d, j, cfg.cfg, cfg.knobs,
+                                               r.ns > 0 ? o_flags : nullptr,
+                                               (const int32_t *)(din + L.w_skip), s, &e);
+        if (why) return e ? e : hipErrorInvalidValue;
+    }
+    return hipMemcpyAsync(hout, dout, L.out_bytes, hipMemcpyDeviceToHost, s);
+}
+
+bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why) {
+    Request &r = *rp;
+    const Layout &L = r.lay;
+    // stage every input in pinned memory: one copy per request
+    uint8_t *hin = hpool + r.host_off;
+    auto put = [&](size_t off, const void *src, size_t bytes) {
+        if (bytes) memcpy(hin + off, src, bytes);
+    };
+    put(L.reads, r.reads.data(), r.reads.size());
+    put(L.read_off, r.read_off.data(), (size_t)r.nr * 8);
+    put(L.read_len, r.read_len.data(), (size_t)r.nr * 4);
+    put(L.s_read, r.s_read.data(), (size_t)r.ns * 4);
+    put(L.s_start, r.s_start.data(), (size_t)r.ns * 8);
+    put(L.s_len, r.s_len.data(), (size_t)r.ns * 4);
+    put(L.s_limit, r.s_limit.data(), (size_t)r.ns * 4);
+    put(L.w_roff, r.w_roff.data(), (size_t)r.nw * 8);
+    put(L.w_rlen, r.w_rlen.data(), (size_t)r.nw * 4);
+    put(L.w_foff, r.w_foff.data(), (size_t)r.nw * 8);
+    put(L.w_flen, r.w_flen.data(), (size_t)r.nw * 4);
+    put(L.w_off, r.w_off.data(), (size_t)r.nw * 4);
+    put(L.w_band, r.w_band.data(), (size_t)r.nw * 4);
+    put(L.w_abort, r.w_abort.data(), (size_t)r.nw * 4);
+    put(L.w_skip, r.w_skip.data(), (size_t)r.nw * 4);
+    kira::sw_plan_pack_tables(r.plan, hin + L.tables);
+    hipStream_t s = streams[slot];
+    hipError_t e;
+    if (cfg.use_graph) {
+        hipGraph_t graph = nullptr;
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) {
+            why = hipGetErrorString(e);
+            return false;
+        }
+        hipError_t eq = enqueue(r, s);
+        e = hipStreamEndCapture(s, &graph);
+        if (eq || e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            why = hipGetErrorString(eq ? eq : e);
+            return false;
+        }
+        e = hipGraphInstantiate(&r.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e || (e = hipGraphLaunch(r.exec, s))) {
+            why = hipGetErrorString(e);
+            return false;
+        }
+    } else if ((e = enqueue(r, s))) {
+        why = hipGetErrorString(e);
+        return false;
+    }
+    if ((e = hipEventRecord(events[slot], s))) {
+        why = hipGetErrorString(e);
+        return false;
+    }
+    slot_req[slot] = std::move(rp);
+    return true;
+}
+
+void kira_server::finish(int slot, hipError_t e) {
+    std::unique_ptr<Request> rp = std::move(slot_req[slot]);
+    Request &r = *rp;
+    auto res = std::make_unique<Result>();
+    res->t_submit = r.t_submit;
+    res->t_start = r.t_start;
+    if (e != hipSuccess) {
+        res->status = KIRA_ERR_RUNTIME;
+        res->err = hipGetErrorString(e);
+    } else {
+        const Layout &L = r.lay;
+        const uint8_t *hout = hpool + r.host_off + L.in_bytes;
+        auto get32 = [&](std::vector<int32_t> &v, size_t off, size_t n) {
+            v.resize(n);
+            if (n) memcpy(v.data(), hout + off, n * 4);
+        };
+        get32(res->shift, L.o_shift, r.ns);
+        get32(res->best, L.o_best, r.ns);
+        get32(res->second, L.o_second, r.ns);
+        res->flags.assign(hout + L.o_flags, hout + L.o_flags + r.ns);
+        const int32_t *rr = (const int32_t *)(hout + L.o_res);
+        const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
+        res->sw.resize(r.nw);
+        res->skipped.resize(r.nw);
+        for (int32_t k = 0; k < r.nw; k++) {
+            const int32_t *q = rr + (size_t)k * kira::kSwResWords;
+            kira_sw_result &o = res->sw[k];
+            o.score = q[0];
+            o.read_end = q[1];
+            o.ref_end = (uint32_t)q[2];
+            o.early_abort = q[3];
+            o.ref_start = (uint32_t)q[4];
+            o.read_start = q[5];
+            o.n_cigar = q[6];
+            o.cells = q[8];
+            o.cigar_off = (int64_t)res->cigar.size();
+            res->skipped[k] = (uint8_t)(q[9] == 1);
+            if (q[6] < 0 || q[7] != 0) {
+                res->status = KIRA_ERR_RUNTIME;
+                res->err = q[6] < 0 ? "CIGAR capacity exceeded" : "int16 band overflow";
+                o.n_cigar = 0;
+                continue;
+            }
+            const uint32_t *src = cg + r.plan.cigar_off[k];
+            res->cigar.insert(res->cigar.end(), src, src + q[6]);
+        }
+    }
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    res->t_done = now_s();
+    std::lock_guard<std::mutex> lk(mu);
+    dalloc.release(r.dev_off, r.dev_bytes);
+    halloc.release(r.host_off, r.host_bytes);
+    stats.completed++;
+    results[r.ticket] = std::move(res);
+    cv_done.notify_all();
+}
+
+void kira_server::run() {
+    (void)hipSetDevice(device);
+    const int S = (int)streams.size();
+    for (;;) {
+        bool progressed = false;
+        // completions
+        for (int k = 0; k < S; k++) {
+            if (!slot_req[k]) continue;
+            hipError_t q = hipEventQuery(events[k]);
+            if (q == hipErrorNotReady) continue;
+            finish(k, q);
+            progressed = true;
+        }
+        // admissions, first come first served
+        std::unique_lock<std::mutex> lk(mu);
+        int in_flight = 0;
+        for (int k = 0; k < S; k++) in_flight += slot_req[k] ? 1 : 0;
+        while (!pending.empty() && in_flight < S) {
+            Request &r = *pending.front();
+            std::string why;
+            bool ok = layout(r, why);
+            if (ok && r.dev_bytes > dalloc.capacity()) {
+                ok = false;
+                why = "request larger than the device memory budget";
+            }
+            if (ok && r.host_bytes > halloc.capacity()) {
+                ok = false;
+                why = "request larger than the host staging pool";
+            }
+            if (!ok) {
+                auto res = std::make_unique<Result>();
+                res->status = why.find("budget") != std::string::npos ||
+                                      why.find("staging") != std::string::npos
+                                  ? KIRA_ERR_OUT_OF_MEMORY
+                                  : KIRA_ERR_INVALID_ARGUMENT;
+                res->err = why;
+                res->t_submit = res->t_start = r.t_submit;
+                res->t_done = now_s();
+                results[r.ticket] = std::move(res);
+                pending.pop_front();
+                cv_done.notify_all();
+                progressed = true;
+                continue;
+            }
+            size_t doff, hoff;
+            if (!dalloc.alloc(r.dev_bytes, doff)) break;  // wait for memory to free up
+            if (!halloc.alloc(r.host_bytes, hoff)) {
+                dalloc.release(doff, r.dev_bytes);
+                break;
+            }
+            r.dev_off = doff;
+            r.host_off = hoff;
+            int slot = 0;
+            while (slot_req[slot]) slot++;
+            std::unique_ptr<Request> rp = std::move(pending.front());
+            pending.pop_front();
+            const int64_t ticket = rp->ticket;
+            stats.admitted++;
+            stats.peak_device_bytes = std::max<int64_t>(stats.peak_device_bytes, dalloc.used());
+            lk.unlock();
+            rp->t_start = now_s();
+            std::string why2;
+            Request *raw = rp.get();
+            const bool launched = admit(rp, slot, why2);  // on success the slot owns it
+            lk.lock();
+            if (!launched) {
+                dalloc.release(raw->dev_off, raw->dev_bytes);
+                halloc.release(raw->host_off, raw->host_bytes);
+                if (raw->exec) (void)hipGraphExecDestroy(raw->exec);
+                auto res = std::make_unique<Result>();
+                res->status = KIRA_ERR_RUNTIME;
+                res->err = why2;
+                res->t_submit = raw->t_submit;
+                res->t_start = raw->t_start;
+                res->t_done = now_s();
+                results[ticket] = std::move(res);
+                cv_done.notify_all();
+            }
+            progressed = true;
+            in_flight = 0;
+            for (int k = 0; k < S; k++) in_flight += slot_req[k] ? 1 : 0;
+            stats.peak_in_flight = std::max<int32_t>(stats.peak_in_flight, in_flight);
+        }
+        if (stop && pending.empty() && in_flight == 0) return;
+        if (!progressed) cv_work.wait_for(lk, std::chrono::microseconds(in_flight ? 50 : 2000));
+    }
+}
+
+extern "C" {
+
+kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g, const kira_server_config *c,
+                               kira_server **out) {
+    if (!ctx || !g || !c || !out || c->slots < 1 || c->slots > 16 || c->hbm_budget_bytes <= 0 ||
+        c->host_staging_bytes <= 0)
+        return KIRA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    kira_server *s = new (std::nothrow) kira_server();
+    if (!s) return KIRA_ERR_OUT_OF_MEMORY;
+    s->device = ctx->device;
+    s->genome = g;
+    s->cfg = *c;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (!e) e = hipMalloc((void **)&s->dpool, (size_t)c->hbm_budget_bytes);
+    if (!e) e = hipHostMalloc((void **)&s->hpool, (size_t)c->host_staging_bytes, 0);
+    for (int k = 0; !e && k < c->slots; k++) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (!e) s->streams.push_back(st);
+        if (!e) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (!e) s->events.push_back(ev);
+    }
+    if (e) {
+        for (auto st : s->streams) (void)hipStreamDestroy(st);
+        for (auto ev : s->events) (void)hipEventDestroy(ev);
+        if (s->dpool) (void)hipFree(s->dpool);
+        if (s->hpool) (void)hipHostFree(s->hpool);
+        if (ctx) ctx->err = std::string("server create: ") + hipGetErrorString(e);
+        delete s;
+        return KIRA_ERR_OUT_OF_MEMORY;
+    }
+    s->dalloc.init((size_t)c->hbm_budget_bytes);
+    s->halloc.init((size_t)c->host_staging_bytes);
+    s->slot_req.resize(c->slots);
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return KIRA_OK;
+}
+
+kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *ticket) {
+    if (!s || !q || !ticket || q->n_reads < 0 || q->n_sieve < 0 || q->n_sw < 0)
+        return KIRA_ERR_INVALID_ARGUMENT;
+    auto bad = [&](const char *why) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = why;
+        return KIRA_ERR_INVALID_ARGUMENT;
+    };
+    if ((q->n_reads > 0 && (!q->reads || !q->read_off || !q->read_len)) ||
+        (q->n_sieve > 0 && (!q->sieve_read || !q->sieve_win_start || !q->sieve_win_len)) ||
+        (q->n_sw > 0 && (!q->sw_read || !q->sw_win_start || !q->sw_win_len || !q->sw_offset ||
+                         !q->sw_abort)))
+        return bad("missing request array");
+    const int64_t G = s->genome->planes.n_bases;
+    auto r = std::make_unique<Request>();
+    r->nr = q->n_reads;
+    r->ns = q->n_sieve;
+    r->nw = q->n_sw;
+    int64_t total = 0;
+    for (int32_t k = 0; k < r->nr; k++) {
+        if (q->read_off[k] < 0 || q->read_len[k] < 0) return bad("negative read offset / length");
+        total = std::max<int64_t>(total, q->read_off[k] + q->read_len[k]);
+    }
+    // reads are re-based to one contiguous copy
+    r->reads.reserve((size_t)std::max<int64_t>(total, 1));
+    r->read_off.resize(r->nr);
+    r->read_len.assign(q->read_len, q->read_len + r->nr);
+    int64_t pos = 0;
+    for (int32_t k = 0; k < r->nr; k++) {
+        r->read_off[k] = pos;
+        r->reads.insert(r->reads.end(), q->reads + q->read_off[k],
+                        q->reads + q->read_off[k] + q->read_len[k]);
+        pos += q->read_len[k];
+    }
+    r->s_read.assign(q->sieve_read, q->sieve_read + r->ns);
+    r->s_start.assign(q->sieve_win_start, q->sieve_win_start + r->ns);
+    r->s_len.assign(q->sieve_win_len, q->sieve_win_len + r->ns);
+    r->s_limit.resize(r->ns);
+    for (int32_t k = 0; k < r->ns; k++) {
+        if (r->s_read[k] < 0 || r->s_read[k] >= r->nr) return bad("sieve job read out of range");
+        if (r->s_start[k] < 0 || r->s_len[k] < 0 || r->s_start[k] + r->s_len[k] > G)
+            return bad("sieve window outside the genome");
+        r->s_limit[k] = q->sieve_limit ? q->sieve_limit[k] : s->cfg.sieve_limit;
+        r->max_sieve_len = std::max(r->max_sieve_len, r->read_len[r->s_read[k]]);
+    }
+    r->max_sieve_len = std::min(r->max_sieve_len, kira::kMaxReadBases);
+    r->w_roff.resize(r->nw);
+    r->w_rlen.resize(r->nw);
+    r->w_foff.resize(r->nw);
+    r->w_flen.assign(q->sw_win_len, q->sw_win_len + r->nw);
+    r->w_off.assign(q->sw_offset, q->sw_offset + r->nw);
+    r->w_band.resize(r->nw);
+    r->w_abort.assign(q->sw_abort, q->sw_abort + r->nw);
+    r->w_skip.resize(r->nw);
+    for (int32_t k = 0; k < r->nw; k++) {
+        const int32_t rd = q->sw_read[k];
+        if (rd < 0 || rd >= r->nr) return bad("SW job read out of range");
+        if (q->sw_win_start[k] < 0 || q->sw_win_len[k] < 0 ||
+            q->sw_win_start[k] + q->sw_win_len[k] > G)
+            return bad("SW window outside the genome");
+        r->w_roff[k] = r->read_off[rd];
+        r->w_rlen[k] = r->read_len[rd];
+        r->w_foff[k] = q->sw_win_start[k];
+        r->w_band[k] = q->sw_band ? q->sw_band[k] : s->cfg.cfg.bandwidth;
+        r->w_skip[k] = q->sw_skip_sieve ? q->sw_skip_sieve[k] : -1;
+        if (r->w_skip[k] >= r->ns) return bad("SW skip index out of range");
+    }
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return KIRA_ERR_INVALID_ARGUMENT;
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->stats.submitted++;
+    s->pending.push_back(std::move(r));
+    s->cv_work.notify_one();
+    return KIRA_OK;
+}
+
+kira_status kira_server_wait(kira_server *s, int64_t ticket, kira_response *out) {
+    if (!s || !out) return KIRA_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket <= 0 || ticket >= s->next_ticket) return KIRA_ERR_INVALID_ARGUMENT;
+    s->cv_done.wait(lk, [&] { return s->results.count(ticket) > 0; });
+    const Result &r = *s->results[ticket];
+    memset(out, 0, sizeof(*out));
+    out->status = r.status;
+    out->n_sieve = (int32_t)r.shift.size();
+    out->n_sw = (int32_t)r.sw.size();
+    out->shift = r.shift.data();
+    out->best = r.best.data();
+    out->second = r.second.data();
+    out->flags = r.flags.data();
+    out->sw = r.sw.data();
+    out->sw_skipped = r.skipped.data();
+    out->cigar = r.cigar.data();
+    out->cigar_words = (int64_t)r.cigar.size();
+    out->submit_s = r.t_submit;
+    out->start_s = r.t_start;
+    out->done_s = r.t_done;
+    if (r.status != KIRA_OK) s->err = r.err;
+    return KIRA_OK;
+}
+
+void kira_server_release(kira_server *s, int64_t ticket) {
+    if (!s) return;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->results.erase(ticket);
+}
+
+void kira_server_get_stats(kira_server *s, kira_server_stats *out) {
+    if (!s || !out) return;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+}
+
+const char *kira_server_last_error(kira_server *s) {
+    if (!s) return "";
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->err.c_str();
+}
+
+void kira_server_destroy(kira_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (auto st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (auto ev : s->events) (void)hipEventDestroy(ev);
+    if (s->dpool) (void)hipFree(s->dpool);
+    if (s->hpool) (void)hipHostFree(s->hpool);
+    delete s;
+}
+
+}  // extern "C"

@@ -112,7 +112,8 @@ __global__ void pack_2bit_planes_kernel(const uint8_t *__restrict__ packed, int6
 
 // One thread per (read, plane word).
 __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ reads,
-                                        int32_t stride, const int32_t *__restrict__ lens,
+                                        int32_t stride, const int64_t *__restrict__ offs,
+                                        const int32_t *__restrict__ lens,
                                         uint32_t *__restrict__ planes,
                                         uint8_t *__restrict__ valid) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,7 +121,7 @@ __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ r
     int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
     int32_t L = lens[r];
     uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
-    const uint8_t *src = reads + (int64_t)r * stride;
+    const uint8_t *src = reads + (offs ? offs[r] : (int64_t)r * stride);
     for (int b = 0; b < 32; b++) {
         int32_t q = k * 32 + b;
         if (q >= L) break;
@@ -429,11 +430,11 @@ hipError_t launch_pack_2bit_planes(const uint8_t *packed, int64_t n_bytes, Plane
 }
 
 hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stride,
-                                   const int32_t *lens, uint32_t *planes, uint8_t *valid,
-                                   hipStream_t s) {
+                                   const int64_t *offs, const int32_t *lens, uint32_t *planes,
+                                   uint8_t *valid, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(n, reads, stride, lens,
-                                                                          planes, valid);
+    pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(n, reads, stride, offs,
+                                                                          lens, planes, valid);
     return hipGetLastError();
 }
 

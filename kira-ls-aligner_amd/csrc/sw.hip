@@ -406,7 +406,9 @@ __global__ __launch_bounds__(64) void banded_sw_kernel(SwLaunch a) {
         p = __shfl(p, 0);
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + lane];
-        if (job >= 0) {
+        if (job >= 0 && sw_job_skipped(a, job)) {
+            sw_write_skipped(a, job);
+        } else if (job >= 0) {
             uint32_t *slab = a.trace + a.packet_trace_off[p];
             int2 *meta = a.row_meta + a.packet_meta_off[p];
             const int2 end = sw_job<C, PACK>(a, job, lane, a.packet_rows[p], slab, meta);

@@ -43,6 +43,14 @@ template <int N>
 __device__ __forceinline__ int32_t from_prev(int32_t v, int32_t fill) {
     return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
 }
+// In-place forms (bound_ctrl: lanes without a source read 0); the caller
+// overwrites the edge lane, so no copy of the source array is needed.
+__device__ __forceinline__ int32_t next_lane0(int32_t v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x101, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int32_t prev_lane0(int32_t v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x111, 0xF, 0xF, true);
+}
 template <int N>
 __device__ __forceinline__ int32_t rotate(int32_t v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xF, 0xF, false);
@@ -63,25 +71,27 @@ struct CoopBand {
             E[p] = padE;
         }
     }
-    // new[q] = old[q + delta] over the whole 16-lane band
-    __device__ __forceinline__ void shift(int32_t delta) {
+    // new[q] = old[q + delta] over the whole 16-lane band (g: lane in row)
+    __device__ __forceinline__ void shift(int32_t delta, int g) {
         if (delta >= 16 * P || delta <= -16 * P) {
             clear();
             return;
         }
-        while (delta >= P) {
+        while (delta >= P) {  // whole lanes: lane g takes lane g + 1's positions
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                H[p] = from_next<1>(H[p], padH);
-                E[p] = from_next<1>(E[p], padE);
+                const int32_t h = next_lane0(H[p]), e = next_lane0(E[p]);
+                H[p] = g == 15 ? padH : h;
+                E[p] = g == 15 ? padE : e;
             }
             delta -= P;
         }
         while (delta <= -P) {
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                H[p] = from_prev<1>(H[p], padH);
-                E[p] = from_prev<1>(E[p], padE);
+                const int32_t h = prev_lane0(H[p]), e = prev_lane0(E[p]);
+                H[p] = g == 0 ? padH : h;
+                E[p] = g == 0 ? padE : e;
             }
             delta += P;
         }
@@ -151,7 +161,7 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
             bd.clear();
         } else {
             int32_t delta = c - prev_c - 1;
-            if (delta != 0) bd.shift(delta);
+            if (delta != 0) bd.shift(delta, g);
         }
         prev_c = c;
         prev_empty = false;
@@ -176,8 +186,12 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         // valid positions of this lane: p in [vlo, vhi]
         const int32_t vlo = js - base - q0, vhi = je - base - q0;
         const int32_t upH = from_next<1>(bd.H[0], bd.padH), upE = from_next<1>(bd.E[0], bd.padE);
-        int32_t hm[P], ev[P], hp[P];
+        // pass 1: diagonal and E terms, Hpre = max(diag + s, E, 0), this
+        // row's E (stored over the previous row's as soon as it is read)
+        // and the lane's F chain with nothing coming in
+        int32_t hm[P], hp[P];
         uint64_t ebits = 0;  // E-extend bit of each position (P may exceed 32)
+        int32_t f = kMin4;
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const int32_t d = bd.H[p];
@@ -187,73 +201,68 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
             const int32_t s = (qb == rb) ? A : -B;
             hm[p] = imax(d, 0) + s;
             const int32_t efh = uh - go, efe = ue - ge;
-            ev[p] = imax(efh, efe);
+            const int32_t e = imax(efh, efe);
             ebits |= (uint64_t)(efe > efh ? 1u : 0u) << p;
             const bool valid = p >= vlo && p <= vhi;
-            hp[p] = valid ? imax(imax(hm[p], ev[p]), 0) : bd.padH;
-        }
-        // F: local chains, then the carry across lanes
-        int32_t fl[P];
-        int32_t f = kMin4;
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            fl[p] = f;
+            hp[p] = valid ? imax(imax(hm[p], e), 0) : bd.padH;
+            bd.E[p] = valid ? e : bd.padE;
             f = imax(hp[p] - go, f - ge);
         }
+        // F at this lane's first position: max-plus scan of the chains of
+        // the lanes before it (each decays by P * ge across a lane)
         const int32_t dP = P * ge;
-        int32_t x = from_prev<1>(f, kMin4);
-        x = imax(x, from_prev<1>(x, kMin4) - dP);
-        x = imax(x, from_prev<2>(x, kMin4) - 2 * dP);
-        x = imax(x, from_prev<4>(x, kMin4) - 4 * dP);
-        x = imax(x, from_prev<8>(x, kMin4) - 8 * dP);
-        // x = F at this lane's first position from earlier lanes
-        int32_t F[P], Hf[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            F[p] = imax(fl[p], x - p * ge);
-            Hf[p] = imax(hp[p], F[p]);
-        }
-        // F-extend bit of position p compares F and H of position p - 1
-        int32_t lastF = from_prev<1>(F[P - 1], kMin4), lastH = from_prev<1>(Hf[P - 1], bd.padH);
-        bool lastValid = (q0 - 1 >= js - base) && (q0 - 1 <= je - base);
-        int32_t rbest = 0, rbq = -1;
-        uint32_t word = 0;
+        int32_t F = from_prev<1>(f, kMin4);
+        F = imax(F, from_prev<1>(F, kMin4) - dP);
+        F = imax(F, from_prev<2>(F, kMin4) - 2 * dP);
+        F = imax(F, from_prev<4>(F, kMin4) - 4 * dP);
+        F = imax(F, from_prev<8>(F, kMin4) - 8 * dP);
+        // pass 2: F along the lane, final H, trace codes, row key.  The
+        // F-extend bit of position p compares F and H of position p - 1;
+        // for p = 0 that is the previous lane's last position, patched into
+        // word 0 after the loop.
+        int32_t key = 0, prevF = kMin4, prevH = bd.padH;
+        bool prevValid = false;
+        uint32_t word = 0, word0 = 0;
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const bool valid = p >= vlo && p <= vhi;
-            const int32_t h = Hf[p];
-            uint32_t dir = h == 0 ? 0u : (h == hm[p] ? 1u : (h == ev[p] ? 2u : 3u));
-            uint32_t fb = (lastValid && (lastF - ge > lastH - go)) ? 8u : 0u;
+            const int32_t h = imax(hp[p], F);
+            const int32_t e = bd.E[p];
+            uint32_t dir = h == 0 ? 0u : (h == hm[p] ? 1u : (h == e ? 2u : 3u));
+            uint32_t fb = (p > 0 && prevValid && (prevF - ge > prevH - go)) ? 8u : 0u;
             uint32_t code = dir | ((uint32_t)((ebits >> p) & 1u) << 2) | fb;
             if (valid) {
-                if (h > rbest) {
-                    rbest = h;
-                    rbq = q0 + p;
-                }
+                key = imax(key, (h << 10) | (1023 - (q0 + p)));
                 bd.H[p] = h;
-                bd.E[p] = ev[p];
             } else {
                 code = 0;
                 bd.H[p] = bd.padH;
-                bd.E[p] = bd.padE;
             }
-            lastF = F[p];
-            lastH = h;
-            lastValid = valid;
+            const int32_t nextF = imax(hp[p] - go, F - ge);
+            prevF = F;
+            prevH = h;
+            prevValid = valid;
+            F = nextF;
             word |= code << (4 * (p & 7));
             if ((p & 7) == 7 || p == P - 1) {
-                trow[(p >> 3) * 64] = word;
+                if (p < 8) word0 = word;
+                else trow[(p >> 3) * 64] = word;
                 word = 0;
             }
         }
+        {
+            const int32_t lf = from_prev<1>(prevF, kMin4), lh = from_prev<1>(prevH, bd.padH);
+            const bool lv = (q0 - 1 >= js - base) && (q0 - 1 <= je - base);
+            if (vlo <= 0 && vhi >= 0 && lv && (lf - ge > lh - go)) word0 |= 8u;
+            trow[0] = word0;
+        }
         // first strict row maximum over the 16 lanes
-        int32_t key = rbest > 0 ? (rbest << 10) | (1023 - rbq) : 0;
         key = imax(key, rotate<8>(key));
         key = imax(key, rotate<4>(key));
         key = imax(key, rotate<2>(key));
         key = imax(key, rotate<1>(key));
         const int32_t rb_all = key >> 10;
-        const int32_t rbj = key > 0 ? base + (1023 - (key & 1023)) : -1;
+        const int32_t rbj = rb_all > 0 ? base + (1023 - (key & 1023)) : -1;
         if (g == 0)
             meta[(int64_t)(i - 1) * 64 + wl] = make_int2(base, (js - base) | ((je - base) << 16));
         if (rb_all > best) {
@@ -408,7 +417,9 @@ __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
         p = __shfl(p, 0);
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + wl];
-        if (job >= 0) {
+        if (job >= 0 && sw_job_skipped(a, job)) {
+            if (g == 0) sw_write_skipped(a, job);
+        } else if (job >= 0) {
             uint32_t *slab = a.trace + a.packet_trace_off[p];
             int2 *meta = a.row_meta + a.packet_meta_off[p];
             const int2 end = coop_job<P>(a, job, g, wl, a.packet_rows[p], slab, meta);

@@ -25,6 +25,7 @@ constexpr int kNumLane = 6;
 constexpr int kCoopP[] = {9, 17, 25, 33};
 constexpr int kNumCoop = 4;
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
+static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (16 * 33 - 1) / 2;  // 263
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
@@ -71,26 +72,215 @@ int32_t sat_add(int32_t a, int32_t b) {
 
 }  // namespace
 
-struct kira_sw_plan {
-    int32_t n = 0;
-    int32_t n_packets = 0;
-    int32_t class_first[kNumClasses] = {};
-    int32_t class_count[kNumClasses] = {};
-    int32_t *packet_jobs = nullptr, *packet_rows = nullptr, *packet_rowdw = nullptr;
-    int64_t *packet_trace_off = nullptr, *packet_meta_off = nullptr;
-    int32_t *job_slot = nullptr, *cigar_cap = nullptr, *counter = nullptr;
-    int64_t *cigar_off = nullptr;
-    uint32_t *trace = nullptr;
-    int2 *meta = nullptr;
-    int64_t cigar_words = 0;
-    int64_t device_bytes = 0;
-    std::vector<int64_t> host_cigar_off;
-    void release() {
-        void *ptrs[] = {packet_jobs, packet_rows, packet_rowdw, packet_trace_off,
-                        packet_meta_off, job_slot, cigar_cap, counter, cigar_off, trace, meta};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
+namespace kira {
+
+const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *band,
+                           const kira_sw_cfg &cfg, SwPlanLayout &L) {
+    L = SwPlanLayout();
+    L.n = n;
+    std::vector<int> cls(n);
+    const int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
+    const bool costs_small = cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
+                             cfg.gap_extend >= 0 && go <= 16000;
+    // 16-lane jobs: the F scan stays inside int32 and the arg-max key
+    // (score << 10 | position) needs score < 2^21
+    const int32_t kCoopCost = 1 << 20;
+    const bool coop_costs = cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
+                            cfg.gap_extend >= 0 && cfg.match_score <= kCoopCost &&
+                            cfg.mismatch <= kCoopCost && go <= kCoopCost;
+    for (int32_t k = 0; k < n; k++) {
+        int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
+        if (read_len[k] < 0) return "negative read length";
+        // every H is at most read_len * match: int16 storage cannot clamp
+        bool pack = costs_small && (int64_t)read_len[k] * cfg.match_score < 32767;
+        bool coop = coop_costs && (int64_t)read_len[k] * cfg.match_score < (1 << 21);
+        if (w > 64 && coop) {
+            cls[k] = coop_class_of(w);
+            if (cls[k] < 0) return "band wider than 263";
+            continue;
+        }
+        int b = lane_class_of(w);
+        if (b < 0)
+            return w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
+                                 : "band wider than 263";
+        cls[k] = 2 * b + (pack ? 1 : 0);
     }
+    // packets: jobs sorted by (band class, read length descending)
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (cls[x] != cls[y]) return cls[x] < cls[y];
+        return read_len[x] > read_len[y];
+    });
+    L.job_slot.assign(n, 0);
+    for (int c = 0; c < kNumClasses; c++) L.class_first[c] = -1;
+    for (int32_t s = 0; s < n;) {
+        int c = cls[order[s]];
+        const int per = class_jobs_per_packet(c), stride = 64 / per;
+        int32_t e = s;
+        while (e < n && e - s < per && cls[order[e]] == c) e++;
+        int32_t pk = (int32_t)L.packet_rows.size();
+        if (L.class_first[c] < 0) L.class_first[c] = pk;
+        L.class_count[c]++;
+        int32_t rows = read_len[order[s]];
+        int32_t rowdw = class_rowdw(c);
+        for (int32_t l = 0; l < 64; l++) {
+            int32_t job = (l % stride == 0 && s + l / stride < e) ? order[s + l / stride] : -1;
+            L.packet_jobs.push_back(job);
+            if (job >= 0) L.job_slot[job] = pk * 64 + l;
+        }
+        L.packet_rows.push_back(rows);
+        L.packet_rowdw.push_back(rowdw);
+        L.packet_trace_off.push_back(L.trace_words);
+        L.packet_meta_off.push_back(L.meta_rows);
+        L.trace_words += (int64_t)rows * rowdw * 64;
+        L.meta_rows += (int64_t)rows * 64;
+        s = e;
+    }
+    L.n_packets = (int32_t)L.packet_rows.size();
+    L.cigar_cap.resize(n);
+    L.cigar_off.resize(n);
+    for (int32_t k = 0; k < n; k++) {
+        L.cigar_cap[k] = 3 * read_len[k] + 2;
+        L.cigar_off[k] = L.cigar_words;
+        L.cigar_words += L.cigar_cap[k];
+    }
+    return nullptr;
+}
+
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+// table sections in order: packet_jobs, packet_rows, packet_rowdw,
+// packet_trace_off, packet_meta_off, job_slot, cigar_cap, cigar_off
+struct TableSections {
+    size_t off[8], bytes[8], total;
+};
+TableSections table_sections(const SwPlanLayout &L) {
+    TableSections t{};
+    size_t b[8] = {L.packet_jobs.size() * 4,      L.packet_rows.size() * 4,
+                   L.packet_rowdw.size() * 4,     L.packet_trace_off.size() * 8,
+                   L.packet_meta_off.size() * 8,  L.job_slot.size() * 4,
+                   L.cigar_cap.size() * 4,        L.cigar_off.size() * 8};
+    size_t o = 0;
+    for (int k = 0; k < 8; k++) {
+        t.off[k] = o;
+        t.bytes[k] = b[k];
+        o += align256(std::max<size_t>(b[k], 1));
+    }
+    t.total = o;
+    return t;
+}
+}  // namespace
+
+size_t sw_plan_table_bytes(const SwPlanLayout &L) { return table_sections(L).total; }
+
+void sw_plan_pack_tables(const SwPlanLayout &L, uint8_t *dst) {
+    TableSections t = table_sections(L);
+    const void *src[8] = {L.packet_jobs.data(),      L.packet_rows.data(),
+                          L.packet_rowdw.data(),     L.packet_trace_off.data(),
+                          L.packet_meta_off.data(),  L.job_slot.data(),
+                          L.cigar_cap.data(),        L.cigar_off.data()};
+    for (int k = 0; k < 8; k++)
+        if (t.bytes[k]) memcpy(dst + t.off[k], src[k], t.bytes[k]);
+}
+
+size_t sw_plan_slab_bytes(const SwPlanLayout &L) {
+    return align256((size_t)std::max<int64_t>(L.trace_words, 1) * 4) +
+           align256((size_t)std::max<int64_t>(L.meta_rows, 1) * sizeof(int2)) +
+           align256(kNumClasses * 4);
+}
+
+SwPlanDev sw_plan_bind(const SwPlanLayout &L, const uint8_t *tab, uint8_t *slabs) {
+    TableSections t = table_sections(L);
+    SwPlanDev d{};
+    d.packet_jobs = (const int32_t *)(tab + t.off[0]);
+    d.packet_rows = (const int32_t *)(tab + t.off[1]);
+    d.packet_rowdw = (const int32_t *)(tab + t.off[2]);
+    d.packet_trace_off = (const int64_t *)(tab + t.off[3]);
+    d.packet_meta_off = (const int64_t *)(tab + t.off[4]);
+    d.job_slot = (const int32_t *)(tab + t.off[5]);
+    d.cigar_cap = (const int32_t *)(tab + t.off[6]);
+    d.cigar_off = (const int64_t *)(tab + t.off[7]);
+    size_t tb = align256((size_t)std::max<int64_t>(L.trace_words, 1) * 4);
+    size_t mb = align256((size_t)std::max<int64_t>(L.meta_rows, 1) * sizeof(int2));
+    d.trace = (uint32_t *)slabs;
+    d.meta = (int2 *)(slabs + tb);
+    d.counter = (int32_t *)(slabs + tb + mb);
+    return d;
+}
+
+const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira_sw_jobs_dev &j,
+                           const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
+                           const uint8_t *skip_flags, const int32_t *skip_src, hipStream_t s,
+                           hipError_t *err) {
+    *err = hipSuccess;
+    if (L.n == 0) return nullptr;
+    int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
+    // pads are INT32_MIN/4 + cost and must stay negative
+    if (go >= (1 << 29) || cfg.gap_extend >= (1 << 29) || go < 0 || cfg.gap_extend < 0)
+        return "gap costs outside the supported range";
+    SwLaunch a{};
+    a.match_score = cfg.match_score;
+    a.mismatch = cfg.mismatch;
+    a.go = go;
+    a.gap_extend = cfg.gap_extend;
+    a.clip_penalty = cfg.clip_penalty;
+    a.bandwidth = cfg.bandwidth;
+    a.adaptive = kn.adaptive ? 1 : 0;
+    a.max_drift = kn.max_drift;
+    a.dp_xdrop = kn.dp_xdrop >= 0 ? kn.dp_xdrop : cfg.xdrop;
+    a.reads = j.reads;
+    a.read_off = j.read_off;
+    a.read_len = j.read_len;
+    a.refs = j.refs;
+    a.ref_off = j.ref_off;
+    a.ref_len = j.ref_len;
+    a.offset = j.offset;
+    a.band = j.band;
+    a.abort_score = j.abort_score;
+    a.trace = d.trace;
+    a.row_meta = d.meta;
+    a.res = j.res;
+    a.job_slot = d.job_slot;
+    a.cigar = j.cigar;
+    a.cigar_off = d.cigar_off;
+    a.cigar_cap = d.cigar_cap;
+    a.skip_flags = skip_flags;
+    a.skip_src = skip_flags ? skip_src : nullptr;
+    for (int c = 0; c < kNumClasses; c++) {
+        if (L.class_count[c] == 0) continue;
+        SwLaunch ac = a;
+        int32_t f = L.class_first[c];
+        ac.packet_jobs = d.packet_jobs + (int64_t)f * 64;
+        ac.packet_rows = d.packet_rows + f;
+        ac.packet_rowdw = d.packet_rowdw + f;
+        ac.packet_trace_off = d.packet_trace_off + f;
+        ac.packet_meta_off = d.packet_meta_off + f;
+        ac.n_packets = L.class_count[c];
+        ac.counter = d.counter + c;
+        hipError_t e;
+        if (class_is_coop(c)) {
+            int waves = std::min(L.class_count[c], 256 * 20);
+            e = launch_banded_sw_coop(ac, class_coop_p(c), waves, s);
+        } else {
+            int waves = std::min(L.class_count[c], 256 * 16);
+            e = launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, s);
+        }
+        if (e != hipSuccess) {
+            *err = e;
+            return "banded SW launch";
+        }
+    }
+    return nullptr;
+}
+
+}  // namespace kira
+
+struct kira_sw_plan {
+    kira::SwPlanLayout L;
+    kira::SwPlanDev d{};
+    void *mem = nullptr;
+    int64_t device_bytes = 0;
 };
 
 extern "C" {
@@ -101,125 +291,42 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     if (!ctx || !out || !cfg || n < 0 || (n > 0 && !read_len))
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
     *out = nullptr;
-    std::vector<int> cls(n);
-    const int32_t go = sat_add(cfg->gap_open, cfg->gap_extend);
-    const bool costs_small = cfg->match_score >= 0 && cfg->mismatch >= 0 && cfg->gap_open >= 0 &&
-                             cfg->gap_extend >= 0 && go <= 16000;
-    // 16-lane jobs: the F scan stays inside int32 and the arg-max key
-    // (score << 10 | position) needs score < 2^21
-    const int32_t kCoopCost = 1 << 20;
-    const bool coop_costs = cfg->match_score >= 0 && cfg->mismatch >= 0 && cfg->gap_open >= 0 &&
-                            cfg->gap_extend >= 0 && cfg->match_score <= kCoopCost &&
-                            cfg->mismatch <= kCoopCost && go <= kCoopCost;
-    for (int32_t k = 0; k < n; k++) {
-        int32_t w = std::max(band ? band[k] : cfg->bandwidth, 1);
-        if (read_len[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "negative read length");
-        // every H is at most read_len * match: int16 storage cannot clamp
-        bool pack = costs_small && (int64_t)read_len[k] * cfg->match_score < 32767;
-        bool coop = coop_costs && (int64_t)read_len[k] * cfg->match_score < (1 << 21);
-        if (w > 64 && coop) {
-            cls[k] = coop_class_of(w);
-            if (cls[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "band wider than 263");
-            continue;
-        }
-        int b = lane_class_of(w);
-        if (b < 0)
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT,
-                        w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
-                                      : "band wider than 263");
-        cls[k] = 2 * b + (pack ? 1 : 0);
-    }
-    // packets: jobs sorted by (band class, read length descending)
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (cls[x] != cls[y]) return cls[x] < cls[y];
-        return read_len[x] > read_len[y];
-    });
     kira_sw_plan *p = new (std::nothrow) kira_sw_plan();
     if (!p) return KIRA_ERR_OUT_OF_MEMORY;
-    p->n = n;
-    std::vector<int32_t> pj, prows, prowdw, slot(n);
-    std::vector<int64_t> ptoff, pmoff;
-    int64_t toff = 0, moff = 0;
-    for (int c = 0; c < kNumClasses; c++) p->class_first[c] = -1;
-    for (int32_t s = 0; s < n;) {
-        int c = cls[order[s]];
-        const int per = class_jobs_per_packet(c), stride = 64 / per;
-        int32_t e = s;
-        while (e < n && e - s < per && cls[order[e]] == c) e++;
-        int32_t pk = (int32_t)prows.size();
-        if (p->class_first[c] < 0) p->class_first[c] = pk;
-        p->class_count[c]++;
-        int32_t rows = read_len[order[s]];
-        int32_t rowdw = class_rowdw(c);
-        for (int32_t l = 0; l < 64; l++) {
-            int32_t job = (l % stride == 0 && s + l / stride < e) ? order[s + l / stride] : -1;
-            pj.push_back(job);
-            if (job >= 0) slot[job] = pk * 64 + l;
-        }
-        prows.push_back(rows);
-        prowdw.push_back(rowdw);
-        ptoff.push_back(toff);
-        pmoff.push_back(moff);
-        toff += (int64_t)rows * rowdw * 64;
-        moff += (int64_t)rows * 64;
-        s = e;
+    if (const char *why = kira::sw_plan_layout(n, read_len, band, *cfg, p->L)) {
+        delete p;
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, why);
     }
-    p->n_packets = (int32_t)prows.size();
-    std::vector<int64_t> coff(n);
-    std::vector<int32_t> ccap(n);
-    int64_t acc = 0;
-    for (int32_t k = 0; k < n; k++) {
-        ccap[k] = 3 * read_len[k] + 2;
-        coff[k] = acc;
-        acc += ccap[k];
-    }
-    p->cigar_words = acc;
-    p->device_bytes = toff * 4 + moff * (int64_t)sizeof(int2) +
-                      (int64_t)std::max(p->n_packets, 1) * (64 * 4 + 4 * 2 + 8 * 2) +
-                      (int64_t)n * (4 + 4 + 8) + kNumClasses * 4;
-    p->host_cigar_off = coff;
+    const size_t tb = kira::sw_plan_table_bytes(p->L), sb = kira::sw_plan_slab_bytes(p->L);
+    p->device_bytes = (int64_t)(tb + sb);
+    std::vector<uint8_t> host(tb);
+    kira::sw_plan_pack_tables(p->L, host.data());
     hipError_t e = hipSetDevice(ctx->device);
-    int32_t P = std::max(p->n_packets, 1);
-    if (e || (e = dev_alloc(&p->packet_jobs, (size_t)P * 64)) ||
-        (e = dev_alloc(&p->packet_rows, P)) || (e = dev_alloc(&p->packet_rowdw, P)) ||
-        (e = dev_alloc(&p->packet_trace_off, P)) || (e = dev_alloc(&p->packet_meta_off, P)) ||
-        (e = dev_alloc(&p->job_slot, n)) || (e = dev_alloc(&p->cigar_cap, n)) ||
-        (e = dev_alloc(&p->cigar_off, n)) || (e = dev_alloc(&p->counter, kNumClasses)) ||
-        (e = dev_alloc(&p->trace, (size_t)toff)) || (e = dev_alloc(&p->meta, (size_t)moff))) {
-        p->release();
+    if (e == hipSuccess) e = hipMalloc(&p->mem, tb + sb);
+    if (e != hipSuccess) {
         delete p;
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "plan allocation", e);
     }
-    hipStream_t s = ctx->stream;
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
-    };
-    if ((e = up(p->packet_jobs, pj.data(), pj.size() * 4)) ||
-        (e = up(p->packet_rows, prows.data(), prows.size() * 4)) ||
-        (e = up(p->packet_rowdw, prowdw.data(), prowdw.size() * 4)) ||
-        (e = up(p->packet_trace_off, ptoff.data(), ptoff.size() * 8)) ||
-        (e = up(p->packet_meta_off, pmoff.data(), pmoff.size() * 8)) ||
-        (e = up(p->job_slot, slot.data(), (size_t)n * 4)) ||
-        (e = up(p->cigar_cap, ccap.data(), (size_t)n * 4)) ||
-        (e = up(p->cigar_off, coff.data(), (size_t)n * 8)) || (e = hipStreamSynchronize(s))) {
-        p->release();
+    uint8_t *base = (uint8_t *)p->mem;
+    if ((e = hipMemcpyAsync(base, host.data(), tb, hipMemcpyHostToDevice, ctx->stream)) ||
+        (e = hipStreamSynchronize(ctx->stream))) {
+        (void)hipFree(p->mem);
         delete p;
         return fail(ctx, KIRA_ERR_RUNTIME, "plan upload", e);
     }
+    p->d = kira::sw_plan_bind(p->L, base, base + tb);
     *out = p;
     return KIRA_OK;
 }
 
 void kira_banded_sw_plan_destroy(kira_sw_plan *plan) {
     if (!plan) return;
-    plan->release();
+    if (plan->mem) (void)hipFree(plan->mem);
     delete plan;
 }
 
 int64_t kira_banded_sw_plan_cigar_words(const kira_sw_plan *plan) {
-    return plan ? plan->cigar_words : 0;
+    return plan ? plan->L.cigar_words : 0;
 }
 
 int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan) {
@@ -227,7 +334,7 @@ int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan) {
 }
 
 const int64_t *kira_banded_sw_plan_cigar_off(const kira_sw_plan *plan) {
-    return plan ? plan->host_cigar_off.data() : nullptr;
+    return plan ? plan->L.cigar_off.data() : nullptr;
 }
 
 kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
@@ -235,58 +342,10 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
                                    const kira_sw_knobs *kn) {
     if (!ctx || !plan || !j || !cfg || !kn)
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
-    if (plan->n == 0) return KIRA_OK;
-    int32_t go = sat_add(cfg->gap_open, cfg->gap_extend);
-    // pads are INT32_MIN/4 + cost and must stay negative
-    if (go >= (1 << 29) || cfg->gap_extend >= (1 << 29) || go < 0 || cfg->gap_extend < 0)
-        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "gap costs outside the supported range");
-    kira::SwLaunch a{};
-    a.match_score = cfg->match_score;
-    a.mismatch = cfg->mismatch;
-    a.go = go;
-    a.gap_extend = cfg->gap_extend;
-    a.clip_penalty = cfg->clip_penalty;
-    a.bandwidth = cfg->bandwidth;
-    a.adaptive = kn->adaptive ? 1 : 0;
-    a.max_drift = kn->max_drift;
-    a.dp_xdrop = kn->dp_xdrop >= 0 ? kn->dp_xdrop : cfg->xdrop;
-    a.reads = j->reads;
-    a.read_off = j->read_off;
-    a.read_len = j->read_len;
-    a.refs = j->refs;
-    a.ref_off = j->ref_off;
-    a.ref_len = j->ref_len;
-    a.offset = j->offset;
-    a.band = j->band;
-    a.abort_score = j->abort_score;
-    a.trace = plan->trace;
-    a.row_meta = plan->meta;
-    a.res = j->res;
-    a.job_slot = plan->job_slot;
-    a.cigar = j->cigar;
-    a.cigar_off = plan->cigar_off;
-    a.cigar_cap = plan->cigar_cap;
-    hipError_t e = hipSuccess;
-    for (int c = 0; c < kNumClasses; c++) {
-        if (plan->class_count[c] == 0) continue;
-        kira::SwLaunch ac = a;
-        int32_t f = plan->class_first[c];
-        ac.packet_jobs = plan->packet_jobs + (int64_t)f * 64;
-        ac.packet_rows = plan->packet_rows + f;
-        ac.packet_rowdw = plan->packet_rowdw + f;
-        ac.packet_trace_off = plan->packet_trace_off + f;
-        ac.packet_meta_off = plan->packet_meta_off + f;
-        ac.n_packets = plan->class_count[c];
-        ac.counter = plan->counter + c;
-        if (class_is_coop(c)) {
-            int waves = std::min(plan->class_count[c], 256 * 20);
-            e = kira::launch_banded_sw_coop(ac, class_coop_p(c), waves, ctx->stream);
-        } else {
-            int waves = std::min(plan->class_count[c], 256 * 16);
-            e = kira::launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, ctx->stream);
-        }
-        if (e) return fail(ctx, KIRA_ERR_RUNTIME, "banded SW launch", e);
-    }
+    hipError_t e;
+    const char *why = kira::sw_plan_launch(plan->L, plan->d, *j, *cfg, *kn, nullptr, nullptr,
+                                           ctx->stream, &e);
+    if (why) return fail(ctx, e ? KIRA_ERR_RUNTIME : KIRA_ERR_INVALID_ARGUMENT, why, e);
     return KIRA_OK;
 }
 
@@ -336,7 +395,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
         (g.d.push_back(d_flen), e = dev_alloc(&d_off, n)) ||
         (g.d.push_back(d_off), e = dev_alloc(&d_abort, n)) ||
         (g.d.push_back(d_abort), e = dev_alloc(&d_res, (size_t)n * kira::kSwResWords)) ||
-        (g.d.push_back(d_res), e = dev_alloc(&d_cig, (size_t)plan->cigar_words)) ||
+        (g.d.push_back(d_res), e = dev_alloc(&d_cig, (size_t)plan->L.cigar_words)) ||
         (g.d.push_back(d_cig), band && (e = dev_alloc(&d_band, n))))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "batch allocation", e);
     if (d_band) g.d.push_back(d_band);
@@ -357,7 +416,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
                         d_band, d_abort, d_res, d_cig};
     if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, cfg, kn)) != KIRA_OK) return st;
     std::vector<int32_t> res((size_t)n * kira::kSwResWords);
-    std::vector<uint32_t> cig((size_t)plan->cigar_words);
+    std::vector<uint32_t> cig((size_t)plan->L.cigar_words);
     if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
         (e = hipMemcpyAsync(cig.data(), d_cig, cig.size() * 4, hipMemcpyDeviceToHost, s)) ||
         (e = hipStreamSynchronize(s)))
@@ -380,7 +439,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "cigar arena too small");
         o.cigar_off = pos;
         if (r[6] > 0)
-            memcpy(cigar_arena + pos, &cig[(size_t)plan->host_cigar_off[k]], (size_t)r[6] * 4);
+            memcpy(cigar_arena + pos, &cig[(size_t)plan->L.cigar_off[k]], (size_t)r[6] * 4);
         pos += r[6];
     }
     return KIRA_OK;
