@@ -146,6 +146,48 @@ __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ r
     }
 }
 
+// Same for row-major reads with a stride that is a multiple of 32 and a
+// 16-byte aligned base: each thread fetches its 32 bytes with two 16-byte
+// loads (a 32-byte chunk that starts below the read length lies inside the
+// row, since both are multiples of 32), instead of 32 byte loads.
+__global__ void pack_reads_ascii_vec_kernel(int32_t n, const uint8_t *__restrict__ reads,
+                                            int32_t stride, const int32_t *__restrict__ lens,
+                                            uint32_t *__restrict__ planes,
+                                            uint8_t *__restrict__ valid) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * 8) return;
+    int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
+    int32_t L = lens[r];
+    uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
+    if (k * 32 < L) {
+        const uint4 *src = (const uint4 *)(reads + (int64_t)r * stride + k * 32);
+        const uint4 v0 = src[0], v1 = src[1];
+        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        const int nb = L - k * 32 < 32 ? L - k * 32 : 32;
+#pragma unroll
+        for (int b = 0; b < 32; b++) {
+            if (b >= nb) break;
+            uint32_t a0, b0, c0, d0;
+            classify((uint8_t)(w[b >> 2] >> (8 * (b & 3))), a0, b0, c0, d0);
+            lo |= a0 << b;
+            hi |= b0 << b;
+            inv |= c0 << b;
+            lc |= d0 << b;
+        }
+    }
+    uint32_t *dst = planes + (int64_t)r * kReadWords;
+    dst[k] = lo;
+    dst[8 + k] = hi;
+    dst[16 + k] = lc;
+    if (valid) {
+        uint32_t f = inv != 0 ? 1u : 0u;
+        f |= __shfl_xor(f, 1, 8);
+        f |= __shfl_xor(f, 2, 8);
+        f |= __shfl_xor(f, 4, 8);
+        if (k == 0) valid[r] = (L <= kMaxReadBases && f == 0) ? 1 : 0;
+    }
+}
+
 // Reads already 2-bit packed (reference layout): one thread per (read, word).
 __global__ void pack_reads_2bit_kernel(int32_t n, const uint8_t *__restrict__ packed,
                                        int32_t stride, const int32_t *__restrict__ lens,
@@ -433,8 +475,12 @@ hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stri
                                    const int64_t *offs, const int32_t *lens, uint32_t *planes,
                                    uint8_t *valid, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(n, reads, stride, offs,
-                                                                          lens, planes, valid);
+    if (!offs && stride % 32 == 0 && ((uintptr_t)reads & 15) == 0)
+        pack_reads_ascii_vec_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(
+            n, reads, stride, lens, planes, valid);
+    else
+        pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(
+            n, reads, stride, offs, lens, planes, valid);
     return hipGetLastError();
 }
 
