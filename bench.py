@@ -315,6 +315,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
     ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--calib-copy-gb", type=float, default=0.0,
+                    help="after the timed region, run one plain device copy of this many "
+                         "GiB (the reference kernel for scaling PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; --gpus must be 1 "
@@ -385,6 +388,16 @@ def main():
     print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, sieve kernel "
           f"{kernel_ms:.3f} ms", file=sys.stderr, flush=True)
 
+    if args.calib_copy_gb > 0:
+        nbytes = int(args.calib_copy_gb * (1 << 30))
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {nbytes} bytes read + written", file=sys.stderr,
+              flush=True)
+        del src, dst
     flags = out.flags.cpu().numpy()
     shifts = out.shift.cpu().numpy()
     best = out.best.cpu().numpy()
