@@ -325,6 +325,41 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
                                    const kira_sw_jobs_dev *jobs, const kira_sw_cfg *cfg,
                                    const kira_sw_knobs *knobs);
 
+/* ------------------------------------------------------------------ records */
+
+/* One alignment record of the path (the fields of Alignment, src/types.rs,
+ * that build_alignment fills, src/alignment/mod.rs:1200-1254): CIGAR with
+ * soft clips at cigar_out[cigar_off .. +n_cigar], MD text (no NUL) at
+ * md_out[md_off .. +md_len]. */
+typedef struct {
+    uint32_t ref_start, ref_end, read_start, read_end;
+    int32_t score;
+    uint32_t nm;
+    int32_t n_cigar, md_len;
+    int64_t cigar_off, md_off;
+} kira_alignment;
+
+/*
+ * build_alignment for n SW results (host code, no device work): job k's
+ * read is reads[read_off[k] .. +read_len[k]], its window
+ * windows[win_off[k] .. +win_len[k]] starting at reference position
+ * win_start[k]; sw[k] and its CIGAR ops sw_cigar[sw[k].cigar_off ..] as
+ * returned by the SW entry points.  NM / MD follow md.rs:25-204 (MD keeps
+ * reference letters upper-cased, other bytes as N; an identical non-ACGT
+ * base counts in NM).  For a certified sieve hit pass read_start 0,
+ * read_end L, ref_start = shift and the CIGAR {L}M (mod.rs:636-686).
+ * cigar_used / md_used receive the totals needed; with cigar_out / md_out
+ * NULL only the sizes are computed; too small a buffer returns
+ * KIRA_ERR_INVALID_ARGUMENT.
+ */
+kira_status kira_build_alignments(int32_t n, const uint8_t *reads, const int64_t *read_off,
+                                  const int32_t *read_len, const uint8_t *windows,
+                                  const int64_t *win_off, const int32_t *win_len,
+                                  const uint32_t *win_start, const kira_sw_result *sw,
+                                  const uint32_t *sw_cigar, kira_alignment *out,
+                                  uint32_t *cigar_out, int64_t cigar_cap, char *md_out,
+                                  int64_t md_cap, int64_t *cigar_used, int64_t *md_used);
+
 /* ------------------------------------------------------------------ serving */
 
 /*

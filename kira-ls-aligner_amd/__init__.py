@@ -114,6 +114,15 @@ class PrefilterOut(C.Structure):
                 ("cigar", C.c_uint32 * 3)]
 
 
+class AlignmentRec(C.Structure):
+    """kira_alignment: one record of build_alignment (src/alignment/mod.rs:1200-1254)."""
+
+    _fields_ = [("ref_start", C.c_uint32), ("ref_end", C.c_uint32), ("read_start", C.c_uint32),
+                ("read_end", C.c_uint32), ("score", C.c_int32), ("nm", C.c_uint32),
+                ("n_cigar", C.c_int32), ("md_len", C.c_int32), ("cigar_off", C.c_int64),
+                ("md_off", C.c_int64)]
+
+
 class ServerConfig(C.Structure):
     """kira_server_config (include/kira_hip.h)."""
 
@@ -193,6 +202,10 @@ SIGNATURES = {
     "kira_prefilter_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P,
                                        _P, C.POINTER(Span), _P, _P, C.POINTER(SwCfg),
                                        C.POINTER(PrefilterParams), C.POINTER(PrefilterOut)]),
+    "kira_build_alignments": (C.c_int, [C.c_int32, _P, _P, _P, _P, _P, _P, _P,
+                                        C.POINTER(SwResult), _P, C.POINTER(AlignmentRec), _P,
+                                        C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64)]),
     "kira_server_create": (C.c_int, [_P, _P, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "kira_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "kira_server_wait": (C.c_int, [_P, C.c_int64, C.POINTER(Response)]),
@@ -853,3 +866,58 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ records
+CIGAR_OPS = "MIDNS"
+
+
+def cigar_string(ops) -> str:
+    """CigarOp Display order (src/types.rs:184-197)."""
+    return "".join(f"{o >> 4}{CIGAR_OPS[o & 15]}" for o in ops)
+
+
+def build_alignments(jobs, sw_results):
+    """build_alignment (src/alignment/mod.rs:1200-1254) for SW results.
+
+    jobs: sequence of (read: bytes, window: bytes, win_start: int); sw_results:
+    the dicts returned by banded_sw_batch / align_batch_simd.  Returns dicts
+    {ref_start, ref_end, read_start, read_end, score, nm, md, cigar (text)}."""
+    n = len(jobs)
+    if n == 0:
+        return []
+    reads = b"".join(j[0] for j in jobs)
+    wins = b"".join(j[1] for j in jobs)
+    rl = np.array([len(j[0]) for j in jobs], np.int32)
+    wl = np.array([len(j[1]) for j in jobs], np.int32)
+    ro = np.concatenate([[0], np.cumsum(rl[:-1], dtype=np.int64)]).astype(np.int64)
+    wo = np.concatenate([[0], np.cumsum(wl[:-1], dtype=np.int64)]).astype(np.int64)
+    ws = np.array([j[2] for j in jobs], np.uint32)
+    sw = (SwResult * n)()
+    ops = []
+    for k, r in enumerate(sw_results):
+        sw[k] = SwResult(r["ref_start"], r["ref_end"], r["read_start"], r["read_end"],
+                         r["score"], int(r["early_abort"]), len(r["cigar"]), 0, len(ops))
+        ops.extend(r["cigar"])
+    opa = np.array(ops if ops else [0], np.uint32)
+    rb = np.frombuffer(reads, np.uint8) if reads else np.zeros(1, np.uint8)
+    wb = np.frombuffer(wins, np.uint8) if wins else np.zeros(1, np.uint8)
+    cu, mu = C.c_int64(), C.c_int64()
+    args = [n, _np_ptr(rb), _np_ptr(ro), _np_ptr(rl), _np_ptr(wb), _np_ptr(wo), _np_ptr(wl),
+            _np_ptr(ws), sw, _np_ptr(opa)]
+    st = lib().kira_build_alignments(*args, None, None, 0, None, 0, C.byref(cu), C.byref(mu))
+    if st != KIRA_OK:
+        raise HipError(_STATUS.get(st, "InvalidArgument"), "build_alignments")
+    out = (AlignmentRec * n)()
+    cig = np.zeros(max(cu.value, 1), np.uint32)
+    md = C.create_string_buffer(max(mu.value, 1))
+    st = lib().kira_build_alignments(*args, out, _np_ptr(cig), cu.value, md, mu.value,
+                                     C.byref(cu), C.byref(mu))
+    if st != KIRA_OK:
+        raise HipError(_STATUS.get(st, "InvalidArgument"), "build_alignments")
+    raw = md.raw
+    return [dict(ref_start=a.ref_start, ref_end=a.ref_end, read_start=a.read_start,
+                 read_end=a.read_end, score=a.score, nm=a.nm,
+                 md=raw[a.md_off:a.md_off + a.md_len].decode(),
+                 cigar=cigar_string(cig[a.cigar_off:a.cigar_off + a.n_cigar]))
+            for a in out]

@@ -1,0 +1,138 @@
+// Alignment records of the path, host side: build_alignment
+// (src/alignment/mod.rs:1200-1254) turns a SwResult into the record's CIGAR
+// (soft clips added), reference span (window start added) and its NM / MD
+// tags, replaying the CIGAR over the read and window with the MdBuilder
+// rules of src/alignment/md.rs:25-204.  Host code only (no device work).
+#include <string.h>
+
+#include <string>
+
+#include "internal.hpp"
+
+namespace {
+
+constexpr uint32_t kOpM = 0, kOpI = 1, kOpD = 2, kOpN = 3, kOpS = 4;
+
+uint8_t upper(uint8_t b) { return (b >= 'a' && b <= 'z') ? (uint8_t)(b - 32) : b; }
+
+// MD text and edit distance of one alignment (md.rs:25-132).
+struct MdText {
+    std::string text;
+    uint32_t run = 0, nm = 0;
+    void flush() {
+        text += std::to_string(run);
+        run = 0;
+    }
+    // letters keep their (upper-cased) identity, other bytes become N
+    static char letter(uint8_t b) {
+        uint8_t u = upper(b);
+        return (u >= 'A' && u <= 'Z') ? (char)u : 'N';
+    }
+    void column(uint8_t q, uint8_t r) {
+        if (q == '=' || upper(q) == upper(r)) {
+            run++;
+            // an identical ambiguity code still counts as an edit
+            uint8_t u = upper(q);
+            if (!(u == 'A' || u == 'C' || u == 'G' || u == 'T' || u == '=')) nm++;
+        } else {
+            flush();
+            text += letter(r);
+            nm++;
+        }
+    }
+    void deletion(const uint8_t *ref, size_t n) {
+        if (n == 0) return;
+        flush();
+        text += '^';
+        for (size_t k = 0; k < n; k++) text += letter(ref[k]);
+        nm += (uint32_t)n;
+    }
+};
+
+// nm_md_from_cigar (md.rs:162-204); positions past either sequence read as N.
+void replay(const uint8_t *read, size_t read_len, const uint8_t *ref, size_t ref_len,
+            size_t ref_pos, const uint32_t *cig, int32_t n, MdText &md) {
+    size_t q = 0, r = ref_pos;
+    for (int32_t k = 0; k < n; k++) {
+        const size_t len = cig[k] >> 4;
+        switch (cig[k] & 15u) {
+        case kOpM:
+            for (size_t t = 0; t < len; t++, q++, r++)
+                md.column(q < read_len ? read[q] : (uint8_t)'N', r < ref_len ? ref[r] : (uint8_t)'N');
+            break;
+        case kOpI:
+            md.nm += (uint32_t)len;
+            q += len;
+            break;
+        case kOpD: {
+            const size_t end = std::min(r + len, ref_len), start = std::min(r, end);
+            md.deletion(ref + start, end - start);
+            md.nm += (uint32_t)(len - (end - start));  // bases past the window end
+            r += len;
+            break;
+        }
+        case kOpS: q += len; break;
+        case kOpN: r += len; break;
+        default: break;
+        }
+    }
+    md.flush();
+}
+
+}  // namespace
+
+extern "C" kira_status kira_build_alignments(
+    int32_t n, const uint8_t *reads, const int64_t *read_off, const int32_t *read_len,
+    const uint8_t *windows, const int64_t *win_off, const int32_t *win_len,
+    const uint32_t *win_start, const kira_sw_result *sw, const uint32_t *sw_cigar,
+    kira_alignment *out, uint32_t *cigar_out, int64_t cigar_cap, char *md_out, int64_t md_cap,
+    int64_t *cigar_used, int64_t *md_used) {
+    if (n < 0 || (n > 0 && (!reads || !read_off || !read_len || !windows || !win_off ||
+                            !win_len || !win_start || !sw || !sw_cigar)))
+        return KIRA_ERR_INVALID_ARGUMENT;
+    int64_t cpos = 0, mpos = 0;
+    bool fits = true;
+    std::vector<uint32_t> cig;
+    for (int32_t k = 0; k < n; k++) {
+        const kira_sw_result &s = sw[k];
+        const int32_t L = read_len[k];
+        if (s.n_cigar < 0 || s.read_start < 0 || s.read_end > L || s.read_start > s.read_end)
+            return KIRA_ERR_INVALID_ARGUMENT;
+        cig.clear();
+        if (s.read_start > 0) cig.push_back(((uint32_t)s.read_start << 4) | kOpS);
+        cig.insert(cig.end(), sw_cigar + s.cigar_off, sw_cigar + s.cigar_off + s.n_cigar);
+        if (s.read_end < L) cig.push_back(((uint32_t)(L - s.read_end) << 4) | kOpS);
+        MdText md;
+        // the CIGAR carries the leading clip, so the replay starts at read
+        // position 0 (mod.rs:1231-1233)
+        replay(reads + read_off[k], (size_t)L, windows + win_off[k], (size_t)win_len[k],
+               s.ref_start, cig.data(), (int32_t)cig.size(), md);
+        const int64_t nc = (int64_t)cig.size(), nmd = (int64_t)md.text.size();
+        if (out) {
+            kira_alignment &a = out[k];
+            a.ref_start = win_start[k] + s.ref_start;
+            a.ref_end = win_start[k] + s.ref_end;
+            a.read_start = (uint32_t)s.read_start;
+            a.read_end = (uint32_t)s.read_end;
+            a.score = s.score;
+            a.nm = md.nm;
+            a.n_cigar = (int32_t)nc;
+            a.md_len = (int32_t)nmd;
+            a.cigar_off = cpos;
+            a.md_off = mpos;
+        }
+        if (cigar_out && cpos + nc <= cigar_cap)
+            memcpy(cigar_out + cpos, cig.data(), (size_t)nc * 4);
+        else if (cigar_out)
+            fits = false;
+        if (md_out && mpos + nmd <= md_cap)
+            memcpy(md_out + mpos, md.text.data(), (size_t)nmd);
+        else if (md_out)
+            fits = false;
+        cpos += nc;
+        mpos += nmd;
+    }
+    if (cigar_used) *cigar_used = cpos;
+    if (md_used) *md_used = mpos;
+    return fits ? KIRA_OK : KIRA_ERR_INVALID_ARGUMENT;
+}
