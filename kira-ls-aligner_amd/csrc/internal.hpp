@@ -18,10 +18,14 @@ namespace kira {
 constexpr int kPadFront = 2;
 constexpr int kPadBack = 16;
 
+// 2-bit sequence as interleaved bit planes: word w (32 bases) is q[w] =
+// (low code bit, high code bit, non-ACGT flag, lower-case flag), so the four
+// planes of a window share cache lines.  kPadFront / kPadBack zero words
+// surround the sequence.
 struct Planes {
-    uint32_t *lo = nullptr, *hi = nullptr, *inv = nullptr, *lc = nullptr;
+    uint4 *q = nullptr;
     int64_t n_bases = 0;
-    int64_t n_words = 0;  // words per plane including padding
+    int64_t n_words = 0;  // words including padding
 };
 
 // Read planes: 24 words per read — lo[8], hi[8], lc[8] (reads <= 256 bases).
@@ -42,7 +46,8 @@ struct SieveLaunch {
     int32_t *out_shift, *out_best, *out_second;
     uint8_t *out_flags;
     int32_t cert_max_mism, cert_strict;
-    const uint32_t *g_lo, *g_hi, *g_inv, *g_lc;
+    const uint4 *g;   // window text planes (Planes::q)
+    int32_t g_flags;  // the inv / lc planes are meaningful (resident genome)
     int64_t g_bases;
 };
 

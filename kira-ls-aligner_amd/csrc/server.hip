@@ -238,12 +238,9 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
         a.out_flags = o_flags;
         a.cert_max_mism = cfg.cert_max_mism;
         a.cert_strict = cfg.cert_strict;
-        const kira::Planes &gp = genome->planes;
-        a.g_lo = gp.lo;
-        a.g_hi = gp.hi;
-        a.g_inv = gp.inv;
-        a.g_lc = gp.lc;
-        a.g_bases = gp.n_bases;
+        a.g = genome->planes.q;
+        a.g_flags = 1;
+        a.g_bases = genome->planes.n_bases;
         int rw = std::max(1, std::min(8, (r.max_sieve_len + 31) / 32));
         if ((e = kira::launch_sieve(a, rw, false, s))) return e;
     }

@@ -64,10 +64,7 @@ __global__ void pack_ascii_planes_kernel(const uint8_t *__restrict__ seq, int64_
             lc |= d << k;
         }
     }
-    p.lo[w] = lo;
-    p.hi[w] = hi;
-    if (p.inv) p.inv[w] = inv;
-    if (p.lc) p.lc[w] = lc;
+    p.q[w] = make_uint4(lo, hi, inv, lc);
 }
 
 // Split the even/odd bits of a 2-bit-interleaved word (16 bases) into two
@@ -104,10 +101,7 @@ __global__ void pack_2bit_planes_kernel(const uint8_t *__restrict__ packed, int6
             hi = even_bits(x0 >> 1) | (even_bits(x1 >> 1) << 16);
         }
     }
-    p.lo[w] = lo;
-    p.hi[w] = hi;
-    if (p.inv) p.inv[w] = 0;
-    if (p.lc) p.lc[w] = 0;
+    p.q[w] = make_uint4(lo, hi, 0u, 0u);
 }
 
 // One thread per (read, plane word).
@@ -226,18 +220,40 @@ __device__ __forceinline__ uint32_t range_mask(int k, int lo, int hi) {
     return hm & ~lm;
 }
 
-// Window text words starting at absolute base position p (p >= -64).
+// Window text words starting at absolute base position p (p >= -64): the
+// low / high code planes (one 8-byte load per word), optionally also the
+// lower-case plane (one 16-byte load per word).
 template <int NW>
-__device__ __forceinline__ void window_words(const uint32_t *__restrict__ plane, int64_t p,
-                                             uint32_t (&out)[NW]) {
+__device__ __forceinline__ void window_lohi(const uint4 *__restrict__ g, int64_t p,
+                                            uint32_t (&lo)[NW], uint32_t (&hi)[NW]) {
     int64_t q = p + (int64_t)kPadFront * 32;
     int64_t w0 = q >> 5;
     uint32_t s = (uint32_t)(q & 31);
-    uint32_t prev = plane[w0];
+    const uint2 *g2 = reinterpret_cast<const uint2 *>(g);
+    uint2 prev = g2[2 * w0];
 #pragma unroll
     for (int k = 0; k < NW; k++) {
-        uint32_t nx = plane[w0 + k + 1];
-        out[k] = funnel(nx, prev, s);
+        uint2 nx = g2[2 * (w0 + k + 1)];
+        lo[k] = funnel(nx.x, prev.x, s);
+        hi[k] = funnel(nx.y, prev.y, s);
+        prev = nx;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void window_lohilc(const uint4 *__restrict__ g, int64_t p,
+                                              uint32_t (&lo)[NW], uint32_t (&hi)[NW],
+                                              uint32_t (&lc)[NW]) {
+    int64_t q = p + (int64_t)kPadFront * 32;
+    int64_t w0 = q >> 5;
+    uint32_t s = (uint32_t)(q & 31);
+    uint4 prev = g[w0];
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        uint4 nx = g[w0 + k + 1];
+        lo[k] = funnel(nx.x, prev.x, s);
+        hi[k] = funnel(nx.y, prev.y, s);
+        lc[k] = funnel(nx.w, prev.w, s);
         prev = nx;
     }
 }
@@ -249,13 +265,14 @@ __device__ __forceinline__ int masked_mismatch(const uint32_t (&rl)[RW], const u
                                                const uint32_t (&rc)[RW], const SieveLaunch &a,
                                                int64_t pos, int lo, int hi) {
     uint32_t wl[RW], wh[RW], wc[RW] = {};
-    window_words<RW>(a.g_lo, pos, wl);
-    window_words<RW>(a.g_hi, pos, wh);
-    if (a.g_lc) window_words<RW>(a.g_lc, pos, wc);
+    if (a.g_flags)
+        window_lohilc<RW>(a.g, pos, wl, wh, wc);
+    else
+        window_lohi<RW>(a.g, pos, wl, wh);
     int m = 0;
 #pragma unroll
     for (int k = 0; k < RW; k++) {
-        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (a.g_lc ? (rc[k] ^ wc[k]) : rc[k]);
+        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (rc[k] ^ wc[k]);
         m += __popc(d & range_mask(k, lo, hi));
     }
     return m;
@@ -274,13 +291,14 @@ __device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
     if (best > 2) return false;
     int64_t at = s0 + shift;
     uint32_t wl[RW], wh[RW], wc[RW] = {};
-    window_words<RW>(a.g_lo, at, wl);
-    window_words<RW>(a.g_hi, at, wh);
-    if (a.g_lc) window_words<RW>(a.g_lc, at, wc);
+    if (a.g_flags)
+        window_lohilc<RW>(a.g, at, wl, wh, wc);
+    else
+        window_lohi<RW>(a.g, at, wl, wh);
     int n = 0, p0 = -1, p1 = -1;
 #pragma unroll
     for (int k = 0; k < RW; k++) {
-        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (a.g_lc ? (rc[k] ^ wc[k]) : rc[k]);
+        uint32_t d = (rl[k] ^ wl[k]) | (rh[k] ^ wh[k]) | (rc[k] ^ wc[k]);
         d &= range_mask(k, 0, L);
         while (d) {
             int b = __builtin_ctz(d);
@@ -315,13 +333,12 @@ __device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
 // Scan one tile of up to 128 shifts starting at T0.  FULL: all 128 valid.
 template <int RW, bool FULL, bool MASKALL>
 __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
-                                          const uint32_t (&mk)[RW], const uint32_t *g_lo,
-                                          const uint32_t *g_hi, int64_t pos, uint32_t rem,
+                                          const uint32_t (&mk)[RW], const uint4 *g,
+                                          int64_t pos, uint32_t rem,
                                           uint32_t &tb, uint32_t &ts) {
     constexpr int NW = RW + 4;
     uint32_t wl[NW], wh[NW];
-    window_words<NW>(g_lo, pos, wl);
-    window_words<NW>(g_hi, pos, wh);
+    window_lohi<NW>(g, pos, wl, wh);
     uint32_t nb = FULL ? 32u : (rem < 32u ? rem : 32u);
     for (uint32_t b = 0; b < nb; b++) {
         uint32_t xl[RW + 3], xh[RW + 3];
@@ -362,9 +379,9 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
         uint32_t tb = 0xFFFFFFFFu, ts = 0xFFFFFFFFu;
         uint32_t rem = nt - T0;
         if (rem >= 128)
-            scan_tile<RW, true, MASKALL>(rl, rh, mk, a.g_lo, a.g_hi, s0 + T0, rem, tb, ts);
+            scan_tile<RW, true, MASKALL>(rl, rh, mk, a.g, s0 + T0, rem, tb, ts);
         else
-            scan_tile<RW, false, MASKALL>(rl, rh, mk, a.g_lo, a.g_hi, s0 + T0, rem, tb, ts);
+            scan_tile<RW, false, MASKALL>(rl, rh, mk, a.g, s0 + T0, rem, tb, ts);
         uint32_t tbm = tb >> 16;
         uint32_t tsm = ts == 0xFFFFFFFFu ? 0xFFFFFFFFu : (ts >> 16);
         uint32_t hi = gbm > tbm ? gbm : tbm;
@@ -392,11 +409,11 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) 
     // windows that leave the text.
     bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
               s0 + W <= a.g_bases && L <= 32 * RW;
-    if (ok && a.g_inv && W > 0) {
+    if (ok && a.g_flags && W > 0) {
         int64_t q0 = s0 + (int64_t)kPadFront * 32, q1 = q0 + W;
         uint32_t any = 0;
         for (int64_t w = q0 >> 5; w <= ((q1 - 1) >> 5); w++) {
-            uint32_t v = a.g_inv[w];
+            uint32_t v = a.g[w].z;
             int lo = (int)(w == (q0 >> 5) ? (q0 & 31) : 0);
             int hi = (int)(w == ((q1 - 1) >> 5) ? (((q1 - 1) & 31) + 1) : 32);
             uint32_t m = (hi == 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
