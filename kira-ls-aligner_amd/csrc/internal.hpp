@@ -18,12 +18,17 @@ namespace kira {
 constexpr int kPadFront = 2;
 constexpr int kPadBack = 16;
 
-// 2-bit sequence as interleaved bit planes: word w (32 bases) is q[w] =
-// (low code bit, high code bit, non-ACGT flag, lower-case flag), so the four
-// planes of a window share cache lines.  kPadFront / kPadBack zero words
-// surround the sequence.
+// 2-bit sequence as bit planes, 32 bases per word: lh[w] = (low code bit,
+// high code bit) - the sieve scan reads only these, 8 bytes a word; il[w] =
+// (non-ACGT flag, lower-case flag) for the skip check and certification;
+// nz = one bit per word, set when the word holds a non-ACGT base, so the
+// skip check usually touches a few bits of a small, cache-resident array.
+// kPadFront / kPadBack zero words surround the sequence.  il / nz may be
+// null (windows packed from 2-bit data carry no flags).
 struct Planes {
-    uint4 *q = nullptr;
+    uint2 *lh = nullptr;
+    uint2 *il = nullptr;
+    uint32_t *nz = nullptr;
     int64_t n_bases = 0;
     int64_t n_words = 0;  // words including padding
 };
@@ -46,8 +51,8 @@ struct SieveLaunch {
     int32_t *out_shift, *out_best, *out_second;
     uint8_t *out_flags;
     int32_t cert_max_mism, cert_strict;
-    const uint4 *g;   // window text planes (Planes::q)
-    int32_t g_flags;  // the inv / lc planes are meaningful (resident genome)
+    const uint2 *g_lh, *g_il;  // window text planes (Planes); g_il null: no flags
+    const uint32_t *g_nz;
     int64_t g_bases;
 };
 

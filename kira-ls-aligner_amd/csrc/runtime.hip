@@ -56,12 +56,23 @@ enum Slot {
     S_COUNT
 };
 
-kira::Planes planes_in(void *base, int64_t n_bases) {
+// Plane arrays carved from one block: lh, then (with flags) il and the
+// word bitmap.  planes_bytes gives the block size.
+kira::Planes planes_in(void *base, int64_t n_bases, bool with_flags) {
     kira::Planes p;
     p.n_bases = n_bases;
     p.n_words = (n_bases + 31) / 32 + kira::kPadFront + kira::kPadBack;
-    p.q = static_cast<uint4 *>(base);
+    uint8_t *b = static_cast<uint8_t *>(base);
+    p.lh = reinterpret_cast<uint2 *>(b);
+    if (with_flags) {
+        p.il = reinterpret_cast<uint2 *>(b + p.n_words * 8);
+        p.nz = reinterpret_cast<uint32_t *>(b + p.n_words * 16);
+    }
     return p;
+}
+
+size_t planes_bytes(const kira::Planes &p, bool with_flags) {
+    return (size_t)p.n_words * 8 + (with_flags ? (size_t)p.n_words * 8 + ((size_t)p.n_words / 32 + 2) * 4 : 0);
 }
 
 }  // namespace
@@ -173,7 +184,7 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
         limits[i] = (int32_t)std::min<uint32_t>(lim, (uint32_t)INT32_MAX);
     }
     int64_t phase_bases = ref_bytes_per_phase * 4;
-    kira::Planes probe = planes_in(nullptr, phase_bases);
+    kira::Planes probe = planes_in(nullptr, phase_bases, false);
     size_t reads_bytes = (size_t)n * read_bytes_max;
     if ((e = ensure(ctx, S_READS, reads_bytes)) != hipSuccess ||
         (e = ensure(ctx, S_READ_NUCS, (size_t)n * 4)) != hipSuccess ||
@@ -184,7 +195,7 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
         (e = ensure(ctx, S_LIMITS, (size_t)n * 4)) != hipSuccess ||
         (e = ensure(ctx, S_OUT, (size_t)n * 12)) != hipSuccess ||
         (e = ensure(ctx, S_READ_PLANES, (size_t)n * kira::kReadWords * 4)) != hipSuccess ||
-        (e = ensure(ctx, S_WIN_PLANES, (size_t)probe.n_words * sizeof(uint4))) != hipSuccess)
+        (e = ensure(ctx, S_WIN_PLANES, planes_bytes(probe, false))) != hipSuccess)
         return fail(ctx, KIRA_ERR_RUNTIME, "device allocation", e);
     hipStream_t s = ctx->stream;
     uint8_t *d_reads = (uint8_t *)ctx->buf[S_READS];
@@ -195,7 +206,7 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
     int32_t *d_lim = (int32_t *)ctx->buf[S_LIMITS];
     int32_t *d_out = (int32_t *)ctx->buf[S_OUT];
     uint32_t *d_rp = (uint32_t *)ctx->buf[S_READ_PLANES];
-    kira::Planes wp = planes_in(ctx->buf[S_WIN_PLANES], phase_bases);
+    kira::Planes wp = planes_in(ctx->buf[S_WIN_PLANES], phase_bases, false);
 
     if ((e = hipMemcpyAsync(d_reads, reads_packed, reads_bytes, hipMemcpyHostToDevice, s)) ||
         (e = hipMemcpyAsync(d_nucs, read_nucs, (size_t)n * 4, hipMemcpyHostToDevice, s)) ||
@@ -224,8 +235,9 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
     a.out_second = d_out + 2 * (int64_t)n;
     a.out_flags = nullptr;
     a.cert_max_mism = -1;
-    a.g = wp.q;
-    a.g_flags = 0;
+    a.g_lh = wp.lh;
+    a.g_il = nullptr;
+    a.g_nz = nullptr;
     a.g_bases = phase_bases;
     int rw = std::max(1, (max_len + 31) / 32);
     if ((e = kira::launch_sieve(a, rw, false, s)))
@@ -244,9 +256,9 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
     *out = nullptr;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
-    kira::Planes probe = planes_in(nullptr, len);
+    kira::Planes probe = planes_in(nullptr, len, true);
     void *mem = nullptr;
-    if ((e = hipMalloc(&mem, (size_t)probe.n_words * sizeof(uint4))))
+    if ((e = hipMalloc(&mem, planes_bytes(probe, true))))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome allocation", e);
     // the ASCII sequence stays resident too (SW and prefilter windows read
     // bytes), with kRefPad zero bytes on each side for band rows that poke
@@ -256,7 +268,7 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
         (void)hipFree(mem);
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome allocation", e);
     }
-    kira::Planes p = planes_in(mem, len);
+    kira::Planes p = planes_in(mem, len, true);
     if ((e = hipMemsetAsync(d_seq, 0, (size_t)len + 2 * kira::kRefPad, ctx->stream)) ||
         (len > 0 && (e = hipMemcpyAsync(d_seq + kira::kRefPad, seq, (size_t)len,
                                         hipMemcpyHostToDevice, ctx->stream))) ||
@@ -281,7 +293,7 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
 
 void kira_genome_destroy(kira_genome *g) {
     if (!g) return;
-    if (g->planes.q) (void)hipFree(g->planes.q);
+    if (g->planes.lh) (void)hipFree(g->planes.lh);
     if (g->ascii) (void)hipFree(g->ascii - kira::kRefPad);
     delete g;
 }
@@ -326,8 +338,9 @@ kira_status kira_sieve_genome_dev(kira_ctx *ctx, const kira_genome *g,
     a.out_flags = b->out_flags;
     a.cert_max_mism = b->cert_max_mism;
     a.cert_strict = b->cert_strict;
-    a.g = g->planes.q;
-    a.g_flags = 1;
+    a.g_lh = g->planes.lh;
+    a.g_il = g->planes.il;
+    a.g_nz = g->planes.nz;
     a.g_bases = g->planes.n_bases;
     // The read-word count is a launch-wide template parameter; the caller
     // promises every read of the batch is at most this long (256 bases).

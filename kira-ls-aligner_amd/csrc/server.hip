@@ -238,8 +238,9 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
         a.out_flags = o_flags;
         a.cert_max_mism = cfg.cert_max_mism;
         a.cert_strict = cfg.cert_strict;
-        a.g = genome->planes.q;
-        a.g_flags = 1;
+        a.g_lh = genome->planes.lh;
+        a.g_il = genome->planes.il;
+        a.g_nz = genome->planes.nz;
         a.g_bases = genome->planes.n_bases;
         int rw = std::max(1, std::min(8, (r.max_sieve_len + 31) / 32));
         if ((e = kira::launch_sieve(a, rw, false, s))) return e;

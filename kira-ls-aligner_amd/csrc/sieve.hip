@@ -45,14 +45,14 @@ __device__ __forceinline__ void classify(uint8_t c, uint32_t &lo, uint32_t &hi, 
     lc = letter ? 1u : 0u;
 }
 
-// One thread per 32-base word of an ASCII sequence.
+// One thread per 32-base word of an ASCII sequence; each wave also writes
+// the 64 bits of the non-ACGT word bitmap its words cover.
 __global__ void pack_ascii_planes_kernel(const uint8_t *__restrict__ seq, int64_t n,
                                          Planes p) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= p.n_words) return;
     int64_t base = (w - kPadFront) * 32;
     uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
-    if (w >= kPadFront) {
+    if (w >= kPadFront && w < p.n_words) {
         for (int k = 0; k < 32; k++) {
             int64_t q = base + k;
             if (q >= n) break;
@@ -64,7 +64,17 @@ __global__ void pack_ascii_planes_kernel(const uint8_t *__restrict__ seq, int64_
             lc |= d << k;
         }
     }
-    p.q[w] = make_uint4(lo, hi, inv, lc);
+    if (w < p.n_words) {
+        p.lh[w] = make_uint2(lo, hi);
+        p.il[w] = make_uint2(inv, lc);
+    }
+    // blocks are whole waves starting at multiples of 64 words
+    const uint64_t bits = __ballot(inv != 0);
+    const int64_t w0 = w & ~(int64_t)63;
+    if ((threadIdx.x & 63) == 0 && w0 < p.n_words) {
+        p.nz[w0 >> 5] = (uint32_t)bits;
+        if (w0 + 32 < p.n_words) p.nz[(w0 >> 5) + 1] = (uint32_t)(bits >> 32);
+    }
 }
 
 // Split the even/odd bits of a 2-bit-interleaved word (16 bases) into two
@@ -101,7 +111,7 @@ __global__ void pack_2bit_planes_kernel(const uint8_t *__restrict__ packed, int6
             hi = even_bits(x0 >> 1) | (even_bits(x1 >> 1) << 16);
         }
     }
-    p.q[w] = make_uint4(lo, hi, 0u, 0u);
+    p.lh[w] = make_uint2(lo, hi);
 }
 
 // One thread per (read, plane word).
@@ -221,40 +231,24 @@ __device__ __forceinline__ uint32_t range_mask(int k, int lo, int hi) {
 }
 
 // Window text words starting at absolute base position p (p >= -64): the
-// low / high code planes (one 8-byte load per word), optionally also the
-// lower-case plane (one 16-byte load per word).
+// low / high code planes and, with flags, the lower-case plane.
 template <int NW>
-__device__ __forceinline__ void window_lohi(const uint4 *__restrict__ g, int64_t p,
-                                            uint32_t (&lo)[NW], uint32_t (&hi)[NW]) {
+__device__ __forceinline__ void window_words(const SieveLaunch &a, int64_t p, uint32_t (&lo)[NW],
+                                             uint32_t (&hi)[NW], uint32_t (&lc)[NW]) {
     int64_t q = p + (int64_t)kPadFront * 32;
     int64_t w0 = q >> 5;
     uint32_t s = (uint32_t)(q & 31);
-    const uint2 *g2 = reinterpret_cast<const uint2 *>(g);
-    uint2 prev = g2[2 * w0];
+    uint2 prev = a.g_lh[w0];
+    uint32_t pc = a.g_il ? a.g_il[w0].y : 0u;
 #pragma unroll
     for (int k = 0; k < NW; k++) {
-        uint2 nx = g2[2 * (w0 + k + 1)];
+        uint2 nx = a.g_lh[w0 + k + 1];
+        uint32_t nc = a.g_il ? a.g_il[w0 + k + 1].y : 0u;
         lo[k] = funnel(nx.x, prev.x, s);
         hi[k] = funnel(nx.y, prev.y, s);
+        lc[k] = funnel(nc, pc, s);
         prev = nx;
-    }
-}
-
-template <int NW>
-__device__ __forceinline__ void window_lohilc(const uint4 *__restrict__ g, int64_t p,
-                                              uint32_t (&lo)[NW], uint32_t (&hi)[NW],
-                                              uint32_t (&lc)[NW]) {
-    int64_t q = p + (int64_t)kPadFront * 32;
-    int64_t w0 = q >> 5;
-    uint32_t s = (uint32_t)(q & 31);
-    uint4 prev = g[w0];
-#pragma unroll
-    for (int k = 0; k < NW; k++) {
-        uint4 nx = g[w0 + k + 1];
-        lo[k] = funnel(nx.x, prev.x, s);
-        hi[k] = funnel(nx.y, prev.y, s);
-        lc[k] = funnel(nx.w, prev.w, s);
-        prev = nx;
+        pc = nc;
     }
 }
 
@@ -264,11 +258,8 @@ template <int RW>
 __device__ __forceinline__ int masked_mismatch(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
                                                const uint32_t (&rc)[RW], const SieveLaunch &a,
                                                int64_t pos, int lo, int hi) {
-    uint32_t wl[RW], wh[RW], wc[RW] = {};
-    if (a.g_flags)
-        window_lohilc<RW>(a.g, pos, wl, wh, wc);
-    else
-        window_lohi<RW>(a.g, pos, wl, wh);
+    uint32_t wl[RW], wh[RW], wc[RW];
+    window_words<RW>(a, pos, wl, wh, wc);
     int m = 0;
 #pragma unroll
     for (int k = 0; k < RW; k++) {
@@ -290,11 +281,8 @@ __device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
     if (best == 0) return true;
     if (best > 2) return false;
     int64_t at = s0 + shift;
-    uint32_t wl[RW], wh[RW], wc[RW] = {};
-    if (a.g_flags)
-        window_lohilc<RW>(a.g, at, wl, wh, wc);
-    else
-        window_lohi<RW>(a.g, at, wl, wh);
+    uint32_t wl[RW], wh[RW], wc[RW];
+    window_words<RW>(a, at, wl, wh, wc);
     int n = 0, p0 = -1, p1 = -1;
 #pragma unroll
     for (int k = 0; k < RW; k++) {
@@ -330,15 +318,13 @@ __device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
     return true;
 }
 
-// Scan one tile of up to 128 shifts starting at T0.  FULL: all 128 valid.
+// Scan one tile of up to 128 shifts; wl / wh: the tile's window words
+// (RW + 4 of them, aligned to its first shift).  FULL: all 128 valid.
 template <int RW, bool FULL, bool MASKALL>
 __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
-                                          const uint32_t (&mk)[RW], const uint4 *g,
-                                          int64_t pos, uint32_t rem,
+                                          const uint32_t (&mk)[RW], const uint32_t (&wl)[RW + 4],
+                                          const uint32_t (&wh)[RW + 4], uint32_t rem,
                                           uint32_t &tb, uint32_t &ts) {
-    constexpr int NW = RW + 4;
-    uint32_t wl[NW], wh[NW];
-    window_lohi<NW>(g, pos, wl, wh);
     uint32_t nb = FULL ? 32u : (rem < 32u ? rem : 32u);
     for (uint32_t b = 0; b < nb; b++) {
         uint32_t xl[RW + 3], xh[RW + 3];
@@ -375,13 +361,35 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
     gbm = 0xFFFFFFFFu;
     gbt = 0;
     gsm = 0xFFFFFFFFu;
+    // Each window word is loaded once per job: a tile of 128 shifts needs the
+    // NW + 1 raw words from its start, and the next tile starts 4 words on,
+    // so the raw words slide by 4 and only 4 new ones are loaded.
+    constexpr int NW = RW + 4;
+    const int64_t q = s0 + (int64_t)kPadFront * 32;
+    const int64_t w0 = q >> 5;
+    const uint32_t sh = (uint32_t)(q & 31);
+    uint2 raw[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; k++) raw[k] = a.g_lh[w0 + k];
     for (uint32_t T0 = 0; T0 < nt; T0 += 128) {
+        if (T0 > 0) {
+#pragma unroll
+            for (int k = 0; k + 4 <= NW; k++) raw[k] = raw[k + 4];
+#pragma unroll
+            for (int k = NW - 3; k <= NW; k++) raw[k] = a.g_lh[w0 + (T0 >> 5) + k];
+        }
+        uint32_t wl[NW], wh[NW];
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            wl[k] = funnel(raw[k + 1].x, raw[k].x, sh);
+            wh[k] = funnel(raw[k + 1].y, raw[k].y, sh);
+        }
         uint32_t tb = 0xFFFFFFFFu, ts = 0xFFFFFFFFu;
         uint32_t rem = nt - T0;
         if (rem >= 128)
-            scan_tile<RW, true, MASKALL>(rl, rh, mk, a.g, s0 + T0, rem, tb, ts);
+            scan_tile<RW, true, MASKALL>(rl, rh, mk, wl, wh, rem, tb, ts);
         else
-            scan_tile<RW, false, MASKALL>(rl, rh, mk, a.g, s0 + T0, rem, tb, ts);
+            scan_tile<RW, false, MASKALL>(rl, rh, mk, wl, wh, rem, tb, ts);
         uint32_t tbm = tb >> 16;
         uint32_t tsm = ts == 0xFFFFFFFFu ? 0xFFFFFFFFu : (ts >> 16);
         uint32_t hi = gbm > tbm ? gbm : tbm;
@@ -409,11 +417,13 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) 
     // windows that leave the text.
     bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
               s0 + W <= a.g_bases && L <= 32 * RW;
-    if (ok && a.g_flags && W > 0) {
+    if (ok && a.g_il && W > 0) {
         int64_t q0 = s0 + (int64_t)kPadFront * 32, q1 = q0 + W;
         uint32_t any = 0;
         for (int64_t w = q0 >> 5; w <= ((q1 - 1) >> 5); w++) {
-            uint32_t v = a.g[w].z;
+            // the word bitmap rules out almost every word without touching il
+            if (!((a.g_nz[w >> 5] >> (w & 31)) & 1u)) continue;
+            uint32_t v = a.g_il[w].x;
             int lo = (int)(w == (q0 >> 5) ? (q0 & 31) : 0);
             int hi = (int)(w == ((q1 - 1) >> 5) ? (((q1 - 1) & 31) + 1) : 32);
             uint32_t m = (hi == 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
