@@ -428,6 +428,15 @@ def main():
     bytes_per_launch = n_reads * ((L + 3) // 4) + n_jobs * (((W + 3) // 4) + 12)
     ops_per_launch = n_jobs * n_shifts * words * 4
     gbs = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # on this workload (scaled by a plain 1 GiB copy), kept beside the profiles
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sieve_kernel_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("workload_reads") == n_reads and t.get("window") == W:
+            traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024
     tops = ops_per_launch / (kernel_ms * 1e-3) / 1e12
     line = {
         "metric": "reads/s + SW GCUPS on 1 MI355X vs host CPU; p50/p99 batch latency",
@@ -455,7 +464,7 @@ def main():
         "true_locus_recall": true_hit,
         "parity_vs_oracle_sample": parity,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic},
         "roofline_valu": {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
                           "unit": "Tops/s", "frac": tops / VALU_PEAK_TOPS,
                           "ops_per_launch": ops_per_launch},
