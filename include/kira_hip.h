@@ -177,7 +177,7 @@ typedef struct {
  * :1370-1650) over a batch, host arrays in and out.  Job k aligns
  * reads[read_off[k] .. +read_len[k]] against refs[ref_off[k] .. +ref_len[k]]
  * with band centre offset offset[k], band band[k] (NULL: cfg->bandwidth) and
- * abort_score[k] (<= INT32_MIN/8 disables the abort bound).  Bands up to 263
+ * abort_score[k] (<= INT32_MIN/8 disables the abort bound).  Bands up to 271
  * are supported; bands above 128 also need match, mismatch and gap costs in
  * [0, 2^20] and read_len * match < 2^21, and other jobs return
  * KIRA_ERR_INVALID_ARGUMENT for the whole batch.  cigar_arena
@@ -288,8 +288,8 @@ kira_status kira_align_batch_simd(kira_ctx *ctx, int32_t n_batches, const int32_
 /*
  * Device-resident form for batches that repeat a shape: a plan fixes the
  * job count, read lengths and bands (it sorts jobs by band class and length
- * into packets of 64 lane-per-job jobs, or of 4 jobs of 16 lanes for bands
- * above 64, and sizes the trace slabs); run_dev then only
+ * into packets of 64 lane-per-job jobs, or of 4 (2) jobs of 16 (32) lanes
+ * for bands above 64, and sizes the trace slabs); run_dev then only
  * launches kernels on the context stream.  All kira_sw_jobs_dev pointers are
  * device pointers; refs must have at least 1024 readable bytes before and
  * after every window.  res: n x 10 int32 (score, read_end, ref_end,

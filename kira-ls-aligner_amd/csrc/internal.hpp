@@ -121,8 +121,9 @@ __host__ __device__ inline void sw_write_skipped(const SwLaunch &a, int32_t job)
 }
 
 hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
-// Wide bands: 16 lanes per job, P band positions per lane (P in 9, 17, 25, 33).
-hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int n_waves, hipStream_t s);
+// Wide bands: G lanes per job (16 or 32), P band positions per lane
+// ((P, G) in (9, 16), (17, 16), (13, 32), (17, 32)).
+hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, hipStream_t s);
 
 // Host-side packetisation of a banded SW batch (jobs sorted by band class
 // and read length into packets, trace / meta slab offsets, CIGAR arena).

@@ -1,26 +1,28 @@
 // Banded adaptive Smith-Waterman (banded_sw_with, src/alignment/mod.rs:1370-1650)
-// for wide bands: one DPP row of 16 lanes runs one job.
+// for wide bands: a group of G = 16 (one DPP row) or 32 lanes runs one job.
 //
 // A lane-per-job band of 2w + 1 > 129 cells no longer fits the register
-// file, so here the band of row i is spread over the 16 lanes of a row
-// group: lane g holds positions q = g*P .. g*P + P - 1 (diagonal
-// coordinates, column j = c_i - w + q, as in the lane-per-job kernel), and
-// C = 16 P >= 2w + 1 (positions past 2w are always outside the band).
+// file, so here the band of row i is spread over the G lanes of a group:
+// lane g holds positions q = g*P .. g*P + P - 1 (diagonal coordinates,
+// column j = c_i - w + q, as in the lane-per-job kernel), and C = G P >=
+// 2w + 1 (positions past 2w are always outside the band).  Wide bands use
+// 32 lanes rather than more positions per lane, which keeps a lane's band
+// slice small enough for several waves per SIMD.
 //
 // Per row every cell's diagonal / E term is local (the up neighbour of a
 // lane's last position is the next lane's first, one DPP move).  The
 // horizontal gap F is a max-plus prefix: with go = open + extend >= extend,
 //   F[q] = max(Hpre[q-1] - go, F[q-1] - ge),   Hpre = max(diag + s, E, 0)
 // equals the reference's F over the final H, so each lane runs its own
-// chain, a 4-step DPP scan carries the chains across lanes (decaying by
-// P * ge per lane), and each position takes max(local, carried - p * ge).
-// The row's first strict arg-max is a 4-step DPP rotate-reduce of
-// (score << 10 | 1023 - q).  Everything else (pads for absent cells, trace
+// chain, a 4-step DPP scan carries the chains across the lanes of a row
+// (decaying by P * ge per lane; a row_bcast:15 step joins two rows for
+// G = 32), and each lane continues its chain from that carry.  The row's
+// first strict arg-max is a DPP rotate-reduce of (score << 10 | 1023 - q).  Everything else (pads for absent cells, trace
 // codes, centre update, x-drop / abort stops, last-row clip rule) follows
 // the lane-per-job kernel, so both give identical results.
 //
-// Trace slab per packet of 4 jobs: [row][word][wave lane], word k of lane g
-// holding the 4-bit codes of positions g*P + 8k .. g*P + 8k + 7.
+// Trace slab per packet of 64 / G jobs: [row][word][wave lane], word k of
+// lane g holding the 4-bit codes of positions g*P + 8k .. g*P + 8k + 7.
 #include "internal.hpp"
 
 namespace kira {
@@ -43,13 +45,36 @@ template <int N>
 __device__ __forceinline__ int32_t from_prev(int32_t v, int32_t fill) {
     return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
 }
-// In-place forms (bound_ctrl: lanes without a source read 0); the caller
-// overwrites the edge lane, so no copy of the source array is needed.
-__device__ __forceinline__ int32_t next_lane0(int32_t v) {
-    return __builtin_amdgcn_mov_dpp(v, 0x101, 0xF, 0xF, true);
+// Neighbour lanes inside a group of G lanes (g: lane within the group):
+// G = 16 is one DPP row; G = 32 uses the wave shifts and patches the group
+// edge.  Lanes without a neighbour read `fill`.
+template <int G>
+__device__ __forceinline__ int32_t nb_next(int32_t v, int32_t fill, int g) {
+    if constexpr (G == 16) {
+        return __builtin_amdgcn_update_dpp(fill, v, 0x101, 0xF, 0xF, false);  // row_shl:1
+    } else {
+        const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+        return g == G - 1 ? fill : t;
+    }
 }
+template <int G>
+__device__ __forceinline__ int32_t nb_prev(int32_t v, int32_t fill, int g) {
+    if constexpr (G == 16) {
+        return __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    } else {
+        const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+        return g == 0 ? fill : t;
+    }
+}
+// In-place forms (bound_ctrl: lanes without a source read 0); the caller
+// overwrites the group-edge lane, so no copy of the source array is needed.
+template <int G>
+__device__ __forceinline__ int32_t next_lane0(int32_t v) {
+    return __builtin_amdgcn_mov_dpp(v, G == 16 ? 0x101 : 0x130, 0xF, 0xF, true);
+}
+template <int G>
 __device__ __forceinline__ int32_t prev_lane0(int32_t v) {
-    return __builtin_amdgcn_mov_dpp(v, 0x111, 0xF, 0xF, true);
+    return __builtin_amdgcn_mov_dpp(v, G == 16 ? 0x111 : 0x138, 0xF, 0xF, true);
 }
 template <int N>
 __device__ __forceinline__ int32_t rotate(int32_t v) {
@@ -60,7 +85,7 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s)
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
 }
 
-template <int P>
+template <int P, int G>
 struct CoopBand {
     int32_t H[P], E[P];
     int32_t padH, padE;
@@ -71,32 +96,32 @@ struct CoopBand {
             E[p] = padE;
         }
     }
-    // new[q] = old[q + delta] over the whole 16-lane band (g: lane in row)
+    // new[q] = old[q + delta] over the whole band (g: lane in the group)
     __device__ __forceinline__ void shift(int32_t delta, int g) {
-        if (delta >= 16 * P || delta <= -16 * P) {
+        if (delta >= G * P || delta <= -G * P) {
             clear();
             return;
         }
         while (delta >= P) {  // whole lanes: lane g takes lane g + 1's positions
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const int32_t h = next_lane0(H[p]), e = next_lane0(E[p]);
-                H[p] = g == 15 ? padH : h;
-                E[p] = g == 15 ? padE : e;
+                const int32_t h = next_lane0<G>(H[p]), e = next_lane0<G>(E[p]);
+                H[p] = g == G - 1 ? padH : h;
+                E[p] = g == G - 1 ? padE : e;
             }
             delta -= P;
         }
         while (delta <= -P) {
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const int32_t h = prev_lane0(H[p]), e = prev_lane0(E[p]);
+                const int32_t h = prev_lane0<G>(H[p]), e = prev_lane0<G>(E[p]);
                 H[p] = g == 0 ? padH : h;
                 E[p] = g == 0 ? padE : e;
             }
             delta += P;
         }
         while (delta > 0) {
-            int32_t nh = from_next<1>(H[0], padH), ne = from_next<1>(E[0], padE);
+            int32_t nh = nb_next<G>(H[0], padH, g), ne = nb_next<G>(E[0], padE, g);
 #pragma unroll
             for (int p = 0; p < P - 1; p++) {
                 H[p] = H[p + 1];
@@ -107,7 +132,7 @@ struct CoopBand {
             delta--;
         }
         while (delta < 0) {
-            int32_t nh = from_prev<1>(H[P - 1], padH), ne = from_prev<1>(E[P - 1], padE);
+            int32_t nh = nb_prev<G>(H[P - 1], padH, g), ne = nb_prev<G>(E[P - 1], padE, g);
 #pragma unroll
             for (int p = P - 1; p > 0; p--) {
                 H[p] = H[p - 1];
@@ -120,9 +145,9 @@ struct CoopBand {
     }
 };
 
-// One job on the 16 lanes of a DPP row; g = lane within the row, wl = wave
-// lane of g = 0 (the job's slot column in the packet's meta slab).
-template <int P>
+// One job on a group of G lanes; g = lane within the group, wl = wave lane
+// of g = 0 (the job's slot column in the packet's meta slab).
+template <int P, int G>
 __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t rows,
                          uint32_t *slab, int2 *meta) {
     constexpr int RW = (P + 7) / 8;
@@ -138,7 +163,7 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
     const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
     const int32_t q0 = g * P;
 
-    CoopBand<P> bd;
+    CoopBand<P, G> bd;
     bd.padH = kMin4 + go;
     bd.padE = kMin4 + ge;
     bd.clear();
@@ -185,7 +210,7 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         }
         // valid positions of this lane: p in [vlo, vhi]
         const int32_t vlo = js - base - q0, vhi = je - base - q0;
-        const int32_t upH = from_next<1>(bd.H[0], bd.padH), upE = from_next<1>(bd.E[0], bd.padE);
+        const int32_t upH = nb_next<G>(bd.H[0], bd.padH, g), upE = nb_next<G>(bd.E[0], bd.padE, g);
         // pass 1: diagonal and E terms, Hpre = max(diag + s, E, 0), this
         // row's E (stored over the previous row's as soon as it is read)
         // and the lane's F chain with nothing coming in
@@ -216,6 +241,13 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         F = imax(F, from_prev<2>(F, kMin4) - 2 * dP);
         F = imax(F, from_prev<4>(F, kMin4) - 4 * dP);
         F = imax(F, from_prev<8>(F, kMin4) - 8 * dP);
+        if constexpr (G == 32) {
+            // rows 1 and 3 add the carry out of rows 0 and 2: lane 15's F at
+            // the next lane's start, decayed per lane of the receiving row
+            const int32_t t = imax(f, F - dP);
+            const int32_t y = __builtin_amdgcn_update_dpp(kMin4, t, 0x142, 0xA, 0xF, false);
+            F = imax(F, y - (g & 15) * dP);
+        }
         // pass 2: F along the lane, final H, trace codes, row key.  The
         // F-extend bit of position p compares F and H of position p - 1;
         // for p = 0 that is the previous lane's last position, patched into
@@ -251,7 +283,7 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
             }
         }
         {
-            const int32_t lf = from_prev<1>(prevF, kMin4), lh = from_prev<1>(prevH, bd.padH);
+            const int32_t lf = nb_prev<G>(prevF, kMin4, g), lh = nb_prev<G>(prevH, bd.padH, g);
             const bool lv = (q0 - 1 >= js - base) && (q0 - 1 <= je - base);
             if (vlo <= 0 && vhi >= 0 && lv && (lf - ge > lh - go)) word0 |= 8u;
             trow[0] = word0;
@@ -261,6 +293,7 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
         key = imax(key, rotate<4>(key));
         key = imax(key, rotate<2>(key));
         key = imax(key, rotate<1>(key));
+        if constexpr (G == 32) key = imax(key, __shfl_xor(key, 16));
         const int32_t rb_all = key >> 10;
         const int32_t rbj = rb_all > 0 ? base + (1023 - (key & 1023)) : -1;
         if (g == 0)
@@ -313,13 +346,12 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
     return make_int2(bi, bj);
 }
 
-// Traceback (mod.rs:1581-1626) by the job's 16 lanes together: each round
+// Traceback (mod.rs:1581-1626) by the job's G lanes together: each round
 // lane k fetches the trace code of cell (i - k, j - k), so a run of
-// diagonal moves costs one dependent meta + trace load per 16 cells
-// instead of per cell; the walk itself (the same state machine as the
-// lane-per-job traceback) runs redundantly in all 16 lanes and lane 0
-// writes the CIGAR.
-template <int P>
+// diagonal moves costs one dependent meta + trace load per G cells instead
+// of per cell; the walk itself (the same state machine as the lane-per-job
+// traceback) runs redundantly in all G lanes and lane 0 writes the CIGAR.
+template <int P, int G>
 __device__ void coop_traceback(const SwLaunch &a, int32_t job, int g, int wl, int2 end,
                                const uint32_t *slab, const int2 *meta) {
     constexpr int RW = (P + 7) / 8;
@@ -350,7 +382,7 @@ __device__ void coop_traceback(const SwLaunch &a, int32_t job, int g, int wl, in
         const int32_t i0 = i, j0 = j;
         for (;;) {
             const int32_t k = i0 - i;
-            if (k >= 16 || j0 - j != k || i <= 0 || j <= 0) break;
+            if (k >= G || j0 - j != k || i <= 0 || j <= 0) break;
             const int32_t okk = __shfl(ok, wl + k);
             const uint32_t tr = (uint32_t)__shfl((int32_t)tw, wl + k);
             if (!okk) {  // the cell lies outside its row's band
@@ -407,10 +439,10 @@ __device__ void coop_traceback(const SwLaunch &a, int32_t job, int g, int wl, in
     }
 }
 
-template <int P>
+template <int P, int G>
 __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
     const int lane = threadIdx.x;
-    const int g = lane & 15, wl = lane & ~15;
+    const int g = lane & (G - 1), wl = lane & ~(G - 1);
     for (;;) {
         int32_t p = 0;
         if (lane == 0) p = atomicAdd(a.counter, 1);
@@ -422,11 +454,11 @@ __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
         } else if (job >= 0) {
             uint32_t *slab = a.trace + a.packet_trace_off[p];
             int2 *meta = a.row_meta + a.packet_meta_off[p];
-            const int2 end = coop_job<P>(a, job, g, wl, a.packet_rows[p], slab, meta);
+            const int2 end = coop_job<P, G>(a, job, g, wl, a.packet_rows[p], slab, meta);
             // every lane reads trace words its neighbours wrote
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            coop_traceback<P>(a, job, g, wl, end, slab, meta);
+            coop_traceback<P, G>(a, job, g, wl, end, slab, meta);
         }
     }
 }
@@ -435,16 +467,14 @@ __global__ void zero_coop_counter_kernel(int32_t *c) { *c = 0; }
 
 }  // namespace
 
-hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int n_waves, hipStream_t s) {
+hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, hipStream_t s) {
     zero_coop_counter_kernel<<<1, 1, 0, s>>>(a.counter);
     if (a.n_packets > 0) {
-        switch (P) {
-        case 9: banded_sw_coop_kernel<9><<<n_waves, 64, 0, s>>>(a); break;
-        case 17: banded_sw_coop_kernel<17><<<n_waves, 64, 0, s>>>(a); break;
-        case 25: banded_sw_coop_kernel<25><<<n_waves, 64, 0, s>>>(a); break;
-        case 33: banded_sw_coop_kernel<33><<<n_waves, 64, 0, s>>>(a); break;
-        default: return hipErrorInvalidValue;
-        }
+        if (G == 16 && P == 9) banded_sw_coop_kernel<9, 16><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 16 && P == 17) banded_sw_coop_kernel<17, 16><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 32 && P == 13) banded_sw_coop_kernel<13, 32><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 32 && P == 17) banded_sw_coop_kernel<17, 32><<<n_waves, 64, 0, s>>>(a);
+        else return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

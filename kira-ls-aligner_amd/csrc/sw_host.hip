@@ -17,22 +17,26 @@ namespace {
 // Job classes.  Lane-per-job: a job with band w runs in the smallest
 // C >= 2w + 1 of kLaneC, with the band stored as packed int16 when its
 // scores provably fit, else as int32 (class = band index * 2 + packed).
-// Bands above 64 (C = 257 would spill the register file) run 16 lanes per
-// job with P = kCoopP[k] positions a lane (16 P >= 2w + 1) when the costs
-// keep its int32 scan and arg-max key exact; otherwise C = 257 lane-per-job.
+// Bands above 64 (C = 257 would spill the register file) run kCoop[k].g
+// lanes per job with P = kCoop[k].p positions a lane (G P >= 2w + 1) when
+// the costs keep its int32 scan and arg-max key exact; otherwise C = 257
+// lane-per-job.
 constexpr int kLaneC[] = {9, 17, 33, 65, 129, 257};
 constexpr int kNumLane = 6;
-constexpr int kCoopP[] = {9, 17, 25, 33};
+struct CoopShape {
+    int p, g;
+};
+constexpr CoopShape kCoop[] = {{9, 16}, {17, 16}, {13, 32}, {17, 32}};
 constexpr int kNumCoop = 4;
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
-constexpr int kMaxBand = (16 * 33 - 1) / 2;  // 263
+constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
-int class_coop_p(int c) { return kCoopP[c - 2 * kNumLane]; }
-int class_jobs_per_packet(int c) { return class_is_coop(c) ? 4 : 64; }
+CoopShape class_coop(int c) { return kCoop[c - 2 * kNumLane]; }
+int class_jobs_per_packet(int c) { return class_is_coop(c) ? 64 / class_coop(c).g : 64; }
 int class_rowdw(int c) {
-    return class_is_coop(c) ? (class_coop_p(c) + 7) / 8 : (kLaneC[c / 2] + 7) / 8;
+    return class_is_coop(c) ? (class_coop(c).p + 7) / 8 : (kLaneC[c / 2] + 7) / 8;
 }
 
 int lane_class_of(int32_t w) {
@@ -44,7 +48,7 @@ int lane_class_of(int32_t w) {
 
 int coop_class_of(int32_t w) {
     for (int k = 0; k < kNumCoop; k++)
-        if (16 * kCoopP[k] >= 2 * w + 1) return 2 * kNumLane + k;
+        if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) return 2 * kNumLane + k;
     return -1;
 }
 
@@ -96,13 +100,13 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
         bool coop = coop_costs && (int64_t)read_len[k] * cfg.match_score < (1 << 21);
         if (w > 64 && coop) {
             cls[k] = coop_class_of(w);
-            if (cls[k] < 0) return "band wider than 263";
+            if (cls[k] < 0) return "band wider than 271";
             continue;
         }
         int b = lane_class_of(w);
         if (b < 0)
             return w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
-                                 : "band wider than 263";
+                                 : "band wider than 271";
         cls[k] = 2 * b + (pack ? 1 : 0);
     }
     // packets: jobs sorted by (band class, read length descending)
@@ -261,7 +265,7 @@ const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira
         hipError_t e;
         if (class_is_coop(c)) {
             int waves = std::min(L.class_count[c], 256 * 20);
-            e = launch_banded_sw_coop(ac, class_coop_p(c), waves, s);
+            e = launch_banded_sw_coop(ac, class_coop(c).p, class_coop(c).g, waves, s);
         } else {
             int waves = std::min(L.class_count[c], 256 * 16);
             e = launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, s);

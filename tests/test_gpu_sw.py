@@ -118,12 +118,12 @@ def test_reference_known_answers(pkg, ctx):
 
 
 def test_wide_bands_and_long_windows(pkg, ctx):
-    """Bands 65..263 (16 lanes per job) on long windows at long-read error
+    """Bands 65..271 (16 and 32 lanes per job) on long windows at long-read error
     rates (5% substitutions, 5% indels), next to narrow-band jobs."""
     rng = random.Random(11)
     jobs = []
     for k in range(96):
-        w = rng.choice([65, 70, 71, 72, 100, 128, 129, 135, 136, 199, 200, 250, 263, 16, 40])
+        w = rng.choice([65, 70, 71, 72, 100, 128, 129, 135, 136, 199, 200, 207, 208, 250, 271, 16, 40])
         L = rng.randrange(50, 2500)
         flank = max(2 * w, 50)
         g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank + 20))
@@ -140,7 +140,7 @@ def test_wide_band_limits(pkg, ctx):
     ref = gen_ref(400)
     read = ref[100:200]
     with pytest.raises(pkg.HipError) as e:
-        pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 264)], pkg.sw_cfg())
+        pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 272)], pkg.sw_cfg())
     assert e.value.kind == "InvalidArgument"
     # scores past the 16-lane kernel's exact range: bands up to 128 still run
     # (lane per job, int32), wider ones are refused
