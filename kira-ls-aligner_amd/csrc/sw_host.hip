@@ -31,6 +31,7 @@ constexpr int kNumCoop = 4;
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
+constexpr int kMinLaneJobs = 8192;
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
 CoopShape class_coop(int c) { return kCoop[c - 2 * kNumLane]; }
@@ -92,12 +93,14 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
     const bool coop_costs = cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
                             cfg.gap_extend >= 0 && cfg.match_score <= kCoopCost &&
                             cfg.mismatch <= kCoopCost && go <= kCoopCost;
+    std::vector<uint8_t> coop_ok(n);
     for (int32_t k = 0; k < n; k++) {
         int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
         if (read_len[k] < 0) return "negative read length";
         // every H is at most read_len * match: int16 storage cannot clamp
         bool pack = costs_small && (int64_t)read_len[k] * cfg.match_score < 32767;
         bool coop = coop_costs && (int64_t)read_len[k] * cfg.match_score < (1 << 21);
+        coop_ok[k] = coop;
         if (w > 64 && coop) {
             cls[k] = coop_class_of(w);
             if (cls[k] < 0) return "band wider than 271";
@@ -108,6 +111,20 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
             return w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
                                  : "band wider than 271";
         cls[k] = 2 * b + (pack ? 1 : 0);
+    }
+    // A lane-per-job class runs one wave per 64 jobs, so a class of a few
+    // thousand jobs occupies a few dozen of the 1024 SIMDs.  Its jobs are
+    // then worth more as 16-lane jobs (16x the waves): bands 33-64 (C = 129)
+    // fit the 16 x 9 positions of the smallest 16-lane class.
+    {
+        int count[kNumClasses] = {};
+        for (int32_t k = 0; k < n; k++) count[cls[k]]++;
+        const int c129 = 2 * 4, c129p = 2 * 4 + 1;
+        for (int32_t k = 0; k < n; k++) {
+            const int c = cls[k];
+            if ((c == c129 || c == c129p) && count[c] < kMinLaneJobs && coop_ok[k])
+                cls[k] = 2 * kNumLane;  // (9, 16)
+        }
     }
     // packets: jobs sorted by (band class, read length descending)
     std::vector<int32_t> order(n);

@@ -150,3 +150,39 @@ def test_wide_band_limits(pkg, ctx):
     r, = pkg.banded_sw_batch(ctx, [(read, ref, 100, ABORT_NONE, 100)], big)
     want = O.banded_sw(read, ref, 100, O.cfg(30000, 4, 6, 1, 100, 100, 5), ABORT_NONE, O.knobs())
     assert r == want
+
+
+def test_large_batch_keeps_lane_per_job_classes(pkg, ctx):
+    """A class of >= 8192 jobs of band 33-64 runs lane per job (smaller ones
+    move to 16-lane jobs); both must agree with the oracle."""
+    import numpy as np
+    rng = random.Random(23)
+    jobs = []
+    for k in range(9000):
+        w = rng.choice([33, 48, 64])
+        L = rng.randrange(60, 160)
+        flank = 2 * w
+        g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank))
+        read = mutate(rng, g[flank:flank + L], 0.03, 0.01) or b"A"
+        jobs.append((read, g, flank, ABORT_NONE, w))
+    c = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
+    print(f"[sw] large batch {len(jobs)} jobs", flush=True)
+    got = pkg.banded_sw_batch(ctx, jobs, c, pkg.sw_knobs())
+    reads = np.frombuffer(b"".join(j[0] for j in jobs), np.uint8)
+    refs = np.frombuffer(b"".join(j[1] for j in jobs), np.uint8)
+    rl = np.array([len(j[0]) for j in jobs], np.int32)
+    fl = np.array([len(j[1]) for j in jobs], np.int32)
+    ro = np.concatenate([[0], np.cumsum(rl[:-1])]).astype(np.int64)
+    fo = np.concatenate([[0], np.cumsum(fl[:-1])]).astype(np.int64)
+    off = np.array([j[2] for j in jobs], np.int32)
+    ab = np.array([j[3] for j in jobs], np.int32)
+    bands = np.array([j[4] for j in jobs], np.int32)
+    sc, en, cells = O.banded_sw_batch_mt(reads, ro, rl, refs, fo, fl, off, ab,
+                                         O.cfg(1, 4, 6, 1, 32, 100, 5), bands=bands, threads=8)
+    assert [g["score"] for g in got] == sc.tolist()
+    assert [g["ref_end"] for g in got] == en.tolist()
+    assert [g["cells"] for g in got] == cells.tolist()
+    for k in range(0, 9000, 45):
+        read, ref, o, a, w = jobs[k]
+        want = O.banded_sw(read, ref, o, O.cfg(1, 4, 6, 1, w, 100, 5), a, O.knobs())
+        assert got[k] == want, k
