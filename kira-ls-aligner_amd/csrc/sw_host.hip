@@ -31,7 +31,7 @@ constexpr int kNumCoop = 4;
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
-constexpr int kMinLaneJobs = 8192;
+constexpr int kMinLaneJobs = 65536;  // one 64-job wave on each of the 1024 SIMDs
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
 CoopShape class_coop(int c) { return kCoop[c - 2 * kNumLane]; }
@@ -112,10 +112,10 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
                                  : "band wider than 271";
         cls[k] = 2 * b + (pack ? 1 : 0);
     }
-    // A lane-per-job class runs one wave per 64 jobs, so a class of a few
-    // thousand jobs occupies a few dozen of the 1024 SIMDs.  Its jobs are
-    // then worth more as 16-lane jobs (16x the waves): bands 33-64 (C = 129)
-    // fit the 16 x 9 positions of the smallest 16-lane class.
+    // A lane-per-job class runs one wave per 64 jobs, and C = 129 fits one
+    // wave per SIMD, so a class of fewer than 64 x 1024 jobs leaves SIMDs
+    // idle.  Its jobs are then worth more as 16-lane jobs (16x the waves):
+    // bands 33-64 fit the 16 x 9 positions of the smallest 16-lane class.
     {
         int count[kNumClasses] = {};
         for (int32_t k = 0; k < n; k++) count[cls[k]]++;

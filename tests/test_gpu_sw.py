@@ -153,14 +153,14 @@ def test_wide_band_limits(pkg, ctx):
 
 
 def test_large_batch_keeps_lane_per_job_classes(pkg, ctx):
-    """A class of >= 8192 jobs of band 33-64 runs lane per job (smaller ones
+    """A class of >= 65536 jobs of band 33-64 runs lane per job (smaller ones
     move to 16-lane jobs); both must agree with the oracle."""
     import numpy as np
     rng = random.Random(23)
     jobs = []
-    for k in range(9000):
+    for k in range(66000):
         w = rng.choice([33, 48, 64])
-        L = rng.randrange(60, 160)
+        L = rng.randrange(40, 80)
         flank = 2 * w
         g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank))
         read = mutate(rng, g[flank:flank + L], 0.03, 0.01) or b"A"
@@ -182,7 +182,7 @@ def test_large_batch_keeps_lane_per_job_classes(pkg, ctx):
     assert [g["score"] for g in got] == sc.tolist()
     assert [g["ref_end"] for g in got] == en.tolist()
     assert [g["cells"] for g in got] == cells.tolist()
-    for k in range(0, 9000, 45):
+    for k in range(0, 66000, 330):
         read, ref, o, a, w = jobs[k]
         want = O.banded_sw(read, ref, o, O.cfg(1, 4, 6, 1, w, 100, 5), a, O.knobs())
         assert got[k] == want, k
