@@ -32,6 +32,7 @@ constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
 constexpr int kMinLaneJobs = 65536;  // one 64-job wave on each of the 1024 SIMDs
+constexpr int64_t kLongJobCells = 1 << 16;
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
 CoopShape class_coop(int c) { return kCoop[c - 2 * kNumLane]; }
@@ -104,6 +105,14 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
         if (w > 64 && coop) {
             cls[k] = coop_class_of(w);
             if (cls[k] < 0) return "band wider than 271";
+            continue;
+        }
+        // one long lane-per-job job serialises read_len x (2w + 1) cells on
+        // one lane and becomes the straggler of its class: run it on 16
+        // lanes instead
+        if (coop && w > 16 && (int64_t)read_len[k] * (2 * w + 1) > kLongJobCells &&
+            coop_class_of(w) >= 0) {
+            cls[k] = coop_class_of(w);
             continue;
         }
         int b = lane_class_of(w);
