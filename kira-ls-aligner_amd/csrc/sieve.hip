@@ -335,6 +335,8 @@ __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32
         }
 #pragma unroll
         for (int aa = 0; aa < 4; aa++) {
+            // a partial tile stops at its last shift (129 shifts = 128 + 1)
+            if (!FULL && (uint32_t)aa * 32u >= rem) break;
             uint32_t m = 0;
 #pragma unroll
             for (int k = 0; k < RW; k++) {
