@@ -449,6 +449,109 @@ const char *kira_server_last_error(kira_server *s);
 /* Drains every queued request, then frees the server. */
 void kira_server_destroy(kira_server *s);
 
+/* ------------------------------------------------------------------ stage 4 */
+
+/* Chain (src/types.rs:131-140) without its anchors: contig, reference and
+ * oriented-read span, strand, chain score.  Stage 3 lists a read's chains
+ * best first. */
+typedef struct {
+    uint32_t ref_id, ref_start, ref_end, read_start, read_end;
+    int32_t strand; /* 0 forward, 1 reverse */
+    int32_t score;
+} kira_chain;
+
+/* AlignmentStageConfig (src/pipeline/stage4_alignment.rs:33-54) and the
+ * environment knobs stage 4 reads: tail_ext KIRA_UNGAP_TAIL (1),
+ * ungap_min_id KIRA_UNGAP_MINID (-1 unset), short_dp_topk KIRA_SHORT_DPTOPK
+ * (0 unset), ambig_div KIRA_AMBIG_DIV (5), xs_min_ratio_pct KIRA_XS_MINRATIO
+ * (0), spectral_pct / spectral_floor (router.rs:73-74: 3, 2), cert_strict
+ * KIRA_CERT_STRICT (0).  simd is the SimdMode stage 4 would detect on its
+ * host (it sets the lane-group size; SCALAR sends every job to banded SW).
+ * supplementary runs emit_supplementary_alignments (split_read.rs:22-57). */
+typedef struct {
+    kira_sw_cfg cfg;
+    kira_sw_knobs knobs;
+    float min_chain_ratio;
+    int32_t accept_enable, accept_only_top1, accept_span_slack;
+    float accept_min_identity;
+    int32_t accept_max_mismatches, accept_require_score_margin;
+    int32_t dp_topk, dp_abort_margin, long_read_threshold, max_alignments;
+    int32_t tail_ext, ungap_min_id, short_dp_topk, ambig_div, xs_min_ratio_pct;
+    int32_t spectral_pct, spectral_floor, cert_strict;
+    kira_simd_mode simd;
+    int32_t supplementary;
+} kira_stage4_cfg;
+
+/* Alignment (src/types.rs) as stage 4 leaves it: kind 0 AcceptedUngapped,
+ * 1 DpAligned; CIGAR with soft clips at cigar[cigar_off .. +n_cigar]
+ * (len << 4 | op, M0 I1 D2 N3 S4); MD text (no NUL) at md[md_off .. +md_len];
+ * xs_score valid iff has_xs. */
+typedef struct {
+    int32_t kind;
+    uint32_t ref_id, ref_start, ref_end, read_start, read_end;
+    int32_t score, mapq, is_rev, is_supplementary;
+    uint32_t nm;
+    int32_t has_xs, xs_score;
+    int32_t n_cigar, md_len;
+    int64_t cigar_off, md_off;
+} kira_stage4_aln;
+
+/* AlignmentBatchStats counters (stage4_alignment.rs:57-100) the driver
+ * fills; reason_counts = prefilter_reason_counts (PrefilterReason::idx). */
+typedef struct {
+    int64_t reads, chains_total, chains_used, exact_matches;
+    int64_t prefilter_accept, prefilter_reject, prefilter_fallback;
+    int64_t reason_counts[10];
+    int64_t gpu_spectral_resolved, dp_simd, dp_scalar, dp_early_abort, dp_reads;
+    int64_t supplementary, unmapped;
+} kira_stage4_stats;
+
+typedef struct kira_stage4_result kira_stage4_result;
+
+/* Read-only view of a result, valid until kira_stage4_result_free:
+ * read k's alignments are alns[aln_off[k] .. aln_off[k+1]) in the order
+ * stage 4 appends them.  phase_ms: host wall time of chain triage (exact
+ * match + ungapped prefilter), sieve + certification, lane-group SW, banded
+ * SW, and records + supplementary alignments. */
+typedef struct {
+    int32_t n_reads;
+    const int64_t *aln_off;
+    const kira_stage4_aln *alns;
+    const uint32_t *cigar;
+    const char *md;
+    int64_t n_alns, cigar_words, md_bytes;
+    kira_stage4_stats stats;
+    double phase_ms[5];
+} kira_stage4_view;
+
+/*
+ * stage4_alignment::run (src/pipeline/stage4_alignment.rs:959-1268, with
+ * process_read_prefilter :535-957) with the GPU dispatcher active, for
+ * single-end reads and KIRA_TWOTIER=0 (the two-tier candidate reorder,
+ * :346-451, runs the CPU WFA/Myers fast path and is not supported).  Per
+ * read: chain selection (min_chain_ratio, top-k), exact-match shortcut,
+ * ungapped prefilter; fallback chains of reads of <= 150 bases become lane
+ * jobs on build_simd_window windows (:1426-1445), dispatched to the 2-bit
+ * sieve with certification (:1272-1358), the rest to align_batch_simd lane
+ * groups by (read, window) shape; longer reads go to banded SW on
+ * clamp_window windows (mod.rs:851-877).  Then the chain-score XS proxy
+ * (:1215-1250) and supplementary alignments.  The genome holds every contig
+ * back to back: contig i is bases contig_off[i] .. +contig_len[i].  Read k
+ * is reads[read_off[k] .. +read_len[k]] (forward strand); its chains are
+ * chains[chain_off[k] .. chain_off[k+1]); repeat_min_occ may be NULL (0).
+ * A chain whose contig, read span or reference span is out of range fails
+ * the batch with KIRA_ERR_INVALID_ARGUMENT.  Exact matching, prefilter,
+ * sieve and SW run on the device; records (NM / MD) are host work.
+ */
+kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_contigs,
+                            const int64_t *contig_off, const int64_t *contig_len,
+                            int32_t n_reads, const uint8_t *reads, const int64_t *read_off,
+                            const int32_t *read_len, const int32_t *repeat_min_occ,
+                            const int64_t *chain_off, const kira_chain *chains,
+                            const kira_stage4_cfg *cfg, kira_stage4_result **out);
+kira_status kira_stage4_result_view(const kira_stage4_result *r, kira_stage4_view *view);
+void kira_stage4_result_free(kira_stage4_result *r);
+
 #ifdef __cplusplus
 }
 #endif

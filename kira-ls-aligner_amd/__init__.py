@@ -167,6 +167,55 @@ PREFILTER_OTHER = (None, "read_len0", "non_monotonic", "anchor0", "span0", "neg_
                    "span_oob", "margin_fail")
 
 
+class Chain(C.Structure):
+    """kira_chain = Chain (src/types.rs:131-140) without anchors."""
+
+    _fields_ = [(n, C.c_uint32) for n in ("ref_id", "ref_start", "ref_end", "read_start",
+                                          "read_end")] + [("strand", C.c_int32),
+                                                          ("score", C.c_int32)]
+
+
+class Stage4Cfg(C.Structure):
+    """kira_stage4_cfg = AlignmentStageConfig (src/pipeline/stage4_alignment.rs:33-54)
+    plus the environment knobs stage 4 reads."""
+
+    _fields_ = [("cfg", SwCfg), ("knobs", SwKnobs), ("min_chain_ratio", C.c_float),
+                ("accept_enable", C.c_int32), ("accept_only_top1", C.c_int32),
+                ("accept_span_slack", C.c_int32), ("accept_min_identity", C.c_float)] + [
+        (n, C.c_int32) for n in (
+            "accept_max_mismatches", "accept_require_score_margin", "dp_topk", "dp_abort_margin",
+            "long_read_threshold", "max_alignments", "tail_ext", "ungap_min_id", "short_dp_topk",
+            "ambig_div", "xs_min_ratio_pct", "spectral_pct", "spectral_floor", "cert_strict",
+            "simd", "supplementary")]
+
+
+class Stage4Aln(C.Structure):
+    _fields_ = [("kind", C.c_int32)] + [(n, C.c_uint32) for n in (
+        "ref_id", "ref_start", "ref_end", "read_start", "read_end")] + [
+        (n, C.c_int32) for n in ("score", "mapq", "is_rev", "is_supplementary")] + [
+        ("nm", C.c_uint32), ("has_xs", C.c_int32), ("xs_score", C.c_int32),
+        ("n_cigar", C.c_int32), ("md_len", C.c_int32), ("cigar_off", C.c_int64),
+        ("md_off", C.c_int64)]
+
+
+STAGE4_STATS = ["reads", "chains_total", "chains_used", "exact_matches", "prefilter_accept",
+                "prefilter_reject", "prefilter_fallback"]
+STAGE4_STATS_TAIL = ["gpu_spectral_resolved", "dp_simd", "dp_scalar", "dp_early_abort",
+                     "dp_reads", "supplementary", "unmapped"]
+
+
+class Stage4Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in STAGE4_STATS] + [("reason_counts", C.c_int64 * 10)] + [
+        (n, C.c_int64) for n in STAGE4_STATS_TAIL]
+
+
+class Stage4View(C.Structure):
+    _fields_ = [("n_reads", C.c_int32), ("aln_off", C.POINTER(C.c_int64)),
+                ("alns", C.POINTER(Stage4Aln)), ("cigar", C.POINTER(C.c_uint32)),
+                ("md", C.c_void_p), ("n_alns", C.c_int64), ("cigar_words", C.c_int64),
+                ("md_bytes", C.c_int64), ("stats", Stage4Stats), ("phase_ms", C.c_double * 5)]
+
+
 class SwJobsDev(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
                                           "ref_len", "offset", "band", "abort_score", "res",
@@ -221,6 +270,11 @@ SIGNATURES = {
     "kira_banded_sw_plan_cigar_off": (C.POINTER(C.c_int64), [_P]),
     "kira_banded_sw_run_dev": (C.c_int, [_P, _P, C.POINTER(SwJobsDev), C.POINTER(SwCfg),
                                          C.POINTER(SwKnobs)]),
+    "kira_stage4_run": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P, _P, _P,
+                                  C.POINTER(Chain), C.POINTER(Stage4Cfg),
+                                  C.POINTER(C.c_void_p)]),
+    "kira_stage4_result_view": (C.c_int, [_P, C.POINTER(Stage4View)]),
+    "kira_stage4_result_free": (None, [_P]),
 }
 
 _lib = None
@@ -921,3 +975,107 @@ def build_alignments(jobs, sw_results):
                  md=raw[a.md_off:a.md_off + a.md_len].decode(),
                  cigar=cigar_string(cig[a.cigar_off:a.cigar_off + a.n_cigar]))
             for a in out]
+
+
+# ------------------------------------------------------------ stage 4
+SIMD_MODES = {"scalar": 0, "avx2": 1, "avxvnni": 2, "neon": 3}
+ALN_KINDS = ["accepted_ungapped", "dp_aligned"]
+
+
+def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth=32, xdrop=100,
+               clip_penalty=5, adaptive=True, max_drift=16, dp_xdrop=-1, min_chain_ratio=0.4,
+               accept_enable=True, accept_only_top1=True, accept_span_slack=15,
+               accept_min_identity=98.5, accept_max_mismatches=5, accept_require_score_margin=0,
+               dp_topk=1, dp_abort_margin=20, long_read_threshold=500, max_alignments=1,
+               tail_ext=True, ungap_min_id=-1, short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0,
+               spectral_pct=3, spectral_floor=2, cert_strict=False, simd="avx2",
+               supplementary=True) -> Stage4Cfg:
+    """Stage-4 configuration; defaults are `mem`'s (src/cli/commands/mem.rs:183-205)."""
+    return Stage4Cfg(
+        SwCfg(match_score, mismatch, gap_open, gap_extend, bandwidth, xdrop, clip_penalty),
+        SwKnobs(int(adaptive), max_drift, dp_xdrop), min_chain_ratio, int(accept_enable),
+        int(accept_only_top1), accept_span_slack, accept_min_identity, accept_max_mismatches,
+        accept_require_score_margin, dp_topk, dp_abort_margin, long_read_threshold,
+        max_alignments, int(tail_ext), ungap_min_id, short_dp_topk, ambig_div, xs_min_ratio_pct,
+        spectral_pct, spectral_floor, int(cert_strict), SIMD_MODES[simd], int(supplementary))
+
+
+class Stage4Result:
+    """Owner of a kira_stage4_result; `alignments()` decodes it."""
+
+    def __init__(self, h):
+        self.h = h
+        v = Stage4View()
+        st = lib().kira_stage4_result_view(h, C.byref(v))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_result_view")
+        self.view = v
+        s = v.stats
+        self.stats = {f: getattr(s, f) for f in STAGE4_STATS + STAGE4_STATS_TAIL}
+        self.stats["reason_counts"] = list(s.reason_counts)
+        self.phase_ms = list(v.phase_ms)
+
+    def alignments(self):
+        v = self.view
+        n = v.n_reads
+        off = np.ctypeslib.as_array(v.aln_off, shape=(n + 1,)).copy()
+        cig = np.ctypeslib.as_array(v.cigar, shape=(v.cigar_words,)).copy() \
+            if v.cigar_words else np.zeros(0, np.uint32)
+        md = C.string_at(v.md, v.md_bytes) if v.md_bytes else b""
+        out = []
+        for r in range(n):
+            recs = []
+            for k in range(int(off[r]), int(off[r + 1])):
+                a = v.alns[k]
+                recs.append(dict(
+                    kind=ALN_KINDS[a.kind], ref_id=a.ref_id, ref_start=a.ref_start,
+                    ref_end=a.ref_end, read_start=a.read_start, read_end=a.read_end,
+                    cigar=cigar_string(cig[a.cigar_off:a.cigar_off + a.n_cigar]),
+                    score=a.score, mapq=a.mapq, is_rev=bool(a.is_rev),
+                    is_supplementary=bool(a.is_supplementary), nm=a.nm,
+                    md=md[a.md_off:a.md_off + a.md_len].decode(),
+                    xs_score=a.xs_score if a.has_xs else None))
+            out.append(recs)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().kira_stage4_result_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stage4_run(ctx: Context, genome: Genome, contig_off, contig_len, reads, chains,
+               cfg: Optional[Stage4Cfg] = None, repeat_min_occ=None) -> Stage4Result:
+    """stage4_alignment::run (src/pipeline/stage4_alignment.rs:959-1268) on the device.
+
+    genome holds the contigs back to back (contig i = bases contig_off[i] ..
+    +contig_len[i]); reads: list of bytes (forward strand); chains: per read,
+    a list of (ref_id, ref_start, ref_end, read_start, read_end, strand 0/1,
+    score) in stage-3 order."""
+    cfg = cfg or stage4_cfg()
+    n = len(reads)
+    co = _arr(contig_off, np.int64)
+    cl = _arr(contig_len, np.int64)
+    rl = np.array([len(r) for r in reads], np.int32)
+    ro = np.zeros(n, np.int64)
+    if n > 1:
+        ro[1:] = np.cumsum(rl[:-1], dtype=np.int64)
+    rb = np.frombuffer(b"".join(reads), np.uint8) if rl.sum() else np.zeros(1, np.uint8)
+    coff = np.zeros(n + 1, np.int64)
+    coff[1:] = np.cumsum([len(c) for c in chains], dtype=np.int64)
+    flat = [c for cs in chains for c in cs]
+    ch = (Chain * max(len(flat), 1))(*[Chain(*c) for c in flat])
+    rmo = _arr(repeat_min_occ, np.int32) if repeat_min_occ is not None else None
+    h = C.c_void_p()
+    st = lib().kira_stage4_run(ctx.h, genome.h, len(co), _np_ptr(co), _np_ptr(cl), n,
+                               _np_ptr(rb), _np_ptr(ro), _np_ptr(rl),
+                               _np_ptr(rmo) if rmo is not None else None, _np_ptr(coff), ch,
+                               C.byref(cfg), C.byref(h))
+    ctx.check(st, "stage4_run")
+    return Stage4Result(h)

@@ -210,6 +210,17 @@ struct PrefilterLaunch {
 };
 
 hipError_t launch_prefilter(const PrefilterLaunch &a, hipStream_t s);
+// identity floor of the accept gate (prefilter.rs:255-256)
+uint16_t prefilter_id_floor(const kira_prefilter_params &p);
+
+// Alignment records (host): build_alignment (src/alignment/mod.rs:1200-1254)
+// of one SwResult appends the soft-clipped CIGAR to `cig` and the MD text to
+// `md` and returns NM; ungapped_tags (md.rs:147-151) does the same for an
+// ungapped span of n columns.
+uint32_t build_record(const uint8_t *read, int32_t L, const uint8_t *win, int64_t W,
+                      const kira_sw_result &s, const uint32_t *ops, std::vector<uint32_t> &cig,
+                      std::string &md);
+uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md);
 
 }  // namespace kira
 
@@ -227,4 +238,5 @@ struct kira_genome {
     kira_ctx *ctx = nullptr;
     kira::Planes planes;
     uint8_t *ascii = nullptr;  // base 0 of the ASCII copy; kRefPad zero bytes on each side
+    std::vector<uint8_t> host;  // host copy, for alignment records (NM / MD)
 };

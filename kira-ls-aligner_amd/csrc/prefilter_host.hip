@@ -21,6 +21,19 @@ kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e =
 }
 }  // namespace
 
+namespace kira {
+
+uint16_t prefilter_id_floor(const kira_prefilter_params &p) {
+    // (accept_min_identity * 100.0).round() as u16 (prefilter.rs:255), then
+    // KIRA_UNGAP_MINID overrides it (:256)
+    float f = roundf(p.accept_min_identity * 100.0f);
+    uint16_t v = f != f ? 0 : (f <= 0 ? 0 : (f >= 65535.0f ? 65535 : (uint16_t)f));
+    if (p.ungap_min_id >= 0) v = (uint16_t)p.ungap_min_id;
+    return v;
+}
+
+}  // namespace kira
+
 extern "C" kira_status kira_prefilter_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
                                             int64_t reads_bytes, const int64_t *read_off,
                                             const int32_t *read_len, const uint8_t *refs,
@@ -43,11 +56,7 @@ extern "C" kira_status kira_prefilter_batch(kira_ctx *ctx, int32_t n, const uint
             (int64_t)c.read_start > read_len[k] || (int64_t)c.ref_start > ref_len[k])
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain span outside its read or contig");
     }
-    // (accept_min_identity * 100.0).round() as u16 (prefilter.rs:255), then
-    // KIRA_UNGAP_MINID overrides it (:256)
-    float f = roundf(prm->accept_min_identity * 100.0f);
-    uint16_t floor_ = f != f ? 0 : (f <= 0 ? 0 : (f >= 65535.0f ? 65535 : (uint16_t)f));
-    if (prm->ungap_min_id >= 0) floor_ = (uint16_t)prm->ungap_min_id;
+    const uint16_t floor_ = kira::prefilter_id_floor(*prm);
     uint8_t *d_reads = nullptr, *d_refs = nullptr, *d_top = nullptr;
     int64_t *d_ro = nullptr, *d_fo = nullptr, *d_fl = nullptr;
     int32_t *d_rl = nullptr, *d_margin = nullptr;

@@ -81,6 +81,34 @@ void replay(const uint8_t *read, size_t read_len, const uint8_t *ref, size_t ref
 
 }  // namespace
 
+namespace kira {
+
+uint32_t build_record(const uint8_t *read, int32_t L, const uint8_t *win, int64_t W,
+                      const kira_sw_result &s, const uint32_t *ops, std::vector<uint32_t> &cig,
+                      std::string &md_out) {
+    const size_t c0 = cig.size();
+    if (s.read_start > 0) cig.push_back(((uint32_t)s.read_start << 4) | kOpS);
+    cig.insert(cig.end(), ops, ops + s.n_cigar);
+    if (s.read_end < L) cig.push_back(((uint32_t)(L - s.read_end) << 4) | kOpS);
+    MdText md;
+    // the CIGAR carries the leading clip, so the replay starts at read
+    // position 0 (mod.rs:1231-1233)
+    replay(read, (size_t)L, win, (size_t)W, s.ref_start, cig.data() + c0,
+           (int32_t)(cig.size() - c0), md);
+    md_out += md.text;
+    return md.nm;
+}
+
+uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md_out) {
+    MdText md;
+    for (int64_t k = 0; k < n; k++) md.column(read[k], ref[k]);
+    md.flush();
+    md_out += md.text;
+    return md.nm;
+}
+
+}  // namespace kira
+
 extern "C" kira_status kira_build_alignments(
     int32_t n, const uint8_t *reads, const int64_t *read_off, const int32_t *read_len,
     const uint8_t *windows, const int64_t *win_off, const int32_t *win_len,
@@ -93,21 +121,17 @@ extern "C" kira_status kira_build_alignments(
     int64_t cpos = 0, mpos = 0;
     bool fits = true;
     std::vector<uint32_t> cig;
+    std::string mdt;
     for (int32_t k = 0; k < n; k++) {
         const kira_sw_result &s = sw[k];
         const int32_t L = read_len[k];
         if (s.n_cigar < 0 || s.read_start < 0 || s.read_end > L || s.read_start > s.read_end)
             return KIRA_ERR_INVALID_ARGUMENT;
         cig.clear();
-        if (s.read_start > 0) cig.push_back(((uint32_t)s.read_start << 4) | kOpS);
-        cig.insert(cig.end(), sw_cigar + s.cigar_off, sw_cigar + s.cigar_off + s.n_cigar);
-        if (s.read_end < L) cig.push_back(((uint32_t)(L - s.read_end) << 4) | kOpS);
-        MdText md;
-        // the CIGAR carries the leading clip, so the replay starts at read
-        // position 0 (mod.rs:1231-1233)
-        replay(reads + read_off[k], (size_t)L, windows + win_off[k], (size_t)win_len[k],
-               s.ref_start, cig.data(), (int32_t)cig.size(), md);
-        const int64_t nc = (int64_t)cig.size(), nmd = (int64_t)md.text.size();
+        mdt.clear();
+        const uint32_t nm = kira::build_record(reads + read_off[k], L, windows + win_off[k],
+                                               win_len[k], s, sw_cigar + s.cigar_off, cig, mdt);
+        const int64_t nc = (int64_t)cig.size(), nmd = (int64_t)mdt.size();
         if (out) {
             kira_alignment &a = out[k];
             a.ref_start = win_start[k] + s.ref_start;
@@ -115,7 +139,7 @@ extern "C" kira_status kira_build_alignments(
             a.read_start = (uint32_t)s.read_start;
             a.read_end = (uint32_t)s.read_end;
             a.score = s.score;
-            a.nm = md.nm;
+            a.nm = nm;
             a.n_cigar = (int32_t)nc;
             a.md_len = (int32_t)nmd;
             a.cigar_off = cpos;
@@ -126,7 +150,7 @@ extern "C" kira_status kira_build_alignments(
         else if (cigar_out)
             fits = false;
         if (md_out && mpos + nmd <= md_cap)
-            memcpy(md_out + mpos, md.text.data(), (size_t)nmd);
+            memcpy(md_out + mpos, mdt.data(), (size_t)nmd);
         else if (md_out)
             fits = false;
         cpos += nc;

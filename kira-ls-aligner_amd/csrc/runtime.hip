@@ -279,10 +279,18 @@ kira_status kira_genome_create(kira_ctx *ctx, const uint8_t *seq, int64_t len,
         return fail(ctx, KIRA_ERR_RUNTIME, "genome packing", e);
     }
     kira_genome *g = new (std::nothrow) kira_genome();
+    if (g) {
+        try {
+            g->host.assign(seq, seq + len);
+        } catch (const std::bad_alloc &) {
+            delete g;
+            g = nullptr;
+        }
+    }
     if (!g) {
         (void)hipFree(mem);
         (void)hipFree(d_seq);
-        return KIRA_ERR_OUT_OF_MEMORY;
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "genome host copy");
     }
     g->ctx = ctx;
     g->planes = p;

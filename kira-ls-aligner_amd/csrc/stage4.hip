@@ -1,0 +1,883 @@
+// Stage-4 alignment driver: src/pipeline/stage4_alignment.rs run() (:959-1268)
+// with process_read_prefilter (:535-957), the GPU dispatcher active,
+// single-end reads and the two-tier candidate reorder off.  Chain selection,
+// job routing and the record text (NM / MD) are host work; exact matching,
+// the ungapped prefilter, the 2-bit sieve with certification, the lane-group
+// SW and the banded SW run on the device.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+
+struct kira_stage4_result {
+    std::vector<int64_t> aln_off;
+    std::vector<kira_stage4_aln> alns;
+    std::vector<uint32_t> cigar;
+    std::string md;
+    kira_stage4_stats stats{};
+    double phase_ms[5] = {};
+};
+
+namespace {
+
+constexpr uint32_t kOpM = 0, kOpI = 1, kOpS = 4;
+
+// split_read.rs:10-19
+constexpr float kSuppMinScoreRatio = 0.5f;
+constexpr uint32_t kSuppMinReadSpan = 50;
+constexpr uint32_t kSuppMaxOverlapPct = 50;
+constexpr size_t kMaxSuppPerRead = 4;
+
+// exact_match_exists (mod.rs:818-832) on candidate chains whose spans
+// already cover the whole read: one wave per chain compares the oriented
+// read with the contig bytes, 64 at a time.
+__global__ void exact_match_kernel(int32_t n, const uint8_t *__restrict__ reads,
+                                   const int64_t *__restrict__ read_off,
+                                   const int32_t *__restrict__ read_len,
+                                   const uint8_t *__restrict__ genome,
+                                   const int64_t *__restrict__ ref_pos,
+                                   uint8_t *__restrict__ out) {
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= n) return;  // whole wave
+    const uint8_t *q = reads + read_off[w];
+    const uint8_t *r = genome + ref_pos[w];
+    const int32_t L = read_len[w];
+    bool equal = true;
+    for (int32_t base = 0; base < L; base += 64) {  // wave-uniform trip count
+        const int32_t i = base + lane;
+        const bool d = i < L && q[i] != r[i];
+        if (__any(d)) {
+            equal = false;
+            break;
+        }
+    }
+    if (lane == 0) out[w] = equal ? 1 : 0;
+}
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e = hipSuccess) {
+    if (ctx) {
+        char buf[512];
+        if (e != hipSuccess)
+            snprintf(buf, sizeof buf, "stage4: %s: %s", what, hipGetErrorString(e));
+        else
+            snprintf(buf, sizeof buf, "stage4: %s", what);
+        ctx->err = buf;
+    }
+    return st;
+}
+
+// device allocations released together
+struct DevMem {
+    std::vector<void *> v;
+    ~DevMem() {
+        for (void *x : v) (void)hipFree(x);
+    }
+    template <class T>
+    hipError_t alloc(T **p, size_t count) {
+        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count * sizeof(T), 1));
+        if (e == hipSuccess) v.push_back((void *)*p);
+        return e;
+    }
+};
+
+int32_t sat_sub(int32_t a, int32_t b) {
+    int64_t v = (int64_t)a - b;
+    return (int32_t)std::min<int64_t>(INT32_MAX, std::max<int64_t>(INT32_MIN, v));
+}
+int32_t wrap_i32(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
+// `x as i32` of a float (saturating, NaN -> 0)
+int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483647.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+uint32_t usub(uint32_t a, uint32_t b) { return a > b ? a - b : 0; }
+
+// identical_base_is_edit (md.rs:134-137)
+bool identical_is_edit(uint8_t b) {
+    uint8_t u = (b >= 'a' && b <= 'z') ? (uint8_t)(b - 32) : b;
+    return !(u == 'A' || u == 'C' || u == 'G' || u == 'T' || u == '=');
+}
+
+// src/seq.rs:1-17
+uint8_t comp_base(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 'T';
+    case 'C': case 'c': return 'G';
+    case 'G': case 'g': return 'C';
+    case 'T': case 't': return 'A';
+    default: return 'N';
+    }
+}
+
+// original_interval / read_overlap_pct (split_read.rs:128-148)
+std::pair<uint32_t, uint32_t> original_interval(uint32_t s, uint32_t e, bool rev, uint32_t L) {
+    return rev ? std::make_pair(usub(L, e), usub(L, s)) : std::make_pair(s, e);
+}
+uint32_t read_overlap_pct(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
+    uint32_t al = usub(a1, a0), bl = usub(b1, b0);
+    if (al == 0 || bl == 0) return 0;
+    uint32_t s = std::max(a0, b0), e = std::min(a1, b1);
+    if (e <= s) return 0;
+    uint64_t ov = (uint64_t)(e - s) * 100u;
+    return (uint32_t)std::min<uint64_t>(ov, 0xFFFFFFFFu) / al;
+}
+
+struct Rec {
+    int32_t read;
+    kira_stage4_aln a;
+};
+
+// a chain sent to the prefilter (and the exact-match check)
+struct Sel {
+    int32_t read;
+    int64_t chain;
+    uint8_t top1;
+    int32_t margin;
+};
+
+// an SW job: oriented read, window [win, win + wlen) in contig coordinates
+struct Job {
+    int32_t read;
+    int64_t chain;
+    int64_t win;
+    int32_t wlen;
+    int32_t abort;
+};
+
+struct Driver {
+    kira_ctx *ctx;
+    const kira_genome *g;
+    const int64_t *contig_off, *contig_len;
+    int32_t n;
+    const uint8_t *reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const int64_t *chain_off;
+    const kira_chain *chains;
+    const kira_stage4_cfg &c;
+    kira_stage4_result &R;
+    hipStream_t s;
+
+    // forward and reverse-complement copy of every read, back to back
+    std::vector<uint8_t> orient;
+    std::vector<int64_t> ooff;  // read k: forward at ooff[k], reverse at ooff[k] + L
+    uint8_t *d_orient = nullptr;
+    DevMem mem;
+    std::vector<Rec> recs;
+    const int32_t *rmo = nullptr;  // ReadRecord::repeat_min_occ per read, may be NULL
+
+    int32_t rmo_of(int32_t r) const { return rmo ? rmo[r] : 0; }
+
+    int64_t oriented_off(int32_t r, bool rev) const { return ooff[r] + (rev ? read_len[r] : 0); }
+    const uint8_t *oriented(int32_t r, bool rev) const {
+        return orient.data() + oriented_off(r, rev);
+    }
+    const uint8_t *contig(uint32_t id) const { return g->host.data() + contig_off[id]; }
+
+    void push(int32_t read, const kira_stage4_aln &a) { recs.push_back(Rec{read, a}); }
+
+    kira_stage4_aln base_aln(int32_t kind, const kira_chain &ch) const {
+        kira_stage4_aln a{};
+        a.kind = kind;
+        a.ref_id = ch.ref_id;
+        a.is_rev = ch.strand == 1;
+        return a;
+    }
+
+    // exact_match_alignment (mod.rs:778-815)
+    kira_stage4_aln exact_aln(int32_t r, const kira_chain &ch) {
+        const int32_t L = read_len[r];
+        const uint8_t *q = oriented(r, ch.strand == 1);
+        kira_stage4_aln a = base_aln(0, ch);
+        a.ref_start = ch.ref_start;
+        a.ref_end = ch.ref_start + (uint32_t)L;
+        a.read_start = 0;
+        a.read_end = (uint32_t)L;
+        a.score = wrap_i32((int64_t)c.cfg.match_score * L);
+        a.mapq = 0;
+        uint32_t nm = 0;
+        for (int32_t k = 0; k < L; k++) nm += identical_is_edit(q[k]) ? 1 : 0;
+        a.nm = nm;
+        a.cigar_off = (int64_t)R.cigar.size();
+        a.n_cigar = 1;
+        R.cigar.push_back(((uint32_t)L << 4) | kOpM);
+        a.md_off = (int64_t)R.md.size();
+        R.md += std::to_string(L);
+        a.md_len = (int32_t)(R.md.size() - a.md_off);
+        return a;
+    }
+
+    // build_alignment (mod.rs:1200-1254) of an SW result over window
+    // [win, win + wlen) of the chain's contig
+    kira_stage4_aln sw_aln(int32_t r, const kira_chain &ch, int64_t win, int32_t wlen,
+                           const kira_sw_result &sw, const uint32_t *ops) {
+        kira_stage4_aln a = base_aln(1, ch);
+        const int32_t L = read_len[r];
+        a.ref_start = (uint32_t)win + sw.ref_start;
+        a.ref_end = (uint32_t)win + sw.ref_end;
+        a.read_start = (uint32_t)sw.read_start;
+        a.read_end = (uint32_t)sw.read_end;
+        a.score = sw.score;
+        a.mapq = 0;
+        a.cigar_off = (int64_t)R.cigar.size();
+        a.md_off = (int64_t)R.md.size();
+        a.nm = kira::build_record(oriented(r, ch.strand == 1), L, contig(ch.ref_id) + win, wlen,
+                                  sw, ops, R.cigar, R.md);
+        a.n_cigar = (int32_t)(R.cigar.size() - a.cigar_off);
+        a.md_len = (int32_t)(R.md.size() - a.md_off);
+        return a;
+    }
+
+    bool exact_candidate(int32_t r, const kira_chain &ch) const {
+        const uint32_t L = (uint32_t)read_len[r];
+        return ch.read_start == 0 && ch.read_end == L && ch.ref_end >= ch.ref_start &&
+               ch.ref_end - ch.ref_start == L &&
+               (int64_t)ch.ref_start + L <= contig_len[ch.ref_id];
+    }
+
+    // Banded SW (align_chain_with_meta's DP, mod.rs:865-883) on clamp_window
+    // windows (:1766-1771), on the device; appends one record per job.
+    kira_status banded(const std::vector<Job> &jobs, bool supplementary, bool count) {
+        const int32_t m = (int32_t)jobs.size();
+        if (m == 0) return KIRA_OK;
+        std::vector<int64_t> ro(m), fo(m);
+        std::vector<int32_t> rl(m), fl(m), off(m), ab(m);
+        for (int32_t t = 0; t < m; t++) {
+            const Job &j = jobs[t];
+            const kira_chain &ch = chains[j.chain];
+            ro[t] = oriented_off(j.read, ch.strand == 1);
+            rl[t] = read_len[j.read];
+            fo[t] = contig_off[ch.ref_id] + j.win;
+            fl[t] = j.wlen;
+            // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
+            off[t] = wrap_i32((int64_t)(int32_t)ch.ref_start - (int32_t)(uint32_t)j.win -
+                              (int32_t)ch.read_start);
+            ab[t] = j.abort;
+        }
+        kira_sw_plan *plan = nullptr;
+        kira_status st = kira_banded_sw_plan_create(ctx, m, rl.data(), nullptr, &c.cfg, &plan);
+        if (st != KIRA_OK) return st;
+        struct PlanGuard {
+            kira_sw_plan *p;
+            ~PlanGuard() { kira_banded_sw_plan_destroy(p); }
+        } pg{plan};
+        DevMem dm;
+        int64_t *d_ro, *d_fo;
+        int32_t *d_rl, *d_fl, *d_off, *d_ab, *d_res;
+        uint32_t *d_cig;
+        const int64_t words = kira_banded_sw_plan_cigar_words(plan);
+        hipError_t e;
+        if ((e = dm.alloc(&d_ro, m)) || (e = dm.alloc(&d_fo, m)) || (e = dm.alloc(&d_rl, m)) ||
+            (e = dm.alloc(&d_fl, m)) || (e = dm.alloc(&d_off, m)) || (e = dm.alloc(&d_ab, m)) ||
+            (e = dm.alloc(&d_res, (size_t)m * kira::kSwResWords)) ||
+            (e = dm.alloc(&d_cig, (size_t)words)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
+        if ((e = hipMemcpyAsync(d_ro, ro.data(), m * 8, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_fo, fo.data(), m * 8, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_rl, rl.data(), m * 4, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_fl, fl.data(), m * 4, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_off, off.data(), m * 4, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_ab, ab.data(), m * 4, hipMemcpyHostToDevice, s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+        kira_sw_jobs_dev jd{d_orient, d_ro, d_rl, g->ascii, d_fo, d_fl, d_off,
+                            nullptr,  d_ab, d_res, d_cig};
+        if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, &c.cfg, &c.knobs)) != KIRA_OK) return st;
+        std::vector<int32_t> res((size_t)m * kira::kSwResWords);
+        std::vector<uint32_t> cig((size_t)words);
+        if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
+            (words && (e = hipMemcpyAsync(cig.data(), d_cig, cig.size() * 4,
+                                          hipMemcpyDeviceToHost, s))) ||
+            (e = hipStreamSynchronize(s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
+        const int64_t *coff = kira_banded_sw_plan_cigar_off(plan);
+        for (int32_t t = 0; t < m; t++) {
+            const int32_t *w = &res[(size_t)t * kira::kSwResWords];
+            if (w[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
+            if (w[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
+            kira_sw_result sw{};
+            sw.score = w[0];
+            sw.read_end = w[1];
+            sw.ref_end = (uint32_t)w[2];
+            sw.early_abort = w[3];
+            sw.ref_start = (uint32_t)w[4];
+            sw.read_start = w[5];
+            sw.n_cigar = w[6];
+            const Job &j = jobs[t];
+            kira_stage4_aln a =
+                sw_aln(j.read, chains[j.chain], j.win, j.wlen, sw, cig.data() + coff[t]);
+            a.is_supplementary = supplementary ? 1 : 0;
+            push(j.read, a);
+            if (count) {
+                R.stats.dp_scalar++;
+                if (sw.early_abort) R.stats.dp_early_abort++;
+            }
+        }
+        return KIRA_OK;
+    }
+
+    // clamp_window (mod.rs:1766-1771) job of a chain
+    Job clamp_job(int32_t r, int64_t chain, int32_t abort) const {
+        const kira_chain &ch = chains[chain];
+        const uint32_t ref_len = (uint32_t)contig_len[ch.ref_id];
+        const uint32_t flank = std::max<uint32_t>((uint32_t)std::max(c.cfg.bandwidth, 1) * 2u, 50u);
+        const uint32_t start = usub(ch.ref_start, flank);
+        uint32_t end = std::min<uint32_t>(ch.ref_end + flank, ref_len);
+        end = std::max(end, start + 1);
+        return Job{r, chain, (int64_t)start, (int32_t)(end - start), abort};
+    }
+
+    kira_status run();
+};
+
+kira_status Driver::run() {
+    kira_stage4_stats &st = R.stats;
+    st.reads = n;
+    const int32_t lanes = c.simd == KIRA_SIMD_AVXVNNI ? 32
+                          : c.simd == KIRA_SIMD_AVX2  ? 16
+                          : c.simd == KIRA_SIMD_NEON  ? 4
+                                                      : 1;
+    const int32_t ambig_div = c.ambig_div >= 1 ? c.ambig_div : 5;
+    auto t0 = Clock::now();
+
+    // oriented reads
+    int64_t total = 0;
+    ooff.resize(n);
+    for (int32_t r = 0; r < n; r++) {
+        ooff[r] = total;
+        total += 2 * (int64_t)read_len[r];
+    }
+    orient.resize((size_t)total);
+    for (int32_t r = 0; r < n; r++) {
+        const uint8_t *src = reads + read_off[r];
+        uint8_t *f = orient.data() + ooff[r];
+        const int32_t L = read_len[r];
+        memcpy(f, src, (size_t)L);
+        for (int32_t k = 0; k < L; k++) f[L + k] = comp_base(src[L - 1 - k]);
+    }
+
+    // chain selection (:549-708, two-tier and mate guidance off)
+    std::vector<Sel> sels;
+    std::vector<int32_t> chain_best(n, 0), chain_second(n, 0);
+    for (int32_t r = 0; r < n; r++) {
+        const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+        st.chains_total += nc;
+        if (nc == 0) continue;
+        const kira_chain *cl = chains + c0;
+        const int32_t best = cl[0].score;
+        const int32_t min_score = best > 0 ? f2i(ceilf((float)best * c.min_chain_ratio)) : best;
+        const int32_t second = nc > 1 ? cl[1].score : 0;
+        const int32_t margin = nc > 1 ? wrap_i32((int64_t)best - second) : INT32_MAX;
+        chain_best[r] = best;
+        chain_second[r] = second;
+        const bool short_read = read_len[r] <= 300;
+        const bool ambiguous =
+            nc > 1 && (margin <= std::max(best / ambig_div, 8) || rmo_of(r) > 1);
+        const int32_t topk = short_read
+                                 ? (c.short_dp_topk >= 1 ? c.short_dp_topk : (ambiguous ? 2 : 1))
+                                 : std::max(c.dp_topk, 1);
+        int32_t selected = 0;
+        for (int64_t k = 0; k < nc; k++) {
+            if (cl[k].score < min_score) continue;
+            if (selected >= topk) break;
+            sels.push_back(Sel{r, c0 + k, (uint8_t)(selected == 0), margin});
+            selected++;
+        }
+    }
+    st.chains_used = (int64_t)sels.size();
+
+    // exact-match check and prefilter of every selected chain, on the device
+    const int32_t ns = (int32_t)sels.size();
+    std::vector<uint8_t> exact(ns, 0);
+    std::vector<kira_prefilter_out> pf(ns);
+    hipError_t e;
+    if ((e = mem.alloc(&d_orient, (size_t)total + 8)) ||
+        (total && (e = hipMemcpyAsync(d_orient, orient.data(), (size_t)total,
+                                      hipMemcpyHostToDevice, s))))
+        return fail(ctx, e == hipErrorOutOfMemory ? KIRA_ERR_OUT_OF_MEMORY : KIRA_ERR_RUNTIME,
+                    "read upload", e);
+    if (ns > 0) {
+        std::vector<int64_t> ro(ns), fo(ns), fl(ns);
+        std::vector<int32_t> rl(ns), mg(ns);
+        std::vector<kira_span> sp(ns);
+        std::vector<uint8_t> top(ns);
+        std::vector<int64_t> ex_ro, ex_pos;
+        std::vector<int32_t> ex_rl, ex_sel;
+        for (int32_t k = 0; k < ns; k++) {
+            const Sel &x = sels[k];
+            const kira_chain &ch = chains[x.chain];
+            ro[k] = oriented_off(x.read, ch.strand == 1);
+            rl[k] = read_len[x.read];
+            fo[k] = contig_off[ch.ref_id];
+            fl[k] = contig_len[ch.ref_id];
+            sp[k] = kira_span{ch.ref_start, ch.ref_end, ch.read_start, ch.read_end};
+            top[k] = x.top1;
+            mg[k] = x.margin;
+            if (exact_candidate(x.read, ch)) {
+                ex_sel.push_back(k);
+                ex_ro.push_back(ro[k]);
+                ex_rl.push_back(rl[k]);
+                ex_pos.push_back(fo[k] + ch.ref_start);
+            }
+        }
+        const int32_t ne = (int32_t)ex_sel.size();
+        DevMem dm;
+        int64_t *d_ro, *d_fo, *d_fl, *d_ero = nullptr, *d_epos = nullptr;
+        int32_t *d_rl, *d_mg, *d_erl = nullptr;
+        kira_span *d_sp;
+        uint8_t *d_top, *d_ex = nullptr;
+        kira_prefilter_out *d_out;
+        if ((e = dm.alloc(&d_ro, ns)) || (e = dm.alloc(&d_fo, ns)) || (e = dm.alloc(&d_fl, ns)) ||
+            (e = dm.alloc(&d_rl, ns)) || (e = dm.alloc(&d_mg, ns)) || (e = dm.alloc(&d_sp, ns)) ||
+            (e = dm.alloc(&d_top, ns)) || (e = dm.alloc(&d_out, ns)) ||
+            (ne && ((e = dm.alloc(&d_ero, ne)) || (e = dm.alloc(&d_epos, ne)) ||
+                    (e = dm.alloc(&d_erl, ne)) || (e = dm.alloc(&d_ex, ne)))))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "triage allocation", e);
+        auto up = [&](void *d, const void *h, size_t bytes) {
+            return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+        };
+        if ((e = up(d_ro, ro.data(), ns * 8)) || (e = up(d_fo, fo.data(), ns * 8)) ||
+            (e = up(d_fl, fl.data(), ns * 8)) || (e = up(d_rl, rl.data(), ns * 4)) ||
+            (e = up(d_mg, mg.data(), ns * 4)) ||
+            (e = up(d_sp, sp.data(), ns * sizeof(kira_span))) || (e = up(d_top, top.data(), ns)) ||
+            (ne && ((e = up(d_ero, ex_ro.data(), ne * 8)) || (e = up(d_epos, ex_pos.data(), ne * 8)) ||
+                    (e = up(d_erl, ex_rl.data(), ne * 4)))))
+            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+        if (ne) {
+            exact_match_kernel<<<(unsigned)((ne + 3) / 4), 256, 0, s>>>(ne, d_orient, d_ero, d_erl,
+                                                                       g->ascii, d_epos, d_ex);
+            if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "exact-match launch", e);
+        }
+        kira::PrefilterLaunch a{};
+        a.n = ns;
+        a.reads = d_orient;
+        a.read_off = d_ro;
+        a.read_len = d_rl;
+        a.refs = g->ascii;
+        a.ref_off = d_fo;
+        a.ref_len = d_fl;
+        a.spans = d_sp;
+        a.is_top1 = d_top;
+        a.score_margin = d_mg;
+        a.match_score = c.cfg.match_score;
+        a.mismatch = c.cfg.mismatch;
+        a.xdrop = c.cfg.xdrop;
+        a.p.accept_enable = (c.accept_enable && c.max_alignments <= 1) ? 1 : 0;
+        a.p.accept_only_top1 = c.accept_only_top1;
+        a.p.multi_alignments_enabled = c.max_alignments > 1 ? 1 : 0;
+        a.p.tail_ext = c.tail_ext;
+        a.p.accept_span_slack = c.accept_span_slack;
+        a.p.accept_max_mismatches = c.accept_max_mismatches;
+        a.p.accept_require_score_margin = c.accept_require_score_margin;
+        a.p.long_read_threshold = c.long_read_threshold;
+        a.p.accept_min_identity = c.accept_min_identity;
+        a.p.ungap_min_id = c.ungap_min_id;
+        a.id_floor = kira::prefilter_id_floor(a.p);
+        a.out = d_out;
+        if ((e = kira::launch_prefilter(a, s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "prefilter launch", e);
+        std::vector<uint8_t> ex(ne);
+        if ((e = hipMemcpyAsync(pf.data(), d_out, ns * sizeof(kira_prefilter_out),
+                                hipMemcpyDeviceToHost, s)) ||
+            (ne && (e = hipMemcpyAsync(ex.data(), d_ex, ne, hipMemcpyDeviceToHost, s))) ||
+            (e = hipStreamSynchronize(s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
+        for (int32_t k = 0; k < ne; k++) exact[ex_sel[k]] = ex[k];
+    }
+
+    // outcomes in chain order (:758-934)
+    std::vector<Job> simd, scalar;
+    std::vector<uint8_t> dp_used(n, 0);
+    const int32_t band = std::max(c.cfg.bandwidth, 1);
+    for (int32_t k = 0; k < ns; k++) {
+        const Sel &x = sels[k];
+        const kira_chain &ch = chains[x.chain];
+        const int32_t L = read_len[x.read];
+        if (exact[k]) {
+            st.exact_matches++;
+            push(x.read, exact_aln(x.read, ch));
+            continue;
+        }
+        const kira_prefilter_out &o = pf[k];
+        st.reason_counts[o.reason < 9 ? o.reason : 9]++;
+        const int32_t abort = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
+        if (o.result == 0) {
+            // build_ungapped_alignment (prefilter.rs:455-517)
+            st.prefilter_accept++;
+            kira_stage4_aln a = base_aln(0, ch);
+            a.ref_start = o.aln_ref_start;
+            a.ref_end = o.aln_ref_end;
+            a.read_start = o.aln_read_start;
+            a.read_end = o.aln_read_end;
+            a.score = o.aln_score;
+            a.mapq = 60;
+            a.cigar_off = (int64_t)R.cigar.size();
+            a.n_cigar = o.n_cigar;
+            R.cigar.insert(R.cigar.end(), o.cigar, o.cigar + o.n_cigar);
+            a.md_off = (int64_t)R.md.size();
+            a.nm = kira::ungapped_tags(oriented(x.read, ch.strand == 1) + o.aln_read_start,
+                                       contig(ch.ref_id) + o.aln_ref_start,
+                                       (int64_t)o.aln_read_end - o.aln_read_start, R.md);
+            a.md_len = (int32_t)(R.md.size() - a.md_off);
+            push(x.read, a);
+            continue;
+        }
+        if (o.result == 1) {
+            st.prefilter_reject++;
+            continue;
+        }
+        st.prefilter_fallback++;
+        dp_used[x.read] = 1;
+        const int bucket = L <= 120 ? 0 : (L <= 150 ? 1 : 2);
+        if (bucket < 2 && lanes > 1) {
+            // build_simd_window (:1426-1445)
+            const int32_t flank = band * 2;
+            const int64_t wlen = (int64_t)L + 2 * (int64_t)flank;
+            const int32_t desired = wrap_i32((int64_t)(int32_t)ch.ref_start -
+                                             (int32_t)ch.read_start - flank);
+            if (desired >= 0 && desired + wlen <= contig_len[ch.ref_id]) {
+                simd.push_back(Job{x.read, x.chain, desired, (int32_t)wlen, abort});
+                continue;
+            }
+        }
+        scalar.push_back(clamp_job(x.read, x.chain, abort));
+    }
+    for (int32_t r = 0; r < n; r++) st.dp_reads += dp_used[r];
+    R.phase_ms[0] = ms_since(t0);
+
+    // gpu_dispatch_spectral (:1272-1358): sieve + certification
+    t0 = Clock::now();
+    std::vector<uint8_t> keep(simd.size(), 1);
+    const int32_t nj = (int32_t)simd.size();
+    if (nj > 0) {
+        std::vector<int64_t> ro(nj), ws(nj);
+        std::vector<int32_t> rl(nj), wl(nj), lim(nj);
+        int32_t max_len = 0;
+        for (int32_t t = 0; t < nj; t++) {
+            const Job &j = simd[t];
+            const kira_chain &ch = chains[j.chain];
+            ro[t] = oriented_off(j.read, ch.strand == 1);
+            rl[t] = read_len[j.read];
+            ws[t] = contig_off[ch.ref_id] + j.win;
+            wl[t] = j.wlen;
+            // spectral_max_mismatches (router.rs:191-195)
+            lim[t] = std::max<int32_t>((int32_t)((int64_t)rl[t] * c.spectral_pct / 100),
+                                       c.spectral_floor);
+            max_len = std::max(max_len, rl[t]);
+        }
+        DevMem dm;
+        int64_t *d_ro, *d_ws;
+        int32_t *d_rl, *d_wl, *d_lim, *d_shift, *d_best, *d_second;
+        uint32_t *d_planes;
+        uint8_t *d_valid, *d_flags;
+        if ((e = dm.alloc(&d_ro, nj)) || (e = dm.alloc(&d_ws, nj)) || (e = dm.alloc(&d_rl, nj)) ||
+            (e = dm.alloc(&d_wl, nj)) || (e = dm.alloc(&d_lim, nj)) ||
+            (e = dm.alloc(&d_shift, nj)) || (e = dm.alloc(&d_best, nj)) ||
+            (e = dm.alloc(&d_second, nj)) || (e = dm.alloc(&d_planes, (size_t)nj * 24)) ||
+            (e = dm.alloc(&d_valid, nj)) || (e = dm.alloc(&d_flags, nj)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "sieve allocation", e);
+        auto up = [&](void *d, const void *h, size_t bytes) {
+            return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+        };
+        if ((e = up(d_ro, ro.data(), nj * 8)) || (e = up(d_ws, ws.data(), nj * 8)) ||
+            (e = up(d_rl, rl.data(), nj * 4)) || (e = up(d_wl, wl.data(), nj * 4)) ||
+            (e = up(d_lim, lim.data(), nj * 4)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+        if ((e = kira::launch_pack_reads_ascii(nj, d_orient, 0, d_ro, d_rl, d_planes, d_valid, s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "read packing", e);
+        kira_sieve_batch b{};
+        b.n_jobs = nj;
+        b.max_read_len = max_len;
+        b.read_planes = d_planes;
+        b.read_lens = d_rl;
+        b.read_valid = d_valid;
+        b.job_read = nullptr;
+        b.jobs_per_read = 1;
+        b.win_start = d_ws;
+        b.win_len = d_wl;
+        b.limit = d_lim;
+        b.out_shift = d_shift;
+        b.out_best = d_best;
+        b.out_second = d_second;
+        b.out_flags = d_flags;
+        // the certification limit is the job's own dispatch limit
+        b.cert_max_mism = INT32_MAX;
+        b.cert_strict = c.cert_strict;
+        kira_status kst = kira_sieve_genome_dev(ctx, g, &b);
+        if (kst != KIRA_OK) return kst;
+        std::vector<int32_t> shift(nj), best(nj);
+        std::vector<uint8_t> flags(nj);
+        if ((e = hipMemcpyAsync(shift.data(), d_shift, nj * 4, hipMemcpyDeviceToHost, s)) ||
+            (e = hipMemcpyAsync(best.data(), d_best, nj * 4, hipMemcpyDeviceToHost, s)) ||
+            (e = hipMemcpyAsync(flags.data(), d_flags, nj, hipMemcpyDeviceToHost, s)) ||
+            (e = hipStreamSynchronize(s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
+        for (int32_t t = 0; t < nj; t++) {
+            if (!(flags[t] & 4)) continue;
+            // build_certified_spectral_alignment (mod.rs:636-686)
+            const Job &j = simd[t];
+            const kira_chain &ch = chains[j.chain];
+            const int32_t L = rl[t], mm = best[t];
+            kira_stage4_aln a = base_aln(1, ch);
+            a.ref_start = (uint32_t)(j.win + shift[t]);
+            a.ref_end = a.ref_start + (uint32_t)L;
+            a.read_start = 0;
+            a.read_end = (uint32_t)L;
+            a.score = wrap_i32((int64_t)(L - mm) * c.cfg.match_score - (int64_t)mm * c.cfg.mismatch);
+            a.mapq = 0;
+            a.cigar_off = (int64_t)R.cigar.size();
+            a.n_cigar = 1;
+            R.cigar.push_back(((uint32_t)L << 4) | kOpM);
+            a.md_off = (int64_t)R.md.size();
+            a.nm = kira::ungapped_tags(oriented(j.read, ch.strand == 1),
+                                       contig(ch.ref_id) + j.win + shift[t], L, R.md);
+            a.md_len = (int32_t)(R.md.size() - a.md_off);
+            push(j.read, a);
+            st.gpu_spectral_resolved++;
+            keep[t] = 0;
+        }
+    }
+    R.phase_ms[1] = ms_since(t0);
+
+    // align_batch_simd lane groups: BTreeMap over (read, window) length,
+    // chunks of `lanes` jobs (:1153-1186)
+    t0 = Clock::now();
+    {
+        std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> buckets;
+        for (int32_t t = 0; t < nj; t++)
+            if (keep[t]) buckets[{read_len[simd[t].read], simd[t].wlen}].push_back(t);
+        std::vector<int32_t> order, batch_off{0};
+        for (auto &kv : buckets) {
+            const std::vector<int32_t> &v = kv.second;
+            for (size_t b = 0; b < v.size(); b += (size_t)lanes) {
+                size_t e2 = std::min(v.size(), b + (size_t)lanes);
+                order.insert(order.end(), v.begin() + b, v.begin() + e2);
+                batch_off.push_back((int32_t)order.size());
+            }
+        }
+        const int32_t m = (int32_t)order.size();
+        if (m > 0) {
+            std::vector<kira_batch_input> in(m);
+            int64_t cap = 0;
+            for (int32_t i = 0; i < m; i++) {
+                const Job &j = simd[order[i]];
+                const kira_chain &ch = chains[j.chain];
+                kira_batch_input &x = in[i];
+                x.read = oriented(j.read, ch.strand == 1);
+                x.read_len = (uint32_t)read_len[j.read];
+                x.window = contig(ch.ref_id) + j.win;
+                x.window_len = (uint32_t)j.wlen;
+                x.win_start = (uint32_t)j.win;
+                x.chain_ref_start = ch.ref_start;
+                x.chain_ref_end = ch.ref_end;
+                x.chain_read_start = ch.read_start;
+                x.chain_read_end = ch.read_end;
+                x.abort_score = j.abort;
+                cap += (int64_t)x.read_len + x.window_len + 2;
+            }
+            std::vector<kira_sw_result> out(m);
+            std::vector<uint32_t> arena((size_t)std::max<int64_t>(cap, 1));
+            kira_status kst = kira_align_batch_simd(
+                ctx, (int32_t)batch_off.size() - 1, batch_off.data(), in.data(), c.simd, c.simd,
+                &c.cfg, &c.knobs, out.data(), arena.data(), cap);
+            if (kst != KIRA_OK) return kst;
+            for (int32_t i = 0; i < m; i++) {
+                const Job &j = simd[order[i]];
+                push(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
+                                    arena.data() + out[i].cigar_off));
+                st.dp_simd++;
+                if (out[i].early_abort) st.dp_early_abort++;
+            }
+        }
+    }
+    R.phase_ms[2] = ms_since(t0);
+
+    // scalar jobs: align_chain_with_meta (stage4:1196-1205)
+    t0 = Clock::now();
+    kira_status kst = banded(scalar, false, true);
+    if (kst != KIRA_OK) return kst;
+    R.phase_ms[3] = ms_since(t0);
+
+    t0 = Clock::now();
+    std::stable_sort(recs.begin(), recs.end(),
+                     [](const Rec &x, const Rec &y) { return x.read < y.read; });
+    std::vector<int64_t> first(n + 1, 0);
+    for (const Rec &x : recs) first[x.read + 1]++;
+    for (int32_t r = 0; r < n; r++) first[r + 1] += first[r];
+
+    // XS from the chain-score proxy (:1215-1250)
+    for (int32_t r = 0; r < n; r++) {
+        if (first[r + 1] - first[r] != 1) continue;
+        const int32_t cb = chain_best[r], cs = chain_second[r];
+        if (cb <= 0 || cs <= 0) continue;
+        kira_stage4_aln &a = recs[first[r]].a;
+        if (a.has_xs) continue;
+        const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
+        if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
+            a.has_xs = 1;
+            a.xs_score = sub;
+        }
+    }
+
+    // emit_supplementary_alignments (split_read.rs:22-57)
+    if (c.supplementary) {
+        std::vector<Job> sj;
+        for (int32_t r = 0; r < n; r++) {
+            const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+            if (first[r + 1] == first[r] || nc < 2) continue;
+            const kira_stage4_aln &p = recs[first[r]].a;
+            const kira_chain *cl = chains + c0;
+            // pick_supplementary_chains (split_read.rs:60-125)
+            const int32_t ps = std::max(cl[0].score, 1);
+            const int32_t min_score = f2i(ceilf((float)ps * kSuppMinScoreRatio));
+            uint32_t L = 0;
+            for (int32_t k = 0; k < p.n_cigar; k++) {
+                uint32_t op = R.cigar[p.cigar_off + k];
+                if ((op & 15u) == kOpM || (op & 15u) == kOpI || (op & 15u) == kOpS) L += op >> 4;
+            }
+            auto pi = original_interval(p.read_start, p.read_end, p.is_rev != 0, L);
+            std::vector<int64_t> picks;
+            for (int64_t k = 1; k < nc; k++) {
+                const kira_chain &ch = cl[k];
+                if (ch.score < min_score) break;
+                if (usub(ch.read_end, ch.read_start) < kSuppMinReadSpan) continue;
+                auto ci = original_interval(ch.read_start, ch.read_end, ch.strand == 1, L);
+                if (read_overlap_pct(pi.first, pi.second, ci.first, ci.second) > kSuppMaxOverlapPct)
+                    continue;
+                bool dup = false;
+                for (int64_t q : picks) {
+                    const kira_chain &pc = chains[q];
+                    auto qi = original_interval(pc.read_start, pc.read_end, pc.strand == 1, L);
+                    if (read_overlap_pct(qi.first, qi.second, ci.first, ci.second) >
+                        kSuppMaxOverlapPct) {
+                        dup = true;
+                        break;
+                    }
+                }
+                if (dup) continue;
+                picks.push_back(c0 + k);
+                if (picks.size() >= kMaxSuppPerRead) break;
+            }
+            for (int64_t q : picks) {
+                const kira_chain &ch = chains[q];
+                // align_oriented_chain_with_meta: exact shortcut, else banded SW
+                if (exact_candidate(r, ch) &&
+                    memcmp(oriented(r, ch.strand == 1), contig(ch.ref_id) + ch.ref_start,
+                           (size_t)read_len[r]) == 0) {
+                    kira_stage4_aln a = exact_aln(r, ch);
+                    a.is_supplementary = 1;
+                    push(r, a);
+                    st.supplementary++;
+                    continue;
+                }
+                sj.push_back(clamp_job(r, q, INT32_MIN / 8));
+            }
+        }
+        if ((kst = banded(sj, true, false)) != KIRA_OK) return kst;
+        st.supplementary += (int64_t)sj.size();
+        std::stable_sort(recs.begin(), recs.end(),
+                         [](const Rec &x, const Rec &y) { return x.read < y.read; });
+    }
+
+    R.aln_off.assign(n + 1, 0);
+    for (const Rec &x : recs) R.aln_off[x.read + 1]++;
+    for (int32_t r = 0; r < n; r++) R.aln_off[r + 1] += R.aln_off[r];
+    R.alns.resize(recs.size());
+    for (size_t k = 0; k < recs.size(); k++) R.alns[k] = recs[k].a;
+    for (int32_t r = 0; r < n; r++) st.unmapped += R.aln_off[r + 1] == R.aln_off[r] ? 1 : 0;
+    R.phase_ms[4] = ms_since(t0);
+    return KIRA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_contigs,
+                            const int64_t *contig_off, const int64_t *contig_len,
+                            int32_t n_reads, const uint8_t *reads, const int64_t *read_off,
+                            const int32_t *read_len, const int32_t *repeat_min_occ,
+                            const int64_t *chain_off, const kira_chain *chains,
+                            const kira_stage4_cfg *cfg, kira_stage4_result **out) {
+    if (!ctx) return KIRA_ERR_INVALID_ARGUMENT;
+    if (!g || !cfg || !out || n_reads < 0 || n_contigs < 0 ||
+        (n_contigs > 0 && (!contig_off || !contig_len)) ||
+        (n_reads > 0 && (!reads || !read_off || !read_len || !chain_off)))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
+    *out = nullptr;
+    const int64_t glen = (int64_t)g->host.size();
+    for (int32_t i = 0; i < n_contigs; i++)
+        if (contig_off[i] < 0 || contig_len[i] < 0 || contig_off[i] + contig_len[i] > glen)
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "contig outside the genome");
+    if (n_reads > 0 && (chain_off[0] != 0 || (chain_off[n_reads] > 0 && !chains)))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid chain offsets");
+    for (int32_t r = 0; r < n_reads; r++) {
+        if (read_len[r] < 0 || read_off[r] < 0 || chain_off[r + 1] < chain_off[r])
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid read or chain offsets");
+        for (int64_t k = chain_off[r]; k < chain_off[r + 1]; k++) {
+            const kira_chain &ch = chains[k];
+            if (ch.ref_id >= (uint32_t)n_contigs || contig_len[ch.ref_id] == 0 ||
+                ch.read_start > (uint32_t)read_len[r] || ch.read_end > (uint32_t)read_len[r] ||
+                ch.ref_start > contig_len[ch.ref_id] || ch.ref_end > contig_len[ch.ref_id] ||
+                (ch.strand != 0 && ch.strand != 1))
+                return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain outside its read or contig");
+        }
+    }
+    kira_stage4_result *res = new (std::nothrow) kira_stage4_result();
+    if (!res) return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "result allocation");
+    kira_status st;
+    try {
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) {
+            delete res;
+            return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
+        }
+        Driver d{ctx,      g,        contig_off, contig_len, n_reads, reads, read_off,
+                 read_len, chain_off, chains,    *cfg,       *res,    ctx->stream};
+        d.rmo = repeat_min_occ;
+        st = d.run();
+    } catch (const std::bad_alloc &) {
+        st = fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    if (st != KIRA_OK) {
+        delete res;
+        return st;
+    }
+    *out = res;
+    return KIRA_OK;
+}
+
+kira_status kira_stage4_result_view(const kira_stage4_result *r, kira_stage4_view *v) {
+    if (!r || !v) return KIRA_ERR_INVALID_ARGUMENT;
+    v->n_reads = (int32_t)r->aln_off.size() - 1;
+    v->aln_off = r->aln_off.data();
+    v->alns = r->alns.data();
+    v->cigar = r->cigar.data();
+    v->md = r->md.data();
+    v->n_alns = (int64_t)r->alns.size();
+    v->cigar_words = (int64_t)r->cigar.size();
+    v->md_bytes = (int64_t)r->md.size();
+    v->stats = r->stats;
+    for (int k = 0; k < 5; k++) v->phase_ms[k] = r->phase_ms[k];
+    return KIRA_OK;
+}
+
+void kira_stage4_result_free(kira_stage4_result *r) { delete r; }
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+"""GPU parity of the stage-4 driver (kira_stage4_run) with its CPU
+restatement (oracle/stage4.py): every record and every counter."""
+import time
+
+import pytest
+
+from oracle import stage4 as S
+from tests.kira_pkg import load
+from tests.stage4_data import make_batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    p = load()
+    assert p.is_available(), "no GPU visible"
+    return p
+
+
+@pytest.fixture(scope="module")
+def ctx(pkg):
+    c = pkg.Context(0)
+    yield c
+    c.close()
+
+
+def run_both(pkg, ctx, b, **cfg):
+    print(f"[stage4] {len(b['reads'])} reads, cfg {cfg}", flush=True)
+    genome = pkg.Genome(ctx, b["genome"])
+    t0 = time.time()
+    res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                         pkg.stage4_cfg(**cfg), b["repeat_min_occ"])
+    got, gst = res.alignments(), res.stats
+    print(f"[stage4] gpu done in {time.time() - t0:.2f}s, phases {res.phase_ms}", flush=True)
+    res.close()
+    genome.close()
+    want, wst = S.run(b["contigs"], b["reads"], b["chains"], S.default_cfg(**cfg),
+                      b["repeat_min_occ"])
+    return got, gst, want, wst
+
+
+@pytest.mark.parametrize("cfg", [dict(), dict(simd="avxvnni"), dict(simd="neon", bandwidth=20),
+                                 dict(simd="scalar", supplementary=False),
+                                 dict(cert_strict=True, min_chain_ratio=0.2, dp_abort_margin=5)])
+def test_records_and_counters_match_oracle(pkg, ctx, cfg):
+    b = make_batch(11, 2500)
+    got, gst, want, wst = run_both(pkg, ctx, b, **cfg)
+    for r in range(len(want)):
+        assert got[r] == want[r], (r, b["chains"][r])
+    assert gst == wst
+    # every branch of the driver ran
+    if not cfg:
+        for k in ("exact_matches", "prefilter_accept", "prefilter_reject",
+                  "gpu_spectral_resolved", "dp_simd", "dp_scalar", "supplementary",
+                  "unmapped", "dp_early_abort"):
+            assert gst[k] > 0, k
+
+
+def test_empty_batch_and_reads_without_chains(pkg, ctx):
+    b = make_batch(12, 20)
+    genome = pkg.Genome(ctx, b["genome"])
+    res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], [], [])
+    assert res.alignments() == [] and res.stats["reads"] == 0
+    res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], b["reads"][:5],
+                         [[]] * 5)
+    assert res.alignments() == [[]] * 5 and res.stats["unmapped"] == 5
+    bad = [[(7, 0, 10, 0, 10, 0, 10)]]  # contig 7 does not exist
+    with pytest.raises(pkg.HipError):
+        pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], b["reads"][:1], bad)
+    genome.close()

@@ -1,0 +1,168 @@
+"""The stage-4 driver restatement (oracle/stage4.py) on rule-level cases with
+hand-computed answers, and the supplementary picks on the reference's own
+split-read unit tests (tests/unit/pipeline_split_read.rs)."""
+from oracle import stage4 as S
+
+I32_MIN = -(2 ** 31)
+
+
+def chain(score, rs, re, ref_id, ref_start=1000, strand=0):
+    # make_chain (tests/unit/pipeline_split_read.rs:4-15)
+    return (ref_id, ref_start, ref_start + (re - rs), rs, re, strand, score)
+
+
+def primary(rs, re):
+    # make_aln (tests/unit/pipeline_split_read.rs:17-40)
+    return dict(read_start=rs, read_end=re, is_rev=False, cigar=f"{re - rs}M")
+
+
+def test_split_read_known_answers():
+    p = S.pick_supplementary_chains
+    assert p(primary(0, 80), [chain(100, 0, 80, 0), chain(70, 80, 150, 1)]) == [1]
+    assert p(primary(0, 150), [chain(100, 0, 150, 0), chain(90, 10, 150, 1)]) == []
+    assert p(primary(0, 80), [chain(100, 0, 80, 0), chain(30, 80, 150, 1)]) == []
+    assert p(primary(0, 80), [chain(100, 0, 80, 0), chain(80, 100, 130, 1)]) == []
+    cl = [chain(100, 0, 80, 0), chain(80, 80, 150, 1), chain(75, 82, 150, 2)]
+    picks = p(primary(0, 80), cl)
+    assert len(picks) == 1 and cl[picks[0]][0] == 1
+    cl = [chain(200, 0, 200, 0)] + [chain(150, 200 + k * 60, 255 + k * 60, k + 1)
+                                    for k in range(6)]
+    assert len(p(primary(0, 200), cl)) == S.MAX_SUPP_PER_READ
+    assert p(primary(0, 80), [chain(100, 0, 80, 0)]) == []
+
+
+def test_overlap_pct_known_answers():
+    o = S.read_overlap_pct
+    assert o(0, 50, 50, 100) == 0
+    assert o(0, 100, 0, 100) == 100
+    assert o(0, 100, 50, 100) == 50
+    assert o(0, 100, 50, 150) == 50
+    assert o(50, 100, 0, 100) == 100
+    assert o(50, 50, 0, 100) == 0
+
+
+def test_original_interval_flips_reverse_reads():
+    assert S.original_interval(10, 40, True, 150) == (110, 140)
+    assert S.original_interval(10, 40, False, 150) == (10, 40)
+    assert S.original_interval(0, 200, True, 150) == (0, 150)  # saturating
+
+
+def test_chain_selection_rules():
+    cfg = S.default_cfg()
+    c = lambda s: (0, 0, 10, 0, 10, 0, s)  # noqa: E731
+    # min_chain_ratio 0.4: ceil(100 * 0.4) = 40, so 39 is skipped but 40 kept
+    sel, best, second, margin = S.select_chains(150, [c(100), c(39), c(40)], 0, cfg)
+    assert (best, second, margin) == (100, 39, 61)
+    assert sel == [0]  # unambiguous (61 > max(100 / 5, 8)): top-1 only
+    # ambiguous: margin 15 <= 20 -> two DP placements, skipping the low one
+    sel, *_ = S.select_chains(150, [c(100), c(30), c(85)], 0, cfg)
+    assert sel == [0]  # margin = 100 - 30 = 70: not ambiguous
+    sel, *_ = S.select_chains(150, [c(100), c(85), c(30), c(80)], 0, cfg)
+    assert sel == [0, 1]
+    # repeat_min_occ > 1 makes any multi-chain read ambiguous
+    sel, *_ = S.select_chains(150, [c(100), c(50)], 2, cfg)
+    assert sel == [0, 1]
+    # margin floor 8: best 20 -> max(4, 8) = 8
+    sel, *_ = S.select_chains(150, [c(20), c(12)], 0, cfg)
+    assert sel == [0, 1]
+    sel, *_ = S.select_chains(150, [c(20), c(11)], 0, cfg)
+    assert sel == [0]
+    # long reads use dp_topk
+    sel, *_ = S.select_chains(301, [c(100), c(99), c(98)], 0, S.default_cfg(dp_topk=2))
+    assert sel == [0, 1]
+    # KIRA_SHORT_DPTOPK overrides the short-read top-k
+    sel, *_ = S.select_chains(150, [c(100), c(99), c(98)], 0, S.default_cfg(short_dp_topk=3))
+    assert sel == [0, 1, 2]
+    # non-positive best: min_score = best, so only chains scoring as well pass
+    sel, *_ = S.select_chains(150, [c(0), c(-5)], 0, cfg)
+    assert sel == [0]
+    sel, *_ = S.select_chains(150, [c(-3), c(-3)], 0, cfg)
+    assert sel == [0, 1]
+
+
+def _genome(n, seed=3):
+    import random
+    rng = random.Random(seed)
+    return bytes(rng.choice(b"ACGT") for _ in range(n))
+
+
+def test_exact_match_shortcut_and_xs_proxy():
+    g = _genome(5000)
+    read = g[1000:1150]
+    chains = [[(0, 1000, 1150, 0, 150, 0, 150), (0, 3000, 3100, 0, 100, 0, 60)]]
+    alns, st = S.run([g], [read], chains)
+    a, = alns[0]
+    assert (a["kind"], a["cigar"], a["ref_start"], a["ref_end"], a["nm"], a["md"], a["mapq"],
+            a["score"]) == ("accepted_ungapped", "150M", 1000, 1150, 0, "150", 0, 150)
+    # XS = score * second / best = 150 * 60 / 150
+    assert a["xs_score"] == 60
+    assert st["exact_matches"] == 1 and st["chains_used"] == 1 and st["unmapped"] == 0
+
+
+def test_reverse_strand_read_and_prefilter_accept():
+    g = bytearray(_genome(5000, 4))
+    seg = bytes(g[2000:2150])
+    read = S.reverse_complement(seg)
+    mut = bytearray(seg)
+    mut[75] = ord("A") if mut[75] != ord("A") else ord("C")
+    read = S.reverse_complement(bytes(mut))
+    alns, st = S.run([bytes(g)], [read], [[(0, 2000, 2150, 0, 150, 1, 150)]])
+    a, = alns[0]
+    # one mismatch: the ungapped prefilter accepts with mapq 60
+    assert (a["kind"], a["is_rev"], a["cigar"], a["nm"], a["mapq"]) == \
+        ("accepted_ungapped", True, "150M", 1, 60)
+    assert a["score"] == 149 - 4
+    assert st["prefilter_accept"] == 1
+
+
+def test_indel_read_goes_to_lane_sw_and_long_read_to_banded():
+    g = _genome(20000, 5)
+    src = g[5000:5150]
+    # an anchor shorter than half the read: the prefilter rejects (span_too_small)
+    short = src[:70] + src[73:]
+    alns, st = S.run([g], [short], [[(0, 5000, 5070, 0, 70, 0, 70)]])
+    assert alns == [[]] and st["prefilter_reject"] == 1 and st["reason_counts"][5] == 1
+    assert st["unmapped"] == 1
+    read = src[:100] + src[103:]  # 3-bp deletion: the prefilter falls back
+    alns, st = S.run([g], [read], [[(0, 5000, 5100, 0, 100, 0, 100)]])
+    a, = alns[0]
+    assert st["prefilter_fallback"] == 1 and st["dp_simd"] == 1 and st["dp_scalar"] == 0
+    assert (a["kind"], a["cigar"], a["ref_start"], a["nm"], a["md"]) == \
+        ("dp_aligned", "100M3D47M", 5000, 3, "100^" + src[100:103].decode() + "47")
+    long_read = g[8000:8400]
+    alns, st = S.run([g], [long_read[:200] + long_read[202:]],
+                     [[(0, 8000, 8200, 0, 200, 0, 200)]])
+    assert st["dp_scalar"] == 1 and st["dp_simd"] == 0
+    assert "2D" in alns[0][0]["cigar"]
+    # scalar SIMD mode: no lane jobs at all
+    alns, st = S.run([g], [read], [[(0, 5000, 5100, 0, 100, 0, 100)]],
+                     S.default_cfg(simd="scalar"))
+    assert st["dp_simd"] == 0 and st["dp_scalar"] == 1
+
+
+def test_sieve_certifies_a_not_top1_chain():
+    g = bytearray(_genome(20000, 6))
+    src = bytes(g[6000:6150])
+    g[12000:12150] = src  # a perfect repeat copy
+    mut = bytearray(src)
+    mut[60] = ord("A") if mut[60] != ord("A") else ord("G")
+    read = bytes(mut)
+    # two equal-score chains: the read is ambiguous, both get DP; the second
+    # is not top-1 so the prefilter falls back and the sieve certifies it
+    chains = [[(0, 6000, 6150, 0, 150, 0, 150), (0, 12000, 12150, 0, 150, 0, 150)]]
+    alns, st = S.run([bytes(g)], [read], chains)
+    assert st["prefilter_accept"] == 1 and st["gpu_spectral_resolved"] == 1
+    kinds = [(a["kind"], a["ref_start"], a["mapq"]) for a in alns[0]]
+    assert kinds == [("accepted_ungapped", 6000, 60), ("dp_aligned", 12000, 0)]
+    assert alns[0][1]["cigar"] == "150M" and alns[0][1]["score"] == 149 - 4
+
+
+def test_supplementary_alignment_of_a_chimeric_read():
+    g = _genome(40000, 7)
+    read = g[3000:3080] + g[20000:20070]
+    chains = [[(0, 3000, 3080, 0, 80, 0, 80), (0, 20000, 20070, 80, 150, 0, 70)]]
+    alns, st = S.run([g], [read], chains)
+    assert st["supplementary"] == 1
+    assert [a["is_supplementary"] for a in alns[0]] == [False, True]
+    sup = alns[0][1]
+    assert sup["ref_start"] == 20000 and sup["cigar"] == "80S70M"
