@@ -311,6 +311,10 @@ def main():
     ap.add_argument("--sw-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
     ap.add_argument("--long-windows", type=int, default=82_000)
+    ap.add_argument("--stage4-reads", type=int, default=200_000)
+    ap.add_argument("--stage4-steps", type=int, default=3)
+    ap.add_argument("--stage4-only", action="store_true",
+                    help="run only the stage-4 driver section (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
@@ -328,6 +332,10 @@ def main():
     pkg = load_pkg()
     if not pkg.is_available():
         raise SystemExit("no GPU visible to libkira_hip")
+    if args.stage4_only:
+        ctx = pkg.Context(0)
+        print(json.dumps({"stage4": bench_stage4(pkg, ctx, args)}), flush=True)
+        return
     t_gen = time.time()
     work = make_sieve_workload(n_reads=args.reads)
     t_gen = time.time() - t_gen
@@ -483,7 +491,68 @@ def main():
         del reads, lens, win_start, win_len, planes, valid, out
         torch.cuda.empty_cache()
         line["serving"] = bench_serving(pkg, ctx, genome, work, lw, args)
+    if args.stage4_reads > 0:
+        line["stage4"] = bench_stage4(pkg, ctx, args)
     print(json.dumps(line), flush=True)
+
+
+def bench_stage4(pkg, ctx, args):
+    """The stage-4 driver (kira_stage4_run) over one batch of single-end reads
+    with their chains: reads/s per call, host-wall phase split, and the CPU
+    restatement on a sample of the same batch (a Python driver over the C
+    oracle, one thread) with a parity check on that sample."""
+    from tests.stage4_data import make_batch
+    from oracle import stage4 as S
+    t0 = time.time()
+    b = make_batch(21, args.stage4_reads, n_contigs=8, contig_len=2_500_000)
+    print(f"[bench] stage4 batch: {len(b['reads'])} reads, "
+          f"{sum(len(c) for c in b['chains'])} chains, generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    genome = pkg.Genome(ctx, b["genome"])
+    cfg = pkg.stage4_cfg()
+    # host arrays as a caller hands them over (marshalled once, outside the timing)
+    x = pkg.Stage4Inputs(b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                         b["repeat_min_occ"])
+    pkg.stage4_run(ctx, genome, x, cfg=cfg).close()
+    times, phases = [], []
+    for _ in range(args.stage4_steps):
+        t0 = time.perf_counter()
+        res = pkg.stage4_run(ctx, genome, x, cfg=cfg)
+        times.append(time.perf_counter() - t0)
+        phases.append(res.phase_ms)
+        stats = res.stats
+        res.close()
+    ms = 1e3 * float(np.median(times))
+    n = len(b["reads"])
+    k = int(np.argsort(times)[len(times) // 2])
+    out = {"workload": "stage-4 driver: 8 x 2.5 Mbp contigs (20% repeat families), "
+                       f"{n} single-end reads of 100-250 bp (subs, indels, N, chimeras), "
+                       "1-3 chains each, mem defaults, AVX2 lane groups",
+           "reads": n, "ms_per_batch": ms, "reads_per_s": n / (ms * 1e-3),
+           "phase_ms": dict(zip(["triage", "sieve", "lane_sw", "banded_sw", "records"],
+                                phases[k])),
+           "stats": {kk: v for kk, v in stats.items() if kk != "reason_counts"}}
+    print(f"[bench] stage4: {n} reads, {ms:.1f} ms/batch, {out['reads_per_s']:.0f} reads/s, "
+          f"phases {out['phase_ms']}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        m = 2000
+        sub = {key: b[key][:m] for key in ("reads", "chains", "repeat_min_occ")}
+        t0 = time.perf_counter()
+        want, _ = S.run(b["contigs"], sub["reads"], sub["chains"], S.default_cfg(),
+                        sub["repeat_min_occ"])
+        cpu_s = time.perf_counter() - t0
+        res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], sub["reads"],
+                             sub["chains"], cfg, sub["repeat_min_occ"])
+        parity = res.alignments() == want
+        res.close()
+        out["cpu_baseline"] = {"value": m / cpu_s, "unit": "reads/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} reads of the batch through the CPU "
+                                         "restatement (Python driver over the C oracle)"}
+        out["parity_vs_oracle_sample"] = parity
+        print(f"[bench] stage4 cpu baseline {m / cpu_s:.0f} reads/s (1 thread); "
+              f"parity={parity}", file=sys.stderr, flush=True)
+    genome.close()
+    return out
 
 
 def make_requests(work, lw, n_req, seed=5, max_reads=50_000):

@@ -1050,32 +1050,51 @@ class Stage4Result:
             pass
 
 
-def stage4_run(ctx: Context, genome: Genome, contig_off, contig_len, reads, chains,
-               cfg: Optional[Stage4Cfg] = None, repeat_min_occ=None) -> Stage4Result:
+CHAIN_DTYPE = np.dtype([("ref_id", "<u4"), ("ref_start", "<u4"), ("ref_end", "<u4"),
+                        ("read_start", "<u4"), ("read_end", "<u4"), ("strand", "<i4"),
+                        ("score", "<i4")])
+assert CHAIN_DTYPE.itemsize == C.sizeof(Chain)
+
+
+class Stage4Inputs:
+    """A stage-4 batch as the flat host arrays kira_stage4_run reads.
+
+    contig i = genome bases contig_off[i] .. +contig_len[i]; reads: list of
+    bytes (forward strand); chains: per read, a list of (ref_id, ref_start,
+    ref_end, read_start, read_end, strand 0/1, score) in stage-3 order."""
+
+    def __init__(self, contig_off, contig_len, reads, chains, repeat_min_occ=None):
+        n = len(reads)
+        self.n = n
+        self.co = _arr(contig_off, np.int64)
+        self.cl = _arr(contig_len, np.int64)
+        self.rl = np.array([len(r) for r in reads], np.int32)
+        self.ro = np.zeros(n, np.int64)
+        if n > 1:
+            self.ro[1:] = np.cumsum(self.rl[:-1], dtype=np.int64)
+        self.rb = np.frombuffer(b"".join(reads), np.uint8) if self.rl.sum() \
+            else np.zeros(1, np.uint8)
+        self.coff = np.zeros(n + 1, np.int64)
+        self.coff[1:] = np.cumsum([len(c) for c in chains], dtype=np.int64)
+        flat = [c for cs in chains for c in cs]
+        self.ch = np.array(flat, dtype=CHAIN_DTYPE) if flat else np.zeros(1, CHAIN_DTYPE)
+        self.rmo = _arr(repeat_min_occ, np.int32) if repeat_min_occ is not None else None
+
+
+def stage4_run(ctx: Context, genome: Genome, contig_off, contig_len=None, reads=None,
+               chains=None, cfg: Optional[Stage4Cfg] = None,
+               repeat_min_occ=None) -> Stage4Result:
     """stage4_alignment::run (src/pipeline/stage4_alignment.rs:959-1268) on the device.
 
-    genome holds the contigs back to back (contig i = bases contig_off[i] ..
-    +contig_len[i]); reads: list of bytes (forward strand); chains: per read,
-    a list of (ref_id, ref_start, ref_end, read_start, read_end, strand 0/1,
-    score) in stage-3 order."""
+    Either pass a Stage4Inputs as contig_off, or the contig table, reads and
+    chains as Stage4Inputs takes them."""
+    x = contig_off if isinstance(contig_off, Stage4Inputs) else \
+        Stage4Inputs(contig_off, contig_len, reads, chains, repeat_min_occ)
     cfg = cfg or stage4_cfg()
-    n = len(reads)
-    co = _arr(contig_off, np.int64)
-    cl = _arr(contig_len, np.int64)
-    rl = np.array([len(r) for r in reads], np.int32)
-    ro = np.zeros(n, np.int64)
-    if n > 1:
-        ro[1:] = np.cumsum(rl[:-1], dtype=np.int64)
-    rb = np.frombuffer(b"".join(reads), np.uint8) if rl.sum() else np.zeros(1, np.uint8)
-    coff = np.zeros(n + 1, np.int64)
-    coff[1:] = np.cumsum([len(c) for c in chains], dtype=np.int64)
-    flat = [c for cs in chains for c in cs]
-    ch = (Chain * max(len(flat), 1))(*[Chain(*c) for c in flat])
-    rmo = _arr(repeat_min_occ, np.int32) if repeat_min_occ is not None else None
     h = C.c_void_p()
-    st = lib().kira_stage4_run(ctx.h, genome.h, len(co), _np_ptr(co), _np_ptr(cl), n,
-                               _np_ptr(rb), _np_ptr(ro), _np_ptr(rl),
-                               _np_ptr(rmo) if rmo is not None else None, _np_ptr(coff), ch,
-                               C.byref(cfg), C.byref(h))
+    st = lib().kira_stage4_run(ctx.h, genome.h, len(x.co), _np_ptr(x.co), _np_ptr(x.cl), x.n,
+                               _np_ptr(x.rb), _np_ptr(x.ro), _np_ptr(x.rl),
+                               _np_ptr(x.rmo) if x.rmo is not None else None, _np_ptr(x.coff),
+                               x.ch.ctypes.data_as(C.POINTER(Chain)), C.byref(cfg), C.byref(h))
     ctx.check(st, "stage4_run")
     return Stage4Result(h)

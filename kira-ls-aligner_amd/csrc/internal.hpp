@@ -192,6 +192,48 @@ struct GroupLaunch {
 hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,
                            hipStream_t s);
 
+// SW batches over device-resident sequences with host-side job tables: job
+// k's read is d_reads + read_off[k] (read_len[k] bytes, 8 readable bytes
+// past the arena end), its window d_refs + ref_off[k] (ref_len[k] bytes;
+// banded SW needs kRefPad readable bytes on each side).  offset / band /
+// abort_score as in kira_banded_sw_batch (band may be null).
+struct SwJobsHost {
+    int32_t n;
+    const uint8_t *d_reads;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const uint8_t *d_refs;
+    const int64_t *ref_off;
+    const int32_t *ref_len;
+    const int32_t *offset;
+    const int32_t *band;
+    const int32_t *abort_score;
+};
+
+// Downloads n results (kSwResWords each) and their CIGARs, which the kernels
+// left at src_off[k] of d_cig: the CIGARs are packed back to back on the
+// device first and appended to `cigar`; out[k].cigar_off indexes `cigar`.
+kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, const uint32_t *d_cig,
+                             const int64_t *src_off, bool banded, kira_sw_result *out,
+                             std::vector<uint32_t> &cigar);
+// banded_sw_with (src/alignment/mod.rs:1370-1650) over a job table.
+kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg &cfg,
+                          const kira_sw_knobs &kn, kira_sw_result *out,
+                          std::vector<uint32_t> &cigar);
+// align_batch_simd's dispatch (src/alignment/mod.rs:1118-1149) of batches
+// batch_off[b] .. batch_off[b+1]: group_of[k] = lane group of job k, or -1
+// for the banded SW of BatchInput::scalar_sw; group_neg = each group's
+// lane-type minus infinity.
+void simd_groups(int32_t n_batches, const int32_t *batch_off, const uint32_t *read_len,
+                 const uint32_t *win_len, kira_simd_mode mode, kira_simd_mode detected,
+                 const kira_sw_cfg &cfg, std::vector<int32_t> &group_of,
+                 std::vector<int32_t> &group_neg);
+// Full-window lane groups (sw_batch_avx2 / sw_batch_int8 / sw_batch_neon)
+// over a job table whose every job is in group group_of[k].
+kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *group_of,
+                         const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg,
+                         kira_sw_result *out, std::vector<uint32_t> &cigar);
+
 struct PrefilterLaunch {
     int64_t n;
     const uint8_t *reads;

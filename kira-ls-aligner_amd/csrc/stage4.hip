@@ -10,7 +10,6 @@
 
 #include <algorithm>
 #include <chrono>
-#include <map>
 #include <new>
 #include <string>
 #include <utility>
@@ -252,13 +251,14 @@ struct Driver {
                (int64_t)ch.ref_start + L <= contig_len[ch.ref_id];
     }
 
-    // Banded SW (align_chain_with_meta's DP, mod.rs:865-883) on clamp_window
-    // windows (:1766-1771), on the device; appends one record per job.
-    kira_status banded(const std::vector<Job> &jobs, bool supplementary, bool count) {
+    // Banded SW over device-resident reads and windows; appends one record
+    // per job.  offset: the band centre offset of each job.
+    kira_status banded(const std::vector<Job> &jobs, const std::vector<int32_t> &offset,
+                       bool supplementary, bool count) {
         const int32_t m = (int32_t)jobs.size();
         if (m == 0) return KIRA_OK;
         std::vector<int64_t> ro(m), fo(m);
-        std::vector<int32_t> rl(m), fl(m), off(m), ab(m);
+        std::vector<int32_t> rl(m), fl(m), ab(m);
         for (int32_t t = 0; t < m; t++) {
             const Job &j = jobs[t];
             const kira_chain &ch = chains[j.chain];
@@ -266,70 +266,38 @@ struct Driver {
             rl[t] = read_len[j.read];
             fo[t] = contig_off[ch.ref_id] + j.win;
             fl[t] = j.wlen;
-            // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
-            off[t] = wrap_i32((int64_t)(int32_t)ch.ref_start - (int32_t)(uint32_t)j.win -
-                              (int32_t)ch.read_start);
             ab[t] = j.abort;
         }
-        kira_sw_plan *plan = nullptr;
-        kira_status st = kira_banded_sw_plan_create(ctx, m, rl.data(), nullptr, &c.cfg, &plan);
+        kira::SwJobsHost sj{m,         d_orient,  ro.data(), rl.data(), g->ascii,
+                            fo.data(), fl.data(), offset.data(), nullptr, ab.data()};
+        std::vector<kira_sw_result> res(m);
+        std::vector<uint32_t> cig;
+        kira_status st = kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, res.data(), cig);
         if (st != KIRA_OK) return st;
-        struct PlanGuard {
-            kira_sw_plan *p;
-            ~PlanGuard() { kira_banded_sw_plan_destroy(p); }
-        } pg{plan};
-        DevMem dm;
-        int64_t *d_ro, *d_fo;
-        int32_t *d_rl, *d_fl, *d_off, *d_ab, *d_res;
-        uint32_t *d_cig;
-        const int64_t words = kira_banded_sw_plan_cigar_words(plan);
-        hipError_t e;
-        if ((e = dm.alloc(&d_ro, m)) || (e = dm.alloc(&d_fo, m)) || (e = dm.alloc(&d_rl, m)) ||
-            (e = dm.alloc(&d_fl, m)) || (e = dm.alloc(&d_off, m)) || (e = dm.alloc(&d_ab, m)) ||
-            (e = dm.alloc(&d_res, (size_t)m * kira::kSwResWords)) ||
-            (e = dm.alloc(&d_cig, (size_t)words)))
-            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
-        if ((e = hipMemcpyAsync(d_ro, ro.data(), m * 8, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_fo, fo.data(), m * 8, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_rl, rl.data(), m * 4, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_fl, fl.data(), m * 4, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_off, off.data(), m * 4, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_ab, ab.data(), m * 4, hipMemcpyHostToDevice, s)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
-        kira_sw_jobs_dev jd{d_orient, d_ro, d_rl, g->ascii, d_fo, d_fl, d_off,
-                            nullptr,  d_ab, d_res, d_cig};
-        if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, &c.cfg, &c.knobs)) != KIRA_OK) return st;
-        std::vector<int32_t> res((size_t)m * kira::kSwResWords);
-        std::vector<uint32_t> cig((size_t)words);
-        if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
-            (words && (e = hipMemcpyAsync(cig.data(), d_cig, cig.size() * 4,
-                                          hipMemcpyDeviceToHost, s))) ||
-            (e = hipStreamSynchronize(s)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
-        const int64_t *coff = kira_banded_sw_plan_cigar_off(plan);
         for (int32_t t = 0; t < m; t++) {
-            const int32_t *w = &res[(size_t)t * kira::kSwResWords];
-            if (w[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
-            if (w[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
-            kira_sw_result sw{};
-            sw.score = w[0];
-            sw.read_end = w[1];
-            sw.ref_end = (uint32_t)w[2];
-            sw.early_abort = w[3];
-            sw.ref_start = (uint32_t)w[4];
-            sw.read_start = w[5];
-            sw.n_cigar = w[6];
             const Job &j = jobs[t];
-            kira_stage4_aln a =
-                sw_aln(j.read, chains[j.chain], j.win, j.wlen, sw, cig.data() + coff[t]);
+            kira_stage4_aln a = sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
+                                       cig.data() + res[t].cigar_off);
             a.is_supplementary = supplementary ? 1 : 0;
             push(j.read, a);
             if (count) {
                 R.stats.dp_scalar++;
-                if (sw.early_abort) R.stats.dp_early_abort++;
+                if (res[t].early_abort) R.stats.dp_early_abort++;
             }
         }
         return KIRA_OK;
+    }
+
+    // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
+    // (align_oriented_chain_with_meta, mod.rs:867-869)
+    std::vector<int32_t> chain_offsets(const std::vector<Job> &jobs) const {
+        std::vector<int32_t> off(jobs.size());
+        for (size_t t = 0; t < jobs.size(); t++) {
+            const kira_chain &ch = chains[jobs[t].chain];
+            off[t] = wrap_i32((int64_t)(int32_t)ch.ref_start - (int32_t)(uint32_t)jobs[t].win -
+                              (int32_t)ch.read_start);
+        }
+        return off;
     }
 
     // clamp_window (mod.rs:1766-1771) job of a chain
@@ -659,58 +627,78 @@ kira_status Driver::run() {
     // chunks of `lanes` jobs (:1153-1186)
     t0 = Clock::now();
     {
-        std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> buckets;
+        std::vector<int32_t> order;
         for (int32_t t = 0; t < nj; t++)
-            if (keep[t]) buckets[{read_len[simd[t].read], simd[t].wlen}].push_back(t);
-        std::vector<int32_t> order, batch_off{0};
-        for (auto &kv : buckets) {
-            const std::vector<int32_t> &v = kv.second;
-            for (size_t b = 0; b < v.size(); b += (size_t)lanes) {
-                size_t e2 = std::min(v.size(), b + (size_t)lanes);
-                order.insert(order.end(), v.begin() + b, v.begin() + e2);
-                batch_off.push_back((int32_t)order.size());
-            }
-        }
+            if (keep[t]) order.push_back(t);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            const int32_t lx = read_len[simd[x].read], ly = read_len[simd[y].read];
+            return lx != ly ? lx < ly : simd[x].wlen < simd[y].wlen;
+        });
         const int32_t m = (int32_t)order.size();
-        if (m > 0) {
-            std::vector<kira_batch_input> in(m);
-            int64_t cap = 0;
-            for (int32_t i = 0; i < m; i++) {
-                const Job &j = simd[order[i]];
+        std::vector<int32_t> batch_off{0};
+        for (int32_t i = 0; i < m;) {
+            int32_t e = i;
+            while (e < m && read_len[simd[order[e]].read] == read_len[simd[order[i]].read] &&
+                   simd[order[e]].wlen == simd[order[i]].wlen)
+                e++;
+            for (int32_t b = i; b < e; b += lanes) batch_off.push_back(std::min(e, b + lanes));
+            i = e;
+        }
+        std::vector<uint32_t> rl(m), wl(m);
+        for (int32_t i = 0; i < m; i++) {
+            rl[i] = (uint32_t)read_len[simd[order[i]].read];
+            wl[i] = (uint32_t)simd[order[i]].wlen;
+        }
+        std::vector<int32_t> group_of, group_neg;
+        kira::simd_groups((int32_t)batch_off.size() - 1, batch_off.data(), rl.data(), wl.data(),
+                          c.simd, c.simd, c.cfg, group_of, group_neg);
+        std::vector<kira_sw_result> out(m);
+        std::vector<uint32_t> cig;
+        for (int pass = 0; pass < 2; pass++) {
+            // pass 0: jobs outside any lane group run BatchInput::scalar_sw
+            // (banded SW, offset clamped to [-L, W], mod.rs:60-72); pass 1: groups
+            std::vector<int32_t> sel;
+            for (int32_t i = 0; i < m; i++)
+                if ((group_of[i] >= 0) == (pass == 1)) sel.push_back(i);
+            const int32_t ms = (int32_t)sel.size();
+            if (ms == 0) continue;
+            std::vector<int64_t> ro(ms), fo(ms);
+            std::vector<int32_t> srl(ms), sfl(ms), off(ms), ab(ms), gof(ms);
+            for (int32_t t = 0; t < ms; t++) {
+                const Job &j = simd[order[sel[t]]];
                 const kira_chain &ch = chains[j.chain];
-                kira_batch_input &x = in[i];
-                x.read = oriented(j.read, ch.strand == 1);
-                x.read_len = (uint32_t)read_len[j.read];
-                x.window = contig(ch.ref_id) + j.win;
-                x.window_len = (uint32_t)j.wlen;
-                x.win_start = (uint32_t)j.win;
-                x.chain_ref_start = ch.ref_start;
-                x.chain_ref_end = ch.ref_end;
-                x.chain_read_start = ch.read_start;
-                x.chain_read_end = ch.read_end;
-                x.abort_score = j.abort;
-                cap += (int64_t)x.read_len + x.window_len + 2;
+                ro[t] = oriented_off(j.read, ch.strand == 1);
+                srl[t] = read_len[j.read];
+                fo[t] = contig_off[ch.ref_id] + j.win;
+                sfl[t] = j.wlen;
+                int64_t o = (int64_t)ch.ref_start - j.win - (int64_t)ch.read_start;
+                o = std::max<int64_t>(o, -(int64_t)srl[t]);
+                off[t] = (int32_t)std::min<int64_t>(o, (int64_t)j.wlen);
+                ab[t] = j.abort;
+                gof[t] = group_of[sel[t]];
             }
-            std::vector<kira_sw_result> out(m);
-            std::vector<uint32_t> arena((size_t)std::max<int64_t>(cap, 1));
-            kira_status kst = kira_align_batch_simd(
-                ctx, (int32_t)batch_off.size() - 1, batch_off.data(), in.data(), c.simd, c.simd,
-                &c.cfg, &c.knobs, out.data(), arena.data(), cap);
+            kira::SwJobsHost sj{ms,        d_orient,   ro.data(),  srl.data(), g->ascii,
+                                fo.data(), sfl.data(), off.data(), nullptr,    ab.data()};
+            std::vector<kira_sw_result> r(ms);
+            kira_status kst =
+                pass == 0 ? kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, r.data(), cig)
+                          : kira::group_sw_dev(ctx, sj, gof.data(), group_neg, c.cfg, r.data(), cig);
             if (kst != KIRA_OK) return kst;
-            for (int32_t i = 0; i < m; i++) {
-                const Job &j = simd[order[i]];
-                push(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
-                                    arena.data() + out[i].cigar_off));
-                st.dp_simd++;
-                if (out[i].early_abort) st.dp_early_abort++;
-            }
+            for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
+        }
+        for (int32_t i = 0; i < m; i++) {
+            const Job &j = simd[order[i]];
+            push(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
+                                cig.data() + out[i].cigar_off));
+            st.dp_simd++;
+            if (out[i].early_abort) st.dp_early_abort++;
         }
     }
     R.phase_ms[2] = ms_since(t0);
 
     // scalar jobs: align_chain_with_meta (stage4:1196-1205)
     t0 = Clock::now();
-    kira_status kst = banded(scalar, false, true);
+    kira_status kst = banded(scalar, chain_offsets(scalar), false, true);
     if (kst != KIRA_OK) return kst;
     R.phase_ms[3] = ms_since(t0);
 
@@ -789,7 +777,7 @@ kira_status Driver::run() {
                 sj.push_back(clamp_job(r, q, INT32_MIN / 8));
             }
         }
-        if ((kst = banded(sj, true, false)) != KIRA_OK) return kst;
+        if ((kst = banded(sj, chain_offsets(sj), true, false)) != KIRA_OK) return kst;
         st.supplementary += (int64_t)sj.size();
         std::stable_sort(recs.begin(), recs.end(),
                          [](const Rec &x, const Rec &y) { return x.read < y.read; });

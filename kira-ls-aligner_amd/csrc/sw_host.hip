@@ -76,6 +76,35 @@ int32_t sat_add(int32_t a, int32_t b) {
     return s > INT32_MAX ? INT32_MAX : (s < INT32_MIN ? INT32_MIN : (int32_t)s);
 }
 
+// Packs each job's CIGAR (left at src_off[k] in a slot sized for the worst
+// case) to dst_off[k], so only the ops themselves cross PCIe.
+__global__ void compact_cigar_kernel(int32_t n, const int32_t *__restrict__ res,
+                                     const uint32_t *__restrict__ src,
+                                     const int64_t *__restrict__ src_off,
+                                     const int64_t *__restrict__ dst_off,
+                                     uint32_t *__restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t c = res[k * kira::kSwResWords + 6];
+    const uint32_t *a = src + src_off[k];
+    uint32_t *b = dst + dst_off[k];
+    for (int32_t i = 0; i < c; i++) b[i] = a[i];
+}
+
+// device allocations released together
+struct DevMem {
+    std::vector<void *> v;
+    ~DevMem() {
+        for (void *x : v) (void)hipFree(x);
+    }
+    template <class T>
+    hipError_t alloc(T **p, size_t count) {
+        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) v.push_back((void *)*p);
+        return e;
+    }
+};
+
 }  // namespace
 
 namespace kira {
@@ -379,6 +408,107 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
     return KIRA_OK;
 }
 
+}  // extern "C"
+
+namespace kira {
+
+kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, const uint32_t *d_cig,
+                             const int64_t *src_off, bool banded, kira_sw_result *out,
+                             std::vector<uint32_t> &cigar) {
+    if (n == 0) return KIRA_OK;
+    hipStream_t s = ctx->stream;
+    std::vector<int32_t> res((size_t)n * kSwResWords);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
+        (e = hipStreamSynchronize(s)))
+        return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
+    std::vector<int64_t> dst(n);
+    int64_t total = 0;
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t *r = &res[(size_t)k * kSwResWords];
+        if (r[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
+        if (banded && r[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
+        dst[k] = total;
+        total += r[6];
+    }
+    const size_t base = cigar.size();
+    cigar.resize(base + (size_t)total);
+    if (total > 0) {
+        DevMem dm;
+        int64_t *d_src, *d_dst;
+        uint32_t *d_out;
+        if ((e = dm.alloc(&d_src, n)) || (e = dm.alloc(&d_dst, n)) ||
+            (e = dm.alloc(&d_out, (size_t)total)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "CIGAR compaction allocation", e);
+        if ((e = hipMemcpyAsync(d_src, src_off, (size_t)n * 8, hipMemcpyHostToDevice, s)) ||
+            (e = hipMemcpyAsync(d_dst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+        compact_cigar_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, d_res, d_cig, d_src,
+                                                                         d_dst, d_out);
+        if ((e = hipGetLastError()) ||
+            (e = hipMemcpyAsync(cigar.data() + base, d_out, (size_t)total * 4,
+                                hipMemcpyDeviceToHost, s)) ||
+            (e = hipStreamSynchronize(s)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR compaction", e);
+    }
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t *r = &res[(size_t)k * kSwResWords];
+        kira_sw_result &o = out[k];
+        o.score = r[0];
+        o.read_end = r[1];
+        o.ref_end = (uint32_t)r[2];
+        o.early_abort = r[3];
+        o.ref_start = (uint32_t)r[4];
+        o.read_start = r[5];
+        o.n_cigar = r[6];
+        o.cells = r[8];
+        o.cigar_off = (int64_t)base + dst[k];
+    }
+    return KIRA_OK;
+}
+
+kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg &cfg,
+                          const kira_sw_knobs &kn, kira_sw_result *out,
+                          std::vector<uint32_t> &cigar) {
+    const int32_t n = j.n;
+    if (n == 0) return KIRA_OK;
+    kira_sw_plan *plan = nullptr;
+    kira_status st = kira_banded_sw_plan_create(ctx, n, j.read_len, j.band, &cfg, &plan);
+    if (st != KIRA_OK) return st;
+    struct PlanGuard {
+        kira_sw_plan *p;
+        ~PlanGuard() { kira_banded_sw_plan_destroy(p); }
+    } pg{plan};
+    DevMem dm;
+    int64_t *d_ro, *d_fo;
+    int32_t *d_rl, *d_fl, *d_off, *d_band = nullptr, *d_ab, *d_res;
+    uint32_t *d_cig;
+    hipError_t e;
+    hipStream_t s = ctx->stream;
+    if ((e = dm.alloc(&d_ro, n)) || (e = dm.alloc(&d_fo, n)) || (e = dm.alloc(&d_rl, n)) ||
+        (e = dm.alloc(&d_fl, n)) || (e = dm.alloc(&d_off, n)) || (e = dm.alloc(&d_ab, n)) ||
+        (e = dm.alloc(&d_res, (size_t)n * kSwResWords)) ||
+        (e = dm.alloc(&d_cig, (size_t)plan->L.cigar_words)) ||
+        (j.band && (e = dm.alloc(&d_band, n))))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
+    auto up = [&](void *d, const void *h, size_t bytes) {
+        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    };
+    if ((e = up(d_ro, j.read_off, (size_t)n * 8)) || (e = up(d_fo, j.ref_off, (size_t)n * 8)) ||
+        (e = up(d_rl, j.read_len, (size_t)n * 4)) || (e = up(d_fl, j.ref_len, (size_t)n * 4)) ||
+        (e = up(d_off, j.offset, (size_t)n * 4)) || (e = up(d_ab, j.abort_score, (size_t)n * 4)) ||
+        (j.band && (e = up(d_band, j.band, (size_t)n * 4))))
+        return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+    kira_sw_jobs_dev jd{j.d_reads, d_ro, d_rl, j.d_refs, d_fo, d_fl, d_off,
+                        d_band,    d_ab, d_res, d_cig};
+    if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, &cfg, &kn)) != KIRA_OK) return st;
+    return fetch_sw_results(ctx, n, d_res, d_cig, plan->L.cigar_off.data(), true, out, cigar);
+}
+
+}  // namespace kira
+
+extern "C" {
+
 kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
                                  int64_t reads_bytes, const int64_t *read_off,
                                  const int32_t *read_len, const uint8_t *refs,
@@ -397,81 +527,29 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
             ref_off[k] < 0 || ref_len[k] < 0 || ref_off[k] + ref_len[k] > refs_bytes)
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "job outside its buffers");
     }
-    kira_sw_plan *plan = nullptr;
-    kira_status st = kira_banded_sw_plan_create(ctx, n, read_len, band, cfg, &plan);
-    if (st != KIRA_OK) return st;
-    struct Guard {
-        kira_sw_plan *p;
-        std::vector<void *> d;
-        ~Guard() {
-            for (void *x : d) (void)hipFree(x);
-            kira_banded_sw_plan_destroy(p);
-        }
-    } g{plan, {}};
+    DevMem dm;
     uint8_t *d_reads, *d_refs;
-    int64_t *d_roff, *d_foff;
-    int32_t *d_rlen, *d_flen, *d_off, *d_band = nullptr, *d_abort, *d_res;
-    uint32_t *d_cig;
     hipError_t e;
+    hipStream_t s = ctx->stream;
     std::vector<int64_t> foff(n);
     for (int32_t k = 0; k < n; k++) foff[k] = ref_off[k] + kira::kRefPad;
-    hipStream_t s = ctx->stream;
-    if ((e = dev_alloc(&d_reads, (size_t)reads_bytes + 8)) ||
-        (g.d.push_back(d_reads), e = dev_alloc(&d_refs, (size_t)refs_bytes + 2 * kira::kRefPad)) ||
-        (g.d.push_back(d_refs), e = dev_alloc(&d_roff, n)) ||
-        (g.d.push_back(d_roff), e = dev_alloc(&d_foff, n)) ||
-        (g.d.push_back(d_foff), e = dev_alloc(&d_rlen, n)) ||
-        (g.d.push_back(d_rlen), e = dev_alloc(&d_flen, n)) ||
-        (g.d.push_back(d_flen), e = dev_alloc(&d_off, n)) ||
-        (g.d.push_back(d_off), e = dev_alloc(&d_abort, n)) ||
-        (g.d.push_back(d_abort), e = dev_alloc(&d_res, (size_t)n * kira::kSwResWords)) ||
-        (g.d.push_back(d_res), e = dev_alloc(&d_cig, (size_t)plan->L.cigar_words)) ||
-        (g.d.push_back(d_cig), band && (e = dev_alloc(&d_band, n))))
+    if ((e = dm.alloc(&d_reads, (size_t)reads_bytes + 8)) ||
+        (e = dm.alloc(&d_refs, (size_t)refs_bytes + 2 * kira::kRefPad)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "batch allocation", e);
-    if (d_band) g.d.push_back(d_band);
     if ((e = hipMemsetAsync(d_refs, 0, (size_t)refs_bytes + 2 * kira::kRefPad, s)) ||
         (reads_bytes && (e = hipMemcpyAsync(d_reads, reads, (size_t)reads_bytes,
                                             hipMemcpyHostToDevice, s))) ||
         (refs_bytes && (e = hipMemcpyAsync(d_refs + kira::kRefPad, refs, (size_t)refs_bytes,
-                                           hipMemcpyHostToDevice, s))) ||
-        (e = hipMemcpyAsync(d_roff, read_off, (size_t)n * 8, hipMemcpyHostToDevice, s)) ||
-        (e = hipMemcpyAsync(d_foff, foff.data(), (size_t)n * 8, hipMemcpyHostToDevice, s)) ||
-        (e = hipMemcpyAsync(d_rlen, read_len, (size_t)n * 4, hipMemcpyHostToDevice, s)) ||
-        (e = hipMemcpyAsync(d_flen, ref_len, (size_t)n * 4, hipMemcpyHostToDevice, s)) ||
-        (e = hipMemcpyAsync(d_off, offset, (size_t)n * 4, hipMemcpyHostToDevice, s)) ||
-        (e = hipMemcpyAsync(d_abort, abort_score, (size_t)n * 4, hipMemcpyHostToDevice, s)) ||
-        (d_band && (e = hipMemcpyAsync(d_band, band, (size_t)n * 4, hipMemcpyHostToDevice, s))))
+                                           hipMemcpyHostToDevice, s))))
         return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
-    kira_sw_jobs_dev jd{d_reads, d_roff, d_rlen, d_refs, d_foff, d_flen, d_off,
-                        d_band, d_abort, d_res, d_cig};
-    if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, cfg, kn)) != KIRA_OK) return st;
-    std::vector<int32_t> res((size_t)n * kira::kSwResWords);
-    std::vector<uint32_t> cig((size_t)plan->L.cigar_words);
-    if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
-        (e = hipMemcpyAsync(cig.data(), d_cig, cig.size() * 4, hipMemcpyDeviceToHost, s)) ||
-        (e = hipStreamSynchronize(s)))
-        return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
-    int64_t pos = 0;
-    for (int32_t k = 0; k < n; k++) {
-        const int32_t *r = &res[(size_t)k * kira::kSwResWords];
-        kira_sw_result &o = out[k];
-        o.score = r[0];
-        o.read_end = r[1];
-        o.ref_end = (uint32_t)r[2];
-        o.early_abort = r[3];
-        o.ref_start = (uint32_t)r[4];
-        o.read_start = r[5];
-        o.n_cigar = r[6];
-        o.cells = r[8];
-        if (r[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
-        if (r[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
-        if (pos + r[6] > arena_cap || (r[6] > 0 && !cigar_arena))
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "cigar arena too small");
-        o.cigar_off = pos;
-        if (r[6] > 0)
-            memcpy(cigar_arena + pos, &cig[(size_t)plan->L.cigar_off[k]], (size_t)r[6] * 4);
-        pos += r[6];
-    }
+    kira::SwJobsHost j{n, d_reads, read_off, read_len, d_refs, foff.data(), ref_len, offset, band,
+                       abort_score};
+    std::vector<uint32_t> cig;
+    kira_status st = kira::banded_sw_dev(ctx, j, *cfg, *kn, out, cig);
+    if (st != KIRA_OK) return st;
+    if ((int64_t)cig.size() > arena_cap || (!cig.empty() && !cigar_arena))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "cigar arena too small");
+    if (!cig.empty()) memcpy(cigar_arena, cig.data(), cig.size() * 4);
     return KIRA_OK;
 }
 
