@@ -12,6 +12,7 @@
 #include <chrono>
 #include <new>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -143,6 +144,44 @@ struct Rec {
     kira_stage4_aln a;
 };
 
+// Records made off the main thread: CIGAR / MD offsets are relative to the
+// arena until it is merged into the result.
+struct Arena {
+    std::vector<uint32_t> cig;
+    std::string md;
+    std::vector<Rec> recs;
+};
+
+int host_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h ? h : 1u, 16u));
+}
+
+// f(t, begin, end) on up to host_threads() threads over [0, n) in chunks of
+// at least `grain`; a std::bad_alloc in any chunk is rethrown here.
+template <class F>
+void parallel_chunks(int64_t n, int64_t grain, F &&f) {
+    if (n <= 0) return;
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), (n + grain - 1) / grain));
+    if (T == 1) {
+        f(0, (int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<char> failed(T, 0);
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&, t]() {
+            try {
+                f(t, n * t / T, n * (t + 1) / T);
+            } catch (...) {
+                failed[t] = 1;
+            }
+        });
+    for (auto &x : th) x.join();
+    for (int t = 0; t < T; t++)
+        if (failed[t]) throw std::bad_alloc();
+}
+
 // a chain sent to the prefilter (and the exact-match check)
 struct Sel {
     int32_t read;
@@ -190,7 +229,26 @@ struct Driver {
     }
     const uint8_t *contig(uint32_t id) const { return g->host.data() + contig_off[id]; }
 
-    void push(int32_t read, const kira_stage4_aln &a) { recs.push_back(Rec{read, a}); }
+    // Builds the records of items [0, n) in parallel (f(i, arena) appends
+    // zero or more) and appends them to `recs` in item order.
+    template <class F>
+    void build(int64_t n, F &&f) {
+        if (n <= 0) return;
+        std::vector<Arena> ar((size_t)host_threads());
+        parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; i++) f(i, ar[t]);
+        });
+        for (Arena &A : ar) {
+            const int64_t bc = (int64_t)R.cigar.size(), bm = (int64_t)R.md.size();
+            R.cigar.insert(R.cigar.end(), A.cig.begin(), A.cig.end());
+            R.md += A.md;
+            for (Rec &x : A.recs) {
+                x.a.cigar_off += bc;
+                x.a.md_off += bm;
+                recs.push_back(x);
+            }
+        }
+    }
 
     kira_stage4_aln base_aln(int32_t kind, const kira_chain &ch) const {
         kira_stage4_aln a{};
@@ -201,7 +259,7 @@ struct Driver {
     }
 
     // exact_match_alignment (mod.rs:778-815)
-    kira_stage4_aln exact_aln(int32_t r, const kira_chain &ch) {
+    kira_stage4_aln exact_aln(int32_t r, const kira_chain &ch, Arena &A) const {
         const int32_t L = read_len[r];
         const uint8_t *q = oriented(r, ch.strand == 1);
         kira_stage4_aln a = base_aln(0, ch);
@@ -214,19 +272,19 @@ struct Driver {
         uint32_t nm = 0;
         for (int32_t k = 0; k < L; k++) nm += identical_is_edit(q[k]) ? 1 : 0;
         a.nm = nm;
-        a.cigar_off = (int64_t)R.cigar.size();
+        a.cigar_off = (int64_t)A.cig.size();
         a.n_cigar = 1;
-        R.cigar.push_back(((uint32_t)L << 4) | kOpM);
-        a.md_off = (int64_t)R.md.size();
-        R.md += std::to_string(L);
-        a.md_len = (int32_t)(R.md.size() - a.md_off);
+        A.cig.push_back(((uint32_t)L << 4) | kOpM);
+        a.md_off = (int64_t)A.md.size();
+        A.md += std::to_string(L);
+        a.md_len = (int32_t)(A.md.size() - a.md_off);
         return a;
     }
 
     // build_alignment (mod.rs:1200-1254) of an SW result over window
     // [win, win + wlen) of the chain's contig
     kira_stage4_aln sw_aln(int32_t r, const kira_chain &ch, int64_t win, int32_t wlen,
-                           const kira_sw_result &sw, const uint32_t *ops) {
+                           const kira_sw_result &sw, const uint32_t *ops, Arena &A) const {
         kira_stage4_aln a = base_aln(1, ch);
         const int32_t L = read_len[r];
         a.ref_start = (uint32_t)win + sw.ref_start;
@@ -235,12 +293,12 @@ struct Driver {
         a.read_end = (uint32_t)sw.read_end;
         a.score = sw.score;
         a.mapq = 0;
-        a.cigar_off = (int64_t)R.cigar.size();
-        a.md_off = (int64_t)R.md.size();
+        a.cigar_off = (int64_t)A.cig.size();
+        a.md_off = (int64_t)A.md.size();
         a.nm = kira::build_record(oriented(r, ch.strand == 1), L, contig(ch.ref_id) + win, wlen,
-                                  sw, ops, R.cigar, R.md);
-        a.n_cigar = (int32_t)(R.cigar.size() - a.cigar_off);
-        a.md_len = (int32_t)(R.md.size() - a.md_off);
+                                  sw, ops, A.cig, A.md);
+        a.n_cigar = (int32_t)(A.cig.size() - a.cigar_off);
+        a.md_len = (int32_t)(A.md.size() - a.md_off);
         return a;
     }
 
@@ -251,11 +309,12 @@ struct Driver {
                (int64_t)ch.ref_start + L <= contig_len[ch.ref_id];
     }
 
-    // Banded SW over device-resident reads and windows; appends one record
-    // per job.  offset: the band centre offset of each job.
-    kira_status banded(const std::vector<Job> &jobs, const std::vector<int32_t> &offset,
-                       bool supplementary, bool count) {
+    // Banded SW over device-resident reads and windows (offset: each job's
+    // band centre offset); results and CIGARs in job order.
+    kira_status banded_run(const std::vector<Job> &jobs, const std::vector<int32_t> &offset,
+                           std::vector<kira_sw_result> &res, std::vector<uint32_t> &cig) {
         const int32_t m = (int32_t)jobs.size();
+        res.resize(m);
         if (m == 0) return KIRA_OK;
         std::vector<int64_t> ro(m), fo(m);
         std::vector<int32_t> rl(m), fl(m), ab(m);
@@ -268,24 +327,20 @@ struct Driver {
             fl[t] = j.wlen;
             ab[t] = j.abort;
         }
-        kira::SwJobsHost sj{m,         d_orient,  ro.data(), rl.data(), g->ascii,
-                            fo.data(), fl.data(), offset.data(), nullptr, ab.data()};
-        std::vector<kira_sw_result> res(m);
-        std::vector<uint32_t> cig;
-        kira_status st = kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, res.data(), cig);
-        if (st != KIRA_OK) return st;
-        for (int32_t t = 0; t < m; t++) {
-            const Job &j = jobs[t];
-            kira_stage4_aln a = sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
-                                       cig.data() + res[t].cigar_off);
-            a.is_supplementary = supplementary ? 1 : 0;
-            push(j.read, a);
-            if (count) {
-                R.stats.dp_scalar++;
-                if (res[t].early_abort) R.stats.dp_early_abort++;
-            }
-        }
-        return KIRA_OK;
+        kira::SwJobsHost sj{m,         d_orient,  ro.data(),     rl.data(), g->ascii,
+                            fo.data(), fl.data(), offset.data(), nullptr,   ab.data()};
+        return kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, res.data(), cig);
+    }
+
+    // Stable grouping of `recs` by read; first[r] .. first[r+1] are read r's.
+    void sort_by_read(std::vector<int64_t> &first) {
+        first.assign((size_t)n + 1, 0);
+        for (const Rec &x : recs) first[x.read + 1]++;
+        for (int32_t r = 0; r < n; r++) first[r + 1] += first[r];
+        std::vector<int64_t> pos(first.begin(), first.end() - 1);
+        std::vector<Rec> out(recs.size());
+        for (const Rec &x : recs) out[pos[x.read]++] = x;
+        recs.swap(out);
     }
 
     // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
@@ -332,13 +387,17 @@ kira_status Driver::run() {
         total += 2 * (int64_t)read_len[r];
     }
     orient.resize((size_t)total);
-    for (int32_t r = 0; r < n; r++) {
-        const uint8_t *src = reads + read_off[r];
-        uint8_t *f = orient.data() + ooff[r];
-        const int32_t L = read_len[r];
-        memcpy(f, src, (size_t)L);
-        for (int32_t k = 0; k < L; k++) f[L + k] = comp_base(src[L - 1 - k]);
-    }
+    uint8_t comp[256];
+    for (int b = 0; b < 256; b++) comp[b] = comp_base((uint8_t)b);
+    parallel_chunks(n, 4096, [&](int, int64_t b, int64_t e) {
+        for (int64_t r = b; r < e; r++) {
+            const uint8_t *src = reads + read_off[r];
+            uint8_t *f = orient.data() + ooff[r];
+            const int32_t L = read_len[r];
+            memcpy(f, src, (size_t)L);
+            for (int32_t k = 0; k < L; k++) f[L + k] = comp[src[L - 1 - k]];
+        }
+    });
 
     // chain selection (:549-708, two-tier and mate guidance off)
     std::vector<Sel> sels;
@@ -472,6 +531,7 @@ kira_status Driver::run() {
     // outcomes in chain order (:758-934)
     std::vector<Job> simd, scalar;
     std::vector<uint8_t> dp_used(n, 0);
+    std::vector<int32_t> accepted;  // selected chains that end in a record here
     const int32_t band = std::max(c.cfg.bandwidth, 1);
     for (int32_t k = 0; k < ns; k++) {
         const Sel &x = sels[k];
@@ -479,31 +539,15 @@ kira_status Driver::run() {
         const int32_t L = read_len[x.read];
         if (exact[k]) {
             st.exact_matches++;
-            push(x.read, exact_aln(x.read, ch));
+            accepted.push_back(k);
             continue;
         }
         const kira_prefilter_out &o = pf[k];
         st.reason_counts[o.reason < 9 ? o.reason : 9]++;
         const int32_t abort = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
         if (o.result == 0) {
-            // build_ungapped_alignment (prefilter.rs:455-517)
             st.prefilter_accept++;
-            kira_stage4_aln a = base_aln(0, ch);
-            a.ref_start = o.aln_ref_start;
-            a.ref_end = o.aln_ref_end;
-            a.read_start = o.aln_read_start;
-            a.read_end = o.aln_read_end;
-            a.score = o.aln_score;
-            a.mapq = 60;
-            a.cigar_off = (int64_t)R.cigar.size();
-            a.n_cigar = o.n_cigar;
-            R.cigar.insert(R.cigar.end(), o.cigar, o.cigar + o.n_cigar);
-            a.md_off = (int64_t)R.md.size();
-            a.nm = kira::ungapped_tags(oriented(x.read, ch.strand == 1) + o.aln_read_start,
-                                       contig(ch.ref_id) + o.aln_ref_start,
-                                       (int64_t)o.aln_read_end - o.aln_read_start, R.md);
-            a.md_len = (int32_t)(R.md.size() - a.md_off);
-            push(x.read, a);
+            accepted.push_back(k);
             continue;
         }
         if (o.result == 1) {
@@ -527,6 +571,33 @@ kira_status Driver::run() {
         scalar.push_back(clamp_job(x.read, x.chain, abort));
     }
     for (int32_t r = 0; r < n; r++) st.dp_reads += dp_used[r];
+    build((int64_t)accepted.size(), [&](int64_t i, Arena &A) {
+        const int32_t k = accepted[i];
+        const Sel &x = sels[k];
+        const kira_chain &ch = chains[x.chain];
+        if (exact[k]) {
+            A.recs.push_back(Rec{x.read, exact_aln(x.read, ch, A)});
+            return;
+        }
+        // build_ungapped_alignment (prefilter.rs:455-517)
+        const kira_prefilter_out &o = pf[k];
+        kira_stage4_aln a = base_aln(0, ch);
+        a.ref_start = o.aln_ref_start;
+        a.ref_end = o.aln_ref_end;
+        a.read_start = o.aln_read_start;
+        a.read_end = o.aln_read_end;
+        a.score = o.aln_score;
+        a.mapq = 60;
+        a.cigar_off = (int64_t)A.cig.size();
+        a.n_cigar = o.n_cigar;
+        A.cig.insert(A.cig.end(), o.cigar, o.cigar + o.n_cigar);
+        a.md_off = (int64_t)A.md.size();
+        a.nm = kira::ungapped_tags(oriented(x.read, ch.strand == 1) + o.aln_read_start,
+                                   contig(ch.ref_id) + o.aln_ref_start,
+                                   (int64_t)o.aln_read_end - o.aln_read_start, A.md);
+        a.md_len = (int32_t)(A.md.size() - a.md_off);
+        A.recs.push_back(Rec{x.read, a});
+    });
     R.phase_ms[0] = ms_since(t0);
 
     // gpu_dispatch_spectral (:1272-1358): sieve + certification
@@ -596,9 +667,16 @@ kira_status Driver::run() {
             (e = hipMemcpyAsync(flags.data(), d_flags, nj, hipMemcpyDeviceToHost, s)) ||
             (e = hipStreamSynchronize(s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
-        for (int32_t t = 0; t < nj; t++) {
-            if (!(flags[t] & 4)) continue;
+        std::vector<int32_t> cert;
+        for (int32_t t = 0; t < nj; t++)
+            if (flags[t] & 4) {
+                cert.push_back(t);
+                keep[t] = 0;
+            }
+        st.gpu_spectral_resolved += (int64_t)cert.size();
+        build((int64_t)cert.size(), [&](int64_t i, Arena &A) {
             // build_certified_spectral_alignment (mod.rs:636-686)
+            const int32_t t = cert[i];
             const Job &j = simd[t];
             const kira_chain &ch = chains[j.chain];
             const int32_t L = rl[t], mm = best[t];
@@ -609,17 +687,15 @@ kira_status Driver::run() {
             a.read_end = (uint32_t)L;
             a.score = wrap_i32((int64_t)(L - mm) * c.cfg.match_score - (int64_t)mm * c.cfg.mismatch);
             a.mapq = 0;
-            a.cigar_off = (int64_t)R.cigar.size();
+            a.cigar_off = (int64_t)A.cig.size();
             a.n_cigar = 1;
-            R.cigar.push_back(((uint32_t)L << 4) | kOpM);
-            a.md_off = (int64_t)R.md.size();
+            A.cig.push_back(((uint32_t)L << 4) | kOpM);
+            a.md_off = (int64_t)A.md.size();
             a.nm = kira::ungapped_tags(oriented(j.read, ch.strand == 1),
-                                       contig(ch.ref_id) + j.win + shift[t], L, R.md);
-            a.md_len = (int32_t)(R.md.size() - a.md_off);
-            push(j.read, a);
-            st.gpu_spectral_resolved++;
-            keep[t] = 0;
-        }
+                                       contig(ch.ref_id) + j.win + shift[t], L, A.md);
+            a.md_len = (int32_t)(A.md.size() - a.md_off);
+            A.recs.push_back(Rec{j.read, a});
+        });
     }
     R.phase_ms[1] = ms_since(t0);
 
@@ -686,10 +762,12 @@ kira_status Driver::run() {
             if (kst != KIRA_OK) return kst;
             for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
         }
-        for (int32_t i = 0; i < m; i++) {
+        build(m, [&](int64_t i, Arena &A) {
             const Job &j = simd[order[i]];
-            push(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
-                                cig.data() + out[i].cigar_off));
+            A.recs.push_back(Rec{j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
+                                                cig.data() + out[i].cigar_off, A)});
+        });
+        for (int32_t i = 0; i < m; i++) {
             st.dp_simd++;
             if (out[i].early_abort) st.dp_early_abort++;
         }
@@ -698,16 +776,26 @@ kira_status Driver::run() {
 
     // scalar jobs: align_chain_with_meta (stage4:1196-1205)
     t0 = Clock::now();
-    kira_status kst = banded(scalar, chain_offsets(scalar), false, true);
-    if (kst != KIRA_OK) return kst;
+    {
+        std::vector<kira_sw_result> res;
+        std::vector<uint32_t> cig;
+        kira_status kst = banded_run(scalar, chain_offsets(scalar), res, cig);
+        if (kst != KIRA_OK) return kst;
+        build((int64_t)scalar.size(), [&](int64_t t, Arena &A) {
+            const Job &j = scalar[t];
+            A.recs.push_back(Rec{j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
+                                                cig.data() + res[t].cigar_off, A)});
+        });
+        for (const kira_sw_result &x : res) {
+            st.dp_scalar++;
+            if (x.early_abort) st.dp_early_abort++;
+        }
+    }
     R.phase_ms[3] = ms_since(t0);
 
     t0 = Clock::now();
-    std::stable_sort(recs.begin(), recs.end(),
-                     [](const Rec &x, const Rec &y) { return x.read < y.read; });
-    std::vector<int64_t> first(n + 1, 0);
-    for (const Rec &x : recs) first[x.read + 1]++;
-    for (int32_t r = 0; r < n; r++) first[r + 1] += first[r];
+    std::vector<int64_t> first;
+    sort_by_read(first);
 
     // XS from the chain-score proxy (:1215-1250)
     for (int32_t r = 0; r < n; r++) {
@@ -725,6 +813,13 @@ kira_status Driver::run() {
 
     // emit_supplementary_alignments (split_read.rs:22-57)
     if (c.supplementary) {
+        // picks in the order split_read.rs appends them: job -1 = exact match
+        struct Pick {
+            int32_t read;
+            int64_t chain;
+            int32_t job;
+        };
+        std::vector<Pick> all;
         std::vector<Job> sj;
         for (int32_t r = 0; r < n; r++) {
             const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
@@ -768,24 +863,36 @@ kira_status Driver::run() {
                 if (exact_candidate(r, ch) &&
                     memcmp(oriented(r, ch.strand == 1), contig(ch.ref_id) + ch.ref_start,
                            (size_t)read_len[r]) == 0) {
-                    kira_stage4_aln a = exact_aln(r, ch);
-                    a.is_supplementary = 1;
-                    push(r, a);
-                    st.supplementary++;
+                    all.push_back(Pick{r, q, -1});
                     continue;
                 }
+                all.push_back(Pick{r, q, (int32_t)sj.size()});
                 sj.push_back(clamp_job(r, q, INT32_MIN / 8));
             }
         }
-        if ((kst = banded(sj, chain_offsets(sj), true, false)) != KIRA_OK) return kst;
-        st.supplementary += (int64_t)sj.size();
-        std::stable_sort(recs.begin(), recs.end(),
-                         [](const Rec &x, const Rec &y) { return x.read < y.read; });
+        std::vector<kira_sw_result> res;
+        std::vector<uint32_t> cig;
+        kira_status kst = banded_run(sj, chain_offsets(sj), res, cig);
+        if (kst != KIRA_OK) return kst;
+        build((int64_t)all.size(), [&](int64_t i, Arena &A) {
+            const Pick &p = all[i];
+            const kira_chain &ch = chains[p.chain];
+            kira_stage4_aln a;
+            if (p.job < 0) {
+                a = exact_aln(p.read, ch, A);
+            } else {
+                const Job &j = sj[p.job];
+                a = sw_aln(p.read, ch, j.win, j.wlen, res[p.job], cig.data() + res[p.job].cigar_off,
+                           A);
+            }
+            a.is_supplementary = 1;
+            A.recs.push_back(Rec{p.read, a});
+        });
+        st.supplementary += (int64_t)all.size();
+        sort_by_read(first);
     }
 
-    R.aln_off.assign(n + 1, 0);
-    for (const Rec &x : recs) R.aln_off[x.read + 1]++;
-    for (int32_t r = 0; r < n; r++) R.aln_off[r + 1] += R.aln_off[r];
+    R.aln_off = first;
     R.alns.resize(recs.size());
     for (size_t k = 0; k < recs.size(); k++) R.alns[k] = recs[k].a;
     for (int32_t r = 0; r < n; r++) st.unmapped += R.aln_off[r + 1] == R.aln_off[r] ? 1 : 0;
