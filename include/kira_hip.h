@@ -552,6 +552,61 @@ kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_conti
 kira_status kira_stage4_result_view(const kira_stage4_result *r, kira_stage4_view *view);
 void kira_stage4_result_free(kira_stage4_result *r);
 
+/* ------------------------------------------------------------------ records out */
+
+/*
+ * left_normalize_indels (src/alignment/normalize.rs:55-190), host code: slide
+ * every I / D of `cigar` (n_ops words len << 4 | op) as far left as the
+ * contig (0-based ref_start = the alignment's first reference base) and the
+ * strand-oriented read allow.  Returns 1 and writes the new CIGAR to out
+ * (*out_n words; out_cap >= 2 * n_ops + 1 always suffices) when anything
+ * moved, 0 (*out_n = n_ops, out untouched) when the CIGAR is already
+ * canonical, -1 on invalid arguments or too small an out_cap.
+ */
+int32_t kira_left_normalize_indels(const uint32_t *cigar, int32_t n_ops, const uint8_t *contig,
+                                   int64_t contig_len, uint32_t ref_start, const uint8_t *read,
+                                   int32_t read_len, uint32_t *out, int32_t out_cap,
+                                   int32_t *out_n);
+
+/*
+ * normalize_alignments (normalize.rs:197-249) over a stage-4 result, in
+ * place: every record's CIGAR is left-normalized against its contig and the
+ * read in the record's orientation (reads as passed to kira_stage4_run), and
+ * NM / MD are recomputed for the records that moved.  Host code.
+ */
+kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome *g,
+                                       int32_t n_contigs, const int64_t *contig_off,
+                                       const int64_t *contig_len, int32_t n_reads,
+                                       const uint8_t *reads, const int64_t *read_off,
+                                       const int32_t *read_len);
+
+/* The OutputConfig fields single-end records use (src/io/mod.rs:797-850):
+ * tags NM / MD / AS / XS / RG, bwa-mem -M (split_as_secondary) and -Y
+ * (soft_clip_supplementary); read_group is the @RG header line or NULL. */
+typedef struct {
+    int32_t write_nm, write_md, write_as, write_xs, write_rg;
+    int32_t split_as_secondary, soft_clip_supplementary;
+    const char *read_group;
+} kira_sam_cfg;
+
+/*
+ * SAM text of a stage-4 result, host code: for each read in order, every
+ * record through SamFormatter::append_alignment (src/io/mod.rs:1036-1140)
+ * for a single-end read, or one append_unmapped line (:971-1033) when it has
+ * none.  MAPQ is the record's own (stage 5 is not run); no XA / SA tags.
+ * Read k: id read_ids[k], SEQ seqs[seq_off[k] .. +seq_len[k]] (forward, as
+ * aligned), QUAL quals[seq_off[k] ..] or none when quals is NULL.  A record
+ * whose CIGAR does not consume the read (checked_cigar, :1737-1750) fails
+ * the call with KIRA_ERR_INVALID_ARGUMENT.  *used = bytes of text; with out
+ * NULL only the size is computed; cap too small returns
+ * KIRA_ERR_INVALID_ARGUMENT.
+ */
+kira_status kira_stage4_sam(const kira_stage4_result *r, int32_t n_contigs,
+                            const char *const *contig_names, int32_t n_reads,
+                            const char *const *read_ids, const uint8_t *seqs,
+                            const int64_t *seq_off, const int32_t *seq_len, const uint8_t *quals,
+                            const kira_sam_cfg *cfg, char *out, int64_t cap, int64_t *used);
+
 #ifdef __cplusplus
 }
 #endif

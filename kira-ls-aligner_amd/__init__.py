@@ -216,6 +216,15 @@ class Stage4View(C.Structure):
                 ("md_bytes", C.c_int64), ("stats", Stage4Stats), ("phase_ms", C.c_double * 5)]
 
 
+class SamCfg(C.Structure):
+    """kira_sam_cfg: the OutputConfig fields of single-end records (src/io/mod.rs:797-850)."""
+
+    _fields_ = [(n, C.c_int32) for n in ("write_nm", "write_md", "write_as", "write_xs",
+                                          "write_rg", "split_as_secondary",
+                                          "soft_clip_supplementary")] + [
+        ("read_group", C.c_char_p)]
+
+
 class SwJobsDev(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
                                           "ref_len", "offset", "band", "abort_score", "res",
@@ -275,6 +284,12 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p)]),
     "kira_stage4_result_view": (C.c_int, [_P, C.POINTER(Stage4View)]),
     "kira_stage4_result_free": (None, [_P]),
+    "kira_left_normalize_indels": (C.c_int32, [_P, C.c_int32, _P, C.c_int64, C.c_uint32, _P,
+                                               C.c_int32, _P, C.c_int32,
+                                               C.POINTER(C.c_int32)]),
+    "kira_stage4_left_normalize": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P]),
+    "kira_stage4_sam": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P, _P,
+                                  C.POINTER(SamCfg), _P, C.c_int64, C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -1038,6 +1053,44 @@ class Stage4Result:
             out.append(recs)
         return out
 
+    def _refresh(self):
+        v = Stage4View()
+        st = lib().kira_stage4_result_view(self.h, C.byref(v))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_result_view")
+        self.view = v
+
+    def left_normalize(self, genome: "Genome", x: "Stage4Inputs"):
+        """normalize_alignments (src/alignment/normalize.rs:197-249) in place."""
+        st = lib().kira_stage4_left_normalize(self.h, genome.h, len(x.co), _np_ptr(x.co),
+                                              _np_ptr(x.cl), x.n, _np_ptr(x.rb), _np_ptr(x.ro),
+                                              _np_ptr(x.rl))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_left_normalize")
+        self._refresh()
+
+    def sam(self, contig_names, read_ids, x: "Stage4Inputs", quals=None,
+            cfg: Optional[SamCfg] = None) -> bytes:
+        """SamFormatter::append_alignment / append_unmapped text (src/io/mod.rs:971-1140)
+        of every read, single-end.  quals: list of bytes per read or None."""
+        cfg = cfg or sam_cfg()
+        names = (C.c_char_p * max(len(contig_names), 1))(*[n.encode() for n in contig_names])
+        ids = (C.c_char_p * max(x.n, 1))(*[i.encode() for i in read_ids])
+        qb = None
+        if quals is not None:
+            qb = np.frombuffer(b"".join(quals), np.uint8) if x.rl.sum() else np.zeros(1, np.uint8)
+        used = C.c_int64()
+        args = [self.h, len(contig_names), names, x.n, ids, _np_ptr(x.rb), _np_ptr(x.ro),
+                _np_ptr(x.rl), _np_ptr(qb) if qb is not None else None, C.byref(cfg)]
+        st = lib().kira_stage4_sam(*args, None, 0, C.byref(used))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_sam")
+        buf = C.create_string_buffer(max(used.value, 1))
+        st = lib().kira_stage4_sam(*args, buf, used.value, C.byref(used))
+        if st != KIRA_OK:
+            raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_sam")
+        return buf.raw[:used.value]
+
     def close(self):
         if self.h:
             lib().kira_stage4_result_free(self.h)
@@ -1098,3 +1151,29 @@ def stage4_run(ctx: Context, genome: Genome, contig_off, contig_len=None, reads=
                                x.ch.ctypes.data_as(C.POINTER(Chain)), C.byref(cfg), C.byref(h))
     ctx.check(st, "stage4_run")
     return Stage4Result(h)
+
+
+def sam_cfg(write_nm=True, write_md=True, write_as=True, write_xs=True, write_rg=True,
+            split_as_secondary=False, soft_clip_supplementary=False,
+            read_group: Optional[str] = None) -> SamCfg:
+    """OutputConfig::full() defaults (src/io/mod.rs:833-850)."""
+    return SamCfg(int(write_nm), int(write_md), int(write_as), int(write_xs), int(write_rg),
+                  int(split_as_secondary), int(soft_clip_supplementary),
+                  read_group.encode() if read_group is not None else None)
+
+
+def left_normalize_indels(cigar, contig: bytes, ref_start: int, read: bytes):
+    """left_normalize_indels (src/alignment/normalize.rs:55-190), host code.
+    cigar: list of len << 4 | op words.  Returns the new CIGAR or None."""
+    n = len(cigar)
+    ops = np.array(cigar if cigar else [0], np.uint32)
+    out = np.zeros(2 * n + 1, np.uint32)
+    cb = np.frombuffer(contig, np.uint8) if contig else np.zeros(1, np.uint8)
+    rb = np.frombuffer(read, np.uint8) if read else np.zeros(1, np.uint8)
+    on = C.c_int32()
+    r = lib().kira_left_normalize_indels(_np_ptr(ops), n, _np_ptr(cb), len(contig), ref_start,
+                                         _np_ptr(rb), len(read), _np_ptr(out), len(out),
+                                         C.byref(on))
+    if r < 0:
+        raise HipError("InvalidArgument", "left_normalize_indels")
+    return [int(v) for v in out[:on.value]] if r == 1 else None

@@ -263,6 +263,14 @@ uint32_t build_record(const uint8_t *read, int32_t L, const uint8_t *win, int64_
                       const kira_sw_result &s, const uint32_t *ops, std::vector<uint32_t> &cig,
                       std::string &md);
 uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md);
+// compute_nm_md (src/alignment/mod.rs:1774-1785): replay `ops` from read
+// position 0 and reference position ref_pos; MD appended to `md`, NM returned.
+uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_len, int64_t ref_pos,
+               const uint32_t *ops, int32_t n, std::string &md);
+// left_normalize_indels (src/alignment/normalize.rs:55-190) in place on a
+// CIGAR over a whole contig and the strand-oriented read; true if it moved.
+bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
+                    uint32_t ref_start, const uint8_t *read, int64_t read_len);
 
 }  // namespace kira
 

@@ -99,6 +99,94 @@ uint32_t build_record(const uint8_t *read, int32_t L, const uint8_t *win, int64_
     return md.nm;
 }
 
+uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_len, int64_t ref_pos,
+               const uint32_t *ops, int32_t n, std::string &md_out) {
+    MdText md;
+    replay(read, (size_t)L, ref, (size_t)ref_len, (size_t)ref_pos, ops, n, md);
+    md_out += md.text;
+    return md.nm;
+}
+
+bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
+                    uint32_t ref_start, const uint8_t *read, int64_t read_len) {
+    auto op_of = [](uint32_t w) { return w & 15u; };
+    auto len_of = [](uint32_t w) { return w >> 4; };
+    auto make = [](uint32_t len, uint32_t op) { return (len << 4) | op; };
+    bool any = false;
+    for (uint32_t w : ops)
+        if (op_of(w) == kOpI || op_of(w) == kOpD) any = true;
+    if (!any) return false;
+    // bytes past either sequence compare as absent (Option::None)
+    auto rb = [&](int64_t i) -> int { return i >= 0 && i < ref_len ? ref[i] : -1; };
+    auto qb = [&](int64_t i) -> int { return i >= 0 && i < read_len ? read[i] : -1; };
+    auto acgt = [](int b) { return b == 'A' || b == 'C' || b == 'G' || b == 'T'; };
+    const int64_t rs = ref_start;
+    bool changed = false;
+    size_t i = 0;
+    while (i < ops.size()) {
+        const uint32_t kind = op_of(ops[i]);
+        if (kind != kOpI && kind != kOpD) {
+            i++;
+            continue;
+        }
+        const int64_t g = len_of(ops[i]);
+        // read / reference position of this op's start
+        int64_t bq = 0, br = 0;
+        for (size_t k = 0; k < i; k++) {
+            const uint32_t o = op_of(ops[k]), l = len_of(ops[k]);
+            if (o == kOpM || o == kOpI || o == kOpS) bq += l;
+            if (o == kOpM || o == kOpD || o == kOpN) br += l;
+        }
+        // left-shift budget: matching aligned bases right before the block
+        const int64_t m_run = (i > 0 && op_of(ops[i - 1]) == kOpM) ? len_of(ops[i - 1]) : 0;
+        int64_t shift_max = 0;
+        while (shift_max < m_run && br - shift_max >= 1 && bq - shift_max >= 1) {
+            const int r = rb(rs + br - shift_max - 1);
+            if (!acgt(r < 0 ? 'N' : r) || r != qb(bq - shift_max - 1)) break;
+            shift_max++;
+        }
+        // one base per step: the base entering the block on the left must
+        // equal the one leaving it on the right
+        int64_t shift = 0;
+        while (shift < shift_max) {
+            const int before = rb(rs + br - shift - 1);
+            const int after = kind == kOpD ? rb(rs + br + g - shift - 1) : qb(bq + g - shift - 1);
+            if (before != after) break;
+            shift++;
+        }
+        if (shift == 0) {
+            i++;
+            continue;
+        }
+        changed = true;
+        if (shift == m_run) {
+            // the gap crosses the whole M block: swap, merge with a same-kind
+            // gap now on its left, and look at the merged gap again
+            std::swap(ops[i - 1], ops[i]);
+            i--;
+            if (i > 0 && op_of(ops[i - 1]) == kind) {
+                ops[i - 1] = make(len_of(ops[i - 1]) + len_of(ops[i]), kind);
+                ops.erase(ops.begin() + (ptrdiff_t)i);
+                i--;
+            }
+            continue;
+        }
+        ops[i - 1] = make(len_of(ops[i - 1]) - (uint32_t)shift, kOpM);
+        ops.insert(ops.begin() + (ptrdiff_t)i + 1, make((uint32_t)shift, kOpM));
+        i++;
+    }
+    if (!changed) return false;
+    std::vector<uint32_t> compact;
+    for (uint32_t w : ops) {
+        if (!compact.empty() && op_of(compact.back()) == op_of(w))
+            compact.back() = make(len_of(compact.back()) + len_of(w), op_of(w));
+        else
+            compact.push_back(w);
+    }
+    ops.swap(compact);
+    return true;
+}
+
 uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md_out) {
     MdText md;
     for (int64_t k = 0; k < n; k++) md.column(read[k], ref[k]);
@@ -159,4 +247,23 @@ extern "C" kira_status kira_build_alignments(
     if (cigar_used) *cigar_used = cpos;
     if (md_used) *md_used = mpos;
     return fits ? KIRA_OK : KIRA_ERR_INVALID_ARGUMENT;
+}
+
+extern "C" int32_t kira_left_normalize_indels(const uint32_t *cigar, int32_t n_ops,
+                                              const uint8_t *contig, int64_t contig_len,
+                                              uint32_t ref_start, const uint8_t *read,
+                                              int32_t read_len, uint32_t *out, int32_t out_cap,
+                                              int32_t *out_n) {
+    if (n_ops < 0 || (n_ops > 0 && !cigar) || contig_len < 0 || (contig_len > 0 && !contig) ||
+        read_len < 0 || (read_len > 0 && !read) || !out_n)
+        return -1;
+    std::vector<uint32_t> ops(cigar, cigar + n_ops);
+    if (!kira::left_normalize(ops, contig, contig_len, ref_start, read, read_len)) {
+        *out_n = n_ops;
+        return 0;
+    }
+    *out_n = (int32_t)ops.size();
+    if ((int32_t)ops.size() > out_cap || !out) return -1;
+    memcpy(out, ops.data(), ops.size() * 4);
+    return 1;
 }

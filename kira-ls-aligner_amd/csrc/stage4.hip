@@ -976,3 +976,253 @@ kira_status kira_stage4_result_view(const kira_stage4_result *r, kira_stage4_vie
 void kira_stage4_result_free(kira_stage4_result *r) { delete r; }
 
 }  // extern "C"
+
+namespace {
+
+// append_reverse_complement (src/io/mod.rs:1509-1533): IUPAC-aware, case kept
+uint8_t sam_comp(uint8_t b) {
+    switch (b) {
+    case 'A': return 'T';
+    case 'a': return 't';
+    case 'C': return 'G';
+    case 'c': return 'g';
+    case 'G': return 'C';
+    case 'g': return 'c';
+    case 'T': case 'U': return 'A';
+    case 't': case 'u': return 'a';
+    case 'R': return 'Y';
+    case 'Y': return 'R';
+    case 'S': return 'S';
+    case 'W': return 'W';
+    case 'K': return 'M';
+    case 'M': return 'K';
+    case 'B': return 'V';
+    case 'D': return 'H';
+    case 'H': return 'D';
+    case 'V': return 'B';
+    default: return 'N';
+    }
+}
+
+void put_u(std::string &o, uint64_t v) { o += std::to_string(v); }
+void put_i(std::string &o, int64_t v) { o += std::to_string(v); }
+
+// extract_rg_id (src/io/mod.rs:1788-1799)
+std::string rg_id(const char *rg) {
+    std::string t(rg);
+    if (t.compare(0, 4, "@RG\t") == 0) t = t.substr(4);
+    size_t p = 0;
+    while (p <= t.size()) {
+        size_t e = t.find('\t', p);
+        if (e == std::string::npos) e = t.size();
+        if (t.compare(p, 3, "ID:") == 0 && e - p >= 3) return t.substr(p + 3, e - p - 3);
+        p = e + 1;
+    }
+    return t;
+}
+
+const char kOpChar[] = "MIDNS";
+
+}  // namespace
+
+extern "C" {
+
+kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome *g,
+                                       int32_t n_contigs, const int64_t *contig_off,
+                                       const int64_t *contig_len, int32_t n_reads,
+                                       const uint8_t *reads, const int64_t *read_off,
+                                       const int32_t *read_len) {
+    if (!r || !g || n_contigs < 0 || (n_contigs > 0 && (!contig_off || !contig_len)) ||
+        n_reads != (int32_t)r->aln_off.size() - 1 ||
+        (n_reads > 0 && (!reads || !read_off || !read_len)))
+        return KIRA_ERR_INVALID_ARGUMENT;
+    const int64_t glen = (int64_t)g->host.size();
+    for (int32_t i = 0; i < n_contigs; i++)
+        if (contig_off[i] < 0 || contig_len[i] < 0 || contig_off[i] + contig_len[i] > glen)
+            return KIRA_ERR_INVALID_ARGUMENT;
+    for (const kira_stage4_aln &a : r->alns)
+        if (a.ref_id >= (uint32_t)n_contigs) return KIRA_ERR_INVALID_ARGUMENT;
+    try {
+        // normalize_alignments (normalize.rs:197-249): every record of every
+        // read; NM / MD recomputed for the records that moved
+        std::vector<uint32_t> cig;
+        std::string md;
+        cig.reserve(r->cigar.size());
+        md.reserve(r->md.size());
+        std::vector<uint8_t> rc;
+        std::vector<uint32_t> ops;
+        for (int32_t k = 0; k < n_reads; k++) {
+            const uint8_t *fwd = reads + read_off[k];
+            const int32_t L = read_len[k];
+            bool have_rc = false;
+            for (int64_t q = r->aln_off[k]; q < r->aln_off[k + 1]; q++) {
+                kira_stage4_aln &a = r->alns[q];
+                ops.assign(r->cigar.begin() + a.cigar_off,
+                           r->cigar.begin() + a.cigar_off + a.n_cigar);
+                const uint8_t *read = fwd;
+                if (a.is_rev) {
+                    if (!have_rc) {
+                        rc.resize((size_t)L);
+                        for (int32_t t = 0; t < L; t++) rc[t] = comp_base(fwd[L - 1 - t]);
+                        have_rc = true;
+                    }
+                    read = rc.data();
+                }
+                const uint8_t *contig = g->host.data() + contig_off[a.ref_id];
+                const bool moved =
+                    kira::left_normalize(ops, contig, contig_len[a.ref_id], a.ref_start, read, L);
+                const int64_t c0 = (int64_t)cig.size(), m0 = (int64_t)md.size();
+                cig.insert(cig.end(), ops.begin(), ops.end());
+                if (moved)
+                    a.nm = kira::nm_md(read, L, contig, contig_len[a.ref_id], a.ref_start,
+                                       ops.data(), (int32_t)ops.size(), md);
+                else
+                    md.append(r->md, (size_t)a.md_off, (size_t)a.md_len);
+                a.cigar_off = c0;
+                a.n_cigar = (int32_t)ops.size();
+                a.md_off = m0;
+                a.md_len = (int32_t)(md.size() - m0);
+            }
+        }
+        r->cigar.swap(cig);
+        r->md.swap(md);
+    } catch (const std::bad_alloc &) {
+        return KIRA_ERR_OUT_OF_MEMORY;
+    }
+    return KIRA_OK;
+}
+
+kira_status kira_stage4_sam(const kira_stage4_result *r, int32_t n_contigs,
+                            const char *const *contig_names, int32_t n_reads,
+                            const char *const *read_ids, const uint8_t *seqs,
+                            const int64_t *seq_off, const int32_t *seq_len, const uint8_t *quals,
+                            const kira_sam_cfg *cfg, char *out, int64_t cap, int64_t *used) {
+    if (!r || !cfg || !used || n_contigs < 0 || (n_contigs > 0 && !contig_names) ||
+        n_reads != (int32_t)r->aln_off.size() - 1 ||
+        (n_reads > 0 && (!read_ids || !seqs || !seq_off || !seq_len)))
+        return KIRA_ERR_INVALID_ARGUMENT;
+    for (const kira_stage4_aln &a : r->alns)
+        if (a.ref_id >= (uint32_t)n_contigs) return KIRA_ERR_INVALID_ARGUMENT;
+    const std::string rg = (cfg->write_rg && cfg->read_group) ? rg_id(cfg->read_group) : "";
+    const bool have_rg = cfg->write_rg && cfg->read_group;
+    std::vector<std::string> parts((size_t)host_threads());
+    std::vector<char> bad(parts.size(), 0);
+    try {
+        parallel_chunks(n_reads, 2048, [&](int t, int64_t b, int64_t e) {
+            std::string &o = parts[t];
+            for (int64_t k = b; k < e; k++) {
+                const uint8_t *seq = seqs + seq_off[k];
+                const int64_t n = seq_len[k];
+                const uint8_t *qual = quals ? quals + seq_off[k] : nullptr;
+                if (r->aln_off[k] == r->aln_off[k + 1]) {
+                    // append_unmapped (io/mod.rs:971-1033) without a mate
+                    o += read_ids[k];
+                    o += "\t4\t*\t0\t0\t*\t*\t0\t0\t";
+                    if (n == 0) o += '*';
+                    else o.append((const char *)seq, (size_t)n);
+                    o += '\t';
+                    if (qual && n > 0) o.append((const char *)qual, (size_t)n);
+                    else o += '*';
+                    o += '\n';
+                    continue;
+                }
+                for (int64_t q = r->aln_off[k]; q < r->aln_off[k + 1]; q++) {
+                    // append_alignment (io/mod.rs:1036-1140), single-end
+                    const kira_stage4_aln &a = r->alns[q];
+                    const uint32_t *ops = r->cigar.data() + a.cigar_off;
+                    const int32_t nc = a.n_cigar;
+                    uint64_t consumed = 0;
+                    for (int32_t i = 0; i < nc; i++) {
+                        const uint32_t op = ops[i] & 15u;
+                        if (op == kOpM || op == kOpI || op == kOpS) consumed += ops[i] >> 4;
+                    }
+                    if (nc == 0 || consumed != (uint64_t)n) {  // checked_cigar (:1737-1750)
+                        bad[t] = 1;
+                        return;
+                    }
+                    uint16_t flag = 0;
+                    if (a.is_rev) flag |= 0x10;
+                    if (a.is_supplementary) flag |= cfg->split_as_secondary ? 0x100 : 0x800;
+                    // bwa-mem hard-clips supplementary segments unless -Y
+                    const bool hard = a.is_supplementary && !cfg->soft_clip_supplementary;
+                    uint32_t lead = 0, trail = 0;
+                    if (hard) {
+                        if ((ops[0] & 15u) == kOpS) lead = ops[0] >> 4;
+                        if (nc > 1 && (ops[nc - 1] & 15u) == kOpS) trail = ops[nc - 1] >> 4;
+                    }
+                    const int64_t ke = n > (int64_t)trail ? n - trail : 0;
+                    const int64_t ks = std::min<int64_t>(lead, ke);
+                    o += read_ids[k];
+                    o += '\t';
+                    put_u(o, flag);
+                    o += '\t';
+                    o += contig_names[a.ref_id];
+                    o += '\t';
+                    put_u(o, (uint64_t)a.ref_start + 1);
+                    o += '\t';
+                    put_u(o, (uint32_t)a.mapq);
+                    o += '\t';
+                    for (int32_t i = 0; i < nc; i++) {
+                        const uint32_t op = ops[i] & 15u;
+                        put_u(o, ops[i] >> 4);
+                        o += (op == kOpS && hard && (i == 0 || i + 1 == nc)) ? 'H' : kOpChar[op];
+                    }
+                    o += "\t*\t0\t0\t";
+                    if (ks == ke) {
+                        o += '*';
+                    } else if (a.is_rev) {
+                        for (int64_t i = n - ks - 1; i >= n - ke; i--) o += (char)sam_comp(seq[i]);
+                    } else {
+                        o.append((const char *)seq + ks, (size_t)(ke - ks));
+                    }
+                    o += '\t';
+                    if (!qual || ks == ke) {
+                        o += '*';
+                    } else if (a.is_rev) {
+                        for (int64_t i = n - ks - 1; i >= n - ke; i--) o += (char)qual[i];
+                    } else {
+                        o.append((const char *)qual + ks, (size_t)(ke - ks));
+                    }
+                    if (cfg->write_nm) {
+                        o += "\tNM:i:";
+                        put_u(o, a.nm);
+                    }
+                    if (cfg->write_md) {
+                        o += "\tMD:Z:";
+                        o.append(r->md, (size_t)a.md_off, (size_t)a.md_len);
+                    }
+                    if (cfg->write_as) {
+                        o += "\tAS:i:";
+                        put_i(o, a.score);
+                    }
+                    if (cfg->write_xs && a.has_xs) {
+                        o += "\tXS:i:";
+                        put_i(o, a.xs_score);
+                    }
+                    if (have_rg) {
+                        o += "\tRG:Z:";
+                        o += rg;
+                    }
+                    o += '\n';
+                }
+            }
+        });
+    } catch (const std::bad_alloc &) {
+        return KIRA_ERR_OUT_OF_MEMORY;
+    }
+    for (char x : bad)
+        if (x) return KIRA_ERR_INVALID_ARGUMENT;
+    int64_t total = 0;
+    for (const std::string &x : parts) total += (int64_t)x.size();
+    *used = total;
+    if (!out) return KIRA_OK;
+    if (total > cap) return KIRA_ERR_INVALID_ARGUMENT;
+    int64_t pos = 0;
+    for (const std::string &x : parts) {
+        memcpy(out + pos, x.data(), x.size());
+        pos += (int64_t)x.size();
+    }
+    return KIRA_OK;
+}
+
+}  // extern "C"

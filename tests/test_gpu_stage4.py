@@ -69,3 +69,35 @@ def test_empty_batch_and_reads_without_chains(pkg, ctx):
     with pytest.raises(pkg.HipError):
         pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], b["reads"][:1], bad)
     genome.close()
+
+
+@pytest.mark.parametrize("sam_kw", [dict(), dict(split_as_secondary=True, write_md=False),
+                                    dict(soft_clip_supplementary=True,
+                                         read_group="@RG\tID:g7\tSM:s")])
+def test_left_normalized_sam_matches_oracle(pkg, ctx, sam_kw):
+    # the SW tracebacks already place most gaps leftmost, so few records move
+    # here; the moving cases are pinned on the host by tests/test_oracle_sam.py
+    import random
+    from oracle import sam as OS
+    b = make_batch(13, 1500)
+    rng = random.Random(3)
+    quals = [bytes(rng.randrange(33, 74) for _ in r) for r in b["reads"]]
+    ids = [f"r{k}:x" for k in range(len(b["reads"]))]
+    names = [f"chr{k + 1}" for k in range(len(b["contigs"]))]
+    print(f"[stage4] sam {sam_kw}", flush=True)
+    genome = pkg.Genome(ctx, b["genome"])
+    x = pkg.Stage4Inputs(b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                         b["repeat_min_occ"])
+    res = pkg.stage4_run(ctx, genome, x)
+    res.left_normalize(genome, x)
+    got_recs = res.alignments()
+    got = res.sam(names, ids, x, quals, pkg.sam_cfg(**sam_kw))
+    res.close()
+    genome.close()
+    want_recs, _ = S.run(b["contigs"], b["reads"], b["chains"], S.default_cfg(),
+                         b["repeat_min_occ"])
+    OS.normalize_alignments(b["contigs"], b["reads"], want_recs)
+    assert got_recs == want_recs
+    want = OS.sam_text(ids, b["reads"], quals, want_recs, names, OS.default_cfg(**sam_kw))
+    assert got == want
+    assert got.count(b"\n") == sum(max(len(r), 1) for r in want_recs)
