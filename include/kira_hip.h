@@ -572,13 +572,17 @@ int32_t kira_left_normalize_indels(const uint32_t *cigar, int32_t n_ops, const u
  * normalize_alignments (normalize.rs:197-249) over a stage-4 result, in
  * place: every record's CIGAR is left-normalized against its contig and the
  * read in the record's orientation (reads as passed to kira_stage4_run), and
- * NM / MD are recomputed for the records that moved.  Host code.
+ * NM / MD are recomputed for the records that moved.  original (may be NULL,
+ * same offsets as reads) holds ReadRecord::original_seq — the read with its
+ * IUPAC letters — for the reads with has_original[k] set; their records get
+ * NM / MD from it whether or not they moved.  Host code.
  */
 kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome *g,
                                        int32_t n_contigs, const int64_t *contig_off,
                                        const int64_t *contig_len, int32_t n_reads,
                                        const uint8_t *reads, const int64_t *read_off,
-                                       const int32_t *read_len);
+                                       const int32_t *read_len, const uint8_t *original,
+                                       const uint8_t *has_original);
 
 /* The OutputConfig fields single-end records use (src/io/mod.rs:797-850):
  * tags NM / MD / AS / XS / RG, bwa-mem -M (split_as_secondary) and -Y

@@ -287,7 +287,8 @@ SIGNATURES = {
     "kira_left_normalize_indels": (C.c_int32, [_P, C.c_int32, _P, C.c_int64, C.c_uint32, _P,
                                                C.c_int32, _P, C.c_int32,
                                                C.POINTER(C.c_int32)]),
-    "kira_stage4_left_normalize": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P]),
+    "kira_stage4_left_normalize": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P,
+                                             _P, _P]),
     "kira_stage4_sam": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P, _P,
                                   C.POINTER(SamCfg), _P, C.c_int64, C.POINTER(C.c_int64)]),
 }
@@ -1060,11 +1061,20 @@ class Stage4Result:
             raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_result_view")
         self.view = v
 
-    def left_normalize(self, genome: "Genome", x: "Stage4Inputs"):
-        """normalize_alignments (src/alignment/normalize.rs:197-249) in place."""
+    def left_normalize(self, genome: "Genome", x: "Stage4Inputs", original=None):
+        """normalize_alignments (src/alignment/normalize.rs:197-249) in place.
+        original: per read the IUPAC-bearing original sequence or None."""
+        ob = hb = None
+        if original is not None:
+            ob = np.frombuffer(b"".join(o if o is not None else r
+                                        for o, r in zip(original, x.reads)), np.uint8) \
+                if x.rl.sum() else np.zeros(1, np.uint8)
+            hb = np.array([o is not None for o in original], np.uint8)
         st = lib().kira_stage4_left_normalize(self.h, genome.h, len(x.co), _np_ptr(x.co),
                                               _np_ptr(x.cl), x.n, _np_ptr(x.rb), _np_ptr(x.ro),
-                                              _np_ptr(x.rl))
+                                              _np_ptr(x.rl),
+                                              _np_ptr(ob) if ob is not None else None,
+                                              _np_ptr(hb) if hb is not None else None)
         if st != KIRA_OK:
             raise HipError(_STATUS.get(st, "InvalidArgument"), "stage4_left_normalize")
         self._refresh()
@@ -1131,6 +1141,7 @@ class Stage4Inputs:
         self.coff[1:] = np.cumsum([len(c) for c in chains], dtype=np.int64)
         flat = [c for cs in chains for c in cs]
         self.ch = np.array(flat, dtype=CHAIN_DTYPE) if flat else np.zeros(1, CHAIN_DTYPE)
+        self.reads = reads
         self.rmo = _arr(repeat_min_occ, np.int32) if repeat_min_occ is not None else None
 
 

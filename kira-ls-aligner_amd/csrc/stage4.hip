@@ -1031,7 +1031,8 @@ kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome 
                                        int32_t n_contigs, const int64_t *contig_off,
                                        const int64_t *contig_len, int32_t n_reads,
                                        const uint8_t *reads, const int64_t *read_off,
-                                       const int32_t *read_len) {
+                                       const int32_t *read_len, const uint8_t *original,
+                                       const uint8_t *has_original) {
     if (!r || !g || n_contigs < 0 || (n_contigs > 0 && (!contig_off || !contig_len)) ||
         n_reads != (int32_t)r->aln_off.size() - 1 ||
         (n_reads > 0 && (!reads || !read_off || !read_len)))
@@ -1049,12 +1050,16 @@ kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome 
         std::string md;
         cig.reserve(r->cigar.size());
         md.reserve(r->md.size());
-        std::vector<uint8_t> rc;
+        std::vector<uint8_t> rc, orc;
         std::vector<uint32_t> ops;
         for (int32_t k = 0; k < n_reads; k++) {
             const uint8_t *fwd = reads + read_off[k];
             const int32_t L = read_len[k];
-            bool have_rc = false;
+            // ReadRecord::original_seq: the read's IUPAC letters, which NM / MD
+            // are recomputed from for every record (normalize.rs:213-244)
+            const uint8_t *orig =
+                (original && has_original && has_original[k]) ? original + read_off[k] : nullptr;
+            bool have_rc = false, have_orc = false;
             for (int64_t q = r->aln_off[k]; q < r->aln_off[k + 1]; q++) {
                 kira_stage4_aln &a = r->alns[q];
                 ops.assign(r->cigar.begin() + a.cigar_off,
@@ -1073,8 +1078,20 @@ kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome 
                     kira::left_normalize(ops, contig, contig_len[a.ref_id], a.ref_start, read, L);
                 const int64_t c0 = (int64_t)cig.size(), m0 = (int64_t)md.size();
                 cig.insert(cig.end(), ops.begin(), ops.end());
-                if (moved)
-                    a.nm = kira::nm_md(read, L, contig, contig_len[a.ref_id], a.ref_start,
+                const uint8_t *tags_read = read;
+                if (orig) {
+                    tags_read = orig;
+                    if (a.is_rev) {
+                        if (!have_orc) {
+                            orc.resize((size_t)L);
+                            for (int32_t t = 0; t < L; t++) orc[t] = sam_comp(orig[L - 1 - t]);
+                            have_orc = true;
+                        }
+                        tags_read = orc.data();
+                    }
+                }
+                if (moved || orig)
+                    a.nm = kira::nm_md(tags_read, L, contig, contig_len[a.ref_id], a.ref_start,
                                        ops.data(), (int32_t)ops.size(), md);
                 else
                     md.append(r->md, (size_t)a.md_off, (size_t)a.md_len);

@@ -88,17 +88,21 @@ def left_normalize_indels(cigar, ref, ref_start, read):
     return out
 
 
-def normalize_alignments(contigs, reads, alns):
-    """normalize_alignments (normalize.rs:197-249) on oracle record dicts, in place."""
-    for read, recs in zip(reads, alns):
+def normalize_alignments(contigs, reads, alns, original=None):
+    """normalize_alignments (normalize.rs:197-249) on oracle record dicts, in
+    place.  original: per read its IUPAC-bearing original_seq or None; such
+    reads get NM / MD from it for every record."""
+    original = original or [None] * len(reads)
+    for read, orig, recs in zip(reads, original, alns):
         for a in recs:
             oriented = reverse_complement(read) if a["is_rev"] else read
             ref = contigs[a["ref_id"]]
             new = left_normalize_indels(parse_cigar(a["cigar"]), ref, a["ref_start"], oriented)
             if new is not None:
                 a["cigar"] = cigar_text(new)
-                a["nm"], a["md"] = O.nm_md_from_cigar(oriented, ref, 0, a["ref_start"],
-                                                      a["cigar"])
+            if new is not None or orig is not None:
+                tags = oriented if orig is None else (_rc_iupac(orig) if a["is_rev"] else orig)
+                a["nm"], a["md"] = O.nm_md_from_cigar(tags, ref, 0, a["ref_start"], a["cigar"])
 
 
 IUPAC_COMP = {ord(a): ord(b) for a, b in zip("AaCcGgTUtuRYSWKMBDHV", "TtGgCcAAaaYRSWMKVHDB")}
