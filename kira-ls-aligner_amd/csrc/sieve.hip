@@ -25,6 +25,15 @@ __device__ __forceinline__ uint32_t or_xor(uint32_t t, uint32_t a, uint32_t b) {
 }
 
 // Median of three (v_med3_u32).
+// popcount(x) + acc as one v_bcnt_u32_b32 (its second operand is the
+// addend), keeping a word-by-word accumulation chain instead of a tree of
+// separate counts and three-way adds
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t med3u(uint32_t x, uint32_t y, uint32_t z) {
     uint32_t lo = x < y ? x : y, hi = x > y ? x : y;
     uint32_t t = lo > z ? lo : z;
@@ -320,11 +329,13 @@ __device__ bool certify(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
 
 // Scan one tile of up to 128 shifts; wl / wh: the tile's window words
 // (RW + 4 of them, aligned to its first shift).  FULL: all 128 valid.
+// Shift aa * 32 + b of the tile is ranked (count << 16) | b against its own
+// group aa's best / second (tb[aa], ts[aa]); the caller adds aa * 32 back.
 template <int RW, bool FULL, bool MASKALL>
 __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32_t (&rh)[RW],
                                           const uint32_t (&mk)[RW], const uint32_t (&wl)[RW + 4],
                                           const uint32_t (&wh)[RW + 4], uint32_t rem,
-                                          uint32_t &tb, uint32_t &ts) {
+                                          uint32_t (&tb)[4], uint32_t (&ts)[4]) {
     uint32_t nb = FULL ? 32u : (rem < 32u ? rem : 32u);
     for (uint32_t b = 0; b < nb; b++) {
         uint32_t xl[RW + 3], xh[RW + 3];
@@ -343,14 +354,13 @@ __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32
                 // (rl ^ xl) | (rh ^ xh) as xor + one 3-input bit op
                 uint32_t d = or_xor(rl[k] ^ xl[aa + k], rh[k], xh[aa + k]);
                 if (MASKALL || k == RW - 1) d &= mk[k];
-                m += __popc(d);
+                m = k == 0 ? __popc(d) : bcnt_acc(d, m);
             }
-            uint32_t tl = (uint32_t)aa * 32u + b;
-            uint32_t rank = (m << 16) | tl;
-            if (!FULL && tl >= rem) rank = 0xFFFFFFFFu;
+            uint32_t rank = (m << 16) | b;
+            if (!FULL && (uint32_t)aa * 32u + b >= rem) rank = 0xFFFFFFFFu;
             // tb <= ts always, so min(ts, max(tb, rank)) is the median of the three
-            ts = med3u(tb, rank, ts);
-            tb = tb < rank ? tb : rank;
+            ts[aa] = med3u(tb[aa], rank, ts[aa]);
+            tb[aa] = tb[aa] < rank ? tb[aa] : rank;
         }
     }
 }
@@ -386,12 +396,26 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
             wl[k] = funnel(raw[k + 1].x, raw[k].x, sh);
             wh[k] = funnel(raw[k + 1].y, raw[k].y, sh);
         }
-        uint32_t tb = 0xFFFFFFFFu, ts = 0xFFFFFFFFu;
+        uint32_t gb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        uint32_t gs[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
         uint32_t rem = nt - T0;
         if (rem >= 128)
-            scan_tile<RW, true, MASKALL>(rl, rh, mk, wl, wh, rem, tb, ts);
+            scan_tile<RW, true, MASKALL>(rl, rh, mk, wl, wh, rem, gb, gs);
         else
-            scan_tile<RW, false, MASKALL>(rl, rh, mk, wl, wh, rem, tb, ts);
+            scan_tile<RW, false, MASKALL>(rl, rh, mk, wl, wh, rem, gb, gs);
+        // merge the four groups: ranks become (count << 16) | tile shift, and
+        // each group's (best, second) are its two smallest, so the union's are
+        // min of the bests and min(max of the bests, the seconds)
+        uint32_t tb = 0xFFFFFFFFu, ts = 0xFFFFFFFFu;
+#pragma unroll
+        for (int aa = 0; aa < 4; aa++) {
+            const uint32_t b = gb[aa] == 0xFFFFFFFFu ? gb[aa] : gb[aa] + 32u * aa;
+            const uint32_t c = gs[aa] == 0xFFFFFFFFu ? gs[aa] : gs[aa] + 32u * aa;
+            const uint32_t hi = tb > b ? tb : b;
+            ts = ts < c ? ts : c;
+            ts = ts < hi ? ts : hi;
+            tb = tb < b ? tb : b;
+        }
         uint32_t tbm = tb >> 16;
         uint32_t tsm = ts == 0xFFFFFFFFu ? 0xFFFFFFFFu : (ts >> 16);
         uint32_t hi = gbm > tbm ? gbm : tbm;
