@@ -336,8 +336,7 @@ __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32
                                           const uint32_t (&mk)[RW], const uint32_t (&wl)[RW + 4],
                                           const uint32_t (&wh)[RW + 4], uint32_t rem,
                                           uint32_t (&tb)[4], uint32_t (&ts)[4]) {
-    uint32_t nb = FULL ? 32u : (rem < 32u ? rem : 32u);
-    for (uint32_t b = 0; b < nb; b++) {
+    auto step = [&](uint32_t b) {
         uint32_t xl[RW + 3], xh[RW + 3];
 #pragma unroll
         for (int k = 0; k < RW + 3; k++) {
@@ -362,6 +361,14 @@ __device__ __forceinline__ void scan_tile(const uint32_t (&rl)[RW], const uint32
             ts[aa] = med3u(tb[aa], rank, ts[aa]);
             tb[aa] = tb[aa] < rank ? tb[aa] : rank;
         }
+    };
+    if constexpr (FULL) {
+        // two shifts per trip: the loop-carried best / second stay in place
+#pragma unroll 2
+        for (uint32_t b = 0; b < 32u; b++) step(b);
+    } else {
+        const uint32_t nb = rem < 32u ? rem : 32u;
+        for (uint32_t b = 0; b < nb; b++) step(b);
     }
 }
 
