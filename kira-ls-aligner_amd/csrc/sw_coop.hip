@@ -258,18 +258,20 @@ __device__ int2 coop_job(const SwLaunch &a, int32_t job, int g, int wl, int32_t 
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const bool valid = p >= vlo && p <= vhi;
-            const int32_t h = imax(hp[p], F);
+            const uint32_t vm = valid ? 0xFFFFFFFFu : 0u;
             const int32_t e = bd.E[p];
-            uint32_t dir = h == 0 ? 0u : (h == hm[p] ? 1u : (h == e ? 2u : 3u));
+            // Keyed maximum of the H candidates: value << 2 | priority with
+            // STOP (the zero candidate) 3 > diagonal 2 > E 1 > F 0, so ties
+            // resolve as the reference orders them and the low two bits name
+            // the winner (direction = 3 - priority).  Every value here is
+            // >= i32::MIN/4 and < 2^21, so the shifted keys do not overflow.
+            const int32_t K = imax(imax((hm[p] << 2) | 2, (e << 2) | 1), imax(F << 2, 3));
+            const int32_t h = K >> 2;
             uint32_t fb = (p > 0 && prevValid && (prevF - ge > prevH - go)) ? 8u : 0u;
-            uint32_t code = dir | ((uint32_t)((ebits >> p) & 1u) << 2) | fb;
-            if (valid) {
-                key = imax(key, (h << 10) | (1023 - (q0 + p)));
-                bd.H[p] = h;
-            } else {
-                code = 0;
-                bd.H[p] = bd.padH;
-            }
+            uint32_t code = (~(uint32_t)K & 3u) | ((uint32_t)((ebits >> p) & 1u) << 2) | fb;
+            code &= vm;
+            key = imax(key, (int32_t)((uint32_t)((h << 10) | (1023 - (q0 + p))) & vm));
+            bd.H[p] = (int32_t)(((uint32_t)h & vm) | ((uint32_t)bd.padH & ~vm));
             const int32_t nextF = imax(hp[p] - go, F - ge);
             prevF = F;
             prevH = h;
