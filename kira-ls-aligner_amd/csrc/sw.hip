@@ -137,8 +137,20 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
         int32_t e = imax(efh, efe);
         int32_t ffh = hl - go, fff = fl - ge;
         int32_t f = imax(ffh, fff);
-        int32_t h = imax(imax(hm, e), imax(f, 0));
-        uint32_t dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
+        int32_t h;
+        uint32_t dir;
+        if constexpr (PACK) {
+            // keyed maximum: value << 2 | priority with STOP (the zero
+            // candidate) 3 > diagonal 2 > E 1 > F 0 resolves ties as the
+            // reference does; direction = 3 - priority.  Packed bands keep
+            // every value in [-16384 - go, 32767), so the keys cannot overflow.
+            const int32_t K = imax(imax((hm << 2) | 2, (e << 2) | 1), imax(f << 2, 3));
+            h = K >> 2;
+            dir = ~(uint32_t)K & 3u;
+        } else {
+            h = imax(imax(hm, e), imax(f, 0));
+            dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
+        }
         uint32_t code = dir | (efe > efh ? 4u : 0u) | (fff > ffh ? 8u : 0u);
         if (MASK && !valid) {
             bd.set(q, bd.padH, bd.padE);

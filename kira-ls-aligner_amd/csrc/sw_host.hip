@@ -115,8 +115,10 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
     L.n = n;
     std::vector<int> cls(n);
     const int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
+    // packed bands: also bounds the diagonal candidate (>= -mismatch) so the
+    // kernel's keyed maximum (value << 2 | priority) stays inside int32
     const bool costs_small = cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
-                             cfg.gap_extend >= 0 && go <= 16000;
+                             cfg.gap_extend >= 0 && go <= 16000 && cfg.mismatch <= 16000;
     // 16-lane jobs: the F scan stays inside int32 and the arg-max key
     // (score << 10 | position) needs score < 2^21
     const int32_t kCoopCost = 1 << 20;
