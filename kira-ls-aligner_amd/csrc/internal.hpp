@@ -187,6 +187,8 @@ struct GroupLaunch {
     uint32_t *cigar;
     const int64_t *cigar_off;
     const int32_t *cigar_cap;
+    // scores and costs keep value << 4 inside int32 (the keyed maximum)
+    int32_t keyed;
 };
 
 hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,

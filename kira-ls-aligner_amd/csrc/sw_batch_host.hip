@@ -212,6 +212,18 @@ kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *grou
     a.cigar = d_cig;
     a.cigar_off = d_coff;
     a.cigar_cap = d_ccap;
+    // the keyed maximum shifts scores left by 4: every H <= min(L, W) * match
+    // and every candidate >= -max(mismatch, open + extend) must stay in 2^27
+    {
+        const int64_t kLim = int64_t(1) << 20;
+        int64_t hmax = 0;
+        for (int32_t t = 0; t < m; t++)
+            hmax = std::max<int64_t>(hmax, (int64_t)std::min(j.read_len[t], j.ref_len[t]) *
+                                               std::max(cfg.match_score, 0));
+        a.keyed = cfg.match_score >= 0 && cfg.match_score <= kLim && cfg.mismatch >= 0 &&
+                  cfg.mismatch <= kLim && cfg.gap_open >= 0 && cfg.gap_extend >= 0 &&
+                  a.go <= kLim && hmax < (int64_t(1) << 27);
+    }
     if ((e = launch_group_sw(a, m, ng, std::min(P, 256 * 16), s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "lane-group launch", e);
     return fetch_sw_results(ctx, m, d_res, d_cig, coff.data(), false, out, cigar);

@@ -31,6 +31,7 @@ constexpr int kStrip = 16;
 
 __host__ __device__ __forceinline__ int32_t gmax(int32_t a, int32_t b) { return a > b ? a : b; }
 
+template <bool KEYED>
 __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int32_t job,
                                                       int lane, int32_t rows, int32_t words,
                                                       uint32_t *trace, int2 *bnd, int2 *rstat) {
@@ -66,7 +67,7 @@ __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int3
             int32_t diag = diag_row_prev;
             diag_row_prev = hl;
             const uint8_t qb = read[i - 1];
-            int32_t rmax = 0, rarg = 0;
+            int32_t rmax = 0, rarg = 0, rkey = 0;
             uint32_t w0 = 0, w1 = 0;
 #pragma unroll
             for (int c = 0; c < kStrip; c++) {
@@ -77,21 +78,38 @@ __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int3
                     int32_t e = gmax(efh, efe);
                     int32_t ffh = hl - go, fff = fl - ge;
                     int32_t f = gmax(ffh, fff);
-                    int32_t h = gmax(gmax(hm, e), gmax(f, 0));
-                    uint32_t dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
+                    int32_t h;
+                    uint32_t dir;
+                    if constexpr (KEYED) {
+                        // keyed maximum: value << 2 | priority, STOP (the
+                        // zero candidate) 3 > diagonal 2 > E 1 > F 0, the
+                        // reference's tie order; direction = 3 - priority
+                        const int32_t K = gmax(gmax((hm << 2) | 2, (e << 2) | 1), gmax(f << 2, 3));
+                        h = K >> 2;
+                        dir = ~(uint32_t)K & 3u;
+                        // first strict row maximum of the strip
+                        rkey = gmax(rkey, (h << 4) | (kStrip - 1 - c));
+                    } else {
+                        h = gmax(gmax(hm, e), gmax(f, 0));
+                        dir = h == 0 ? 0u : (h == hm ? 1u : (h == e ? 2u : 3u));
+                        if (h > rmax) {
+                            rmax = h;
+                            rarg = j0 + c;
+                        }
+                    }
                     uint32_t code = dir | (efe > efh ? 4u : 0u) | (fff > ffh ? 8u : 0u);
                     diag = Hup[c];
                     Hup[c] = h;
                     Eup[c] = e;
                     hl = h;
                     fl = f;
-                    if (h > rmax) {
-                        rmax = h;
-                        rarg = j0 + c;
-                    }
                     if (c < 8) w0 |= code << (4 * c);
                     else w1 |= code << (4 * (c - 8));
                 }
+            }
+            if constexpr (KEYED) {
+                rmax = rkey >> 4;
+                rarg = rmax > 0 ? j0 + (kStrip - 1) - (rkey & (kStrip - 1)) : 0;
             }
             uint32_t *tr = trace + ((int64_t)(i - 1) * words + 2 * st) * 64 + lane;
             tr[0] = w0;
@@ -108,7 +126,7 @@ __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int3
     }
 }
 
-template <int DUMMY>
+template <bool KEYED>
 __global__ __launch_bounds__(64) void group_dp_kernel(GroupLaunch a) {
     const int lane = threadIdx.x;
     for (;;) {
@@ -118,7 +136,7 @@ __global__ __launch_bounds__(64) void group_dp_kernel(GroupLaunch a) {
         if (p >= a.n_packets) return;
         const int32_t job = a.packet_jobs[(int64_t)p * 64 + lane];
         if (job >= 0)
-            group_dp_job(a, job, lane, a.packet_rows[p], a.packet_words[p],
+            group_dp_job<KEYED>(a, job, lane, a.packet_rows[p], a.packet_words[p],
                          a.trace + a.packet_trace_off[p], a.bnd + a.packet_row_off[p],
                          a.rstat + a.packet_row_off[p]);
     }
@@ -266,7 +284,10 @@ hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_group
     if (n_jobs == 0) return hipSuccess;
     int32_t m = n_groups > 1 ? n_groups : 1;
     group_reset_kernel<<<(m + 255) / 256, 256, 0, s>>>(a.counter, a.group_stop, n_groups);
-    group_dp_kernel<0><<<n_waves, 64, 0, s>>>(a);
+    if (a.keyed)
+        group_dp_kernel<true><<<n_waves, 64, 0, s>>>(a);
+    else
+        group_dp_kernel<false><<<n_waves, 64, 0, s>>>(a);
     group_stop_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
     group_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
     return hipGetLastError();
