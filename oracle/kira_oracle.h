@@ -171,6 +171,33 @@ void ko_banded_sw_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_
                            const ko_knobs *kn, int32_t *out_score, int32_t *out_end,
                            int64_t *out_cells, int threads);
 
+/* ---- WFA / Myers fast path (kira_wfa_oracle.c) ---- */
+
+/* bounded_edit_distance — src/alignment/myers.rs:47-367: semi-global edit
+ * distance of the pattern against the text (best end position, last on
+ * ties); 0 when it exceeds max_k. */
+int ko_myers_bounded(const uint8_t *pattern, size_t m, const uint8_t *text, size_t n,
+                     size_t max_k, size_t *dist, size_t *pos);
+
+/* WfaAlignment — wfa.rs:115-126 (cigar caller-owned, capacity cigar_cap) */
+typedef struct {
+    int32_t score;                 /* WFA loss of the match-rewarding objective */
+    int32_t text_start, text_end;
+    int32_t n_cigar;
+    uint32_t *cigar;
+    int32_t cigar_cap;
+} ko_wfa_aln;
+
+/* wfa_align_scored / wfa_score_only_scored — wfa.rs:58-81 with WfaOptions
+ * {max_score, adaptive_drop (<= 0: none), text_begin_free}.  1 found, 0 none
+ * within max_score (or unsupported costs), -1 CIGAR capacity too small. */
+int ko_wfa_align_scored(const uint8_t *p, int32_t m, const uint8_t *t, int32_t n, const ko_cfg *cfg,
+                        int32_t max_score, int32_t adaptive_drop, int32_t text_begin_free,
+                        ko_wfa_aln *out);
+int ko_wfa_score_only_scored(const uint8_t *p, int32_t m, const uint8_t *t, int32_t n,
+                             const ko_cfg *cfg, int32_t max_score, int32_t adaptive_drop,
+                             int32_t text_begin_free, int32_t *score);
+
 #ifdef __cplusplus
 }
 #endif

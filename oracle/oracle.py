@@ -132,8 +132,59 @@ def lib():
                                           C.c_int32, C.c_char_p, C.c_size_t]
         L.ko_banded_sw_batch_mt.argtypes = [C.c_int32, P, P, P, P, P, P, P, P, P,
                                             C.POINTER(Cfg), C.POINTER(Knobs), P, P, P, C.c_int]
+        L.ko_myers_bounded.restype = C.c_int
+        L.ko_myers_bounded.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
+                                       C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.ko_wfa_align_scored.restype = C.c_int
+        L.ko_wfa_align_scored.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32,
+                                          C.POINTER(Cfg), C.c_int32, C.c_int32, C.c_int32,
+                                          C.POINTER(WfaAln)]
+        L.ko_wfa_score_only_scored.restype = C.c_int
+        L.ko_wfa_score_only_scored.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32,
+                                               C.POINTER(Cfg), C.c_int32, C.c_int32, C.c_int32,
+                                               C.POINTER(C.c_int32)]
         _lib = L
     return _lib
+
+
+class WfaAln(C.Structure):
+    _fields_ = [("score", C.c_int32), ("text_start", C.c_int32), ("text_end", C.c_int32),
+                ("n_cigar", C.c_int32), ("cigar", C.POINTER(C.c_uint32)),
+                ("cigar_cap", C.c_int32)]
+
+
+def myers_bounded(pattern: bytes, text: bytes, max_k: int):
+    """bounded_edit_distance (src/alignment/myers.rs:47-367): (dist, end) or None."""
+    d, p = C.c_size_t(), C.c_size_t()
+    ok = lib().ko_myers_bounded(pattern, len(pattern), text, len(text), max_k, C.byref(d),
+                                C.byref(p))
+    return (d.value, p.value) if ok else None
+
+
+def wfa_align_scored(pattern: bytes, text: bytes, c: "Cfg", max_score: int, adaptive_drop=0,
+                     text_begin_free=0):
+    """wfa_align_scored (src/alignment/wfa.rs:58-67): dict(score (the WFA loss),
+    cigar (words), text_start, text_end) or None."""
+    cap = 2 * (len(pattern) + len(text)) + 8
+    buf = (C.c_uint32 * cap)()
+    a = WfaAln(0, 0, 0, 0, C.cast(buf, C.POINTER(C.c_uint32)), cap)
+    r = lib().ko_wfa_align_scored(pattern, len(pattern), text, len(text), C.byref(c), max_score,
+                                  adaptive_drop, text_begin_free, C.byref(a))
+    if r < 0:
+        raise RuntimeError("CIGAR capacity")
+    if r == 0:
+        return None
+    return dict(score=a.score, cigar=[buf[k] for k in range(a.n_cigar)],
+                text_start=a.text_start, text_end=a.text_end)
+
+
+def wfa_score_only_scored(pattern: bytes, text: bytes, c: "Cfg", max_score: int,
+                          adaptive_drop=0, text_begin_free=0):
+    """wfa_score_only_scored (wfa.rs:69-81): the affine score or None."""
+    v = C.c_int32()
+    ok = lib().ko_wfa_score_only_scored(pattern, len(pattern), text, len(text), C.byref(c),
+                                        max_score, adaptive_drop, text_begin_free, C.byref(v))
+    return v.value if ok else None
 
 
 def _p(a: np.ndarray):
