@@ -611,6 +611,56 @@ kira_status kira_stage4_sam(const kira_stage4_result *r, int32_t n_contigs,
                             const int64_t *seq_off, const int32_t *seq_len, const uint8_t *quals,
                             const kira_sam_cfg *cfg, char *out, int64_t cap, int64_t *used);
 
+/* ------------------------------------------------------------------ WFA / Myers */
+
+/*
+ * bounded_edit_distance (src/alignment/myers.rs:47-367) for n (pattern,
+ * text) pairs, host arrays in and out: pattern k is
+ * patterns[pat_off[k] .. +pat_len[k]], text k texts[text_off[k] .. +text_len[k]].
+ * Semi-global unit-cost edit distance (the whole pattern against the best
+ * substring of the text; A / C / G / T match in either case, any other byte
+ * matches nothing, itself included).  found[k] = 1 with dist[k] and end[k]
+ * (one past the best end position in the text, the last of equal minima)
+ * when the distance is <= max_k[k] (>= 0), else found[k] = 0.
+ */
+kira_status kira_myers_batch(kira_ctx *ctx, int32_t n, const uint8_t *patterns,
+                             int64_t patterns_bytes, const int64_t *pat_off,
+                             const int32_t *pat_len, const uint8_t *texts, int64_t texts_bytes,
+                             const int64_t *text_off, const int32_t *text_len,
+                             const int32_t *max_k, uint8_t *found, int32_t *dist, int32_t *end);
+
+/* WfaAlignment (src/alignment/wfa.rs:20-39) of one pair: found 0 when no
+ * alignment costs <= max_score.  score = the loss of wfa_align_scored (the
+ * affine score wfa_score_only_scored returns is pat_len * match_score -
+ * score); the text span [text_start, text_end); the CIGAR, n_cigar words
+ * len << 4 | op (M0 I1 D2), at cigar_arena[cigar_off]. */
+typedef struct {
+    int32_t found, score, text_start, text_end, n_cigar;
+    int64_t cigar_off;
+} kira_wfa_result;
+
+/*
+ * wfa_align_scored / wfa_score_only_scored (src/alignment/wfa.rs:58-81 over
+ * wfa_align_with_costs :549-648) for n pairs, host arrays in and out.  Costs
+ * from cfg as WfaCosts::scored (:41-55): mismatch A + B, gap open O,
+ * insertion extend A + E, deletion extend E; unless match_score > 0,
+ * mismatch > 0, gap_open >= 0 and gap_extend > 0 no pair is found.
+ * WfaOptions per pair: max_score[k] (<= 65535), text_begin_free[k] (free
+ * leading text bases; NULL = 0), adaptive_drop for all (<= 0: exact).  The
+ * whole pattern aligns against text starting within its first
+ * text_begin_free + 1 bases and ending anywhere; bytes match when equal.
+ * want_cigar 0 skips the traceback (scores only; cigar_arena may be NULL).
+ * The CIGARs are packed back to back into cigar_arena (arena_cap words; too
+ * few returns KIRA_ERR_INVALID_ARGUMENT).  pat_len + text_len <= 60000.
+ */
+kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *patterns,
+                           int64_t patterns_bytes, const int64_t *pat_off, const int32_t *pat_len,
+                           const uint8_t *texts, int64_t texts_bytes, const int64_t *text_off,
+                           const int32_t *text_len, const int32_t *max_score,
+                           const int32_t *text_begin_free, int32_t adaptive_drop,
+                           const kira_sw_cfg *cfg, int32_t want_cigar, kira_wfa_result *out,
+                           uint32_t *cigar_arena, int64_t arena_cap);
+
 #ifdef __cplusplus
 }
 #endif

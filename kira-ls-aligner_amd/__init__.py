@@ -71,6 +71,14 @@ class SwResult(C.Structure):
                 ("n_cigar", C.c_int32), ("cells", C.c_int32), ("cigar_off", C.c_int64)]
 
 
+class WfaResult(C.Structure):
+    """kira_wfa_result = WfaAlignment (src/alignment/wfa.rs:20-39); score is
+    the WFA loss."""
+
+    _fields_ = [("found", C.c_int32), ("score", C.c_int32), ("text_start", C.c_int32),
+                ("text_end", C.c_int32), ("n_cigar", C.c_int32), ("cigar_off", C.c_int64)]
+
+
 class BatchInput(C.Structure):
     """kira_batch_input = BatchInput (src/alignment/mod.rs:48-56)."""
 
@@ -291,6 +299,11 @@ SIGNATURES = {
                                              _P, _P]),
     "kira_stage4_sam": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P, _P,
                                   C.POINTER(SamCfg), _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "kira_myers_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P,
+                                   _P, _P, _P, _P]),
+    "kira_wfa_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P,
+                                 _P, _P, C.c_int32, C.POINTER(SwCfg), C.c_int32, _P, _P,
+                                 C.c_int64]),
 }
 
 _lib = None
@@ -1171,6 +1184,67 @@ def sam_cfg(write_nm=True, write_md=True, write_as=True, write_xs=True, write_rg
     return SamCfg(int(write_nm), int(write_md), int(write_as), int(write_xs), int(write_rg),
                   int(split_as_secondary), int(soft_clip_supplementary),
                   read_group.encode() if read_group is not None else None)
+
+
+def _concat(seqs):
+    """Back-to-back bytes of seqs: (buffer, offsets int64, lengths int32)."""
+    ln = np.array([len(x) for x in seqs], dtype=np.int32)
+    off = np.zeros(len(seqs), dtype=np.int64)
+    if len(seqs) > 1:
+        off[1:] = np.cumsum(ln[:-1], dtype=np.int64)
+    blob = b"".join(seqs)
+    buf = np.frombuffer(blob, dtype=np.uint8) if blob else np.zeros(1, np.uint8)
+    return buf, len(blob), off, ln
+
+
+def myers_batch(ctx: Context, pairs):
+    """bounded_edit_distance (src/alignment/myers.rs:47-367) over a batch.
+    pairs: sequence of (pattern, text, max_k).  Returns (dist, end) or None
+    per pair."""
+    n = len(pairs)
+    if n == 0:
+        return []
+    pb, pn, po, pl = _concat([p[0] for p in pairs])
+    tb, tn, to, tl = _concat([p[1] for p in pairs])
+    mk = np.array([p[2] for p in pairs], dtype=np.int32)
+    found = np.zeros(n, np.uint8)
+    dist = np.zeros(n, np.int32)
+    end = np.zeros(n, np.int32)
+    st = lib().kira_myers_batch(ctx.h, n, _np_ptr(pb), pn, _np_ptr(po), _np_ptr(pl), _np_ptr(tb),
+                                tn, _np_ptr(to), _np_ptr(tl), _np_ptr(mk), _np_ptr(found),
+                                _np_ptr(dist), _np_ptr(end))
+    ctx.check(st, "myers_batch")
+    return [(int(dist[k]), int(end[k])) if found[k] else None for k in range(n)]
+
+
+def wfa_batch(ctx: Context, pairs, cfg: SwCfg, adaptive_drop: int = 0, want_cigar: bool = True):
+    """wfa_align_scored (src/alignment/wfa.rs:58-67) over a batch.  pairs:
+    sequence of (pattern, text, max_score, text_begin_free).  Returns per
+    pair None or dict(score (the loss), text_start, text_end, cigar)."""
+    n = len(pairs)
+    if n == 0:
+        return []
+    pb, pn, po, pl = _concat([p[0] for p in pairs])
+    tb, tn, to, tl = _concat([p[1] for p in pairs])
+    ms = np.array([p[2] for p in pairs], dtype=np.int32)
+    tbf = np.array([p[3] if len(p) > 3 else 0 for p in pairs], dtype=np.int32)
+    out = (WfaResult * n)()
+    cap = int((pl.astype(np.int64) + tl).sum() + 2 * n + 2) if want_cigar else 0
+    arena = np.zeros(max(cap, 1), dtype=np.uint32)
+    st = lib().kira_wfa_batch(ctx.h, n, _np_ptr(pb), pn, _np_ptr(po), _np_ptr(pl), _np_ptr(tb),
+                              tn, _np_ptr(to), _np_ptr(tl), _np_ptr(ms), _np_ptr(tbf),
+                              int(adaptive_drop), C.byref(cfg), int(bool(want_cigar)), out,
+                              _np_ptr(arena), cap)
+    ctx.check(st, "wfa_batch")
+    res = []
+    for k in range(n):
+        o = out[k]
+        if not o.found:
+            res.append(None)
+            continue
+        res.append(dict(score=o.score, text_start=o.text_start, text_end=o.text_end,
+                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+    return res
 
 
 def left_normalize_indels(cigar, contig: bytes, ref_start: int, read: bytes):
