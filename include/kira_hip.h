@@ -661,6 +661,71 @@ kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *patterns,
                            const kira_sw_cfg *cfg, int32_t want_cigar, kira_wfa_result *out,
                            uint32_t *cigar_arena, int64_t arena_cap);
 
+/* ------------------------------------------------------------------ fast DP cascade */
+
+/* The router knobs of the fast path (src/alignment/router.rs:48-215), with
+ * the reference defaults kira_fast_knobs_default fills in brackets:
+ * wfa_max_read_len KIRA_WFA_MAX_LEN [300]; algo KIRA_ALGO 0 packed [0],
+ * 1 spectral, 2 wfa, 3 sw; quality_gate / quality_pct
+ * KIRA_CHAIN_QUALITY_GATE [0] / KIRA_CHAIN_QUALITY_PCT [70]; wfa_budget_pct
+ * KIRA_WFA_BUDGET_PCT [25]; myers_pct / myers_floor KIRA_MYERS_BOUND_PCT [15]
+ * / KIRA_MYERS_BOUND_FLOOR [4]; spectral_pct / spectral_floor
+ * KIRA_SPECTRAL_MISM_PCT [3] / KIRA_SPECTRAL_MISM_FLOOR [2]; wfa_ends_free
+ * KIRA_WFA_ENDS_FREE [0]; wfa_lead KIRA_WFA_LEAD [0]; wfa_adaptive
+ * KIRA_WFA_ADAPTIVE [0 = exact]; cert_strict KIRA_CERT_STRICT [0]. */
+typedef struct {
+    int32_t wfa_max_read_len, algo, quality_gate, quality_pct, wfa_budget_pct;
+    int32_t myers_pct, myers_floor, spectral_pct, spectral_floor;
+    int32_t wfa_ends_free, wfa_lead, wfa_adaptive, cert_strict;
+} kira_fast_knobs;
+
+void kira_fast_knobs_default(kira_fast_knobs *k);
+
+/* The Alignment try_fast_dp_alignment returns (mod.rs:636-686, 481-575):
+ * kind 0 none (the caller falls back to SW), 1 packed spectral certificate,
+ * 2 WFA; CIGAR (no clips; len << 4 | op, M0 I1 D2) at
+ * cigar_out[cigar_off .. +n_cigar], MD text (no NUL) at md_out[md_off ..
+ * +md_len]; ref_start / ref_end in contig coordinates. */
+typedef struct {
+    int32_t kind;
+    uint32_t ref_start, ref_end, read_start, read_end;
+    int32_t score;
+    uint32_t nm;
+    int32_t n_cigar, md_len;
+    int64_t cigar_off, md_off;
+} kira_fast_aln;
+
+/* Chains past the router gates, certified by the packed scan, passed by
+ * Myers, improved by the origin retry, declined by the partial-read check
+ * (or a CIGAR that does not span the read), and aligned by WFA. */
+typedef struct {
+    int64_t attempted, spectral, myers_pass, origin_retries, clip_declined, wfa;
+} kira_fast_stats;
+
+/*
+ * try_fast_dp_alignment_inner (src/alignment/mod.rs:285-440) for n chains
+ * against a resident genome (contig i = bases contig_off[i] .. +contig_len[i]):
+ * job k aligns reads[read_off[k] .. +read_len[k]], already oriented to the
+ * chain's strand, in the window the chain's contig, ref_start and read_start
+ * imply; chains[k].score feeds the chain-quality gate.  The packed scan and
+ * certification, Myers and WFA run on the device; the origin probe of the
+ * retry, the partial-read check and NM / MD run on the host.  A chain with an
+ * out-of-range contig, or a read outside reads_bytes, fails the call with
+ * KIRA_ERR_INVALID_ARGUMENT.  cigar_used / md_used receive the totals; too
+ * small a cigar_cap / md_cap returns KIRA_ERR_INVALID_ARGUMENT (the totals
+ * are still set).  Sum(2 * read_len + bandwidth + 66) words and 4 times
+ * that in MD bytes always suffice.  knobs NULL = the defaults; stats may be
+ * NULL.
+ */
+kira_status kira_fast_dp_batch(kira_ctx *ctx, const kira_genome *g, int32_t n_contigs,
+                               const int64_t *contig_off, const int64_t *contig_len, int32_t n,
+                               const uint8_t *reads, int64_t reads_bytes, const int64_t *read_off,
+                               const int32_t *read_len, const kira_chain *chains,
+                               const kira_sw_cfg *cfg, const kira_fast_knobs *knobs,
+                               kira_fast_aln *out, uint32_t *cigar_out, int64_t cigar_cap,
+                               char *md_out, int64_t md_cap, int64_t *cigar_used,
+                               int64_t *md_used, kira_fast_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,29 @@ class WfaResult(C.Structure):
                 ("text_end", C.c_int32), ("n_cigar", C.c_int32), ("cigar_off", C.c_int64)]
 
 
+class FastKnobs(C.Structure):
+    """kira_fast_knobs = the router knobs of the fast path (src/alignment/router.rs:48-215)."""
+
+    _fields_ = [(n, C.c_int32) for n in (
+        "wfa_max_read_len", "algo", "quality_gate", "quality_pct", "wfa_budget_pct", "myers_pct",
+        "myers_floor", "spectral_pct", "spectral_floor", "wfa_ends_free", "wfa_lead",
+        "wfa_adaptive", "cert_strict")]
+
+
+class FastAln(C.Structure):
+    """kira_fast_aln: the Alignment try_fast_dp_alignment returns."""
+
+    _fields_ = [("kind", C.c_int32), ("ref_start", C.c_uint32), ("ref_end", C.c_uint32),
+                ("read_start", C.c_uint32), ("read_end", C.c_uint32), ("score", C.c_int32),
+                ("nm", C.c_uint32), ("n_cigar", C.c_int32), ("md_len", C.c_int32),
+                ("cigar_off", C.c_int64), ("md_off", C.c_int64)]
+
+
+class FastStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("attempted", "spectral", "myers_pass", "origin_retries",
+                                         "clip_declined", "wfa")]
+
+
 class BatchInput(C.Structure):
     """kira_batch_input = BatchInput (src/alignment/mod.rs:48-56)."""
 
@@ -304,6 +327,11 @@ SIGNATURES = {
     "kira_wfa_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P,
                                  _P, _P, C.c_int32, C.POINTER(SwCfg), C.c_int32, _P, _P,
                                  C.c_int64]),
+    "kira_fast_knobs_default": (None, [C.POINTER(FastKnobs)]),
+    "kira_fast_dp_batch": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, C.c_int64, _P, _P,
+                                     _P, C.POINTER(SwCfg), C.POINTER(FastKnobs), _P, _P,
+                                     C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64), C.POINTER(FastStats)]),
 }
 
 _lib = None
@@ -1245,6 +1273,61 @@ def wfa_batch(ctx: Context, pairs, cfg: SwCfg, adaptive_drop: int = 0, want_ciga
         res.append(dict(score=o.score, text_start=o.text_start, text_end=o.text_end,
                         cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
     return res
+
+
+FAST_ALGOS = {"packed": 0, "spectral": 1, "wfa": 2, "sw": 3}
+
+
+def fast_knobs(**kw) -> FastKnobs:
+    """The fast-path router knobs, reference defaults overridden by kw
+    (algo may be given by name)."""
+    k = FastKnobs()
+    lib().kira_fast_knobs_default(C.byref(k))
+    for name, v in kw.items():
+        if name == "algo" and isinstance(v, str):
+            v = FAST_ALGOS[v]
+        setattr(k, name, int(v))
+    return k
+
+
+def fast_dp_batch(ctx: Context, genome: "Genome", contig_off, contig_len, reads, chains,
+                  cfg: SwCfg, knobs: Optional[FastKnobs] = None):
+    """try_fast_dp_alignment_inner (src/alignment/mod.rs:285-440) over a batch.
+
+    reads: list of strand-oriented read bytes; chains: one (ref_id, ref_start,
+    ref_end, read_start, read_end, strand, score) per read.  Returns (records,
+    stats): per read None or dict(kind 'packed' | 'wfa', ref_start, ref_end,
+    score, nm, md, cigar (words))."""
+    n = len(reads)
+    co = np.ascontiguousarray(contig_off, dtype=np.int64)
+    cl = np.ascontiguousarray(contig_len, dtype=np.int64)
+    rb, rn, ro, rl = _concat(list(reads))
+    ch = np.zeros(max(n, 1), dtype=CHAIN_DTYPE)
+    for k, c in enumerate(chains):
+        ch[k] = tuple(c)
+    out = (FastAln * max(n, 1))()
+    band = max(cfg.bandwidth, 1)
+    cap = int((2 * rl.astype(np.int64) + band + 66).sum()) + 1
+    cig = np.zeros(cap, dtype=np.uint32)
+    md = C.create_string_buffer(4 * cap)
+    cu, mu, stats = C.c_int64(), C.c_int64(), FastStats()
+    st = lib().kira_fast_dp_batch(ctx.h, genome.h, len(co), _np_ptr(co), _np_ptr(cl), n,
+                                  _np_ptr(rb), rn, _np_ptr(ro), _np_ptr(rl), _np_ptr(ch),
+                                  C.byref(cfg), C.byref(knobs or fast_knobs()), out, _np_ptr(cig),
+                                  cap, md, len(md), C.byref(cu), C.byref(mu), C.byref(stats))
+    ctx.check(st, "fast_dp_batch")
+    recs = []
+    raw = md.raw
+    for k in range(n):
+        o = out[k]
+        if o.kind == 0:
+            recs.append(None)
+            continue
+        recs.append(dict(kind="packed" if o.kind == 1 else "wfa", ref_start=o.ref_start,
+                         ref_end=o.ref_end, score=o.score, nm=o.nm,
+                         md=raw[o.md_off:o.md_off + o.md_len].decode(),
+                         cigar=[int(x) for x in cig[o.cigar_off:o.cigar_off + o.n_cigar]]))
+    return recs, {f: getattr(stats, f) for f, _ in FastStats._fields_}
 
 
 def left_normalize_indels(cigar, contig: bytes, ref_start: int, read: bytes):

@@ -274,6 +274,37 @@ uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_l
 bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
                     uint32_t ref_start, const uint8_t *read, int64_t read_len);
 
+// Myers / WFA on device arrays, shared by the batch entry points and the fast
+// DP cascade.  WfaCosts::scored (src/alignment/wfa.rs:41-55): mismatch x,
+// gap open o, insertion extend ie, deletion extend de; false when the config
+// has none (no pair aligns).
+struct WfaPen {
+    int32_t x, o, ie, de;
+};
+bool wfa_pen(const kira_sw_cfg &cfg, WfaPen &pen);
+// Pairs on the device (pattern / text bytes, offsets, lengths, WfaOptions)
+// with host copies of the sizes the launch is planned from.
+struct WfaDevJobs {
+    int32_t n = 0;
+    const uint8_t *pat = nullptr, *txt = nullptr;
+    const int64_t *pat_off = nullptr, *txt_off = nullptr;
+    const int32_t *pat_len = nullptr, *txt_len = nullptr, *max_score = nullptr, *tbf = nullptr;
+    const int32_t *h_pat_len = nullptr, *h_txt_len = nullptr, *h_max_score = nullptr,
+                  *h_tbf = nullptr;
+};
+constexpr int kWfaRes = 6;  // found, loss, text_start, text_end, n_cigar, status
+// wfa_align_scored for every pair; res[6 k ..] as kWfaRes, the CIGARs (when
+// want_cigar) back to back in cigar from cig_off[k].  Synchronous.
+kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32_t drop,
+                    bool want_cigar, std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
+                    std::vector<int64_t> &cig_off);
+// bounded_edit_distance for n pairs on device arrays into device found /
+// dist / end; h_pat_len is the host copy of pat_len.  Synchronous.
+kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_t *pat_off,
+                      const int32_t *pat_len, const int32_t *h_pat_len, const uint8_t *txt,
+                      const int64_t *txt_off, const int32_t *txt_len, const int32_t *max_k,
+                      uint8_t *found, int32_t *dist, int32_t *end);
+
 }  // namespace kira
 
 struct kira_ctx {

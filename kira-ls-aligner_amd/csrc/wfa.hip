@@ -267,12 +267,9 @@ hipError_t launch_myers_class(int b, int32_t cnt, const int32_t *jobs, const Mye
 constexpr int32_t kNone = INT32_MIN / 4;  // NONE offset, wfa.rs
 constexpr int32_t kWfaMaxLen = 60000;     // pattern + text bytes staged in LDS
 constexpr int32_t kWfaMaxScore = 65535;
-constexpr int kWfaRes = 6;  // found, score, text_start, text_end, n_cigar, status
+using kira::kWfaRes;
 
-// WfaCosts (wfa.rs:41-55): mismatch x, gap open o, insertion / deletion extend
-struct Pen {
-    int32_t x, o, ie, de;
-};
+using Pen = kira::WfaPen;
 
 struct WfaArgs {
     const uint8_t *pat, *txt;
@@ -717,6 +714,201 @@ bool check_ranges(int32_t n, const int64_t *off, const int32_t *len, int64_t byt
 
 }  // namespace
 
+namespace kira {
+
+bool wfa_pen(const kira_sw_cfg &cfg, WfaPen &pen) {
+    const int64_t x = (int64_t)cfg.match_score + cfg.mismatch;
+    const int64_t ie = (int64_t)cfg.match_score + cfg.gap_extend;
+    if (cfg.match_score <= 0 || cfg.mismatch <= 0 || cfg.gap_open < 0 || cfg.gap_extend <= 0 ||
+        x > INT32_MAX || ie > INT32_MAX)
+        return false;
+    pen = WfaPen{(int32_t)x, cfg.gap_open, (int32_t)ie, cfg.gap_extend};
+    return true;
+}
+
+kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_t *pat_off,
+                      const int32_t *pat_len, const int32_t *h_pat_len, const uint8_t *txt,
+                      const int64_t *txt_off, const int32_t *txt_len, const int32_t *max_k,
+                      uint8_t *found, int32_t *dist, int32_t *end) {
+    if (n == 0) return KIRA_OK;
+    // classes by block count; 0 = above 512 bases
+    std::vector<int32_t> order, cls_start(kMyersRegBlocks + 2, 0), long_off;
+    try {
+        order.resize(n);
+        std::vector<int32_t> cls(n);
+        for (int32_t k = 0; k < n; k++) {
+            const int32_t b = (h_pat_len[k] + 63) / 64;
+            cls[k] = b > kMyersRegBlocks ? 0 : std::max(b, 1);
+            cls_start[cls[k] + 1]++;
+        }
+        for (int c = 0; c <= kMyersRegBlocks; c++) cls_start[c + 1] += cls_start[c];
+        std::vector<int32_t> pos(cls_start.begin(), cls_start.end() - 1);
+        for (int32_t k = 0; k < n; k++) order[pos[cls[k]]++] = k;
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    const int32_t n_long = cls_start[1];
+    std::vector<int64_t> loff(n_long + 1, 0);
+    for (int32_t t = 0; t < n_long; t++)
+        loff[t + 1] = loff[t] + 6 * (int64_t)((h_pat_len[order[t]] + 63) / 64);
+    DevMem dm;
+    int32_t *d_order = nullptr;
+    int64_t *d_loff = nullptr;
+    uint64_t *d_scratch = nullptr;
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    if ((e = dm.alloc(&d_order, n)) || (e = dm.alloc(&d_loff, n_long + 1)) ||
+        (e = dm.alloc(&d_scratch, loff[n_long])))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "Myers allocation", e);
+    if ((e = hipMemcpyAsync(d_order, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(d_loff, loff.data(), 8 * (size_t)(n_long + 1), hipMemcpyHostToDevice,
+                            st)))
+        return fail(ctx, KIRA_ERR_DRIVER, "Myers upload", e);
+    MyersArgs a{pat, txt, pat_off, txt_off, pat_len, txt_len, max_k, found, dist, end};
+    if (n_long) {
+        hipLaunchKernelGGL(myers_long_kernel, dim3((n_long + 255) / 256), dim3(256), 0, st, n_long,
+                           d_order, d_loff, d_scratch, a);
+        if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "Myers launch", e);
+    }
+    for (int b = 1; b <= kMyersRegBlocks; b++) {
+        const int32_t c0 = cls_start[b], cnt = cls_start[b + 1] - c0;
+        if ((e = launch_myers_class(b, cnt, d_order + c0, a, st)))
+            return fail(ctx, KIRA_ERR_RUNTIME, "Myers launch", e);
+    }
+    if ((e = hipStreamSynchronize(st))) return fail(ctx, KIRA_ERR_RUNTIME, "Myers", e);
+    return KIRA_OK;
+}
+
+kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32_t drop,
+                    bool want_cigar, std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
+                    std::vector<int64_t> &cig_out) {
+    const int32_t n = j.n;
+    try {
+        res.assign((size_t)n * kWfaRes, 0);
+        cig_out.assign(n, 0);
+        cigar.clear();
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    if (n == 0) return KIRA_OK;
+    int32_t m_max = 0, n_max = 0, tbf_max = 0, s_max = 0;
+    for (int32_t k = 0; k < n; k++) {
+        if (j.h_pat_len[k] + (int64_t)j.h_txt_len[k] > kWfaMaxLen)
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA pair longer than 60000 bytes");
+        if (j.h_max_score[k] > kWfaMaxScore)
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA max_score above 65535");
+        m_max = std::max(m_max, j.h_pat_len[k]);
+        n_max = std::max(n_max, j.h_txt_len[k]);
+        const int32_t t = std::min(std::max(j.h_tbf[k], 0), std::max(j.h_txt_len[k] - 1, 0));
+        tbf_max = std::max(tbf_max, t);
+        s_max = std::max(s_max, j.h_max_score[k]);
+    }
+    // CIGAR slots: every op has length >= 1 (<= m + n ops), and op runs
+    // alternate between M and gaps costing >= o + min(ie, de) each
+    std::vector<int64_t> cig_off;
+    std::vector<int32_t> cig_cap;
+    int64_t cig_total = 0;
+    try {
+        cig_off.resize(n);
+        cig_cap.resize(n);
+        const int64_t gap_min = (int64_t)pen.o + std::min(pen.ie, pen.de);
+        for (int32_t k = 0; k < n; k++) {
+            int64_t c = 0;
+            if (want_cigar) {
+                c = (int64_t)j.h_pat_len[k] + j.h_txt_len[k];
+                c = std::min<int64_t>(c, 2 * (std::max(j.h_max_score[k], 0) / gap_min) + 1) + 1;
+            }
+            cig_off[k] = cig_total;
+            cig_cap[k] = (int32_t)c;
+            cig_total += c;
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    int64_t val_cap = wfa_slot_values(pen, s_max, m_max, n_max, tbf_max);
+    val_cap = (val_cap + 3) & ~(int64_t)3;
+    const int64_t slot_ints = val_cap + 12 * (int64_t)(s_max + 1);
+    const int64_t slot_bytes = 4 * slot_ints;
+    const int64_t kSlabBudget = (int64_t)4 << 30;
+    int64_t slots = std::min<int64_t>({(int64_t)n, 2048, kSlabBudget / slot_bytes});
+    if (slots < 1) {
+        if (slot_bytes > ((int64_t)16 << 30))
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA history of one pair above 16 GiB");
+        slots = 1;
+    }
+    DevMem dm;
+    int64_t *d_coff = nullptr, *d_dst = nullptr;
+    int32_t *d_res = nullptr, *d_ccap = nullptr, *d_slab = nullptr, *d_queue = nullptr;
+    uint32_t *d_cig = nullptr, *d_cig2 = nullptr;
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    if ((e = dm.alloc(&d_res, (size_t)n * kWfaRes)) || (e = dm.alloc(&d_coff, n)) ||
+        (e = dm.alloc(&d_ccap, n)) || (e = dm.alloc(&d_cig, cig_total)) ||
+        (e = dm.alloc(&d_slab, (size_t)(slots * slot_ints))) || (e = dm.alloc(&d_queue, 1)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA allocation", e);
+    if ((e = hipMemcpyAsync(d_coff, cig_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(d_ccap, cig_cap.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemsetAsync(d_queue, 0, sizeof(int32_t), st)))
+        return fail(ctx, KIRA_ERR_DRIVER, "WFA upload", e);
+    WfaArgs a{};
+    a.pat = j.pat;
+    a.txt = j.txt;
+    a.pat_off = j.pat_off;
+    a.txt_off = j.txt_off;
+    a.pat_len = j.pat_len;
+    a.txt_len = j.txt_len;
+    a.max_score = j.max_score;
+    a.tbf = j.tbf;
+    a.drop = drop > 0 ? drop : 0;
+    a.want_cigar = want_cigar ? 1 : 0;
+    a.pen = pen;
+    a.res = d_res;
+    a.cigar = d_cig;
+    a.cig_off = d_coff;
+    a.cig_cap = d_ccap;
+    a.slab = d_slab;
+    a.slot_ints = slot_ints;
+    a.val_cap = val_cap;
+    a.desc_layers = s_max + 1;
+    a.queue = d_queue;
+    a.n = n;
+    const size_t lds = 4 * (size_t)((m_max + 3) / 4 + 2 + (n_max + 3) / 4 + 2);
+    hipLaunchKernelGGL(wfa_kernel, dim3((uint32_t)slots), dim3(64), lds, st, a);
+    if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA launch", e);
+    if ((e = hipMemcpyAsync(res.data(), d_res, 4 * res.size(), hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return fail(ctx, KIRA_ERR_RUNTIME, "WFA", e);
+    int64_t used = 0;
+    std::vector<int64_t> dst(n);
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t *r = &res[(size_t)k * kWfaRes];
+        if (r[5] != 0)
+            return fail(ctx, KIRA_ERR_RUNTIME,
+                        r[5] == 1 ? "WFA history slot overflow" : "WFA CIGAR slot overflow");
+        dst[k] = cig_out[k] = used;
+        used += r[4];
+    }
+    if (!want_cigar || used == 0) return KIRA_OK;
+    try {
+        cigar.resize(used);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    if ((e = dm.alloc(&d_dst, n)) || (e = dm.alloc(&d_cig2, used)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA allocation", e);
+    if ((e = hipMemcpyAsync(d_dst, dst.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)))
+        return fail(ctx, KIRA_ERR_DRIVER, "WFA upload", e);
+    hipLaunchKernelGGL(wfa_compact_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, d_res, d_cig,
+                       d_coff, d_dst, d_cig2);
+    if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA compact launch", e);
+    if ((e = hipMemcpyAsync(cigar.data(), d_cig2, 4 * (size_t)used, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return fail(ctx, KIRA_ERR_RUNTIME, "WFA", e);
+    return KIRA_OK;
+}
+
+}  // namespace kira
+
 extern "C" kira_status kira_myers_batch(kira_ctx *ctx, int32_t n, const uint8_t *patterns,
                                         int64_t patterns_bytes, const int64_t *pat_off,
                                         const int32_t *pat_len, const uint8_t *texts,
@@ -736,39 +928,16 @@ extern "C" kira_status kira_myers_batch(kira_ctx *ctx, int32_t n, const uint8_t 
         if (max_k[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "max_k < 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
-    // classes by block count; 0 = above 512 bases
-    std::vector<int32_t> order;
-    std::vector<int32_t> cls_start(kMyersRegBlocks + 2, 0);
-    try {
-        order.resize(n);
-        std::vector<int32_t> cls(n);
-        for (int32_t k = 0; k < n; k++) {
-            const int32_t b = (pat_len[k] + 63) / 64;
-            cls[k] = b > kMyersRegBlocks ? 0 : std::max(b, 1);
-            cls_start[cls[k] + 1]++;
-        }
-        for (int c = 0; c <= kMyersRegBlocks; c++) cls_start[c + 1] += cls_start[c];
-        std::vector<int32_t> pos(cls_start.begin(), cls_start.end() - 1);
-        for (int32_t k = 0; k < n; k++) order[pos[cls[k]]++] = k;
-    } catch (const std::bad_alloc &) {
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
-    }
-    const int32_t n_long = cls_start[1];
-    std::vector<int64_t> long_off(n_long + 1, 0);
-    for (int32_t t = 0; t < n_long; t++)
-        long_off[t + 1] = long_off[t] + 6 * (int64_t)((pat_len[order[t]] + 63) / 64);
     DevMem dm;
     uint8_t *d_pat = nullptr, *d_txt = nullptr, *d_found = nullptr;
-    int64_t *d_poff = nullptr, *d_toff = nullptr, *d_loff = nullptr;
+    int64_t *d_poff = nullptr, *d_toff = nullptr;
     int32_t *d_plen = nullptr, *d_tlen = nullptr, *d_k = nullptr, *d_dist = nullptr,
-            *d_end = nullptr, *d_order = nullptr;
-    uint64_t *d_scratch = nullptr;
+            *d_end = nullptr;
     hipStream_t st = ctx->stream;
     if ((e = dm.alloc(&d_pat, patterns_bytes)) || (e = dm.alloc(&d_txt, texts_bytes)) ||
         (e = dm.alloc(&d_poff, n)) || (e = dm.alloc(&d_toff, n)) || (e = dm.alloc(&d_plen, n)) ||
         (e = dm.alloc(&d_tlen, n)) || (e = dm.alloc(&d_k, n)) || (e = dm.alloc(&d_found, n)) ||
-        (e = dm.alloc(&d_dist, n)) || (e = dm.alloc(&d_end, n)) || (e = dm.alloc(&d_order, n)) ||
-        (e = dm.alloc(&d_loff, n_long + 1)) || (e = dm.alloc(&d_scratch, long_off[n_long])))
+        (e = dm.alloc(&d_dist, n)) || (e = dm.alloc(&d_end, n)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "Myers batch allocation", e);
     if ((patterns_bytes &&
          (e = hipMemcpyAsync(d_pat, patterns, patterns_bytes, hipMemcpyHostToDevice, st))) ||
@@ -777,22 +946,11 @@ extern "C" kira_status kira_myers_batch(kira_ctx *ctx, int32_t n, const uint8_t 
         (e = hipMemcpyAsync(d_toff, text_off, 8 * (size_t)n, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_plen, pat_len, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_tlen, text_len, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemcpyAsync(d_k, max_k, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemcpyAsync(d_order, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemcpyAsync(d_loff, long_off.data(), 8 * (size_t)(n_long + 1),
-                            hipMemcpyHostToDevice, st)))
+        (e = hipMemcpyAsync(d_k, max_k, 4 * (size_t)n, hipMemcpyHostToDevice, st)))
         return fail(ctx, KIRA_ERR_DRIVER, "Myers batch upload", e);
-    MyersArgs a{d_pat, d_txt, d_poff, d_toff, d_plen, d_tlen, d_k, d_found, d_dist, d_end};
-    if (n_long) {
-        hipLaunchKernelGGL(myers_long_kernel, dim3((n_long + 255) / 256), dim3(256), 0, st, n_long,
-                           d_order, d_loff, d_scratch, a);
-        if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "Myers launch", e);
-    }
-    for (int b = 1; b <= kMyersRegBlocks; b++) {
-        const int32_t c0 = cls_start[b], cnt = cls_start[b + 1] - c0;
-        if ((e = launch_myers_class(b, cnt, d_order + c0, a, st)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "Myers launch", e);
-    }
+    kira_status rs = kira::myers_dev(ctx, n, d_pat, d_poff, d_plen, pat_len, d_txt, d_toff, d_tlen,
+                                     d_k, d_found, d_dist, d_end);
+    if (rs != KIRA_OK) return rs;
     if ((e = hipMemcpyAsync(found, d_found, (size_t)n, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(dist, d_dist, 4 * (size_t)n, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(end, d_end, 4 * (size_t)n, hipMemcpyDeviceToHost, st)) ||
@@ -819,77 +977,28 @@ extern "C" kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *p
     if (!check_ranges(n, pat_off, pat_len, patterns_bytes) ||
         !check_ranges(n, text_off, text_len, texts_bytes))
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "pattern or text range outside its buffer");
-    // WfaCosts::scored (wfa.rs:41-55): None for every pair when invalid
-    const int64_t x64 = (int64_t)cfg->match_score + cfg->mismatch;
-    const int64_t ie64 = (int64_t)cfg->match_score + cfg->gap_extend;
-    if (cfg->match_score <= 0 || cfg->mismatch <= 0 || cfg->gap_open < 0 || cfg->gap_extend <= 0 ||
-        x64 > INT32_MAX || ie64 > INT32_MAX) {
+    Pen pen;
+    if (!kira::wfa_pen(*cfg, pen)) {
         for (int32_t k = 0; k < n; k++) out[k] = kira_wfa_result{0, 0, 0, 0, 0, 0};
         return KIRA_OK;
     }
-    const Pen pen{(int32_t)x64, cfg->gap_open, (int32_t)ie64, cfg->gap_extend};
-    int32_t m_max = 0, n_max = 0, tbf_max = 0, s_max = 0;
-    for (int32_t k = 0; k < n; k++) {
-        if (pat_len[k] + (int64_t)text_len[k] > kWfaMaxLen)
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA pair longer than 60000 bytes");
-        if (max_score[k] > kWfaMaxScore)
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA max_score above 65535");
-        m_max = std::max(m_max, pat_len[k]);
-        n_max = std::max(n_max, text_len[k]);
-        const int32_t t = text_begin_free ? std::min(std::max(text_begin_free[k], 0),
-                                                     std::max(text_len[k] - 1, 0))
-                                          : 0;
-        tbf_max = std::max(tbf_max, t);
-        s_max = std::max(s_max, max_score[k]);
-    }
-    // CIGAR slots: every op has length >= 1 (<= m + n ops), and op runs
-    // alternate between M and gaps costing >= o + min(ie, de) each
-    std::vector<int64_t> cig_off;
-    std::vector<int32_t> cig_cap;
-    int64_t cig_total = 0;
+    std::vector<int32_t> tbf;
     try {
-        cig_off.resize(n);
-        cig_cap.resize(n);
-        const int64_t gap_min = (int64_t)pen.o + std::min(pen.ie, pen.de);
-        for (int32_t k = 0; k < n; k++) {
-            int64_t c = 0;
-            if (want_cigar) {
-                c = (int64_t)pat_len[k] + text_len[k];
-                c = std::min<int64_t>(c, 2 * (std::max(max_score[k], 0) / gap_min) + 1) + 1;
-            }
-            cig_off[k] = cig_total;
-            cig_cap[k] = (int32_t)c;
-            cig_total += c;
-        }
+        tbf.assign(n, 0);
+        if (text_begin_free) tbf.assign(text_begin_free, text_begin_free + n);
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
-    }
-    int64_t val_cap = wfa_slot_values(pen, s_max, m_max, n_max, tbf_max);
-    val_cap = (val_cap + 3) & ~(int64_t)3;
-    const int64_t slot_ints = val_cap + 12 * (int64_t)(s_max + 1);
-    const int64_t slot_bytes = 4 * slot_ints;
-    const int64_t kSlabBudget = (int64_t)4 << 30;
-    int64_t slots = std::min<int64_t>({(int64_t)n, 2048, kSlabBudget / std::max<int64_t>(slot_bytes, 1)});
-    if (slots < 1) {
-        if (slot_bytes > ((int64_t)16 << 30))
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA history of one pair above 16 GiB");
-        slots = 1;
     }
     hipError_t e = hipSetDevice(ctx->device);
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
     DevMem dm;
     uint8_t *d_pat = nullptr, *d_txt = nullptr;
-    int64_t *d_poff = nullptr, *d_toff = nullptr, *d_coff = nullptr, *d_dst = nullptr;
-    int32_t *d_plen = nullptr, *d_tlen = nullptr, *d_ms = nullptr, *d_tbf = nullptr,
-            *d_res = nullptr, *d_ccap = nullptr, *d_slab = nullptr, *d_queue = nullptr;
-    uint32_t *d_cig = nullptr, *d_cig2 = nullptr;
+    int64_t *d_poff = nullptr, *d_toff = nullptr;
+    int32_t *d_plen = nullptr, *d_tlen = nullptr, *d_ms = nullptr, *d_tbf = nullptr;
     hipStream_t st = ctx->stream;
     if ((e = dm.alloc(&d_pat, patterns_bytes)) || (e = dm.alloc(&d_txt, texts_bytes)) ||
         (e = dm.alloc(&d_poff, n)) || (e = dm.alloc(&d_toff, n)) || (e = dm.alloc(&d_plen, n)) ||
-        (e = dm.alloc(&d_tlen, n)) || (e = dm.alloc(&d_ms, n)) || (e = dm.alloc(&d_tbf, n)) ||
-        (e = dm.alloc(&d_res, (size_t)n * kWfaRes)) || (e = dm.alloc(&d_coff, n)) ||
-        (e = dm.alloc(&d_ccap, n)) || (e = dm.alloc(&d_cig, cig_total)) ||
-        (e = dm.alloc(&d_slab, (size_t)(slots * slot_ints))) || (e = dm.alloc(&d_queue, 1)))
+        (e = dm.alloc(&d_tlen, n)) || (e = dm.alloc(&d_ms, n)) || (e = dm.alloc(&d_tbf, n)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA batch allocation", e);
     if ((patterns_bytes &&
          (e = hipMemcpyAsync(d_pat, patterns, patterns_bytes, hipMemcpyHostToDevice, st))) ||
@@ -899,69 +1008,33 @@ extern "C" kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *p
         (e = hipMemcpyAsync(d_plen, pat_len, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_tlen, text_len, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_ms, max_score, 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = text_begin_free
-                 ? hipMemcpyAsync(d_tbf, text_begin_free, 4 * (size_t)n, hipMemcpyHostToDevice, st)
-                 : hipMemsetAsync(d_tbf, 0, 4 * (size_t)n, st)) ||
-        (e = hipMemcpyAsync(d_coff, cig_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemcpyAsync(d_ccap, cig_cap.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemsetAsync(d_queue, 0, sizeof(int32_t), st)))
+        (e = hipMemcpyAsync(d_tbf, tbf.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)))
         return fail(ctx, KIRA_ERR_DRIVER, "WFA batch upload", e);
-    WfaArgs a{};
-    a.pat = d_pat;
-    a.txt = d_txt;
-    a.pat_off = d_poff;
-    a.txt_off = d_toff;
-    a.pat_len = d_plen;
-    a.txt_len = d_tlen;
-    a.max_score = d_ms;
-    a.tbf = d_tbf;
-    a.drop = adaptive_drop > 0 ? adaptive_drop : 0;
-    a.want_cigar = want_cigar ? 1 : 0;
-    a.pen = pen;
-    a.res = d_res;
-    a.cigar = d_cig;
-    a.cig_off = d_coff;
-    a.cig_cap = d_ccap;
-    a.slab = d_slab;
-    a.slot_ints = slot_ints;
-    a.val_cap = val_cap;
-    a.desc_layers = s_max + 1;
-    a.queue = d_queue;
-    a.n = n;
-    const size_t lds = 4 * (size_t)((m_max + 3) / 4 + 2 + (n_max + 3) / 4 + 2);
-    hipLaunchKernelGGL(wfa_kernel, dim3((uint32_t)slots), dim3(64), lds, st, a);
-    if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA launch", e);
+    kira::WfaDevJobs j;
+    j.n = n;
+    j.pat = d_pat;
+    j.txt = d_txt;
+    j.pat_off = d_poff;
+    j.txt_off = d_toff;
+    j.pat_len = d_plen;
+    j.txt_len = d_tlen;
+    j.max_score = d_ms;
+    j.tbf = d_tbf;
+    j.h_pat_len = pat_len;
+    j.h_txt_len = text_len;
+    j.h_max_score = max_score;
+    j.h_tbf = tbf.data();
     std::vector<int32_t> res;
-    try {
-        res.resize((size_t)n * kWfaRes);
-    } catch (const std::bad_alloc &) {
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
-    }
-    if ((e = hipMemcpyAsync(res.data(), d_res, 4 * res.size(), hipMemcpyDeviceToHost, st)) ||
-        (e = hipStreamSynchronize(st)))
-        return fail(ctx, KIRA_ERR_RUNTIME, "WFA batch", e);
-    int64_t used = 0;
-    std::vector<int64_t> dst(n);
+    std::vector<uint32_t> cig;
+    std::vector<int64_t> coff;
+    kira_status rs = kira::wfa_dev(ctx, j, pen, adaptive_drop, want_cigar != 0, res, cig, coff);
+    if (rs != KIRA_OK) return rs;
+    if ((int64_t)cig.size() > arena_cap && !cig.empty())
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "CIGAR arena too small");
     for (int32_t k = 0; k < n; k++) {
         const int32_t *r = &res[(size_t)k * kWfaRes];
-        if (r[5] != 0)
-            return fail(ctx, KIRA_ERR_RUNTIME,
-                        r[5] == 1 ? "WFA history slot overflow" : "WFA CIGAR slot overflow");
-        dst[k] = used;
-        out[k] = kira_wfa_result{r[0], r[1], r[2], r[3], r[4], used};
-        used += r[4];
+        out[k] = kira_wfa_result{r[0], r[1], r[2], r[3], r[4], coff[k]};
     }
-    if (!want_cigar || used == 0) return KIRA_OK;
-    if (used > arena_cap) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "CIGAR arena too small");
-    if ((e = dm.alloc(&d_dst, n)) || (e = dm.alloc(&d_cig2, used)))
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA batch allocation", e);
-    if ((e = hipMemcpyAsync(d_dst, dst.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)))
-        return fail(ctx, KIRA_ERR_DRIVER, "WFA batch upload", e);
-    hipLaunchKernelGGL(wfa_compact_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, d_res, d_cig,
-                       d_coff, d_dst, d_cig2);
-    if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA compact launch", e);
-    if ((e = hipMemcpyAsync(cigar_arena, d_cig2, 4 * (size_t)used, hipMemcpyDeviceToHost, st)) ||
-        (e = hipStreamSynchronize(st)))
-        return fail(ctx, KIRA_ERR_RUNTIME, "WFA batch", e);
+    if (!cig.empty()) memcpy(cigar_arena, cig.data(), 4 * cig.size());
     return KIRA_OK;
 }
