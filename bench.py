@@ -315,6 +315,10 @@ def main():
     ap.add_argument("--stage4-steps", type=int, default=3)
     ap.add_argument("--stage4-only", action="store_true",
                     help="run only the stage-4 driver section (for profiling)")
+    ap.add_argument("--fastpath-chains", type=int, default=200_000)
+    ap.add_argument("--fastpath-steps", type=int, default=3)
+    ap.add_argument("--fastpath-only", action="store_true",
+                    help="run only the fast DP cascade section (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
@@ -335,6 +339,10 @@ def main():
     if args.stage4_only:
         ctx = pkg.Context(0)
         print(json.dumps({"stage4": bench_stage4(pkg, ctx, args)}), flush=True)
+        return
+    if args.fastpath_only:
+        ctx = pkg.Context(0)
+        print(json.dumps({"fastpath": bench_fastpath(pkg, ctx, args)}), flush=True)
         return
     t_gen = time.time()
     work = make_sieve_workload(n_reads=args.reads)
@@ -493,7 +501,113 @@ def main():
         line["serving"] = bench_serving(pkg, ctx, genome, work, lw, args)
     if args.stage4_reads > 0:
         line["stage4"] = bench_stage4(pkg, ctx, args)
+    if args.fastpath_chains > 0:
+        line["fastpath"] = bench_fastpath(pkg, ctx, args)
     print(json.dumps(line), flush=True)
+
+
+# the read kinds of tests/fastpath_data.py weighted like the chains that reach
+# the reference's fast path on real data (about half certified ungapped, half
+# gapped): exact, substitutions, insertion, deletion, 5' deletion, foreign
+# tail, unrelated, N, lower case, contig end, over-long, across an N run
+FAST_MIX = [0.18, 0.30, 0.16, 0.16, 0.08, 0.02, 0.02, 0.02, 0.0, 0.02, 0.02, 0.02]
+
+
+def bench_fastpath(pkg, ctx, args):
+    """The fast DP cascade (kira_fast_dp_batch: packed certificate, Myers,
+    WFA with the origin retry) over one batch of chains, host arrays in and
+    records out, plus the WFA and Myers batch entry points alone on the
+    cascade's gapped pairs; the CPU restatement on a sample with parity."""
+    from tests.fastpath_data import make_fast_batch
+    from oracle import fastpath as F
+    from oracle import oracle as O
+    t0 = time.time()
+    n = args.fastpath_chains
+    b = make_fast_batch(31, n, lens=[2_500_000] * 8, mix=FAST_MIX)
+    print(f"[bench] fastpath batch: {n} chains generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    genome = pkg.Genome(ctx, b["genome"])
+    cfg = pkg.sw_cfg()
+    kn = pkg.fast_knobs()
+    # host arrays as a caller hands them over (marshalled once, outside the timing)
+    x = pkg.FastInputs(b["contig_off"], b["contig_len"], b["reads"], b["chains"])
+    pkg.fast_dp_run(ctx, genome, x, cfg, kn)
+    times, runs = [], []
+    for _ in range(args.fastpath_steps):
+        t0 = time.perf_counter()
+        r = pkg.fast_dp_run(ctx, genome, x, cfg, kn)
+        times.append(time.perf_counter() - t0)
+        runs.append(r)
+    k = int(np.argsort(times)[len(times) // 2])
+    arr, cig, md, stats = runs[k]
+    ms = 1e3 * times[k]
+    out = {"workload": f"fast DP cascade: 8 x 2.5 Mbp contigs, {n} chains of 76-250 bp reads "
+                       "(exact, substitutions, indels, 5' deletions, foreign tails, N), "
+                       "router defaults",
+           "chains": n, "ms_per_batch": ms, "chains_per_s": n / (ms * 1e-3), "stats": stats}
+    # the gapped pairs alone through the WFA and Myers entry points
+    pats, texts = [], []
+    for read, ch in zip(b["reads"], b["chains"]):
+        c = b["contigs"][ch[0]]
+        ws = ch[1] - ch[3]
+        if len(read) > 300 or ws < 0 or ws + len(read) + 32 > len(c):
+            continue
+        pats.append(read)
+        texts.append(c[ws:ws + len(read) + 32])
+    budget = [max((len(r) * 25 // 100) * 5, 42) for r in pats]
+    wx = pkg.PairInputs(pats, texts, budget)
+    mx = pkg.PairInputs(pats, texts, [len(r) * 15 // 100 + 4 for r in pats])
+    pkg.wfa_run(ctx, pkg.PairInputs(pats[:1000], texts[:1000], budget[:1000]), cfg)
+    t0 = time.perf_counter()
+    wres, _ = pkg.wfa_run(ctx, wx, cfg)
+    t_wfa = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pkg.myers_run(ctx, mx)
+    t_myers = time.perf_counter() - t0
+    out["wfa_pairs"] = len(pats)
+    out["wfa_pairs_per_s"] = len(pats) / t_wfa
+    out["wfa_found"] = int(wres["found"].sum())
+    out["myers_pairs_per_s"] = len(pats) / t_myers
+    print(f"[bench] fastpath: {n} chains, {ms:.1f} ms/batch, {out['chains_per_s']:.0f} chains/s, "
+          f"stats {stats}; WFA {out['wfa_pairs_per_s']:.0f} pairs/s, Myers "
+          f"{out['myers_pairs_per_s']:.0f} pairs/s (host arrays in/out)", file=sys.stderr,
+          flush=True)
+    if not args.no_cpu:
+        m = 4000
+        ocfg, router = O.cfg(), F.Router()
+        t0 = time.perf_counter()
+        want = [F.fast_dp(r, b["contigs"][ch[0]], ch[1], ch[3], ch[6], ocfg, router)
+                for r, ch in zip(b["reads"][:m], b["chains"][:m])]
+        cpu_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for r, t, bud in zip(pats[:m], texts[:m], budget[:m]):
+            O.wfa_align_scored(r, t, ocfg, bud)
+        cpu_wfa = time.perf_counter() - t0
+        norm = [None if w is None else dict(kind=1 if w[0] == "packed" else 2,
+                                            ref_start=w[1]["ref_start"], ref_end=w[1]["ref_end"],
+                                            score=w[1]["score"], nm=w[1]["nm"], md=w[1]["md"],
+                                            cigar=w[1]["cigar"])
+                for w in want]
+        gotm = []
+        for o in arr[:m]:
+            if o["kind"] == 0:
+                gotm.append(None)
+                continue
+            c0, m0 = int(o["cigar_off"]), int(o["md_off"])
+            gotm.append(dict(kind=int(o["kind"]), ref_start=int(o["ref_start"]),
+                             ref_end=int(o["ref_end"]), score=int(o["score"]), nm=int(o["nm"]),
+                             md=md[m0:m0 + int(o["md_len"])].decode(),
+                             cigar=[int(v) for v in cig[c0:c0 + int(o["n_cigar"])]]))
+        parity = norm == gotm
+        out["cpu_baseline"] = {"value": m / cpu_s, "unit": "chains/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} chains through the CPU restatement "
+                                         "(Python cascade over the C Myers / WFA oracle)"}
+        out["cpu_wfa_pairs_per_s"] = m / cpu_wfa
+        out["parity_vs_oracle_sample"] = parity
+        print(f"[bench] fastpath cpu baseline {m / cpu_s:.0f} chains/s, WFA {m / cpu_wfa:.0f} "
+              f"pairs/s (1 thread); parity={parity}", file=sys.stderr, flush=True)
+    genome.close()
+    return out
 
 
 def bench_stage4(pkg, ctx, args):

@@ -697,9 +697,12 @@ typedef struct {
 
 /* Chains past the router gates, certified by the packed scan, passed by
  * Myers, improved by the origin retry, declined by the partial-read check
- * (or a CIGAR that does not span the read), and aligned by WFA. */
+ * (or a CIGAR that does not span the read), and aligned by WFA; host wall
+ * time of the spectral scan, Myers, the WFA passes (with the origin probe)
+ * and the records. */
 typedef struct {
     int64_t attempted, spectral, myers_pass, origin_retries, clip_declined, wfa;
+    double phase_ms[4];
 } kira_fast_stats;
 
 /*

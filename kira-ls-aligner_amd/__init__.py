@@ -99,7 +99,7 @@ class FastAln(C.Structure):
 
 class FastStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("attempted", "spectral", "myers_pass", "origin_retries",
-                                         "clip_declined", "wfa")]
+                                         "clip_declined", "wfa")] + [("phase_ms", C.c_double * 4)]
 
 
 class BatchInput(C.Structure):
@@ -1225,53 +1225,83 @@ def _concat(seqs):
     return buf, len(blob), off, ln
 
 
+class PairInputs:
+    """(pattern, text) pairs as flat host arrays, with a per-pair int (Myers
+    max_k or WFA max_score) and an optional second one (WFA
+    text_begin_free)."""
+
+    def __init__(self, patterns, texts, a, b=None):
+        self.n = len(patterns)
+        self.pb, self.pn, self.po, self.pl = _concat(list(patterns))
+        self.tb, self.tn, self.to, self.tl = _concat(list(texts))
+        self.a = np.ascontiguousarray(a, dtype=np.int32)
+        self.b = np.ascontiguousarray(b if b is not None else np.zeros(self.n), dtype=np.int32)
+
+
+WFA_RES_DTYPE = np.dtype([("found", "<i4"), ("score", "<i4"), ("text_start", "<i4"),
+                          ("text_end", "<i4"), ("n_cigar", "<i4"), ("cigar_off", "<i8")],
+                         align=True)
+assert WFA_RES_DTYPE.itemsize == C.sizeof(WfaResult)
+
+
+def myers_run(ctx: Context, x: PairInputs):
+    """kira_myers_batch on marshalled pairs (a = max_k): (found, dist, end) arrays."""
+    n = x.n
+    found = np.zeros(max(n, 1), np.uint8)
+    dist = np.zeros(max(n, 1), np.int32)
+    end = np.zeros(max(n, 1), np.int32)
+    st = lib().kira_myers_batch(ctx.h, n, _np_ptr(x.pb), x.pn, _np_ptr(x.po), _np_ptr(x.pl),
+                                _np_ptr(x.tb), x.tn, _np_ptr(x.to), _np_ptr(x.tl), _np_ptr(x.a),
+                                _np_ptr(found), _np_ptr(dist), _np_ptr(end))
+    ctx.check(st, "myers_batch")
+    return found[:n], dist[:n], end[:n]
+
+
 def myers_batch(ctx: Context, pairs):
     """bounded_edit_distance (src/alignment/myers.rs:47-367) over a batch.
     pairs: sequence of (pattern, text, max_k).  Returns (dist, end) or None
     per pair."""
-    n = len(pairs)
-    if n == 0:
+    if not pairs:
         return []
-    pb, pn, po, pl = _concat([p[0] for p in pairs])
-    tb, tn, to, tl = _concat([p[1] for p in pairs])
-    mk = np.array([p[2] for p in pairs], dtype=np.int32)
-    found = np.zeros(n, np.uint8)
-    dist = np.zeros(n, np.int32)
-    end = np.zeros(n, np.int32)
-    st = lib().kira_myers_batch(ctx.h, n, _np_ptr(pb), pn, _np_ptr(po), _np_ptr(pl), _np_ptr(tb),
-                                tn, _np_ptr(to), _np_ptr(tl), _np_ptr(mk), _np_ptr(found),
-                                _np_ptr(dist), _np_ptr(end))
-    ctx.check(st, "myers_batch")
-    return [(int(dist[k]), int(end[k])) if found[k] else None for k in range(n)]
+    x = PairInputs([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs])
+    found, dist, end = myers_run(ctx, x)
+    return [(int(dist[k]), int(end[k])) if found[k] else None for k in range(x.n)]
+
+
+def wfa_run(ctx: Context, x: PairInputs, cfg: SwCfg, adaptive_drop: int = 0,
+            want_cigar: bool = True):
+    """kira_wfa_batch on marshalled pairs (a = max_score, b =
+    text_begin_free): (WFA_RES_DTYPE records, CIGAR words)."""
+    n = x.n
+    out = np.zeros(max(n, 1), dtype=WFA_RES_DTYPE)
+    cap = int((x.pl.astype(np.int64) + x.tl).sum() + 2 * n + 2) if want_cigar else 0
+    arena = np.zeros(max(cap, 1), dtype=np.uint32)
+    st = lib().kira_wfa_batch(ctx.h, n, _np_ptr(x.pb), x.pn, _np_ptr(x.po), _np_ptr(x.pl),
+                              _np_ptr(x.tb), x.tn, _np_ptr(x.to), _np_ptr(x.tl), _np_ptr(x.a),
+                              _np_ptr(x.b), int(adaptive_drop), C.byref(cfg),
+                              int(bool(want_cigar)), _np_ptr(out), _np_ptr(arena), cap)
+    ctx.check(st, "wfa_batch")
+    return out[:n], arena
 
 
 def wfa_batch(ctx: Context, pairs, cfg: SwCfg, adaptive_drop: int = 0, want_cigar: bool = True):
     """wfa_align_scored (src/alignment/wfa.rs:58-67) over a batch.  pairs:
     sequence of (pattern, text, max_score, text_begin_free).  Returns per
     pair None or dict(score (the loss), text_start, text_end, cigar)."""
-    n = len(pairs)
-    if n == 0:
+    if not pairs:
         return []
-    pb, pn, po, pl = _concat([p[0] for p in pairs])
-    tb, tn, to, tl = _concat([p[1] for p in pairs])
-    ms = np.array([p[2] for p in pairs], dtype=np.int32)
-    tbf = np.array([p[3] if len(p) > 3 else 0 for p in pairs], dtype=np.int32)
-    out = (WfaResult * n)()
-    cap = int((pl.astype(np.int64) + tl).sum() + 2 * n + 2) if want_cigar else 0
-    arena = np.zeros(max(cap, 1), dtype=np.uint32)
-    st = lib().kira_wfa_batch(ctx.h, n, _np_ptr(pb), pn, _np_ptr(po), _np_ptr(pl), _np_ptr(tb),
-                              tn, _np_ptr(to), _np_ptr(tl), _np_ptr(ms), _np_ptr(tbf),
-                              int(adaptive_drop), C.byref(cfg), int(bool(want_cigar)), out,
-                              _np_ptr(arena), cap)
-    ctx.check(st, "wfa_batch")
+    x = PairInputs([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs],
+                   [p[3] if len(p) > 3 else 0 for p in pairs])
+    out, arena = wfa_run(ctx, x, cfg, adaptive_drop, want_cigar)
     res = []
-    for k in range(n):
-        o = out[k]
-        if not o.found:
+    for o in out:
+        if not o["found"]:
             res.append(None)
             continue
-        res.append(dict(score=o.score, text_start=o.text_start, text_end=o.text_end,
-                        cigar=[int(x) for x in arena[o.cigar_off:o.cigar_off + o.n_cigar]]))
+        c0 = int(o["cigar_off"])
+        res.append(dict(score=int(o["score"]), text_start=int(o["text_start"]),
+                        text_end=int(o["text_end"]),
+                        cigar=[int(v) for v in arena[c0:c0 + int(o["n_cigar"])]]))
     return res
 
 
@@ -1290,44 +1320,70 @@ def fast_knobs(**kw) -> FastKnobs:
     return k
 
 
+FAST_ALN_DTYPE = np.dtype([("kind", "<i4"), ("ref_start", "<u4"), ("ref_end", "<u4"),
+                           ("read_start", "<u4"), ("read_end", "<u4"), ("score", "<i4"),
+                           ("nm", "<u4"), ("n_cigar", "<i4"), ("md_len", "<i4"),
+                           ("cigar_off", "<i8"), ("md_off", "<i8")], align=True)
+assert FAST_ALN_DTYPE.itemsize == C.sizeof(FastAln)
+
+
+class FastInputs:
+    """A fast-path batch as the flat host arrays kira_fast_dp_batch reads:
+    contig i = genome bases contig_off[i] .. +contig_len[i]; reads: list of
+    strand-oriented bytes; chains: one (ref_id, ref_start, ref_end,
+    read_start, read_end, strand, score) per read."""
+
+    def __init__(self, contig_off, contig_len, reads, chains):
+        self.n = len(reads)
+        self.co = np.ascontiguousarray(contig_off, dtype=np.int64)
+        self.cl = np.ascontiguousarray(contig_len, dtype=np.int64)
+        self.rb, self.rn, self.ro, self.rl = _concat(list(reads))
+        self.ch = (np.array([tuple(c) for c in chains], dtype=CHAIN_DTYPE) if self.n
+                   else np.zeros(1, dtype=CHAIN_DTYPE))
+
+
+def fast_dp_run(ctx: Context, genome: "Genome", x: FastInputs, cfg: SwCfg,
+                knobs: Optional[FastKnobs] = None):
+    """kira_fast_dp_batch on marshalled inputs.  Returns (records as a
+    FAST_ALN_DTYPE array, CIGAR words, MD bytes, stats dict)."""
+    n = x.n
+    out = np.zeros(max(n, 1), dtype=FAST_ALN_DTYPE)
+    band = max(cfg.bandwidth, 1)
+    cap = int((2 * x.rl.astype(np.int64) + band + 66).sum()) + 1
+    cig = np.zeros(cap, dtype=np.uint32)
+    md = np.zeros(4 * cap, dtype=np.uint8)
+    cu, mu, stats = C.c_int64(), C.c_int64(), FastStats()
+    st = lib().kira_fast_dp_batch(ctx.h, genome.h, len(x.co), _np_ptr(x.co), _np_ptr(x.cl), n,
+                                  _np_ptr(x.rb), x.rn, _np_ptr(x.ro), _np_ptr(x.rl), _np_ptr(x.ch),
+                                  C.byref(cfg), C.byref(knobs or fast_knobs()), _np_ptr(out),
+                                  _np_ptr(cig), cap, _np_ptr(md), len(md), C.byref(cu),
+                                  C.byref(mu), C.byref(stats))
+    ctx.check(st, "fast_dp_batch")
+    sd = {f: getattr(stats, f) for f, _ in FastStats._fields_ if f != "phase_ms"}
+    sd["phase_ms"] = dict(zip(["spectral", "myers", "wfa", "records"], list(stats.phase_ms)))
+    return out[:n], cig[:cu.value], md[:mu.value].tobytes(), sd
+
+
 def fast_dp_batch(ctx: Context, genome: "Genome", contig_off, contig_len, reads, chains,
                   cfg: SwCfg, knobs: Optional[FastKnobs] = None):
     """try_fast_dp_alignment_inner (src/alignment/mod.rs:285-440) over a batch.
 
-    reads: list of strand-oriented read bytes; chains: one (ref_id, ref_start,
-    ref_end, read_start, read_end, strand, score) per read.  Returns (records,
-    stats): per read None or dict(kind 'packed' | 'wfa', ref_start, ref_end,
-    score, nm, md, cigar (words))."""
-    n = len(reads)
-    co = np.ascontiguousarray(contig_off, dtype=np.int64)
-    cl = np.ascontiguousarray(contig_len, dtype=np.int64)
-    rb, rn, ro, rl = _concat(list(reads))
-    ch = np.zeros(max(n, 1), dtype=CHAIN_DTYPE)
-    for k, c in enumerate(chains):
-        ch[k] = tuple(c)
-    out = (FastAln * max(n, 1))()
-    band = max(cfg.bandwidth, 1)
-    cap = int((2 * rl.astype(np.int64) + band + 66).sum()) + 1
-    cig = np.zeros(cap, dtype=np.uint32)
-    md = C.create_string_buffer(4 * cap)
-    cu, mu, stats = C.c_int64(), C.c_int64(), FastStats()
-    st = lib().kira_fast_dp_batch(ctx.h, genome.h, len(co), _np_ptr(co), _np_ptr(cl), n,
-                                  _np_ptr(rb), rn, _np_ptr(ro), _np_ptr(rl), _np_ptr(ch),
-                                  C.byref(cfg), C.byref(knobs or fast_knobs()), out, _np_ptr(cig),
-                                  cap, md, len(md), C.byref(cu), C.byref(mu), C.byref(stats))
-    ctx.check(st, "fast_dp_batch")
+    Returns (records, stats): per read None or dict(kind 'packed' | 'wfa',
+    ref_start, ref_end, score, nm, md, cigar (words))."""
+    x = FastInputs(contig_off, contig_len, reads, chains)
+    out, cig, md, stats = fast_dp_run(ctx, genome, x, cfg, knobs)
     recs = []
-    raw = md.raw
-    for k in range(n):
-        o = out[k]
-        if o.kind == 0:
+    for o in out:
+        if o["kind"] == 0:
             recs.append(None)
             continue
-        recs.append(dict(kind="packed" if o.kind == 1 else "wfa", ref_start=o.ref_start,
-                         ref_end=o.ref_end, score=o.score, nm=o.nm,
-                         md=raw[o.md_off:o.md_off + o.md_len].decode(),
-                         cigar=[int(x) for x in cig[o.cigar_off:o.cigar_off + o.n_cigar]]))
-    return recs, {f: getattr(stats, f) for f, _ in FastStats._fields_}
+        c0, m0 = int(o["cigar_off"]), int(o["md_off"])
+        recs.append(dict(kind="packed" if o["kind"] == 1 else "wfa",
+                         ref_start=int(o["ref_start"]), ref_end=int(o["ref_end"]),
+                         score=int(o["score"]), nm=int(o["nm"]),
+                         md=md[m0:m0 + int(o["md_len"])].decode(),
+                         cigar=[int(v) for v in cig[c0:c0 + int(o["n_cigar"])]]))
+    return recs, stats
 
 
 def left_normalize_indels(cigar, contig: bytes, ref_start: int, read: bytes):

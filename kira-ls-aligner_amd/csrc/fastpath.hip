@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <new>
 #include <string>
 #include <vector>
@@ -21,6 +22,9 @@
 #include "internal.hpp"
 
 namespace {
+
+using kira::host_threads;
+using kira::parallel_chunks;
 
 kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e = hipSuccess) {
     if (ctx) {
@@ -228,11 +232,16 @@ struct Job {
     int32_t shift, mism; // certificate
     // final WFA
     int32_t loss, text_start, text_end, n_ops;
-    int64_t ops_at;      // into the job's CIGAR store
+    const uint32_t *ops; // the CIGAR of the alignment kept
     int64_t ftext;       // final text: offset from genome base 0
     int32_t ftext_len;
     int32_t first_loss;
+    int32_t retry;       // index into the retry list, or -1
 };
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 
 }  // namespace
 
@@ -364,6 +373,7 @@ extern "C" kira_status kira_fast_dp_batch(
     auto text_of = [&](const Job &j) { return contig_off[j.contig] + j.ws; };
     try {
         // ---- packed spectral certificate
+        auto t_sp = std::chrono::steady_clock::now();
         std::vector<int32_t> cand;
         if (kn.algo == 0 || kn.algo == 1) {
             lists.clear();
@@ -400,6 +410,8 @@ extern "C" kira_status kira_fast_dp_batch(
         }
         for (int32_t q = 0; q < (int32_t)jobs.size(); q++)
             if (jobs[q].stage == 0) cand.push_back(q);
+        st_local.phase_ms[0] = ms_since(t_sp);
+        auto t_my = std::chrono::steady_clock::now();
         // ---- Myers reject (router.rs:196-202)
         std::vector<int32_t> wfa_jobs;
         if (!cand.empty() && pen_ok) {
@@ -427,11 +439,9 @@ extern "C" kira_status kira_fast_dp_batch(
                 if (found[i]) wfa_jobs.push_back(cand[i]);
             st_local.myers_pass = (int64_t)wfa_jobs.size();
         }
+        st_local.phase_ms[1] = ms_since(t_my);
         // ---- scored WFA (wfa_score_budget, router.rs:160-166)
-        std::vector<uint32_t> ops_store;
-        std::vector<int32_t> retry_jobs;
-        std::vector<int64_t> retry_text;
-        std::vector<int32_t> retry_len, retry_free;
+        auto t_wfa = std::chrono::steady_clock::now();
         const int32_t drop = kn.wfa_adaptive > 0 ? kn.wfa_adaptive : 0;
         auto run_wfa = [&](const std::vector<int32_t> &list, const std::vector<int64_t> &tofs,
                            const std::vector<int32_t> &tlens, const std::vector<int32_t> &budget,
@@ -460,6 +470,12 @@ extern "C" kira_status kira_fast_dp_batch(
             wj.h_tbf = lists.y.data();
             return kira::wfa_dev(ctx, wj, pen, drop, true, wres, wcig, woff);
         };
+        // the CIGARs of both passes stay alive until the records are built
+        std::vector<int32_t> wres1, wres2;
+        std::vector<uint32_t> wcig1, wcig2;
+        std::vector<int64_t> woff1, woff2;
+        std::vector<int32_t> retry_jobs, retry_len, retry_free;
+        std::vector<int64_t> retry_text;
         if (!wfa_jobs.empty()) {
             const size_t m = wfa_jobs.size();
             std::vector<int64_t> tofs(m);
@@ -479,167 +495,215 @@ extern "C" kira_status kira_fast_dp_batch(
                 j.ftext = tofs[i];
                 j.ftext_len = tlens[i];
             }
-            std::vector<int32_t> wres;
-            std::vector<uint32_t> wcig;
-            std::vector<int64_t> woff;
-            kira_status rs = run_wfa(wfa_jobs, tofs, tlens, budget, tbf, wres, wcig, woff);
+            kira_status rs = run_wfa(wfa_jobs, tofs, tlens, budget, tbf, wres1, wcig1, woff1);
             if (rs != KIRA_OK) return rs;
+            std::vector<int32_t> suspects;
             for (size_t i = 0; i < m; i++) {
                 Job &j = jobs[wfa_jobs[i]];
-                const int32_t *r = &wres[i * kira::kWfaRes];
+                const int32_t *r = &wres1[i * kira::kWfaRes];
+                j.retry = -1;
                 if (!r[0]) continue;
                 j.stage = 2;
                 j.loss = j.first_loss = r[1];
                 j.text_start = r[2];
                 j.text_end = r[3];
                 j.n_ops = r[4];
-                j.ops_at = (int64_t)ops_store.size();
-                ops_store.insert(ops_store.end(), wcig.begin() + woff[i],
-                                 wcig.begin() + woff[i] + r[4]);
+                j.ops = wcig1.data() + woff1[i];
                 // suspect origin (mod.rs:373-383): a leading gap, or a short
                 // leading match followed by more ops
-                const uint32_t f = j.n_ops ? ops_store[j.ops_at] : 0;
+                const uint32_t f = j.n_ops ? j.ops[0] : 0;
                 const bool suspect =
                     j.n_ops > 0 && ((f & 15u) != 0 || ((f >> 4) < 19 && j.n_ops > 1));
                 const int64_t lead = std::min<int64_t>({(int64_t)std::max(kn.wfa_lead, 0), j.ws,
                                                         (int64_t)band});
                 if (!(suspect && lead == 0 && kn.wfa_ends_free == 0)) continue;
-                const int64_t slack = std::min<int64_t>({(int64_t)band, 32, j.ws});
-                if (slack <= 0) continue;
-                const uint8_t *contig = gh + contig_off[j.contig];
-                const int64_t clen = contig_len[j.contig];
-                const uint8_t *rd = reads + read_off[j.k];
-                int64_t origin = j.ws, prefix = 0;
-                const int64_t hi = std::min<int64_t>(j.ws + slack, clen - 1);
-                for (int64_t s0 = j.ws - slack; s0 <= hi; s0++) {
-                    const int64_t mlen = common_prefix_len(rd, j.L, contig + s0, clen - s0);
-                    if (mlen > prefix) {
-                        prefix = mlen;
-                        origin = s0;
+                if (std::min<int64_t>({(int64_t)band, 32, j.ws}) <= 0) continue;
+                j.retry = (int32_t)suspects.size();
+                suspects.push_back(wfa_jobs[i]);
+            }
+            // the origin probe (mod.rs:389-399): the longest exact read prefix
+            // among the starts within the slack
+            const size_t ns = suspects.size();
+            retry_jobs = suspects;
+            retry_text.resize(ns);
+            retry_len.resize(ns);
+            retry_free.resize(ns);
+            parallel_chunks((int64_t)ns, 512, [&](int, int64_t b0, int64_t e0) {
+                for (int64_t i = b0; i < e0; i++) {
+                    const Job &j = jobs[suspects[i]];
+                    const int64_t slack = std::min<int64_t>({(int64_t)band, 32, j.ws});
+                    const uint8_t *contig = gh + contig_off[j.contig];
+                    const int64_t clen = contig_len[j.contig];
+                    const uint8_t *rd = reads + read_off[j.k];
+                    int64_t origin = j.ws, prefix = 0;
+                    const int64_t hi = std::min<int64_t>(j.ws + slack, clen - 1);
+                    for (int64_t s0 = j.ws - slack; s0 <= hi; s0++) {
+                        const int64_t mlen = common_prefix_len(rd, j.L, contig + s0, clen - s0);
+                        if (mlen > prefix) {
+                            prefix = mlen;
+                            origin = s0;
+                        }
+                    }
+                    if (prefix >= 4) {
+                        const int64_t oend = std::min<int64_t>(origin + j.L + band, clen);
+                        retry_text[i] = contig_off[j.contig] + origin;
+                        retry_len[i] = (int32_t)(oend - origin);
+                        retry_free[i] = 0;
+                    } else {
+                        retry_text[i] = contig_off[j.contig] + j.ws - slack;
+                        retry_len[i] = (int32_t)(j.tl + slack);
+                        retry_free[i] = (int32_t)(2 * slack);
                     }
                 }
-                if (prefix >= 4) {
-                    const int64_t oend = std::min<int64_t>(origin + j.L + band, clen);
-                    retry_text.push_back(contig_off[j.contig] + origin);
-                    retry_len.push_back((int32_t)(oend - origin));
-                    retry_free.push_back(0);
-                } else {
-                    retry_text.push_back(contig_off[j.contig] + j.ws - slack);
-                    retry_len.push_back((int32_t)(j.tl + slack));
-                    retry_free.push_back((int32_t)(2 * slack));
-                }
-                retry_jobs.push_back(wfa_jobs[i]);
-            }
+            });
         }
         // ---- the bounded ends-free retry (mod.rs:384-426)
         if (!retry_jobs.empty()) {
             const size_t m = retry_jobs.size();
             std::vector<int32_t> budget(m);
             for (size_t i = 0; i < m; i++) budget[i] = jobs[retry_jobs[i]].first_loss;
-            std::vector<int32_t> wres;
-            std::vector<uint32_t> wcig;
-            std::vector<int64_t> woff;
-            kira_status rs = run_wfa(retry_jobs, retry_text, retry_len, budget, retry_free, wres,
-                                     wcig, woff);
+            kira_status rs = run_wfa(retry_jobs, retry_text, retry_len, budget, retry_free, wres2,
+                                     wcig2, woff2);
             if (rs != KIRA_OK) return rs;
             for (size_t i = 0; i < m; i++) {
                 Job &j = jobs[retry_jobs[i]];
-                const int32_t *r = &wres[i * kira::kWfaRes];
+                const int32_t *r = &wres2[i * kira::kWfaRes];
                 if (!r[0] || r[1] >= j.loss) continue;
                 st_local.origin_retries++;
                 j.loss = r[1];
                 j.text_start = r[2];
                 j.text_end = r[3];
                 j.n_ops = r[4];
-                j.ops_at = (int64_t)ops_store.size();
-                ops_store.insert(ops_store.end(), wcig.begin() + woff[i],
-                                 wcig.begin() + woff[i] + r[4]);
+                j.ops = wcig2.data() + woff2[i];
                 j.ftext = retry_text[i];
                 j.ftext_len = retry_len[i];
             }
         }
+        st_local.phase_ms[2] = ms_since(t_wfa);
         // ---- records (host): partial-read check, wfa_result_to_alignment
-        std::vector<uint32_t> cig_all;
-        std::string md_all;
-        for (Job &j : jobs) {
-            kira_fast_aln &a = res[j.k];
-            const uint8_t *rd = reads + read_off[j.k];
-            if (j.stage == 1) {
-                const uint8_t *ref = gh + contig_off[j.contig] + j.ws + j.shift;
-                a.kind = 1;
-                a.ref_start = (uint32_t)(j.ws + j.shift);
-                a.ref_end = a.ref_start + (uint32_t)j.L;
-                a.read_start = 0;
-                a.read_end = (uint32_t)j.L;
-                a.score = (j.L - j.mism) * c.match_score - j.mism * c.mismatch;
-                a.cigar_off = (int64_t)cig_all.size();
-                a.n_cigar = 1;
-                cig_all.push_back((uint32_t)j.L << 4);
-                a.md_off = (int64_t)md_all.size();
-                a.nm = kira::ungapped_tags(rd, ref, j.L, md_all);
-                a.md_len = (int32_t)((int64_t)md_all.size() - a.md_off);
-                st_local.spectral++;
-                continue;
-            }
-            if (j.stage != 2) continue;
-            const uint8_t *text = gh + j.ftext;
-            const uint32_t *ops = ops_store.data() + j.ops_at;
-            const int64_t perfect = (int64_t)std::min<int64_t>(
-                std::max<int64_t>((int64_t)c.match_score * j.L, INT32_MIN), INT32_MAX);
-            if ((int64_t)j.loss * 10 > perfect * 3 &&
-                clipping_can_improve(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops, c)) {
-                st_local.clip_declined++;
-                continue;
-            }
-            int64_t cref = 0, cq = 0;
-            for (int32_t z = 0; z < j.n_ops; z++) {
-                const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
-                if (op == 0 || op == 2 || op == 3) cref += len;
-                if (op == 0 || op == 1 || op == 4) cq += len;
-            }
-            if (cq != j.L || j.text_start + cref > j.ftext_len || j.text_end > j.ftext_len) {
-                st_local.clip_declined++;
-                continue;
-            }
-            int32_t score = 0;
-            int64_t qi = 0, ti = j.text_start;
-            for (int32_t z = 0; z < j.n_ops; z++) {
-                const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
-                if (op == 0) {
-                    for (uint32_t u = 0; u < len; u++, qi++, ti++)
-                        score += rd[qi] == text[ti] ? c.match_score : -c.mismatch;
-                } else if (op == 1 || op == 2) {
-                    score -= c.gap_open + c.gap_extend * (int32_t)len;
-                    if (op == 1)
-                        qi += len;
-                    else
-                        ti += len;
+        auto t_rec = std::chrono::steady_clock::now();
+        const int T = host_threads();
+        struct Part {
+            std::vector<uint32_t> cig;
+            std::string md;
+            int64_t spectral = 0, wfa = 0, declined = 0;
+        };
+        std::vector<Part> parts(T);
+        const int64_t nj = (int64_t)jobs.size();
+        std::vector<int64_t> bounds(T + 1, nj);
+        for (int t = 0; t <= T; t++) bounds[t] = nj * t / T;
+        parallel_chunks(T, 1, [&](int, int64_t t0_, int64_t t1_) {
+            for (int64_t t = t0_; t < t1_; t++) {
+                Part &P = parts[t];
+                for (int64_t q = bounds[t]; q < bounds[t + 1]; q++) {
+                    const Job &j = jobs[q];
+                    kira_fast_aln &a = res[j.k];
+                    const uint8_t *rd = reads + read_off[j.k];
+                    if (j.stage == 1) {
+                        const uint8_t *ref = gh + contig_off[j.contig] + j.ws + j.shift;
+                        a.kind = 1;
+                        a.ref_start = (uint32_t)(j.ws + j.shift);
+                        a.ref_end = a.ref_start + (uint32_t)j.L;
+                        a.read_start = 0;
+                        a.read_end = (uint32_t)j.L;
+                        a.score = (j.L - j.mism) * c.match_score - j.mism * c.mismatch;
+                        a.cigar_off = (int64_t)P.cig.size();
+                        a.n_cigar = 1;
+                        P.cig.push_back((uint32_t)j.L << 4);
+                        a.md_off = (int64_t)P.md.size();
+                        a.nm = kira::ungapped_tags(rd, ref, j.L, P.md);
+                        a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
+                        P.spectral++;
+                        continue;
+                    }
+                    if (j.stage != 2) continue;
+                    const uint8_t *text = gh + j.ftext;
+                    const uint32_t *ops = j.ops;
+                    const int64_t perfect = std::min<int64_t>(
+                        std::max<int64_t>((int64_t)c.match_score * j.L, INT32_MIN), INT32_MAX);
+                    if ((int64_t)j.loss * 10 > perfect * 3 &&
+                        clipping_can_improve(rd, j.L, text, j.ftext_len, j.text_start, ops,
+                                             j.n_ops, c)) {
+                        P.declined++;
+                        continue;
+                    }
+                    int64_t cref = 0, cq = 0;
+                    for (int32_t z = 0; z < j.n_ops; z++) {
+                        const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
+                        if (op == 0 || op == 2 || op == 3) cref += len;
+                        if (op == 0 || op == 1 || op == 4) cq += len;
+                    }
+                    if (cq != j.L || j.text_start + cref > j.ftext_len || j.text_end > j.ftext_len) {
+                        P.declined++;
+                        continue;
+                    }
+                    int32_t score = 0;
+                    int64_t qi = 0, ti = j.text_start;
+                    for (int32_t z = 0; z < j.n_ops; z++) {
+                        const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
+                        if (op == 0) {
+                            for (uint32_t u = 0; u < len; u++, qi++, ti++)
+                                score += rd[qi] == text[ti] ? c.match_score : -c.mismatch;
+                        } else if (op == 1 || op == 2) {
+                            score -= c.gap_open + c.gap_extend * (int32_t)len;
+                            if (op == 1)
+                                qi += len;
+                            else
+                                ti += len;
+                        }
+                    }
+                    const int64_t fstart = j.ftext - contig_off[j.contig];
+                    a.kind = 2;
+                    a.ref_start = (uint32_t)(fstart + j.text_start);
+                    a.ref_end = (uint32_t)(fstart + j.text_end);
+                    a.read_start = 0;
+                    a.read_end = (uint32_t)j.L;
+                    a.score = score;
+                    a.cigar_off = (int64_t)P.cig.size();
+                    a.n_cigar = j.n_ops;
+                    P.cig.insert(P.cig.end(), ops, ops + j.n_ops);
+                    a.md_off = (int64_t)P.md.size();
+                    a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops, P.md);
+                    a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
+                    P.wfa++;
                 }
             }
-            const int64_t fstart = j.ftext - contig_off[j.contig];
-            a.kind = 2;
-            a.ref_start = (uint32_t)(fstart + j.text_start);
-            a.ref_end = (uint32_t)(fstart + j.text_end);
-            a.read_start = 0;
-            a.read_end = (uint32_t)j.L;
-            a.score = score;
-            a.cigar_off = (int64_t)cig_all.size();
-            a.n_cigar = j.n_ops;
-            cig_all.insert(cig_all.end(), ops, ops + j.n_ops);
-            a.md_off = (int64_t)md_all.size();
-            a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops, md_all);
-            a.md_len = (int32_t)((int64_t)md_all.size() - a.md_off);
-            st_local.wfa++;
+        });
+        // place each part's CIGAR words and MD text after the previous parts'
+        int64_t cig_total = 0, md_total = 0;
+        std::vector<int64_t> cig_base(T), md_base(T);
+        for (int t = 0; t < T; t++) {
+            cig_base[t] = cig_total;
+            md_base[t] = md_total;
+            cig_total += (int64_t)parts[t].cig.size();
+            md_total += (int64_t)parts[t].md.size();
+            st_local.spectral += parts[t].spectral;
+            st_local.wfa += parts[t].wfa;
+            st_local.clip_declined += parts[t].declined;
         }
-        if (cigar_used) *cigar_used = (int64_t)cig_all.size();
-        if (md_used) *md_used = (int64_t)md_all.size();
-        if (stats) *stats = st_local;
-        if ((int64_t)cig_all.size() > cigar_cap || (int64_t)md_all.size() > md_cap ||
-            (!cig_all.empty() && !cigar_out) || (!md_all.empty() && !md_out))
+        for (int t = 0; t < T; t++)
+            for (int64_t q = bounds[t]; q < bounds[t + 1]; q++) {
+                kira_fast_aln &a = res[jobs[q].k];
+                if (a.kind == 0) continue;
+                a.cigar_off += cig_base[t];
+                a.md_off += md_base[t];
+            }
+        if (cigar_used) *cigar_used = cig_total;
+        if (md_used) *md_used = md_total;
+        if (cig_total > cigar_cap || md_total > md_cap || (cig_total && !cigar_out) ||
+            (md_total && !md_out)) {
+            if (stats) *stats = st_local;
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "CIGAR or MD buffer too small");
-        if (!cig_all.empty()) memcpy(cigar_out, cig_all.data(), 4 * cig_all.size());
-        if (!md_all.empty()) memcpy(md_out, md_all.data(), md_all.size());
+        }
+        for (int t = 0; t < T; t++) {
+            if (!parts[t].cig.empty())
+                memcpy(cigar_out + cig_base[t], parts[t].cig.data(), 4 * parts[t].cig.size());
+            if (!parts[t].md.empty())
+                memcpy(md_out + md_base[t], parts[t].md.data(), parts[t].md.size());
+        }
         memcpy(out, res.data(), sizeof(kira_fast_aln) * (size_t)n);
+        st_local.phase_ms[3] = ms_since(t_rec);
+        if (stats) *stats = st_local;
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }

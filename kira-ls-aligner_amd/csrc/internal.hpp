@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kira_hip.h"
@@ -273,6 +276,37 @@ uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_l
 // CIGAR over a whole contig and the strand-oriented read; true if it moved.
 bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
                     uint32_t ref_start, const uint8_t *read, int64_t read_len);
+
+// Host worker threads for record building: the machine's, at most 16.
+inline int host_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h ? h : 1u, 16u));
+}
+
+// f(t, begin, end) on up to host_threads() threads over [0, n) in chunks of
+// at least `grain`; a std::bad_alloc in any chunk is rethrown here.
+template <class F>
+void parallel_chunks(int64_t n, int64_t grain, F &&f) {
+    if (n <= 0) return;
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), (n + grain - 1) / grain));
+    if (T == 1) {
+        f(0, (int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<char> failed(T, 0);
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&, t]() {
+            try {
+                f(t, n * t / T, n * (t + 1) / T);
+            } catch (...) {
+                failed[t] = 1;
+            }
+        });
+    for (auto &x : th) x.join();
+    for (int t = 0; t < T; t++)
+        if (failed[t]) throw std::bad_alloc();
+}
 
 // Myers / WFA on device arrays, shared by the batch entry points and the fast
 // DP cascade.  WfaCosts::scored (src/alignment/wfa.rs:41-55): mismatch x,

@@ -152,35 +152,8 @@ struct Arena {
     std::vector<Rec> recs;
 };
 
-int host_threads() {
-    const unsigned h = std::thread::hardware_concurrency();
-    return (int)std::max(1u, std::min(h ? h : 1u, 16u));
-}
-
-// f(t, begin, end) on up to host_threads() threads over [0, n) in chunks of
-// at least `grain`; a std::bad_alloc in any chunk is rethrown here.
-template <class F>
-void parallel_chunks(int64_t n, int64_t grain, F &&f) {
-    if (n <= 0) return;
-    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), (n + grain - 1) / grain));
-    if (T == 1) {
-        f(0, (int64_t)0, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::vector<char> failed(T, 0);
-    for (int t = 0; t < T; t++)
-        th.emplace_back([&, t]() {
-            try {
-                f(t, n * t / T, n * (t + 1) / T);
-            } catch (...) {
-                failed[t] = 1;
-            }
-        });
-    for (auto &x : th) x.join();
-    for (int t = 0; t < T; t++)
-        if (failed[t]) throw std::bad_alloc();
-}
+using kira::host_threads;
+using kira::parallel_chunks;
 
 // a chain sent to the prefilter (and the exact-match check)
 struct Sel {

@@ -706,6 +706,28 @@ int64_t wfa_slot_values(const Pen &pen, int32_t S, int32_t m, int32_t n, int32_t
     return total;
 }
 
+// The history slab stays with the context between calls (the last of its
+// scratch slots; kira_release_buffers frees it), regrown to the need.
+constexpr int kSlabSlot = 15;
+hipError_t slab_for(kira_ctx *ctx, size_t need, int32_t **p) {
+    if (ctx->cap[kSlabSlot] < need) {
+        if (ctx->buf[kSlabSlot]) {
+            hipError_t e = hipFree(ctx->buf[kSlabSlot]);
+            ctx->buf[kSlabSlot] = nullptr;
+            ctx->cap[kSlabSlot] = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&ctx->buf[kSlabSlot], need);
+        if (e != hipSuccess) {
+            ctx->buf[kSlabSlot] = nullptr;
+            return e;
+        }
+        ctx->cap[kSlabSlot] = need;
+    }
+    *p = (int32_t *)ctx->buf[kSlabSlot];
+    return hipSuccess;
+}
+
 bool check_ranges(int32_t n, const int64_t *off, const int32_t *len, int64_t bytes) {
     for (int32_t k = 0; k < n; k++)
         if (off[k] < 0 || len[k] < 0 || off[k] > bytes || len[k] > bytes - off[k]) return false;
@@ -829,8 +851,10 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     val_cap = (val_cap + 3) & ~(int64_t)3;
     const int64_t slot_ints = val_cap + 12 * (int64_t)(s_max + 1);
     const int64_t slot_bytes = 4 * slot_ints;
+    // one slot per resident wave: up to 32 single-wave work-groups on each
+    // of the 256 CUs, within a 4 GiB slab
     const int64_t kSlabBudget = (int64_t)4 << 30;
-    int64_t slots = std::min<int64_t>({(int64_t)n, 2048, kSlabBudget / slot_bytes});
+    int64_t slots = std::min<int64_t>({(int64_t)n, 8192, kSlabBudget / slot_bytes});
     if (slots < 1) {
         if (slot_bytes > ((int64_t)16 << 30))
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA history of one pair above 16 GiB");
@@ -844,7 +868,8 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     hipError_t e;
     if ((e = dm.alloc(&d_res, (size_t)n * kWfaRes)) || (e = dm.alloc(&d_coff, n)) ||
         (e = dm.alloc(&d_ccap, n)) || (e = dm.alloc(&d_cig, cig_total)) ||
-        (e = dm.alloc(&d_slab, (size_t)(slots * slot_ints))) || (e = dm.alloc(&d_queue, 1)))
+        (e = slab_for(ctx, 4 * (size_t)(slots * slot_ints), &d_slab)) ||
+        (e = dm.alloc(&d_queue, 1)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA allocation", e);
     if ((e = hipMemcpyAsync(d_coff, cig_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_ccap, cig_cap.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st)) ||

@@ -23,9 +23,10 @@ def _sub(rng, s: bytes, k: int) -> bytes:
     return bytes(b)
 
 
-def make_fast_batch(seed: int, n: int):
+def make_fast_batch(seed: int, n: int, lens=(20000, 15000, 5000), mix=None):
+    """mix: optional weights over the 12 read kinds (default: round robin)."""
     rng = np.random.default_rng(seed)
-    lens = [20000, 15000, 5000]
+    lens = list(lens)
     contigs = [bytearray(_rand(rng, L)) for L in lens]
     contigs[0][7000:7040] = b"N" * 40
     contigs = [bytes(c) for c in contigs]
@@ -33,10 +34,12 @@ def make_fast_batch(seed: int, n: int):
     contig_off = np.cumsum([0] + lens[:-1]).astype(np.int64)
     contig_len = np.array(lens, dtype=np.int64)
     reads, chains = [], []
+    kinds = (np.arange(n) % 12 if mix is None
+             else rng.choice(12, size=n, p=np.asarray(mix, float) / np.sum(mix)))
     for t in range(n):
-        kind = t % 12
+        kind = int(kinds[t])
         L = int(rng.choice([150, 150, 150, 100, 250, 76]))
-        cid = int(rng.integers(3))
+        cid = int(rng.integers(len(lens)))
         clen = lens[cid]
         pos = int(rng.integers(40, clen - L - 60))
         ref = contigs[cid]
