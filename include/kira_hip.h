@@ -460,6 +460,24 @@ typedef struct {
     int32_t score;
 } kira_chain;
 
+/* The router knobs of the fast path (src/alignment/router.rs:48-215), with
+ * the reference defaults kira_fast_knobs_default fills in brackets:
+ * wfa_max_read_len KIRA_WFA_MAX_LEN [300]; algo KIRA_ALGO 0 packed [0],
+ * 1 spectral, 2 wfa, 3 sw; quality_gate / quality_pct
+ * KIRA_CHAIN_QUALITY_GATE [0] / KIRA_CHAIN_QUALITY_PCT [70]; wfa_budget_pct
+ * KIRA_WFA_BUDGET_PCT [25]; myers_pct / myers_floor KIRA_MYERS_BOUND_PCT [15]
+ * / KIRA_MYERS_BOUND_FLOOR [4]; spectral_pct / spectral_floor
+ * KIRA_SPECTRAL_MISM_PCT [3] / KIRA_SPECTRAL_MISM_FLOOR [2]; wfa_ends_free
+ * KIRA_WFA_ENDS_FREE [0]; wfa_lead KIRA_WFA_LEAD [0]; wfa_adaptive
+ * KIRA_WFA_ADAPTIVE [0 = exact]; cert_strict KIRA_CERT_STRICT [0]. */
+typedef struct {
+    int32_t wfa_max_read_len, algo, quality_gate, quality_pct, wfa_budget_pct;
+    int32_t myers_pct, myers_floor, spectral_pct, spectral_floor;
+    int32_t wfa_ends_free, wfa_lead, wfa_adaptive, cert_strict;
+} kira_fast_knobs;
+
+void kira_fast_knobs_default(kira_fast_knobs *k);
+
 /* AlignmentStageConfig (src/pipeline/stage4_alignment.rs:33-54) and the
  * environment knobs stage 4 reads: tail_ext KIRA_UNGAP_TAIL (1),
  * ungap_min_id KIRA_UNGAP_MINID (-1 unset), short_dp_topk KIRA_SHORT_DPTOPK
@@ -480,6 +498,12 @@ typedef struct {
     int32_t spectral_pct, spectral_floor, cert_strict;
     kira_simd_mode simd;
     int32_t supplementary;
+    /* cpu_mode: the configuration the reference runs without --gpu (the
+     * dispatcher inactive, stage4_alignment.rs:868-882, 1073-1106): chains
+     * that fall back from the prefilter try the fast DP cascade (router knobs
+     * `fast`) before SW, and lane jobs skip the sieve dispatch. */
+    int32_t cpu_mode;
+    kira_fast_knobs fast;
 } kira_stage4_cfg;
 
 /* Alignment (src/types.rs) as stage 4 leaves it: kind 0 AcceptedUngapped,
@@ -504,6 +528,7 @@ typedef struct {
     int64_t reason_counts[10];
     int64_t gpu_spectral_resolved, dp_simd, dp_scalar, dp_early_abort, dp_reads;
     int64_t supplementary, unmapped;
+    int64_t packed_spectral_resolved, wfa_resolved; /* cpu_mode fast path */
 } kira_stage4_stats;
 
 typedef struct kira_stage4_result kira_stage4_result;
@@ -535,7 +560,9 @@ typedef struct {
  * sieve with certification (:1272-1358), the rest to align_batch_simd lane
  * groups by (read, window) shape; longer reads go to banded SW on
  * clamp_window windows (mod.rs:851-877).  Then the chain-score XS proxy
- * (:1215-1250) and supplementary alignments.  The genome holds every contig
+ * (:1215-1250) and supplementary alignments.  With cfg->cpu_mode the
+ * fallback chains go through the fast DP cascade first (as
+ * kira_fast_dp_batch) and the sieve dispatch is skipped.  The genome holds every contig
  * back to back: contig i is bases contig_off[i] .. +contig_len[i].  Read k
  * is reads[read_off[k] .. +read_len[k]] (forward strand); its chains are
  * chains[chain_off[k] .. chain_off[k+1]); repeat_min_occ may be NULL (0).
@@ -662,24 +689,6 @@ kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *patterns,
                            uint32_t *cigar_arena, int64_t arena_cap);
 
 /* ------------------------------------------------------------------ fast DP cascade */
-
-/* The router knobs of the fast path (src/alignment/router.rs:48-215), with
- * the reference defaults kira_fast_knobs_default fills in brackets:
- * wfa_max_read_len KIRA_WFA_MAX_LEN [300]; algo KIRA_ALGO 0 packed [0],
- * 1 spectral, 2 wfa, 3 sw; quality_gate / quality_pct
- * KIRA_CHAIN_QUALITY_GATE [0] / KIRA_CHAIN_QUALITY_PCT [70]; wfa_budget_pct
- * KIRA_WFA_BUDGET_PCT [25]; myers_pct / myers_floor KIRA_MYERS_BOUND_PCT [15]
- * / KIRA_MYERS_BOUND_FLOOR [4]; spectral_pct / spectral_floor
- * KIRA_SPECTRAL_MISM_PCT [3] / KIRA_SPECTRAL_MISM_FLOOR [2]; wfa_ends_free
- * KIRA_WFA_ENDS_FREE [0]; wfa_lead KIRA_WFA_LEAD [0]; wfa_adaptive
- * KIRA_WFA_ADAPTIVE [0 = exact]; cert_strict KIRA_CERT_STRICT [0]. */
-typedef struct {
-    int32_t wfa_max_read_len, algo, quality_gate, quality_pct, wfa_budget_pct;
-    int32_t myers_pct, myers_floor, spectral_pct, spectral_floor;
-    int32_t wfa_ends_free, wfa_lead, wfa_adaptive, cert_strict;
-} kira_fast_knobs;
-
-void kira_fast_knobs_default(kira_fast_knobs *k);
 
 /* The Alignment try_fast_dp_alignment returns (mod.rs:636-686, 481-575):
  * kind 0 none (the caller falls back to SW), 1 packed spectral certificate,

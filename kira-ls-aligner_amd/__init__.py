@@ -217,7 +217,7 @@ class Stage4Cfg(C.Structure):
             "accept_max_mismatches", "accept_require_score_margin", "dp_topk", "dp_abort_margin",
             "long_read_threshold", "max_alignments", "tail_ext", "ungap_min_id", "short_dp_topk",
             "ambig_div", "xs_min_ratio_pct", "spectral_pct", "spectral_floor", "cert_strict",
-            "simd", "supplementary")]
+            "simd", "supplementary", "cpu_mode")] + [("fast", FastKnobs)]
 
 
 class Stage4Aln(C.Structure):
@@ -232,7 +232,8 @@ class Stage4Aln(C.Structure):
 STAGE4_STATS = ["reads", "chains_total", "chains_used", "exact_matches", "prefilter_accept",
                 "prefilter_reject", "prefilter_fallback"]
 STAGE4_STATS_TAIL = ["gpu_spectral_resolved", "dp_simd", "dp_scalar", "dp_early_abort",
-                     "dp_reads", "supplementary", "unmapped"]
+                     "dp_reads", "supplementary", "unmapped", "packed_spectral_resolved",
+                     "wfa_resolved"]
 
 
 class Stage4Stats(C.Structure):
@@ -1046,15 +1047,21 @@ def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth
                dp_topk=1, dp_abort_margin=20, long_read_threshold=500, max_alignments=1,
                tail_ext=True, ungap_min_id=-1, short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0,
                spectral_pct=3, spectral_floor=2, cert_strict=False, simd="avx2",
-               supplementary=True) -> Stage4Cfg:
-    """Stage-4 configuration; defaults are `mem`'s (src/cli/commands/mem.rs:183-205)."""
+               supplementary=True, cpu_mode=False, fast=None) -> Stage4Cfg:
+    """Stage-4 configuration; defaults are `mem`'s (src/cli/commands/mem.rs:183-205).
+    cpu_mode: the reference's configuration without --gpu (fallback chains try
+    the fast DP cascade first); fast: dict of router knob overrides."""
+    fk = FastKnobs(300, 0, 0, 70, 25, 15, 4, 3, 2, 0, 0, 0, 0)  # router.rs defaults
+    for name, v in (fast or {}).items():
+        setattr(fk, name, FAST_ALGOS[v] if name == "algo" and isinstance(v, str) else int(v))
     return Stage4Cfg(
         SwCfg(match_score, mismatch, gap_open, gap_extend, bandwidth, xdrop, clip_penalty),
         SwKnobs(int(adaptive), max_drift, dp_xdrop), min_chain_ratio, int(accept_enable),
         int(accept_only_top1), accept_span_slack, accept_min_identity, accept_max_mismatches,
         accept_require_score_margin, dp_topk, dp_abort_margin, long_read_threshold,
         max_alignments, int(tail_ext), ungap_min_id, short_dp_topk, ambig_div, xs_min_ratio_pct,
-        spectral_pct, spectral_floor, int(cert_strict), SIMD_MODES[simd], int(supplementary))
+        spectral_pct, spectral_floor, int(cert_strict), SIMD_MODES[simd], int(supplementary),
+        int(cpu_mode), fk)
 
 
 class Stage4Result:

@@ -245,41 +245,24 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 }  // namespace
 
-extern "C" kira_status kira_fast_dp_batch(
-    kira_ctx *ctx, const kira_genome *g, int32_t n_contigs, const int64_t *contig_off,
-    const int64_t *contig_len, int32_t n, const uint8_t *reads, int64_t reads_bytes,
-    const int64_t *read_off, const int32_t *read_len, const kira_chain *chains,
-    const kira_sw_cfg *cfg, const kira_fast_knobs *knobs_in, kira_fast_aln *out,
-    uint32_t *cigar_out, int64_t cigar_cap, char *md_out, int64_t md_cap, int64_t *cigar_used,
-    int64_t *md_used, kira_fast_stats *stats) {
-    if (!ctx || !g || !cfg || n < 0 || n_contigs < 0 || reads_bytes < 0)
-        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
-    kira_fast_knobs kn;
-    kira_fast_knobs_default(&kn);
-    if (knobs_in) kn = *knobs_in;
-    kira_fast_stats st_local;
+namespace kira {
+
+kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *contig_off,
+                        const int64_t *contig_len, int32_t n, const uint8_t *reads,
+                        int64_t reads_bytes, const uint8_t *d_reads_in, const int64_t *read_off,
+                        const int32_t *read_len, const kira_chain *chains,
+                        const kira_sw_cfg *cfg, const kira_fast_knobs &kn,
+                        std::vector<kira_fast_aln> &res_out, std::vector<uint32_t> &cig_out,
+                        std::string &md_out, kira_fast_stats &st_local) {
     memset(&st_local, 0, sizeof st_local);
-    if (cigar_used) *cigar_used = 0;
-    if (md_used) *md_used = 0;
-    if (n == 0) {
-        if (stats) *stats = st_local;
-        return KIRA_OK;
+    try {
+        res_out.assign(n, kira_fast_aln{});
+        cig_out.clear();
+        md_out.clear();
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
-    if (!read_off || !read_len || !chains || !out || (reads_bytes && !reads) ||
-        (n_contigs && (!contig_off || !contig_len)))
-        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
-    const int64_t glen = (int64_t)g->host.size();
-    for (int32_t c = 0; c < n_contigs; c++)
-        if (contig_off[c] < 0 || contig_len[c] < 0 || contig_off[c] > glen ||
-            contig_len[c] > glen - contig_off[c])
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "contig outside the genome");
-    for (int32_t k = 0; k < n; k++) {
-        if (read_off[k] < 0 || read_len[k] < 0 || read_off[k] > reads_bytes ||
-            read_len[k] > reads_bytes - read_off[k])
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "read outside its buffer");
-        if (chains[k].ref_id >= (uint32_t)n_contigs)
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain contig out of range");
-    }
+    if (n == 0) return KIRA_OK;
     const kira_sw_cfg c = *cfg;
     const int32_t band = std::max(c.bandwidth, 1);
     kira::WfaPen pen;
@@ -287,11 +270,10 @@ extern "C" kira_status kira_fast_dp_batch(
     const uint8_t *gh = g->host.data();
     // ---- router gates and windows (host)
     std::vector<Job> jobs;
-    std::vector<kira_fast_aln> res(n);
+    std::vector<kira_fast_aln> &res = res_out;
     try {
         jobs.reserve(n);
         for (int32_t k = 0; k < n; k++) {
-            res[k] = kira_fast_aln{};
             const int32_t L = read_len[k];
             if (L == 0 || L > kn.wfa_max_read_len || kn.algo == 3) continue;
             if (kn.quality_gate) {  // fast_path_worth_attempting (router.rs:169-189)
@@ -322,12 +304,15 @@ extern "C" kira_status kira_fast_dp_batch(
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
     hipStream_t st = ctx->stream;
     DevMem dm;
-    uint8_t *d_reads = nullptr;
-    if ((e = dm.alloc(&d_reads, reads_bytes)))
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path allocation", e);
-    if (reads_bytes &&
-        (e = hipMemcpyAsync(d_reads, reads, reads_bytes, hipMemcpyHostToDevice, st)))
-        return fail(ctx, KIRA_ERR_DRIVER, "fast path upload", e);
+    const uint8_t *d_reads = d_reads_in;
+    if (!d_reads) {
+        uint8_t *d = nullptr;
+        if ((e = dm.alloc(&d, reads_bytes)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path allocation", e);
+        if (reads_bytes && (e = hipMemcpyAsync(d, reads, reads_bytes, hipMemcpyHostToDevice, st)))
+            return fail(ctx, KIRA_ERR_DRIVER, "fast path upload", e);
+        d_reads = d;
+    }
     // uploads a list of jobs' (read offset, read len, text offset, text len,
     // extra) arrays
     struct Lists {
@@ -688,25 +673,72 @@ extern "C" kira_status kira_fast_dp_batch(
                 a.cigar_off += cig_base[t];
                 a.md_off += md_base[t];
             }
-        if (cigar_used) *cigar_used = cig_total;
-        if (md_used) *md_used = md_total;
-        if (cig_total > cigar_cap || md_total > md_cap || (cig_total && !cigar_out) ||
-            (md_total && !md_out)) {
-            if (stats) *stats = st_local;
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "CIGAR or MD buffer too small");
-        }
+        cig_out.reserve(cig_total);
+        md_out.reserve(md_total);
         for (int t = 0; t < T; t++) {
-            if (!parts[t].cig.empty())
-                memcpy(cigar_out + cig_base[t], parts[t].cig.data(), 4 * parts[t].cig.size());
-            if (!parts[t].md.empty())
-                memcpy(md_out + md_base[t], parts[t].md.data(), parts[t].md.size());
+            cig_out.insert(cig_out.end(), parts[t].cig.begin(), parts[t].cig.end());
+            md_out += parts[t].md;
         }
-        memcpy(out, res.data(), sizeof(kira_fast_aln) * (size_t)n);
         st_local.phase_ms[3] = ms_since(t_rec);
-        if (stats) *stats = st_local;
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    return KIRA_OK;
+}
+
+}  // namespace kira
+
+extern "C" kira_status kira_fast_dp_batch(
+    kira_ctx *ctx, const kira_genome *g, int32_t n_contigs, const int64_t *contig_off,
+    const int64_t *contig_len, int32_t n, const uint8_t *reads, int64_t reads_bytes,
+    const int64_t *read_off, const int32_t *read_len, const kira_chain *chains,
+    const kira_sw_cfg *cfg, const kira_fast_knobs *knobs_in, kira_fast_aln *out,
+    uint32_t *cigar_out, int64_t cigar_cap, char *md_out, int64_t md_cap, int64_t *cigar_used,
+    int64_t *md_used, kira_fast_stats *stats) {
+    if (!ctx || !g || !cfg || n < 0 || n_contigs < 0 || reads_bytes < 0)
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
+    kira_fast_knobs kn;
+    kira_fast_knobs_default(&kn);
+    if (knobs_in) kn = *knobs_in;
+    kira_fast_stats st;
+    memset(&st, 0, sizeof st);
+    if (cigar_used) *cigar_used = 0;
+    if (md_used) *md_used = 0;
+    if (n == 0) {
+        if (stats) *stats = st;
+        return KIRA_OK;
+    }
+    if (!read_off || !read_len || !chains || !out || (reads_bytes && !reads) ||
+        (n_contigs && (!contig_off || !contig_len)))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
+    const int64_t glen = (int64_t)g->host.size();
+    for (int32_t c = 0; c < n_contigs; c++)
+        if (contig_off[c] < 0 || contig_len[c] < 0 || contig_off[c] > glen ||
+            contig_len[c] > glen - contig_off[c])
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "contig outside the genome");
+    for (int32_t k = 0; k < n; k++) {
+        if (read_off[k] < 0 || read_len[k] < 0 || read_off[k] > reads_bytes ||
+            read_len[k] > reads_bytes - read_off[k])
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "read outside its buffer");
+        if (chains[k].ref_id >= (uint32_t)n_contigs)
+            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain contig out of range");
+    }
+    std::vector<kira_fast_aln> res;
+    std::vector<uint32_t> cig;
+    std::string md;
+    kira_status rs = kira::fast_dp_run(ctx, g, contig_off, contig_len, n, reads, reads_bytes,
+                                       nullptr, read_off, read_len, chains, cfg, kn, res, cig, md,
+                                       st);
+    if (rs != KIRA_OK) return rs;
+    if (cigar_used) *cigar_used = (int64_t)cig.size();
+    if (md_used) *md_used = (int64_t)md.size();
+    if (stats) *stats = st;
+    if ((int64_t)cig.size() > cigar_cap || (int64_t)md.size() > md_cap ||
+        (!cig.empty() && !cigar_out) || (!md.empty() && !md_out))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "CIGAR or MD buffer too small");
+    if (!cig.empty()) memcpy(cigar_out, cig.data(), 4 * cig.size());
+    if (!md.empty()) memcpy(md_out, md.data(), md.size());
+    memcpy(out, res.data(), sizeof(kira_fast_aln) * (size_t)n);
     return KIRA_OK;
 }
 

@@ -501,26 +501,25 @@ kira_status Driver::run() {
         for (int32_t k = 0; k < ne; k++) exact[ex_sel[k]] = ex[k];
     }
 
-    // outcomes in chain order (:758-934)
+    // outcomes in chain order (:758-934): 0 nothing, 1 a record here, 2 fallback
     std::vector<Job> simd, scalar;
-    std::vector<uint8_t> dp_used(n, 0);
+    std::vector<uint8_t> dp_used(n, 0), outcome(ns, 0);
+    std::vector<int32_t> abort_of(ns, 0);
     std::vector<int32_t> accepted;  // selected chains that end in a record here
     const int32_t band = std::max(c.cfg.bandwidth, 1);
     for (int32_t k = 0; k < ns; k++) {
         const Sel &x = sels[k];
-        const kira_chain &ch = chains[x.chain];
-        const int32_t L = read_len[x.read];
         if (exact[k]) {
             st.exact_matches++;
-            accepted.push_back(k);
+            outcome[k] = 1;
             continue;
         }
         const kira_prefilter_out &o = pf[k];
         st.reason_counts[o.reason < 9 ? o.reason : 9]++;
-        const int32_t abort = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
+        abort_of[k] = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
         if (o.result == 0) {
             st.prefilter_accept++;
-            accepted.push_back(k);
+            outcome[k] = 1;
             continue;
         }
         if (o.result == 1) {
@@ -529,6 +528,52 @@ kira_status Driver::run() {
         }
         st.prefilter_fallback++;
         dp_used[x.read] = 1;
+        outcome[k] = 2;
+    }
+    // without the GPU dispatcher, a fallback chain tries the fast DP cascade
+    // first (try_fast_dp_alignment, :868-882; accept_fast_result, :289-299)
+    std::vector<int32_t> fast_of(ns, -1);
+    std::vector<kira_fast_aln> fres;
+    std::vector<uint32_t> fcig;
+    std::string fmd;
+    if (c.cpu_mode) {
+        std::vector<int32_t> fk, frl;
+        std::vector<int64_t> fro;
+        std::vector<kira_chain> fch;
+        for (int32_t k = 0; k < ns; k++) {
+            if (outcome[k] != 2) continue;
+            const Sel &x = sels[k];
+            const kira_chain &ch = chains[x.chain];
+            fk.push_back(k);
+            fro.push_back(oriented_off(x.read, ch.strand == 1));
+            frl.push_back(read_len[x.read]);
+            fch.push_back(ch);
+        }
+        kira_fast_stats fst;
+        kira_status fs = kira::fast_dp_run(ctx, g, contig_off, contig_len, (int32_t)fk.size(),
+                                           orient.data(), (int64_t)orient.size(), d_orient,
+                                           fro.data(), frl.data(), fch.data(), &c.cfg, c.fast,
+                                           fres, fcig, fmd, fst);
+        if (fs != KIRA_OK) return fs;
+        for (size_t i = 0; i < fk.size(); i++) {
+            if (fres[i].kind == 0) continue;
+            fast_of[fk[i]] = (int32_t)i;
+            if (fres[i].kind == 1)
+                st.packed_spectral_resolved++;
+            else
+                st.wfa_resolved++;
+        }
+    }
+    for (int32_t k = 0; k < ns; k++) {
+        if (outcome[k] == 0) continue;
+        if (outcome[k] == 1 || fast_of[k] >= 0) {
+            accepted.push_back(k);
+            continue;
+        }
+        const Sel &x = sels[k];
+        const kira_chain &ch = chains[x.chain];
+        const int32_t L = read_len[x.read];
+        const int32_t abort = abort_of[k];
         const int bucket = L <= 120 ? 0 : (L <= 150 ? 1 : 2);
         if (bucket < 2 && lanes > 1) {
             // build_simd_window (:1426-1445)
@@ -550,6 +595,27 @@ kira_status Driver::run() {
         const kira_chain &ch = chains[x.chain];
         if (exact[k]) {
             A.recs.push_back(Rec{x.read, exact_aln(x.read, ch, A)});
+            return;
+        }
+        if (fast_of[k] >= 0) {
+            // the fast path's Alignment (mod.rs:636-686, 481-575)
+            const kira_fast_aln &f = fres[fast_of[k]];
+            kira_stage4_aln a = base_aln(1, ch);
+            a.ref_start = f.ref_start;
+            a.ref_end = f.ref_end;
+            a.read_start = f.read_start;
+            a.read_end = f.read_end;
+            a.score = f.score;
+            a.mapq = 0;
+            a.nm = f.nm;
+            a.cigar_off = (int64_t)A.cig.size();
+            a.n_cigar = f.n_cigar;
+            A.cig.insert(A.cig.end(), fcig.begin() + f.cigar_off,
+                         fcig.begin() + f.cigar_off + f.n_cigar);
+            a.md_off = (int64_t)A.md.size();
+            A.md.append(fmd, (size_t)f.md_off, (size_t)f.md_len);
+            a.md_len = f.md_len;
+            A.recs.push_back(Rec{x.read, a});
             return;
         }
         // build_ungapped_alignment (prefilter.rs:455-517)
@@ -577,7 +643,7 @@ kira_status Driver::run() {
     t0 = Clock::now();
     std::vector<uint8_t> keep(simd.size(), 1);
     const int32_t nj = (int32_t)simd.size();
-    if (nj > 0) {
+    if (nj > 0 && !c.cpu_mode) {
         std::vector<int64_t> ro(nj), ws(nj);
         std::vector<int32_t> rl(nj), wl(nj), lim(nj);
         int32_t max_len = 0;

@@ -2,9 +2,10 @@
 
 Follows src/pipeline/stage4_alignment.rs run() (:959-1268) and
 process_read_prefilter (:535-957) in the configuration the GPU dispatcher is
-active (`--gpu`), single-end, with KIRA_TWOTIER=0 (the two-tier reorder,
-:346-451, needs the CPU WFA/Myers fast path, which is not restated) and
-KIRA_TWOTIER_MAPQ unset (its default).  Supplementary alignments follow
+active (`--gpu`), or with cpu_mode the one without it (fallback chains try
+the fast DP cascade of oracle/fastpath.py before SW; no sieve dispatch);
+single-end, with KIRA_TWOTIER=0 (the two-tier reorder, :346-451, is not
+restated) and KIRA_TWOTIER_MAPQ unset (its default).  Supplementary alignments follow
 src/pipeline/split_read.rs:22-151.  Every numeric step goes through the C
 oracle (oracle.py): prefilter_chain, the 2-bit sieve + certification,
 align_batch_simd's SW, banded SW and build_alignment.
@@ -16,6 +17,7 @@ tests/test_oracle_stage4.py, and the pinned building blocks above.
 """
 import math
 
+from oracle import fastpath as F
 from oracle import oracle as O
 
 I32_MIN = -(2 ** 31)
@@ -39,7 +41,8 @@ def default_cfg(**kw):
              accept_require_score_margin=0, dp_topk=1, dp_abort_margin=20,
              long_read_threshold=500, max_alignments=1, tail_ext=True, ungap_min_id=-1,
              short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0, spectral_pct=3,
-             spectral_floor=2, cert_strict=False, simd="avx2", supplementary=True)
+             spectral_floor=2, cert_strict=False, simd="avx2", supplementary=True,
+             cpu_mode=False, fast={})
     c.update(kw)
     return c
 
@@ -232,9 +235,10 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
     st = dict(reads=n, chains_total=0, chains_used=0, exact_matches=0, prefilter_accept=0,
               prefilter_reject=0, prefilter_fallback=0, reason_counts=[0] * 10,
               gpu_spectral_resolved=0, dp_simd=0, dp_scalar=0, dp_early_abort=0, dp_reads=0,
-              supplementary=0, unmapped=0)
+              supplementary=0, unmapped=0, packed_spectral_resolved=0, wfa_resolved=0)
     alns = [[] for _ in range(n)]
     simd_jobs, scalar_jobs = [], []
+    router = F.Router(**cfg["fast"])
     cb, cs = [0] * n, [0] * n
     for r in range(n):
         read_fwd = reads[r]
@@ -281,6 +285,17 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
                 continue
             st["prefilter_fallback"] += 1
             dp_used = True
+            if cfg["cpu_mode"]:
+                # without the GPU dispatcher: try_fast_dp_alignment first
+                # (:868-882), accept_fast_result (:289-299)
+                fr = F.fast_dp(read, contig, ch[1], ch[3], ch[6], c, router)
+                if fr is not None:
+                    kind, a = fr
+                    st["packed_spectral_resolved" if kind == "packed" else "wfa_resolved"] += 1
+                    alns[r].append(_record("dp_aligned", ref_id, a["ref_start"], a["ref_end"], 0,
+                                           len(read), O.cigar_str(a["cigar"]), a["score"], 0,
+                                           is_rev, a["nm"], a["md"]))
+                    continue
             if bucket < 2 and lanes > 1:
                 w = O.build_simd_window(len(contig), ch[1], ch[3], L, cfg["bandwidth"])
                 if w is not None:
@@ -291,9 +306,12 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
             scalar_jobs.append((r, read, is_rev, ref_id, span, abort))
         st["dp_reads"] += int(dp_used)
 
-    # gpu_dispatch_spectral (:1272-1358)
+    # gpu_dispatch_spectral (:1272-1358), only with the dispatcher active
     kept = []
     for j in simd_jobs:
+        if cfg["cpu_mode"]:
+            kept.append(j)
+            continue
         read, win = j["read"], j["window"]
         if not (O.pack(read)[1] and O.pack(win)[1]):
             kept.append(j)

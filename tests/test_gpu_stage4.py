@@ -57,6 +57,22 @@ def test_records_and_counters_match_oracle(pkg, ctx, cfg):
             assert gst[k] > 0, k
 
 
+@pytest.mark.parametrize("cfg", [dict(cpu_mode=True), dict(cpu_mode=True, simd="scalar"),
+                                 dict(cpu_mode=True, fast=dict(wfa_lead=6, wfa_adaptive=10)),
+                                 dict(cpu_mode=True, fast=dict(algo="wfa", cert_strict=1))])
+def test_cpu_mode_fast_path_matches_oracle(pkg, ctx, cfg):
+    """The configuration without the GPU dispatcher: fallback chains try the
+    fast DP cascade before SW (stage4_alignment.rs:868-882)."""
+    b = make_batch(14, 2500)
+    got, gst, want, wst = run_both(pkg, ctx, b, **cfg)
+    for r in range(len(want)):
+        assert got[r] == want[r], (r, b["chains"][r])
+    assert gst == wst
+    assert gst["gpu_spectral_resolved"] == 0 and gst["wfa_resolved"] > 0
+    if cfg.get("fast", {}).get("algo") != "wfa":
+        assert gst["packed_spectral_resolved"] > 0
+
+
 def test_empty_batch_and_reads_without_chains(pkg, ctx):
     b = make_batch(12, 20)
     genome = pkg.Genome(ctx, b["genome"])

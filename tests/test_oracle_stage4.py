@@ -166,3 +166,43 @@ def test_supplementary_alignment_of_a_chimeric_read():
     assert [a["is_supplementary"] for a in alns[0]] == [False, True]
     sup = alns[0][1]
     assert sup["ref_start"] == 20000 and sup["cigar"] == "80S70M"
+
+
+def test_cpu_mode_resolves_fallbacks_on_the_fast_path():
+    """Without the GPU dispatcher (stage4_alignment.rs:868-882) the deletion read
+    that goes to lane SW above is resolved by WFA instead: same CIGAR, the
+    full-query DP score, no SW job; a read the cascade declines (foreign tail)
+    still reaches SW, after the records of the chains resolved in order."""
+    g = _genome(20000, 5)
+    src = g[5000:5150]
+    read = src[:100] + src[103:]
+    ch = [[(0, 5000, 5100, 0, 100, 0, 100)]]
+    alns, st = S.run([g], [read], ch, S.default_cfg(cpu_mode=True))
+    a, = alns[0]
+    assert st["wfa_resolved"] == 1 and st["dp_simd"] == 0 and st["dp_scalar"] == 0
+    assert st["prefilter_fallback"] == 1 and st["dp_reads"] == 1
+    # src[100] == src[103] == G: the deletion may sit after base 100 or 101;
+    # WFA's traceback takes the later place, SW's the earlier one
+    assert src[100] == src[103] and src[101] != src[104]
+    assert (a["kind"], a["cigar"], a["ref_start"], a["ref_end"], a["nm"], a["mapq"]) == \
+        ("dp_aligned", "101M3D46M", 5000, 5150, 3, 0)
+    assert a["score"] == 147 - (6 + 3)
+    # a read with a foreign 40-base tail: WFA's loss is above 30% and clipping
+    # improves it, so the chain falls through to lane SW (which soft-clips)
+    tail = _genome(40, 9)
+    partial = g[9000:9110] + tail
+    alns, st = S.run([g], [partial], [[(0, 9000, 9100, 0, 100, 0, 100)]],
+                     S.default_cfg(cpu_mode=True))
+    assert st["wfa_resolved"] == 0 and st["dp_simd"] == 1
+    assert alns[0][0]["cigar"].endswith("S")
+    # the sieve dispatch does not run in cpu mode: a substitution-only read is
+    # certified by the fast path's packed scan instead
+    sub = bytearray(src)
+    sub[60] = ord("A") if sub[60] != ord("A") else ord("C")
+    sub[100] = ord("A") if sub[100] != ord("A") else ord("C")
+    sub[140] = ord("A") if sub[140] != ord("A") else ord("C")
+    ch3 = [[(0, 5000, 5050, 0, 50, 0, 50)]]
+    _, st_gpu = S.run([g], [bytes(sub)], ch3)
+    _, st_cpu = S.run([g], [bytes(sub)], ch3, S.default_cfg(cpu_mode=True))
+    assert st_cpu["gpu_spectral_resolved"] == 0
+    assert st_gpu["prefilter_fallback"] == st_cpu["prefilter_fallback"] == 1
