@@ -504,6 +504,12 @@ typedef struct {
      * `fast`) before SW, and lane jobs skip the sieve dispatch. */
     int32_t cpu_mode;
     kira_fast_knobs fast;
+    /* twotier KIRA_TWOTIER (1) / twotier_k KIRA_TWOTIER_K (5): the candidate
+     * reorder of uncertain short reads (two_tier_reorder,
+     * stage4_alignment.rs:346-451, 596-683), which scores up to twotier_k
+     * passing chains with the fast DP cascade, orders them by that score,
+     * keeps their alignments, sets the DP budget and the competitor XS. */
+    int32_t twotier, twotier_k;
 } kira_stage4_cfg;
 
 /* Alignment (src/types.rs) as stage 4 leaves it: kind 0 AcceptedUngapped,
@@ -552,15 +558,17 @@ typedef struct {
 /*
  * stage4_alignment::run (src/pipeline/stage4_alignment.rs:959-1268, with
  * process_read_prefilter :535-957) with the GPU dispatcher active, for
- * single-end reads and KIRA_TWOTIER=0 (the two-tier candidate reorder,
- * :346-451, runs the CPU WFA/Myers fast path and is not supported).  Per
- * read: chain selection (min_chain_ratio, top-k), exact-match shortcut,
+ * single-end reads.  Per read: chain selection (min_chain_ratio, top-k, and
+ * with cfg->twotier the candidate reorder of uncertain short reads, whose
+ * verified fast-path alignments are kept and whose runner-up score becomes
+ * XS), exact-match shortcut,
  * ungapped prefilter; fallback chains of reads of <= 150 bases become lane
  * jobs on build_simd_window windows (:1426-1445), dispatched to the 2-bit
  * sieve with certification (:1272-1358), the rest to align_batch_simd lane
  * groups by (read, window) shape; longer reads go to banded SW on
- * clamp_window windows (mod.rs:851-877).  Then the chain-score XS proxy
- * (:1215-1250) and supplementary alignments.  With cfg->cpu_mode the
+ * clamp_window windows (mod.rs:851-877).  Then XS (the two-tier
+ * competitor, else the chain-score proxy, :1215-1250) and supplementary
+ * alignments.  With cfg->cpu_mode the
  * fallback chains go through the fast DP cascade first (as
  * kira_fast_dp_batch) and the sieve dispatch is skipped.  The genome holds every contig
  * back to back: contig i is bases contig_off[i] .. +contig_len[i].  Read k

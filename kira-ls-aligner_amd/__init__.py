@@ -217,7 +217,8 @@ class Stage4Cfg(C.Structure):
             "accept_max_mismatches", "accept_require_score_margin", "dp_topk", "dp_abort_margin",
             "long_read_threshold", "max_alignments", "tail_ext", "ungap_min_id", "short_dp_topk",
             "ambig_div", "xs_min_ratio_pct", "spectral_pct", "spectral_floor", "cert_strict",
-            "simd", "supplementary", "cpu_mode")] + [("fast", FastKnobs)]
+            "simd", "supplementary", "cpu_mode")] + [("fast", FastKnobs), ("twotier", C.c_int32),
+                                                     ("twotier_k", C.c_int32)]
 
 
 class Stage4Aln(C.Structure):
@@ -1047,10 +1048,12 @@ def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth
                dp_topk=1, dp_abort_margin=20, long_read_threshold=500, max_alignments=1,
                tail_ext=True, ungap_min_id=-1, short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0,
                spectral_pct=3, spectral_floor=2, cert_strict=False, simd="avx2",
-               supplementary=True, cpu_mode=False, fast=None) -> Stage4Cfg:
+               supplementary=True, cpu_mode=False, fast=None, twotier=True,
+               twotier_k=5) -> Stage4Cfg:
     """Stage-4 configuration; defaults are `mem`'s (src/cli/commands/mem.rs:183-205).
     cpu_mode: the reference's configuration without --gpu (fallback chains try
-    the fast DP cascade first); fast: dict of router knob overrides."""
+    the fast DP cascade first); fast: dict of router knob overrides; twotier /
+    twotier_k: KIRA_TWOTIER (on) / KIRA_TWOTIER_K (5)."""
     fk = FastKnobs(300, 0, 0, 70, 25, 15, 4, 3, 2, 0, 0, 0, 0)  # router.rs defaults
     for name, v in (fast or {}).items():
         setattr(fk, name, FAST_ALGOS[v] if name == "algo" and isinstance(v, str) else int(v))
@@ -1061,7 +1064,7 @@ def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth
         accept_require_score_margin, dp_topk, dp_abort_margin, long_read_threshold,
         max_alignments, int(tail_ext), ungap_min_id, short_dp_topk, ambig_div, xs_min_ratio_pct,
         spectral_pct, spectral_floor, int(cert_strict), SIMD_MODES[simd], int(supplementary),
-        int(cpu_mode), fk)
+        int(cpu_mode), fk, int(twotier), twotier_k)
 
 
 class Stage4Result:

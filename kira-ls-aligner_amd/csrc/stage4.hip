@@ -100,6 +100,10 @@ int32_t sat_sub(int32_t a, int32_t b) {
     return (int32_t)std::min<int64_t>(INT32_MAX, std::max<int64_t>(INT32_MIN, v));
 }
 int32_t wrap_i32(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
+int32_t sat_mul(int32_t a, int32_t b) {
+    int64_t v = (int64_t)a * b;
+    return (int32_t)std::min<int64_t>(INT32_MAX, std::max<int64_t>(INT32_MIN, v));
+}
 // `x as i32` of a float (saturating, NaN -> 0)
 int32_t f2i(float x) {
     if (x != x) return 0;
@@ -161,6 +165,7 @@ struct Sel {
     int64_t chain;
     uint8_t top1;
     int32_t margin;
+    int32_t cached;  // the two-tier search's verified alignment (index), or -1
 };
 
 // an SW job: oriented read, window [win, win + wlen) in contig coordinates
@@ -372,33 +377,165 @@ kira_status Driver::run() {
         }
     });
 
-    // chain selection (:549-708, two-tier and mate guidance off)
+    // chain selection (:549-708, mate guidance off), in two passes: the
+    // two-tier candidate search of uncertain short reads (:596-627) runs its
+    // fast cascades for the whole batch at once, then each read selects
     std::vector<Sel> sels;
-    std::vector<int32_t> chain_best(n, 0), chain_second(n, 0);
+    std::vector<int32_t> chain_best(n, 0), chain_second(n, 0), min_score_of(n, 0),
+        topk_of(n, 1), margin_of(n, INT32_MAX);
+    std::vector<uint8_t> has_comp(n, 0);
+    std::vector<int32_t> comp_of(n, 0);
+    // two-tier candidates: (read, chain) in read order, passing order
+    struct Cand {
+        int32_t read;
+        int64_t chain;
+        int32_t score;   // INT32_MIN: the cascade found nothing
+        int32_t fast;    // index into tt_res, or -1
+    };
+    std::vector<Cand> cand;
+    std::vector<int64_t> cand_off(n + 1, 0);
     for (int32_t r = 0; r < n; r++) {
+        cand_off[r] = (int64_t)cand.size();
         const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
         st.chains_total += nc;
         if (nc == 0) continue;
         const kira_chain *cl = chains + c0;
         const int32_t best = cl[0].score;
-        const int32_t min_score = best > 0 ? f2i(ceilf((float)best * c.min_chain_ratio)) : best;
+        int32_t min_score = best > 0 ? f2i(ceilf((float)best * c.min_chain_ratio)) : best;
         const int32_t second = nc > 1 ? cl[1].score : 0;
         const int32_t margin = nc > 1 ? wrap_i32((int64_t)best - second) : INT32_MAX;
         chain_best[r] = best;
         chain_second[r] = second;
-        const bool short_read = read_len[r] <= 300;
+        margin_of[r] = margin;
+        const int32_t L = read_len[r];
+        const bool short_read = L <= 300;
         const bool ambiguous =
             nc > 1 && (margin <= std::max(best / ambig_div, 8) || rmo_of(r) > 1);
-        const int32_t topk = short_read
-                                 ? (c.short_dp_topk >= 1 ? c.short_dp_topk : (ambiguous ? 2 : 1))
-                                 : std::max(c.dp_topk, 1);
+        const bool weak = nc > 1 && best < wrap_i32((int64_t)L * 2) / 3;
+        if (short_read && weak && c.twotier) min_score = std::min(min_score, best / 5);
+        min_score_of[r] = min_score;
+        topk_of[r] = short_read ? (c.short_dp_topk >= 1 ? c.short_dp_topk : (ambiguous ? 2 : 1))
+                                : std::max(c.dp_topk, 1);
+        if (!(c.twotier && short_read && (ambiguous || weak))) continue;
+        const int32_t kmax = c.twotier_k >= 2 ? c.twotier_k : 5;
+        std::vector<int64_t> passing;
+        for (int64_t k = 0; k < nc && (int32_t)passing.size() < kmax; k++)
+            if (cl[k].score >= min_score) passing.push_back(c0 + k);
+        if (passing.size() < 2) continue;
+        for (int64_t ci : passing) cand.push_back(Cand{r, ci, INT32_MIN, -1});
+    }
+    cand_off[n] = (int64_t)cand.size();
+    // exact copies score A * L (exact_match_exists, mod.rs:818-832); the rest
+    // go through the fast cascade together
+    std::vector<kira_fast_aln> tt_res;
+    std::vector<uint32_t> tt_cig;
+    std::string tt_md;
+    {
+        std::vector<int32_t> fk, frl;
+        std::vector<int64_t> fro;
+        std::vector<kira_chain> fch;
+        for (size_t i = 0; i < cand.size(); i++) {
+            Cand &x = cand[i];
+            const kira_chain &ch = chains[x.chain];
+            const int32_t L = read_len[x.read];
+            const uint8_t *q = oriented(x.read, ch.strand == 1);
+            if (exact_candidate(x.read, ch) &&
+                memcmp(q, contig(ch.ref_id) + ch.ref_start, (size_t)L) == 0) {
+                x.score = wrap_i32((int64_t)c.cfg.match_score * L);
+                continue;
+            }
+            fk.push_back((int32_t)i);
+            fro.push_back(oriented_off(x.read, ch.strand == 1));
+            frl.push_back(L);
+            fch.push_back(ch);
+        }
+        if (!fk.empty()) {
+            kira_fast_stats fst;
+            kira_status fs = kira::fast_dp_run(ctx, g, contig_off, contig_len, (int32_t)fk.size(),
+                                               orient.data(), (int64_t)orient.size(), d_orient,
+                                               fro.data(), frl.data(), fch.data(), &c.cfg, c.fast,
+                                               tt_res, tt_cig, tt_md, fst);
+            if (fs != KIRA_OK) return fs;
+            for (size_t i = 0; i < fk.size(); i++)
+                if (tt_res[i].kind != 0) {
+                    cand[fk[i]].score = tt_res[i].score;
+                    cand[fk[i]].fast = (int32_t)i;
+                }
+        }
+    }
+    for (int32_t r = 0; r < n; r++) {
+        const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+        if (nc == 0) continue;
+        const kira_chain *cl = chains + c0;
+        int32_t topk = topk_of[r];
+        std::vector<int64_t> order;
+        std::vector<int32_t> cached_of;  // per entry of `order`
+        const int64_t a0 = cand_off[r], a1 = cand_off[r + 1];
+        if (a1 > a0) {
+            // two_tier_reorder (:346-451): by score, ties by chain index
+            std::vector<int64_t> idx;
+            for (int64_t i = a0; i < a1; i++) idx.push_back(i);
+            std::stable_sort(idx.begin(), idx.end(), [&](int64_t u, int64_t v) {
+                if (cand[u].score != cand[v].score) return cand[u].score > cand[v].score;
+                return cand[u].chain < cand[v].chain;
+            });
+            const int32_t s0 = cand[idx[0]].score;
+            const int32_t mrg = std::max(sat_mul(wrap_i32((int64_t)c.cfg.match_score +
+                                                          c.cfg.mismatch), 2), 1);
+            bool any_none = false;
+            for (int64_t i : idx) any_none |= cand[i].score == INT32_MIN;
+            int32_t dp_limit;
+            if (any_none) {
+                dp_limit = 2;
+            } else if (idx.size() >= 3 && cand[idx[2]].score == s0) {
+                dp_limit = 2;
+            } else {
+                const int32_t floor_s = sat_sub(s0, mrg);
+                int32_t cnt = 0;
+                for (int64_t i : idx) {
+                    if (cand[i].score < floor_s) break;
+                    cnt++;
+                }
+                dp_limit = std::min(std::max(cnt, 1), 3);
+            }
+            std::vector<uint8_t> in(nc, 0);
+            for (int64_t i : idx) {
+                order.push_back(cand[i].chain);
+                cached_of.push_back(cand[i].fast);
+                in[cand[i].chain - c0] = 1;
+            }
+            for (int64_t k = 0; k < nc; k++)
+                if (!in[k]) {
+                    order.push_back(c0 + k);
+                    cached_of.push_back(-1);
+                }
+            if (c.short_dp_topk < 1) topk = dp_limit;
+            if (topk == 1) {
+                // the runner-up's affine score becomes XS (:673-683)
+                bool have = false;
+                int32_t mx = INT32_MIN;
+                for (int64_t i : idx) {
+                    if (cand[i].chain == order[0] || cand[i].score == INT32_MIN) continue;
+                    mx = have ? std::max(mx, cand[i].score) : cand[i].score;
+                    have = true;
+                }
+                has_comp[r] = have;
+                comp_of[r] = mx;
+            }
+        } else {
+            for (int64_t k = 0; k < nc; k++) {
+                order.push_back(c0 + k);
+                cached_of.push_back(-1);
+            }
+        }
         int32_t selected = 0;
-        for (int64_t k = 0; k < nc; k++) {
-            if (cl[k].score < min_score) continue;
+        for (size_t p = 0; p < order.size(); p++) {
+            if (chains[order[p]].score < min_score_of[r]) continue;
             if (selected >= topk) break;
-            sels.push_back(Sel{r, c0 + k, (uint8_t)(selected == 0), margin});
+            sels.push_back(Sel{r, order[p], (uint8_t)(selected == 0), margin_of[r], cached_of[p]});
             selected++;
         }
+        (void)cl;
     }
     st.chains_used = (int64_t)sels.size();
 
@@ -514,6 +651,16 @@ kira_status Driver::run() {
             outcome[k] = 1;
             continue;
         }
+        if (x.cached >= 0) {
+            // the candidate search's verified alignment (:736-747)
+            if (tt_res[x.cached].kind == 1)
+                st.packed_spectral_resolved++;
+            else
+                st.wfa_resolved++;
+            dp_used[x.read] = 1;
+            outcome[k] = 1;
+            continue;
+        }
         const kira_prefilter_out &o = pf[k];
         st.reason_counts[o.reason < 9 ? o.reason : 9]++;
         abort_of[k] = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
@@ -597,9 +744,13 @@ kira_status Driver::run() {
             A.recs.push_back(Rec{x.read, exact_aln(x.read, ch, A)});
             return;
         }
-        if (fast_of[k] >= 0) {
-            // the fast path's Alignment (mod.rs:636-686, 481-575)
-            const kira_fast_aln &f = fres[fast_of[k]];
+        if (fast_of[k] >= 0 || x.cached >= 0) {
+            // the fast path's Alignment (mod.rs:636-686, 481-575), from the
+            // candidate search or the fallback
+            const bool tt = x.cached >= 0;
+            const kira_fast_aln &f = tt ? tt_res[x.cached] : fres[fast_of[k]];
+            const std::vector<uint32_t> &fc = tt ? tt_cig : fcig;
+            const std::string &fm = tt ? tt_md : fmd;
             kira_stage4_aln a = base_aln(1, ch);
             a.ref_start = f.ref_start;
             a.ref_end = f.ref_end;
@@ -610,10 +761,10 @@ kira_status Driver::run() {
             a.nm = f.nm;
             a.cigar_off = (int64_t)A.cig.size();
             a.n_cigar = f.n_cigar;
-            A.cig.insert(A.cig.end(), fcig.begin() + f.cigar_off,
-                         fcig.begin() + f.cigar_off + f.n_cigar);
+            A.cig.insert(A.cig.end(), fc.begin() + f.cigar_off,
+                         fc.begin() + f.cigar_off + f.n_cigar);
             a.md_off = (int64_t)A.md.size();
-            A.md.append(fmd, (size_t)f.md_off, (size_t)f.md_len);
+            A.md.append(fm, (size_t)f.md_off, (size_t)f.md_len);
             a.md_len = f.md_len;
             A.recs.push_back(Rec{x.read, a});
             return;
@@ -836,9 +987,16 @@ kira_status Driver::run() {
     std::vector<int64_t> first;
     sort_by_read(first);
 
-    // XS from the chain-score proxy (:1215-1250)
+    // XS: the two-tier competitor's affine score, else the chain-score proxy
+    // (:1215-1250)
     for (int32_t r = 0; r < n; r++) {
         if (first[r + 1] - first[r] != 1) continue;
+        if (has_comp[r]) {
+            kira_stage4_aln &a = recs[first[r]].a;
+            a.has_xs = 1;
+            a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
+            continue;
+        }
         const int32_t cb = chain_best[r], cs = chain_second[r];
         if (cb <= 0 || cs <= 0) continue;
         kira_stage4_aln &a = recs[first[r]].a;

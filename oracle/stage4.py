@@ -4,8 +4,9 @@ Follows src/pipeline/stage4_alignment.rs run() (:959-1268) and
 process_read_prefilter (:535-957) in the configuration the GPU dispatcher is
 active (`--gpu`), or with cpu_mode the one without it (fallback chains try
 the fast DP cascade of oracle/fastpath.py before SW; no sieve dispatch);
-single-end, with KIRA_TWOTIER=0 (the two-tier reorder, :346-451, is not
-restated) and KIRA_TWOTIER_MAPQ unset (its default).  Supplementary alignments follow
+single-end, with the two-tier candidate reorder of uncertain short reads
+(:346-451, KIRA_TWOTIER, on by default; off with twotier=False) and
+KIRA_TWOTIER_MAPQ unset (its default).  Supplementary alignments follow
 src/pipeline/split_read.rs:22-151.  Every numeric step goes through the C
 oracle (oracle.py): prefilter_chain, the 2-bit sieve + certification,
 align_batch_simd's SW, banded SW and build_alignment.
@@ -42,7 +43,7 @@ def default_cfg(**kw):
              long_read_threshold=500, max_alignments=1, tail_ext=True, ungap_min_id=-1,
              short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0, spectral_pct=3,
              spectral_floor=2, cert_strict=False, simd="avx2", supplementary=True,
-             cpu_mode=False, fast={})
+             cpu_mode=False, fast={}, twotier=True, twotier_k=5)
     c.update(kw)
     return c
 
@@ -120,12 +121,15 @@ def pick_supplementary_chains(primary, chains):
     return picks
 
 
-def select_chains(read_len, chains, repeat_min_occ, cfg):
-    """The chain loop of process_read_prefilter (:549-708) with two-tier and
-    mate guidance off.  Returns (selected chain indices, chain_best,
-    chain_second, score_margin)."""
+def select_chains(read_len, chains, repeat_min_occ, cfg, ranker=None):
+    """The chain loop of process_read_prefilter (:549-708), mate guidance off.
+    ranker(passing, min_score) runs two_tier_reorder (:346-451) over the
+    passing chain indices and returns (scores dict, cached dict, dp_limit) or
+    None.  Returns (selected chain indices, chain_best, chain_second,
+    score_margin, cached fast results of the selected chains, competitor
+    score or None)."""
     if not chains:
-        return [], 0, 0, I32_MAX
+        return [], 0, 0, I32_MAX, {}, None
     best = chains[0][6]
     # (best as f32 * min_chain_ratio).ceil() as i32 (:567-571)
     min_score = int(math.ceil(_f32(_f32(best) * _f32(cfg["min_chain_ratio"])))) if best > 0 \
@@ -136,18 +140,74 @@ def select_chains(read_len, chains, repeat_min_occ, cfg):
     div = cfg["ambig_div"]
     q = int(best / div)  # i32 division truncates toward zero
     ambiguous = len(chains) > 1 and (margin <= max(q, 8) or repeat_min_occ > 1)
+    weak = len(chains) > 1 and best < int(read_len * 2 / 3)
+    twotier = cfg["twotier"]
+    if short_read and weak and twotier:
+        min_score = min(min_score, int(best / 5))
+    override = cfg["short_dp_topk"] if cfg["short_dp_topk"] >= 1 else None
     if short_read:
-        topk = cfg["short_dp_topk"] if cfg["short_dp_topk"] >= 1 else (2 if ambiguous else 1)
+        topk = override if override is not None else (2 if ambiguous else 1)
     else:
         topk = max(cfg["dp_topk"], 1)
+    order = list(range(len(chains)))
+    cached, competitor = {}, None
+    if twotier and short_read and (ambiguous or weak) and ranker is not None:
+        passing = [ci for ci, c in enumerate(chains) if c[6] >= min_score][:cfg["twotier_k"]]
+        if len(passing) >= 2:
+            scores, cached, dp_limit, order = ranker(passing)
+            if override is None:
+                topk = dp_limit
+            if topk == 1:
+                others = [s for ci, s in scores.items() if ci != order[0]]
+                competitor = max(others) if others else None
     sel = []
-    for ci, c in enumerate(chains):
-        if c[6] < min_score:
+    for ci in order:
+        if chains[ci][6] < min_score:
             continue
         if len(sel) >= topk:
             break
         sel.append(ci)
-    return sel, best, second, margin
+    return sel, best, second, margin, {ci: cached[ci] for ci in sel if ci in cached}, competitor
+
+
+def two_tier_reorder(read_fwd, chains, contigs, c, router, passing):
+    """two_tier_reorder (:346-451) on the passing chains: exact copies score
+    A * L, the rest the fast cascade's alignment score (its result is kept);
+    order by score (ties by chain index), then the other chains in order."""
+    rev = None
+    scored, cached = [], {}
+    for ci in passing:
+        ch = chains[ci]
+        is_rev = ch[5] == 1
+        if is_rev and rev is None:
+            rev = reverse_complement(read_fwd)
+        read = rev if is_rev else read_fwd
+        contig = contigs[ch[0]]
+        if exact_match_exists(read, contig, (ch[1], ch[2], ch[3], ch[4])):
+            scored.append((ci, _wrap_i32(c.match_score * len(read))))
+            continue
+        hit = F.fast_dp(read, contig, ch[1], ch[3], ch[6], c, router)
+        if hit is not None:
+            cached[ci] = hit
+        scored.append((ci, hit[1]["score"] if hit is not None else I32_MIN))
+    scored.sort(key=lambda t: (-t[1], t[0]))
+    scores = {ci: s for ci, s in scored if s != I32_MIN}
+    margin = max(_sat_i32((c.match_score + c.mismatch) * 2), 1)
+    if any(s == I32_MIN for _, s in scored):
+        dp_limit = 2
+    elif len(scored) >= 3 and scored[2][1] == scored[0][1]:
+        dp_limit = 2
+    else:
+        top = _sat_i32(scored[0][1] - margin)
+        n_close = 0
+        for _, s in scored:
+            if s < top:
+                break
+            n_close += 1
+        dp_limit = min(max(n_close, 1), 3)
+    order = [ci for ci, _ in scored] + [ci for ci in range(len(chains))
+                                        if ci not in {x for x, _ in scored}]
+    return scores, cached, dp_limit, order
 
 
 def exact_match_exists(read, contig, span):
@@ -240,13 +300,17 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
     simd_jobs, scalar_jobs = [], []
     router = F.Router(**cfg["fast"])
     cb, cs = [0] * n, [0] * n
+    competitor = [None] * n
     for r in range(n):
         read_fwd = reads[r]
         L = len(read_fwd)
         bucket = 0 if L <= 120 else (1 if L <= 150 else 2)
         cl = chains[r]
         st["chains_total"] += len(cl)
-        sel, cb[r], cs[r], margin = select_chains(L, cl, repeat_min_occ[r], cfg)
+        ranker = (lambda passing, rf=read_fwd, cl=cl:
+                  two_tier_reorder(rf, cl, contigs, c, router, passing))
+        sel, cb[r], cs[r], margin, cached, competitor[r] = select_chains(
+            L, cl, repeat_min_occ[r], cfg, ranker)
         dp_used = False
         for rank, ci in enumerate(sel):
             ch = cl[ci]
@@ -258,6 +322,15 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
             if exact_match_exists(read, contig, span):
                 st["exact_matches"] += 1
                 alns[r].append(_exact_record(read, span, ref_id, is_rev, cfg))
+                continue
+            if ci in cached:
+                # a complete alignment the candidate search verified (:736-747)
+                kind, a = cached[ci]
+                st["packed_spectral_resolved" if kind == "packed" else "wfa_resolved"] += 1
+                alns[r].append(_record("dp_aligned", ref_id, a["ref_start"], a["ref_end"], 0,
+                                       len(read), O.cigar_str(a["cigar"]), a["score"], 0,
+                                       is_rev, a["nm"], a["md"]))
+                dp_used = True
                 continue
             pf = O.prefilter_chain(
                 read, contig, span, c, is_top1=rank == 0, accept_enable=accept_allowed,
@@ -366,9 +439,15 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
     for j, (rec, _) in zip(scalar_jobs, sc):
         alns[j[0]].append(rec)
 
-    # XS from the chain-score proxy (:1230-1250); competitor scores need two-tier
+    # XS: the two-tier competitor's affine score, else the chain-score proxy
+    # (:1224-1250)
     for r in range(n):
-        if len(alns[r]) != 1 or cb[r] <= 0 or cs[r] <= 0:
+        if len(alns[r]) != 1:
+            continue
+        if competitor[r] is not None:
+            alns[r][0]["xs_score"] = min(max(competitor[r], 0), max(alns[r][0]["score"], 0))
+            continue
+        if cb[r] <= 0 or cs[r] <= 0:
             continue
         first = alns[r][0]
         # (score as i64 * cs as i64 / cb as i64) as i32: truncating division,

@@ -51,7 +51,7 @@ def test_chain_selection_rules():
     cfg = S.default_cfg()
     c = lambda s: (0, 0, 10, 0, 10, 0, s)  # noqa: E731
     # min_chain_ratio 0.4: ceil(100 * 0.4) = 40, so 39 is skipped but 40 kept
-    sel, best, second, margin = S.select_chains(150, [c(100), c(39), c(40)], 0, cfg)
+    sel, best, second, margin, *_ = S.select_chains(150, [c(100), c(39), c(40)], 0, cfg)
     assert (best, second, margin) == (100, 39, 61)
     assert sel == [0]  # unambiguous (61 > max(100 / 5, 8)): top-1 only
     # ambiguous: margin 15 <= 20 -> two DP placements, skipping the low one
@@ -150,11 +150,17 @@ def test_sieve_certifies_a_not_top1_chain():
     # two equal-score chains: the read is ambiguous, both get DP; the second
     # is not top-1 so the prefilter falls back and the sieve certifies it
     chains = [[(0, 6000, 6150, 0, 150, 0, 150), (0, 12000, 12150, 0, 150, 0, 150)]]
-    alns, st = S.run([bytes(g)], [read], chains)
+    alns, st = S.run([bytes(g)], [read], chains, S.default_cfg(twotier=False))
     assert st["prefilter_accept"] == 1 and st["gpu_spectral_resolved"] == 1
     kinds = [(a["kind"], a["ref_start"], a["mapq"]) for a in alns[0]]
     assert kinds == [("accepted_ungapped", 6000, 60), ("dp_aligned", 12000, 0)]
     assert alns[0][1]["cigar"] == "150M" and alns[0][1]["score"] == 149 - 4
+    # with the two-tier reorder (the default) the candidate search certifies
+    # both copies (equal scores: dp_limit 2) and stage 4 keeps those results
+    alns, st = S.run([bytes(g)], [read], chains)
+    assert st["packed_spectral_resolved"] == 2 and st["gpu_spectral_resolved"] == 0
+    assert [(a["kind"], a["ref_start"], a["score"]) for a in alns[0]] == \
+        [("dp_aligned", 6000, 145), ("dp_aligned", 12000, 145)]
 
 
 def test_supplementary_alignment_of_a_chimeric_read():
@@ -206,3 +212,66 @@ def test_cpu_mode_resolves_fallbacks_on_the_fast_path():
     _, st_cpu = S.run([g], [bytes(sub)], ch3, S.default_cfg(cpu_mode=True))
     assert st_cpu["gpu_spectral_resolved"] == 0
     assert st_gpu["prefilter_fallback"] == st_cpu["prefilter_fallback"] == 1
+
+
+def _repeat_genome():
+    """A 20 kbp genome with a second copy of 2000..2150 at 8000 that differs
+    at three spaced positions."""
+    g = bytearray(_genome(20000, 11))
+    g[8000:8150] = g[2000:2150]
+    for p in (30, 75, 120):
+        g[8000 + p] = ord("A") if g[8000 + p] != ord("A") else ord("C")
+    return bytes(g)
+
+
+def test_two_tier_reorders_an_ambiguous_read_and_scores_the_competitor():
+    """two_tier_reorder (stage4_alignment.rs:346-451): the read is the second
+    copy exactly, but its chain ranks second.  The candidate search scores the
+    exact copy A * L = 150 and the first copy by the fast cascade (three
+    mismatches: WFA loss 15, score 135); the 15-point gap exceeds the 2 (A + B)
+    = 10 margin, so one DP placement (dp_limit 1) on the exact copy, and the
+    first copy's affine score becomes XS."""
+    g = _repeat_genome()
+    read = g[8000:8150]
+    chains = [[(0, 2000, 2150, 0, 150, 0, 100), (0, 8000, 8150, 0, 150, 0, 95)]]
+    alns, st = S.run([g], [read], chains)
+    a, = alns[0]
+    assert (a["ref_start"], a["cigar"], a["score"], a["xs_score"]) == (8000, "150M", 150, 135)
+    assert st["chains_used"] == 1 and st["exact_matches"] == 1
+    assert st["prefilter_accept"] + st["prefilter_fallback"] + st["prefilter_reject"] == 0
+    # with the reorder off: ambiguous (margin 5 <= 20), both chains get DP; the
+    # exact copy's record comes first, the first copy's lane-SW record after
+    alns, st = S.run([g], [read], chains, S.default_cfg(twotier=False))
+    assert [x["ref_start"] for x in alns[0]] == [8000, 2000] and st["chains_used"] == 2
+    assert st["prefilter_fallback"] == 1 and st["dp_simd"] == 1
+
+
+def test_two_tier_reuses_the_verified_alignment():
+    """A 2-base deletion read of the second copy: both candidates go through the
+    cascade (WFA scores 142 and 122, 20 apart: dp_limit 1); the winner's WFA
+    alignment is taken as it is, with no prefilter, and the loser's score is XS.
+    The weak chain list (best 60 < 2 L / 3) lowers the ratio floor from
+    ceil(60 * 0.4) = 24 to 60 / 5 = 12."""
+    g = _repeat_genome()
+    read = g[8000:8060] + g[8062:8152]
+    c = lambda s: [(0, 2000, 2060, 0, 60, 0, 60), (0, 8000, 8060, 0, 60, 0, 58),  # noqa: E731
+                   (0, 15000, 15020, 0, 20, 0, s)]
+    seen = []
+    S.select_chains(150, c(12), 0, S.default_cfg(),
+                    lambda passing: seen.append(passing) or ({}, {}, 2, [0, 1, 2]))
+    S.select_chains(150, c(11), 0, S.default_cfg(),
+                    lambda passing: seen.append(passing) or ({}, {}, 2, [0, 1, 2]))
+    assert seen == [[0, 1, 2], [0, 1]]
+    sel, _, _, _, _, comp = S.select_chains(
+        150, c(11), 0, S.default_cfg(), lambda passing: ({0: 1, 1: 2}, {}, 1, [1, 0, 2]))
+    assert sel == [1] and comp == 1          # the ranker's order and dp_limit are used
+    cfg = S.default_cfg(supplementary=False)
+    alns, st = S.run([g], [read], [c(11)], cfg)
+    a, = alns[0]
+    assert (a["ref_start"], a["ref_end"], a["kind"], a["mapq"], a["cigar"], a["score"],
+            a["xs_score"]) == (8000, 8152, "dp_aligned", 0, "60M2D90M", 142, 122)
+    assert st["wfa_resolved"] == 1 and st["chains_used"] == 1
+    assert sum(st["reason_counts"]) == 0 and st["dp_simd"] == 0 and st["dp_reads"] == 1
+    # a candidate the cascade cannot align is unknown, not poor: dp_limit 2
+    alns, st = S.run([g], [read], [c(12)], cfg)
+    assert [x["ref_start"] for x in alns[0]] == [8000, 2000] and st["wfa_resolved"] == 2
