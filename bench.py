@@ -627,27 +627,38 @@ def bench_stage4(pkg, ctx, args):
     # host arrays as a caller hands them over (marshalled once, outside the timing)
     x = pkg.Stage4Inputs(b["contig_off"], b["contig_len"], b["reads"], b["chains"],
                          b["repeat_min_occ"])
-    pkg.stage4_run(ctx, genome, x, cfg=cfg).close()
-    times, phases = [], []
-    for _ in range(args.stage4_steps):
-        t0 = time.perf_counter()
-        res = pkg.stage4_run(ctx, genome, x, cfg=cfg)
-        times.append(time.perf_counter() - t0)
-        phases.append(res.phase_ms)
-        stats = res.stats
-        res.close()
-    ms = 1e3 * float(np.median(times))
     n = len(b["reads"])
-    k = int(np.argsort(times)[len(times) // 2])
+
+    def timed(c):
+        pkg.stage4_run(ctx, genome, x, cfg=c).close()
+        times, phases, stats = [], [], []
+        for _ in range(args.stage4_steps):
+            t0 = time.perf_counter()
+            res = pkg.stage4_run(ctx, genome, x, cfg=c)
+            times.append(time.perf_counter() - t0)
+            phases.append(res.phase_ms)
+            stats.append(res.stats)
+            res.close()
+        k = int(np.argsort(times)[len(times) // 2])
+        ms = 1e3 * times[k]
+        return {"ms_per_batch": ms, "reads_per_s": n / (ms * 1e-3),
+                "phase_ms": dict(zip(["triage", "sieve", "lane_sw", "banded_sw", "records"],
+                                     phases[k])),
+                "stats": {kk: v for kk, v in stats[k].items() if kk != "reason_counts"}}
+
     out = {"workload": "stage-4 driver: 8 x 2.5 Mbp contigs (20% repeat families), "
                        f"{n} single-end reads of 100-250 bp (subs, indels, N, chimeras), "
-                       "1-3 chains each, mem defaults, AVX2 lane groups",
-           "reads": n, "ms_per_batch": ms, "reads_per_s": n / (ms * 1e-3),
-           "phase_ms": dict(zip(["triage", "sieve", "lane_sw", "banded_sw", "records"],
-                                phases[k])),
-           "stats": {kk: v for kk, v in stats.items() if kk != "reason_counts"}}
-    print(f"[bench] stage4: {n} reads, {ms:.1f} ms/batch, {out['reads_per_s']:.0f} reads/s, "
-          f"phases {out['phase_ms']}", file=sys.stderr, flush=True)
+                       "1-3 chains each, mem defaults (two-tier reorder on), AVX2 lane groups",
+           "reads": n}
+    out.update(timed(cfg))
+    ms = out["ms_per_batch"]
+    print(f"[bench] stage4 (--gpu): {n} reads, {ms:.1f} ms/batch, {out['reads_per_s']:.0f} "
+          f"reads/s, phases {out['phase_ms']}", file=sys.stderr, flush=True)
+    cpu_cfg = pkg.stage4_cfg(cpu_mode=True)
+    out["cpu_mode"] = timed(cpu_cfg)
+    print(f"[bench] stage4 (cpu mode, fast DP cascade): {out['cpu_mode']['ms_per_batch']:.1f} "
+          f"ms/batch, {out['cpu_mode']['reads_per_s']:.0f} reads/s, phases "
+          f"{out['cpu_mode']['phase_ms']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         m = 2000
         sub = {key: b[key][:m] for key in ("reads", "chains", "repeat_min_occ")}
@@ -658,6 +669,12 @@ def bench_stage4(pkg, ctx, args):
         res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], sub["reads"],
                              sub["chains"], cfg, sub["repeat_min_occ"])
         parity = res.alignments() == want
+        res.close()
+        want_cpu, _ = S.run(b["contigs"], sub["reads"], sub["chains"],
+                            S.default_cfg(cpu_mode=True), sub["repeat_min_occ"])
+        res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], sub["reads"],
+                             sub["chains"], cpu_cfg, sub["repeat_min_occ"])
+        out["cpu_mode"]["parity_vs_oracle_sample"] = res.alignments() == want_cpu
         res.close()
         out["cpu_baseline"] = {"value": m / cpu_s, "unit": "reads/s", "cores": 1, "kind": "port",
                                "sample": f"first {m} reads of the batch through the CPU "
