@@ -592,9 +592,11 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaArgs a) {
                     if (nlo > nhi) {
                         dm = Desc{0, -1, 0, 0};
                     } else {
-                        for (int32_t base = nlo; base <= nhi; base += 64) {
+                        // the same diagonal-to-lane mapping as the pass that
+                        // stored the values: each lane rereads its own
+                        for (int32_t base = dm.lo; base <= dm.hi; base += 64) {
                             const int32_t k = base + lane;
-                            if (k > nhi) continue;
+                            if (k < nlo || k > nhi) continue;
                             const int32_t v = vals[dm.org + k];
                             if (v == kNone) continue;
                             if ((int64_t)2 * v + k < th)
@@ -613,7 +615,8 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaArgs a) {
                 descs[s * 3 + 2] = make_int4(dd.lo, dd.hi, dd.org, 0);
             }
             // check_done (wfa.rs:348-357): the first diagonal from lo
-            const int32_t fd = -wave_max(-first_done);
+            int32_t fd = INT32_MAX;
+            if (__ballot(first_done != INT32_MAX)) fd = -wave_max(-first_done);
             __syncthreads();
             if (fd != INT32_MAX) {
                 k_done = fd;
