@@ -341,6 +341,20 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
     return v;
 }
 
+// Matching bytes going back from pattern position i_end - 1 and text position
+// j_end - 1, at most lim (<= both positions), four a compare.
+__device__ __forceinline__ int32_t back_match(const uint32_t *P, int32_t i_end,
+                                              const uint32_t *T, int32_t j_end, int32_t lim) {
+    int32_t e = 0;
+    while (e + 4 <= lim) {
+        const uint32_t x = ld4(P, i_end - 4 - e) ^ ld4(T, j_end - 4 - e);
+        if (x) return e + (__builtin_clz(x) >> 3);
+        e += 4;
+    }
+    while (e < lim && ld1(P, i_end - 1 - e) == ld1(T, j_end - 1 - e)) e++;
+    return e;
+}
+
 // Appends one op to a CIGAR written backwards from the top of its slot
 // (the traceback emits ops last to first); an op of the kind just written
 // lengthens it (wfa.rs:954-966).
@@ -380,13 +394,15 @@ __device__ int32_t wfa_traceback(const uint32_t *P, int32_t m, const uint32_t *T
             const int32_t iv = get(1, s, k), dv = get(2, s, k);
             const int32_t pv = s >= pen.x ? get(0, s - pen.x, k) : kNone;
             const int32_t mv = pv != kNone ? pv + 1 : kNone;
-            int32_t walk = off;
-            while (walk > 0 && walk != iv && walk != dv && walk != mv) {
-                const int32_t i = walk - 1, j = walk - 1 + k;
-                if (j < 0 || i >= m || j >= n) break;
-                if (ld1(P, i) != ld1(T, j)) break;
-                walk--;
-            }
+            // the walk back over matches stops at the highest of: 0, the
+            // text start (walk = -k), an offset of I, D or the mismatch source
+            // at or below off, or just above the first mismatch going back;
+            // cells always satisfy off <= m and off + k <= n
+            int32_t lo = max(0, -k);
+            if (iv != kNone && iv <= off && iv > lo) lo = iv;
+            if (dv != kNone && dv <= off && dv > lo) lo = dv;
+            if (mv != kNone && mv <= off && mv > lo) lo = mv;
+            const int32_t walk = off - back_match(P, off, T, off + k, off - lo);
             if (off - walk > 0) cg.push(0, (uint32_t)(off - walk));
             off = walk;
             if (s == 0 && off == 0) break;
