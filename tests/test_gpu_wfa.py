@@ -124,3 +124,26 @@ def test_wfa_edges(pkg, ctx):
     assert pkg.wfa_batch(ctx, pairs, pkg.sw_cfg(0, 4, 6, 1)) == [None] * len(pairs)
     with pytest.raises(pkg.HipError):
         pkg.wfa_batch(ctx, [(b"A", b"A", 70000, 0)], pkg.sw_cfg())
+
+
+def test_long_pairs_near_the_size_limits(pkg, ctx):
+    """Patterns of thousands of bases: Myers' global-scratch kernel (above 512
+    bases) and WFA with 28 KB of pattern + text staged in LDS."""
+    rng = np.random.default_rng(17)
+    pairs_m, pairs_w = [], []
+    for m in (700, 3000, 14000):
+        text = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=m + 40).tolist())
+        pat = bytearray(text[20:20 + m])
+        for i in rng.integers(0, m, size=4):
+            pat[int(i)] = ord("A") if pat[int(i)] != ord("A") else ord("C")
+        del pat[m // 2:m // 2 + 3]
+        pairs_m.append((bytes(pat), text, 12))
+        pairs_w.append((bytes(pat), text[20:], 60, 0))
+    print(f"[long] patterns {[len(p[0]) for p in pairs_m]}", flush=True)
+    assert pkg.myers_batch(ctx, pairs_m) == [O.myers_bounded(p, t, k) for p, t, k in pairs_m]
+    c = O.cfg()
+    got = pkg.wfa_batch(ctx, pairs_w, pkg.sw_cfg())
+    want = [O.wfa_align_scored(p, t, c, ms, 0, tbf) for p, t, ms, tbf in pairs_w]
+    assert got == want and all(g is not None for g in got)
+    with pytest.raises(pkg.HipError):
+        pkg.wfa_batch(ctx, [(b"A" * 40000, b"A" * 30000, 10, 0)], pkg.sw_cfg())
