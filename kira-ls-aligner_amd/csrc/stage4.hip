@@ -418,11 +418,16 @@ kira_status Driver::run() {
                                 : std::max(c.dp_topk, 1);
         if (!(c.twotier && short_read && (ambiguous || weak))) continue;
         const int32_t kmax = c.twotier_k >= 2 ? c.twotier_k : 5;
-        std::vector<int64_t> passing;
-        for (int64_t k = 0; k < nc && (int32_t)passing.size() < kmax; k++)
-            if (cl[k].score >= min_score) passing.push_back(c0 + k);
-        if (passing.size() < 2) continue;
-        for (int64_t ci : passing) cand.push_back(Cand{r, ci, INT32_MIN, -1});
+        int32_t np = 0;
+        for (int64_t k = 0; k < nc && np < kmax; k++)
+            if (cl[k].score >= min_score) np++;
+        if (np < 2) continue;
+        np = 0;
+        for (int64_t k = 0; k < nc && np < kmax; k++)
+            if (cl[k].score >= min_score) {
+                cand.push_back(Cand{r, c0 + k, INT32_MIN, -1});
+                np++;
+            }
     }
     cand_off[n] = (int64_t)cand.size();
     // exact copies score A * L (exact_match_exists, mod.rs:818-832); the rest
@@ -463,17 +468,21 @@ kira_status Driver::run() {
                 }
         }
     }
+    // per-read scratch, reused across reads
+    std::vector<int64_t> order, idx;
+    std::vector<int32_t> cached_of;  // per entry of `order`
+    std::vector<uint8_t> in;
     for (int32_t r = 0; r < n; r++) {
         const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
         if (nc == 0) continue;
         const kira_chain *cl = chains + c0;
         int32_t topk = topk_of[r];
-        std::vector<int64_t> order;
-        std::vector<int32_t> cached_of;  // per entry of `order`
+        order.clear();
+        cached_of.clear();
         const int64_t a0 = cand_off[r], a1 = cand_off[r + 1];
         if (a1 > a0) {
             // two_tier_reorder (:346-451): by score, ties by chain index
-            std::vector<int64_t> idx;
+            idx.clear();
             for (int64_t i = a0; i < a1; i++) idx.push_back(i);
             std::stable_sort(idx.begin(), idx.end(), [&](int64_t u, int64_t v) {
                 if (cand[u].score != cand[v].score) return cand[u].score > cand[v].score;
@@ -498,7 +507,7 @@ kira_status Driver::run() {
                 }
                 dp_limit = std::min(std::max(cnt, 1), 3);
             }
-            std::vector<uint8_t> in(nc, 0);
+            in.assign(nc, 0);
             for (int64_t i : idx) {
                 order.push_back(cand[i].chain);
                 cached_of.push_back(cand[i].fast);
@@ -1018,6 +1027,7 @@ kira_status Driver::run() {
         };
         std::vector<Pick> all;
         std::vector<Job> sj;
+        std::vector<int64_t> picks;
         for (int32_t r = 0; r < n; r++) {
             const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
             if (first[r + 1] == first[r] || nc < 2) continue;
@@ -1032,7 +1042,7 @@ kira_status Driver::run() {
                 if ((op & 15u) == kOpM || (op & 15u) == kOpI || (op & 15u) == kOpS) L += op >> 4;
             }
             auto pi = original_interval(p.read_start, p.read_end, p.is_rev != 0, L);
-            std::vector<int64_t> picks;
+            picks.clear();
             for (int64_t k = 1; k < nc; k++) {
                 const kira_chain &ch = cl[k];
                 if (ch.score < min_score) break;
