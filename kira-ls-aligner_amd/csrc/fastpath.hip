@@ -253,7 +253,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                         const int32_t *read_len, const kira_chain *chains,
                         const kira_sw_cfg *cfg, const kira_fast_knobs &kn,
                         std::vector<kira_fast_aln> &res_out, std::vector<uint32_t> &cig_out,
-                        std::string &md_out, kira_fast_stats &st_local) {
+                        std::string &md_out, kira_fast_stats &st_local, bool want_md) {
     memset(&st_local, 0, sizeof st_local);
     try {
         res_out.assign(n, kira_fast_aln{});
@@ -596,7 +596,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                         a.n_cigar = 1;
                         P.cig.push_back((uint32_t)j.L << 4);
                         a.md_off = (int64_t)P.md.size();
-                        a.nm = kira::ungapped_tags(rd, ref, j.L, P.md);
+                        if (want_md) a.nm = kira::ungapped_tags(rd, ref, j.L, P.md);
                         a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
                         P.spectral++;
                         continue;
@@ -648,7 +648,9 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                     a.n_cigar = j.n_ops;
                     P.cig.insert(P.cig.end(), ops, ops + j.n_ops);
                     a.md_off = (int64_t)P.md.size();
-                    a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops, P.md);
+                    if (want_md)
+                        a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops,
+                                           P.md);
                     a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
                     P.wfa++;
                 }
@@ -728,7 +730,7 @@ extern "C" kira_status kira_fast_dp_batch(
     std::string md;
     kira_status rs = kira::fast_dp_run(ctx, g, contig_off, contig_len, n, reads, reads_bytes,
                                        nullptr, read_off, read_len, chains, cfg, kn, res, cig, md,
-                                       st);
+                                       st, true);
     if (rs != KIRA_OK) return rs;
     if (cigar_used) *cigar_used = (int64_t)cig.size();
     if (md_used) *md_used = (int64_t)md.size();

@@ -343,14 +343,15 @@ kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_
 // chain k's read (oriented) at reads[read_off[k]], and at d_reads +
 // read_off[k] on the device when d_reads is not NULL (else the reads are
 // uploaded).  Arguments already validated.  res[k].kind 0 = not resolved;
-// CIGAR words and MD text back to back in cig / md.  Synchronous.
+// CIGAR words and MD text back to back in cig / md; without want_md the
+// records carry no NM / MD (md_len 0).  Synchronous.
 kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *contig_off,
                         const int64_t *contig_len, int32_t n, const uint8_t *reads,
                         int64_t reads_bytes, const uint8_t *d_reads, const int64_t *read_off,
                         const int32_t *read_len, const kira_chain *chains,
                         const kira_sw_cfg *cfg, const kira_fast_knobs &kn,
                         std::vector<kira_fast_aln> &res, std::vector<uint32_t> &cig,
-                        std::string &md, kira_fast_stats &stats);
+                        std::string &md, kira_fast_stats &stats, bool want_md);
 
 }  // namespace kira
 

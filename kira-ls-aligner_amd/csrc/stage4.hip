@@ -459,7 +459,7 @@ kira_status Driver::run() {
             kira_status fs = kira::fast_dp_run(ctx, g, contig_off, contig_len, (int32_t)fk.size(),
                                                orient.data(), (int64_t)orient.size(), d_orient,
                                                fro.data(), frl.data(), fch.data(), &c.cfg, c.fast,
-                                               tt_res, tt_cig, tt_md, fst);
+                                               tt_res, tt_cig, tt_md, fst, false);
             if (fs != KIRA_OK) return fs;
             for (size_t i = 0; i < fk.size(); i++)
                 if (tt_res[i].kind != 0) {
@@ -709,7 +709,7 @@ kira_status Driver::run() {
         kira_status fs = kira::fast_dp_run(ctx, g, contig_off, contig_len, (int32_t)fk.size(),
                                            orient.data(), (int64_t)orient.size(), d_orient,
                                            fro.data(), frl.data(), fch.data(), &c.cfg, c.fast,
-                                           fres, fcig, fmd, fst);
+                                           fres, fcig, fmd, fst, true);
         if (fs != KIRA_OK) return fs;
         for (size_t i = 0; i < fk.size(); i++) {
             if (fres[i].kind == 0) continue;
@@ -767,14 +767,25 @@ kira_status Driver::run() {
             a.read_end = f.read_end;
             a.score = f.score;
             a.mapq = 0;
-            a.nm = f.nm;
             a.cigar_off = (int64_t)A.cig.size();
             a.n_cigar = f.n_cigar;
             A.cig.insert(A.cig.end(), fc.begin() + f.cigar_off,
                          fc.begin() + f.cigar_off + f.n_cigar);
             a.md_off = (int64_t)A.md.size();
-            A.md.append(fm, (size_t)f.md_off, (size_t)f.md_len);
-            a.md_len = f.md_len;
+            if (tt) {
+                // the candidate search builds no MD: only the kept ones need it
+                const uint8_t *q = oriented(x.read, ch.strand == 1);
+                a.nm = f.kind == 1 ? kira::ungapped_tags(q, contig(ch.ref_id) + f.ref_start,
+                                                         (int64_t)read_len[x.read], A.md)
+                                   : kira::nm_md(q, read_len[x.read], contig(ch.ref_id),
+                                                 contig_len[ch.ref_id], f.ref_start,
+                                                 A.cig.data() + a.cigar_off, f.n_cigar, A.md);
+                a.md_len = (int32_t)((int64_t)A.md.size() - a.md_off);
+            } else {
+                a.nm = f.nm;
+                A.md.append(fm, (size_t)f.md_off, (size_t)f.md_len);
+                a.md_len = f.md_len;
+            }
             A.recs.push_back(Rec{x.read, a});
             return;
         }
