@@ -463,7 +463,7 @@ typedef struct {
 /* The router knobs of the fast path (src/alignment/router.rs:48-215), with
  * the reference defaults kira_fast_knobs_default fills in brackets:
  * wfa_max_read_len KIRA_WFA_MAX_LEN [300]; algo KIRA_ALGO 0 packed [0],
- * 1 spectral, 2 wfa, 3 sw; quality_gate / quality_pct
+ * 1 spectral, 2 wfa, 3 sw (other values: packed); quality_gate / quality_pct
  * KIRA_CHAIN_QUALITY_GATE [0] / KIRA_CHAIN_QUALITY_PCT [70]; wfa_budget_pct
  * KIRA_WFA_BUDGET_PCT [25]; myers_pct / myers_floor KIRA_MYERS_BOUND_PCT [15]
  * / KIRA_MYERS_BOUND_FLOOR [4]; spectral_pct / spectral_floor

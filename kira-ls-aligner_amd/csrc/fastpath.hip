@@ -251,7 +251,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                         const int64_t *contig_len, int32_t n, const uint8_t *reads,
                         int64_t reads_bytes, const uint8_t *d_reads_in, const int64_t *read_off,
                         const int32_t *read_len, const kira_chain *chains,
-                        const kira_sw_cfg *cfg, const kira_fast_knobs &kn,
+                        const kira_sw_cfg *cfg, const kira_fast_knobs &kn_in,
                         std::vector<kira_fast_aln> &res_out, std::vector<uint32_t> &cig_out,
                         std::string &md_out, kira_fast_stats &st_local, bool want_md) {
     memset(&st_local, 0, sizeof st_local);
@@ -263,6 +263,17 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
     if (n == 0) return KIRA_OK;
+    // parse_algo (router.rs:28-37): anything unrecognised is packed; the
+    // percentages and floors are u32 knobs
+    kira_fast_knobs kn = kn_in;
+    if (kn.algo < 0 || kn.algo > 3) kn.algo = 0;
+    kn.quality_pct = std::max(kn.quality_pct, 0);
+    kn.wfa_budget_pct = std::max(kn.wfa_budget_pct, 0);
+    kn.myers_pct = std::max(kn.myers_pct, 0);
+    kn.myers_floor = std::max(kn.myers_floor, 0);
+    kn.spectral_pct = std::max(kn.spectral_pct, 0);
+    kn.spectral_floor = std::max(kn.spectral_floor, 0);
+    kn.wfa_ends_free = std::max(kn.wfa_ends_free, 0);
     const kira_sw_cfg c = *cfg;
     const int32_t band = std::max(c.bandwidth, 1);
     kira::WfaPen pen;

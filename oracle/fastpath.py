@@ -139,7 +139,9 @@ def fast_dp(read: bytes, contig: bytes, ref_start: int, read_start: int, chain_s
     """try_fast_dp_alignment_inner (mod.rs:285-440).  Returns (kind, record)
     with kind 'packed' or 'wfa', or None (the caller falls back to SW)."""
     L = len(read)
-    if L == 0 or L > r.wfa_max_read_len or r.algo == "sw":
+    # parse_algo (router.rs:28-37): anything unrecognised is packed
+    algo = r.algo if r.algo in ("packed", "spectral", "wfa", "sw") else "packed"
+    if L == 0 or L > r.wfa_max_read_len or algo == "sw":
         return None
     if not worth_attempting(chain_score, L, r):
         return None
@@ -151,7 +153,7 @@ def fast_dp(read: bytes, contig: bytes, ref_start: int, read_start: int, chain_s
     if text_len < L:
         return None
     text = contig[win_start:win_start + text_len]
-    if r.algo in ("packed", "spectral"):
+    if algo in ("packed", "spectral"):
         a = spectral(read, text, win_start, c, r)
         if a is not None:
             return "packed", a

@@ -50,6 +50,7 @@ CASES = [
     ("ends-free", dict(wfa_ends_free=6), dict(wfa_ends_free=6)),
     ("adaptive", dict(wfa_adaptive=12), dict(wfa_adaptive=12)),
     ("gate-strict", dict(quality_gate=1, cert_strict=1), dict(quality_gate=True, cert_strict=True)),
+    ("algo-unknown", dict(algo=7), dict(algo="bogus")),
     ("tight", dict(myers_pct=5, myers_floor=2, wfa_budget_pct=10, spectral_pct=1,
                    spectral_floor=1), dict(myers_pct=5, myers_floor=2, wfa_budget_pct=10,
                                            spectral_pct=1, spectral_floor=1)),

@@ -92,6 +92,7 @@ def test_router_gates():
     assert F.fast_dp(b"", g, 500, 0, 1, C) is None
     assert F.fast_dp(g[0:301], g, 0, 0, 1, C) is None                        # > 300 bases
     assert F.fast_dp(read, g, 500, 0, 1, C, F.Router(algo="sw")) is None
+    assert F.fast_dp(read, g, 500, 0, 1, C, F.Router(algo="bogus"))[0] == "packed"
     assert F.fast_dp(read, g, 10, 20, 1, C) is None                          # negative start
     assert F.fast_dp(read, g, len(g), 0, 1, C) is None                       # past the contig
     assert F.fast_dp(read, g, len(g) - 100, 0, 1, C) is None                 # window too short
