@@ -60,6 +60,8 @@ def test_records_and_counters_match_oracle(pkg, ctx, cfg):
 
 
 @pytest.mark.parametrize("cfg", [dict(cpu_mode=True), dict(cpu_mode=True, simd="scalar"),
+                                 dict(cpu_mode=True, simd="avxvnni", twotier_k=3),
+                                 dict(cpu_mode=True, simd="neon", twotier=False, bandwidth=24),
                                  dict(cpu_mode=True, fast=dict(wfa_lead=6, wfa_adaptive=10)),
                                  dict(cpu_mode=True, fast=dict(algo="wfa", cert_strict=1))])
 def test_cpu_mode_fast_path_matches_oracle(pkg, ctx, cfg):
