@@ -16,11 +16,24 @@ constexpr uint32_t kOpM = 0, kOpI = 1, kOpD = 2, kOpN = 3, kOpS = 4;
 uint8_t upper(uint8_t b) { return (b >= 'a' && b <= 'z') ? (uint8_t)(b - 32) : b; }
 
 // MD text and edit distance of one alignment (md.rs:25-132).
+// decimal digits of v appended to s
+inline void append_u32(std::string &s, uint32_t v) {
+    char buf[10];
+    int n = 0;
+    do {
+        buf[n++] = (char)('0' + v % 10);
+        v /= 10;
+    } while (v);
+    while (n) s.push_back(buf[--n]);
+}
+
+// MD text appended straight to the caller's string
 struct MdText {
-    std::string text;
+    std::string &text;
     uint32_t run = 0, nm = 0;
+    explicit MdText(std::string &out) : text(out) {}
     void flush() {
-        text += std::to_string(run);
+        append_u32(text, run);
         run = 0;
     }
     // letters keep their (upper-cased) identity, other bytes become N
@@ -28,12 +41,15 @@ struct MdText {
         uint8_t u = upper(b);
         return (u >= 'A' && u <= 'Z') ? (char)u : 'N';
     }
+    static bool edit_when_equal(uint8_t q) {
+        // an identical ambiguity code still counts as an edit
+        const uint8_t u = upper(q);
+        return !(u == 'A' || u == 'C' || u == 'G' || u == 'T' || u == '=');
+    }
     void column(uint8_t q, uint8_t r) {
-        if (q == '=' || upper(q) == upper(r)) {
+        if (q == r || q == '=' || upper(q) == upper(r)) {
             run++;
-            // an identical ambiguity code still counts as an edit
-            uint8_t u = upper(q);
-            if (!(u == 'A' || u == 'C' || u == 'G' || u == 'T' || u == '=')) nm++;
+            nm += edit_when_equal(q) ? 1u : 0u;
         } else {
             flush();
             text += letter(r);
@@ -90,20 +106,18 @@ uint32_t build_record(const uint8_t *read, int32_t L, const uint8_t *win, int64_
     if (s.read_start > 0) cig.push_back(((uint32_t)s.read_start << 4) | kOpS);
     cig.insert(cig.end(), ops, ops + s.n_cigar);
     if (s.read_end < L) cig.push_back(((uint32_t)(L - s.read_end) << 4) | kOpS);
-    MdText md;
+    MdText md(md_out);
     // the CIGAR carries the leading clip, so the replay starts at read
     // position 0 (mod.rs:1231-1233)
     replay(read, (size_t)L, win, (size_t)W, s.ref_start, cig.data() + c0,
            (int32_t)(cig.size() - c0), md);
-    md_out += md.text;
     return md.nm;
 }
 
 uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_len, int64_t ref_pos,
                const uint32_t *ops, int32_t n, std::string &md_out) {
-    MdText md;
+    MdText md(md_out);
     replay(read, (size_t)L, ref, (size_t)ref_len, (size_t)ref_pos, ops, n, md);
-    md_out += md.text;
     return md.nm;
 }
 
@@ -188,10 +202,9 @@ bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_
 }
 
 uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md_out) {
-    MdText md;
+    MdText md(md_out);
     for (int64_t k = 0; k < n; k++) md.column(read[k], ref[k]);
     md.flush();
-    md_out += md.text;
     return md.nm;
 }
 
