@@ -53,7 +53,9 @@ void *kira_ctx_stream(kira_ctx *ctx);
 void kira_ctx_set_stream(kira_ctx *ctx, void *stream);
 void kira_ctx_use_own_stream(kira_ctx *ctx);
 
-/* CudaBackend::release_buffers() — src/cuda/backend.rs:252-261. */
+/* CudaBackend::release_buffers() — src/cuda/backend.rs:252-261: frees the
+ * context's device scratch, including the temporaries the entry points keep
+ * cached between calls (so a warm context allocates nothing per call). */
 void kira_release_buffers(kira_ctx *ctx);
 
 /* ------------------------------------------------------------------ sieve */

@@ -38,18 +38,7 @@ kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e =
     return st;
 }
 
-struct DevMem {
-    std::vector<void *> v;
-    ~DevMem() {
-        for (void *x : v) (void)hipFree(x);
-    }
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) v.push_back((void *)*p);
-        return e;
-    }
-};
+using kira::DevMem;
 
 // bitpacked.rs:89-91: A / C / G / T in either case
 __device__ __forceinline__ bool packable(uint32_t b) {
@@ -314,7 +303,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
     hipError_t e = hipSetDevice(ctx->device);
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
     hipStream_t st = ctx->stream;
-    DevMem dm;
+    DevMem dm(ctx);
     const uint8_t *d_reads = d_reads_in;
     if (!d_reads) {
         uint8_t *d = nullptr;

@@ -8,6 +8,8 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "kira_hip.h"
@@ -277,6 +279,39 @@ uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_l
 bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
                     uint32_t ref_start, const uint8_t *read, int64_t read_len);
 
+// Device blocks cached on the context between calls, so an entry point's
+// temporaries cost no hipMalloc / hipFree once warm: sizes round up to a
+// quarter of their power of two (at least 256 B), a freed block waits in its
+// size's list for the next request of that size.  Blocks above 1 GiB are not
+// cached.  One call at a time per context (as for every context state).
+hipError_t cached_alloc(kira_ctx *ctx, void **p, size_t bytes, size_t *size_class);
+void cached_free(kira_ctx *ctx, void *p, size_t size_class);
+
+// The device allocations of one scope, back to the context's cache (or freed,
+// without a context) when the scope ends.
+struct DevMem {
+    kira_ctx *ctx = nullptr;
+    std::vector<std::pair<void *, size_t>> v;
+    DevMem() = default;
+    explicit DevMem(kira_ctx *c) : ctx(c) {}
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() {
+        for (auto &x : v) cached_free(ctx, x.first, x.second);
+    }
+    template <class T>
+    hipError_t alloc(T **p, size_t count) {
+        void *q = nullptr;
+        size_t cls = 0;
+        hipError_t e = cached_alloc(ctx, &q, std::max<size_t>(count * sizeof(T), 1), &cls);
+        if (e == hipSuccess) {
+            *p = (T *)q;
+            v.emplace_back(q, cls);
+        }
+        return e;
+    }
+};
+
 // Host worker threads for record building: the machine's, at most 16.
 inline int host_threads() {
     const unsigned h = std::thread::hardware_concurrency();
@@ -363,6 +398,8 @@ struct kira_ctx {
     // geometric-growth device scratch for the reference-layout batch entry
     void *buf[16] = {};
     size_t cap[16] = {};
+    // free cached device blocks by size class (kira::cached_alloc)
+    std::unordered_map<size_t, std::vector<void *>> free_blocks;
 };
 
 struct kira_genome {

@@ -114,6 +114,9 @@ void kira_release_buffers(kira_ctx *ctx) {
         ctx->buf[k] = nullptr;
         ctx->cap[k] = 0;
     }
+    for (auto &kv : ctx->free_blocks)
+        for (void *p : kv.second) (void)hipFree(p);
+    ctx->free_blocks.clear();
 }
 
 void kira_ctx_destroy(kira_ctx *ctx) {
@@ -360,3 +363,48 @@ kira_status kira_sieve_genome_dev(kira_ctx *ctx, const kira_genome *g,
 }
 
 }  // extern "C"
+
+namespace kira {
+
+hipError_t cached_alloc(kira_ctx *ctx, void **p, size_t bytes, size_t *size_class) {
+    const size_t kMaxCached = (size_t)1 << 30;
+    if (!ctx || bytes > kMaxCached) {
+        *size_class = 0;
+        return hipMalloc(p, bytes);
+    }
+    size_t pow2 = 256;
+    while (pow2 < bytes) pow2 <<= 1;
+    const size_t step = std::max<size_t>(pow2 / 4, 64);
+    const size_t cls = std::max<size_t>((bytes + step - 1) / step * step, 256);
+    *size_class = cls;
+    auto it = ctx->free_blocks.find(cls);
+    if (it != ctx->free_blocks.end() && !it->second.empty()) {
+        *p = it->second.back();
+        it->second.pop_back();
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, cls);
+    if (e != hipSuccess) {
+        // give the cached blocks back to the device and try once more
+        for (auto &kv : ctx->free_blocks)
+            for (void *q : kv.second) (void)hipFree(q);
+        ctx->free_blocks.clear();
+        e = hipMalloc(p, cls);
+    }
+    return e;
+}
+
+void cached_free(kira_ctx *ctx, void *p, size_t size_class) {
+    if (!p) return;
+    if (!ctx || size_class == 0) {
+        (void)hipFree(p);
+        return;
+    }
+    try {
+        ctx->free_blocks[size_class].push_back(p);
+    } catch (const std::bad_alloc &) {
+        (void)hipFree(p);
+    }
+}
+
+}  // namespace kira

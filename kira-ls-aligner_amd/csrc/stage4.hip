@@ -82,18 +82,7 @@ kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e =
 }
 
 // device allocations released together
-struct DevMem {
-    std::vector<void *> v;
-    ~DevMem() {
-        for (void *x : v) (void)hipFree(x);
-    }
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count * sizeof(T), 1));
-        if (e == hipSuccess) v.push_back((void *)*p);
-        return e;
-    }
-};
+using kira::DevMem;
 
 int32_t sat_sub(int32_t a, int32_t b) {
     int64_t v = (int64_t)a - b;
@@ -583,7 +572,7 @@ kira_status Driver::run() {
             }
         }
         const int32_t ne = (int32_t)ex_sel.size();
-        DevMem dm;
+        DevMem dm(ctx);
         int64_t *d_ro, *d_fo, *d_fl, *d_ero = nullptr, *d_epos = nullptr;
         int32_t *d_rl, *d_mg, *d_erl = nullptr;
         kira_span *d_sp;
@@ -830,7 +819,7 @@ kira_status Driver::run() {
                                        c.spectral_floor);
             max_len = std::max(max_len, rl[t]);
         }
-        DevMem dm;
+        DevMem dm(ctx);
         int64_t *d_ro, *d_ws;
         int32_t *d_rl, *d_wl, *d_lim, *d_shift, *d_best, *d_second;
         uint32_t *d_planes;
@@ -1164,6 +1153,7 @@ kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_conti
         Driver d{ctx,      g,        contig_off, contig_len, n_reads, reads, read_off,
                  read_len, chain_off, chains,    *cfg,       *res,    ctx->stream};
         d.rmo = repeat_min_occ;
+        d.mem.ctx = ctx;
         st = d.run();
     } catch (const std::bad_alloc &) {
         st = fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");

@@ -92,18 +92,7 @@ __global__ void compact_cigar_kernel(int32_t n, const int32_t *__restrict__ res,
 }
 
 // device allocations released together
-struct DevMem {
-    std::vector<void *> v;
-    ~DevMem() {
-        for (void *x : v) (void)hipFree(x);
-    }
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) v.push_back((void *)*p);
-        return e;
-    }
-};
+using kira::DevMem;
 
 }  // namespace
 
@@ -436,7 +425,7 @@ kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, con
     const size_t base = cigar.size();
     cigar.resize(base + (size_t)total);
     if (total > 0) {
-        DevMem dm;
+        DevMem dm(ctx);
         int64_t *d_src, *d_dst;
         uint32_t *d_out;
         if ((e = dm.alloc(&d_src, n)) || (e = dm.alloc(&d_dst, n)) ||
@@ -481,7 +470,7 @@ kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg 
         kira_sw_plan *p;
         ~PlanGuard() { kira_banded_sw_plan_destroy(p); }
     } pg{plan};
-    DevMem dm;
+    DevMem dm(ctx);
     int64_t *d_ro, *d_fo;
     int32_t *d_rl, *d_fl, *d_off, *d_band = nullptr, *d_ab, *d_res;
     uint32_t *d_cig;
@@ -529,7 +518,7 @@ kira_status kira_banded_sw_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads,
             ref_off[k] < 0 || ref_len[k] < 0 || ref_off[k] + ref_len[k] > refs_bytes)
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "job outside its buffers");
     }
-    DevMem dm;
+    DevMem dm(ctx);
     uint8_t *d_reads, *d_refs;
     hipError_t e;
     hipStream_t s = ctx->stream;

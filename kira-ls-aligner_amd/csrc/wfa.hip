@@ -36,18 +36,7 @@ kira_status fail(kira_ctx *ctx, kira_status st, const char *what, hipError_t e =
     return st;
 }
 
-struct DevMem {
-    std::vector<void *> v;
-    ~DevMem() {
-        for (void *x : v) (void)hipFree(x);
-    }
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) v.push_back((void *)*p);
-        return e;
-    }
-};
+using kira::DevMem;
 
 // ------------------------------------------------------------------ Myers
 
@@ -792,7 +781,7 @@ kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_
     std::vector<int64_t> loff(n_long + 1, 0);
     for (int32_t t = 0; t < n_long; t++)
         loff[t + 1] = loff[t] + 6 * (int64_t)((h_pat_len[order[t]] + 63) / 64);
-    DevMem dm;
+    DevMem dm(ctx);
     int32_t *d_order = nullptr;
     int64_t *d_loff = nullptr;
     uint64_t *d_scratch = nullptr;
@@ -879,7 +868,7 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA history of one pair above 16 GiB");
         slots = 1;
     }
-    DevMem dm;
+    DevMem dm(ctx);
     int64_t *d_coff = nullptr, *d_dst = nullptr;
     int32_t *d_res = nullptr, *d_ccap = nullptr, *d_slab = nullptr, *d_queue = nullptr;
     uint32_t *d_cig = nullptr, *d_cig2 = nullptr;
@@ -972,7 +961,7 @@ extern "C" kira_status kira_myers_batch(kira_ctx *ctx, int32_t n, const uint8_t 
         if (max_k[k] < 0) return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "max_k < 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
-    DevMem dm;
+    DevMem dm(ctx);
     uint8_t *d_pat = nullptr, *d_txt = nullptr, *d_found = nullptr;
     int64_t *d_poff = nullptr, *d_toff = nullptr;
     int32_t *d_plen = nullptr, *d_tlen = nullptr, *d_k = nullptr, *d_dist = nullptr,
@@ -1035,7 +1024,7 @@ extern "C" kira_status kira_wfa_batch(kira_ctx *ctx, int32_t n, const uint8_t *p
     }
     hipError_t e = hipSetDevice(ctx->device);
     if (e) return fail(ctx, KIRA_ERR_DRIVER, "set device", e);
-    DevMem dm;
+    DevMem dm(ctx);
     uint8_t *d_pat = nullptr, *d_txt = nullptr;
     int64_t *d_poff = nullptr, *d_toff = nullptr;
     int32_t *d_plen = nullptr, *d_tlen = nullptr, *d_ms = nullptr, *d_tbf = nullptr;
