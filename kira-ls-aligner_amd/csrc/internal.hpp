@@ -286,6 +286,11 @@ bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_
 // cached.  One call at a time per context (as for every context state).
 hipError_t cached_alloc(kira_ctx *ctx, void **p, size_t bytes, size_t *size_class);
 void cached_free(kira_ctx *ctx, void *p, size_t size_class);
+// Grow-only pinned host buffer `slot` (0-3) of the context, kept until
+// kira_release_buffers: staging for the large host-device copies, which run
+// at full PCIe rate only from pinned memory.  Its contents are not kept when
+// it grows.
+hipError_t pinned_buf(kira_ctx *ctx, int slot, size_t bytes, void **out);
 
 // The device allocations of one scope, back to the context's cache (or freed,
 // without a context) when the scope ends.
@@ -307,6 +312,27 @@ struct DevMem {
         if (e == hipSuccess) {
             *p = (T *)q;
             v.emplace_back(q, cls);
+        }
+        return e;
+    }
+};
+
+// A typed view of pinned staging buffer `slot` of the context.  Slots in
+// use: 0 the stage-4 oriented reads, 1 its prefilter outcomes, 2 its
+// prefilter job arrays.
+template <class T>
+struct Pinned {
+    T *p = nullptr;
+    size_t n = 0;
+    T *data() const { return p; }
+    size_t size() const { return n; }
+    T &operator[](size_t i) const { return p[i]; }
+    hipError_t resize(kira_ctx *ctx, int slot, size_t count) {
+        void *q = nullptr;
+        hipError_t e = pinned_buf(ctx, slot, count * sizeof(T), &q);
+        if (e == hipSuccess) {
+            p = (T *)q;
+            n = count;
         }
         return e;
     }
@@ -400,6 +426,9 @@ struct kira_ctx {
     size_t cap[16] = {};
     // free cached device blocks by size class (kira::cached_alloc)
     std::unordered_map<size_t, std::vector<void *>> free_blocks;
+    // pinned host staging buffers (kira::pinned_buf)
+    void *pin[4] = {};
+    size_t pin_cap[4] = {};
 };
 
 struct kira_genome {

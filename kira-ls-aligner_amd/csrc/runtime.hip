@@ -117,6 +117,11 @@ void kira_release_buffers(kira_ctx *ctx) {
     for (auto &kv : ctx->free_blocks)
         for (void *p : kv.second) (void)hipFree(p);
     ctx->free_blocks.clear();
+    for (int k = 0; k < 4; k++) {
+        if (ctx->pin[k]) (void)hipHostFree(ctx->pin[k]);
+        ctx->pin[k] = nullptr;
+        ctx->pin_cap[k] = 0;
+    }
 }
 
 void kira_ctx_destroy(kira_ctx *ctx) {
@@ -405,6 +410,28 @@ void cached_free(kira_ctx *ctx, void *p, size_t size_class) {
     } catch (const std::bad_alloc &) {
         (void)hipFree(p);
     }
+}
+
+hipError_t pinned_buf(kira_ctx *ctx, int slot, size_t bytes, void **out) {
+    bytes = std::max<size_t>(bytes, 1);
+    if (bytes > ctx->pin_cap[slot]) {
+        if (ctx->pin[slot]) {
+            // copies from or to the old buffer may still be in flight
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return e;
+            (void)hipHostFree(ctx->pin[slot]);
+            ctx->pin[slot] = nullptr;
+            ctx->pin_cap[slot] = 0;
+        }
+        const size_t cap = ((bytes + bytes / 4) + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, cap, 0);
+        if (e != hipSuccess) return e;
+        ctx->pin[slot] = p;
+        ctx->pin_cap[slot] = cap;
+    }
+    *out = ctx->pin[slot];
+    return hipSuccess;
 }
 
 }  // namespace kira

@@ -145,6 +145,13 @@ struct Arena {
     std::vector<Rec> recs;
 };
 
+// Room for `extra` more elements, growing geometrically.
+template <class V>
+void grow_for(V &v, size_t extra) {
+    const size_t need = v.size() + extra;
+    if (need > v.capacity()) v.reserve(std::max(need, v.capacity() * 2));
+}
+
 using kira::host_threads;
 using kira::parallel_chunks;
 
@@ -181,7 +188,7 @@ struct Driver {
     hipStream_t s;
 
     // forward and reverse-complement copy of every read, back to back
-    std::vector<uint8_t> orient;
+    kira::Pinned<uint8_t> orient;
     std::vector<int64_t> ooff;  // read k: forward at ooff[k], reverse at ooff[k] + L
     uint8_t *d_orient = nullptr;
     DevMem mem;
@@ -205,6 +212,15 @@ struct Driver {
         parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
             for (int64_t i = b; i < e; i++) f(i, ar[t]);
         });
+        size_t nr = 0, nc = 0, nm = 0;
+        for (const Arena &A : ar) {
+            nr += A.recs.size();
+            nc += A.cig.size();
+            nm += A.md.size();
+        }
+        grow_for(recs, nr);
+        grow_for(R.cigar, nc);
+        grow_for(R.md, nm);
         for (Arena &A : ar) {
             const int64_t bc = (int64_t)R.cigar.size(), bm = (int64_t)R.md.size();
             R.cigar.insert(R.cigar.end(), A.cig.begin(), A.cig.end());
@@ -299,15 +315,15 @@ struct Driver {
         return kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, res.data(), cig);
     }
 
-    // Stable grouping of `recs` by read; first[r] .. first[r+1] are read r's.
-    void sort_by_read(std::vector<int64_t> &first) {
+    // Groups the records by read without moving them: read r's records are
+    // recs[perm[k]] for k in [first[r], first[r + 1]), in record order.
+    void group_by_read(std::vector<int64_t> &first, std::vector<int64_t> &perm) const {
         first.assign((size_t)n + 1, 0);
         for (const Rec &x : recs) first[x.read + 1]++;
         for (int32_t r = 0; r < n; r++) first[r + 1] += first[r];
         std::vector<int64_t> pos(first.begin(), first.end() - 1);
-        std::vector<Rec> out(recs.size());
-        for (const Rec &x : recs) out[pos[x.read]++] = x;
-        recs.swap(out);
+        perm.resize(recs.size());
+        for (size_t k = 0; k < recs.size(); k++) perm[pos[recs[k].read]++] = (int64_t)k;
     }
 
     // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
@@ -353,7 +369,9 @@ kira_status Driver::run() {
         ooff[r] = total;
         total += 2 * (int64_t)read_len[r];
     }
-    orient.resize((size_t)total);
+    hipError_t e;
+    if ((e = orient.resize(ctx, 0, (size_t)total)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned read staging", e);
     uint8_t comp[256];
     for (int b = 0; b < 256; b++) comp[b] = comp_base((uint8_t)b);
     parallel_chunks(n, 4096, [&](int, int64_t b, int64_t e) {
@@ -365,6 +383,13 @@ kira_status Driver::run() {
             for (int32_t k = 0; k < L; k++) f[L + k] = comp[src[L - 1 - k]];
         }
     });
+
+    // the oriented reads go to the device once; every phase reads them there
+    if ((e = mem.alloc(&d_orient, (size_t)total + 8)) ||
+        (total && (e = hipMemcpyAsync(d_orient, orient.data(), (size_t)total,
+                                      hipMemcpyHostToDevice, s))))
+        return fail(ctx, e == hipErrorOutOfMemory ? KIRA_ERR_OUT_OF_MEMORY : KIRA_ERR_RUNTIME,
+                    "read upload", e);
 
     // chain selection (:549-708, mate guidance off), in two passes: the
     // two-tier candidate search of uncertain short reads (:596-627) runs its
@@ -428,13 +453,22 @@ kira_status Driver::run() {
         std::vector<int32_t> fk, frl;
         std::vector<int64_t> fro;
         std::vector<kira_chain> fch;
+        std::vector<uint8_t> is_copy(cand.size(), 0);
+        parallel_chunks((int64_t)cand.size(), 8192, [&](int, int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; i++) {
+                const Cand &x = cand[i];
+                const kira_chain &ch = chains[x.chain];
+                is_copy[i] = exact_candidate(x.read, ch) &&
+                             memcmp(oriented(x.read, ch.strand == 1),
+                                    contig(ch.ref_id) + ch.ref_start,
+                                    (size_t)read_len[x.read]) == 0;
+            }
+        });
         for (size_t i = 0; i < cand.size(); i++) {
             Cand &x = cand[i];
             const kira_chain &ch = chains[x.chain];
             const int32_t L = read_len[x.read];
-            const uint8_t *q = oriented(x.read, ch.strand == 1);
-            if (exact_candidate(x.read, ch) &&
-                memcmp(q, contig(ch.ref_id) + ch.ref_start, (size_t)L) == 0) {
+            if (is_copy[i]) {
                 x.score = wrap_i32((int64_t)c.cfg.match_score * L);
                 continue;
             }
@@ -457,101 +491,125 @@ kira_status Driver::run() {
                 }
         }
     }
-    // per-read scratch, reused across reads
-    std::vector<int64_t> order, idx;
-    std::vector<int32_t> cached_of;  // per entry of `order`
-    std::vector<uint8_t> in;
-    for (int32_t r = 0; r < n; r++) {
-        const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
-        if (nc == 0) continue;
-        const kira_chain *cl = chains + c0;
-        int32_t topk = topk_of[r];
-        order.clear();
-        cached_of.clear();
-        const int64_t a0 = cand_off[r], a1 = cand_off[r + 1];
-        if (a1 > a0) {
-            // two_tier_reorder (:346-451): by score, ties by chain index
-            idx.clear();
-            for (int64_t i = a0; i < a1; i++) idx.push_back(i);
-            std::stable_sort(idx.begin(), idx.end(), [&](int64_t u, int64_t v) {
-                if (cand[u].score != cand[v].score) return cand[u].score > cand[v].score;
-                return cand[u].chain < cand[v].chain;
-            });
-            const int32_t s0 = cand[idx[0]].score;
-            const int32_t mrg = std::max(sat_mul(wrap_i32((int64_t)c.cfg.match_score +
-                                                          c.cfg.mismatch), 2), 1);
-            bool any_none = false;
-            for (int64_t i : idx) any_none |= cand[i].score == INT32_MIN;
-            int32_t dp_limit;
-            if (any_none) {
-                dp_limit = 2;
-            } else if (idx.size() >= 3 && cand[idx[2]].score == s0) {
-                dp_limit = 2;
-            } else {
-                const int32_t floor_s = sat_sub(s0, mrg);
-                int32_t cnt = 0;
-                for (int64_t i : idx) {
-                    if (cand[i].score < floor_s) break;
-                    cnt++;
+    // each read selects; threads take read ranges and their selections are
+    // joined in read order
+    std::vector<std::vector<Sel>> part((size_t)host_threads());
+    parallel_chunks(n, 4096, [&](int t, int64_t rb, int64_t re) {
+        std::vector<Sel> &out = part[(size_t)t];
+        // per-read scratch, reused across reads
+        std::vector<int64_t> order, idx;
+        std::vector<int32_t> cached_of;  // per entry of `order`
+        std::vector<uint8_t> in;
+        for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
+            const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+            if (nc == 0) continue;
+            const kira_chain *cl = chains + c0;
+            int32_t topk = topk_of[r];
+            order.clear();
+            cached_of.clear();
+            const int64_t a0 = cand_off[r], a1 = cand_off[r + 1];
+            if (a1 > a0) {
+                // two_tier_reorder (:346-451): by score, ties by chain index (a
+                // total order: the chains of a read are distinct)
+                idx.clear();
+                for (int64_t i = a0; i < a1; i++) idx.push_back(i);
+                std::sort(idx.begin(), idx.end(), [&](int64_t u, int64_t v) {
+                    if (cand[u].score != cand[v].score) return cand[u].score > cand[v].score;
+                    return cand[u].chain < cand[v].chain;
+                });
+                const int32_t s0 = cand[idx[0]].score;
+                const int32_t mrg = std::max(sat_mul(wrap_i32((int64_t)c.cfg.match_score +
+                                                              c.cfg.mismatch), 2), 1);
+                bool any_none = false;
+                for (int64_t i : idx) any_none |= cand[i].score == INT32_MIN;
+                int32_t dp_limit;
+                if (any_none) {
+                    dp_limit = 2;
+                } else if (idx.size() >= 3 && cand[idx[2]].score == s0) {
+                    dp_limit = 2;
+                } else {
+                    const int32_t floor_s = sat_sub(s0, mrg);
+                    int32_t cnt = 0;
+                    for (int64_t i : idx) {
+                        if (cand[i].score < floor_s) break;
+                        cnt++;
+                    }
+                    dp_limit = std::min(std::max(cnt, 1), 3);
                 }
-                dp_limit = std::min(std::max(cnt, 1), 3);
-            }
-            in.assign(nc, 0);
-            for (int64_t i : idx) {
-                order.push_back(cand[i].chain);
-                cached_of.push_back(cand[i].fast);
-                in[cand[i].chain - c0] = 1;
-            }
-            for (int64_t k = 0; k < nc; k++)
-                if (!in[k]) {
+                in.assign(nc, 0);
+                for (int64_t i : idx) {
+                    order.push_back(cand[i].chain);
+                    cached_of.push_back(cand[i].fast);
+                    in[cand[i].chain - c0] = 1;
+                }
+                for (int64_t k = 0; k < nc; k++)
+                    if (!in[k]) {
+                        order.push_back(c0 + k);
+                        cached_of.push_back(-1);
+                    }
+                if (c.short_dp_topk < 1) topk = dp_limit;
+                if (topk == 1) {
+                    // the runner-up's affine score becomes XS (:673-683)
+                    bool have = false;
+                    int32_t mx = INT32_MIN;
+                    for (int64_t i : idx) {
+                        if (cand[i].chain == order[0] || cand[i].score == INT32_MIN) continue;
+                        mx = have ? std::max(mx, cand[i].score) : cand[i].score;
+                        have = true;
+                    }
+                    has_comp[r] = have;
+                    comp_of[r] = mx;
+                }
+            } else {
+                for (int64_t k = 0; k < nc; k++) {
                     order.push_back(c0 + k);
                     cached_of.push_back(-1);
                 }
-            if (c.short_dp_topk < 1) topk = dp_limit;
-            if (topk == 1) {
-                // the runner-up's affine score becomes XS (:673-683)
-                bool have = false;
-                int32_t mx = INT32_MIN;
-                for (int64_t i : idx) {
-                    if (cand[i].chain == order[0] || cand[i].score == INT32_MIN) continue;
-                    mx = have ? std::max(mx, cand[i].score) : cand[i].score;
-                    have = true;
-                }
-                has_comp[r] = have;
-                comp_of[r] = mx;
             }
-        } else {
-            for (int64_t k = 0; k < nc; k++) {
-                order.push_back(c0 + k);
-                cached_of.push_back(-1);
+            int32_t selected = 0;
+            for (size_t p = 0; p < order.size(); p++) {
+                if (chains[order[p]].score < min_score_of[r]) continue;
+                if (selected >= topk) break;
+                out.push_back(
+                    Sel{r, order[p], (uint8_t)(selected == 0), margin_of[r], cached_of[p]});
+                selected++;
             }
+            (void)cl;
         }
-        int32_t selected = 0;
-        for (size_t p = 0; p < order.size(); p++) {
-            if (chains[order[p]].score < min_score_of[r]) continue;
-            if (selected >= topk) break;
-            sels.push_back(Sel{r, order[p], (uint8_t)(selected == 0), margin_of[r], cached_of[p]});
-            selected++;
+    });
+    {
+        size_t tot = 0;
+        for (const auto &v : part) tot += v.size();
+        sels.reserve(tot);
+        for (auto &v : part) {
+            sels.insert(sels.end(), v.begin(), v.end());
+            std::vector<Sel>().swap(v);
         }
-        (void)cl;
     }
     st.chains_used = (int64_t)sels.size();
 
     // exact-match check and prefilter of every selected chain, on the device
     const int32_t ns = (int32_t)sels.size();
     std::vector<uint8_t> exact(ns, 0);
-    std::vector<kira_prefilter_out> pf(ns);
-    hipError_t e;
-    if ((e = mem.alloc(&d_orient, (size_t)total + 8)) ||
-        (total && (e = hipMemcpyAsync(d_orient, orient.data(), (size_t)total,
-                                      hipMemcpyHostToDevice, s))))
-        return fail(ctx, e == hipErrorOutOfMemory ? KIRA_ERR_OUT_OF_MEMORY : KIRA_ERR_RUNTIME,
-                    "read upload", e);
+    kira::Pinned<kira_prefilter_out> pf;
+    if (ns > 0 && (e = pf.resize(ctx, 1, (size_t)ns)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned prefilter staging", e);
     if (ns > 0) {
-        std::vector<int64_t> ro(ns), fo(ns), fl(ns);
-        std::vector<int32_t> rl(ns), mg(ns);
-        std::vector<kira_span> sp(ns);
-        std::vector<uint8_t> top(ns);
+        // the job arrays side by side in one pinned block, uploaded with one copy
+        const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t o_fo = al((size_t)ns * 8), o_fl = o_fo + al((size_t)ns * 8),
+                     o_rl = o_fl + al((size_t)ns * 8), o_mg = o_rl + al((size_t)ns * 4),
+                     o_sp = o_mg + al((size_t)ns * 4),
+                     o_top = o_sp + al((size_t)ns * sizeof(kira_span)),
+                     o_end = o_top + al((size_t)ns);
+        kira::Pinned<uint8_t> hb;
+        if ((e = hb.resize(ctx, 2, o_end)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned prefilter staging", e);
+        int64_t *ro = (int64_t *)hb.data(), *fo = (int64_t *)(hb.data() + o_fo),
+                *fl = (int64_t *)(hb.data() + o_fl);
+        int32_t *rl = (int32_t *)(hb.data() + o_rl), *mg = (int32_t *)(hb.data() + o_mg);
+        kira_span *sp = (kira_span *)(hb.data() + o_sp);
+        uint8_t *top = hb.data() + o_top;
         std::vector<int64_t> ex_ro, ex_pos;
         std::vector<int32_t> ex_rl, ex_sel;
         for (int32_t k = 0; k < ns; k++) {
@@ -573,25 +631,26 @@ kira_status Driver::run() {
         }
         const int32_t ne = (int32_t)ex_sel.size();
         DevMem dm(ctx);
-        int64_t *d_ro, *d_fo, *d_fl, *d_ero = nullptr, *d_epos = nullptr;
-        int32_t *d_rl, *d_mg, *d_erl = nullptr;
-        kira_span *d_sp;
-        uint8_t *d_top, *d_ex = nullptr;
+        uint8_t *db = nullptr;
+        int64_t *d_ero = nullptr, *d_epos = nullptr;
+        int32_t *d_erl = nullptr;
+        uint8_t *d_ex = nullptr;
         kira_prefilter_out *d_out;
-        if ((e = dm.alloc(&d_ro, ns)) || (e = dm.alloc(&d_fo, ns)) || (e = dm.alloc(&d_fl, ns)) ||
-            (e = dm.alloc(&d_rl, ns)) || (e = dm.alloc(&d_mg, ns)) || (e = dm.alloc(&d_sp, ns)) ||
-            (e = dm.alloc(&d_top, ns)) || (e = dm.alloc(&d_out, ns)) ||
+        if ((e = dm.alloc(&db, o_end)) || (e = dm.alloc(&d_out, ns)) ||
             (ne && ((e = dm.alloc(&d_ero, ne)) || (e = dm.alloc(&d_epos, ne)) ||
                     (e = dm.alloc(&d_erl, ne)) || (e = dm.alloc(&d_ex, ne)))))
             return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "triage allocation", e);
+        const int64_t *d_ro = (const int64_t *)db, *d_fo = (const int64_t *)(db + o_fo),
+                      *d_fl = (const int64_t *)(db + o_fl);
+        const int32_t *d_rl = (const int32_t *)(db + o_rl), *d_mg = (const int32_t *)(db + o_mg);
+        const kira_span *d_sp = (const kira_span *)(db + o_sp);
+        const uint8_t *d_top = db + o_top;
         auto up = [&](void *d, const void *h, size_t bytes) {
             return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
         };
-        if ((e = up(d_ro, ro.data(), ns * 8)) || (e = up(d_fo, fo.data(), ns * 8)) ||
-            (e = up(d_fl, fl.data(), ns * 8)) || (e = up(d_rl, rl.data(), ns * 4)) ||
-            (e = up(d_mg, mg.data(), ns * 4)) ||
-            (e = up(d_sp, sp.data(), ns * sizeof(kira_span))) || (e = up(d_top, top.data(), ns)) ||
-            (ne && ((e = up(d_ero, ex_ro.data(), ne * 8)) || (e = up(d_epos, ex_pos.data(), ne * 8)) ||
+        if ((e = up(db, hb.data(), o_end)) ||
+            (ne && ((e = up(d_ero, ex_ro.data(), ne * 8)) ||
+                    (e = up(d_epos, ex_pos.data(), ne * 8)) ||
                     (e = up(d_erl, ex_rl.data(), ne * 4)))))
             return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
         if (ne) {
@@ -993,22 +1052,22 @@ kira_status Driver::run() {
     R.phase_ms[3] = ms_since(t0);
 
     t0 = Clock::now();
-    std::vector<int64_t> first;
-    sort_by_read(first);
+    std::vector<int64_t> first, perm;
+    group_by_read(first, perm);
 
     // XS: the two-tier competitor's affine score, else the chain-score proxy
     // (:1215-1250)
     for (int32_t r = 0; r < n; r++) {
         if (first[r + 1] - first[r] != 1) continue;
         if (has_comp[r]) {
-            kira_stage4_aln &a = recs[first[r]].a;
+            kira_stage4_aln &a = recs[perm[first[r]]].a;
             a.has_xs = 1;
             a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
             continue;
         }
         const int32_t cb = chain_best[r], cs = chain_second[r];
         if (cb <= 0 || cs <= 0) continue;
-        kira_stage4_aln &a = recs[first[r]].a;
+        kira_stage4_aln &a = recs[perm[first[r]]].a;
         if (a.has_xs) continue;
         const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
         if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
@@ -1025,57 +1084,77 @@ kira_status Driver::run() {
             int64_t chain;
             int32_t job;
         };
+        // threads take read ranges; their picks are joined in read order
+        struct Part {
+            std::vector<Pick> all;
+            std::vector<Job> sj;
+        };
+        std::vector<Part> parts((size_t)host_threads());
+        parallel_chunks(n, 4096, [&](int t, int64_t rb, int64_t re) {
+            Part &P = parts[(size_t)t];
+            std::vector<int64_t> picks;
+            for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
+                const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+                if (first[r + 1] == first[r] || nc < 2) continue;
+                const kira_stage4_aln &p = recs[perm[first[r]]].a;
+                const kira_chain *cl = chains + c0;
+                // pick_supplementary_chains (split_read.rs:60-125)
+                const int32_t ps = std::max(cl[0].score, 1);
+                const int32_t min_score = f2i(ceilf((float)ps * kSuppMinScoreRatio));
+                uint32_t L = 0;
+                for (int32_t k = 0; k < p.n_cigar; k++) {
+                    uint32_t op = R.cigar[p.cigar_off + k];
+                    if ((op & 15u) == kOpM || (op & 15u) == kOpI || (op & 15u) == kOpS)
+                        L += op >> 4;
+                }
+                auto pi = original_interval(p.read_start, p.read_end, p.is_rev != 0, L);
+                picks.clear();
+                for (int64_t k = 1; k < nc; k++) {
+                    const kira_chain &ch = cl[k];
+                    if (ch.score < min_score) break;
+                    if (usub(ch.read_end, ch.read_start) < kSuppMinReadSpan) continue;
+                    auto ci = original_interval(ch.read_start, ch.read_end, ch.strand == 1, L);
+                    if (read_overlap_pct(pi.first, pi.second, ci.first, ci.second) >
+                        kSuppMaxOverlapPct)
+                        continue;
+                    bool dup = false;
+                    for (int64_t q : picks) {
+                        const kira_chain &pc = chains[q];
+                        auto qi =
+                            original_interval(pc.read_start, pc.read_end, pc.strand == 1, L);
+                        if (read_overlap_pct(qi.first, qi.second, ci.first, ci.second) >
+                            kSuppMaxOverlapPct) {
+                            dup = true;
+                            break;
+                        }
+                    }
+                    if (dup) continue;
+                    picks.push_back(c0 + k);
+                    if (picks.size() >= kMaxSuppPerRead) break;
+                }
+                for (int64_t q : picks) {
+                    const kira_chain &ch = chains[q];
+                    // align_oriented_chain_with_meta: exact shortcut, else banded SW
+                    if (exact_candidate(r, ch) &&
+                        memcmp(oriented(r, ch.strand == 1), contig(ch.ref_id) + ch.ref_start,
+                               (size_t)read_len[r]) == 0) {
+                        P.all.push_back(Pick{r, q, -1});
+                        continue;
+                    }
+                    P.all.push_back(Pick{r, q, (int32_t)P.sj.size()});
+                    P.sj.push_back(clamp_job(r, q, INT32_MIN / 8));
+                }
+            }
+        });
         std::vector<Pick> all;
         std::vector<Job> sj;
-        std::vector<int64_t> picks;
-        for (int32_t r = 0; r < n; r++) {
-            const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
-            if (first[r + 1] == first[r] || nc < 2) continue;
-            const kira_stage4_aln &p = recs[first[r]].a;
-            const kira_chain *cl = chains + c0;
-            // pick_supplementary_chains (split_read.rs:60-125)
-            const int32_t ps = std::max(cl[0].score, 1);
-            const int32_t min_score = f2i(ceilf((float)ps * kSuppMinScoreRatio));
-            uint32_t L = 0;
-            for (int32_t k = 0; k < p.n_cigar; k++) {
-                uint32_t op = R.cigar[p.cigar_off + k];
-                if ((op & 15u) == kOpM || (op & 15u) == kOpI || (op & 15u) == kOpS) L += op >> 4;
+        for (Part &P : parts) {
+            const int32_t base = (int32_t)sj.size();
+            for (Pick x : P.all) {
+                if (x.job >= 0) x.job += base;
+                all.push_back(x);
             }
-            auto pi = original_interval(p.read_start, p.read_end, p.is_rev != 0, L);
-            picks.clear();
-            for (int64_t k = 1; k < nc; k++) {
-                const kira_chain &ch = cl[k];
-                if (ch.score < min_score) break;
-                if (usub(ch.read_end, ch.read_start) < kSuppMinReadSpan) continue;
-                auto ci = original_interval(ch.read_start, ch.read_end, ch.strand == 1, L);
-                if (read_overlap_pct(pi.first, pi.second, ci.first, ci.second) > kSuppMaxOverlapPct)
-                    continue;
-                bool dup = false;
-                for (int64_t q : picks) {
-                    const kira_chain &pc = chains[q];
-                    auto qi = original_interval(pc.read_start, pc.read_end, pc.strand == 1, L);
-                    if (read_overlap_pct(qi.first, qi.second, ci.first, ci.second) >
-                        kSuppMaxOverlapPct) {
-                        dup = true;
-                        break;
-                    }
-                }
-                if (dup) continue;
-                picks.push_back(c0 + k);
-                if (picks.size() >= kMaxSuppPerRead) break;
-            }
-            for (int64_t q : picks) {
-                const kira_chain &ch = chains[q];
-                // align_oriented_chain_with_meta: exact shortcut, else banded SW
-                if (exact_candidate(r, ch) &&
-                    memcmp(oriented(r, ch.strand == 1), contig(ch.ref_id) + ch.ref_start,
-                           (size_t)read_len[r]) == 0) {
-                    all.push_back(Pick{r, q, -1});
-                    continue;
-                }
-                all.push_back(Pick{r, q, (int32_t)sj.size()});
-                sj.push_back(clamp_job(r, q, INT32_MIN / 8));
-            }
+            sj.insert(sj.end(), P.sj.begin(), P.sj.end());
         }
         std::vector<kira_sw_result> res;
         std::vector<uint32_t> cig;
@@ -1096,12 +1175,14 @@ kira_status Driver::run() {
             A.recs.push_back(Rec{p.read, a});
         });
         st.supplementary += (int64_t)all.size();
-        sort_by_read(first);
+        group_by_read(first, perm);
     }
 
     R.aln_off = first;
     R.alns.resize(recs.size());
-    for (size_t k = 0; k < recs.size(); k++) R.alns[k] = recs[k].a;
+    parallel_chunks((int64_t)recs.size(), 16384, [&](int, int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++) R.alns[k] = recs[perm[k]].a;
+    });
     for (int32_t r = 0; r < n; r++) st.unmapped += R.aln_off[r + 1] == R.aln_off[r] ? 1 : 0;
     R.phase_ms[4] = ms_since(t0);
     return KIRA_OK;
