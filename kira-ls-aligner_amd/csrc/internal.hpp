@@ -132,7 +132,7 @@ hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, h
 
 // Host-side packetisation of a banded SW batch (jobs sorted by band class
 // and read length into packets, trace / meta slab offsets, CIGAR arena).
-constexpr int kSwClasses = 16;
+constexpr int kSwClasses = 17;
 struct SwPlanLayout {
     int32_t n = 0, n_packets = 0;
     int32_t class_first[kSwClasses] = {}, class_count[kSwClasses] = {};

@@ -472,7 +472,8 @@ __global__ void zero_coop_counter_kernel(int32_t *c) { *c = 0; }
 hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, hipStream_t s) {
     zero_coop_counter_kernel<<<1, 1, 0, s>>>(a.counter);
     if (a.n_packets > 0) {
-        if (G == 16 && P == 9) banded_sw_coop_kernel<9, 16><<<n_waves, 64, 0, s>>>(a);
+        if (G == 16 && P == 5) banded_sw_coop_kernel<5, 16><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 16 && P == 9) banded_sw_coop_kernel<9, 16><<<n_waves, 64, 0, s>>>(a);
         else if (G == 16 && P == 17) banded_sw_coop_kernel<17, 16><<<n_waves, 64, 0, s>>>(a);
         else if (G == 32 && P == 13) banded_sw_coop_kernel<13, 32><<<n_waves, 64, 0, s>>>(a);
         else if (G == 32 && P == 17) banded_sw_coop_kernel<17, 32><<<n_waves, 64, 0, s>>>(a);

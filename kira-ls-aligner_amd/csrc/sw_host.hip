@@ -26,8 +26,8 @@ constexpr int kNumLane = 6;
 struct CoopShape {
     int p, g;
 };
-constexpr CoopShape kCoop[] = {{9, 16}, {17, 16}, {13, 32}, {17, 32}};
-constexpr int kNumCoop = 4;
+constexpr CoopShape kCoop[] = {{5, 16}, {9, 16}, {17, 16}, {13, 32}, {17, 32}};
+constexpr int kNumCoop = 5;
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
@@ -141,18 +141,21 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
                                  : "band wider than 271";
         cls[k] = 2 * b + (pack ? 1 : 0);
     }
-    // A lane-per-job class runs one wave per 64 jobs, and C = 129 fits one
-    // wave per SIMD, so a class of fewer than 64 x 1024 jobs leaves SIMDs
-    // idle.  Its jobs are then worth more as 16-lane jobs (16x the waves):
-    // bands 33-64 fit the 16 x 9 positions of the smallest 16-lane class.
+    // A lane-per-job class runs one wave per 64 jobs, so a class of fewer
+    // than 64 x 1024 jobs leaves SIMDs idle, and a lone wave on a SIMD waits
+    // on each dependent instruction and LDS read of its row loop (37k jobs
+    // of band 32 took 4.8 ms lane per job, 2.3 ms as (9, 16) jobs).  Such
+    // jobs run as 16-lane jobs (16x the waves) in the smallest 16-lane
+    // class that holds the band: (5, 16) up to band 39, else (9, 16).
     {
         int count[kNumClasses] = {};
         for (int32_t k = 0; k < n; k++) count[cls[k]]++;
-        const int c129 = 2 * 4, c129p = 2 * 4 + 1;
+        const int c65 = 2 * 3, c65p = 2 * 3 + 1, c129 = 2 * 4, c129p = 2 * 4 + 1;
         for (int32_t k = 0; k < n; k++) {
             const int c = cls[k];
-            if ((c == c129 || c == c129p) && count[c] < kMinLaneJobs && coop_ok[k])
-                cls[k] = 2 * kNumLane;  // (9, 16)
+            if ((c == c65 || c == c65p || c == c129 || c == c129p) && count[c] < kMinLaneJobs &&
+                coop_ok[k])
+                cls[k] = coop_class_of(std::max(band ? band[k] : cfg.bandwidth, 1));
         }
     }
     // packets: jobs sorted by (band class, read length descending)

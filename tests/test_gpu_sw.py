@@ -152,21 +152,22 @@ def test_wide_band_limits(pkg, ctx):
     assert r == want
 
 
-def test_large_batch_keeps_lane_per_job_classes(pkg, ctx):
-    """A class of >= 65536 jobs of band 33-64 runs lane per job (smaller ones
-    move to 16-lane jobs); both must agree with the oracle."""
+@pytest.mark.parametrize("widths", [(33, 48, 64), (17, 24, 32)])
+def test_large_batch_keeps_lane_per_job_classes(pkg, ctx, widths):
+    """A class of >= 65536 jobs of band 17-32 or 33-64 runs lane per job
+    (smaller ones move to 16-lane jobs); both must agree with the oracle."""
     import numpy as np
-    rng = random.Random(23)
+    rng = random.Random(23 + widths[0])
     jobs = []
     for k in range(66000):
-        w = rng.choice([33, 48, 64])
+        w = rng.choice(widths)
         L = rng.randrange(40, 80)
         flank = 2 * w
         g = bytes(rng.choice(b"ACGT") for _ in range(L + 2 * flank))
         read = mutate(rng, g[flank:flank + L], 0.03, 0.01) or b"A"
         jobs.append((read, g, flank, ABORT_NONE, w))
     c = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
-    print(f"[sw] large batch {len(jobs)} jobs", flush=True)
+    print(f"[sw] large batch {len(jobs)} jobs, bands {widths}", flush=True)
     got = pkg.banded_sw_batch(ctx, jobs, c, pkg.sw_knobs())
     reads = np.frombuffer(b"".join(j[0] for j in jobs), np.uint8)
     refs = np.frombuffer(b"".join(j[1] for j in jobs), np.uint8)
