@@ -313,26 +313,40 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             return fail(ctx, KIRA_ERR_DRIVER, "fast path upload", e);
         d_reads = d;
     }
-    // uploads a list of jobs' (read offset, read len, text offset, text len,
-    // extra) arrays
+    // one launch's job lists (read offset, read len, text offset, text len,
+    // two extras), written straight into pinned buffer 3 of the context side
+    // by side and uploaded with one copy; every phase has synchronised before
+    // the next one rewrites them
     struct Lists {
-        std::vector<int64_t> roff, toff;
-        std::vector<int32_t> rlen, tlen, x, y;
-        void clear() {
-            roff.clear();
-            toff.clear();
-            rlen.clear();
-            tlen.clear();
-            x.clear();
-            y.clear();
+        int64_t *roff = nullptr, *toff = nullptr;
+        int32_t *rlen = nullptr, *tlen = nullptr, *x = nullptr, *y = nullptr;
+        uint8_t *base = nullptr;
+        size_t n = 0, bytes = 0, a8 = 0, a4 = 0;
+        hipError_t begin(kira_ctx *cx, size_t m) {
+            a8 = (m * 8 + 255) & ~(size_t)255;
+            a4 = (m * 4 + 255) & ~(size_t)255;
+            bytes = 2 * a8 + 4 * a4;
+            void *p = nullptr;
+            hipError_t r = kira::pinned_buf(cx, 3, bytes, &p);
+            if (r != hipSuccess) return r;
+            base = (uint8_t *)p;
+            roff = (int64_t *)base;
+            toff = (int64_t *)(base + a8);
+            rlen = (int32_t *)(base + 2 * a8);
+            tlen = (int32_t *)(base + 2 * a8 + a4);
+            x = (int32_t *)(base + 2 * a8 + 2 * a4);
+            y = (int32_t *)(base + 2 * a8 + 3 * a4);
+            n = 0;
+            return hipSuccess;
         }
         void add(int64_t ro, int32_t rl, int64_t to, int32_t tn, int32_t a = 0, int32_t b = 0) {
-            roff.push_back(ro);
-            rlen.push_back(rl);
-            toff.push_back(to);
-            tlen.push_back(tn);
-            x.push_back(a);
-            y.push_back(b);
+            roff[n] = ro;
+            rlen[n] = rl;
+            toff[n] = to;
+            tlen[n] = tn;
+            x[n] = a;
+            y[n] = b;
+            n++;
         }
     } lists;
     struct DevLists {
@@ -340,20 +354,17 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         int32_t *rlen = nullptr, *tlen = nullptr, *x = nullptr, *y = nullptr;
     };
     auto upload = [&](DevLists &d) -> hipError_t {
-        const size_t m = lists.roff.size();
+        uint8_t *db = nullptr;
         hipError_t r;
-        if ((r = dm.alloc(&d.roff, m)) || (r = dm.alloc(&d.toff, m)) || (r = dm.alloc(&d.rlen, m)) ||
-            (r = dm.alloc(&d.tlen, m)) || (r = dm.alloc(&d.x, m)) || (r = dm.alloc(&d.y, m)))
-            return r;
-        if (m == 0) return hipSuccess;
-        if ((r = hipMemcpyAsync(d.roff, lists.roff.data(), 8 * m, hipMemcpyHostToDevice, st)) ||
-            (r = hipMemcpyAsync(d.toff, lists.toff.data(), 8 * m, hipMemcpyHostToDevice, st)) ||
-            (r = hipMemcpyAsync(d.rlen, lists.rlen.data(), 4 * m, hipMemcpyHostToDevice, st)) ||
-            (r = hipMemcpyAsync(d.tlen, lists.tlen.data(), 4 * m, hipMemcpyHostToDevice, st)) ||
-            (r = hipMemcpyAsync(d.x, lists.x.data(), 4 * m, hipMemcpyHostToDevice, st)) ||
-            (r = hipMemcpyAsync(d.y, lists.y.data(), 4 * m, hipMemcpyHostToDevice, st)))
-            return r;
-        return hipSuccess;
+        if ((r = dm.alloc(&db, lists.bytes))) return r;
+        d.roff = (int64_t *)db;
+        d.toff = (int64_t *)(db + lists.a8);
+        d.rlen = (int32_t *)(db + 2 * lists.a8);
+        d.tlen = (int32_t *)(db + 2 * lists.a8 + lists.a4);
+        d.x = (int32_t *)(db + 2 * lists.a8 + 2 * lists.a4);
+        d.y = (int32_t *)(db + 2 * lists.a8 + 3 * lists.a4);
+        if (lists.n == 0) return hipSuccess;
+        return hipMemcpyAsync(db, lists.base, lists.bytes, hipMemcpyHostToDevice, st);
     };
     auto text_of = [&](const Job &j) { return contig_off[j.contig] + j.ws; };
     try {
@@ -361,14 +372,15 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         auto t_sp = std::chrono::steady_clock::now();
         std::vector<int32_t> cand;
         if (kn.algo == 0 || kn.algo == 1) {
-            lists.clear();
+            if ((e = lists.begin(ctx, jobs.size())))
+                return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
             int32_t max_lt = 0;
             for (int32_t q = 0; q < (int32_t)jobs.size(); q++) {
                 const Job &j = jobs[q];
                 lists.add(read_off[j.k], j.L, text_of(j), j.tl);
                 max_lt = std::max(max_lt, j.L + j.tl);
             }
-            const int32_t m = (int32_t)lists.roff.size();
+            const int32_t m = (int32_t)lists.n;
             if (m) {
                 DevLists d;
                 int32_t *d_out = nullptr;
@@ -400,7 +412,8 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         // ---- Myers reject (router.rs:196-202)
         std::vector<int32_t> wfa_jobs;
         if (!cand.empty() && pen_ok) {
-            lists.clear();
+            if ((e = lists.begin(ctx, cand.size())))
+                return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
             for (int32_t q : cand) {
                 const Job &j = jobs[q];
                 lists.add(read_off[j.k], j.L, text_of(j), j.tl,
@@ -413,7 +426,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             if ((e = upload(d)) || (e = dm.alloc(&d_found, m)) || (e = dm.alloc(&d_dist, m)) ||
                 (e = dm.alloc(&d_end, m)))
                 return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path Myers", e);
-            kira_status rs = kira::myers_dev(ctx, m, d_reads, d.roff, d.rlen, lists.rlen.data(),
+            kira_status rs = kira::myers_dev(ctx, m, d_reads, d.roff, d.rlen, lists.rlen,
                                              g->ascii, d.toff, d.tlen, d.x, d_found, d_dist, d_end);
             if (rs != KIRA_OK) return rs;
             std::vector<uint8_t> found(m);
@@ -432,7 +445,8 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                            const std::vector<int32_t> &tlens, const std::vector<int32_t> &budget,
                            const std::vector<int32_t> &tbf, std::vector<int32_t> &wres,
                            std::vector<uint32_t> &wcig, std::vector<int64_t> &woff) -> kira_status {
-            lists.clear();
+            if ((e = lists.begin(ctx, list.size())))
+                return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
             for (size_t i = 0; i < list.size(); i++) {
                 const Job &j = jobs[list[i]];
                 lists.add(read_off[j.k], j.L, tofs[i], tlens[i], budget[i], tbf[i]);
@@ -449,10 +463,10 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             wj.txt_len = d.tlen;
             wj.max_score = d.x;
             wj.tbf = d.y;
-            wj.h_pat_len = lists.rlen.data();
-            wj.h_txt_len = lists.tlen.data();
-            wj.h_max_score = lists.x.data();
-            wj.h_tbf = lists.y.data();
+            wj.h_pat_len = lists.rlen;
+            wj.h_txt_len = lists.tlen;
+            wj.h_max_score = lists.x;
+            wj.h_tbf = lists.y;
             return kira::wfa_dev(ctx, wj, pen, drop, true, wres, wcig, woff);
         };
         // the CIGARs of both passes stay alive until the records are built

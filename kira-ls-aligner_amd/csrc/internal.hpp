@@ -319,7 +319,7 @@ struct DevMem {
 
 // A typed view of pinned staging buffer `slot` of the context.  Slots in
 // use: 0 the stage-4 oriented reads, 1 its prefilter outcomes, 2 its
-// prefilter job arrays.
+// prefilter job arrays, 3 the fast DP cascade's job lists.
 template <class T>
 struct Pinned {
     T *p = nullptr;
