@@ -54,8 +54,9 @@ void kira_ctx_set_stream(kira_ctx *ctx, void *stream);
 void kira_ctx_use_own_stream(kira_ctx *ctx);
 
 /* CudaBackend::release_buffers() — src/cuda/backend.rs:252-261: frees the
- * context's device scratch, including the temporaries the entry points keep
- * cached between calls (so a warm context allocates nothing per call). */
+ * context's device scratch and pinned host staging buffers, including the
+ * device temporaries the entry points keep cached between calls (a context
+ * warm for the batch sizes in use makes no hipMalloc / hipFree per call). */
 void kira_release_buffers(kira_ctx *ctx);
 
 /* ------------------------------------------------------------------ sieve */
