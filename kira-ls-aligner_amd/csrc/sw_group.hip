@@ -54,19 +54,34 @@ __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int3
             Eup[c] = neg;
         }
         int32_t diag_row_prev = 0;  // H[i-1][j0-1]
-        for (int32_t i = 1; i <= L; i++) {
-            int32_t hl, fl;
+        // row i's left boundary (H, F), its statistics so far and its read
+        // byte, loaded two rows ahead: the loop would otherwise wait on one
+        // dependent global load per row (each row stores only its own entry,
+        // so the early loads read what the previous strip wrote)
+        auto ld = [&](int32_t i, int2 &b, int2 &r, uint8_t &q) {
+            if (i > L) return;
             if (st == 0) {
-                hl = 0;  // column 0 is H = 0, F starts at the sentinel
-                fl = neg;
+                b = make_int2(0, neg);  // column 0 is H = 0, F starts at the sentinel
             } else {
-                int2 b = bnd[(int64_t)(i - 1) * 64 + lane];
-                hl = b.x;
-                fl = b.y;
+                b = bnd[(int64_t)(i - 1) * 64 + lane];
+                r = rstat[(int64_t)(i - 1) * 64 + lane];
             }
+            q = read[i - 1];
+        };
+        int2 b1 = make_int2(0, neg), b2 = b1, r1 = make_int2(0, 0), r2 = r1;
+        uint8_t q1 = 0, q2 = 0;
+        ld(1, b1, r1, q1);
+        ld(2, b2, r2, q2);
+        for (int32_t i = 1; i <= L; i++) {
+            int32_t hl = b1.x, fl = b1.y;
+            const int2 prev_stat = r1;
+            const uint8_t qb = q1;
+            b1 = b2;
+            r1 = r2;
+            q1 = q2;
+            ld(i + 2, b2, r2, q2);
             int32_t diag = diag_row_prev;
             diag_row_prev = hl;
-            const uint8_t qb = read[i - 1];
             int32_t rmax = 0, rarg = 0, rkey = 0;
             uint32_t w0 = 0, w1 = 0;
 #pragma unroll
@@ -116,12 +131,7 @@ __host__ __device__ __forceinline__ void group_dp_job(const GroupLaunch &a, int3
             tr[64] = w1;
             bnd[(int64_t)(i - 1) * 64 + lane] = make_int2(hl, fl);
             int2 *rs = rstat + (int64_t)(i - 1) * 64 + lane;
-            if (st == 0) {
-                *rs = make_int2(rmax, rarg);
-            } else {
-                int2 prev = *rs;
-                if (rmax > prev.x) *rs = make_int2(rmax, rarg);
-            }
+            if (st == 0 || rmax > prev_stat.x) *rs = make_int2(rmax, rarg);
         }
     }
 }
