@@ -407,43 +407,52 @@ kira_status Driver::run() {
         int32_t fast;    // index into tt_res, or -1
     };
     std::vector<Cand> cand;
+    // per read, in parallel: the selection bounds and the number of two-tier
+    // candidates (cand_off[r + 1]); then, after the prefix sum, the candidates
     std::vector<int64_t> cand_off(n + 1, 0);
-    for (int32_t r = 0; r < n; r++) {
-        cand_off[r] = (int64_t)cand.size();
-        const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
-        st.chains_total += nc;
-        if (nc == 0) continue;
-        const kira_chain *cl = chains + c0;
-        const int32_t best = cl[0].score;
-        int32_t min_score = best > 0 ? f2i(ceilf((float)best * c.min_chain_ratio)) : best;
-        const int32_t second = nc > 1 ? cl[1].score : 0;
-        const int32_t margin = nc > 1 ? wrap_i32((int64_t)best - second) : INT32_MAX;
-        chain_best[r] = best;
-        chain_second[r] = second;
-        margin_of[r] = margin;
-        const int32_t L = read_len[r];
-        const bool short_read = L <= 300;
-        const bool ambiguous =
-            nc > 1 && (margin <= std::max(best / ambig_div, 8) || rmo_of(r) > 1);
-        const bool weak = nc > 1 && best < wrap_i32((int64_t)L * 2) / 3;
-        if (short_read && weak && c.twotier) min_score = std::min(min_score, best / 5);
-        min_score_of[r] = min_score;
-        topk_of[r] = short_read ? (c.short_dp_topk >= 1 ? c.short_dp_topk : (ambiguous ? 2 : 1))
-                                : std::max(c.dp_topk, 1);
-        if (!(c.twotier && short_read && (ambiguous || weak))) continue;
-        const int32_t kmax = c.twotier_k >= 2 ? c.twotier_k : 5;
-        int32_t np = 0;
-        for (int64_t k = 0; k < nc && np < kmax; k++)
-            if (cl[k].score >= min_score) np++;
-        if (np < 2) continue;
-        np = 0;
-        for (int64_t k = 0; k < nc && np < kmax; k++)
-            if (cl[k].score >= min_score) {
-                cand.push_back(Cand{r, c0 + k, INT32_MIN, -1});
-                np++;
-            }
-    }
-    cand_off[n] = (int64_t)cand.size();
+    const int32_t kmax = c.twotier_k >= 2 ? c.twotier_k : 5;
+    parallel_chunks(n, 4096, [&](int, int64_t rb, int64_t re) {
+        for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
+            const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+            if (nc == 0) continue;
+            const kira_chain *cl = chains + c0;
+            const int32_t best = cl[0].score;
+            int32_t min_score = best > 0 ? f2i(ceilf((float)best * c.min_chain_ratio)) : best;
+            const int32_t second = nc > 1 ? cl[1].score : 0;
+            const int32_t margin = nc > 1 ? wrap_i32((int64_t)best - second) : INT32_MAX;
+            chain_best[r] = best;
+            chain_second[r] = second;
+            margin_of[r] = margin;
+            const int32_t L = read_len[r];
+            const bool short_read = L <= 300;
+            const bool ambiguous =
+                nc > 1 && (margin <= std::max(best / ambig_div, 8) || rmo_of(r) > 1);
+            const bool weak = nc > 1 && best < wrap_i32((int64_t)L * 2) / 3;
+            if (short_read && weak && c.twotier) min_score = std::min(min_score, best / 5);
+            min_score_of[r] = min_score;
+            topk_of[r] = short_read
+                             ? (c.short_dp_topk >= 1 ? c.short_dp_topk : (ambiguous ? 2 : 1))
+                             : std::max(c.dp_topk, 1);
+            if (!(c.twotier && short_read && (ambiguous || weak))) continue;
+            int32_t np = 0;
+            for (int64_t k = 0; k < nc && np < kmax; k++)
+                if (cl[k].score >= min_score) np++;
+            if (np >= 2) cand_off[r + 1] = np;
+        }
+    });
+    st.chains_total += chain_off[n] - chain_off[0];
+    for (int32_t r = 0; r < n; r++) cand_off[r + 1] += cand_off[r];
+    cand.resize((size_t)cand_off[n]);
+    parallel_chunks(n, 4096, [&](int, int64_t rb, int64_t re) {
+        for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
+            int64_t o = cand_off[r];
+            if (o == cand_off[r + 1]) continue;
+            const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
+            for (int64_t k = 0; k < nc && o < cand_off[r + 1]; k++)
+                if (chains[c0 + k].score >= min_score_of[r])
+                    cand[(size_t)o++] = Cand{r, c0 + k, INT32_MIN, -1};
+        }
+    });
     // exact copies score A * L (exact_match_exists, mod.rs:818-832); the rest
     // go through the fast cascade together
     std::vector<kira_fast_aln> tt_res;
@@ -612,22 +621,27 @@ kira_status Driver::run() {
         uint8_t *top = hb.data() + o_top;
         std::vector<int64_t> ex_ro, ex_pos;
         std::vector<int32_t> ex_rl, ex_sel;
-        for (int32_t k = 0; k < ns; k++) {
-            const Sel &x = sels[k];
-            const kira_chain &ch = chains[x.chain];
-            ro[k] = oriented_off(x.read, ch.strand == 1);
-            rl[k] = read_len[x.read];
-            fo[k] = contig_off[ch.ref_id];
-            fl[k] = contig_len[ch.ref_id];
-            sp[k] = kira_span{ch.ref_start, ch.ref_end, ch.read_start, ch.read_end};
-            top[k] = x.top1;
-            mg[k] = x.margin;
-            if (exact_candidate(x.read, ch)) {
-                ex_sel.push_back(k);
-                ex_ro.push_back(ro[k]);
-                ex_rl.push_back(rl[k]);
-                ex_pos.push_back(fo[k] + ch.ref_start);
+        std::vector<uint8_t> is_ex((size_t)ns);
+        parallel_chunks(ns, 16384, [&](int, int64_t b0, int64_t e0) {
+            for (int64_t k = b0; k < e0; k++) {
+                const Sel &x = sels[k];
+                const kira_chain &ch = chains[x.chain];
+                ro[k] = oriented_off(x.read, ch.strand == 1);
+                rl[k] = read_len[x.read];
+                fo[k] = contig_off[ch.ref_id];
+                fl[k] = contig_len[ch.ref_id];
+                sp[k] = kira_span{ch.ref_start, ch.ref_end, ch.read_start, ch.read_end};
+                top[k] = x.top1;
+                mg[k] = x.margin;
+                is_ex[k] = exact_candidate(x.read, ch) ? 1 : 0;
             }
+        });
+        for (int32_t k = 0; k < ns; k++) {
+            if (!is_ex[k]) continue;
+            ex_sel.push_back(k);
+            ex_ro.push_back(ro[k]);
+            ex_rl.push_back(rl[k]);
+            ex_pos.push_back(fo[k] + chains[sels[k].chain].ref_start);
         }
         const int32_t ne = (int32_t)ex_sel.size();
         DevMem dm(ctx);
