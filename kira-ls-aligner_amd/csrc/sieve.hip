@@ -54,6 +54,18 @@ __device__ __forceinline__ void classify(uint8_t c, uint32_t &lo, uint32_t &hi, 
     lc = letter ? 1u : 0u;
 }
 
+// Bit 7 of each byte of x: set when the byte is not zero (no carry leaves a
+// byte, so the test is exact per byte).
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t x) {
+    return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+
+// Bits 0, 8, 16 and 24 of x to bits 0-3: the multiplier places bit 8 k at
+// bit 21 + k and every other partial product outside bits 21-24.
+__device__ __forceinline__ uint32_t gather4(uint32_t x) {
+    return (((x & 0x01010101u) * 0x00204081u) >> 21) & 0xFu;
+}
+
 // One thread per 32-base word of an ASCII sequence; each wave also writes
 // the 64 bits of the non-ACGT word bitmap its words cover.
 __global__ void pack_ascii_planes_kernel(const uint8_t *__restrict__ seq, int64_t n,
@@ -177,16 +189,25 @@ __global__ void pack_reads_ascii_vec_kernel(int32_t n, const uint8_t *__restrict
         const uint4 v0 = src[0], v1 = src[1];
         const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
         const int nb = L - k * 32 < 32 ? L - k * 32 : 32;
+        // classify() on four bytes at once: with u = c & 0xDF (upper case),
+        // A C G T are 0x41 0x43 0x47 0x54, so code bit 1 is bit 2 of the
+        // byte and code bit 0 is bit 1 ^ bit 2; bit 5 marks lower case
 #pragma unroll
-        for (int b = 0; b < 32; b++) {
-            if (b >= nb) break;
-            uint32_t a0, b0, c0, d0;
-            classify((uint8_t)(w[b >> 2] >> (8 * (b & 3))), a0, b0, c0, d0);
-            lo |= a0 << b;
-            hi |= b0 << b;
-            inv |= c0 << b;
-            lc |= d0 << b;
+        for (int q = 0; q < 8; q++) {
+            const uint32_t c = w[q], u = c & 0xDFDFDFDFu;
+            const uint32_t z = nz_bytes(u ^ 0x41414141u) & nz_bytes(u ^ 0x43434343u) &
+                               nz_bytes(u ^ 0x47474747u) & nz_bytes(u ^ 0x54545454u);
+            const uint32_t ok = (~z & 0x80808080u) >> 7;  // bit 0 of each ACGT byte
+            lo |= gather4(((c >> 1) ^ (c >> 2)) & ok) << (4 * q);
+            hi |= gather4((c >> 2) & ok) << (4 * q);
+            inv |= gather4(~ok & 0x01010101u) << (4 * q);
+            lc |= gather4((c >> 5) & ok) << (4 * q);
         }
+        const uint32_t m = nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);  // bases past L
+        lo &= m;
+        hi &= m;
+        inv &= m;
+        lc &= m;
     }
     uint32_t *dst = planes + (int64_t)r * kReadWords;
     dst[k] = lo;
