@@ -70,7 +70,7 @@ hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stri
                                    uint8_t *valid, hipStream_t s);
 hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stride,
                                   const int32_t *lens, uint32_t *planes, hipStream_t s);
-hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s);
+hipError_t launch_sieve(const SieveLaunch &a, int read_words, hipStream_t s);
 
 // Banded SW: results per job, kSwResWords int32 each: score, read_end,
 // ref_end, early_abort, ref_start, read_start, n_cigar, int16-overflow flag,

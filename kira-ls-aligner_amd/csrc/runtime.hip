@@ -248,7 +248,7 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
     a.g_nz = nullptr;
     a.g_bases = phase_bases;
     int rw = std::max(1, (max_len + 31) / 32);
-    if ((e = kira::launch_sieve(a, rw, false, s)))
+    if ((e = kira::launch_sieve(a, rw, s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "sieve launch", e);
     if ((e = hipMemcpyAsync(out_shifts, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, s)) ||
         (e = hipMemcpyAsync(out_mismatches, d_out + n, (size_t)n * 8, hipMemcpyDeviceToHost, s)))
@@ -362,7 +362,7 @@ kira_status kira_sieve_genome_dev(kira_ctx *ctx, const kira_genome *g,
     // promises every read of the batch is at most this long (256 bases).
     int rw = b->max_read_len <= 0 ? 8 : (b->max_read_len + 31) / 32;
     if (rw > 8) return fail(ctx, KIRA_ERR_READ_TOO_LONG, "read longer than 256 bases");
-    hipError_t e = kira::launch_sieve(a, rw, false, ctx->stream);
+    hipError_t e = kira::launch_sieve(a, rw, ctx->stream);
     if (e != hipSuccess) return fail(ctx, KIRA_ERR_RUNTIME, "sieve launch", e);
     return KIRA_OK;
 }

@@ -243,7 +243,7 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
         a.g_nz = genome->planes.nz;
         a.g_bases = genome->planes.n_bases;
         int rw = std::max(1, std::min(8, (r.max_sieve_len + 31) / 32));
-        if ((e = kira::launch_sieve(a, rw, false, s))) return e;
+        if ((e = kira::launch_sieve(a, rw, s))) return e;
     }
     if (r.nw > 0) {
         kira::SwPlanDev d = kira::sw_plan_bind(r.plan, din + L.tables, dscr + L.slabs);

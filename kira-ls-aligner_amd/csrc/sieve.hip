@@ -458,7 +458,7 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
 }
 
 template <int RW>
-__global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) {
+__global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.n_jobs) return;
     int32_t r = a.job_read ? a.job_read[j] : (int32_t)(j / a.jobs_per_read);
@@ -510,7 +510,6 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a, int uniform) 
     }
     uint32_t nt = (uint32_t)(W - L + 1);
     uint32_t gbm, gbt, gsm;
-    (void)uniform;
     if (L > 32 * (RW - 1))
         sieve_core<RW, false>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
     else
@@ -573,19 +572,18 @@ hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stri
     return hipGetLastError();
 }
 
-hipError_t launch_sieve(const SieveLaunch &a, int read_words, bool uniform, hipStream_t s) {
+hipError_t launch_sieve(const SieveLaunch &a, int read_words, hipStream_t s) {
     if (a.n_jobs == 0) return hipSuccess;
     unsigned g = grid_for(a.n_jobs, 256);
-    int u = uniform ? 1 : 0;
     switch (read_words) {
-    case 1: sieve_kernel<1><<<g, 256, 0, s>>>(a, u); break;
-    case 2: sieve_kernel<2><<<g, 256, 0, s>>>(a, u); break;
-    case 3: sieve_kernel<3><<<g, 256, 0, s>>>(a, u); break;
-    case 4: sieve_kernel<4><<<g, 256, 0, s>>>(a, u); break;
-    case 5: sieve_kernel<5><<<g, 256, 0, s>>>(a, u); break;
-    case 6: sieve_kernel<6><<<g, 256, 0, s>>>(a, u); break;
-    case 7: sieve_kernel<7><<<g, 256, 0, s>>>(a, u); break;
-    case 8: sieve_kernel<8><<<g, 256, 0, s>>>(a, u); break;
+    case 1: sieve_kernel<1><<<g, 256, 0, s>>>(a); break;
+    case 2: sieve_kernel<2><<<g, 256, 0, s>>>(a); break;
+    case 3: sieve_kernel<3><<<g, 256, 0, s>>>(a); break;
+    case 4: sieve_kernel<4><<<g, 256, 0, s>>>(a); break;
+    case 5: sieve_kernel<5><<<g, 256, 0, s>>>(a); break;
+    case 6: sieve_kernel<6><<<g, 256, 0, s>>>(a); break;
+    case 7: sieve_kernel<7><<<g, 256, 0, s>>>(a); break;
+    case 8: sieve_kernel<8><<<g, 256, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1,5 +1,6 @@
 """GPU parity of align_batch_simd's SW (lane groups + scalar fallbacks) vs the oracle."""
 import random
+import zlib
 
 import pytest
 
@@ -64,7 +65,9 @@ def make_batches(seed, n_batches, A):
 @pytest.mark.parametrize("cfgv", [(1, 4, 6, 1, 32, 100, 5), (2, 4, 6, 1, 32, 40, 0),
                                   (5000, 4, 6, 1, 32, 100, 5)])
 def test_align_batch_matches_oracle(pkg, ctx, mode, detected, cfgv):
-    batches = make_batches(hash((mode, detected, cfgv)) & 0xFFFF, 40, cfgv[0])
+    # crc32 of the parameters: the same batches in every process (str hashes are salted)
+    batches = make_batches(zlib.crc32(repr((mode, detected, cfgv)).encode()) & 0xFFFF, 40,
+                           cfgv[0])
     got = pkg.align_batch_simd(ctx, batches, pkg.sw_cfg(*cfgv), mode, detected)
     c = O.cfg(*cfgv)
     for b, (jobs, g) in enumerate(zip(batches, got)):
