@@ -89,6 +89,30 @@ kira_status kira_sieve_run_batch(kira_ctx *ctx, int32_t n, const uint8_t *reads_
                                  int32_t *out_shifts, int32_t *out_mismatches);
 
 /*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc in a serving loop, harnesses that
+ * forbid it): kira_sieve_run_batch with the same arguments, results, errors
+ * and sentinels, but every device buffer it stages (packed reads, phase-0
+ * window bytes, job tables, bit planes, results) lies in one fixed
+ * library-owned array of KIRA_NOALLOC_SCRATCH_BYTES per device, kept until
+ * the process ends.  It calls no hipMalloc / hipFree and allocates no host
+ * memory.  A batch whose staging exceeds the array (about
+ * 200 bytes per job plus the phase-0 window bytes) fails with
+ * KIRA_ERR_INVALID_ARGUMENT and a message naming both sizes.  All calls on
+ * one GPU share its array: calls that overlap in time on the same GPU race,
+ * so the caller serialises them (each call synchronises its stream before it
+ * returns).
+ */
+#define KIRA_NOALLOC_SCRATCH_BYTES (64ll << 20)
+kira_status kira_sieve_run_batch_noalloc(kira_ctx *ctx, int32_t n, const uint8_t *reads_packed,
+                                         int32_t read_bytes_max, const int32_t *read_packed_len,
+                                         const int32_t *read_nucs,
+                                         const uint8_t *ref_shifted_flat,
+                                         int64_t ref_bytes_per_phase, const int64_t *ref_offsets,
+                                         const int32_t *ref_nucs, const uint32_t *max_mismatches,
+                                         int32_t *out_shifts, int32_t *out_mismatches);
+
+/*
  * Resident packed reference (no counterpart in the reference, which re-packs
  * and re-uploads every window per batch, src/pipeline/stage4_alignment.rs:1280-1303).
  * The ASCII sequence is uploaded once and packed on the device into two bit

@@ -277,6 +277,8 @@ SIGNATURES = {
     "kira_release_buffers": (None, [_P]),
     "kira_sieve_run_batch": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int64, _P,
                                        _P, _P, _P, _P]),
+    "kira_sieve_run_batch_noalloc": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P,
+                                               C.c_int64, _P, _P, _P, _P, _P]),
     "kira_genome_create": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_void_p)]),
     "kira_genome_destroy": (None, [_P]),
     "kira_genome_len": (C.c_int64, [_P]),
@@ -436,8 +438,12 @@ class HipBackend:
     def __init__(self, device: int = 0):
         self.ctx = Context(device)
 
-    def run_batch(self, jobs: Sequence[HipJob]) -> List[HipResult]:
-        """run_batch — src/cuda/backend.rs:96-249 (same errors, same result order)."""
+    def run_batch(self, jobs: Sequence[HipJob], noalloc: bool = False) -> List[HipResult]:
+        """run_batch — src/cuda/backend.rs:96-249 (same errors, same result order).
+
+        noalloc=True goes through kira_sieve_run_batch_noalloc, which stages
+        the batch in the library's fixed per-device array instead of
+        allocating (batches beyond its size raise InvalidArgument)."""
         n = len(jobs)
         if n == 0:
             return []
@@ -471,10 +477,10 @@ class HipBackend:
             acc += int(sizes[i])
         out_s = np.empty(n, dtype=np.int32)
         out_m = np.empty(2 * n, dtype=np.int32)
-        st = lib().kira_sieve_run_batch(self.ctx.h, n, _np_ptr(reads), READ_BYTES_MAX,
-                                        _np_ptr(plen), _np_ptr(nucs), _np_ptr(flat), total,
-                                        _np_ptr(offsets), _np_ptr(ref_nucs), _np_ptr(limits),
-                                        _np_ptr(out_s), _np_ptr(out_m))
+        fn = lib().kira_sieve_run_batch_noalloc if noalloc else lib().kira_sieve_run_batch
+        st = fn(self.ctx.h, n, _np_ptr(reads), READ_BYTES_MAX, _np_ptr(plen), _np_ptr(nucs),
+                _np_ptr(flat), total, _np_ptr(offsets), _np_ptr(ref_nucs), _np_ptr(limits),
+                _np_ptr(out_s), _np_ptr(out_m))
         self.ctx.check(st, "run_batch")
         return [HipResult(int(out_s[i]), int(out_m[i]),
                           None if out_m[n + i] == I32_MAX else int(out_m[n + i]))

@@ -72,6 +72,11 @@ hipError_t launch_pack_reads_2bit(int32_t n, const uint8_t *packed, int32_t stri
                                   const int32_t *lens, uint32_t *planes, hipStream_t s);
 hipError_t launch_sieve(const SieveLaunch &a, int read_words, hipStream_t s);
 
+// Device address of the fixed staging array of kira_sieve_run_batch_noalloc
+// on `device`, looked up once per device (the first call makes `device`
+// current for the lookup and restores the previous device).
+hipError_t noalloc_setup(int device, void **out);
+
 // Banded SW: results per job, kSwResWords int32 each: score, read_end,
 // ref_end, early_abort, ref_start, read_start, n_cigar, int16-overflow flag,
 // DP cells the reference algorithm evaluates for the job, 0.

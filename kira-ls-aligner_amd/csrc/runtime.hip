@@ -101,6 +101,10 @@ kira_status kira_ctx_create(int device, kira_ctx **out) {
         return KIRA_ERR_DRIVER;
     }
     ctx->stream = ctx->own_stream;
+    // the no-allocation entry point's staging array: its device address is
+    // looked up here, once per device (a failure is reported by that entry
+    // point, which needs the address)
+    (void)kira::noalloc_setup(device, nullptr);
     *out = ctx;
     return KIRA_OK;
 }
