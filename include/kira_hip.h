@@ -129,7 +129,8 @@ const uint8_t *kira_genome_ascii_dev(const kira_genome *g);
 
 /*
  * Read batch packed on the device from ASCII.  reads: n x stride bytes,
- * read_lens[i] <= 256 bases.  planes_out (device, n x 24 uint32: 8 low-plane
+ * read_lens[i] <= min(256, stride) bases (a longer read is not read past its
+ * row and is reported invalid).  planes_out (device, n x 24 uint32: 8 low-plane
  * words, 8 high-plane words, 8 lower-case flag words) and valid_out (device, n bytes, 1 iff every
  * base is A/C/G/T in either case — PackedDna::valid, bitpacked.rs:29-33).
  * All pointers are device pointers; runs on the context stream.

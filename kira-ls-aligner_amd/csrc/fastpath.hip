@@ -382,13 +382,13 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             }
             const int32_t m = (int32_t)lists.n;
             if (m) {
+                const size_t lds = (size_t)max_lt + 16;
+                if (lds > 65536)
+                    return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "fast path window too long");
                 DevLists d;
                 int32_t *d_out = nullptr;
                 if ((e = upload(d)) || (e = dm.alloc(&d_out, 3 * (size_t)m)))
                     return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path spectral", e);
-                const size_t lds = (size_t)max_lt + 16;
-                if (lds > 65536)
-                    return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "fast path window too long");
                 hipLaunchKernelGGL(spectral_kernel, dim3(m), dim3(64), lds, st, m, d_reads, d.roff,
                                    d.rlen, g->ascii, d.toff, d.tlen, kn.spectral_pct,
                                    kn.spectral_floor, kn.cert_strict, d_out);

@@ -298,7 +298,10 @@ void cached_free(kira_ctx *ctx, void *p, size_t size_class);
 hipError_t pinned_buf(kira_ctx *ctx, int slot, size_t bytes, void **out);
 
 // The device allocations of one scope, back to the context's cache (or freed,
-// without a context) when the scope ends.
+// without a context) when the scope ends.  A cached block may be handed to the
+// next call at once, possibly on another stream, so the scope first waits for
+// the context stream: an early error return can leave a copy or kernel that
+// uses the block queued (hipFree, the uncached case, waits by itself).
 struct DevMem {
     kira_ctx *ctx = nullptr;
     std::vector<std::pair<void *, size_t>> v;
@@ -306,9 +309,7 @@ struct DevMem {
     explicit DevMem(kira_ctx *c) : ctx(c) {}
     DevMem(const DevMem &) = delete;
     DevMem &operator=(const DevMem &) = delete;
-    ~DevMem() {
-        for (auto &x : v) cached_free(ctx, x.first, x.second);
-    }
+    ~DevMem();
     template <class T>
     hipError_t alloc(T **p, size_t count) {
         void *q = nullptr;
@@ -442,3 +443,8 @@ struct kira_genome {
     uint8_t *ascii = nullptr;  // base 0 of the ASCII copy; kRefPad zero bytes on each side
     std::vector<uint8_t> host;  // host copy, for alignment records (NM / MD)
 };
+
+inline kira::DevMem::~DevMem() {
+    if (ctx && !v.empty()) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &x : v) cached_free(ctx, x.first, x.second);
+}

@@ -145,11 +145,15 @@ __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ r
     if (t >= (int64_t)n * 8) return;
     int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
     int32_t L = lens[r];
+    // row-major reads: a length past the row width is not read past the row
+    // (and the read is marked invalid)
+    const bool over = !offs && L > stride;
+    const int32_t Le = over ? stride : L;
     uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
     const uint8_t *src = reads + (offs ? offs[r] : (int64_t)r * stride);
     for (int b = 0; b < 32; b++) {
         int32_t q = k * 32 + b;
-        if (q >= L) break;
+        if (q >= Le) break;
         uint32_t a0, b0, c0, d0;
         classify(src[q], a0, b0, c0, d0);
         lo |= a0 << b;
@@ -167,7 +171,7 @@ __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ r
         f |= __shfl_xor(f, 1, 8);
         f |= __shfl_xor(f, 2, 8);
         f |= __shfl_xor(f, 4, 8);
-        if (k == 0) valid[r] = (L <= kMaxReadBases && f == 0) ? 1 : 0;
+        if (k == 0) valid[r] = (L <= kMaxReadBases && !over && f == 0) ? 1 : 0;
     }
 }
 
@@ -183,12 +187,16 @@ __global__ void pack_reads_ascii_vec_kernel(int32_t n, const uint8_t *__restrict
     if (t >= (int64_t)n * 8) return;
     int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
     int32_t L = lens[r];
+    // a length past the row width is not read past the row (the read is
+    // marked invalid)
+    const bool over = L > stride;
+    const int32_t Le = over ? stride : L;
     uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
-    if (k * 32 < L) {
+    if (k * 32 < Le) {
         const uint4 *src = (const uint4 *)(reads + (int64_t)r * stride + k * 32);
         const uint4 v0 = src[0], v1 = src[1];
         const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-        const int nb = L - k * 32 < 32 ? L - k * 32 : 32;
+        const int nb = Le - k * 32 < 32 ? Le - k * 32 : 32;
         // classify() on four bytes at once: with u = c & 0xDF (upper case),
         // A C G T are 0x41 0x43 0x47 0x54, so code bit 1 is bit 2 of the
         // byte and code bit 0 is bit 1 ^ bit 2; bit 5 marks lower case
@@ -218,7 +226,7 @@ __global__ void pack_reads_ascii_vec_kernel(int32_t n, const uint8_t *__restrict
         f |= __shfl_xor(f, 1, 8);
         f |= __shfl_xor(f, 2, 8);
         f |= __shfl_xor(f, 4, 8);
-        if (k == 0) valid[r] = (L <= kMaxReadBases && f == 0) ? 1 : 0;
+        if (k == 0) valid[r] = (L <= kMaxReadBases && !over && f == 0) ? 1 : 0;
     }
 }
 

@@ -166,7 +166,7 @@ def read_chains(path, ids, names):
             if not line.strip() or line.startswith("#"):
                 continue
             c = line.rstrip("\n").split("\t")
-            if len(c) != 8 or c[0] not in by_id or c[1] not in contig or c[6] not in "+-":
+            if len(c) != 8 or c[0] not in by_id or c[1] not in contig or c[6] not in ("+", "-"):
                 raise SessionError(f"{path}:{ln}: malformed chain line")
             chains[by_id[c[0]]].append((contig[c[1]], int(c[2]), int(c[3]), int(c[4]),
                                         int(c[5]), 1 if c[6] == "-" else 0, int(c[7])))

@@ -184,3 +184,64 @@ def test_genome_batch_with_certification(pkg, strict, stride):
     assert sum(1 for w in want if w[3] & 4) > 20  # certification was exercised
     genome.close()
     ctx.close()
+
+
+def _expected_planes(read: bytes, L: int):
+    """Bit planes of a read from the oracle's 2-bit packing (bitpacked.rs:12-20):
+    code bit 0 / bit 1 of base q at bit q of the low / high plane; the lower-case
+    plane marks a/c/g/t bytes."""
+    packed, valid = O.pack(read[:L])
+    lo = [0] * 8
+    hi = [0] * 8
+    lc = [0] * 8
+    for q in range(L):
+        code = (packed[q // 4] >> (2 * (q % 4))) & 3
+        lo[q // 32] |= (code & 1) << (q % 32)
+        hi[q // 32] |= (code >> 1) << (q % 32)
+        if read[q] in b"acgt":
+            lc[q // 32] |= 1 << (q % 32)
+    return lo + hi + lc, valid
+
+
+@pytest.mark.parametrize("stride", [264, 288])
+def test_read_packing_every_byte_value(pkg, stride):
+    """All 256 byte values (upper and lower case A/C/G/T, 'n', bytes >= 0x80) through
+    both device packers (stride 288 takes the four-bytes-at-once kernel), plus reads
+    longer than the row, which must come back invalid."""
+    import torch
+    rng = np.random.default_rng(stride)
+    alphabet = np.frombuffer(b"ACGTacgtnN", np.uint8)
+    reads, lens = [], []
+    for k in range(256):
+        L = int(rng.integers(1, 257))
+        row = rng.choice(alphabet, size=L) if k % 2 else \
+            rng.choice(np.frombuffer(b"ACGTacgt", np.uint8), size=L)
+        row = row.copy()
+        row[k % L] = k  # byte value k somewhere in read k
+        reads.append(bytes(row))
+        lens.append(L)
+    for L in (stride + 1, stride + 40):  # longer than the row: clamped, invalid
+        reads.append(bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=stride)))
+        lens.append(L)
+    n = len(reads)
+    rd = np.zeros((n, stride), np.uint8)
+    for i, r in enumerate(reads):
+        rd[i, :len(r)] = np.frombuffer(r, np.uint8)
+    ctx = pkg.Context(0)
+    dev = torch.device("cuda:0")
+    d_reads = torch.tensor(rd, device=dev)
+    d_lens = torch.tensor(np.asarray(lens, np.int32), device=dev)
+    d_planes = torch.zeros((n, 24), dtype=torch.int32, device=dev)
+    d_valid = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    pkg.pack_reads_dev(ctx, n, d_reads.data_ptr(), stride, d_lens.data_ptr(),
+                       d_planes.data_ptr(), d_valid.data_ptr())
+    torch.cuda.synchronize()
+    planes = d_planes.cpu().numpy().view(np.uint32)
+    valid = d_valid.cpu().numpy()
+    for i in range(256):
+        want, ok = _expected_planes(reads[i], lens[i])
+        assert planes[i].tolist() == want, i
+        assert valid[i] == int(ok), i
+    assert valid[256:].tolist() == [0, 0]
+    ctx.close()
