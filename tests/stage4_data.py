@@ -126,3 +126,67 @@ def make_batch(seed=1, n_reads=2000, n_contigs=3, contig_len=200_000,
     return dict(genome=b"".join(contigs), contig_off=off,
                 contig_len=np.array([len(c) for c in contigs], np.int64), contigs=contigs,
                 reads=reads, chains=chains, repeat_min_occ=rmo)
+
+
+class Lcg:
+    """The seeded 64-bit LCG of the reference's integration tests
+    (tests/common/mod.rs:27-50, Knuth's MMIX constants)."""
+
+    M = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = seed & self.M
+
+    def next_u64(self) -> int:
+        self.s = (self.s * 6364136223846793005 + 1442695040888963407) & self.M
+        return self.s
+
+    def below(self, n: int) -> int:
+        return (self.next_u64() >> 33) % n
+
+    def base(self) -> int:
+        return b"ACGT"[(self.next_u64() >> 33) & 3]
+
+
+def synth_contig(rng: Lcg, length: int) -> bytes:
+    """synth_contig — tests/common/mod.rs:52-54."""
+    return bytes(rng.base() for _ in range(length))
+
+
+def make_c1(n_reads=10_000, contig_len=1_000_000, read_len=150, seed=0x1234_5678_9ABC_DEF0):
+    """The plain `mem` configuration: one uniform-ACGT contig chr1 from the
+    reference's test LCG, single-end reads from uniform loci on either strand,
+    each base substituted with probability 1/100 and an indel event with
+    probability 1/1000 per base (length U{1..3}, insertion or deletion 50/50),
+    truth locus in the read name (`r{i}:chr1:{start}-{end}`, 0-based half-open,
+    the eval format of src/eval/mod.rs:47).  Each read gets the chain stage 3
+    would find for an error-free seed: its truth span, whole read, score L.
+    Returns the dict make_batch returns plus `names`."""
+    rng = Lcg(seed)
+    contig = synth_contig(rng, contig_len)
+    reads, chains, names = [], [], []
+    for i in range(n_reads):
+        start = rng.below(contig_len - read_len - 16)
+        rev = rng.below(2) == 1
+        out = bytearray()
+        p = start
+        while len(out) < read_len:
+            if rng.below(1000) == 0:
+                k = 1 + rng.below(3)
+                if rng.below(2) == 0:
+                    p += k  # deletion from the read
+                else:
+                    out += bytes(rng.base() for _ in range(k))  # insertion
+                continue
+            b = contig[p]
+            p += 1
+            if rng.below(100) == 0:
+                b = b"ACGT"[(b"ACGT".index(b) + 1 + rng.below(3)) % 4]
+            out.append(b)
+        seq = bytes(out[:read_len])
+        reads.append(revcomp(seq) if rev else seq)
+        names.append(f"r{i}:chr1:{start}-{p}")
+        chains.append([(0, start, p, 0, read_len, int(rev), read_len)])
+    return dict(genome=contig, contig_off=np.zeros(1, np.int64),
+                contig_len=np.array([contig_len], np.int64), contigs=[contig], reads=reads,
+                chains=chains, repeat_min_occ=[0] * n_reads, names=names)
