@@ -231,14 +231,22 @@ typedef struct {
 /* prefilter_chain arguments that are the same for a whole batch
  * (src/alignment/prefilter.rs:93-109) plus its environment knobs:
  * tail_ext KIRA_UNGAP_TAIL (:318-326, default 1), ungap_min_id
- * KIRA_UNGAP_MINID (:331-338, -1 = unset).  The opt-in WFA gap probe
- * (KIRA_UNGAP_GAP_CHECK, default off) is not supported. */
+ * KIRA_UNGAP_MINID (:331-338, -1 = unset), ungap_gap_check
+ * KIRA_UNGAP_GAP_CHECK (:343-356, default 0): the final gate
+ * (prefilter.rs:274-287) that turns an accept with mismatches into
+ * Fallback(IndelSuspect) when a gapped alignment of the read against
+ * contig[ref_start - read_start .. + read_len + bandwidth] costs strictly less
+ * than the mismatches (ungapped_beaten_by_gap, src/alignment/mod.rs:163-204;
+ * exact WFA score-only on the device).  With the gate on, (match + mismatch) *
+ * mismatches must stay <= 65536 and read + text <= 60000 bases, else the call
+ * fails with KIRA_ERR_INVALID_ARGUMENT. */
 typedef struct {
     int32_t accept_enable, accept_only_top1, multi_alignments_enabled, tail_ext;
     int32_t accept_span_slack, accept_max_mismatches, accept_require_score_margin;
     int32_t long_read_threshold;
     float accept_min_identity;
     int32_t ungap_min_id;
+    int32_t ungap_gap_check;
 } kira_prefilter_params;
 
 /* PrefilterOutcome (prefilter.rs:62-80) with the accepted alignment of
@@ -538,6 +546,9 @@ typedef struct {
      * passing chains with the fast DP cascade, orders them by that score,
      * keeps their alignments, sets the DP budget and the competitor XS. */
     int32_t twotier, twotier_k;
+    /* ungap_gap_check KIRA_UNGAP_GAP_CHECK (0): the prefilter's final WFA gap
+     * gate, as in kira_prefilter_params. */
+    int32_t ungap_gap_check;
 } kira_stage4_cfg;
 
 /* Alignment (src/types.rs) as stage 4 leaves it: kind 0 AcceptedUngapped,

@@ -124,12 +124,14 @@ class Span(C.Structure):
 
 class PrefilterParams(C.Structure):
     """kira_prefilter_params: prefilter_chain's per-batch arguments
-    (src/alignment/prefilter.rs:93-109) and KIRA_UNGAP_TAIL / KIRA_UNGAP_MINID."""
+    (src/alignment/prefilter.rs:93-109) and KIRA_UNGAP_TAIL / KIRA_UNGAP_MINID /
+    KIRA_UNGAP_GAP_CHECK."""
 
     _fields_ = [(n, C.c_int32) for n in (
         "accept_enable", "accept_only_top1", "multi_alignments_enabled", "tail_ext",
         "accept_span_slack", "accept_max_mismatches", "accept_require_score_margin",
-        "long_read_threshold")] + [("accept_min_identity", C.c_float), ("ungap_min_id", C.c_int32)]
+        "long_read_threshold")] + [("accept_min_identity", C.c_float), ("ungap_min_id", C.c_int32),
+                                   ("ungap_gap_check", C.c_int32)]
 
 
 class PrefilterOut(C.Structure):
@@ -218,7 +220,8 @@ class Stage4Cfg(C.Structure):
             "long_read_threshold", "max_alignments", "tail_ext", "ungap_min_id", "short_dp_topk",
             "ambig_div", "xs_min_ratio_pct", "spectral_pct", "spectral_floor", "cert_strict",
             "simd", "supplementary", "cpu_mode")] + [("fast", FastKnobs), ("twotier", C.c_int32),
-                                                     ("twotier_k", C.c_int32)]
+                                                     ("twotier_k", C.c_int32),
+                                                     ("ungap_gap_check", C.c_int32)]
 
 
 class Stage4Aln(C.Structure):
@@ -814,11 +817,13 @@ def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",
 def prefilter_params(accept_enable=True, accept_only_top1=True, multi_alignments_enabled=False,
                      tail_ext=True, accept_span_slack=5, accept_max_mismatches=2,
                      accept_require_score_margin=0, long_read_threshold=300,
-                     accept_min_identity=98.0, ungap_min_id=-1) -> PrefilterParams:
+                     accept_min_identity=98.0, ungap_min_id=-1,
+                     ungap_gap_check=False) -> PrefilterParams:
     return PrefilterParams(int(accept_enable), int(accept_only_top1),
                            int(multi_alignments_enabled), int(tail_ext), accept_span_slack,
                            accept_max_mismatches, accept_require_score_margin,
-                           long_read_threshold, accept_min_identity, ungap_min_id)
+                           long_read_threshold, accept_min_identity, ungap_min_id,
+                           int(ungap_gap_check))
 
 
 def prefilter_batch(ctx: Context, chains, cfg: SwCfg, params: Optional[PrefilterParams] = None):
@@ -1055,11 +1060,12 @@ def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth
                tail_ext=True, ungap_min_id=-1, short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0,
                spectral_pct=3, spectral_floor=2, cert_strict=False, simd="avx2",
                supplementary=True, cpu_mode=False, fast=None, twotier=True,
-               twotier_k=5) -> Stage4Cfg:
+               twotier_k=5, ungap_gap_check=False) -> Stage4Cfg:
     """Stage-4 configuration; defaults are `mem`'s (src/cli/commands/mem.rs:183-205).
     cpu_mode: the reference's configuration without --gpu (fallback chains try
     the fast DP cascade first); fast: dict of router knob overrides; twotier /
-    twotier_k: KIRA_TWOTIER (on) / KIRA_TWOTIER_K (5)."""
+    twotier_k: KIRA_TWOTIER (on) / KIRA_TWOTIER_K (5); ungap_gap_check:
+    KIRA_UNGAP_GAP_CHECK (off)."""
     fk = FastKnobs(300, 0, 0, 70, 25, 15, 4, 3, 2, 0, 0, 0, 0)  # router.rs defaults
     for name, v in (fast or {}).items():
         setattr(fk, name, FAST_ALGOS[v] if name == "algo" and isinstance(v, str) else int(v))
@@ -1070,7 +1076,7 @@ def stage4_cfg(*, match_score=1, mismatch=4, gap_open=6, gap_extend=1, bandwidth
         accept_require_score_margin, dp_topk, dp_abort_margin, long_read_threshold,
         max_alignments, int(tail_ext), ungap_min_id, short_dp_topk, ambig_div, xs_min_ratio_pct,
         spectral_pct, spectral_floor, int(cert_strict), SIMD_MODES[simd], int(supplementary),
-        int(cpu_mode), fk, int(twotier), twotier_k)
+        int(cpu_mode), fk, int(twotier), twotier_k, int(ungap_gap_check))
 
 
 class Stage4Result:

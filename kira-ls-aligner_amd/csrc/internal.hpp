@@ -264,6 +264,20 @@ struct PrefilterLaunch {
 };
 
 hipError_t launch_prefilter(const PrefilterLaunch &a, hipStream_t s);
+// The prefilter's final gate (prefilter.rs:274-287, ungapped_beaten_by_gap,
+// mod.rs:163-204) over a batch whose outcomes are on the host: every Accept
+// with mismatches gets an exact WFA score-only probe of its read against
+// contig[ref_start - read_start .. + read_len + bandwidth] with budget
+// mismatches * (match + mismatch) - 1; a probe that finds an alignment turns
+// the outcome into Fallback(IndelSuspect), metrics kept.  Chain k's read is
+// d_reads + read_off[k] (read_len[k] bytes) and its contig d_refs +
+// ref_off[k] (ref_len[k] bytes), device memory; the offset, length and span
+// arrays are host arrays.  Synchronous.
+kira_status prefilter_gap_check(kira_ctx *ctx, int32_t n, const uint8_t *d_reads,
+                                const int64_t *read_off, const int32_t *read_len,
+                                const uint8_t *d_refs, const int64_t *ref_off,
+                                const int64_t *ref_len, const kira_span *spans,
+                                const kira_sw_cfg &cfg, kira_prefilter_out *out);
 // identity floor of the accept gate (prefilter.rs:255-256)
 uint16_t prefilter_id_floor(const kira_prefilter_params &p);
 

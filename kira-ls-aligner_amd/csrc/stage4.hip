@@ -696,6 +696,7 @@ kira_status Driver::run() {
         a.p.long_read_threshold = c.long_read_threshold;
         a.p.accept_min_identity = c.accept_min_identity;
         a.p.ungap_min_id = c.ungap_min_id;
+        a.p.ungap_gap_check = c.ungap_gap_check;
         a.id_floor = kira::prefilter_id_floor(a.p);
         a.out = d_out;
         if ((e = kira::launch_prefilter(a, s)))
@@ -707,6 +708,11 @@ kira_status Driver::run() {
             (e = hipStreamSynchronize(s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
         for (int32_t k = 0; k < ne; k++) exact[ex_sel[k]] = ex[k];
+        if (c.ungap_gap_check) {
+            kira_status gs = kira::prefilter_gap_check(ctx, ns, d_orient, ro, rl, g->ascii, fo,
+                                                       fl, sp, c.cfg, pf.data());
+            if (gs != KIRA_OK) return gs;
+        }
     }
 
     // outcomes in chain order (:758-934): 0 nothing, 1 a record here, 2 fallback

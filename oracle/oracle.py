@@ -334,11 +334,33 @@ def min_len_required(read_len, id_x10000, mism, span_slack):
     return lib().ko_min_len_required(read_len, id_x10000, mism, span_slack)
 
 
+def ungapped_beaten_by_gap(read: bytes, ref: bytes, span, ungapped_mism: int, c: Cfg) -> bool:
+    """ungapped_beaten_by_gap — src/alignment/mod.rs:163-204: an exact WFA
+    score-only probe of the read against ref[ref_start - read_start .. + L +
+    bandwidth] with budget mism * (match + mismatch) - 1 finds an alignment."""
+    if ungapped_mism == 0 or len(read) == 0:
+        return False
+    ws = span[0] - span[2]  # chain ref_start - read_start
+    if ws < 0 or ws >= len(ref):
+        return False
+    band = max(c.bandwidth, 1)
+    tl = min(len(read) + band, len(ref) - ws)
+    if tl < len(read):
+        return False
+    cost = ungapped_mism * (c.match_score + c.mismatch)
+    if cost <= 0:
+        return False
+    return wfa_score_only_scored(read, ref[ws:ws + tl], c, cost - 1) is not None
+
+
 def prefilter_chain(read: bytes, ref: bytes, span, c: Cfg, is_top1=True, accept_enable=True,
                     accept_only_top1=True, accept_span_slack=5, accept_min_identity=98.0,
                     accept_max_mismatches=2, accept_require_score_margin=0,
                     score_margin=2 ** 31 - 1, long_read_threshold=300,
-                    multi_alignments_enabled=False, tail_ext=True, ungap_min_id=-1):
+                    multi_alignments_enabled=False, tail_ext=True, ungap_min_id=-1,
+                    ungap_gap_check=False):
+    """prefilter_chain (src/alignment/prefilter.rs:93-295); ungap_gap_check is
+    KIRA_UNGAP_GAP_CHECK, the final gate of :274-287."""
     o = PrefilterOut()
     md = C.create_string_buffer(4 * len(read) + 32)
     sp = Span(*span)
@@ -348,6 +370,10 @@ def prefilter_chain(read: bytes, ref: bytes, span, c: Cfg, is_top1=True, accept_
                              accept_require_score_margin, score_margin, long_read_threshold,
                              int(multi_alignments_enabled), int(tail_ext), ungap_min_id,
                              C.byref(o), md, len(md))
+    if ungap_gap_check and o.result == 0 and o.mism > 0 and \
+            ungapped_beaten_by_gap(read, ref, span, o.mism, c):
+        o.result = 2   # Fallback
+        o.reason = 8   # IndelSuspect
     r = o.reason
     reason = REASONS[r] if r < 9 else OTHER[(r - 9) // 16]
     d = dict(result=RESULTS[o.result], reason=reason)

@@ -43,7 +43,7 @@ def default_cfg(**kw):
              long_read_threshold=500, max_alignments=1, tail_ext=True, ungap_min_id=-1,
              short_dp_topk=0, ambig_div=5, xs_min_ratio_pct=0, spectral_pct=3,
              spectral_floor=2, cert_strict=False, simd="avx2", supplementary=True,
-             cpu_mode=False, fast={}, twotier=True, twotier_k=5)
+             cpu_mode=False, fast={}, twotier=True, twotier_k=5, ungap_gap_check=False)
     c.update(kw)
     return c
 
@@ -341,7 +341,7 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
                 accept_require_score_margin=cfg["accept_require_score_margin"],
                 score_margin=margin, long_read_threshold=cfg["long_read_threshold"],
                 multi_alignments_enabled=multi, tail_ext=cfg["tail_ext"],
-                ungap_min_id=cfg["ungap_min_id"])
+                ungap_min_id=cfg["ungap_min_id"], ungap_gap_check=cfg["ungap_gap_check"])
             reason = pf["reason"]
             st["reason_counts"][O.REASONS.index(reason) if reason in O.REASONS else 9] += 1
             m = pf.get("metrics")

@@ -33,6 +33,7 @@ PARAM_SETS = [
     dict(accept_only_top1=False, long_read_threshold=1000, accept_max_mismatches=20),
     dict(multi_alignments_enabled=True),
     dict(accept_enable=False),
+    dict(ungap_gap_check=True, accept_max_mismatches=5, accept_min_identity=96.0),
 ]
 
 
@@ -139,3 +140,33 @@ def test_rejects_span_outside_read(pkg, ctx):
 
 def test_empty_batch(pkg, ctx):
     assert pkg.prefilter_batch(ctx, [], pkg.sw_cfg()) == []
+
+
+@pytest.mark.parametrize("cfgv", [(1, 4, 6, 1, 32, 100, 5), (1, 6, 6, 1, 50, 100, 5),
+                                  (2, 3, 5, 2, 16, 40, 5)])
+def test_gap_check_gate(pkg, ctx, cfgv):
+    """KIRA_UNGAP_GAP_CHECK on the device (WFA score-only probe per accepted
+    chain with mismatches): reads with an indel just before a run that reaches
+    the read end, plain substitution reads, and random chains."""
+    from tests.test_oracle_prefilter import _end_indel_read
+    rng = random.Random(sum(cfgv))
+    chains = []
+    for k in range(900):
+        if k % 3 == 2:
+            read, ref, span = random_chain(rng)
+            chains.append((read, ref, span, True, I32_MAX))
+            continue
+        ref = bytes(rng.choice(b"ACGT") for _ in range(400))
+        st, L = rng.randrange(100, 200), rng.choice([100, 150])
+        if k % 3 == 0:
+            read, ref = _end_indel_read(rng, ref, st, L)
+        else:
+            read = bytearray(ref[st:st + L])
+            for _ in range(rng.randrange(1, 3)):
+                read[rng.randrange(L)] = rng.choice(b"ACGT")
+            read = bytes(read)
+        chains.append((read, ref, (st + 20, st + 80, 20, 80), True, I32_MAX))
+    prm = dict(ungap_gap_check=True, accept_max_mismatches=5, accept_min_identity=96.0)
+    check_batch(pkg, ctx, chains, prm, cfgv)
+    got = pkg.prefilter_batch(ctx, chains, pkg.sw_cfg(*cfgv), pkg.prefilter_params(**prm))
+    assert sum(g["reason"] == "indel_suspect" for g in got) > 100

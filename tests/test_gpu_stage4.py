@@ -44,7 +44,8 @@ def run_both(pkg, ctx, b, **cfg):
                                  dict(simd="scalar", supplementary=False, twotier_k=2),
                                  dict(cert_strict=True, min_chain_ratio=0.2, dp_abort_margin=5,
                                       twotier=False),
-                                 dict(twotier=False, short_dp_topk=2)])
+                                 dict(twotier=False, short_dp_topk=2),
+                                 dict(ungap_gap_check=True)])
 def test_records_and_counters_match_oracle(pkg, ctx, cfg):
     b = make_batch(11, 2500)
     got, gst, want, wst = run_both(pkg, ctx, b, **cfg)

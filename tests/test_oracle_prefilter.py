@@ -232,3 +232,99 @@ def test_accepted_alignment_replays():
         assert al["score"] == (e - s - nm) - 4 * nm
         clip = (f"{s}S" if s else "") + f"{e - s}M" + (f"{L - e}S" if e < L else "")
         assert al["cigar"] == clip
+
+
+# ---- KIRA_UNGAP_GAP_CHECK: ungapped_beaten_by_gap (mod.rs:163-204) and the
+# final gate of prefilter_chain (prefilter.rs:274-287)
+
+def _gap_cfg(bw=50, a=1, b=4):
+    return O.cfg(a, b, 6, 1, bw, 1000, 5)
+
+
+def test_gap_gate_known_answers():
+    """mod.rs:2502-2550 (gap_gate_flags_indel_not_snps): two substitutions are
+    never beaten by a gap; a 2-bp deletion at read position 48 (the ungapped
+    placement smears the tail into 6 mismatches) is; an exact read never is."""
+    from tests.refgen import gen_ref
+    ref = gen_ref(160)
+    flip = lambda x: ord("C") if x == ord("A") else ord("A")
+    span = lambda n: (10, 10 + n, 0, n)
+    snp = bytearray(ref[10:70])
+    snp[20] = flip(snp[20])
+    snp[40] = flip(snp[40])
+    assert not O.ungapped_beaten_by_gap(bytes(snp), ref, span(60), 2, _gap_cfg())
+    dele = ref[10:58] + ref[60:72]
+    assert O.ungapped_beaten_by_gap(dele, ref, span(len(dele)), 6, _gap_cfg())
+    exact = ref[10:70]
+    assert not O.ungapped_beaten_by_gap(exact, ref, span(60), 0, _gap_cfg())
+
+
+def test_gap_gate_budget_is_strict():
+    """The probe's budget is one below the ungapped cost, mism * (A + B) - 1
+    (mod.rs:193-201), so a gapped alignment must be strictly cheaper.  A 1-bp
+    deletion costs gap open 6 + deletion extend 1 = 7 in WFA terms; with
+    A + B = 7 one mismatch ties it (not beaten) and two do not (beaten)."""
+    from tests.refgen import gen_ref
+    ref = gen_ref(160)
+    read = ref[10:80] + ref[81:100]
+    sp = (10, 10 + len(read), 0, len(read))
+    c = _gap_cfg(a=1, b=6)
+    assert not O.ungapped_beaten_by_gap(read, ref, sp, 1, c)
+    assert O.ungapped_beaten_by_gap(read, ref, sp, 2, c)
+    # windows: a chain implying a negative or past-the-end window start is
+    # never probed (mod.rs:177-184)
+    assert not O.ungapped_beaten_by_gap(read, ref, (5, 100, 6, 100), 2, c)
+    assert not O.ungapped_beaten_by_gap(read, ref, (170, 180, 0, 10), 2, c)
+
+
+def _end_indel_read(rng, ref, st, L):
+    """A read from ref[st:] with one indel a few bases before a run of A that
+    reaches the read end, laid out so the ungapped placement shows exactly two
+    mismatches (u/v before the run), which one gap explains: a 1-bp deletion of
+    u, or a 1-bp insertion w before v."""
+    ref = bytearray(ref)
+    q = rng.randrange(L - 40, L - 4)
+    u, v, w = rng.sample(b"CGT", 3)
+    ref[st + q] = u
+    ref[st + q + 1] = v
+    ref[st + q + 2:st + L + 2] = b"A" * (L - q)
+    if rng.random() < 0.5:
+        read = ref[st:st + q] + ref[st + q + 1:st + L + 1]       # deletion of u
+    else:
+        read = ref[st:st + q] + bytes([w]) + ref[st + q:st + L - 1]  # insertion of w
+    return bytes(read), bytes(ref)
+
+
+def test_gap_gate_turns_end_indels_into_fallback():
+    """Reads whose ungapped accept hides an indel near the read end as
+    mismatches: with the gate on the same chain falls back with IndelSuspect,
+    metrics unchanged, exactly when the probe finds a cheaper gapped
+    alignment; other accepts are untouched."""
+    rng = random.Random(77)
+    flips = kept = 0
+    for k in range(400):
+        ref = bytes(rng.choice(b"ACGT") for _ in range(400))
+        st, L = rng.randrange(100, 200), 150
+        if k % 4:
+            read, ref = _end_indel_read(rng, ref, st, L)
+        else:
+            read = bytearray(ref[st:st + L])
+            read[rng.randrange(L)] = ord("T")
+            read = bytes(read)
+        span = (st + 20, st + 80, 20, 80)
+        c = O.cfg(1, 4, 6, 1, 32, 100, 5)
+        kw = dict(accept_max_mismatches=5, accept_min_identity=96.0)
+        off = O.prefilter_chain(read, ref, span, c, **kw)
+        on = O.prefilter_chain(read, ref, span, c, ungap_gap_check=True, **kw)
+        if off["result"] != "accept":
+            assert on == off
+            continue
+        mism = off["metrics"]["mism"]
+        if mism > 0 and O.ungapped_beaten_by_gap(read, ref, span, mism, c):
+            assert (on["result"], on["reason"]) == ("fallback", "indel_suspect")
+            assert on["metrics"] == off["metrics"] and "alignment" not in on
+            flips += 1
+        else:
+            assert on == off
+            kept += 1
+    assert flips > 150 and kept > 50
