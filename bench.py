@@ -18,6 +18,7 @@ import argparse
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -265,37 +266,80 @@ def pre_shift_rows(bits):
     return out
 
 
-def cpu_baseline_sieve(work, n_sample_reads, threads, limit):
-    """The reference CPU scan (mismatch_count_u64 over every shift,
-    bitpacked.rs:154-270) restated in C (the oracle), pthreads over jobs."""
+def cpu_env():
+    """The host the CPU baselines run on: lscpu's model and core counts, and
+    the CPUs this process may use (its affinity mask)."""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(key, default=0):
+        try:
+            return int(info.get(key, default))
+        except ValueError:
+            return default
+    sockets, cps, tpc = num("Socket(s)", 1), num("Core(s) per socket"), num("Thread(s) per core", 1)
+    return {"model": info.get("Model name"), "sockets": sockets, "cores_per_socket": cps,
+            "threads_per_core": tpc, "physical_cores": sockets * cps,
+            "logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def median_of_runs(fn, runs=5):
+    """One warm-up call, then `runs` timed calls: (median seconds, last result)."""
+    fn()
+    ts, r = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), r
+
+
+def sieve_layout(work, read_idx, limit):
+    """The jobs of the given reads (all their windows) in the reference's SoA
+    launch layout (CudaBackend::run_batch, src/cuda/backend.rs:118-192):
+    packed reads of READ_BYTES_MAX bytes, four pre-shifted window phases."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
     L, W, K = work["read_len"], work["W"], work["windows"]
-    reads = work["reads"][:n_sample_reads]
-    n = n_sample_reads * K
-    starts = work["win_start"][:n]
+    reads = work["reads"][read_idx]
+    n = len(read_idx) * K
+    starts = work["win_start"].reshape(-1, K)[read_idx].reshape(-1)
     texts = work["genome"][starts[:, None] + np.arange(W)[None, :]]
     rb = pack_rows(reads)
     tb = pre_shift_rows(pack_rows(texts))
     nb = tb.shape[2]
-    lay = dict(n=n)
     rp = np.zeros((n, O.READ_BYTES_MAX), dtype=np.uint8)
     rp[:, :rb.shape[1]] = np.repeat(rb, K, axis=0)
-    lay["reads"] = rp.reshape(-1)
-    lay["read_lens"] = np.full(n, L, dtype=np.int32)
-    lay["ref_lens"] = np.full(n, W, dtype=np.int32)
-    lay["limits"] = np.full(n, limit, dtype=np.uint32)
-    lay["ref_phase_bytes"] = np.full(n, nb, dtype=np.int64)
-    lay["ref_offsets"] = (np.arange(n, dtype=np.int64) * nb)
-    lay["ref_flat"] = np.ascontiguousarray(tb.transpose(1, 0, 2)).reshape(-1)
-    lay["ref_bytes_per_phase"] = n * nb
-    O.backend_run_batch_layout(lay, threads=threads)  # warm-up
-    ts = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        shifts, best, second = O.backend_run_batch_layout(lay, threads=threads)
-        ts.append(time.perf_counter() - t0)
-    return float(np.median(ts)), (shifts, best, second)
+    return dict(n=n, reads=rp.reshape(-1), read_lens=np.full(n, L, dtype=np.int32),
+                ref_lens=np.full(n, W, dtype=np.int32),
+                limits=np.full(n, limit, dtype=np.uint32),
+                ref_phase_bytes=np.full(n, nb, dtype=np.int64),
+                ref_offsets=np.arange(n, dtype=np.int64) * nb,
+                ref_flat=np.ascontiguousarray(tb.transpose(1, 0, 2)).reshape(-1),
+                ref_bytes_per_phase=n * nb, texts=texts, read_rows=reads)
+
+
+def sieve_oracle_flags(lay, shifts, best, second, K, cert_max):
+    """Flags of the resident sieve for jobs the oracle scanned: dispatched
+    (the genome and reads are all ACGT), accepted, certified
+    (spectral_hit_is_certified, src/alignment/mod.rs:688-762)."""
+    from oracle import oracle as O
+    L = lay["read_rows"].shape[1]
+    flags = np.ones(lay["n"], dtype=np.uint8)
+    for j in np.nonzero(shifts >= 0)[0]:
+        flags[j] |= 2
+        sh = int(shifts[j])
+        sec = None if second[j] == 2 ** 31 - 1 else int(second[j])
+        if O.certified(lay["read_rows"][j // K].tobytes(), lay["texts"][j, sh:sh + L].tobytes(),
+                       int(best[j]), sec, cert_max, False):
+            flags[j] |= 4
+    return flags
 
 
 # --------------------------------------------------------------- main
@@ -306,13 +350,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--reads", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample-reads", type=int, default=131_072)
+    ap.add_argument("--pcie-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sw-pairs", type=int, default=1_000_000)
     ap.add_argument("--sw-steps", type=int, default=3)
+    ap.add_argument("--sw-tune-cache", default=os.environ.get("KIRA_SW_TUNE_CACHE", ""),
+                    help="file the SW autotuner writes its routes to (default: none)")
     ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
     ap.add_argument("--long-windows", type=int, default=82_000)
     ap.add_argument("--stage4-reads", type=int, default=200_000)
     ap.add_argument("--stage4-steps", type=int, default=3)
+    ap.add_argument("--c1-reads", type=int, default=10_000)
     ap.add_argument("--stage4-only", action="store_true",
                     help="run only the stage-4 driver section (for profiling)")
     ap.add_argument("--fastpath-chains", type=int, default=200_000)
@@ -321,7 +369,7 @@ def main():
                     help="run only the fast DP cascade section (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-budget-gb", type=int, default=96)
+    ap.add_argument("--serve-budget-gb", type=int, default=96, help="GiB")
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--calib-copy-gb", type=float, default=0.0,
                     help="after the timed region, run one plain device copy of this many "
@@ -422,22 +470,49 @@ def main():
     certified = int(((flags & 4) != 0).sum())
     true_hit = float((shifts.reshape(n_reads, K)[:, 0] == 2 * 32).mean())
 
-    # parity spot check of the timed output against the CPU oracle
+    # the drop-in form: the same workload's host arrays in the reference layout
+    # through kira_sieve_run_batch (PCIe both ways), and the resident path with
+    # its host transfers included
+    pcie = bench_resident_pcie(pkg, ctx, genome, rd, lens, win_start, win_len, out, planes,
+                               valid, stream, K, L, limit, cert_max, args.pcie_steps)
+    print(f"[bench] resident path with PCIe: {pcie['ms_per_step']:.2f} ms/step, "
+          f"{pcie['reads_per_s']:.3g} reads/s", file=sys.stderr, flush=True)
+    host = cpu_env()
     cpu = None
     parity = None
+    plain = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        threads = host["affinity_cpus"]
         ns = min(args.cpu_sample_reads, n_reads)
-        t_cpu, (cs, cb, csec) = cpu_baseline_sieve(work, ns, threads, limit)
-        m = ns * K
-        parity = bool(np.array_equal(cs, shifts[:m]) and np.array_equal(cb, best[:m]) and
-                      np.array_equal(csec, second[:m]))
-        cpu = {"value": ns / t_cpu, "unit": "reads/s", "cores": threads, "kind": "port",
-               "sample": f"{ns} reads x {K} windows = {m} scans of 150 bp vs 278 bp "
-                         f"(mismatch_count_u64 + best/second, packing excluded), "
-                         f"median of 3"}
-        print(f"[bench] cpu baseline {ns / t_cpu:.0f} reads/s on {threads} threads; "
-              f"parity={parity}", file=sys.stderr, flush=True)
+        # reads drawn from the whole batch (seeded), every window of each
+        idx = np.sort(np.random.default_rng(17).choice(n_reads, ns, replace=False))
+        lay = sieve_layout(work, idx, limit)
+        t_all, (cs, cb, csec) = median_of_runs(
+            lambda: O.backend_run_batch_layout(lay, threads=threads))
+        n1 = min(ns, 8192)
+        lay1 = sieve_layout(work, idx[:n1], limit)
+        t_one, _ = median_of_runs(lambda: O.backend_run_batch_layout(lay1, threads=1))
+        jobs = (idx[:, None] * K + np.arange(K)[None, :]).reshape(-1)
+        cflags = sieve_oracle_flags(lay, cs, cb, csec, K, cert_max)
+        parity = bool(np.array_equal(cs, shifts[jobs]) and np.array_equal(cb, best[jobs]) and
+                      np.array_equal(csec, second[jobs]) and np.array_equal(cflags, flags[jobs]))
+        cpu = {"value": ns / t_all, "unit": "reads/s", "cores": threads, "kind": "port",
+               "single_thread_value": n1 / t_one,
+               "sample": f"{ns} reads drawn from the whole batch x {K} windows = {ns * K} "
+                         f"scans of 150 bp vs 278 bp (mismatch_count_u64 + best/second, packing "
+                         f"excluded), {threads} threads (the CPUs this process may use); "
+                         f"{n1} of them on 1 thread; 1 warm-up + median of 5",
+               "host": host}
+        print(f"[bench] cpu baseline {ns / t_all:.0f} reads/s on {threads} threads, "
+              f"{n1 / t_one:.0f} on 1; parity (shift, best, second, flags)={parity}",
+              file=sys.stderr, flush=True)
+        plain = bench_plain_sieve(pkg, ctx, lay, ns, cs, cb, csec)
+        print(f"[bench] plain drop-in form (kira_sieve_run_batch, host arrays): "
+              f"{plain['ms_per_call']:.1f} ms per {ns}-read call, {plain['reads_per_s']:.3g} "
+              f"reads/s, parity={plain['parity_vs_oracle']}", file=sys.stderr, flush=True)
+        del lay, lay1
 
     n_shifts = W - L + 1
     words = (L + 31) // 32
@@ -485,6 +560,10 @@ def main():
                           "unit": "Tops/s", "frac": tops / VALU_PEAK_TOPS,
                           "ops_per_launch": ops_per_launch},
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
+        "plain_gpu_form": plain,
+        "batched_vs_plain": (n_reads / (ms_per_step * 1e-3)) / plain["reads_per_s"]
+        if plain else None,
     }
     lw = None
     if args.long_windows > 0 and (args.sw_pairs > 0 or args.serve_requests > 0):
@@ -504,6 +583,73 @@ def main():
     if args.fastpath_chains > 0:
         line["fastpath"] = bench_fastpath(pkg, ctx, args)
     print(json.dumps(line), flush=True)
+
+
+def bench_resident_pcie(pkg, ctx, genome, rd, lens, win_start, win_len, out, planes, valid,
+                        stream, K, L, limit, cert_max, steps):
+    """The headline step with its host transfers: ASCII reads from pinned host
+    memory to the device, packing + sieve, the four outputs back to pinned
+    host memory; HIP events on the launch stream around each step."""
+    import torch
+    h_reads = torch.from_numpy(rd).pin_memory()
+    d_reads = torch.empty(rd.shape, dtype=torch.uint8, device="cuda:0")
+    h_out = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+             for t in (out.shift, out.best, out.second, out.flags)]
+
+    def one():
+        d_reads.copy_(h_reads, non_blocking=True)
+        pkg.sieve_genome(ctx, genome, d_reads, lens, win_start, win_len, jobs_per_read=K,
+                         limit_all=limit, cert_max_mism=cert_max, max_read_len=L, planes=planes,
+                         valid=valid, out=out, stream=stream)
+        for h, d in zip(h_out, (out.shift, out.best, out.second, out.flags)):
+            h.copy_(d, non_blocking=True)
+
+    one()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(steps):
+        one()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    n_reads = rd.shape[0]
+    moved = rd.nbytes + sum(t.numel() * t.element_size() for t in h_out)
+    del h_reads, d_reads, h_out
+    return {"ms_per_step": ms, "reads_per_s": n_reads / (ms * 1e-3), "steps": steps,
+            "host_bytes_moved": moved,
+            "note": "reads H2D from pinned memory + pack + sieve + shift/best/second/flags "
+                    "D2H to pinned memory, per step"}
+
+
+def bench_plain_sieve(pkg, ctx, lay, n_reads, cs, cb, csec, runs=5):
+    """The drop-in form of the reference's --gpu sieve (CudaBackend::run_batch,
+    src/cuda/backend.rs:96-249): host arrays in the reference's SoA layout
+    (packed reads, four pre-shifted window phases) through kira_sieve_run_batch,
+    which uploads, packs the window planes, scans and downloads per call."""
+    import ctypes as C
+    lib = pkg.lib()
+    n = lay["n"]
+    plen = np.full(n, (int(lay["read_lens"][0]) + 3) // 4, dtype=np.int32)
+    out_s = np.empty(n, dtype=np.int32)
+    out_m = np.empty(2 * n, dtype=np.int32)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+
+    def call():
+        st = lib.kira_sieve_run_batch(ctx.h, n, P(lay["reads"]), pkg.READ_BYTES_MAX, P(plen),
+                                      P(lay["read_lens"]), P(lay["ref_flat"]),
+                                      int(lay["ref_bytes_per_phase"]), P(lay["ref_offsets"]),
+                                      P(lay["ref_lens"]), P(lay["limits"]), P(out_s), P(out_m))
+        ctx.check(st, "kira_sieve_run_batch")
+
+    t, _ = median_of_runs(call, runs)
+    parity = bool(np.array_equal(out_s, cs) and np.array_equal(out_m[:n], cb) and
+                  np.array_equal(out_m[n:], csec))
+    return {"ms_per_call": t * 1e3, "reads_per_s": n_reads / t, "jobs_per_s": n / t,
+            "reads": n_reads, "jobs": n, "parity_vs_oracle": parity,
+            "host_bytes_per_call": int(lay["reads"].nbytes + lay["ref_bytes_per_phase"] +
+                                       n * 28 + 12 * n),
+            "note": "1 warm-up + median of 5 calls; host arrays built once outside the timing"}
 
 
 # the read kinds of tests/fastpath_data.py weighted like the chains that reach
@@ -683,6 +829,56 @@ def bench_stage4(pkg, ctx, args):
         print(f"[bench] stage4 cpu baseline {m / cpu_s:.0f} reads/s (1 thread); "
               f"parity={parity}", file=sys.stderr, flush=True)
     genome.close()
+    out["plain_mem_c1"] = bench_c1(pkg, ctx, args)
+    return out
+
+
+def bench_c1(pkg, ctx, args):
+    """The plain `mem` configuration's reads through the stage-4 driver: a
+    1 Mbp contig from the reference's test LCG, 10,000 single-end 150 bp reads
+    (1% substitutions, 0.1% 1-3 bp indels, truth in the name), one chain per
+    read from its truth span, band 32; reads/s per batch, truth recall of the
+    first record, and the CPU restatement on the same reads with parity."""
+    from tests.stage4_data import make_c1
+    from oracle import stage4 as S
+    t0 = time.time()
+    b = make_c1(args.c1_reads)
+    print(f"[bench] C1 (plain mem): {len(b['reads'])} reads generated in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    genome = pkg.Genome(ctx, b["genome"])
+    cfg = pkg.stage4_cfg(bandwidth=32)
+    x = pkg.Stage4Inputs(b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                         b["repeat_min_occ"])
+
+    def run():
+        return pkg.stage4_run(ctx, genome, x, cfg=cfg)
+
+    t, res = median_of_runs(run, 5)
+    got = res.alignments()
+    res.close()
+    n = len(b["reads"])
+    hit = 0
+    for name, recs in zip(b["names"], got):
+        s0, e0 = map(int, name.rsplit(":", 1)[1].split("-"))
+        hit += bool(recs) and recs[0]["ref_start"] < e0 and recs[0]["ref_end"] > s0
+    out = {"workload": "plain mem: 1 Mbp contig (reference test LCG), "
+                       f"{n} x 150 bp single-end reads, 1% subs + 0.1% 1-3 bp indels, "
+                       "truth-span chains, band 32, mem defaults, --gpu configuration",
+           "reads": n, "ms_per_batch": t * 1e3, "reads_per_s": n / t,
+           "truth_recall": hit / n}
+    if not args.no_cpu:
+        tc, (want, _) = median_of_runs(
+            lambda: S.run(b["contigs"], b["reads"], b["chains"], S.default_cfg(bandwidth=32),
+                          b["repeat_min_occ"]), 3)
+        out["parity_vs_oracle"] = got == want
+        out["cpu_baseline"] = {"value": n / tc, "unit": "reads/s", "cores": 1, "kind": "port",
+                               "sample": f"all {n} reads through the stage-4 restatement "
+                                         "(Python driver over the C oracle), 1 warm-up + "
+                                         "median of 3"}
+    print(f"[bench] C1: {out['ms_per_batch']:.1f} ms per batch, {out['reads_per_s']:.0f} "
+          f"reads/s, recall {out['truth_recall']:.4f}, parity {out.get('parity_vs_oracle')}",
+          file=sys.stderr, flush=True)
+    genome.close()
     return out
 
 
@@ -770,11 +966,17 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
     budget = args.serve_budget_gb << 30
     # warm-up (first launches, pool first touch)
     serve_stream(pkg, ctx, genome, reqs[:2], None, slots=4, budget=budget, graph=False)
+    # saturated capacity: every request queued at once, continuous (slots in
+    # flight) and plain (one request on the device at a time)
     lat, span, st, _, _ = serve_stream(pkg, ctx, genome, reqs, None, slots=args.serve_slots,
                                        budget=budget, graph=False)
     capacity = total_reads / span
-    print(f"[bench] serving capacity (all queued at once): {capacity:.0f} reads/s, "
-          f"peak in flight {st['peak_in_flight']}", file=sys.stderr, flush=True)
+    _, span1, _, _, _ = serve_stream(pkg, ctx, genome, reqs, None, slots=1, budget=budget,
+                                     graph=False)
+    capacity_plain = total_reads / span1
+    print(f"[bench] serving capacity (all queued at once): continuous {capacity:.0f} reads/s "
+          f"(peak in flight {st['peak_in_flight']}), plain {capacity_plain:.0f} reads/s, "
+          f"ratio {capacity / capacity_plain:.2f}", file=sys.stderr, flush=True)
     rate = args.serve_load * capacity / (total_reads / n_req)  # requests per second
     rng = np.random.default_rng(11)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
@@ -784,8 +986,10 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
                        "reads x 4 chain-gap windows, band 5%), resident genome, HBM-budgeted "
                        "admission",
            "requests": n_req, "reads": total_reads, "capacity_reads_per_s": capacity,
+           "capacity_plain_reads_per_s": capacity_plain,
+           "capacity_continuous_vs_plain": capacity / capacity_plain,
            "load": args.serve_load, "arrival_rate_per_s": rate,
-           "hbm_budget_gb": args.serve_budget_gb, "slots": args.serve_slots}
+           "hbm_budget_gib": args.serve_budget_gb, "slots": args.serve_slots}
     modes = [("continuous", args.serve_slots, False), ("continuous_graph", args.serve_slots, True),
              ("plain_one_batch_at_a_time", 1, False)]
     results = {}
@@ -797,7 +1001,7 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
                      "p99_ms": float(np.percentile(lat, 99) * 1e3),
                      "mean_ms": float(lat.mean() * 1e3), "reads_per_s": total_reads / span,
                      "peak_in_flight": st["peak_in_flight"],
-                     "peak_device_gb": st["peak_device_bytes"] / 1e9,
+                     "peak_device_gib": st["peak_device_bytes"] / 2 ** 30,
                      "sw_jobs_dropped_as_certified": skipped}
         print(f"[bench] serving {name}: p50 {out[name]['p50_ms']:.1f} ms, p99 "
               f"{out[name]['p99_ms']:.1f} ms, {out[name]['reads_per_s']:.0f} reads/s, peak in "
@@ -817,10 +1021,13 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
 
 
-def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label):
+def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label, sample_from=0,
+                 cpu_timing=True):
     """One device-resident banded SW batch (plan + timed run_dev launches);
-    GCUPS over the cells the reference algorithm evaluates, oracle parity and
-    the oracle's CPU rate on the first `cpu_pairs` jobs."""
+    GCUPS over the cells the reference algorithm evaluates; oracle parity
+    (score, end, cells) on `cpu_pairs` jobs from job `sample_from`, and, with
+    cpu_timing, the oracle's CPU rate on them (all the process's CPUs and one
+    thread, 1 warm-up + median of 5)."""
     import torch
     dev = torch.device("cuda:0")
     n = sw["n"]
@@ -829,6 +1036,7 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label):
     t0 = time.time()
     plan = pkg.BandedSwPlan(ctx, sw["read_len"], sw["band"], cfg)
     t_plan = time.time() - t0
+    routes = plan.route_counts()
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     d = dict(reads=T(sw["reads"]), read_off=T(sw["read_off"]), read_len=T(sw["read_len"]),
              refs=T(sw["refs"]), ref_off=T(sw["ref_off"]), ref_len=T(sw["ref_len"]),
@@ -856,10 +1064,12 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label):
     gcups = cells / (ms * 1e-3) / 1e9
     ops = cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12
     print(f"[bench] {label}: {n} jobs, {ms:.2f} ms/batch ({wall_ms:.2f} wall), "
-          f"{cells / n:.0f} cells/job, {gcups:.1f} GCUPS, plan {plan.device_bytes / 1e9:.1f} GB",
+          f"{cells / n:.0f} cells/job, {gcups:.1f} GCUPS, plan {plan.device_bytes / 2 ** 30:.1f} "
+          f"GiB, routes {routes}",
           file=sys.stderr, flush=True)
     out = {"jobs": n, "cells": cells, "ms_per_batch": ms, "wall_ms_per_batch": wall_ms,
-           "gcups": gcups, "plan_ms": t_plan * 1000, "plan_device_gb": plan.device_bytes / 1e9,
+           "gcups": gcups, "plan_ms": t_plan * 1000,
+           "plan_device_gib": plan.device_bytes / 2 ** 30, "routes": routes,
            "steps": steps,
            "roofline": {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
                         "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS,
@@ -871,27 +1081,40 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label):
     if not args.no_cpu and cpu_pairs > 0:
         sys.path.insert(0, ROOT)
         from oracle import oracle as O
-        m = min(cpu_pairs, n)
+        m0 = min(sample_from, n)
+        m = min(cpu_pairs, n - m0)
         threads = len(os.sched_getaffinity(0))
         oc = O.cfg(1, 4, 6, 1, 32, 100, 5)
-        a = [sw["reads"], sw["read_off"][:m], sw["read_len"][:m], sw["refs"], sw["ref_off"][:m],
-             sw["ref_len"][:m], sw["offset"][:m], sw["abort"][:m]]
-        O.banded_sw_batch_mt(*a, oc, bands=sw["band"][:m], threads=threads)  # warm-up
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            sc, en, cc = O.banded_sw_batch_mt(*a, oc, bands=sw["band"][:m], threads=threads)
-            ts.append(time.perf_counter() - t0)
-        t = float(np.median(ts))
+
+        def job_args(a, b):
+            sl = slice(a, b)
+            return [sw["reads"], sw["read_off"][sl], sw["read_len"][sl], sw["refs"],
+                    sw["ref_off"][sl], sw["ref_len"][sl], sw["offset"][sl], sw["abort"][sl]]
+
+        full = job_args(m0, m0 + m)
+        run_all = lambda: O.banded_sw_batch_mt(*full, oc, bands=sw["band"][m0:m0 + m],
+                                               threads=threads)
+        if cpu_timing:
+            t, (sc, en, cc) = median_of_runs(run_all)
+            m1 = max(1, m // 16)
+            one = job_args(m0, m0 + m1)
+            t1, (_, _, c1) = median_of_runs(
+                lambda: O.banded_sw_batch_mt(*one, oc, bands=sw["band"][m0:m0 + m1], threads=1))
+            out["cpu_baseline"] = {
+                "value": float(cc.sum()) / t / 1e9, "unit": "GCUPS", "cores": threads,
+                "kind": "port", "single_thread_value": float(c1.sum()) / t1 / 1e9,
+                "sample": f"jobs {m0}..{m0 + m} of the same batch (scalar banded_sw_with, "
+                          f"pthreads over jobs, {threads} threads = the CPUs this process may "
+                          f"use; the first {m1} on 1 thread), 1 warm-up + median of 5"}
+            print(f"[bench] {label} cpu baseline {cc.sum() / t / 1e9:.2f} GCUPS on {threads} "
+                  f"threads, {c1.sum() / t1 / 1e9:.2f} on 1", file=sys.stderr, flush=True)
+        else:
+            sc, en, cc = run_all()
         out["parity_vs_oracle_sample"] = bool(
-            np.array_equal(sc, r[:m, 0]) and np.array_equal(en, r[:m, 2]) and
-            np.array_equal(cc, r[:m, 8].astype(np.int64)))
-        out["cpu_baseline"] = {"value": float(cc.sum()) / t / 1e9, "unit": "GCUPS",
-                               "cores": threads, "kind": "port",
-                               "sample": f"first {m} jobs of the same batch (scalar "
-                                         f"banded_sw_with, pthreads over jobs), median of 3"}
-        print(f"[bench] {label} cpu baseline {cc.sum() / t / 1e9:.2f} GCUPS on {threads} "
-              f"threads; parity={out['parity_vs_oracle_sample']}", file=sys.stderr, flush=True)
+            np.array_equal(sc, r[m0:m0 + m, 0]) and np.array_equal(en, r[m0:m0 + m, 2]) and
+            np.array_equal(cc, r[m0:m0 + m, 8].astype(np.int64)))
+        print(f"[bench] {label} parity on jobs {m0}..{m0 + m}: "
+              f"{out['parity_vs_oracle_sample']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -905,27 +1128,76 @@ def concat_sw(a, b):
     return out
 
 
+# Shape buckets of the banded SW autotuner (include/kira_hip.h): band buckets
+# end at these bands, read lengths bucket by log2, group sizes by log4.
+SW_BAND_EDGES = [4, 8, 16, 32, 39, 64, 71, 128, 135, 207, 271]
+
+
+def sw_tune_shapes(*workloads):
+    """One (band, read_len, n_jobs) shape per (band bucket, length bucket) group
+    of each workload's jobs (each workload is one batch): its largest band, its
+    median read length and its job count."""
+    shapes = []
+    for sw in workloads:
+        bb = np.searchsorted(np.array(SW_BAND_EDGES), sw["band"], side="left")
+        lb = np.floor(np.log2(np.maximum(sw["read_len"], 1))).astype(np.int64)
+        key = bb * 64 + lb
+        for k in np.unique(key):
+            m = key == k
+            shapes.append((int(sw["band"][m].max()), int(np.median(sw["read_len"][m])),
+                           int(m.sum())))
+    return shapes
+
+
 def bench_sw(pkg, ctx, genome, args, stream, lw):
     """Uneven banded SW (1M pairs, L U{100..300}, band U{16,32,64,128}, adaptive,
     x-drop 100, no abort) in one device-resident batch; long-read chain-gap
-    windows (500-5000 bp, band 5%); and both in one queue."""
+    windows (500-5000 bp, band 5%); and both in one queue.  Each runs first on
+    the built-in routing, then on the routes the autotuner picks for the
+    workloads' shapes (the headline figures)."""
     t0 = time.time()
     sw = make_sw_workload(genome, n_pairs=args.sw_pairs)
     print(f"[bench] SW workload: {sw['n']} pairs generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
+    pkg.sw_tune_clear(ctx)
+    builtin = {"short": run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, 0,
+                                     "SW short pairs (built-in routing)")}
+    if lw is not None:
+        builtin["long"] = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 0,
+                                       "SW long windows (built-in routing)")
+    mixed = concat_sw(sw, lw) if lw is not None else None
+    shapes = sw_tune_shapes(*[x for x in (sw, lw, mixed) if x is not None])
+    t0 = time.time()
+    cache = args.sw_tune_cache or None
+    tuned = pkg.sw_autotune(ctx, shapes, pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5), pkg.sw_knobs(),
+                            cache_path=cache)
+    t_tune = time.time() - t0
+    chosen = [dict(band=b, read_len=l, jobs=n, route=t["route"], candidates=t["candidates"],
+                   ms_best=t["ms_best"], ms_builtin=t["ms_builtin"])
+              for (b, l, n), t in zip(shapes, tuned)]
+    print(f"[bench] SW autotune: {len(shapes)} shapes in {t_tune:.1f}s; "
+          f"{sum(c['route'] != 'built-in' for c in chosen)} re-routed", file=sys.stderr,
+          flush=True)
     out = run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, args.cpu_sample_pairs,
                        "SW short pairs")
+    out["builtin_routing"] = {k: builtin["short"][k] for k in ("ms_per_batch", "gcups",
+                                                                "routes")}
+    out["tuned_vs_builtin"] = out["gcups"] / builtin["short"]["gcups"]
+    out["autotune"] = {"shapes": chosen, "seconds": t_tune, "cache": cache}
     out["workload"] = ("uneven banded SW: 1M pairs, L U{100..300}, band U{16,32,64,128}, "
                        "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
                        "routed per job")
     out["pairs"] = out["jobs"]
     if lw is not None:
         lo = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 256, "SW long windows")
+        lo["builtin_routing"] = {k: builtin["long"][k] for k in ("ms_per_batch", "gcups",
+                                                                  "routes")}
+        lo["tuned_vs_builtin"] = lo["gcups"] / builtin["long"]["gcups"]
         lo["workload"] = ("long-read chain-gap windows: 500-5000 bp, band ceil(5% len) "
                           "(25-250), 5% subs, 2.5% ins, 2.5% del, exact int32 (int16 storage only where proven)")
         out["long_windows"] = lo
-        mixed = concat_sw(sw, lw)
-        mx = run_sw_batch(pkg, ctx, mixed, args, stream, args.sw_steps, 0, "SW mixed queue")
+        mx = run_sw_batch(pkg, ctx, mixed, args, stream, args.sw_steps, 512, "SW mixed queue",
+                          sample_from=sw["n"] - 256, cpu_timing=False)
         mx["workload"] = "the short pairs and the long windows in one plan / one queue"
         out["mixed_queue"] = mx
     return out

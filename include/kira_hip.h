@@ -361,6 +361,50 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
                                    const kira_sw_jobs_dev *jobs, const kira_sw_cfg *cfg,
                                    const kira_sw_knobs *knobs);
 
+/*
+ * Banded SW autotuner (the reference picks lane groups per (L, W) shape and a
+ * lane count per host, src/pipeline/stage4_alignment.rs:1064-1069, 1153-1160,
+ * src/simd/mod.rs:30; here the choice is a routing of each job to a kernel
+ * form, timed on the device).  A shape key is (band bucket, read-length
+ * bucket, job-count bucket): band buckets end at bands 4, 8, 16, 32, 39, 64,
+ * 71, 128, 135, 207, 271 (the bands of one bucket can take the same routes),
+ * read lengths bucket by log2 and the jobs of a batch's (band bucket, length
+ * bucket) group by log4.  Routes: 1 one lane per job (int16 or int32 band by
+ * the score bound), 2 + k the cooperative shape k of (P, G) = (5, 16),
+ * (9, 16), (17, 16), (13, 32), (17, 32).  Routing never changes results.
+ *
+ * kira_sw_autotune times, for each shape, every eligible route on a seeded
+ * synthetic batch of that shape (read_len bases, band, n_jobs jobs up to
+ * 262144; reads with 3% substitutions and 0.5% indels against windows of the
+ * clamp_window geometry), checks that every route returns identical results,
+ * keeps the fastest in the context's table and, when cache_path is not NULL,
+ * writes the table to that file (lines "<device> <key> <route> <ms> <ms of the
+ * built-in routing>", keyed by the device's architecture name and CU count;
+ * other devices' lines are kept).  kira_sw_tune_load reads this device's lines
+ * of such a file into the context; kira_sw_tune_clear empties the table.
+ * Plans built on the context afterwards (kira_banded_sw_plan_create, every
+ * banded SW entry point, servers created on it) route by the table.  Not to
+ * be called while a server created on the context runs other work.
+ */
+typedef struct {
+    int32_t band, read_len, n_jobs;
+} kira_sw_shape;
+typedef struct {
+    uint32_t key;           /* shape key the choice is stored under */
+    int32_t route;          /* chosen route */
+    int32_t n_candidates;   /* routes timed */
+    float ms_best;          /* device time of one batch on the chosen route */
+    float ms_builtin;       /* same batch on the built-in routing */
+} kira_sw_tune_result;
+kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_sw_knobs *knobs,
+                             int32_t n_shapes, const kira_sw_shape *shapes,
+                             const char *cache_path, kira_sw_tune_result *out);
+kira_status kira_sw_tune_load(kira_ctx *ctx, const char *cache_path, int32_t *n_loaded);
+void kira_sw_tune_clear(kira_ctx *ctx);
+/* Jobs of a plan per route: counts[1] one lane per job, counts[2 + k]
+ * cooperative shape k (counts has 7 entries; counts[0] is 0). */
+void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts);
+
 /* ------------------------------------------------------------------ records */
 
 /* One alignment record of the path (the fields of Alignment, src/types.rs,

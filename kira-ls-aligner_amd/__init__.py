@@ -261,6 +261,15 @@ class SamCfg(C.Structure):
         ("read_group", C.c_char_p)]
 
 
+class SwShape(C.Structure):
+    _fields_ = [("band", C.c_int32), ("read_len", C.c_int32), ("n_jobs", C.c_int32)]
+
+
+class SwTuneResult(C.Structure):
+    _fields_ = [("key", C.c_uint32), ("route", C.c_int32), ("n_candidates", C.c_int32),
+                ("ms_best", C.c_float), ("ms_builtin", C.c_float)]
+
+
 class SwJobsDev(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("reads", "read_off", "read_len", "refs", "ref_off",
                                           "ref_len", "offset", "band", "abort_score", "res",
@@ -317,6 +326,11 @@ SIGNATURES = {
     "kira_banded_sw_plan_cigar_off": (C.POINTER(C.c_int64), [_P]),
     "kira_banded_sw_run_dev": (C.c_int, [_P, _P, C.POINTER(SwJobsDev), C.POINTER(SwCfg),
                                          C.POINTER(SwKnobs)]),
+    "kira_sw_autotune": (C.c_int, [_P, C.POINTER(SwCfg), C.POINTER(SwKnobs), C.c_int32,
+                                   C.POINTER(SwShape), C.c_char_p, C.POINTER(SwTuneResult)]),
+    "kira_sw_tune_load": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
+    "kira_sw_tune_clear": (None, [_P]),
+    "kira_banded_sw_plan_route_counts": (None, [_P, _P]),
     "kira_stage4_run": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P, _P, _P,
                                   C.POINTER(Chain), C.POINTER(Stage4Cfg),
                                   C.POINTER(C.c_void_p)]),
@@ -742,6 +756,12 @@ class BandedSwPlan:
         self.cigar_off = (np.ctypeslib.as_array(ptr, shape=(self.n,)).copy() if self.n
                           else np.zeros(0, dtype=np.int64))
 
+    def route_counts(self) -> dict:
+        """Jobs per route (kira_banded_sw_plan_route_counts), by SW_ROUTES name."""
+        c = np.zeros(7, dtype=np.int32)
+        lib().kira_banded_sw_plan_route_counts(self.h, _np_ptr(c))
+        return {SW_ROUTES[r]: int(c[r]) for r in range(1, 7) if c[r]}
+
     def close(self):
         if self.h:
             lib().kira_banded_sw_plan_destroy(self.h)
@@ -772,6 +792,37 @@ def banded_sw_run_dev(ctx: Context, plan: BandedSwPlan, *, reads, read_off, read
         cigar)])
     ctx.check(lib().kira_banded_sw_run_dev(ctx.h, plan.h, C.byref(j), C.byref(cfg),
                                            C.byref(knobs)), "banded_sw_run_dev")
+
+
+SW_ROUTES = {0: "built-in", 1: "lane per job", 2: "(5, 16)", 3: "(9, 16)", 4: "(17, 16)",
+             5: "(13, 32)", 6: "(17, 32)"}
+
+
+def sw_autotune(ctx: Context, shapes, cfg: SwCfg, knobs: Optional[SwKnobs] = None,
+                cache_path: Optional[str] = None):
+    """kira_sw_autotune: shapes = [(band, read_len, n_jobs)].  Returns one dict per
+    shape: key, route (SW_ROUTES), candidates, ms_best, ms_builtin."""
+    knobs = knobs or sw_knobs()
+    n = len(shapes)
+    arr = (SwShape * max(n, 1))(*[SwShape(*map(int, sh)) for sh in shapes])
+    out = (SwTuneResult * max(n, 1))()
+    ctx.check(lib().kira_sw_autotune(ctx.h, C.byref(cfg), C.byref(knobs), n, arr,
+                                     cache_path.encode() if cache_path else None, out),
+              "sw_autotune")
+    return [dict(key=int(o.key), route=SW_ROUTES[o.route], route_id=int(o.route),
+                 candidates=int(o.n_candidates), ms_best=float(o.ms_best),
+                 ms_builtin=float(o.ms_builtin)) for o in out[:n]]
+
+
+def sw_tune_load(ctx: Context, cache_path: str) -> int:
+    """kira_sw_tune_load: this device's routes from a cache file into the context."""
+    k = C.c_int32()
+    ctx.check(lib().kira_sw_tune_load(ctx.h, cache_path.encode(), C.byref(k)), "sw_tune_load")
+    return int(k.value)
+
+
+def sw_tune_clear(ctx: Context):
+    lib().kira_sw_tune_clear(ctx.h)
 
 
 def align_batch_simd(ctx: Context, batches, cfg: SwCfg, mode: str = "avx2",

@@ -145,9 +145,35 @@ struct SwPlanLayout {
     std::vector<int64_t> packet_trace_off, packet_meta_off, cigar_off;
     int64_t trace_words = 0, meta_rows = 0, cigar_words = 0;
 };
-// nullptr on success, else the reason the batch cannot run.
+// Routing of banded SW jobs chosen by the autotuner per shape key (band
+// bucket, read-length bucket, job-count bucket; sw_tune_key).  A route is
+// kSwRouteLane (one lane per job, int16 or int32 band by the score bound) or
+// kSwRouteCoop + k (cooperative shape k of the (P, G) table); shapes without
+// an entry keep the built-in routing.
+constexpr int kSwRouteDefault = 0, kSwRouteLane = 1, kSwRouteCoop = 2;
+struct SwTuneTable {
+    std::unordered_map<uint32_t, uint8_t> route;
+};
+// Bucket of a band: the bands of one bucket have the same eligible routes.
+int sw_band_bucket(int32_t w);
+// Bucket of a read length (log2) and of a job count (log4).
+int sw_len_bucket(int32_t read_len);
+int sw_count_bucket(int64_t jobs);
+uint32_t sw_tune_key(int band_bucket, int len_bucket, int count_bucket);
+// Routes a job of band w may take under cfg (bit r set: route r eligible;
+// bit kSwRouteDefault always set).
+uint32_t sw_routes_for(int32_t w, int32_t read_len, const kira_sw_cfg &cfg);
+// Number of cooperative shapes and shape k's (P, G).
+int sw_num_coop_shapes();
+void sw_coop_shape(int k, int *P, int *G);
+
+// nullptr on success, else the reason the batch cannot run.  With a tuning
+// table, jobs whose shape key has an entry take the tuned route (if eligible);
+// force_route >= 0 sends every job that is eligible for it there (the tuner's
+// candidates).
 const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *band,
-                           const kira_sw_cfg &cfg, SwPlanLayout &out);
+                           const kira_sw_cfg &cfg, SwPlanLayout &out,
+                           const SwTuneTable *tune = nullptr, int force_route = -1);
 // The layout's tables as one block (host copy written by sw_plan_pack_tables,
 // device copy bound by sw_plan_bind; both 256-byte aligned sections).
 size_t sw_plan_table_bytes(const SwPlanLayout &L);
@@ -449,6 +475,8 @@ struct kira_ctx {
     // pinned host staging buffers (kira::pinned_buf)
     void *pin[4] = {};
     size_t pin_cap[4] = {};
+    // banded SW routes chosen by the autotuner (kira_sw_autotune / kira_sw_tune_load)
+    kira::SwTuneTable tune;
 };
 
 struct kira_genome {

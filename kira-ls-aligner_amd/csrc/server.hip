@@ -146,6 +146,8 @@ struct kira_server {
     bool stop = false;
     std::string err;
     kira_server_stats stats{};
+    // the creating context's banded SW routes, copied at creation
+    kira::SwTuneTable tune;
 
     void run();
     bool layout(Request &r, std::string &why);
@@ -157,7 +159,7 @@ struct kira_server {
 bool kira_server::layout(Request &r, std::string &why) {
     if (r.laid_out) return true;
     if (const char *w = kira::sw_plan_layout(r.nw, r.w_rlen.data(), r.w_band.data(), cfg.cfg,
-                                             r.plan)) {
+                                             r.plan, &tune)) {
         why = w;
         return false;
     }
@@ -480,6 +482,7 @@ kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g, const kira_s
     kira_server *s = new (std::nothrow) kira_server();
     if (!s) return KIRA_ERR_OUT_OF_MEMORY;
     s->device = ctx->device;
+    s->tune = ctx->tune;
     s->genome = g;
     s->cfg = *c;
     hipError_t e = hipSetDevice(ctx->device);

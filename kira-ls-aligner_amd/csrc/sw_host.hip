@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <new>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "internal.hpp"
@@ -98,8 +99,67 @@ using kira::DevMem;
 
 namespace kira {
 
+// Band bucket edges: the largest band of each bucket.  Each edge is where the
+// smallest lane class (2w + 1 <= 9, 17, 33, 65, 129, 257) or the set of
+// cooperative shapes that hold 2w + 1 (80, 144, 272, 416, 544 positions)
+// changes, so every band of a bucket has the same eligible routes.
+constexpr int32_t kBandEdges[] = {4, 8, 16, 32, 39, 64, 71, 128, 135, 207, 271};
+constexpr int kNumBandBuckets = sizeof(kBandEdges) / sizeof(kBandEdges[0]);
+
+int sw_band_bucket(int32_t w) {
+    for (int k = 0; k < kNumBandBuckets; k++)
+        if (w <= kBandEdges[k]) return k;
+    return kNumBandBuckets;
+}
+
+int sw_len_bucket(int32_t read_len) {
+    int b = 0;
+    for (uint32_t v = (uint32_t)std::max(read_len, 1); v > 1; v >>= 1) b++;
+    return std::min(b, 31);
+}
+
+int sw_count_bucket(int64_t jobs) {
+    int b = 0;
+    for (int64_t v = std::max<int64_t>(jobs, 1); v >= 4; v >>= 2) b++;
+    return std::min(b, 15);
+}
+
+uint32_t sw_tune_key(int band_bucket, int len_bucket, int count_bucket) {
+    return ((uint32_t)band_bucket << 16) | ((uint32_t)len_bucket << 8) | (uint32_t)count_bucket;
+}
+
+int sw_num_coop_shapes() { return kNumCoop; }
+
+void sw_coop_shape(int k, int *P, int *G) {
+    *P = kCoop[k].p;
+    *G = kCoop[k].g;
+}
+
+namespace {
+// The plan's cost bounds (below): the 16-lane jobs keep their F scan and
+// arg-max key exact only with costs <= 2^20 and read_len * match < 2^21.
+bool coop_costs_ok(const kira_sw_cfg &cfg) {
+    const int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
+    const int32_t kCoopCost = 1 << 20;
+    return cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
+           cfg.gap_extend >= 0 && cfg.match_score <= kCoopCost && cfg.mismatch <= kCoopCost &&
+           go <= kCoopCost;
+}
+}  // namespace
+
+uint32_t sw_routes_for(int32_t w, int32_t read_len, const kira_sw_cfg &cfg) {
+    w = std::max(w, 1);
+    uint32_t r = 1u << kSwRouteDefault;
+    if (lane_class_of(w) >= 0) r |= 1u << kSwRouteLane;
+    if (coop_costs_ok(cfg) && (int64_t)read_len * cfg.match_score < (1 << 21))
+        for (int k = 0; k < kNumCoop; k++)
+            if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) r |= 1u << (kSwRouteCoop + k);
+    return r;
+}
+
 const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *band,
-                           const kira_sw_cfg &cfg, SwPlanLayout &L) {
+                           const kira_sw_cfg &cfg, SwPlanLayout &L, const SwTuneTable *tune,
+                           int force_route) {
     L = SwPlanLayout();
     L.n = n;
     std::vector<int> cls(n);
@@ -110,10 +170,7 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
                              cfg.gap_extend >= 0 && go <= 16000 && cfg.mismatch <= 16000;
     // 16-lane jobs: the F scan stays inside int32 and the arg-max key
     // (score << 10 | position) needs score < 2^21
-    const int32_t kCoopCost = 1 << 20;
-    const bool coop_costs = cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open >= 0 &&
-                            cfg.gap_extend >= 0 && cfg.match_score <= kCoopCost &&
-                            cfg.mismatch <= kCoopCost && go <= kCoopCost;
+    const bool coop_costs = coop_costs_ok(cfg);
     std::vector<uint8_t> coop_ok(n);
     for (int32_t k = 0; k < n; k++) {
         int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
@@ -156,6 +213,36 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
             if ((c == c65 || c == c65p || c == c129 || c == c129p) && count[c] < kMinLaneJobs &&
                 coop_ok[k])
                 cls[k] = coop_class_of(std::max(band ? band[k] : cfg.bandwidth, 1));
+        }
+    }
+    // tuned (or forced) routes override the built-in choice for the jobs that
+    // may take them; the shape key counts the jobs of each (band bucket,
+    // length bucket) group of the batch
+    if (force_route >= 0 || (tune && !tune->route.empty())) {
+        std::unordered_map<uint32_t, int64_t> group;
+        std::vector<uint32_t> gk(n);
+        for (int32_t k = 0; k < n; k++) {
+            const int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
+            gk[k] = sw_tune_key(sw_band_bucket(w), sw_len_bucket(read_len[k]), 0);
+            group[gk[k]]++;
+        }
+        for (int32_t k = 0; k < n; k++) {
+            int route = force_route;
+            if (route < 0) {
+                auto it = tune->route.find(gk[k] | (uint32_t)sw_count_bucket(group[gk[k]]));
+                if (it == tune->route.end()) continue;
+                route = it->second;
+            }
+            const int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
+            if (route <= kSwRouteDefault || !((sw_routes_for(w, read_len[k], cfg) >> route) & 1u))
+                continue;
+            if (route == kSwRouteLane) {
+                const bool pack =
+                    costs_small && (int64_t)read_len[k] * cfg.match_score < 32767;
+                cls[k] = 2 * lane_class_of(w) + (pack ? 1 : 0);
+            } else {
+                cls[k] = 2 * kNumLane + (route - kSwRouteCoop);
+            }
         }
     }
     // packets: jobs sorted by (band class, read length descending)
@@ -346,7 +433,7 @@ kira_status kira_banded_sw_plan_create(kira_ctx *ctx, int32_t n, const int32_t *
     *out = nullptr;
     kira_sw_plan *p = new (std::nothrow) kira_sw_plan();
     if (!p) return KIRA_ERR_OUT_OF_MEMORY;
-    if (const char *why = kira::sw_plan_layout(n, read_len, band, *cfg, p->L)) {
+    if (const char *why = kira::sw_plan_layout(n, read_len, band, *cfg, p->L, &ctx->tune)) {
         delete p;
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, why);
     }
@@ -380,6 +467,21 @@ void kira_banded_sw_plan_destroy(kira_sw_plan *plan) {
 
 int64_t kira_banded_sw_plan_cigar_words(const kira_sw_plan *plan) {
     return plan ? plan->L.cigar_words : 0;
+}
+
+void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts) {
+    if (!counts) return;
+    for (int r = 0; r < kira::kSwRouteCoop + kNumCoop; r++) counts[r] = 0;
+    if (!plan) return;
+    for (int c = 0; c < kNumClasses; c++) {
+        int32_t jobs = 0;
+        for (int32_t p = 0; p < plan->L.class_count[c]; p++) {
+            const int32_t pk = plan->L.class_first[c] + p;
+            for (int l = 0; l < 64; l++) jobs += plan->L.packet_jobs[(size_t)pk * 64 + l] >= 0;
+        }
+        counts[class_is_coop(c) ? kira::kSwRouteCoop + (c - 2 * kNumLane) : kira::kSwRouteLane] +=
+            jobs;
+    }
 }
 
 int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan) {
