@@ -365,6 +365,9 @@ def main():
                     help="run only the stage-4 driver section (for profiling)")
     ap.add_argument("--fastpath-chains", type=int, default=200_000)
     ap.add_argument("--fastpath-steps", type=int, default=3)
+    ap.add_argument("--sw-only", action="store_true",
+                    help="run only the banded SW section (for profiling)")
+    ap.add_argument("--sw-genome-mbp", type=int, default=100)
     ap.add_argument("--fastpath-only", action="store_true",
                     help="run only the fast DP cascade section (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -391,6 +394,16 @@ def main():
     if args.fastpath_only:
         ctx = pkg.Context(0)
         print(json.dumps({"fastpath": bench_fastpath(pkg, ctx, args)}), flush=True)
+        return
+    if args.sw_only:
+        # the SW legs alone (for profiling): genome as in the headline workload
+        ctx = pkg.Context(0)
+        genome = make_genome(np.random.default_rng(2), args.sw_genome_mbp * 1_000_000)[0]
+        lw = make_long_workload(genome, n_windows=args.long_windows) \
+            if args.long_windows > 0 else None
+        stream = torch.cuda.Stream(torch.device("cuda:0"))
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"sw": bench_sw(pkg, ctx, genome, args, stream, lw)}), flush=True)
         return
     t_gen = time.time()
     work = make_sieve_workload(n_reads=args.reads)
@@ -1018,6 +1031,11 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
     return out
 
 
+# kira_banded_sw_plan_job_class ids: lane-per-job band sizes (int32 / int16
+# storage), then the cooperative (P, G) shapes
+SW_CLASS_NAMES = [f"lane C={c}{' i16' if p else ''}" for c in (9, 17, 33, 65, 129, 257)
+                  for p in (0, 1)] + ["coop (5,16)", "coop (9,16)", "coop (17,16)",
+                                      "coop (13,32)", "coop (17,32)"]
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
 
 
@@ -1037,6 +1055,7 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label, sample_fro
     plan = pkg.BandedSwPlan(ctx, sw["read_len"], sw["band"], cfg)
     t_plan = time.time() - t0
     routes = plan.route_counts()
+    job_class = plan.job_class()
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     d = dict(reads=T(sw["reads"]), read_off=T(sw["read_off"]), read_len=T(sw["read_len"]),
              refs=T(sw["refs"]), ref_off=T(sw["ref_off"]), ref_len=T(sw["ref_len"]),
@@ -1061,6 +1080,12 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label, sample_fro
     ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     r = res.cpu().numpy()
     cells = int(r[:, 8].astype(np.int64).sum())
+    # cells and jobs per kernel class, in launch order (one dispatch each)
+    per_class = {}
+    for c in np.unique(job_class):
+        m = job_class == c
+        per_class[SW_CLASS_NAMES[int(c)]] = {"jobs": int(m.sum()),
+                                             "cells": int(r[m, 8].astype(np.int64).sum())}
     gcups = cells / (ms * 1e-3) / 1e9
     ops = cells * SW_OPS_PER_CELL / (ms * 1e-3) / 1e12
     print(f"[bench] {label}: {n} jobs, {ms:.2f} ms/batch ({wall_ms:.2f} wall), "
@@ -1070,6 +1095,7 @@ def run_sw_batch(pkg, ctx, sw, args, stream, steps, cpu_pairs, label, sample_fro
     out = {"jobs": n, "cells": cells, "ms_per_batch": ms, "wall_ms_per_batch": wall_ms,
            "gcups": gcups, "plan_ms": t_plan * 1000,
            "plan_device_gib": plan.device_bytes / 2 ** 30, "routes": routes,
+           "classes": per_class,
            "steps": steps,
            "roofline": {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
                         "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS,

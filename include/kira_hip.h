@@ -404,6 +404,11 @@ void kira_sw_tune_clear(kira_ctx *ctx);
 /* Jobs of a plan per route: counts[1] one lane per job, counts[2 + k]
  * cooperative shape k (counts has 7 entries; counts[0] is 0). */
 void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts);
+/* Kernel class of each job of a plan (n entries): 2 b + packed for one lane
+ * per job with a band of 9, 17, 33, 65, 129, 257 positions (b = 0..5), 12 + k
+ * for cooperative shape k.  The plan launches its non-empty classes in this
+ * order, one kernel each. */
+void kira_banded_sw_plan_job_class(const kira_sw_plan *plan, int32_t *cls);
 
 /* ------------------------------------------------------------------ records */
 

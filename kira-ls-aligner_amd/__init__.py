@@ -331,6 +331,7 @@ SIGNATURES = {
     "kira_sw_tune_load": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
     "kira_sw_tune_clear": (None, [_P]),
     "kira_banded_sw_plan_route_counts": (None, [_P, _P]),
+    "kira_banded_sw_plan_job_class": (None, [_P, _P]),
     "kira_stage4_run": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P, _P, _P,
                                   C.POINTER(Chain), C.POINTER(Stage4Cfg),
                                   C.POINTER(C.c_void_p)]),
@@ -761,6 +762,12 @@ class BandedSwPlan:
         c = np.zeros(7, dtype=np.int32)
         lib().kira_banded_sw_plan_route_counts(self.h, _np_ptr(c))
         return {SW_ROUTES[r]: int(c[r]) for r in range(1, 7) if c[r]}
+
+    def job_class(self) -> np.ndarray:
+        """Kernel class per job (kira_banded_sw_plan_job_class)."""
+        c = np.zeros(max(self.n, 1), dtype=np.int32)
+        lib().kira_banded_sw_plan_job_class(self.h, _np_ptr(c))
+        return c[:self.n]
 
     def close(self):
         if self.h:

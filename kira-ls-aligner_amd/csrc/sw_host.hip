@@ -484,6 +484,18 @@ void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts)
     }
 }
 
+void kira_banded_sw_plan_job_class(const kira_sw_plan *plan, int32_t *cls) {
+    if (!plan || !cls) return;
+    for (int c = 0; c < kNumClasses; c++)
+        for (int32_t p = 0; p < plan->L.class_count[c]; p++) {
+            const int32_t pk = plan->L.class_first[c] + p;
+            for (int l = 0; l < 64; l++) {
+                const int32_t job = plan->L.packet_jobs[(size_t)pk * 64 + l];
+                if (job >= 0) cls[job] = c;
+            }
+        }
+}
+
 int64_t kira_banded_sw_plan_device_bytes(const kira_sw_plan *plan) {
     return plan ? plan->device_bytes : 0;
 }
