@@ -90,11 +90,15 @@ struct Band {
 
 // One row of the band.  MASK: some positions fall outside the band or the
 // window and must hold pads.
+// The up neighbour of the last position (q = C - 1) lies one position past the
+// stored band: it is a real cell when the centre moved by less than one
+// column (the band was shifted towards higher positions and its top element
+// dropped), else outside the previous band.  The caller passes it in.
 template <int C, bool PACK, bool MASK>
 __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_t *ref,
                                                  int32_t base, int32_t js, int32_t je, uint8_t qb,
                                                  const SwLaunch &a, uint32_t *trace_row,
-                                                 int lane) {
+                                                 int lane, int32_t top_h, int32_t top_e) {
     const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
     // the F chain lives in registers only: exact i32 absent values
     const int32_t fpadH = kMin4 + go, fpadF = kMin4 + ge;
@@ -125,8 +129,8 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
 #pragma unroll
     for (int q = 0; q < C; q++) {
         int32_t d = bd.h(q);
-        int32_t uh = q + 1 < C ? bd.h(q + 1) : bd.padH;
-        int32_t ue = q + 1 < C ? bd.e(q + 1) : bd.padE;
+        int32_t uh = q + 1 < C ? bd.h(q + 1) : top_h;
+        int32_t ue = q + 1 < C ? bd.e(q + 1) : top_e;
         bool valid = true;
         if (MASK) valid = (base + q >= js) && (base + q <= je);
         // edge rows may reach past the window (and the arena): guarded loads
@@ -181,9 +185,15 @@ __host__ __device__ __forceinline__ Row band_row(Band<C, PACK> &bd, const uint8_
     return r;
 }
 
-// Shift the band so that position q holds what was at q + delta.
+// Shift the band so that position q holds what was at q + delta.  top_h /
+// top_e receive what was at C + delta (the up neighbour of position C - 1
+// in the shifted band): the element the last single-step shift towards
+// higher positions dropped, else the pads.
 template <int C, bool PACK>
-__host__ __device__ __forceinline__ void shift_band(Band<C, PACK> &bd, int32_t delta) {
+__host__ __device__ __forceinline__ void shift_band(Band<C, PACK> &bd, int32_t delta,
+                                                    int32_t &top_h, int32_t &top_e) {
+    top_h = bd.padH;
+    top_e = bd.padE;
     if (delta >= C || delta <= -C) {
         bd.clear();
         return;
@@ -195,6 +205,8 @@ __host__ __device__ __forceinline__ void shift_band(Band<C, PACK> &bd, int32_t d
         delta--;
     }
     while (delta < 0) {
+        top_h = bd.h(C - 1);
+        top_e = bd.e(C - 1);
 #pragma unroll
         for (int q = C - 1; q > 0; q--) bd.move(q, q - 1);
         bd.set(0, bd.padH, bd.padE);
@@ -245,11 +257,12 @@ __host__ __device__ int2 sw_job(const SwLaunch &a, int32_t job, int lane, int32_
             meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, 1);
             continue;
         }
+        int32_t top_h = bd.padH, top_e = bd.padE;
         if (prev_empty) {
             bd.clear();
         } else {
             int32_t delta = c - prev_c - 1;
-            if (delta != 0) shift_band<C, PACK>(bd, delta);
+            if (delta != 0) shift_band<C, PACK>(bd, delta, top_h, top_e);
         }
         prev_c = c;
         prev_empty = false;
@@ -257,9 +270,9 @@ __host__ __device__ int2 sw_job(const SwLaunch &a, int32_t job, int lane, int32_
         const uint8_t qb = read[i - 1];
         Row r;
         if (js == base && je == base + C - 1)
-            r = band_row<C, PACK, false>(bd, ref, base, js, je, qb, a, trow, lane);
+            r = band_row<C, PACK, false>(bd, ref, base, js, je, qb, a, trow, lane, top_h, top_e);
         else
-            r = band_row<C, PACK, true>(bd, ref, base, js, je, qb, a, trow, lane);
+            r = band_row<C, PACK, true>(bd, ref, base, js, je, qb, a, trow, lane, top_h, top_e);
         meta[(int64_t)(i - 1) * 64 + lane] = make_int2(base, (js - base) | ((je - base) << 16));
         const int32_t rbj = r.rbq >= 0 ? base + r.rbq : -1;
         if (r.rb > best) {

@@ -278,8 +278,26 @@ kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_s
                 best_ms = base.ms;
                 continue;
             }
-            if (r.res != base.res || r.cig != base.cig)
-                return fail(ctx, KIRA_ERR_RUNTIME, "banded SW routes disagree on a tuning batch");
+            if (r.res != base.res || r.cig != base.cig) {
+                // name the first differing job and field
+                size_t at = 0;
+                while (at < r.res.size() && r.res[at] == base.res[at]) at++;
+                char buf[256];
+                if (at < r.res.size()) {
+                    const size_t job = at / kira::kSwResWords, f = at % kira::kSwResWords;
+                    snprintf(buf, sizeof buf,
+                             "banded SW routes disagree on a tuning batch: band %d, read length "
+                             "%d, route %d vs built-in, job %zu (read %d bases), result word "
+                             "%zu: %d vs %d",
+                             w, L, cand[c], job, S.read_len[job], f, r.res[at], base.res[at]);
+                } else {
+                    snprintf(buf, sizeof buf,
+                             "banded SW routes disagree on a tuning batch: band %d, read length "
+                             "%d, route %d vs built-in, CIGAR words",
+                             w, L, cand[c]);
+                }
+                return fail(ctx, KIRA_ERR_RUNTIME, buf);
+            }
             if (r.ms < best_ms) {
                 best_ms = r.ms;
                 best_route = cand[c];
