@@ -1162,7 +1162,7 @@ SW_BAND_EDGES = [4, 8, 16, 32, 39, 64, 71, 128, 135, 207, 271]
 def sw_tune_shapes(*workloads):
     """One (band, read_len, n_jobs) shape per (band bucket, length bucket) group
     of each workload's jobs (each workload is one batch): its largest band, its
-    median read length and its job count."""
+    median read length, and the number of the batch's jobs in its band bucket."""
     shapes = []
     for sw in workloads:
         bb = np.searchsorted(np.array(SW_BAND_EDGES), sw["band"], side="left")
@@ -1171,7 +1171,7 @@ def sw_tune_shapes(*workloads):
         for k in np.unique(key):
             m = key == k
             shapes.append((int(sw["band"][m].max()), int(np.median(sw["read_len"][m])),
-                           int(m.sum())))
+                           int((bb == k // 64).sum())))
     return shapes
 
 

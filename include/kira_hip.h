@@ -368,8 +368,8 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
  * form, timed on the device).  A shape key is (band bucket, read-length
  * bucket, job-count bucket): band buckets end at bands 4, 8, 16, 32, 39, 64,
  * 71, 128, 135, 207, 271 (the bands of one bucket can take the same routes),
- * read lengths bucket by log2 and the jobs of a batch's (band bucket, length
- * bucket) group by log4.  Routes: 1 one lane per job (int16 or int32 band by
+ * read lengths bucket by log2, and the count is the batch's number of jobs in
+ * the job's band bucket (all lengths), bucketed by log4.  Routes: 1 one lane per job (int16 or int32 band by
  * the score bound), 2 + k the cooperative shape k of (P, G) = (5, 16),
  * (9, 16), (17, 16), (13, 32), (17, 32).  Routing never changes results.
  *
@@ -386,6 +386,9 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
  * banded SW entry point, servers created on it) route by the table.  Not to
  * be called while a server created on the context runs other work.
  */
+/* band, read length, and the number of jobs of the band's bucket in the
+ * batches to be tuned for (the synthetic batch has that many jobs, all of
+ * this read length, at most 262144) */
 typedef struct {
     int32_t band, read_len, n_jobs;
 } kira_sw_shape;

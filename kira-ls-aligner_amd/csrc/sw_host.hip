@@ -216,20 +216,22 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
         }
     }
     // tuned (or forced) routes override the built-in choice for the jobs that
-    // may take them; the shape key counts the jobs of each (band bucket,
-    // length bucket) group of the batch
+    // may take them; the shape key's count is the batch's number of jobs in
+    // the job's band bucket (the jobs a kernel class shares the chip with)
     if (force_route >= 0 || (tune && !tune->route.empty())) {
-        std::unordered_map<uint32_t, int64_t> group;
+        int64_t per_band[kNumBandBuckets + 1] = {};
         std::vector<uint32_t> gk(n);
         for (int32_t k = 0; k < n; k++) {
             const int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
-            gk[k] = sw_tune_key(sw_band_bucket(w), sw_len_bucket(read_len[k]), 0);
-            group[gk[k]]++;
+            const int bb = sw_band_bucket(w);
+            gk[k] = sw_tune_key(bb, sw_len_bucket(read_len[k]), 0);
+            per_band[bb]++;
         }
         for (int32_t k = 0; k < n; k++) {
             int route = force_route;
             if (route < 0) {
-                auto it = tune->route.find(gk[k] | (uint32_t)sw_count_bucket(group[gk[k]]));
+                const int bb = (int)(gk[k] >> 16);
+                auto it = tune->route.find(gk[k] | (uint32_t)sw_count_bucket(per_band[bb]));
                 if (it == tune->route.end()) continue;
                 route = it->second;
             }
