@@ -1035,7 +1035,9 @@ def bench_serving(pkg, ctx, genome, work, lw, args):
 # storage), then the cooperative (P, G) shapes
 SW_CLASS_NAMES = [f"lane C={c}{' i16' if p else ''}" for c in (9, 17, 33, 65, 129, 257)
                   for p in (0, 1)] + ["coop (5,16)", "coop (9,16)", "coop (17,16)",
-                                      "coop (13,32)", "coop (17,32)"]
+                                      "coop (13,32)", "coop (17,32)", "coop (6,16) i16",
+                                      "coop (10,16) i16", "coop (18,16) i16",
+                                      "coop (14,32) i16", "coop (18,32) i16"]
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
 
 
@@ -1156,7 +1158,7 @@ def concat_sw(a, b):
 
 # Shape buckets of the banded SW autotuner (include/kira_hip.h): band buckets
 # end at these bands, read lengths bucket by log2, group sizes by log4.
-SW_BAND_EDGES = [4, 8, 16, 32, 39, 64, 71, 128, 135, 207, 271]
+SW_BAND_EDGES = [4, 8, 16, 32, 39, 47, 64, 71, 79, 128, 135, 143, 207, 223, 271]
 
 
 def sw_tune_shapes(*workloads):
@@ -1195,8 +1197,15 @@ def bench_sw(pkg, ctx, genome, args, stream, lw):
     shapes = sw_tune_shapes(*[x for x in (sw, lw, mixed) if x is not None])
     t0 = time.time()
     cache = args.sw_tune_cache or None
-    tuned = pkg.sw_autotune(ctx, shapes, pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5), pkg.sw_knobs(),
-                            cache_path=cache)
+    # one shape a call, so that each finished shape is reported as it ends
+    tuned = []
+    for k, shape in enumerate(shapes):
+        t, = pkg.sw_autotune(ctx, [shape], pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5), pkg.sw_knobs(),
+                             cache_path=cache)
+        tuned.append(t)
+        print(f"[bench] SW autotune {k + 1}/{len(shapes)}: band {shape[0]}, read {shape[1]}, "
+              f"{shape[2]} jobs -> {t['route']} ({t['ms_best']:.2f} ms, built-in "
+              f"{t['ms_builtin']:.2f} ms, {t['candidates']} timed)", file=sys.stderr, flush=True)
     t_tune = time.time() - t0
     chosen = [dict(band=b, read_len=l, jobs=n, route=t["route"], candidates=t["candidates"],
                    ms_best=t["ms_best"], ms_builtin=t["ms_builtin"])

@@ -366,12 +366,16 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
  * lane count per host, src/pipeline/stage4_alignment.rs:1064-1069, 1153-1160,
  * src/simd/mod.rs:30; here the choice is a routing of each job to a kernel
  * form, timed on the device).  A shape key is (band bucket, read-length
- * bucket, job-count bucket): band buckets end at bands 4, 8, 16, 32, 39, 64,
- * 71, 128, 135, 207, 271 (the bands of one bucket can take the same routes),
+ * bucket, job-count bucket): band buckets end at bands 4, 8, 16, 32, 39, 47,
+ * 64, 71, 79, 128, 135, 143, 207, 223, 271 (the bands of one bucket can take
+ * the same routes),
  * read lengths bucket by log2, and the count is the batch's number of jobs in
  * the job's band bucket (all lengths), bucketed by log4.  Routes: 1 one lane per job (int16 or int32 band by
  * the score bound), 2 + k the cooperative shape k of (P, G) = (5, 16),
- * (9, 16), (17, 16), (13, 32), (17, 32).  Routing never changes results.
+ * (9, 16), (17, 16), (13, 32), (17, 32) in int32 and (6, 16), (10, 16),
+ * (18, 16), (14, 32), (18, 32) in packed int16 pairs (k = 5..9: gap open > 0,
+ * read_len x match <= 8191, mismatch <= 8000, open + extend <= 4000, extend
+ * <= 500).  Routing never changes results.
  *
  * kira_sw_autotune times, for each shape, every eligible route on a seeded
  * synthetic batch of that shape (read_len bases, band, n_jobs jobs up to
@@ -405,11 +409,12 @@ kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_s
 kira_status kira_sw_tune_load(kira_ctx *ctx, const char *cache_path, int32_t *n_loaded);
 void kira_sw_tune_clear(kira_ctx *ctx);
 /* Jobs of a plan per route: counts[1] one lane per job, counts[2 + k]
- * cooperative shape k (counts has 7 entries; counts[0] is 0). */
+ * cooperative shape k (counts has 12 entries; counts[0] is 0). */
 void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts);
 /* Kernel class of each job of a plan (n entries): 2 b + packed for one lane
  * per job with a band of 9, 17, 33, 65, 129, 257 positions (b = 0..5), 12 + k
- * for cooperative shape k.  The plan launches its non-empty classes in this
+ * for cooperative shape k (k = 0..9, the route table's order).  The plan
+ * launches its non-empty classes in this
  * order, one kernel each. */
 void kira_banded_sw_plan_job_class(const kira_sw_plan *plan, int32_t *cls);
 

@@ -759,9 +759,9 @@ class BandedSwPlan:
 
     def route_counts(self) -> dict:
         """Jobs per route (kira_banded_sw_plan_route_counts), by SW_ROUTES name."""
-        c = np.zeros(7, dtype=np.int32)
+        c = np.zeros(len(SW_ROUTES), dtype=np.int32)
         lib().kira_banded_sw_plan_route_counts(self.h, _np_ptr(c))
-        return {SW_ROUTES[r]: int(c[r]) for r in range(1, 7) if c[r]}
+        return {SW_ROUTES[r]: int(c[r]) for r in range(1, len(SW_ROUTES)) if c[r]}
 
     def job_class(self) -> np.ndarray:
         """Kernel class per job (kira_banded_sw_plan_job_class)."""
@@ -801,8 +801,10 @@ def banded_sw_run_dev(ctx: Context, plan: BandedSwPlan, *, reads, read_off, read
                                            C.byref(knobs)), "banded_sw_run_dev")
 
 
+# (P, G): P band positions per lane, G lanes per job; "i16" = packed int16 pairs
 SW_ROUTES = {0: "built-in", 1: "lane per job", 2: "(5, 16)", 3: "(9, 16)", 4: "(17, 16)",
-             5: "(13, 32)", 6: "(17, 32)"}
+             5: "(13, 32)", 6: "(17, 32)", 7: "(6, 16) i16", 8: "(10, 16) i16",
+             9: "(18, 16) i16", 10: "(14, 32) i16", 11: "(18, 32) i16"}
 
 
 def sw_autotune(ctx: Context, shapes, cfg: SwCfg, knobs: Optional[SwKnobs] = None,

@@ -132,12 +132,15 @@ __host__ __device__ inline void sw_write_skipped(const SwLaunch &a, int32_t job)
 
 hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hipStream_t s);
 // Wide bands: G lanes per job (16 or 32), P band positions per lane
-// ((P, G) in (9, 16), (17, 16), (13, 32), (17, 32)).
+// ((P, G) in (5, 16), (9, 16), (17, 16), (13, 32), (17, 32)).
 hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, hipStream_t s);
+// The packed int16 form: 2 P2 positions a lane as P2 registers of int16 pairs
+// ((P2, G) in (3, 16), (5, 16), (9, 16), (7, 32), (9, 32)).
+hipError_t launch_banded_sw_coop16(const SwLaunch &a, int P2, int G, int n_waves, hipStream_t s);
 
 // Host-side packetisation of a banded SW batch (jobs sorted by band class
 // and read length into packets, trace / meta slab offsets, CIGAR arena).
-constexpr int kSwClasses = 17;
+constexpr int kSwClasses = 22;
 struct SwPlanLayout {
     int32_t n = 0, n_packets = 0;
     int32_t class_first[kSwClasses] = {}, class_count[kSwClasses] = {};
@@ -148,9 +151,10 @@ struct SwPlanLayout {
 // Routing of banded SW jobs chosen by the autotuner per shape key (band
 // bucket, read-length bucket, job-count bucket; sw_tune_key).  A route is
 // kSwRouteLane (one lane per job, int16 or int32 band by the score bound) or
-// kSwRouteCoop + k (cooperative shape k of the (P, G) table); shapes without
-// an entry keep the built-in routing.
-constexpr int kSwRouteDefault = 0, kSwRouteLane = 1, kSwRouteCoop = 2;
+// kSwRouteCoop + k (cooperative shape k of the (P, G) table: k < kSwCoop32
+// the int32 form, the rest the packed int16 form); shapes without an entry
+// keep the built-in routing.
+constexpr int kSwRouteDefault = 0, kSwRouteLane = 1, kSwRouteCoop = 2, kSwCoop32 = 5;
 struct SwTuneTable {
     std::unordered_map<uint32_t, uint8_t> route;
 };

@@ -27,8 +27,11 @@ constexpr int kNumLane = 6;
 struct CoopShape {
     int p, g;
 };
-constexpr CoopShape kCoop[] = {{5, 16}, {9, 16}, {17, 16}, {13, 32}, {17, 32}};
-constexpr int kNumCoop = 5;
+constexpr CoopShape kCoop[] = {{5, 16}, {9, 16}, {17, 16}, {13, 32}, {17, 32},
+                               // packed int16 forms: p positions = 2 x the registers
+                               {6, 16}, {10, 16}, {18, 16}, {14, 32}, {18, 32}};
+constexpr int kNumCoop = 10;
+constexpr int kNumCoop32 = kira::kSwCoop32;  // kCoop[0..4] int32, kCoop[5..9] packed int16
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
@@ -36,6 +39,7 @@ constexpr int kMinLaneJobs = 65536;  // one 64-job wave on each of the 1024 SIMD
 constexpr int64_t kLongJobCells = 1 << 16;
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
+bool class_is_coop16(int c) { return c >= 2 * kNumLane + kNumCoop32; }
 CoopShape class_coop(int c) { return kCoop[c - 2 * kNumLane]; }
 int class_jobs_per_packet(int c) { return class_is_coop(c) ? 64 / class_coop(c).g : 64; }
 int class_rowdw(int c) {
@@ -49,8 +53,10 @@ int lane_class_of(int32_t w) {
     return -1;
 }
 
-int coop_class_of(int32_t w) {
-    for (int k = 0; k < kNumCoop; k++)
+// the smallest cooperative shape of the int32 or the packed int16 form
+// that holds the band
+int coop_class_of(int32_t w, bool packed) {
+    for (int k = packed ? kNumCoop32 : 0; k < (packed ? kNumCoop : kNumCoop32); k++)
         if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) return 2 * kNumLane + k;
     return -1;
 }
@@ -101,9 +107,11 @@ namespace kira {
 
 // Band bucket edges: the largest band of each bucket.  Each edge is where the
 // smallest lane class (2w + 1 <= 9, 17, 33, 65, 129, 257) or the set of
-// cooperative shapes that hold 2w + 1 (80, 144, 272, 416, 544 positions)
-// changes, so every band of a bucket has the same eligible routes.
-constexpr int32_t kBandEdges[] = {4, 8, 16, 32, 39, 64, 71, 128, 135, 207, 271};
+// cooperative shapes that hold 2w + 1 (80, 144, 272, 416, 544 positions in
+// int32, 96, 160, 288, 448, 576 packed) changes, so every band of a bucket
+// has the same eligible routes.
+constexpr int32_t kBandEdges[] = {4,  8,   16,  32,  39,  47,  64, 71,
+                                  79, 128, 135, 143, 207, 223, 271};
 constexpr int kNumBandBuckets = sizeof(kBandEdges) / sizeof(kBandEdges[0]);
 
 int sw_band_bucket(int32_t w) {
@@ -147,12 +155,26 @@ bool coop_costs_ok(const kira_sw_cfg &cfg) {
 }
 }  // namespace
 
+// The packed int16 cooperative form: every key value << 2 | priority and
+// the doubled E / F chains stay inside int16 (scores <= 8191, mismatch <=
+// 8000, open + extend <= 4000, extend <= 500), and gap open > 0 (its F chain
+// tag is the reference's F-extend bit only then).
+bool coop16_ok(int32_t read_len, const kira_sw_cfg &cfg) {
+    const int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
+    return cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open > 0 &&
+           cfg.gap_extend >= 0 && cfg.mismatch <= 8000 && go <= 4000 &&
+           cfg.gap_extend <= 500 && (int64_t)read_len * cfg.match_score <= 8191;
+}
+
 uint32_t sw_routes_for(int32_t w, int32_t read_len, const kira_sw_cfg &cfg) {
     w = std::max(w, 1);
     uint32_t r = 1u << kSwRouteDefault;
     if (lane_class_of(w) >= 0) r |= 1u << kSwRouteLane;
     if (coop_costs_ok(cfg) && (int64_t)read_len * cfg.match_score < (1 << 21))
-        for (int k = 0; k < kNumCoop; k++)
+        for (int k = 0; k < kNumCoop32; k++)
+            if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) r |= 1u << (kSwRouteCoop + k);
+    if (coop16_ok(read_len, cfg))
+        for (int k = kNumCoop32; k < kNumCoop; k++)
             if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) r |= 1u << (kSwRouteCoop + k);
     return r;
 }
@@ -171,6 +193,7 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
     // 16-lane jobs: the F scan stays inside int32 and the arg-max key
     // (score << 10 | position) needs score < 2^21
     const bool coop_costs = coop_costs_ok(cfg);
+    // 0: lane per job only, 1: int32 cooperative shapes, 2: packed ones too
     std::vector<uint8_t> coop_ok(n);
     for (int32_t k = 0; k < n; k++) {
         int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
@@ -178,18 +201,30 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
         // every H is at most read_len * match: int16 storage cannot clamp
         bool pack = costs_small && (int64_t)read_len[k] * cfg.match_score < 32767;
         bool coop = coop_costs && (int64_t)read_len[k] * cfg.match_score < (1 << 21);
-        coop_ok[k] = coop;
+        const bool c16 = coop && coop16_ok(read_len[k], cfg);
+        coop_ok[k] = coop ? (c16 ? 2 : 1) : 0;
+        // the widest band every form holds (the int32 (17, 32) shape); the
+        // packed (18, 32) shape would hold 287 but the limit does not depend
+        // on the costs
+        if (w > kMaxBand) return "band wider than 271";
         if (w > 64 && coop) {
-            cls[k] = coop_class_of(w);
-            if (cls[k] < 0) return "band wider than 271";
+            cls[k] = coop_class_of(w, c16);
+            continue;
+        }
+        // bands above 32 run as packed 16-lane jobs where the costs allow:
+        // at band 64, 100k pairs of 100-300 bp took 3.3 ms as (10, 16) int16
+        // jobs and 3.7 ms lane per job (C = 129); at band 32 lane per job
+        // (C = 65) stays ahead (2.1 ms against 2.6 ms as (6, 16) jobs)
+        if (c16 && w > 32) {
+            cls[k] = coop_class_of(w, true);
             continue;
         }
         // one long lane-per-job job serialises read_len x (2w + 1) cells on
         // one lane and becomes the straggler of its class: run it on 16
         // lanes instead
         if (coop && w > 16 && (int64_t)read_len[k] * (2 * w + 1) > kLongJobCells &&
-            coop_class_of(w) >= 0) {
-            cls[k] = coop_class_of(w);
+            coop_class_of(w, c16) >= 0) {
+            cls[k] = coop_class_of(w, c16);
             continue;
         }
         int b = lane_class_of(w);
@@ -203,7 +238,7 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
     // on each dependent instruction and LDS read of its row loop (37k jobs
     // of band 32 took 4.8 ms lane per job, 2.3 ms as (9, 16) jobs).  Such
     // jobs run as 16-lane jobs (16x the waves) in the smallest 16-lane
-    // class that holds the band: (5, 16) up to band 39, else (9, 16).
+    // class that holds the band, packed int16 where the costs allow.
     {
         int count[kNumClasses] = {};
         for (int32_t k = 0; k < n; k++) count[cls[k]]++;
@@ -212,7 +247,8 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
             const int c = cls[k];
             if ((c == c65 || c == c65p || c == c129 || c == c129p) && count[c] < kMinLaneJobs &&
                 coop_ok[k])
-                cls[k] = coop_class_of(std::max(band ? band[k] : cfg.bandwidth, 1));
+                cls[k] = coop_class_of(std::max(band ? band[k] : cfg.bandwidth, 1),
+                                       coop_ok[k] == 2);
         }
     }
     // tuned (or forced) routes override the built-in choice for the jobs that
@@ -401,7 +437,10 @@ const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira
         ac.n_packets = L.class_count[c];
         ac.counter = d.counter + c;
         hipError_t e;
-        if (class_is_coop(c)) {
+        if (class_is_coop16(c)) {
+            int waves = std::min(L.class_count[c], 256 * 20);
+            e = launch_banded_sw_coop16(ac, class_coop(c).p / 2, class_coop(c).g, waves, s);
+        } else if (class_is_coop(c)) {
             int waves = std::min(L.class_count[c], 256 * 20);
             e = launch_banded_sw_coop(ac, class_coop(c).p, class_coop(c).g, waves, s);
         } else {

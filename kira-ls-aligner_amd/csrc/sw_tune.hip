@@ -104,7 +104,7 @@ struct Run {
 };
 
 // One routing of the synthetic batch: plan with the route forced, one
-// warm-up launch and three timed ones (the fastest counts), results kept.
+// warm-up launch and two timed ones (the faster counts), results kept.
 kira_status run_route(kira_ctx *ctx, const Synth &S, const kira_sw_cfg &cfg,
                       const kira_sw_knobs &kn, int route, const kira_sw_jobs_dev &jd_in,
                       Run &out) {
@@ -141,7 +141,7 @@ kira_status run_route(kira_ctx *ctx, const Synth &S, const kira_sw_cfg &cfg,
         }
     } eg{ev};
     float best = 1e30f;
-    for (int rep = 0; rep < 4; rep++) {
+    for (int rep = 0; rep < 3; rep++) {
         if ((e = hipEventRecord(ev[0], s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "tuning event", e);
         const char *why = kira::sw_plan_launch(L, d, jd, cfg, kn, nullptr, nullptr, s, &e);
@@ -233,13 +233,17 @@ kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_s
         // candidates: the built-in routing and every route the shape may take
         // (one lane per job is not a candidate where a 257-position band would
         // spill the register file and a cooperative shape holds the band)
+        // and, of each cooperative form, the two smallest shapes that hold
+        // the band (a larger one only adds idle positions to every row)
         std::vector<int> cand{-1};
         const uint32_t ok = kira::sw_routes_for(w, L, *cfg);
         const bool coop_any = (ok >> kira::kSwRouteCoop) != 0;
         if (((ok >> kira::kSwRouteLane) & 1u) && !(w > 64 && coop_any))
             cand.push_back(kira::kSwRouteLane);
+        int taken[2] = {0, 0};
         for (int k = 0; k < kira::sw_num_coop_shapes(); k++)
-            if ((ok >> (kira::kSwRouteCoop + k)) & 1u) cand.push_back(kira::kSwRouteCoop + k);
+            if (((ok >> (kira::kSwRouteCoop + k)) & 1u) && taken[k >= kira::kSwCoop32]++ < 2)
+                cand.push_back(kira::kSwRouteCoop + k);
         kira::DevMem dm(ctx);
         uint8_t *d_reads = nullptr, *d_text = nullptr;
         int64_t *d_ro = nullptr, *d_fo = nullptr;

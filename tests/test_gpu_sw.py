@@ -154,8 +154,9 @@ def test_wide_band_limits(pkg, ctx):
 
 @pytest.mark.parametrize("widths", [(33, 48, 64), (17, 24, 32)])
 def test_large_batch_keeps_lane_per_job_classes(pkg, ctx, widths):
-    """A class of >= 65536 jobs of band 17-32 or 33-64 runs lane per job
-    (smaller ones move to 16-lane jobs); both must agree with the oracle."""
+    """A class of >= 65536 jobs of band 17-32 runs lane per job (smaller ones
+    move to 16-lane jobs), bands 33-64 run as packed 16-lane jobs; both must
+    agree with the oracle."""
     import numpy as np
     rng = random.Random(23 + widths[0])
     jobs = []
