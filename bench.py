@@ -1037,7 +1037,8 @@ SW_CLASS_NAMES = [f"lane C={c}{' i16' if p else ''}" for c in (9, 17, 33, 65, 12
                   for p in (0, 1)] + ["coop (5,16)", "coop (9,16)", "coop (17,16)",
                                       "coop (13,32)", "coop (17,32)", "coop (6,16) i16",
                                       "coop (10,16) i16", "coop (18,16) i16",
-                                      "coop (14,32) i16", "coop (18,32) i16"]
+                                      "coop (14,32) i16", "coop (18,32) i16",
+                                      "coop (10,8) i16", "coop (18,8) i16"]
 SW_OPS_PER_CELL = 16  # E, F, H max chains + substitution select, without trace bookkeeping
 
 

@@ -373,7 +373,8 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
  * the job's band bucket (all lengths), bucketed by log4.  Routes: 1 one lane per job (int16 or int32 band by
  * the score bound), 2 + k the cooperative shape k of (P, G) = (5, 16),
  * (9, 16), (17, 16), (13, 32), (17, 32) in int32 and (6, 16), (10, 16),
- * (18, 16), (14, 32), (18, 32) in packed int16 pairs (k = 5..9: gap open > 0,
+ * (18, 16), (14, 32), (18, 32), (10, 8), (18, 8) in packed int16 pairs
+ * (k = 5..11: gap open > 0,
  * read_len x match <= 8191, mismatch <= 8000, open + extend <= 4000, extend
  * <= 500).  Routing never changes results.
  *
@@ -409,11 +410,11 @@ kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_s
 kira_status kira_sw_tune_load(kira_ctx *ctx, const char *cache_path, int32_t *n_loaded);
 void kira_sw_tune_clear(kira_ctx *ctx);
 /* Jobs of a plan per route: counts[1] one lane per job, counts[2 + k]
- * cooperative shape k (counts has 12 entries; counts[0] is 0). */
+ * cooperative shape k (counts has 14 entries; counts[0] is 0). */
 void kira_banded_sw_plan_route_counts(const kira_sw_plan *plan, int32_t *counts);
 /* Kernel class of each job of a plan (n entries): 2 b + packed for one lane
  * per job with a band of 9, 17, 33, 65, 129, 257 positions (b = 0..5), 12 + k
- * for cooperative shape k (k = 0..9, the route table's order).  The plan
+ * for cooperative shape k (k = 0..11, the route table's order).  The plan
  * launches its non-empty classes in this
  * order, one kernel each. */
 void kira_banded_sw_plan_job_class(const kira_sw_plan *plan, int32_t *cls);

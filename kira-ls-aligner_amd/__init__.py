@@ -804,7 +804,8 @@ def banded_sw_run_dev(ctx: Context, plan: BandedSwPlan, *, reads, read_off, read
 # (P, G): P band positions per lane, G lanes per job; "i16" = packed int16 pairs
 SW_ROUTES = {0: "built-in", 1: "lane per job", 2: "(5, 16)", 3: "(9, 16)", 4: "(17, 16)",
              5: "(13, 32)", 6: "(17, 32)", 7: "(6, 16) i16", 8: "(10, 16) i16",
-             9: "(18, 16) i16", 10: "(14, 32) i16", 11: "(18, 32) i16"}
+             9: "(18, 16) i16", 10: "(14, 32) i16", 11: "(18, 32) i16", 12: "(10, 8) i16",
+             13: "(18, 8) i16"}
 
 
 def sw_autotune(ctx: Context, shapes, cfg: SwCfg, knobs: Optional[SwKnobs] = None,

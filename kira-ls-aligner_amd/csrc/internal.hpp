@@ -135,12 +135,12 @@ hipError_t launch_banded_sw(const SwLaunch &a, int C, bool pack, int n_waves, hi
 // ((P, G) in (5, 16), (9, 16), (17, 16), (13, 32), (17, 32)).
 hipError_t launch_banded_sw_coop(const SwLaunch &a, int P, int G, int n_waves, hipStream_t s);
 // The packed int16 form: 2 P2 positions a lane as P2 registers of int16 pairs
-// ((P2, G) in (3, 16), (5, 16), (9, 16), (7, 32), (9, 32)).
+// ((P2, G) in (3, 16), (5, 16), (9, 16), (7, 32), (9, 32), (5, 8), (9, 8)).
 hipError_t launch_banded_sw_coop16(const SwLaunch &a, int P2, int G, int n_waves, hipStream_t s);
 
 // Host-side packetisation of a banded SW batch (jobs sorted by band class
 // and read length into packets, trace / meta slab offsets, CIGAR arena).
-constexpr int kSwClasses = 22;
+constexpr int kSwClasses = 24;
 struct SwPlanLayout {
     int32_t n = 0, n_packets = 0;
     int32_t class_first[kSwClasses] = {}, class_count[kSwClasses] = {};
@@ -171,6 +171,19 @@ uint32_t sw_routes_for(int32_t w, int32_t read_len, const kira_sw_cfg &cfg);
 int sw_num_coop_shapes();
 void sw_coop_shape(int k, int *P, int *G);
 
+// Auxiliary streams a plan launch fans its kernel classes out to, so that
+// the tail of one class (its last, longest jobs) overlaps the next class
+// instead of idling the chip; joined back into the caller's stream with
+// events.  Owned by a context and used by one call at a time.
+constexpr int kSwFan = 3;
+struct SwFanout {
+    hipStream_t aux[kSwFan] = {};
+    hipEvent_t fork = nullptr, join[kSwFan] = {};
+    bool ok = false;
+};
+hipError_t sw_fanout_create(SwFanout &f);
+void sw_fanout_destroy(SwFanout &f);
+
 // nullptr on success, else the reason the batch cannot run.  With a tuning
 // table, jobs whose shape key has an entry take the tuned route (if eligible);
 // force_route >= 0 sends every job that is eligible for it there (the tuner's
@@ -192,12 +205,13 @@ struct SwPlanDev {
     int32_t *counter;
 };
 SwPlanDev sw_plan_bind(const SwPlanLayout &L, const uint8_t *dev_tables, uint8_t *dev_slabs);
-// All launches of one batch on stream s (nullptr on success, else the error
-// and *e the HIP status).
+// All launches of one batch, ordered on stream s (nullptr on success, else
+// the error and *e the HIP status).  With a fan-out, the kernel classes are
+// spread over s and the fan-out's streams, which join s again.
 const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira_sw_jobs_dev &j,
                            const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
                            const uint8_t *skip_flags, const int32_t *skip_src, hipStream_t s,
-                           hipError_t *e);
+                           hipError_t *e, SwFanout *fan = nullptr);
 
 // Full-window lane-group SW (align_batch_simd's SIMD kernels).
 struct GroupLaunch {
@@ -481,6 +495,8 @@ struct kira_ctx {
     size_t pin_cap[4] = {};
     // banded SW routes chosen by the autotuner (kira_sw_autotune / kira_sw_tune_load)
     kira::SwTuneTable tune;
+    // streams the banded SW kernel classes of one plan launch fan out to
+    kira::SwFanout fan;
 };
 
 struct kira_genome {

@@ -101,6 +101,8 @@ kira_status kira_ctx_create(int device, kira_ctx **out) {
         return KIRA_ERR_DRIVER;
     }
     ctx->stream = ctx->own_stream;
+    // without its streams a plan launch keeps every class on one stream
+    (void)kira::sw_fanout_create(ctx->fan);
     // the no-allocation entry point's staging array: its device address is
     // looked up here, once per device (a failure is reported by that entry
     // point, which needs the address)
@@ -131,6 +133,7 @@ void kira_release_buffers(kira_ctx *ctx) {
 void kira_ctx_destroy(kira_ctx *ctx) {
     if (!ctx) return;
     kira_release_buffers(ctx);
+    kira::sw_fanout_destroy(ctx->fan);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

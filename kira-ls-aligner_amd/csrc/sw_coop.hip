@@ -1,5 +1,6 @@
 // Banded adaptive Smith-Waterman (banded_sw_with, src/alignment/mod.rs:1370-1650)
-// for wide bands: a group of G = 16 (one DPP row) or 32 lanes runs one job.
+// for wide bands: a group of G = 16 (one DPP row) or 32 lanes runs one job
+// (8, half a row, for the narrower shapes of the packed form).
 //
 // A lane-per-job band of 2w + 1 > 129 cells no longer fits the register
 // file, so here the band of row i is spread over the G lanes of a group:
@@ -46,12 +47,15 @@ __device__ __forceinline__ int32_t from_prev(int32_t v, int32_t fill) {
     return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
 }
 // Neighbour lanes inside a group of G lanes (g: lane within the group):
-// G = 16 is one DPP row; G = 32 uses the wave shifts and patches the group
-// edge.  Lanes without a neighbour read `fill`.
+// G = 16 is one DPP row; G = 8 is half a row and G = 32 uses the wave
+// shifts, both patching the group edge.  Lanes without a neighbour read `fill`.
 template <int G>
 __device__ __forceinline__ int32_t nb_next(int32_t v, int32_t fill, int g) {
     if constexpr (G == 16) {
         return __builtin_amdgcn_update_dpp(fill, v, 0x101, 0xF, 0xF, false);  // row_shl:1
+    } else if constexpr (G == 8) {  // two groups a row
+        const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x101, 0xF, 0xF, false);
+        return g == G - 1 ? fill : t;
     } else {
         const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
         return g == G - 1 ? fill : t;
@@ -61,6 +65,9 @@ template <int G>
 __device__ __forceinline__ int32_t nb_prev(int32_t v, int32_t fill, int g) {
     if constexpr (G == 16) {
         return __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    } else if constexpr (G == 8) {
+        const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xF, 0xF, false);
+        return g == 0 ? fill : t;
     } else {
         const int32_t t = __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
         return g == 0 ? fill : t;
@@ -70,15 +77,39 @@ __device__ __forceinline__ int32_t nb_prev(int32_t v, int32_t fill, int g) {
 // overwrites the group-edge lane, so no copy of the source array is needed.
 template <int G>
 __device__ __forceinline__ int32_t next_lane0(int32_t v) {
-    return __builtin_amdgcn_mov_dpp(v, G == 16 ? 0x101 : 0x130, 0xF, 0xF, true);
+    return __builtin_amdgcn_mov_dpp(v, G <= 16 ? 0x101 : 0x130, 0xF, 0xF, true);
 }
 template <int G>
 __device__ __forceinline__ int32_t prev_lane0(int32_t v) {
-    return __builtin_amdgcn_mov_dpp(v, G == 16 ? 0x111 : 0x138, 0xF, 0xF, true);
+    return __builtin_amdgcn_mov_dpp(v, G <= 16 ? 0x111 : 0x138, 0xF, 0xF, true);
 }
 template <int N>
 __device__ __forceinline__ int32_t rotate(int32_t v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xF, 0xF, false);
+}
+// from_prev<N> inside groups of G lanes: lanes g < N of a group read fill
+// (rows of 16 lanes hold two groups of 8)
+template <int N, int G>
+__device__ __forceinline__ int32_t grp_prev(int32_t v, int32_t fill, int g) {
+    const int32_t t = from_prev<N>(v, fill);
+    if constexpr (G == 8) return g >= N ? t : fill;
+    else return t;
+}
+// maximum over the G lanes of a group, in every lane of it
+template <int G>
+__device__ __forceinline__ int32_t group_max(int32_t key) {
+    if constexpr (G == 8) {
+        key = imax(key, __builtin_amdgcn_update_dpp(0, key, 0xB1, 0xF, 0xF, false));  // quad [1,0,3,2]
+        key = imax(key, __builtin_amdgcn_update_dpp(0, key, 0x4E, 0xF, 0xF, false));  // quad [2,3,0,1]
+        key = imax(key, __builtin_amdgcn_update_dpp(0, key, 0x141, 0xF, 0xF, false)); // row_half_mirror
+    } else {
+        key = imax(key, rotate<8>(key));
+        key = imax(key, rotate<4>(key));
+        key = imax(key, rotate<2>(key));
+        key = imax(key, rotate<1>(key));
+        if constexpr (G == 32) key = imax(key, __shfl_xor(key, 16));
+    }
+    return key;
 }
 
 __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s) {
@@ -733,11 +764,11 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
         const int32_t f = imax((int32_t)fend.y, (int32_t)fend.x - P2 * 4 * ge);
         // F4 at this lane's position 0: max-plus scan over the lanes
         const int32_t dP = P * 4 * ge;
-        int32_t F = from_prev<1>(f, kPadF4);
-        F = imax(F, from_prev<1>(F, kPadF4) - dP);
-        F = imax(F, from_prev<2>(F, kPadF4) - 2 * dP);
-        F = imax(F, from_prev<4>(F, kPadF4) - 4 * dP);
-        F = imax(F, from_prev<8>(F, kPadF4) - 8 * dP);
+        int32_t F = grp_prev<1, G>(f, kPadF4, g);
+        F = imax(F, grp_prev<1, G>(F, kPadF4, g) - dP);
+        F = imax(F, grp_prev<2, G>(F, kPadF4, g) - 2 * dP);
+        F = imax(F, grp_prev<4, G>(F, kPadF4, g) - 4 * dP);
+        if constexpr (G >= 16) F = imax(F, from_prev<8>(F, kPadF4) - 8 * dP);
         if constexpr (G == 32) {
             const int32_t t = imax(f, F - dP);
             const int32_t y = __builtin_amdgcn_update_dpp(kPadF4, t, 0x142, 0xA, 0xF, false);
@@ -778,12 +809,7 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
             key = imax(key, imax((int32_t)((h4 << 16) | (uint32_t)(0xFFFF - r)),
                                  (int32_t)((h4 & 0xFFFF0000u) | (uint32_t)(0xFFFF - P2 - r))));
         }
-        key -= q0;
-        key = imax(key, rotate<8>(key));
-        key = imax(key, rotate<4>(key));
-        key = imax(key, rotate<2>(key));
-        key = imax(key, rotate<1>(key));
-        if constexpr (G == 32) key = imax(key, __shfl_xor(key, 16));
+        key = group_max<G>(key - q0);
         const int32_t rb_all = key >> 18;
         const int32_t rbj = rb_all > 0 ? base + (0xFFFF - (key & 0xFFFF)) : -1;
         if (g == 0)
@@ -885,6 +911,8 @@ hipError_t launch_banded_sw_coop16(const SwLaunch &a, int P2, int G, int n_waves
         else if (G == 16 && P2 == 9) banded_sw_coop16_kernel<9, 16><<<n_waves, 64, 0, s>>>(a);
         else if (G == 32 && P2 == 7) banded_sw_coop16_kernel<7, 32><<<n_waves, 64, 0, s>>>(a);
         else if (G == 32 && P2 == 9) banded_sw_coop16_kernel<9, 32><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 8 && P2 == 5) banded_sw_coop16_kernel<5, 8><<<n_waves, 64, 0, s>>>(a);
+        else if (G == 8 && P2 == 9) banded_sw_coop16_kernel<9, 8><<<n_waves, 64, 0, s>>>(a);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();

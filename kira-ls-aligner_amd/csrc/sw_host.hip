@@ -18,10 +18,11 @@ namespace {
 // Job classes.  Lane-per-job: a job with band w runs in the smallest
 // C >= 2w + 1 of kLaneC, with the band stored as packed int16 when its
 // scores provably fit, else as int32 (class = band index * 2 + packed).
-// Bands above 64 (C = 257 would spill the register file) run kCoop[k].g
-// lanes per job with P = kCoop[k].p positions a lane (G P >= 2w + 1) when
-// the costs keep its int32 scan and arg-max key exact; otherwise C = 257
-// lane-per-job.
+// Cooperative: kCoop[k].g lanes per job with P = kCoop[k].p positions a lane
+// (G P >= 2w + 1), in int32 when the costs keep its scan and arg-max key
+// exact, or computed in packed int16 pairs when every value fits int16.
+// Bands above 64 (C = 257 would spill the register file) run cooperatively
+// where the costs allow, otherwise C = 257 lane per job.
 constexpr int kLaneC[] = {9, 17, 33, 65, 129, 257};
 constexpr int kNumLane = 6;
 struct CoopShape {
@@ -29,13 +30,20 @@ struct CoopShape {
 };
 constexpr CoopShape kCoop[] = {{5, 16}, {9, 16}, {17, 16}, {13, 32}, {17, 32},
                                // packed int16 forms: p positions = 2 x the registers
-                               {6, 16}, {10, 16}, {18, 16}, {14, 32}, {18, 32}};
-constexpr int kNumCoop = 10;
-constexpr int kNumCoop32 = kira::kSwCoop32;  // kCoop[0..4] int32, kCoop[5..9] packed int16
+                               {6, 16}, {10, 16}, {18, 16}, {14, 32}, {18, 32},
+                               {10, 8}, {18, 8}};
+constexpr int kNumCoop = 12;
+constexpr int kNumCoop32 = kira::kSwCoop32;  // kCoop[0..4] int32, kCoop[5..11] packed int16
 constexpr int kNumClasses = 2 * kNumLane + kNumCoop;
 static_assert(kNumClasses == kira::kSwClasses, "class table");
 constexpr int kMaxBand = (32 * 17 - 1) / 2;  // 271
 constexpr int kMinLaneJobs = 65536;  // one 64-job wave on each of the 1024 SIMDs
+// 8-lane packed jobs fill a wave with 8 jobs: 32768 of them are 4 waves a
+// SIMD.  Fewer leave SIMDs idle and the 16-lane shape (twice the waves) wins:
+// 5.8k long windows of bands 40-71 took 2.5 ms as (10, 16) jobs, 3.0 ms as
+// (18, 8); 100k pairs of 100-300 bp took 2.7 ms as (18, 8) jobs at band 64
+// (3.3 as (10, 16)) and 1.9 ms as (10, 8) at band 32 (2.1 lane per job).
+constexpr int kNarrowMinJobs = 32768;
 constexpr int64_t kLongJobCells = 1 << 16;
 
 bool class_is_coop(int c) { return c >= 2 * kNumLane; }
@@ -53,8 +61,16 @@ int lane_class_of(int32_t w) {
     return -1;
 }
 
-// the smallest cooperative shape of the int32 or the packed int16 form
-// that holds the band
+// the class of cooperative shape (p, g)
+int coop_class_by_shape(int p, int g) {
+    for (int k = 0; k < kNumCoop; k++)
+        if (kCoop[k].p == p && kCoop[k].g == g) return 2 * kNumLane + k;
+    return -1;
+}
+
+// the first cooperative shape of the int32 or the packed int16 form (table
+// order: 16-lane shapes by size, then 32-lane, then 8-lane) that holds the
+// band
 int coop_class_of(int32_t w, bool packed) {
     for (int k = packed ? kNumCoop32 : 0; k < (packed ? kNumCoop : kNumCoop32); k++)
         if (kCoop[k].g * kCoop[k].p >= 2 * w + 1) return 2 * kNumLane + k;
@@ -155,10 +171,10 @@ bool coop_costs_ok(const kira_sw_cfg &cfg) {
 }
 }  // namespace
 
-// The packed int16 cooperative form: every key value << 2 | priority and
-// the doubled E / F chains stay inside int16 (scores <= 8191, mismatch <=
-// 8000, open + extend <= 4000, extend <= 500), and gap open > 0 (its F chain
-// tag is the reference's F-extend bit only then).
+// The packed int16 cooperative form: its x4 keyed values (4 value + tag)
+// stay inside int16 (scores <= 8191, mismatch <= 8000, open + extend <=
+// 4000, extend <= 500), and gap open > 0 (its F-extend bit compares against
+// the pre-F H, which equals the reference's comparison only then).
 bool coop16_ok(int32_t read_len, const kira_sw_cfg &cfg) {
     const int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
     return cfg.match_score >= 0 && cfg.mismatch >= 0 && cfg.gap_open > 0 &&
@@ -207,16 +223,17 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
         // packed (18, 32) shape would hold 287 but the limit does not depend
         // on the costs
         if (w > kMaxBand) return "band wider than 271";
-        if (w > 64 && coop) {
-            cls[k] = coop_class_of(w, c16);
+        // bands of 17 .. 71 run as packed 8- or 16-lane jobs where the costs
+        // allow (decided below by how many there are), wider ones as packed
+        // 16- or 32-lane jobs: at band 64, 100k pairs of 100-300 bp took
+        // 3.7 ms lane per job (C = 129), 3.3 ms as (10, 16) and 2.7 ms as
+        // (18, 8) int16 jobs
+        if (c16 && w > 16 && w <= 71) {
+            cls[k] = -1;
             continue;
         }
-        // bands above 32 run as packed 16-lane jobs where the costs allow:
-        // at band 64, 100k pairs of 100-300 bp took 3.3 ms as (10, 16) int16
-        // jobs and 3.7 ms lane per job (C = 129); at band 32 lane per job
-        // (C = 65) stays ahead (2.1 ms against 2.6 ms as (6, 16) jobs)
-        if (c16 && w > 32) {
-            cls[k] = coop_class_of(w, true);
+        if (w > 64 && coop) {
+            cls[k] = coop_class_of(w, c16);
             continue;
         }
         // one long lane-per-job job serialises read_len x (2w + 1) cells on
@@ -232,6 +249,17 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
             return w <= kMaxBand ? "band above 128 needs costs <= 2^20 and read_len * match < 2^21"
                                  : "band wider than 271";
         cls[k] = 2 * b + (pack ? 1 : 0);
+    }
+    {
+        int64_t narrow[2] = {0, 0};  // packed jobs of bands 17 .. 39, 40 .. 71
+        for (int32_t k = 0; k < n; k++)
+            if (cls[k] < 0) narrow[std::max(band ? band[k] : cfg.bandwidth, 1) > 39]++;
+        const int c8[2] = {coop_class_by_shape(10, 8), coop_class_by_shape(18, 8)};
+        for (int32_t k = 0; k < n; k++) {
+            if (cls[k] >= 0) continue;
+            const int32_t w = std::max(band ? band[k] : cfg.bandwidth, 1);
+            cls[k] = narrow[w > 39] >= kNarrowMinJobs ? c8[w > 39] : coop_class_of(w, true);
+        }
     }
     // A lane-per-job class runs one wave per 64 jobs, so a class of fewer
     // than 64 x 1024 jobs leaves SIMDs idle, and a lone wave on a SIMD waits
@@ -387,10 +415,33 @@ SwPlanDev sw_plan_bind(const SwPlanLayout &L, const uint8_t *tab, uint8_t *slabs
     return d;
 }
 
+hipError_t sw_fanout_create(SwFanout &f) {
+    hipError_t e = hipEventCreateWithFlags(&f.fork, hipEventDisableTiming);
+    for (int k = 0; !e && k < kSwFan; k++) {
+        e = hipStreamCreateWithFlags(&f.aux[k], hipStreamNonBlocking);
+        if (!e) e = hipEventCreateWithFlags(&f.join[k], hipEventDisableTiming);
+    }
+    f.ok = !e;
+    if (e) sw_fanout_destroy(f);
+    return e;
+}
+
+void sw_fanout_destroy(SwFanout &f) {
+    for (int k = 0; k < kSwFan; k++) {
+        if (f.aux[k]) (void)hipStreamDestroy(f.aux[k]);
+        if (f.join[k]) (void)hipEventDestroy(f.join[k]);
+        f.aux[k] = nullptr;
+        f.join[k] = nullptr;
+    }
+    if (f.fork) (void)hipEventDestroy(f.fork);
+    f.fork = nullptr;
+    f.ok = false;
+}
+
 const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira_sw_jobs_dev &j,
                            const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
                            const uint8_t *skip_flags, const int32_t *skip_src, hipStream_t s,
-                           hipError_t *err) {
+                           hipError_t *err, SwFanout *fan) {
     *err = hipSuccess;
     if (L.n == 0) return nullptr;
     int32_t go = sat_add(cfg.gap_open, cfg.gap_extend);
@@ -425,8 +476,32 @@ const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira
     a.cigar_cap = d.cigar_cap;
     a.skip_flags = skip_flags;
     a.skip_src = skip_flags ? skip_src : nullptr;
+    // Classes holding long jobs (packets of >= kFanMinRows rows; a class's
+    // first packet is its longest) go round-robin to the fan-out's streams,
+    // the rest in order to s.  The tail of a class of long jobs (its last
+    // few waves, each running for milliseconds) then overlaps other classes;
+    // classes of short jobs have short tails and lose more to sharing the
+    // chip (1M pairs of 100-300 bp: 23.8 ms in order, 24.8 ms fanned out;
+    // 82k windows of 0.5-5 kbp: 87.9 ms in order, 86.7 ms fanned out).
+    constexpr int32_t kFanMinRows = 1024;
+    int n_long = 0;
+    for (int c = 0; c < kNumClasses; c++)
+        n_long += L.class_count[c] > 0 && L.packet_rows[L.class_first[c]] >= kFanMinRows;
+    const int n_aux = (fan && fan->ok) ? std::min(n_long, kSwFan) : 0;
+    if (n_aux > 0) {
+        hipError_t e = hipEventRecord(fan->fork, s);
+        for (int k = 0; !e && k < n_aux; k++) e = hipStreamWaitEvent(fan->aux[k], fan->fork, 0);
+        if (e) {
+            *err = e;
+            return "banded SW fan-out";
+        }
+    }
+    int i_long = 0;
     for (int c = 0; c < kNumClasses; c++) {
         if (L.class_count[c] == 0) continue;
+        hipStream_t cs = s;
+        if (n_aux > 0 && L.packet_rows[L.class_first[c]] >= kFanMinRows)
+            cs = fan->aux[i_long++ % n_aux];
         SwLaunch ac = a;
         int32_t f = L.class_first[c];
         ac.packet_jobs = d.packet_jobs + (int64_t)f * 64;
@@ -439,17 +514,26 @@ const char *sw_plan_launch(const SwPlanLayout &L, const SwPlanDev &d, const kira
         hipError_t e;
         if (class_is_coop16(c)) {
             int waves = std::min(L.class_count[c], 256 * 20);
-            e = launch_banded_sw_coop16(ac, class_coop(c).p / 2, class_coop(c).g, waves, s);
+            e = launch_banded_sw_coop16(ac, class_coop(c).p / 2, class_coop(c).g, waves, cs);
         } else if (class_is_coop(c)) {
             int waves = std::min(L.class_count[c], 256 * 20);
-            e = launch_banded_sw_coop(ac, class_coop(c).p, class_coop(c).g, waves, s);
+            e = launch_banded_sw_coop(ac, class_coop(c).p, class_coop(c).g, waves, cs);
         } else {
             int waves = std::min(L.class_count[c], 256 * 16);
-            e = launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, s);
+            e = launch_banded_sw(ac, kLaneC[c / 2], (c & 1) != 0, waves, cs);
         }
         if (e != hipSuccess) {
             *err = e;
             return "banded SW launch";
+        }
+    }
+    // s waits for every fan-out stream that took a class
+    for (int k = 0; k < n_aux; k++) {
+        hipError_t e = hipEventRecord(fan->join[k], fan->aux[k]);
+        if (!e) e = hipStreamWaitEvent(s, fan->join[k], 0);
+        if (e) {
+            *err = e;
+            return "banded SW fan-in";
         }
     }
     return nullptr;
@@ -552,7 +636,7 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
     hipError_t e;
     const char *why = kira::sw_plan_launch(plan->L, plan->d, *j, *cfg, *kn, nullptr, nullptr,
-                                           ctx->stream, &e);
+                                           ctx->stream, &e, &ctx->fan);
     if (why) return fail(ctx, e ? KIRA_ERR_RUNTIME : KIRA_ERR_INVALID_ARGUMENT, why, e);
     return KIRA_OK;
 }

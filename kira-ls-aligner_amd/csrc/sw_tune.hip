@@ -144,7 +144,8 @@ kira_status run_route(kira_ctx *ctx, const Synth &S, const kira_sw_cfg &cfg,
     for (int rep = 0; rep < 3; rep++) {
         if ((e = hipEventRecord(ev[0], s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "tuning event", e);
-        const char *why = kira::sw_plan_launch(L, d, jd, cfg, kn, nullptr, nullptr, s, &e);
+        const char *why =
+            kira::sw_plan_launch(L, d, jd, cfg, kn, nullptr, nullptr, s, &e, &ctx->fan);
         if (why) return fail(ctx, KIRA_ERR_RUNTIME, why, e);
         float ms = 0;
         if ((e = hipEventRecord(ev[1], s)) || (e = hipEventSynchronize(ev[1])) ||
@@ -233,17 +234,37 @@ kira_status kira_sw_autotune(kira_ctx *ctx, const kira_sw_cfg *cfg, const kira_s
         // candidates: the built-in routing and every route the shape may take
         // (one lane per job is not a candidate where a 257-position band would
         // spill the register file and a cooperative shape holds the band)
-        // and, of each cooperative form, the two smallest shapes that hold
-        // the band (a larger one only adds idle positions to every row)
+        // and, of each cooperative form (int32, packed) and group width, the
+        // smallest shape that holds the band, unless it holds more than twice
+        // the form's smallest (a larger shape only adds idle positions)
         std::vector<int> cand{-1};
         const uint32_t ok = kira::sw_routes_for(w, L, *cfg);
         const bool coop_any = (ok >> kira::kSwRouteCoop) != 0;
         if (((ok >> kira::kSwRouteLane) & 1u) && !(w > 64 && coop_any))
             cand.push_back(kira::kSwRouteLane);
-        int taken[2] = {0, 0};
-        for (int k = 0; k < kira::sw_num_coop_shapes(); k++)
-            if (((ok >> (kira::kSwRouteCoop + k)) & 1u) && taken[k >= kira::kSwCoop32]++ < 2)
-                cand.push_back(kira::kSwRouteCoop + k);
+        for (int form = 0; form < 2; form++) {
+            const int k0 = form ? kira::kSwCoop32 : 0;
+            const int k1 = form ? kira::sw_num_coop_shapes() : kira::kSwCoop32;
+            int min_cap = 1 << 30;
+            for (int k = k0; k < k1; k++) {
+                int P, G;
+                kira::sw_coop_shape(k, &P, &G);
+                if ((ok >> (kira::kSwRouteCoop + k)) & 1u) min_cap = std::min(min_cap, P * G);
+            }
+            for (int gw = 8; gw <= 32; gw *= 2) {
+                int best_k = -1, best_cap = 1 << 30;
+                for (int k = k0; k < k1; k++) {
+                    int P, G;
+                    kira::sw_coop_shape(k, &P, &G);
+                    if (G == gw && ((ok >> (kira::kSwRouteCoop + k)) & 1u) && P * G < best_cap) {
+                        best_k = k;
+                        best_cap = P * G;
+                    }
+                }
+                if (best_k >= 0 && best_cap <= 2 * min_cap)
+                    cand.push_back(kira::kSwRouteCoop + best_k);
+            }
+        }
         kira::DevMem dm(ctx);
         uint8_t *d_reads = nullptr, *d_text = nullptr;
         int64_t *d_ro = nullptr, *d_fo = nullptr;

@@ -73,7 +73,7 @@ def test_autotune_cache_and_tuned_batches(pkg, ctx, tmp_path):
     assert len(lines) == len(SHAPES)
 
 
-@pytest.mark.parametrize("route", list(range(1, 12)))
+@pytest.mark.parametrize("route", list(range(1, 14)))
 def test_every_route_matches_oracle(pkg, ctx, tmp_path, route):
     """Every job forced onto one route (where the route holds its band) through
     a cache file whose lines name each shape key of the batch."""
@@ -125,7 +125,7 @@ def _forced_context(pkg, ctx, tmp_path, route):
     return local
 
 
-PACKED = [7, 8, 9, 10, 11]
+PACKED = [7, 8, 9, 10, 11, 12, 13]
 
 
 @pytest.mark.parametrize("route", [2, 3, 4, 5, 6] + PACKED)
@@ -171,7 +171,7 @@ def test_cooperative_routes_on_tie_corpus(pkg, ctx, tmp_path, route, adaptive):
 # bands that fill each packed shape (2w + 1 = 2 P2 G - 1; 271, the widest
 # band a plan takes, for the (18, 32) shape) and the cost bounds of the
 # packed form
-PACKED_EDGES = [(7, 47), (8, 79), (9, 143), (10, 223), (11, 271)]
+PACKED_EDGES = [(7, 47), (8, 79), (9, 143), (10, 223), (11, 271), (12, 39), (13, 71)]
 
 
 @pytest.mark.parametrize("route,w", PACKED_EDGES)
