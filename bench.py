@@ -356,6 +356,8 @@ def main():
     ap.add_argument("--sw-steps", type=int, default=3)
     ap.add_argument("--sw-tune-cache", default=os.environ.get("KIRA_SW_TUNE_CACHE", ""),
                     help="file the SW autotuner writes its routes to (default: none)")
+    ap.add_argument("--no-sw-tune", action="store_true",
+                    help="SW legs on the built-in routing only (no autotuner pass)")
     ap.add_argument("--cpu-sample-pairs", type=int, default=65_536)
     ap.add_argument("--long-windows", type=int, default=82_000)
     ap.add_argument("--stage4-reads", type=int, default=200_000)
@@ -1183,52 +1185,60 @@ def bench_sw(pkg, ctx, genome, args, stream, lw):
     x-drop 100, no abort) in one device-resident batch; long-read chain-gap
     windows (500-5000 bp, band 5%); and both in one queue.  Each runs first on
     the built-in routing, then on the routes the autotuner picks for the
-    workloads' shapes (the headline figures)."""
+    workloads' shapes (the headline figures); with --no-sw-tune only on the
+    built-in routing."""
     t0 = time.time()
     sw = make_sw_workload(genome, n_pairs=args.sw_pairs)
     print(f"[bench] SW workload: {sw['n']} pairs generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     pkg.sw_tune_clear(ctx)
-    builtin = {"short": run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, 0,
-                                     "SW short pairs (built-in routing)")}
-    if lw is not None:
-        builtin["long"] = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 0,
-                                       "SW long windows (built-in routing)")
     mixed = concat_sw(sw, lw) if lw is not None else None
-    shapes = sw_tune_shapes(*[x for x in (sw, lw, mixed) if x is not None])
-    t0 = time.time()
-    cache = args.sw_tune_cache or None
-    # one shape a call, so that each finished shape is reported as it ends
-    tuned = []
-    for k, shape in enumerate(shapes):
-        t, = pkg.sw_autotune(ctx, [shape], pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5), pkg.sw_knobs(),
-                             cache_path=cache)
-        tuned.append(t)
-        print(f"[bench] SW autotune {k + 1}/{len(shapes)}: band {shape[0]}, read {shape[1]}, "
-              f"{shape[2]} jobs -> {t['route']} ({t['ms_best']:.2f} ms, built-in "
-              f"{t['ms_builtin']:.2f} ms, {t['candidates']} timed)", file=sys.stderr, flush=True)
-    t_tune = time.time() - t0
-    chosen = [dict(band=b, read_len=l, jobs=n, route=t["route"], candidates=t["candidates"],
-                   ms_best=t["ms_best"], ms_builtin=t["ms_builtin"])
-              for (b, l, n), t in zip(shapes, tuned)]
-    print(f"[bench] SW autotune: {len(shapes)} shapes in {t_tune:.1f}s; "
-          f"{sum(c['route'] != 'built-in' for c in chosen)} re-routed", file=sys.stderr,
-          flush=True)
+    tune = not args.no_sw_tune
+    if tune:
+        builtin = {"short": run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, 0,
+                                         "SW short pairs (built-in routing)")}
+        if lw is not None:
+            builtin["long"] = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 0,
+                                           "SW long windows (built-in routing)")
+        shapes = sw_tune_shapes(*[x for x in (sw, lw, mixed) if x is not None])
+        t0 = time.time()
+        cache = args.sw_tune_cache or None
+        # one shape a call, so that each finished shape is reported as it ends
+        tuned = []
+        for k, shape in enumerate(shapes):
+            t, = pkg.sw_autotune(ctx, [shape], pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5),
+                                 pkg.sw_knobs(), cache_path=cache)
+            tuned.append(t)
+            print(f"[bench] SW autotune {k + 1}/{len(shapes)}: band {shape[0]}, read "
+                  f"{shape[1]}, {shape[2]} jobs -> {t['route']} ({t['ms_best']:.2f} ms, "
+                  f"built-in {t['ms_builtin']:.2f} ms, {t['candidates']} timed)",
+                  file=sys.stderr, flush=True)
+        t_tune = time.time() - t0
+        chosen = [dict(band=b, read_len=l, jobs=n, route=t["route"], candidates=t["candidates"],
+                       ms_best=t["ms_best"], ms_builtin=t["ms_builtin"])
+                  for (b, l, n), t in zip(shapes, tuned)]
+        print(f"[bench] SW autotune: {len(shapes)} shapes in {t_tune:.1f}s; "
+              f"{sum(c['route'] != 'built-in' for c in chosen)} re-routed", file=sys.stderr,
+              flush=True)
     out = run_sw_batch(pkg, ctx, sw, args, stream, args.sw_steps, args.cpu_sample_pairs,
                        "SW short pairs")
-    out["builtin_routing"] = {k: builtin["short"][k] for k in ("ms_per_batch", "gcups",
-                                                                "routes")}
-    out["tuned_vs_builtin"] = out["gcups"] / builtin["short"]["gcups"]
-    out["autotune"] = {"shapes": chosen, "seconds": t_tune, "cache": cache}
+    if tune:
+        out["builtin_routing"] = {k: builtin["short"][k] for k in ("ms_per_batch", "gcups",
+                                                                    "routes")}
+        out["tuned_vs_builtin"] = out["gcups"] / builtin["short"]["gcups"]
+        out["autotune"] = {"shapes": chosen, "seconds": t_tune, "cache": cache}
+    else:
+        out["autotune"] = None
     out["workload"] = ("uneven banded SW: 1M pairs, L U{100..300}, band U{16,32,64,128}, "
                        "1-5% subs + 0.5% indels, adaptive band, x-drop 100, int16/int32 "
                        "routed per job")
     out["pairs"] = out["jobs"]
     if lw is not None:
         lo = run_sw_batch(pkg, ctx, lw, args, stream, args.sw_steps, 256, "SW long windows")
-        lo["builtin_routing"] = {k: builtin["long"][k] for k in ("ms_per_batch", "gcups",
-                                                                  "routes")}
-        lo["tuned_vs_builtin"] = lo["gcups"] / builtin["long"]["gcups"]
+        if tune:
+            lo["builtin_routing"] = {k: builtin["long"][k] for k in ("ms_per_batch", "gcups",
+                                                                      "routes")}
+            lo["tuned_vs_builtin"] = lo["gcups"] / builtin["long"]["gcups"]
         lo["workload"] = ("long-read chain-gap windows: 500-5000 bp, band ceil(5% len) "
                           "(25-250), 5% subs, 2.5% ins, 2.5% del, exact int32 (int16 storage only where proven)")
         out["long_windows"] = lo

@@ -1,6 +1,6 @@
 // Stage-4 alignment driver: src/pipeline/stage4_alignment.rs run() (:959-1268)
-// with process_read_prefilter (:535-957), the GPU dispatcher active,
-// single-end reads and the two-tier candidate reorder off.  Chain selection,
+// with process_read_prefilter (:535-957), the GPU dispatcher active or not,
+// single-end reads, the two-tier candidate reorder on or off.  Chain selection,
 // job routing and the record text (NM / MD) are host work; exact matching,
 // the ungapped prefilter, the 2-bit sieve with certification, the lane-group
 // SW and the banded SW run on the device.

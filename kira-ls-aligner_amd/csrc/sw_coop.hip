@@ -518,8 +518,9 @@ __global__ __launch_bounds__(64) void banded_sw_coop_kernel(SwLaunch a) {
 // Values are kept x4 so that a value and a 2-bit tag share one int16:
 //   H4 = 4 H (band storage; 0 outside the band),
 //   E4 = 4 E + 1 (outside the band -30719); the E candidates are
-//     4 (H - go) + 2 (open) and 4 (E - ge) + 1 (extend), so a tie opens, as
-//     in the reference, and bit 0 of the winner is the E-extend bit,
+//     4 (H - go) + 3 (open) and 4 (E - ge) + 1 (extend), so a tie opens, as
+//     in the reference, bit 1 of the winner is clear where E extends, and
+//     clearing bit 1 leaves the E key 4 E + 1,
 //   the H candidates are keys 4 value + priority, STOP 3 > diagonal 2 > E 1
 //     > F 0, as in the int32 form,
 //   F4 = 4 F and Hg4 = 4 (Hpre - gap open), Hpre = max(diag + s, E, 0):
@@ -578,13 +579,13 @@ struct SplitBand {
             E[r] = kPadE4;
         }
     }
-    // new[q] = old[q + delta] over the whole band (g: lane in the group)
+    // new[q] = old[q + delta] over the whole band (g: lane in the group); a
+    // shift by the whole band or more leaves it clear (G lane moves, each
+    // filling the group edge, rather than a separate clear, which would keep
+    // a copy of the pad constants live in every row)
     __device__ __forceinline__ void shift(int32_t delta, int g) {
         constexpr int P = 2 * P2;
-        if (delta >= G * P || delta <= -G * P) {
-            clear();
-            return;
-        }
+        delta = imin(imax(delta, -G * P), G * P);
         while (delta >= P) {  // whole lanes
 #pragma unroll
             for (int r = 0; r < P2; r++) {
@@ -666,7 +667,7 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
     const int32_t xdrop = a.dp_xdrop;
     const int32_t A = a.match_score, B = a.mismatch, go = a.go, ge = a.gap_extend;
     const int32_t q0 = g * P;
-    const uint32_t kS4 = pk2(4 * A + 2), kNAB4 = pk2(-4 * (A + B)), kOpen = pk2(2 - 4 * go),
+    const uint32_t kS4 = pk2(4 * A + 2), kNAB4 = pk2(-4 * (A + B)), kOpen = pk2(3 - 4 * go),
                    kGe4 = pk2(4 * ge), kGop4 = pk2(4 * (go - ge));
     // lane positions past 2w are outside the band in every row
     uint32_t vstat[P2];
@@ -692,9 +693,8 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
             if (g == 0) meta[(int64_t)(i - 1) * 64 + wl] = make_int2(base, 1);
             continue;
         }
-        if (prev_empty) {
-            bd.clear();
-        } else {
+        // the band is clear at the job's start and after an empty row
+        if (!prev_empty) {
             const int32_t delta = c - prev_c - 1;
             if (delta != 0) bd.shift(delta, g);
         }
@@ -751,7 +751,7 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
             const uint32_t s4 = as_u(as_s16(mis) * as_s16(kNAB4) + as_s16(kS4));
             const uint32_t hmK = pk_add(bd.H[r], s4);
             const uint32_t eK = pk_max(pk_add(uh, kOpen), pk_sub(ue, kGe4));
-            const uint32_t eKey = (eK & kLow2) | 0x00010001u;
+            const uint32_t eKey = eK & 0xFFFDFFFDu;
             bd.E[r] = bitsel(vm[r], eKey, kPadE4);
             ek[r] = eK;
             const uint32_t kh = pk_max(pk_max(hmK, eKey), 0x00030003u);
@@ -798,11 +798,14 @@ __device__ int2 coop16_job(const SwLaunch &a, int32_t job, int g, int wl, int32_
             const uint32_t t = pk_sub_sat(pH, pF);  // sign: F-extend
             pF = ff;
             pH = hg[r];
-            const uint32_t code = (~K & 0x00030003u) | ((ek[r] & 0x00010001u) << 2) |
+            // trace code: direction 3 - priority, E-extend (bit 1 of eK clear),
+            // F-extend (sign of t); the first two are stored inverted here
+            // and flipped once per word
+            const uint32_t code = (K & 0x00030003u) | ((ek[r] << 1) & 0x00040004u) |
                                   ((t >> 12) & 0x00080008u);
             word |= code << (4 * (r & 3));
             if ((r & 3) == 3 || r == P2 - 1) {
-                trow[(r >> 2) * 64] = word;
+                trow[(r >> 2) * 64] = word ^ 0x77777777u;
                 word = 0;
             }
             // (4 h << 16) | (0xFFFF - lane position): first strict maximum
