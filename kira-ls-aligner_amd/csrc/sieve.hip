@@ -465,80 +465,120 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
     }
 }
 
+// Accepted jobs of a workgroup queued for certification (indices into the
+// workgroup's 256 jobs' results), so that certification runs once per
+// workgroup on a compact set of lanes instead of in every wave that holds an
+// accepted job (about one job in sixteen in a read-major batch).
+struct CertQueue {
+    int32_t job[256], shift[256];
+    uint32_t best[256], second[256];
+    int32_t n;
+};
+
 template <int RW>
 __global__ __launch_bounds__(256) void sieve_kernel(SieveLaunch a) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.n_jobs) return;
-    int32_t r = a.job_read ? a.job_read[j] : (int32_t)(j / a.jobs_per_read);
-    int32_t L = a.read_lens[r];
-    int32_t W = a.win_len[j];
-    int64_t s0 = a.win_start[j];
-    int32_t lim = a.limit ? a.limit[j] : a.limit_all;
-
-    // Skip (never dispatch, stage4_alignment.rs:1284-1288) jobs whose read or window is not all ACGT, and
-    // windows that leave the text.
-    bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
-              s0 + W <= a.g_bases && L <= 32 * RW;
-    if (ok && a.g_il && W > 0) {
-        int64_t q0 = s0 + (int64_t)kPadFront * 32, q1 = q0 + W;
-        uint32_t any = 0;
-        for (int64_t w = q0 >> 5; w <= ((q1 - 1) >> 5); w++) {
-            // the word bitmap rules out almost every word without touching il
-            if (!((a.g_nz[w >> 5] >> (w & 31)) & 1u)) continue;
-            uint32_t v = a.g_il[w].x;
-            int lo = (int)(w == (q0 >> 5) ? (q0 & 31) : 0);
-            int hi = (int)(w == ((q1 - 1) >> 5) ? (((q1 - 1) & 31) + 1) : 32);
-            uint32_t m = (hi == 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
-            any |= v & m;
-        }
-        ok = any == 0;
-    }
-    if (!ok) {
-        a.out_shift[j] = -1;
-        a.out_best[j] = INT32_MAX;
-        a.out_second[j] = INT32_MAX;
-        if (a.out_flags) a.out_flags[j] = 0;
-        return;
-    }
-    if (L == 0 || W < L) {
-        a.out_shift[j] = -1;
-        a.out_best[j] = INT32_MAX;
-        a.out_second[j] = INT32_MAX;
-        if (a.out_flags) a.out_flags[j] = 1;
-        return;
-    }
-    const uint32_t *rp = a.read_planes + (int64_t)r * kReadWords;
-    uint32_t rl[RW], rh[RW], mk[RW];
-#pragma unroll
-    for (int k = 0; k < RW; k++) {
-        rl[k] = rp[k];
-        rh[k] = rp[8 + k];
-        int keep = L - 32 * k;
-        mk[k] = keep >= 32 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << keep) - 1u));
-    }
-    uint32_t nt = (uint32_t)(W - L + 1);
-    uint32_t gbm, gbt, gsm;
-    if (L > 32 * (RW - 1))
-        sieve_core<RW, false>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
-    else
-        sieve_core<RW, true>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
-
-    bool acc = (int64_t)gbm <= (int64_t)lim;
-    a.out_shift[j] = acc ? (int32_t)gbt : -1;
-    a.out_best[j] = acc ? (int32_t)gbm : INT32_MAX;
-    a.out_second[j] = gsm == 0xFFFFFFFFu ? INT32_MAX : (int32_t)gsm;
-    if (a.out_flags) {
-        uint8_t f = 1;
-        if (acc) {
-            f |= 2;
-            if (a.cert_max_mism >= 0) {
-                uint32_t rc[RW];
-#pragma unroll
-                for (int k = 0; k < RW; k++) rc[k] = rp[16 + k];
-                if (certify<RW>(rl, rh, rc, a, s0, L, (int)gbt, gbm, gsm)) f |= 4;
+    __shared__ CertQueue cq;
+    if (threadIdx.x == 0) cq.n = 0;
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // every thread reaches the queue barrier below; `done` marks a job whose
+    // results are written
+    bool done = j >= a.n_jobs;
+    int32_t r = 0, L = 0, W = 0, lim = 0;
+    int64_t s0 = 0;
+    if (!done) {
+        r = a.job_read ? a.job_read[j] : (int32_t)(j / a.jobs_per_read);
+        L = a.read_lens[r];
+        W = a.win_len[j];
+        s0 = a.win_start[j];
+        lim = a.limit ? a.limit[j] : a.limit_all;
+        // Skip (never dispatch, stage4_alignment.rs:1284-1288) jobs whose read
+        // or window is not all ACGT, and windows that leave the text.
+        bool ok = (a.read_valid ? a.read_valid[r] != 0 : true) && W >= 0 && s0 >= 0 &&
+                  s0 + W <= a.g_bases && L <= 32 * RW;
+        if (ok && a.g_il && W > 0) {
+            const int64_t q0 = s0 + (int64_t)kPadFront * 32, q1 = q0 + W;
+            const int64_t w0 = q0 >> 5, w1 = (q1 - 1) >> 5;
+            // the word bitmap rules out almost every window at once: its bits
+            // w0 .. w1 lie in two consecutive bitmap words when w1 - w0 < 32
+            // (the bitmap has a spare word at its end)
+            bool maybe = true;
+            if (w1 - w0 < 32) {
+                const uint64_t bits =
+                    (((uint64_t)a.g_nz[(w0 >> 5) + 1] << 32) | a.g_nz[w0 >> 5]) >> (w0 & 31);
+                maybe = (bits & ((2ull << (w1 - w0)) - 1)) != 0;
+            }
+            if (maybe) {
+                uint32_t any = 0;
+                for (int64_t w = w0; w <= w1; w++) {
+                    if (!((a.g_nz[w >> 5] >> (w & 31)) & 1u)) continue;
+                    uint32_t v = a.g_il[w].x;
+                    int lo = (int)(w == w0 ? (q0 & 31) : 0);
+                    int hi = (int)(w == w1 ? (((q1 - 1) & 31) + 1) : 32);
+                    uint32_t m = (hi == 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+                    any |= v & m;
+                }
+                ok = any == 0;
             }
         }
-        a.out_flags[j] = f;
+        if (!ok || L == 0 || W < L) {
+            a.out_shift[j] = -1;
+            a.out_best[j] = INT32_MAX;
+            a.out_second[j] = INT32_MAX;
+            if (a.out_flags) a.out_flags[j] = ok ? 1 : 0;
+            done = true;
+        }
+    }
+    if (!done) {
+        const uint32_t *rp = a.read_planes + (int64_t)r * kReadWords;
+        uint32_t rl[RW], rh[RW], mk[RW];
+#pragma unroll
+        for (int k = 0; k < RW; k++) {
+            rl[k] = rp[k];
+            rh[k] = rp[8 + k];
+            int keep = L - 32 * k;
+            mk[k] = keep >= 32 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << keep) - 1u));
+        }
+        uint32_t nt = (uint32_t)(W - L + 1);
+        uint32_t gbm, gbt, gsm;
+        if (L > 32 * (RW - 1))
+            sieve_core<RW, false>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
+        else
+            sieve_core<RW, true>(a, rl, rh, mk, s0, nt, gbm, gbt, gsm);
+
+        bool acc = (int64_t)gbm <= (int64_t)lim;
+        a.out_shift[j] = acc ? (int32_t)gbt : -1;
+        a.out_best[j] = acc ? (int32_t)gbm : INT32_MAX;
+        a.out_second[j] = gsm == 0xFFFFFFFFu ? INT32_MAX : (int32_t)gsm;
+        if (a.out_flags) {
+            if (acc && a.cert_max_mism >= 0) {
+                const int32_t at = atomicAdd(&cq.n, 1);
+                cq.job[at] = (int32_t)threadIdx.x;
+                cq.shift[at] = (int32_t)gbt;
+                cq.best[at] = gbm;
+                cq.second[at] = gsm;
+            } else {
+                a.out_flags[j] = acc ? 3 : 1;
+            }
+        }
+    }
+    __syncthreads();
+    // certification of the queued jobs, by the first threads of the workgroup
+    const int32_t nq = cq.n;
+    for (int32_t q = (int32_t)threadIdx.x; q < nq; q += (int32_t)blockDim.x) {
+        const int64_t jq = (int64_t)blockIdx.x * blockDim.x + cq.job[q];
+        const int32_t rq = a.job_read ? a.job_read[jq] : (int32_t)(jq / a.jobs_per_read);
+        const uint32_t *rp = a.read_planes + (int64_t)rq * kReadWords;
+        uint32_t rl[RW], rh[RW], rc[RW];
+#pragma unroll
+        for (int k = 0; k < RW; k++) {
+            rl[k] = rp[k];
+            rh[k] = rp[8 + k];
+            rc[k] = rp[16 + k];
+        }
+        const bool cert = certify<RW>(rl, rh, rc, a, a.win_start[jq], a.read_lens[rq],
+                                      cq.shift[q], cq.best[q], cq.second[q]);
+        a.out_flags[jq] = cert ? 7 : 3;
     }
 }
 
