@@ -343,6 +343,25 @@ def sieve_oracle_flags(lay, shifts, best, second, K, cert_max):
 
 
 # --------------------------------------------------------------- main
+# The stage-4 CPU baseline's worker processes: each rebuilds the seeded batch
+# on its first task (the untimed warm-up) and runs the CPU restatement on
+# read ranges.
+_S4 = {}
+
+
+def _s4_run(task):
+    from oracle import stage4 as S
+    n_reads, lo, hi = task
+    if _S4.get("n") != n_reads:
+        from tests.stage4_data import make_batch
+        _S4["b"] = make_batch(21, n_reads, n_contigs=8, contig_len=2_500_000)
+        _S4["n"] = n_reads
+    b = _S4["b"]
+    S.run(b["contigs"], b["reads"][lo:hi], b["chains"][lo:hi], S.default_cfg(),
+          b["repeat_min_occ"][lo:hi])
+    return hi - lo
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +403,22 @@ def main():
         print("bench.py: this build targets one GPU; --gpus must be 1 "
               "(the path shards with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    # the stage-4 CPU baseline's worker processes start (idle) before
+    # anything here touches the GPU
+    args.s4_pool, args.s4_threads = None, len(os.sched_getaffinity(0))
+    if (not args.no_cpu and args.stage4_reads > 0 and not args.sw_only and
+            not args.fastpath_only):
+        import multiprocessing as mp
+        args.s4_pool = mp.get_context("spawn").Pool(args.s4_threads)
+    try:
+        run_bench(args)
+    finally:
+        if args.s4_pool is not None:
+            args.s4_pool.terminate()
+            args.s4_pool.join()
+
+
+def run_bench(args):
     import torch
 
     pkg = load_pkg()
@@ -760,13 +795,36 @@ def bench_fastpath(pkg, ctx, args):
                              md=md[m0:m0 + int(o["md_len"])].decode(),
                              cigar=[int(v) for v in cig[c0:c0 + int(o["n_cigar"])]]))
         parity = norm == gotm
-        out["cpu_baseline"] = {"value": m / cpu_s, "unit": "chains/s", "cores": 1, "kind": "port",
-                               "sample": f"first {m} chains through the CPU restatement "
-                                         "(Python cascade over the C Myers / WFA oracle)"}
+        # the C restatement of the cascade on every CPU of the process and on
+        # one thread (1 warm-up + median of 5), with parity on its sample
+        mc = min(n, 65536)
+        threads = len(os.sched_getaffinity(0))
+        prep = F.fast_dp_prepare(b["reads"][:mc], b["chains"][:mc], b["genome"],
+                                 b["contig_off"], b["contig_len"])
+        t_all, cres = median_of_runs(lambda: F.fast_dp_run_mt(prep, ocfg, router, threads))
+        m1 = max(1, mc // 16)
+        prep1 = F.fast_dp_prepare(b["reads"][:m1], b["chains"][:m1], b["genome"],
+                                  b["contig_off"], b["contig_len"])
+        t_one, _ = median_of_runs(lambda: F.fast_dp_run_mt(prep1, ocfg, router, 1))
+        g = arr[:mc]
+        keys = ("kind", "ref_start", "ref_end", "score", "nm", "n_cigar")
+        live = g["kind"] != 0
+        parity_c = bool(np.array_equal(cres["kind"], g["kind"]) and all(
+            np.array_equal(cres[k][live], g[k][live]) for k in keys[1:]))
+        out["cpu_baseline"] = {
+            "value": mc / t_all, "unit": "chains/s", "cores": threads, "kind": "port",
+            "single_thread_value": m1 / t_one,
+            "sample": f"first {mc} chains of the batch through the C restatement of the "
+                      f"cascade (oracle/kira_fast_oracle.c, pthreads over chains, {threads} "
+                      f"threads = the CPUs this process may use; the first {m1} on 1 "
+                      f"thread), 1 warm-up + median of 5",
+            "python_driver_chains_per_s": m / cpu_s}
         out["cpu_wfa_pairs_per_s"] = m / cpu_wfa
-        out["parity_vs_oracle_sample"] = parity
-        print(f"[bench] fastpath cpu baseline {m / cpu_s:.0f} chains/s, WFA {m / cpu_wfa:.0f} "
-              f"pairs/s (1 thread); parity={parity}", file=sys.stderr, flush=True)
+        out["parity_vs_oracle_sample"] = parity and parity_c
+        print(f"[bench] fastpath cpu baseline {mc / t_all:.0f} chains/s on {threads} threads, "
+              f"{m1 / t_one:.0f} on 1 (C restatement); Python driver {m / cpu_s:.0f} chains/s; "
+              f"WFA {m / cpu_wfa:.0f} pairs/s (1 thread); parity={parity} (MD / CIGAR, "
+              f"{m} chains), {parity_c} (C, {mc} chains)", file=sys.stderr, flush=True)
     genome.close()
     return out
 
@@ -840,9 +898,27 @@ def bench_stage4(pkg, ctx, args):
         out["cpu_baseline"] = {"value": m / cpu_s, "unit": "reads/s", "cores": 1, "kind": "port",
                                "sample": f"first {m} reads of the batch through the CPU "
                                          "restatement (Python driver over the C oracle)"}
+        pool = getattr(args, "s4_pool", None)
+        if pool is not None and n >= 1000 * args.s4_threads:
+            # the same restatement in one process per CPU of this process, a
+            # range of 1000 reads each (1 warm-up + median of 5), and one
+            # range on one process
+            T, chunk = args.s4_threads, 1000
+            ranges = [(n, k * chunk, (k + 1) * chunk) for k in range(T)]
+            t_all, _ = median_of_runs(lambda: sum(pool.map(_s4_run, ranges, chunksize=1)))
+            t_one, _ = median_of_runs(lambda: pool.apply(_s4_run, ((n, 0, chunk),)))
+            out["cpu_baseline"] = {
+                "value": T * chunk / t_all, "unit": "reads/s", "cores": T, "kind": "port",
+                "single_thread_value": chunk / t_one,
+                "sample": f"first {T * chunk} reads of the batch through the CPU restatement "
+                          f"(Python driver over the C oracle), {T} worker processes = the CPUs "
+                          f"this process may use, {chunk} reads each; one range on one "
+                          "process; 1 warm-up + median of 5"}
         out["parity_vs_oracle_sample"] = parity
-        print(f"[bench] stage4 cpu baseline {m / cpu_s:.0f} reads/s (1 thread); "
-              f"parity={parity}", file=sys.stderr, flush=True)
+        cb = out["cpu_baseline"]
+        print(f"[bench] stage4 cpu baseline {cb['value']:.0f} reads/s on {cb['cores']} "
+              f"processes ({m / cpu_s:.0f} on one, first {m} reads); parity={parity}",
+              file=sys.stderr, flush=True)
     genome.close()
     out["plain_mem_c1"] = bench_c1(pkg, ctx, args)
     return out

@@ -195,3 +195,71 @@ def fast_dp(read: bytes, contig: bytes, ref_start: int, read_start: int, chain_s
         return None
     a = wfa_result_to_alignment(read, ftext, fstart, w, c)
     return None if a is None else ("wfa", a)
+
+
+# ---------------------------------------------------------------- C cascade
+import ctypes as _C
+
+import numpy as _np
+
+
+class _Router(_C.Structure):
+    _fields_ = [(n, _C.c_int32) for n in (
+        "wfa_max_read_len", "algo", "quality_gate", "quality_pct", "wfa_budget_pct", "myers_pct",
+        "myers_floor", "spectral_pct", "spectral_floor", "wfa_ends_free", "wfa_lead",
+        "wfa_adaptive", "cert_strict")]
+
+
+FAST_OUT = _np.dtype([("kind", _np.int32), ("ref_start", _np.uint32), ("ref_end", _np.uint32),
+                      ("read_start", _np.uint32), ("read_end", _np.uint32),
+                      ("score", _np.int32), ("nm", _np.uint32), ("n_cigar", _np.int32)])
+_ALGO = {"packed": 0, "spectral": 1, "wfa": 2, "sw": 3}
+
+
+def _router(r: Router) -> _Router:
+    return _Router(r.wfa_max_read_len, _ALGO.get(r.algo, 0), int(r.quality_gate), r.quality_pct,
+                   r.wfa_budget_pct, r.myers_pct, r.myers_floor, r.spectral_pct,
+                   r.spectral_floor, r.wfa_ends_free, r.wfa_lead, r.wfa_adaptive,
+                   int(r.cert_strict))
+
+
+def fast_dp_prepare(reads, chains, genome: bytes, contig_off, contig_len):
+    """The flat arrays ko_fast_dp_batch_mt reads (built once, outside any
+    timing).  chains: (contig, ref_start, ref_end, read_start, read_end,
+    strand, score)."""
+    n = len(reads)
+    lens = _np.array([len(x) for x in reads], _np.int32)
+    off = _np.zeros(n, _np.int64)
+    if n:
+        off[1:] = _np.cumsum(lens[:-1], dtype=_np.int64)
+    ch = _np.array([(x[0], x[1], x[3], x[6]) for x in chains], _np.int64).reshape(-1, 4)
+    return dict(n=n, lens=lens, off=off,
+                flat=_np.frombuffer(b"".join(reads) + b"\0" * 8, _np.uint8),
+                cc=_np.ascontiguousarray(ch[:, 0], _np.int32),
+                rs=_np.ascontiguousarray(ch[:, 1], _np.uint32),
+                qs=_np.ascontiguousarray(ch[:, 2], _np.uint32),
+                sc=_np.ascontiguousarray(ch[:, 3], _np.int32),
+                g=_np.frombuffer(genome, _np.uint8),
+                co=_np.ascontiguousarray(contig_off, _np.int64),
+                cl=_np.ascontiguousarray(contig_len, _np.int64))
+
+
+def fast_dp_run_mt(prep, c, r: Router = Router(), threads: int = 1):
+    """The cascade (kira_fast_oracle.c, the same steps as fast_dp) over every
+    prepared chain on `threads` pthreads.  Returns a FAST_OUT array (kind 0:
+    no result, 1 packed, 2 WFA)."""
+    out = _np.zeros(prep["n"], FAST_OUT)
+    P = lambda a: a.ctypes.data_as(_C.c_void_p)
+    rt = _router(r)
+    O.lib().ko_fast_dp_batch_mt(prep["n"], P(prep["flat"]), P(prep["off"]), P(prep["lens"]),
+                                P(prep["g"]), P(prep["co"]), P(prep["cl"]), P(prep["cc"]),
+                                P(prep["rs"]), P(prep["qs"]), P(prep["sc"]), _C.byref(c),
+                                _C.byref(rt), P(out), int(threads))
+    return out
+
+
+def fast_dp_batch_mt(reads, chains, genome: bytes, contig_off, contig_len, c, r: Router = Router(),
+                     threads: int = 1):
+    """fast_dp_prepare + fast_dp_run_mt."""
+    return fast_dp_run_mt(fast_dp_prepare(reads, chains, genome, contig_off, contig_len), c, r,
+                          threads)

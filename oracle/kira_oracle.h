@@ -198,6 +198,37 @@ int ko_wfa_score_only_scored(const uint8_t *p, int32_t m, const uint8_t *t, int3
                              const ko_cfg *cfg, int32_t max_score, int32_t adaptive_drop,
                              int32_t text_begin_free, int32_t *score);
 
+/* ---- the fast DP cascade (kira_fast_oracle.c) ---- */
+/* The router knobs (src/alignment/router.rs:48-215); algo 0 packed,
+ * 1 spectral, 2 wfa, 3 sw. */
+typedef struct {
+    int32_t wfa_max_read_len, algo, quality_gate, quality_pct, wfa_budget_pct, myers_pct,
+        myers_floor, spectral_pct, spectral_floor, wfa_ends_free, wfa_lead, wfa_adaptive,
+        cert_strict;
+} ko_router;
+/* kind 0 none (the caller falls back to SW), 1 packed certificate, 2 WFA */
+typedef struct {
+    int32_t kind;
+    uint32_t ref_start, ref_end, read_start, read_end;
+    int32_t score;
+    uint32_t nm;
+    int32_t n_cigar;
+} ko_fast_out;
+/* try_fast_dp_alignment_inner (src/alignment/mod.rs:285-440) for one chain;
+ * returns out->kind != 0. */
+int ko_fast_dp(const uint8_t *read, size_t read_len, const uint8_t *contig, size_t contig_len,
+               uint32_t ref_start, uint32_t read_start, int32_t chain_score, const ko_cfg *cfg,
+               const ko_router *router, ko_fast_out *out);
+/* the same over n chains on `threads` pthreads; chain k reads reads +
+ * read_off[k] and contig chain_contig[k] of the genome */
+void ko_fast_dp_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_off,
+                         const int32_t *read_len, const uint8_t *genome,
+                         const int64_t *contig_off, const int64_t *contig_len,
+                         const int32_t *chain_contig, const uint32_t *chain_ref_start,
+                         const uint32_t *chain_read_start, const int32_t *chain_score,
+                         const ko_cfg *cfg, const ko_router *router, ko_fast_out *out,
+                         int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,3 +116,31 @@ def test_myers_reject():
     rng = np.random.default_rng(1)
     read = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=150).tolist())
     assert F.fast_dp(read, g, 1000, 0, 1, C) is None
+
+
+def test_c_cascade_matches_the_python_cascade():
+    """The C restatement used as the multi-core CPU baseline
+    (oracle/kira_fast_oracle.c) gives the Python cascade's kind, coordinates,
+    score and NM on every chain of a mixed batch (all twelve read kinds: exact,
+    substitutions, indels, 5' deletions, foreign tails, N, contig ends, ...),
+    on one thread and on several, with the default and non-default routers."""
+    from tests.fastpath_data import make_fast_batch
+    b = make_fast_batch(5, 1200)
+    for router in (F.Router(), F.Router(wfa_lead=8, cert_strict=True, wfa_budget_pct=15)):
+        want = [F.fast_dp(r, b["contigs"][ch[0]], ch[1], ch[3], ch[6], C, router)
+                for r, ch in zip(b["reads"], b["chains"])]
+        for threads in (1, 4):
+            got = F.fast_dp_batch_mt(b["reads"], b["chains"], b["genome"], b["contig_off"],
+                                     b["contig_len"], C, router, threads)
+            kinds = {0: 0, 1: 0, 2: 0}
+            for k, (w, g) in enumerate(zip(want, got)):
+                kinds[int(g["kind"])] += 1
+                if w is None:
+                    assert g["kind"] == 0, k
+                    continue
+                kind, rec = w
+                assert int(g["kind"]) == (1 if kind == "packed" else 2), k
+                assert (int(g["ref_start"]), int(g["ref_end"]), int(g["score"]), int(g["nm"]),
+                        int(g["n_cigar"])) == (rec["ref_start"], rec["ref_end"], rec["score"],
+                                               rec["nm"], len(rec["cigar"])), k
+            assert min(kinds.values()) > 50, kinds
