@@ -171,6 +171,14 @@ uint32_t sw_routes_for(int32_t w, int32_t read_len, const kira_sw_cfg &cfg);
 int sw_num_coop_shapes();
 void sw_coop_shape(int k, int *P, int *G);
 
+// Packs the CIGARs a plan launch left in its arena (job k at arena_off[k],
+// res[k * kSwResWords + 6] ops) one after another: dst_off[k] receives job k's
+// offset in dst (an exclusive prefix sum of the op counts, computed on the
+// device) and its ops are copied there.  dst may be pinned host memory.
+hipError_t launch_cigar_compact(int32_t n, const int32_t *res, const uint32_t *arena,
+                                const int64_t *arena_off, int64_t *dst_off, uint32_t *dst,
+                                hipStream_t s);
+
 // Auxiliary streams a plan launch fans its kernel classes out to, so that
 // the tail of one class (its last, longest jobs) overlaps the next class
 // instead of idling the chip; joined back into the caller's stream with

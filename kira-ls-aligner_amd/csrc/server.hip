@@ -92,10 +92,13 @@ struct Layout {
     // input section (same offsets on host and device)
     size_t reads, read_off, read_len, s_read, s_start, s_len, s_limit;
     size_t w_roff, w_rlen, w_foff, w_flen, w_off, w_band, w_abort, w_skip, tables, in_bytes;
-    // output section (offsets relative to the output section)
-    size_t o_shift, o_best, o_second, o_flags, o_res, o_cigar, out_bytes;
-    // device-only scratch (relative to the scratch section)
-    size_t planes, valid, slabs, scratch_bytes;
+    // output section (offsets relative to the output section); the packed
+    // CIGARs (o_cigar) follow it in host memory only: the device writes them
+    // there itself, so a copy moves only the ops the jobs produced
+    size_t o_shift, o_best, o_second, o_flags, o_res, o_coff, out_bytes, o_cigar, host_out_bytes;
+    // device-only scratch (relative to the scratch section): read planes,
+    // the trace slabs and the plan's CIGAR arena (worst case per job)
+    size_t planes, valid, slabs, arena, scratch_bytes;
 };
 
 struct Request {
@@ -136,6 +139,7 @@ struct kira_server {
     std::vector<hipEvent_t> events;
     std::vector<std::unique_ptr<Request>> slot_req;  // null = slot free
     uint8_t *dpool = nullptr, *hpool = nullptr;
+    uint8_t *hpool_dev = nullptr;  // hpool as the device addresses it
     RangePool dalloc, halloc;
     std::thread worker;
     std::mutex mu;
@@ -193,15 +197,18 @@ bool kira_server::layout(Request &r, std::string &why) {
     L.o_second = sec((size_t)r.ns * 4);
     L.o_flags = sec((size_t)r.ns);
     L.o_res = sec((size_t)r.nw * kira::kSwResWords * 4);
-    L.o_cigar = sec((size_t)r.plan.cigar_words * 4);
+    L.o_coff = sec((size_t)r.nw * 8);
     L.out_bytes = o;
+    L.o_cigar = sec((size_t)r.plan.cigar_words * 4);
+    L.host_out_bytes = o;
     o = 0;
     L.planes = sec((size_t)r.nr * kira::kReadWords * 4);
     L.valid = sec((size_t)r.nr);
     L.slabs = sec(kira::sw_plan_slab_bytes(r.plan));
+    L.arena = sec((size_t)r.plan.cigar_words * 4);
     L.scratch_bytes = o;
     r.dev_bytes = L.in_bytes + L.out_bytes + L.scratch_bytes;
-    r.host_bytes = L.in_bytes + L.out_bytes;
+    r.host_bytes = L.in_bytes + L.host_out_bytes;
     r.laid_out = true;
     return true;
 }
@@ -210,6 +217,7 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
     const Layout &L = r.lay;
     uint8_t *din = dpool + r.dev_off, *dout = din + L.in_bytes, *dscr = dout + L.out_bytes;
     uint8_t *hin = hpool + r.host_off, *hout = hin + L.in_bytes;
+    uint8_t *hout_dev = hpool_dev + r.host_off + L.in_bytes;
     hipError_t e = hipMemcpyAsync(din, hin, L.in_bytes, hipMemcpyHostToDevice, s);
     if (e) return e;
     uint32_t *planes = (uint32_t *)(dscr + L.planes);
@@ -260,11 +268,16 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
         j.band = (const int32_t *)(din + L.w_band);
         j.abort_score = (const int32_t *)(din + L.w_abort);
         j.res = (int32_t *)(dout + L.o_res);
-        j.cigar = (uint32_t *)(dout + L.o_cigar);
+        j.cigar = (uint32_t *)(dscr + L.arena);
         const char *why = kira::sw_plan_launch(r.plan, d, j, cfg.cfg, cfg.knobs,
                                                r.ns > 0 ? o_flags : nullptr,
                                                (const int32_t *)(din + L.w_skip), s, &e);
         if (why) return e ? e : hipErrorInvalidValue;
+        // the ops, packed, straight into the pinned host buffer
+        if ((e = kira::launch_cigar_compact(r.nw, j.res, j.cigar, d.cigar_off,
+                                            (int64_t *)(dout + L.o_coff),
+                                            (uint32_t *)(hout_dev + L.o_cigar), s)))
+            return e;
     }
     return hipMemcpyAsync(hout, dout, L.out_bytes, hipMemcpyDeviceToHost, s);
 }
@@ -347,6 +360,7 @@ void kira_server::finish(int slot, hipError_t e) {
         get32(res->second, L.o_second, r.ns);
         res->flags.assign(hout + L.o_flags, hout + L.o_flags + r.ns);
         const int32_t *rr = (const int32_t *)(hout + L.o_res);
+        const int64_t *coff = (const int64_t *)(hout + L.o_coff);
         const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
         res->sw.resize(r.nw);
         res->skipped.resize(r.nw);
@@ -369,7 +383,7 @@ void kira_server::finish(int slot, hipError_t e) {
                 o.n_cigar = 0;
                 continue;
             }
-            const uint32_t *src = cg + r.plan.cigar_off[k];
+            const uint32_t *src = cg + coff[k];
             res->cigar.insert(res->cigar.end(), src, src + q[6]);
         }
     }
@@ -488,6 +502,7 @@ kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g, const kira_s
     hipError_t e = hipSetDevice(ctx->device);
     if (!e) e = hipMalloc((void **)&s->dpool, (size_t)c->hbm_budget_bytes);
     if (!e) e = hipHostMalloc((void **)&s->hpool, (size_t)c->host_staging_bytes, 0);
+    if (!e) e = hipHostGetDevicePointer((void **)&s->hpool_dev, s->hpool, 0);
     for (int k = 0; !e && k < c->slots; k++) {
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;

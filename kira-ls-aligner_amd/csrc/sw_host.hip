@@ -114,10 +114,58 @@ __global__ void compact_cigar_kernel(int32_t n, const int32_t *__restrict__ res,
     for (int32_t i = 0; i < c; i++) b[i] = a[i];
 }
 
+// Exclusive prefix sum of the jobs' CIGAR op counts (negative = overflow,
+// counted as 0) in one workgroup: each thread sums a contiguous range, thread
+// 0 scans the 1024 partial sums, each thread writes its range's offsets.
+__global__ __launch_bounds__(1024) void cigar_offsets_kernel(int32_t n,
+                                                             const int32_t *__restrict__ res,
+                                                             int64_t *__restrict__ off) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = ((int64_t)n + 1023) / 1024, lo = t * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    int64_t sum = 0;
+    for (int64_t k = lo; k < hi; k++) {
+        const int32_t c = res[k * kira::kSwResWords + 6];
+        sum += c > 0 ? c : 0;
+    }
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int64_t acc = 0;
+        for (int i = 0; i < 1024; i++) {
+            const int64_t v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    int64_t acc = part[t];
+    for (int64_t k = lo; k < hi; k++) {
+        off[k] = acc;
+        const int32_t c = res[k * kira::kSwResWords + 6];
+        acc += c > 0 ? c : 0;
+    }
+}
+
 // device allocations released together
 using kira::DevMem;
 
 }  // namespace
+
+namespace kira {
+
+hipError_t launch_cigar_compact(int32_t n, const int32_t *res, const uint32_t *arena,
+                                const int64_t *arena_off, int64_t *dst_off, uint32_t *dst,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    cigar_offsets_kernel<<<1, 1024, 0, s>>>(n, res, dst_off);
+    compact_cigar_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, res, arena, arena_off,
+                                                                     dst_off, dst);
+    return hipGetLastError();
+}
+
+}  // namespace kira
 
 namespace kira {
 
