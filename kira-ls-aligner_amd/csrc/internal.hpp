@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstring>
 #include <new>
 #include <string>
 #include <thread>
@@ -439,6 +440,18 @@ void parallel_chunks(int64_t n, int64_t grain, F &&f) {
     for (auto &x : th) x.join();
     for (int t = 0; t < T; t++)
         if (failed[t]) throw std::bad_alloc();
+}
+
+// memcpy split over the host threads when it is large (pinned staging of
+// tens of MB per request)
+inline void parallel_memcpy(void *dst, const void *src, size_t bytes) {
+    if (bytes < ((size_t)8 << 20)) {
+        if (bytes) std::memcpy(dst, src, bytes);
+        return;
+    }
+    parallel_chunks((int64_t)bytes, (int64_t)4 << 20, [&](int, int64_t b, int64_t e) {
+        std::memcpy((char *)dst + b, (const char *)src + b, (size_t)(e - b));
+    });
 }
 
 // Myers / WFA on device arrays, shared by the batch entry points and the fast

@@ -288,7 +288,7 @@ bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why
     // stage every input in pinned memory: one copy per request
     uint8_t *hin = hpool + r.host_off;
     auto put = [&](size_t off, const void *src, size_t bytes) {
-        if (bytes) memcpy(hin + off, src, bytes);
+        kira::parallel_memcpy(hin + off, src, bytes);
     };
     put(L.reads, r.reads.data(), r.reads.size());
     put(L.read_off, r.read_off.data(), (size_t)r.nr * 8);
@@ -364,27 +364,42 @@ void kira_server::finish(int slot, hipError_t e) {
         const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
         res->sw.resize(r.nw);
         res->skipped.resize(r.nw);
-        for (int32_t k = 0; k < r.nw; k++) {
-            const int32_t *q = rr + (size_t)k * kira::kSwResWords;
-            kira_sw_result &o = res->sw[k];
-            o.score = q[0];
-            o.read_end = q[1];
-            o.ref_end = (uint32_t)q[2];
-            o.early_abort = q[3];
-            o.ref_start = (uint32_t)q[4];
-            o.read_start = q[5];
-            o.n_cigar = q[6];
-            o.cells = q[8];
-            o.cigar_off = (int64_t)res->cigar.size();
-            res->skipped[k] = (uint8_t)(q[9] == 1);
-            if (q[6] < 0 || q[7] != 0) {
-                res->status = KIRA_ERR_RUNTIME;
-                res->err = q[6] < 0 ? "CIGAR capacity exceeded" : "int16 band overflow";
-                o.n_cigar = 0;
-                continue;
+        // the device packed every job's ops in job order at coff[k]: one copy
+        // of the packed words, then the per-job fields in parallel
+        int64_t words = 0;
+        if (r.nw > 0) {
+            const int32_t last = rr[(size_t)(r.nw - 1) * kira::kSwResWords + 6];
+            words = coff[r.nw - 1] + (last > 0 ? last : 0);
+        }
+        res->cigar.resize((size_t)words);
+        kira::parallel_memcpy(res->cigar.data(), cg, (size_t)words * 4);
+        std::vector<int32_t> bad(kira::host_threads(), -1);  // last failing job per thread
+        kira::parallel_chunks(r.nw, 4096, [&](int t, int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; k++) {
+                const int32_t *q = rr + (size_t)k * kira::kSwResWords;
+                kira_sw_result &o = res->sw[k];
+                o.score = q[0];
+                o.read_end = q[1];
+                o.ref_end = (uint32_t)q[2];
+                o.early_abort = q[3];
+                o.ref_start = (uint32_t)q[4];
+                o.read_start = q[5];
+                o.n_cigar = q[6];
+                o.cells = q[8];
+                o.cigar_off = coff[k];
+                res->skipped[k] = (uint8_t)(q[9] == 1);
+                if (q[6] < 0 || q[7] != 0) {
+                    o.n_cigar = 0;
+                    bad[t] = (int32_t)k;
+                }
             }
-            const uint32_t *src = cg + coff[k];
-            res->cigar.insert(res->cigar.end(), src, src + q[6]);
+        });
+        int32_t worst = -1;
+        for (int32_t k : bad) worst = std::max(worst, k);
+        if (worst >= 0) {
+            const int32_t *q = rr + (size_t)worst * kira::kSwResWords;
+            res->status = KIRA_ERR_RUNTIME;
+            res->err = q[6] < 0 ? "CIGAR capacity exceeded" : "int16 band overflow";
         }
     }
     if (r.exec) (void)hipGraphExecDestroy(r.exec);
@@ -546,22 +561,23 @@ kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *t
     r->nr = q->n_reads;
     r->ns = q->n_sieve;
     r->nw = q->n_sw;
-    int64_t total = 0;
-    for (int32_t k = 0; k < r->nr; k++) {
+    for (int32_t k = 0; k < r->nr; k++)
         if (q->read_off[k] < 0 || q->read_len[k] < 0) return bad("negative read offset / length");
-        total = std::max<int64_t>(total, q->read_off[k] + q->read_len[k]);
-    }
     // reads are re-based to one contiguous copy
-    r->reads.reserve((size_t)std::max<int64_t>(total, 1));
     r->read_off.resize(r->nr);
     r->read_len.assign(q->read_len, q->read_len + r->nr);
     int64_t pos = 0;
     for (int32_t k = 0; k < r->nr; k++) {
         r->read_off[k] = pos;
-        r->reads.insert(r->reads.end(), q->reads + q->read_off[k],
-                        q->reads + q->read_off[k] + q->read_len[k]);
         pos += q->read_len[k];
     }
+    r->reads.resize((size_t)pos);
+    kira::parallel_chunks(r->nr, 2048, [&](int, int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++)
+            if (q->read_len[k])
+                memcpy(r->reads.data() + r->read_off[k], q->reads + q->read_off[k],
+                       (size_t)q->read_len[k]);
+    });
     r->s_read.assign(q->sieve_read, q->sieve_read + r->ns);
     r->s_start.assign(q->sieve_win_start, q->sieve_win_start + r->ns);
     r->s_len.assign(q->sieve_win_len, q->sieve_win_len + r->ns);
@@ -596,6 +612,13 @@ kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *t
         if (r->w_skip[k] >= r->ns) return bad("SW skip index out of range");
     }
     r->t_submit = now_s();
+    // the job plan and buffer layout on the submitting thread, so the worker
+    // (which stages, launches and collects) does not serialise them; a
+    // request that cannot be laid out is reported by the worker
+    {
+        std::string why;
+        (void)s->layout(*r, why);
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return KIRA_ERR_INVALID_ARGUMENT;
     r->ticket = s->next_ticket++;
