@@ -327,7 +327,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             a4 = (m * 4 + 255) & ~(size_t)255;
             bytes = 2 * a8 + 4 * a4;
             void *p = nullptr;
-            hipError_t r = kira::pinned_buf(cx, 3, bytes, &p);
+            hipError_t r = kira::pinned_buf(cx, kira::kPinFast, bytes, &p);
             if (r != hipSuccess) return r;
             base = (uint8_t *)p;
             roff = (int64_t *)base;
@@ -381,6 +381,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 max_lt = std::max(max_lt, j.L + j.tl);
             }
             const int32_t m = (int32_t)lists.n;
+            trace_mark(".fp_sp_lists");
             if (m) {
                 const size_t lds = (size_t)max_lt + 16;
                 if (lds > 65536)
@@ -392,11 +393,13 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 hipLaunchKernelGGL(spectral_kernel, dim3(m), dim3(64), lds, st, m, d_reads, d.roff,
                                    d.rlen, g->ascii, d.toff, d.tlen, kn.spectral_pct,
                                    kn.spectral_floor, kn.cert_strict, d_out);
+                trace_mark(".fp_sp_launch");
                 std::vector<int32_t> so(3 * (size_t)m);
                 if ((e = hipGetLastError()) ||
                     (e = hipMemcpyAsync(so.data(), d_out, 4 * so.size(), hipMemcpyDeviceToHost, st)) ||
                     (e = hipStreamSynchronize(st)))
                     return fail(ctx, KIRA_ERR_RUNTIME, "fast path spectral", e);
+                trace_mark(".fp_sp_sync");
                 for (int32_t q = 0; q < m; q++)
                     if (so[3 * q]) {
                         jobs[q].stage = 1;
@@ -408,6 +411,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         for (int32_t q = 0; q < (int32_t)jobs.size(); q++)
             if (jobs[q].stage == 0) cand.push_back(q);
         st_local.phase_ms[0] = ms_since(t_sp);
+        trace_mark(".fp_spectral");
         auto t_my = std::chrono::steady_clock::now();
         // ---- Myers reject (router.rs:196-202)
         std::vector<int32_t> wfa_jobs;
@@ -438,6 +442,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             st_local.myers_pass = (int64_t)wfa_jobs.size();
         }
         st_local.phase_ms[1] = ms_since(t_my);
+        trace_mark(".fp_myers");
         // ---- scored WFA (wfa_score_budget, router.rs:160-166)
         auto t_wfa = std::chrono::steady_clock::now();
         const int32_t drop = kn.wfa_adaptive > 0 ? kn.wfa_adaptive : 0;
@@ -579,6 +584,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             }
         }
         st_local.phase_ms[2] = ms_since(t_wfa);
+        trace_mark(".fp_wfa");
         // ---- records (host): partial-read check, wfa_result_to_alignment
         auto t_rec = std::chrono::steady_clock::now();
         const int T = host_threads();
@@ -696,6 +702,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             md_out += parts[t].md;
         }
         st_local.phase_ms[3] = ms_since(t_rec);
+        trace_mark(".fp_records");
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }

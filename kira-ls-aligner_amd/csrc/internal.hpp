@@ -5,6 +5,13 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -257,6 +264,45 @@ struct GroupLaunch {
 hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,
                            hipStream_t s);
 
+// Device blocks cached on the context between calls, so an entry point's
+// temporaries cost no hipMalloc / hipFree once warm: sizes round up to a
+// quarter of their power of two (at least 256 B), a freed block waits in its
+// size's list for the next request of that size.  Blocks above 1 GiB are not
+// cached.  One call at a time per context (as for every context state).
+hipError_t cached_alloc(kira_ctx *ctx, void **p, size_t bytes, size_t *size_class);
+void cached_free(kira_ctx *ctx, void *p, size_t size_class);
+// Grow-only pinned host buffer `slot` (PinSlot) of the context, kept until
+// kira_release_buffers: staging for the large host-device copies, which run
+// at full PCIe rate only from pinned memory.  Its contents are not kept when
+// it grows.
+hipError_t pinned_buf(kira_ctx *ctx, int slot, size_t bytes, void **out);
+
+// The device allocations of one scope, back to the context's cache (or freed,
+// without a context) when the scope ends.  A cached block may be handed to the
+// next call at once, possibly on another stream, so the scope first waits for
+// the context stream: an early error return can leave a copy or kernel that
+// uses the block queued (hipFree, the uncached case, waits by itself).
+struct DevMem {
+    kira_ctx *ctx = nullptr;
+    std::vector<std::pair<void *, size_t>> v;
+    DevMem() = default;
+    explicit DevMem(kira_ctx *c) : ctx(c) {}
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem();
+    template <class T>
+    hipError_t alloc(T **p, size_t count) {
+        void *q = nullptr;
+        size_t cls = 0;
+        hipError_t e = cached_alloc(ctx, &q, std::max<size_t>(count * sizeof(T), 1), &cls);
+        if (e == hipSuccess) {
+            *p = (T *)q;
+            v.emplace_back(q, cls);
+        }
+        return e;
+    }
+};
+
 // SW batches over device-resident sequences with host-side job tables: job
 // k's read is d_reads + read_off[k] (read_len[k] bytes, 8 readable bytes
 // past the arena end), its window d_refs + ref_off[k] (ref_len[k] bytes;
@@ -275,13 +321,34 @@ struct SwJobsHost {
     const int32_t *abort_score;
 };
 
-// Downloads n results (kSwResWords each) and their CIGARs, which the kernels
-// left at src_off[k] of d_cig: the CIGARs are packed back to back on the
-// device first and appended to `cigar`; out[k].cigar_off indexes `cigar`.
-kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, const uint32_t *d_cig,
-                             const int64_t *src_off, bool banded, kira_sw_result *out,
-                             std::vector<uint32_t> &cigar);
-// banded_sw_with (src/alignment/mod.rs:1370-1650) over a job table.
+// Reads back n results (kSwResWords each) and their CIGARs, which the
+// kernels left at d_src_off[k] (device array) of d_cig: a device prefix sum
+// and gather pack the CIGARs back to back into scratch from dm first, and
+// only the ops cross PCIe; they are appended to `cigar` and out[k].cigar_off
+// indexes it.  Waits for the context's stream.
+kira_status fetch_sw_results(kira_ctx *ctx, DevMem &dm, int32_t n, const int32_t *d_res,
+                             const uint32_t *d_cig, const int64_t *d_src_off, int64_t cig_words,
+                             bool banded, kira_sw_result *out, std::vector<uint32_t> &cigar);
+
+// A banded SW batch (banded_sw_with, src/alignment/mod.rs:1370-1650) over
+// device-resident reads and windows, in two steps: launch() lays the plan
+// out on the host, uploads its tables with the job arrays in one copy from
+// pinned buffer `slot` and queues the kernels on the context's stream, then
+// returns; fetch() waits for them and reads the results back.  Device memory
+// comes from the context's block cache and is kept until the object goes.
+struct BandedRun {
+    kira_ctx *ctx;
+    DevMem dm;
+    SwPlanLayout L;
+    int32_t n = 0;
+    int32_t *d_res = nullptr;
+    uint32_t *d_cig = nullptr;
+    const int64_t *d_cig_off = nullptr;
+    explicit BandedRun(kira_ctx *c) : ctx(c), dm(c) {}
+    kira_status launch(const SwJobsHost &j, const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
+                       int slot);
+    kira_status fetch(kira_sw_result *out, std::vector<uint32_t> &cigar);
+};
 kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg &cfg,
                           const kira_sw_knobs &kn, kira_sw_result *out,
                           std::vector<uint32_t> &cigar);
@@ -294,7 +361,21 @@ void simd_groups(int32_t n_batches, const int32_t *batch_off, const uint32_t *re
                  const kira_sw_cfg &cfg, std::vector<int32_t> &group_of,
                  std::vector<int32_t> &group_neg);
 // Full-window lane groups (sw_batch_avx2 / sw_batch_int8 / sw_batch_neon)
-// over a job table whose every job is in group group_of[k].
+// over a job table whose every job is in group group_of[k], in the same two
+// steps as BandedRun.
+struct GroupRun {
+    kira_ctx *ctx;
+    DevMem dm;
+    int32_t n = 0;
+    int64_t cig_words = 0;
+    int32_t *d_res = nullptr;
+    uint32_t *d_cig = nullptr;
+    const int64_t *d_cig_off = nullptr;
+    explicit GroupRun(kira_ctx *c) : ctx(c), dm(c) {}
+    kira_status launch(const SwJobsHost &j, const int32_t *group_of,
+                       const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg, int slot);
+    kira_status fetch(kira_sw_result *out, std::vector<uint32_t> &cigar);
+};
 kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *group_of,
                          const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg,
                          kira_sw_result *out, std::vector<uint32_t> &cigar);
@@ -351,48 +432,21 @@ uint32_t nm_md(const uint8_t *read, int32_t L, const uint8_t *ref, int64_t ref_l
 bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_len,
                     uint32_t ref_start, const uint8_t *read, int64_t read_len);
 
-// Device blocks cached on the context between calls, so an entry point's
-// temporaries cost no hipMalloc / hipFree once warm: sizes round up to a
-// quarter of their power of two (at least 256 B), a freed block waits in its
-// size's list for the next request of that size.  Blocks above 1 GiB are not
-// cached.  One call at a time per context (as for every context state).
-hipError_t cached_alloc(kira_ctx *ctx, void **p, size_t bytes, size_t *size_class);
-void cached_free(kira_ctx *ctx, void *p, size_t size_class);
-// Grow-only pinned host buffer `slot` (0-3) of the context, kept until
-// kira_release_buffers: staging for the large host-device copies, which run
-// at full PCIe rate only from pinned memory.  Its contents are not kept when
-// it grows.
-hipError_t pinned_buf(kira_ctx *ctx, int slot, size_t bytes, void **out);
-
-// The device allocations of one scope, back to the context's cache (or freed,
-// without a context) when the scope ends.  A cached block may be handed to the
-// next call at once, possibly on another stream, so the scope first waits for
-// the context stream: an early error return can leave a copy or kernel that
-// uses the block queued (hipFree, the uncached case, waits by itself).
-struct DevMem {
-    kira_ctx *ctx = nullptr;
-    std::vector<std::pair<void *, size_t>> v;
-    DevMem() = default;
-    explicit DevMem(kira_ctx *c) : ctx(c) {}
-    DevMem(const DevMem &) = delete;
-    DevMem &operator=(const DevMem &) = delete;
-    ~DevMem();
-    template <class T>
-    hipError_t alloc(T **p, size_t count) {
-        void *q = nullptr;
-        size_t cls = 0;
-        hipError_t e = cached_alloc(ctx, &q, std::max<size_t>(count * sizeof(T), 1), &cls);
-        if (e == hipSuccess) {
-            *p = (T *)q;
-            v.emplace_back(q, cls);
-        }
-        return e;
-    }
+// Pinned staging buffers of a context, one per use, so that uploads that are
+// in flight at the same time never share one.
+enum PinSlot {
+    kPinOrient = 0,   // the stage-4 oriented reads
+    kPinPfOut = 1,    // stage-4 prefilter outcomes
+    kPinPfJobs = 2,   // stage-4 prefilter job arrays
+    kPinFast = 3,     // the fast DP cascade's job lists
+    kPinGroup = 4,    // lane-group SW job arrays and packet tables
+    kPinBanded = 5,   // banded SW job arrays and plan tables
+    kPinBanded2 = 6,  // a second banded SW batch in flight with the first
+    kPinSieve = 7,    // stage-4 sieve job arrays
+    kPinSlots = 8
 };
 
-// A typed view of pinned staging buffer `slot` of the context.  Slots in
-// use: 0 the stage-4 oriented reads, 1 its prefilter outcomes, 2 its
-// prefilter job arrays, 3 the fast DP cascade's job lists.
+// A typed view of pinned staging buffer `slot` (PinSlot) of the context.
 template <class T>
 struct Pinned {
     T *p = nullptr;
@@ -417,8 +471,99 @@ inline int host_threads() {
     return (int)std::max(1u, std::min(h ? h : 1u, 16u));
 }
 
+// Persistent host workers (host_threads() - 1 of them, started on first
+// use) for parallel_chunks: starting and joining 16 threads per call cost
+// about a millisecond, and a stage-4 batch makes a few dozen such calls.
+// One call uses the pool at a time; run() returns false when it is busy
+// (a call on another thread, or a nested call), and the caller then starts
+// threads of its own.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool *p = new HostPool();  // never destroyed: workers idle until exit
+        return *p;
+    }
+    // task(t) for every t in [0, n); the calling thread takes part.
+    template <class F>
+    bool run(int n, F &&task) {
+        bool idle = false;
+        if (!busy_.compare_exchange_strong(idle, true)) return false;
+        start_workers();
+        const std::function<void(int)> fn = std::ref(task);
+        Job j;
+        j.task = &fn;
+        j.n = n;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &j;
+            gen_++;
+        }
+        cv_.notify_all();
+        const int ran = drain(j);
+        std::unique_lock<std::mutex> lk(m_);
+        j.done += ran;
+        done_.wait(lk, [&] { return j.done == j.n && j.inside == 0; });
+        job_ = nullptr;
+        busy_.store(false);
+        return true;
+    }
+
+  private:
+    struct Job {
+        const std::function<void(int)> *task = nullptr;
+        int n = 0;
+        std::atomic<int> next{0};
+        int inside = 0, done = 0;  // guarded by m_
+    };
+    std::atomic<bool> busy_{false};
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    Job *job_ = nullptr;
+    std::atomic<uint64_t> gen_{0};  // written under m_
+    bool started_ = false;
+
+    static int drain(Job &j) {
+        int ran = 0;
+        for (;;) {
+            const int t = j.next.fetch_add(1);
+            if (t >= j.n) return ran;
+            (*j.task)(t);
+            ran++;
+        }
+    }
+    void start_workers() {
+        if (started_) return;
+        started_ = true;
+        for (int w = 1; w < host_threads(); w++) std::thread([this] { worker(); }).detach();
+    }
+    void worker() {
+        uint64_t seen = 0;
+        for (;;) {
+            // calls come in bursts: spin a little before sleeping, so the
+            // next call finds its workers awake
+            const auto t_spin = std::chrono::steady_clock::now();
+            while (gen_.load(std::memory_order_acquire) == seen &&
+                   std::chrono::steady_clock::now() - t_spin < std::chrono::microseconds(200))
+                __builtin_ia32_pause();
+            Job *j;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return job_ != nullptr && gen_.load() != seen; });
+                seen = gen_.load();
+                j = job_;
+                j->inside++;
+            }
+            const int ran = drain(*j);
+            std::lock_guard<std::mutex> lk(m_);
+            j->done += ran;
+            if (--j->inside == 0 && j->done == j->n) done_.notify_all();
+        }
+    }
+};
+
 // f(t, begin, end) on up to host_threads() threads over [0, n) in chunks of
-// at least `grain`; a std::bad_alloc in any chunk is rethrown here.
+// at least `grain`; chunk t is [n t / T, n (t + 1) / T), so chunks follow
+// item order.  A std::bad_alloc in any chunk is rethrown here.
 template <class F>
 void parallel_chunks(int64_t n, int64_t grain, F &&f) {
     if (n <= 0) return;
@@ -427,19 +572,83 @@ void parallel_chunks(int64_t n, int64_t grain, F &&f) {
         f(0, (int64_t)0, n);
         return;
     }
-    std::vector<std::thread> th;
     std::vector<char> failed(T, 0);
-    for (int t = 0; t < T; t++)
-        th.emplace_back([&, t]() {
-            try {
-                f(t, n * t / T, n * (t + 1) / T);
-            } catch (...) {
-                failed[t] = 1;
-            }
-        });
-    for (auto &x : th) x.join();
+    auto chunk = [&](int t) {
+        try {
+            f(t, n * t / T, n * (t + 1) / T);
+        } catch (...) {
+            failed[t] = 1;
+        }
+    };
+    if (!HostPool::get().run(T, chunk)) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; t++) th.emplace_back(chunk, t);
+        chunk(0);
+        for (auto &x : th) x.join();
+    }
     for (int t = 0; t < T; t++)
         if (failed[t]) throw std::bad_alloc();
+}
+
+// Wall-clock marks of the steps of one entry-point call, printed to stderr
+// when the environment variable KIRA_TRACE is set (a diagnostic: which host
+// step or synchronisation a call's time goes to).  A StepTrace on the stack
+// collects the marks trace_mark() makes on its thread while it lives.
+struct StepTrace {
+    using Clock = std::chrono::steady_clock;
+    const char *name;
+    bool on;
+    Clock::time_point t0, last;
+    std::vector<std::pair<const char *, double>> marks;
+    StepTrace *outer;
+    static StepTrace *&active() {
+        static thread_local StepTrace *a = nullptr;
+        return a;
+    }
+    explicit StepTrace(const char *n) : name(n), on(std::getenv("KIRA_TRACE") != nullptr) {
+        t0 = last = Clock::now();
+        outer = active();
+        if (on) active() = this;
+    }
+    void mark(const char *what) {
+        const Clock::time_point now = Clock::now();
+        marks.emplace_back(what, std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    }
+    ~StepTrace() {
+        if (!on) return;
+        active() = outer;
+        std::string o = std::string("[kira trace] ") + name + ": total " +
+                        std::to_string(std::chrono::duration<double, std::milli>(last - t0).count()) +
+                        " ms;";
+        for (auto &m : marks) o += std::string(" ") + m.first + " " + std::to_string(m.second);
+        std::fprintf(stderr, "%s\n", o.c_str());
+    }
+};
+inline void trace_mark(const char *what) {
+    if (StepTrace *t = StepTrace::active()) t->mark(what);
+}
+
+// Indices 0 .. n-1 ordered by key[i] ascending, equal keys in index order
+// (what std::stable_sort of the indices by key gives), by two stable
+// counting passes over 16-bit digits.
+inline void stable_order_by_key(const uint32_t *key, int32_t n, std::vector<int32_t> &order) {
+    order.resize((size_t)n);
+    if (n <= 0) return;
+    std::vector<int32_t> tmp((size_t)n);
+    std::vector<int32_t> cnt(65537);
+    for (int pass = 0; pass < 2; pass++) {
+        const int sh = 16 * pass;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        const int32_t *src = pass == 0 ? nullptr : tmp.data();
+        for (int32_t i = 0; i < n; i++) cnt[((key[src ? src[i] : i] >> sh) & 0xFFFFu) + 1]++;
+        for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+        int32_t *dst = pass == 0 ? tmp.data() : order.data();
+        for (int32_t i = 0; i < n; i++) {
+            const int32_t x = src ? src[i] : i;
+            dst[cnt[(key[x] >> sh) & 0xFFFFu]++] = x;
+        }
+    }
 }
 
 // memcpy split over the host threads when it is large (pinned staging of
@@ -511,9 +720,9 @@ struct kira_ctx {
     size_t cap[16] = {};
     // free cached device blocks by size class (kira::cached_alloc)
     std::unordered_map<size_t, std::vector<void *>> free_blocks;
-    // pinned host staging buffers (kira::pinned_buf)
-    void *pin[4] = {};
-    size_t pin_cap[4] = {};
+    // pinned host staging buffers (kira::pinned_buf, kira::PinSlot)
+    void *pin[kira::kPinSlots] = {};
+    size_t pin_cap[kira::kPinSlots] = {};
     // banded SW routes chosen by the autotuner (kira_sw_autotune / kira_sw_tune_load)
     kira::SwTuneTable tune;
     // streams the banded SW kernel classes of one plan launch fan out to

@@ -123,7 +123,7 @@ void kira_release_buffers(kira_ctx *ctx) {
     for (auto &kv : ctx->free_blocks)
         for (void *p : kv.second) (void)hipFree(p);
     ctx->free_blocks.clear();
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kira::kPinSlots; k++) {
         if (ctx->pin[k]) (void)hipHostFree(ctx->pin[k]);
         ctx->pin[k] = nullptr;
         ctx->pin_cap[k] = 0;

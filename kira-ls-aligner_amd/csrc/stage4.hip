@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -132,25 +133,21 @@ uint32_t read_overlap_pct(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
     return (uint32_t)std::min<uint64_t>(ov, 0xFFFFFFFFu) / al;
 }
 
-struct Rec {
-    int32_t read;
-    kira_stage4_aln a;
-};
-
-// Records made off the main thread: CIGAR / MD offsets are relative to the
-// arena until it is merged into the result.
+// The records one thread makes in one build() call, in item order: CIGAR /
+// MD offsets are relative to the arena until the result is assembled.
 struct Arena {
+    std::vector<int32_t> read;  // read of each record
+    std::vector<kira_stage4_aln> recs;
     std::vector<uint32_t> cig;
     std::string md;
-    std::vector<Rec> recs;
+    void add(int32_t r, const kira_stage4_aln &a) {
+        read.push_back(r);
+        recs.push_back(a);
+    }
 };
 
-// Room for `extra` more elements, growing geometrically.
-template <class V>
-void grow_for(V &v, size_t extra) {
-    const size_t need = v.size() + extra;
-    if (need > v.capacity()) v.reserve(std::max(need, v.capacity() * 2));
-}
+// A record's place: arena index << 32 | index in the arena.
+inline int64_t rec_handle(size_t arena, size_t k) { return ((int64_t)arena << 32) | (int64_t)k; }
 
 using kira::host_threads;
 using kira::parallel_chunks;
@@ -192,7 +189,9 @@ struct Driver {
     std::vector<int64_t> ooff;  // read k: forward at ooff[k], reverse at ooff[k] + L
     uint8_t *d_orient = nullptr;
     DevMem mem;
-    std::vector<Rec> recs;
+    // every record so far, arena by arena in record order (never moved: the
+    // result is assembled once, at the end)
+    std::vector<std::unique_ptr<Arena>> arenas;
     const int32_t *rmo = nullptr;  // ReadRecord::repeat_min_occ per read, may be NULL
 
     int32_t rmo_of(int32_t r) const { return rmo ? rmo[r] : 0; }
@@ -203,34 +202,26 @@ struct Driver {
     }
     const uint8_t *contig(uint32_t id) const { return g->host.data() + contig_off[id]; }
 
-    // Builds the records of items [0, n) in parallel (f(i, arena) appends
-    // zero or more) and appends them to `recs` in item order.
+    // Builds the records of items [0, n) in parallel (f(i, arena) adds zero
+    // or more) after every record so far, in item order: thread t's arena
+    // holds the t-th contiguous range of items.
     template <class F>
     void build(int64_t n, F &&f) {
         if (n <= 0) return;
-        std::vector<Arena> ar((size_t)host_threads());
+        const size_t a0 = arenas.size();
+        const int T = host_threads();
+        for (int t = 0; t < T; t++) arenas.emplace_back(new Arena());
         parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
-            for (int64_t i = b; i < e; i++) f(i, ar[t]);
+            Arena &A = *arenas[a0 + (size_t)t];
+            for (int64_t i = b; i < e; i++) f(i, A);
         });
-        size_t nr = 0, nc = 0, nm = 0;
-        for (const Arena &A : ar) {
-            nr += A.recs.size();
-            nc += A.cig.size();
-            nm += A.md.size();
-        }
-        grow_for(recs, nr);
-        grow_for(R.cigar, nc);
-        grow_for(R.md, nm);
-        for (Arena &A : ar) {
-            const int64_t bc = (int64_t)R.cigar.size(), bm = (int64_t)R.md.size();
-            R.cigar.insert(R.cigar.end(), A.cig.begin(), A.cig.end());
-            R.md += A.md;
-            for (Rec &x : A.recs) {
-                x.a.cigar_off += bc;
-                x.a.md_off += bm;
-                recs.push_back(x);
-            }
-        }
+    }
+
+    kira_stage4_aln &rec(int64_t h) {
+        return arenas[(size_t)(h >> 32)]->recs[(size_t)(h & 0xFFFFFFFF)];
+    }
+    const uint32_t *rec_cigar(int64_t h) {
+        return arenas[(size_t)(h >> 32)]->cig.data() + rec(h).cigar_off;
     }
 
     kira_stage4_aln base_aln(int32_t kind, const kira_chain &ch) const {
@@ -293,11 +284,11 @@ struct Driver {
     }
 
     // Banded SW over device-resident reads and windows (offset: each job's
-    // band centre offset); results and CIGARs in job order.
-    kira_status banded_run(const std::vector<Job> &jobs, const std::vector<int32_t> &offset,
-                           std::vector<kira_sw_result> &res, std::vector<uint32_t> &cig) {
+    // band centre offset), launched on the stream; run.fetch() returns the
+    // results and CIGARs in job order.
+    kira_status banded_launch(kira::BandedRun &run, const std::vector<Job> &jobs,
+                              const std::vector<int32_t> &offset, int slot) {
         const int32_t m = (int32_t)jobs.size();
-        res.resize(m);
         if (m == 0) return KIRA_OK;
         std::vector<int64_t> ro(m), fo(m);
         std::vector<int32_t> rl(m), fl(m), ab(m);
@@ -312,18 +303,22 @@ struct Driver {
         }
         kira::SwJobsHost sj{m,         d_orient,  ro.data(),     rl.data(), g->ascii,
                             fo.data(), fl.data(), offset.data(), nullptr,   ab.data()};
-        return kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, res.data(), cig);
+        return run.launch(sj, c.cfg, c.knobs, slot);
     }
 
     // Groups the records by read without moving them: read r's records are
-    // recs[perm[k]] for k in [first[r], first[r + 1]), in record order.
+    // rec(perm[k]) for k in [first[r], first[r + 1]), in record order.
     void group_by_read(std::vector<int64_t> &first, std::vector<int64_t> &perm) const {
         first.assign((size_t)n + 1, 0);
-        for (const Rec &x : recs) first[x.read + 1]++;
+        for (const auto &A : arenas)
+            for (int32_t r : A->read) first[r + 1]++;
         for (int32_t r = 0; r < n; r++) first[r + 1] += first[r];
         std::vector<int64_t> pos(first.begin(), first.end() - 1);
-        perm.resize(recs.size());
-        for (size_t k = 0; k < recs.size(); k++) perm[pos[recs[k].read]++] = (int64_t)k;
+        perm.resize((size_t)first[n]);
+        for (size_t a = 0; a < arenas.size(); a++) {
+            const std::vector<int32_t> &rd = arenas[a]->read;
+            for (size_t k = 0; k < rd.size(); k++) perm[pos[rd[k]]++] = rec_handle(a, k);
+        }
     }
 
     // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
@@ -361,6 +356,7 @@ kira_status Driver::run() {
                                                       : 1;
     const int32_t ambig_div = c.ambig_div >= 1 ? c.ambig_div : 5;
     auto t0 = Clock::now();
+    kira::StepTrace trace("stage4");
 
     // oriented reads
     int64_t total = 0;
@@ -370,7 +366,7 @@ kira_status Driver::run() {
         total += 2 * (int64_t)read_len[r];
     }
     hipError_t e;
-    if ((e = orient.resize(ctx, 0, (size_t)total)))
+    if ((e = orient.resize(ctx, kira::kPinOrient, (size_t)total)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned read staging", e);
     uint8_t comp[256];
     for (int b = 0; b < 256; b++) comp[b] = comp_base((uint8_t)b);
@@ -384,6 +380,7 @@ kira_status Driver::run() {
         }
     });
 
+    kira::trace_mark("orient");
     // the oriented reads go to the device once; every phase reads them there
     if ((e = mem.alloc(&d_orient, (size_t)total + 8)) ||
         (total && (e = hipMemcpyAsync(d_orient, orient.data(), (size_t)total,
@@ -391,6 +388,7 @@ kira_status Driver::run() {
         return fail(ctx, e == hipErrorOutOfMemory ? KIRA_ERR_OUT_OF_MEMORY : KIRA_ERR_RUNTIME,
                     "read upload", e);
 
+    kira::trace_mark("upload");
     // chain selection (:549-708, mate guidance off), in two passes: the
     // two-tier candidate search of uncertain short reads (:596-627) runs its
     // fast cascades for the whole batch at once, then each read selects
@@ -453,6 +451,7 @@ kira_status Driver::run() {
                     cand[(size_t)o++] = Cand{r, c0 + k, INT32_MIN, -1};
         }
     });
+    kira::trace_mark("select1");
     // exact copies score A * L (exact_match_exists, mod.rs:818-832); the rest
     // go through the fast cascade together
     std::vector<kira_fast_aln> tt_res;
@@ -486,6 +485,7 @@ kira_status Driver::run() {
             frl.push_back(L);
             fch.push_back(ch);
         }
+        kira::trace_mark("tt_prep");
         if (!fk.empty()) {
             kira_fast_stats fst;
             kira_status fs = kira::fast_dp_run(ctx, g, contig_off, contig_len, (int32_t)fk.size(),
@@ -500,6 +500,7 @@ kira_status Driver::run() {
                 }
         }
     }
+    kira::trace_mark("tt_cascade");
     // each read selects; threads take read ranges and their selections are
     // joined in read order
     std::vector<std::vector<Sel>> part((size_t)host_threads());
@@ -597,11 +598,12 @@ kira_status Driver::run() {
     }
     st.chains_used = (int64_t)sels.size();
 
+    kira::trace_mark("select2");
     // exact-match check and prefilter of every selected chain, on the device
     const int32_t ns = (int32_t)sels.size();
     std::vector<uint8_t> exact(ns, 0);
     kira::Pinned<kira_prefilter_out> pf;
-    if (ns > 0 && (e = pf.resize(ctx, 1, (size_t)ns)))
+    if (ns > 0 && (e = pf.resize(ctx, kira::kPinPfOut, (size_t)ns)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned prefilter staging", e);
     if (ns > 0) {
         // the job arrays side by side in one pinned block, uploaded with one copy
@@ -612,7 +614,7 @@ kira_status Driver::run() {
                      o_top = o_sp + al((size_t)ns * sizeof(kira_span)),
                      o_end = o_top + al((size_t)ns);
         kira::Pinned<uint8_t> hb;
-        if ((e = hb.resize(ctx, 2, o_end)))
+        if ((e = hb.resize(ctx, kira::kPinPfJobs, o_end)))
             return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned prefilter staging", e);
         int64_t *ro = (int64_t *)hb.data(), *fo = (int64_t *)(hb.data() + o_fo),
                 *fl = (int64_t *)(hb.data() + o_fl);
@@ -643,6 +645,7 @@ kira_status Driver::run() {
             ex_rl.push_back(rl[k]);
             ex_pos.push_back(fo[k] + chains[sels[k].chain].ref_start);
         }
+        kira::trace_mark("pf_prep");
         const int32_t ne = (int32_t)ex_sel.size();
         DevMem dm(ctx);
         uint8_t *db = nullptr;
@@ -707,6 +710,7 @@ kira_status Driver::run() {
             (ne && (e = hipMemcpyAsync(ex.data(), d_ex, ne, hipMemcpyDeviceToHost, s))) ||
             (e = hipStreamSynchronize(s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
+        kira::trace_mark("pf_gpu");
         for (int32_t k = 0; k < ne; k++) exact[ex_sel[k]] = ex[k];
         if (c.ungap_gap_check) {
             kira_status gs = kira::prefilter_gap_check(ctx, ns, d_orient, ro, rl, g->ascii, fo,
@@ -715,6 +719,7 @@ kira_status Driver::run() {
         }
     }
 
+    kira::trace_mark("pf_done");
     // outcomes in chain order (:758-934): 0 nothing, 1 a record here, 2 fallback
     std::vector<Job> simd, scalar;
     std::vector<uint8_t> dp_used(n, 0), outcome(ns, 0);
@@ -812,13 +817,18 @@ kira_status Driver::run() {
         }
         scalar.push_back(clamp_job(x.read, x.chain, abort));
     }
+    kira::trace_mark("route");
     for (int32_t r = 0; r < n; r++) st.dp_reads += dp_used[r];
+    R.phase_ms[0] = ms_since(t0);
+    // the records of the accepted chains are built later, while the device
+    // runs the SW kernels
+    auto build_accepted = [&]() {
     build((int64_t)accepted.size(), [&](int64_t i, Arena &A) {
         const int32_t k = accepted[i];
         const Sel &x = sels[k];
         const kira_chain &ch = chains[x.chain];
         if (exact[k]) {
-            A.recs.push_back(Rec{x.read, exact_aln(x.read, ch, A)});
+            A.add(x.read, exact_aln(x.read, ch, A));
             return;
         }
         if (fast_of[k] >= 0 || x.cached >= 0) {
@@ -854,7 +864,7 @@ kira_status Driver::run() {
                 A.md.append(fm, (size_t)f.md_off, (size_t)f.md_len);
                 a.md_len = f.md_len;
             }
-            A.recs.push_back(Rec{x.read, a});
+            A.add(x.read, a);
             return;
         }
         // build_ungapped_alignment (prefilter.rs:455-517)
@@ -874,14 +884,15 @@ kira_status Driver::run() {
                                    contig(ch.ref_id) + o.aln_ref_start,
                                    (int64_t)o.aln_read_end - o.aln_read_start, A.md);
         a.md_len = (int32_t)(A.md.size() - a.md_off);
-        A.recs.push_back(Rec{x.read, a});
+        A.add(x.read, a);
     });
-    R.phase_ms[0] = ms_since(t0);
+    };
 
     // gpu_dispatch_spectral (:1272-1358): sieve + certification
     t0 = Clock::now();
     std::vector<uint8_t> keep(simd.size(), 1);
     const int32_t nj = (int32_t)simd.size();
+    std::vector<int32_t> cert, sv_shift, sv_best;
     if (nj > 0 && !c.cpu_mode) {
         std::vector<int64_t> ro(nj), ws(nj);
         std::vector<int32_t> rl(nj), wl(nj), lim(nj);
@@ -936,28 +947,35 @@ kira_status Driver::run() {
         // the certification limit is the job's own dispatch limit
         b.cert_max_mism = INT32_MAX;
         b.cert_strict = c.cert_strict;
+        kira::trace_mark("sieve_prep");
         kira_status kst = kira_sieve_genome_dev(ctx, g, &b);
         if (kst != KIRA_OK) return kst;
-        std::vector<int32_t> shift(nj), best(nj);
+        std::vector<int32_t> &shift = sv_shift, &best = sv_best;
+        shift.resize(nj);
+        best.resize(nj);
         std::vector<uint8_t> flags(nj);
         if ((e = hipMemcpyAsync(shift.data(), d_shift, nj * 4, hipMemcpyDeviceToHost, s)) ||
             (e = hipMemcpyAsync(best.data(), d_best, nj * 4, hipMemcpyDeviceToHost, s)) ||
             (e = hipMemcpyAsync(flags.data(), d_flags, nj, hipMemcpyDeviceToHost, s)) ||
             (e = hipStreamSynchronize(s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
-        std::vector<int32_t> cert;
+        kira::trace_mark("sieve_gpu");
         for (int32_t t = 0; t < nj; t++)
             if (flags[t] & 4) {
                 cert.push_back(t);
                 keep[t] = 0;
             }
         st.gpu_spectral_resolved += (int64_t)cert.size();
+    }
+    R.phase_ms[1] = ms_since(t0);
+    auto build_certified = [&]() {
         build((int64_t)cert.size(), [&](int64_t i, Arena &A) {
             // build_certified_spectral_alignment (mod.rs:636-686)
             const int32_t t = cert[i];
             const Job &j = simd[t];
             const kira_chain &ch = chains[j.chain];
-            const int32_t L = rl[t], mm = best[t];
+            const int32_t L = read_len[j.read], mm = sv_best[t];
+            const int32_t *shift = sv_shift.data();
             kira_stage4_aln a = base_aln(1, ch);
             a.ref_start = (uint32_t)(j.win + shift[t]);
             a.ref_end = a.ref_start + (uint32_t)L;
@@ -972,46 +990,69 @@ kira_status Driver::run() {
             a.nm = kira::ungapped_tags(oriented(j.read, ch.strand == 1),
                                        contig(ch.ref_id) + j.win + shift[t], L, A.md);
             a.md_len = (int32_t)(A.md.size() - a.md_off);
-            A.recs.push_back(Rec{j.read, a});
+            A.add(j.read, a);
         });
-    }
-    R.phase_ms[1] = ms_since(t0);
+    };
 
     // align_batch_simd lane groups: BTreeMap over (read, window) length,
-    // chunks of `lanes` jobs (:1153-1186)
+    // chunks of `lanes` jobs (:1153-1186); the groups and the jobs outside
+    // any group are launched, then the scalar jobs, and the host builds the
+    // accepted and certified records while the device runs them
     t0 = Clock::now();
+    std::vector<int32_t> lane_order;  // lane jobs (indices into simd) in group order
+    kira::GroupRun lane_groups(ctx);
+    kira::BandedRun lane_banded(ctx), scalar_run(ctx);
+    std::vector<int32_t> lane_sel[2];  // positions in lane_order of each pass
     {
-        std::vector<int32_t> order;
+        std::vector<int32_t> kept;
         for (int32_t t = 0; t < nj; t++)
-            if (keep[t]) order.push_back(t);
-        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-            const int32_t lx = read_len[simd[x].read], ly = read_len[simd[y].read];
-            return lx != ly ? lx < ly : simd[x].wlen < simd[y].wlen;
-        });
-        const int32_t m = (int32_t)order.size();
+            if (keep[t]) kept.push_back(t);
+        const int32_t m = (int32_t)kept.size();
+        // stable order by (read length, window length)
+        bool fits = true;
+        std::vector<uint32_t> key(m);
+        for (int32_t i = 0; i < m; i++) {
+            const Job &j = simd[kept[i]];
+            fits &= read_len[j.read] <= 65535 && j.wlen <= 65535;
+            key[i] = ((uint32_t)read_len[j.read] << 16) | (uint32_t)(j.wlen & 0xFFFF);
+        }
+        std::vector<int32_t> ord;
+        if (fits) {
+            kira::stable_order_by_key(key.data(), m, ord);
+        } else {
+            ord.resize(m);
+            for (int32_t i = 0; i < m; i++) ord[i] = i;
+            std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+                const Job &a = simd[kept[x]], &b = simd[kept[y]];
+                const int32_t lx = read_len[a.read], ly = read_len[b.read];
+                return lx != ly ? lx < ly : a.wlen < b.wlen;
+            });
+        }
+        lane_order.resize(m);
+        for (int32_t i = 0; i < m; i++) lane_order[i] = kept[ord[i]];
         std::vector<int32_t> batch_off{0};
         for (int32_t i = 0; i < m;) {
             int32_t e = i;
-            while (e < m && read_len[simd[order[e]].read] == read_len[simd[order[i]].read] &&
-                   simd[order[e]].wlen == simd[order[i]].wlen)
+            const Job &ji = simd[lane_order[i]];
+            while (e < m && read_len[simd[lane_order[e]].read] == read_len[ji.read] &&
+                   simd[lane_order[e]].wlen == ji.wlen)
                 e++;
             for (int32_t b = i; b < e; b += lanes) batch_off.push_back(std::min(e, b + lanes));
             i = e;
         }
+        kira::trace_mark("lane_order");
         std::vector<uint32_t> rl(m), wl(m);
         for (int32_t i = 0; i < m; i++) {
-            rl[i] = (uint32_t)read_len[simd[order[i]].read];
-            wl[i] = (uint32_t)simd[order[i]].wlen;
+            rl[i] = (uint32_t)read_len[simd[lane_order[i]].read];
+            wl[i] = (uint32_t)simd[lane_order[i]].wlen;
         }
         std::vector<int32_t> group_of, group_neg;
         kira::simd_groups((int32_t)batch_off.size() - 1, batch_off.data(), rl.data(), wl.data(),
                           c.simd, c.simd, c.cfg, group_of, group_neg);
-        std::vector<kira_sw_result> out(m);
-        std::vector<uint32_t> cig;
         for (int pass = 0; pass < 2; pass++) {
             // pass 0: jobs outside any lane group run BatchInput::scalar_sw
             // (banded SW, offset clamped to [-L, W], mod.rs:60-72); pass 1: groups
-            std::vector<int32_t> sel;
+            std::vector<int32_t> &sel = lane_sel[pass];
             for (int32_t i = 0; i < m; i++)
                 if ((group_of[i] >= 0) == (pass == 1)) sel.push_back(i);
             const int32_t ms = (int32_t)sel.size();
@@ -1019,7 +1060,7 @@ kira_status Driver::run() {
             std::vector<int64_t> ro(ms), fo(ms);
             std::vector<int32_t> srl(ms), sfl(ms), off(ms), ab(ms), gof(ms);
             for (int32_t t = 0; t < ms; t++) {
-                const Job &j = simd[order[sel[t]]];
+                const Job &j = simd[lane_order[sel[t]]];
                 const kira_chain &ch = chains[j.chain];
                 ro[t] = oriented_off(j.read, ch.strand == 1);
                 srl[t] = read_len[j.read];
@@ -1033,43 +1074,80 @@ kira_status Driver::run() {
             }
             kira::SwJobsHost sj{ms,        d_orient,   ro.data(),  srl.data(), g->ascii,
                                 fo.data(), sfl.data(), off.data(), nullptr,    ab.data()};
-            std::vector<kira_sw_result> r(ms);
             kira_status kst =
-                pass == 0 ? kira::banded_sw_dev(ctx, sj, c.cfg, c.knobs, r.data(), cig)
-                          : kira::group_sw_dev(ctx, sj, gof.data(), group_neg, c.cfg, r.data(), cig);
+                pass == 0 ? lane_banded.launch(sj, c.cfg, c.knobs, kira::kPinBanded)
+                          : lane_groups.launch(sj, gof.data(), group_neg, c.cfg, kira::kPinGroup);
+            if (kst != KIRA_OK) return kst;
+        }
+    }
+    kira::trace_mark("lane_launch");
+    double lane_ms = ms_since(t0);
+    // scalar jobs: align_chain_with_meta (stage4:1196-1205)
+    t0 = Clock::now();
+    {
+        kira_status kst = banded_launch(scalar_run, scalar, chain_offsets(scalar), kira::kPinBanded2);
+        if (kst != KIRA_OK) return kst;
+    }
+    kira::trace_mark("scalar_launch");
+    double scalar_ms = ms_since(t0);
+
+    t0 = Clock::now();
+    build_accepted();
+    kira::trace_mark("acc_records");
+    R.phase_ms[0] += ms_since(t0);
+    t0 = Clock::now();
+    build_certified();
+    kira::trace_mark("cert_records");
+    R.phase_ms[1] += ms_since(t0);
+
+    t0 = Clock::now();
+    {
+        const int32_t m = (int32_t)lane_order.size();
+        std::vector<kira_sw_result> out(m);
+        std::vector<uint32_t> cig;
+        kira::BandedRun *br = &lane_banded;
+        for (int pass = 0; pass < 2; pass++) {
+            const std::vector<int32_t> &sel = lane_sel[pass];
+            const int32_t ms = (int32_t)sel.size();
+            if (ms == 0) continue;
+            std::vector<kira_sw_result> r(ms);
+            kira_status kst = pass == 0 ? br->fetch(r.data(), cig) : lane_groups.fetch(r.data(), cig);
             if (kst != KIRA_OK) return kst;
             for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
         }
+        kira::trace_mark("lane_sw");
         build(m, [&](int64_t i, Arena &A) {
-            const Job &j = simd[order[i]];
-            A.recs.push_back(Rec{j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
-                                                cig.data() + out[i].cigar_off, A)});
+            const Job &j = simd[lane_order[i]];
+            A.add(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
+                                 cig.data() + out[i].cigar_off, A));
         });
         for (int32_t i = 0; i < m; i++) {
             st.dp_simd++;
             if (out[i].early_abort) st.dp_early_abort++;
         }
     }
-    R.phase_ms[2] = ms_since(t0);
+    kira::trace_mark("lane_records");
+    R.phase_ms[2] = lane_ms + ms_since(t0);
 
-    // scalar jobs: align_chain_with_meta (stage4:1196-1205)
     t0 = Clock::now();
     {
-        std::vector<kira_sw_result> res;
+        std::vector<kira_sw_result> res(scalar.size());
         std::vector<uint32_t> cig;
-        kira_status kst = banded_run(scalar, chain_offsets(scalar), res, cig);
+        kira_status kst = scalar_run.fetch(res.data(), cig);
         if (kst != KIRA_OK) return kst;
+        kira::trace_mark("scalar_sw");
         build((int64_t)scalar.size(), [&](int64_t t, Arena &A) {
             const Job &j = scalar[t];
-            A.recs.push_back(Rec{j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
-                                                cig.data() + res[t].cigar_off, A)});
+            A.add(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
+                                 cig.data() + res[t].cigar_off, A));
         });
         for (const kira_sw_result &x : res) {
             st.dp_scalar++;
             if (x.early_abort) st.dp_early_abort++;
         }
     }
-    R.phase_ms[3] = ms_since(t0);
+    kira::trace_mark("scalar_records");
+    R.phase_ms[3] = scalar_ms + ms_since(t0);
 
     t0 = Clock::now();
     std::vector<int64_t> first, perm;
@@ -1080,14 +1158,14 @@ kira_status Driver::run() {
     for (int32_t r = 0; r < n; r++) {
         if (first[r + 1] - first[r] != 1) continue;
         if (has_comp[r]) {
-            kira_stage4_aln &a = recs[perm[first[r]]].a;
+            kira_stage4_aln &a = rec(perm[first[r]]);
             a.has_xs = 1;
             a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
             continue;
         }
         const int32_t cb = chain_best[r], cs = chain_second[r];
         if (cb <= 0 || cs <= 0) continue;
-        kira_stage4_aln &a = recs[perm[first[r]]].a;
+        kira_stage4_aln &a = rec(perm[first[r]]);
         if (a.has_xs) continue;
         const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
         if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
@@ -1096,6 +1174,7 @@ kira_status Driver::run() {
         }
     }
 
+    kira::trace_mark("xs");
     // emit_supplementary_alignments (split_read.rs:22-57)
     if (c.supplementary) {
         // picks in the order split_read.rs appends them: job -1 = exact match
@@ -1116,14 +1195,15 @@ kira_status Driver::run() {
             for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
                 const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
                 if (first[r + 1] == first[r] || nc < 2) continue;
-                const kira_stage4_aln &p = recs[perm[first[r]]].a;
+                const kira_stage4_aln &p = rec(perm[first[r]]);
+                const uint32_t *pc = rec_cigar(perm[first[r]]);
                 const kira_chain *cl = chains + c0;
                 // pick_supplementary_chains (split_read.rs:60-125)
                 const int32_t ps = std::max(cl[0].score, 1);
                 const int32_t min_score = f2i(ceilf((float)ps * kSuppMinScoreRatio));
                 uint32_t L = 0;
                 for (int32_t k = 0; k < p.n_cigar; k++) {
-                    uint32_t op = R.cigar[p.cigar_off + k];
+                    uint32_t op = pc[k];
                     if ((op & 15u) == kOpM || (op & 15u) == kOpI || (op & 15u) == kOpS)
                         L += op >> 4;
                 }
@@ -1176,10 +1256,14 @@ kira_status Driver::run() {
             }
             sj.insert(sj.end(), P.sj.begin(), P.sj.end());
         }
-        std::vector<kira_sw_result> res;
+        kira::trace_mark("supp_picks");
+        std::vector<kira_sw_result> res(sj.size());
         std::vector<uint32_t> cig;
-        kira_status kst = banded_run(sj, chain_offsets(sj), res, cig);
+        kira::BandedRun supp_run(ctx);
+        kira_status kst = banded_launch(supp_run, sj, chain_offsets(sj), kira::kPinBanded);
+        if (kst == KIRA_OK) kst = supp_run.fetch(res.data(), cig);
         if (kst != KIRA_OK) return kst;
+        kira::trace_mark("supp_sw");
         build((int64_t)all.size(), [&](int64_t i, Arena &A) {
             const Pick &p = all[i];
             const kira_chain &ch = chains[p.chain];
@@ -1192,18 +1276,46 @@ kira_status Driver::run() {
                            A);
             }
             a.is_supplementary = 1;
-            A.recs.push_back(Rec{p.read, a});
+            A.add(p.read, a);
         });
         st.supplementary += (int64_t)all.size();
         group_by_read(first, perm);
     }
 
+    kira::trace_mark("supp_records");
     R.aln_off = first;
-    R.alns.resize(recs.size());
-    parallel_chunks((int64_t)recs.size(), 16384, [&](int, int64_t b, int64_t e) {
-        for (int64_t k = b; k < e; k++) R.alns[k] = recs[perm[k]].a;
-    });
+    // the result: CIGAR words and MD text of the arenas back to back, the
+    // records in read order with their offsets moved accordingly
+    {
+        const size_t na = arenas.size();
+        std::vector<int64_t> cig_base(na + 1, 0), md_base(na + 1, 0);
+        for (size_t a = 0; a < na; a++) {
+            cig_base[a + 1] = cig_base[a] + (int64_t)arenas[a]->cig.size();
+            md_base[a + 1] = md_base[a] + (int64_t)arenas[a]->md.size();
+        }
+        R.cigar.resize((size_t)cig_base[na]);
+        R.md.resize((size_t)md_base[na]);
+        parallel_chunks((int64_t)na, 1, [&](int, int64_t b, int64_t e) {
+            for (int64_t a = b; a < e; a++) {
+                const Arena &A = *arenas[(size_t)a];
+                if (!A.cig.empty())
+                    memcpy(R.cigar.data() + cig_base[a], A.cig.data(), A.cig.size() * 4);
+                if (!A.md.empty()) memcpy(&R.md[(size_t)md_base[a]], A.md.data(), A.md.size());
+            }
+        });
+        R.alns.resize(perm.size());
+        parallel_chunks((int64_t)perm.size(), 16384, [&](int, int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; k++) {
+                const int64_t h = perm[k];
+                kira_stage4_aln a = rec(h);
+                a.cigar_off += cig_base[(size_t)(h >> 32)];
+                a.md_off += md_base[(size_t)(h >> 32)];
+                R.alns[k] = a;
+            }
+        });
+    }
     for (int32_t r = 0; r < n; r++) st.unmapped += R.aln_off[r + 1] == R.aln_off[r] ? 1 : 0;
+    kira::trace_mark("final");
     R.phase_ms[4] = ms_since(t0);
     return KIRA_OK;
 }

@@ -46,18 +46,6 @@ bool int8_viable(int64_t read_len, const kira_sw_cfg *c) {
            in(c->gap_extend);
 }
 
-struct DevBuf {
-    std::vector<void *> p;
-    ~DevBuf() {
-        for (void *x : p) (void)hipFree(x);
-    }
-    template <class T>
-    hipError_t alloc(T **out, size_t n) {
-        hipError_t e = hipMalloc((void **)out, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) p.push_back(*out);
-        return e;
-    }
-};
 
 }  // namespace
 
@@ -104,83 +92,104 @@ void simd_groups(int32_t n_batches, const int32_t *batch_off, const uint32_t *re
     }
 }
 
-kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *group_of,
-                         const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg,
-                         kira_sw_result *out, std::vector<uint32_t> &cigar) {
-    const int32_t m = j.n;
+kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
+                            const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg,
+                            int slot) {
+    const int32_t m = n = j.n;
     if (m == 0) return KIRA_OK;
     const int32_t ng = (int32_t)group_neg.size();
-    // packets of 64 jobs ordered by (window, read) length
-    std::vector<int32_t> order(m);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (j.ref_len[x] != j.ref_len[y]) return j.ref_len[x] > j.ref_len[y];
-        return j.read_len[x] > j.read_len[y];
-    });
-    std::vector<int32_t> pj, prows, pwords, slot(m);
-    std::vector<int64_t> ptoff, proff;
+    // packets of 64 jobs ordered by (window, read) length, both descending
+    std::vector<int32_t> order;
+    {
+        bool fits = true;
+        std::vector<uint32_t> key(m);
+        for (int32_t t = 0; t < m; t++) {
+            fits &= j.ref_len[t] >= 0 && j.ref_len[t] <= 65535 && j.read_len[t] >= 0 &&
+                    j.read_len[t] <= 65535;
+            key[t] = ((uint32_t)(65535 - j.ref_len[t]) << 16) | (uint32_t)(65535 - j.read_len[t]);
+        }
+        if (fits) {
+            stable_order_by_key(key.data(), m, order);
+        } else {
+            order.resize(m);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+                if (j.ref_len[x] != j.ref_len[y]) return j.ref_len[x] > j.ref_len[y];
+                return j.read_len[x] > j.read_len[y];
+            });
+        }
+    }
+    const int32_t P = (m + 63) / 64;
+    // pinned block: per-job arrays, then per-packet tables
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t m8 = al((size_t)m * 8), m4 = al((size_t)m * 4), p8 = al((size_t)P * 8),
+                 p4 = al((size_t)P * 4);
+    const size_t o_ro = 0, o_fo = o_ro + m8, o_coff = o_fo + m8, o_rl = o_coff + m8,
+                 o_fl = o_rl + m4, o_ab = o_fl + m4, o_gof = o_ab + m4, o_slot = o_gof + m4,
+                 o_ccap = o_slot + m4, o_neg = o_ccap + m4,
+                 o_pj = o_neg + al((size_t)std::max(ng, 1) * 4), o_ptoff = o_pj + P * 256,
+                 o_proff = o_ptoff + p8, o_prows = o_proff + p8, o_pwords = o_prows + p4,
+                 total = o_pwords + p4;
+    void *hp = nullptr;
+    hipError_t e;
+    if ((e = pinned_buf(ctx, slot, total, &hp)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "lane-group staging", e);
+    uint8_t *h = (uint8_t *)hp;
+    int64_t *coff = (int64_t *)(h + o_coff), *ptoff = (int64_t *)(h + o_ptoff),
+            *proff = (int64_t *)(h + o_proff);
+    int32_t *slot_of = (int32_t *)(h + o_slot), *ccap = (int32_t *)(h + o_ccap),
+            *pj = (int32_t *)(h + o_pj), *prows = (int32_t *)(h + o_prows),
+            *pwords = (int32_t *)(h + o_pwords);
     int64_t toff = 0, roff = 0;
-    for (int32_t s = 0; s < m; s += 64) {
-        int32_t e = std::min(m, s + 64);
-        int32_t pk = (int32_t)prows.size();
+    for (int32_t pk = 0; pk < P; pk++) {
+        const int32_t s0 = pk * 64, e0 = std::min(m, s0 + 64);
         int32_t rows = 0, maxw = 0;
         for (int32_t l = 0; l < 64; l++) {
-            int32_t t = s + l < e ? order[s + l] : -1;
-            pj.push_back(t);
+            const int32_t t = s0 + l < e0 ? order[s0 + l] : -1;
+            pj[(size_t)pk * 64 + l] = t;
             if (t >= 0) {
-                slot[t] = pk * 64 + l;
+                slot_of[t] = pk * 64 + l;
                 rows = std::max<int32_t>(rows, j.read_len[t]);
                 maxw = std::max<int32_t>(maxw, j.ref_len[t]);
             }
         }
-        int32_t words = 2 * ((maxw + 15) / 16);
-        prows.push_back(rows);
-        pwords.push_back(words);
-        ptoff.push_back(toff);
-        proff.push_back(roff);
+        const int32_t words = 2 * ((maxw + 15) / 16);
+        prows[pk] = rows;
+        pwords[pk] = words;
+        ptoff[pk] = toff;
+        proff[pk] = roff;
         toff += (int64_t)rows * words * 64;
         roff += (int64_t)rows * 64;
     }
-    std::vector<int64_t> coff(m);
-    std::vector<int32_t> ccap(m);
     int64_t cw = 0;
     for (int32_t t = 0; t < m; t++) {
         ccap[t] = 3 * j.read_len[t] + 2;
         coff[t] = cw;
         cw += ccap[t];
     }
-    DevBuf d;
-    int64_t *d_ro, *d_fo, *d_coff, *d_ptoff, *d_proff;
-    int32_t *d_rl, *d_fl, *d_ab, *d_gof, *d_neg, *d_gstop, *d_pj, *d_prows, *d_pwords,
-        *d_counter, *d_slot, *d_res, *d_ccap;
-    uint32_t *d_trace, *d_cig;
+    cig_words = cw;
+    memcpy(h + o_ro, j.read_off, (size_t)m * 8);
+    memcpy(h + o_fo, j.ref_off, (size_t)m * 8);
+    memcpy(h + o_rl, j.read_len, (size_t)m * 4);
+    memcpy(h + o_fl, j.ref_len, (size_t)m * 4);
+    memcpy(h + o_ab, j.abort_score, (size_t)m * 4);
+    memcpy(h + o_gof, group_of, (size_t)m * 4);
+    if (ng) memcpy(h + o_neg, group_neg.data(), (size_t)ng * 4);
+    trace_mark(".grp_plan");
+    uint8_t *d_blob = nullptr;
+    int32_t *d_gstop, *d_counter;
+    uint32_t *d_trace;
     int2 *d_bnd, *d_rstat;
-    hipError_t e;
-    const int32_t P = (int32_t)prows.size();
-    if ((e = d.alloc(&d_ro, m)) || (e = d.alloc(&d_fo, m)) || (e = d.alloc(&d_coff, m)) ||
-        (e = d.alloc(&d_ptoff, P)) || (e = d.alloc(&d_proff, P)) || (e = d.alloc(&d_rl, m)) ||
-        (e = d.alloc(&d_fl, m)) || (e = d.alloc(&d_ab, m)) || (e = d.alloc(&d_gof, m)) ||
-        (e = d.alloc(&d_neg, ng)) || (e = d.alloc(&d_gstop, ng)) ||
-        (e = d.alloc(&d_pj, (size_t)P * 64)) || (e = d.alloc(&d_prows, P)) ||
-        (e = d.alloc(&d_pwords, P)) || (e = d.alloc(&d_counter, 1)) ||
-        (e = d.alloc(&d_slot, m)) || (e = d.alloc(&d_res, (size_t)m * kSwResWords)) ||
-        (e = d.alloc(&d_ccap, m)) || (e = d.alloc(&d_trace, (size_t)toff)) ||
-        (e = d.alloc(&d_cig, (size_t)cw)) || (e = d.alloc(&d_bnd, (size_t)roff)) ||
-        (e = d.alloc(&d_rstat, (size_t)roff)))
+    if ((e = dm.alloc(&d_blob, total)) || (e = dm.alloc(&d_gstop, std::max(ng, 1))) ||
+        (e = dm.alloc(&d_counter, 1)) || (e = dm.alloc(&d_res, (size_t)m * kSwResWords)) ||
+        (e = dm.alloc(&d_trace, (size_t)toff)) || (e = dm.alloc(&d_cig, (size_t)cw)) ||
+        (e = dm.alloc(&d_bnd, (size_t)roff)) || (e = dm.alloc(&d_rstat, (size_t)roff)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "lane-group allocation", e);
     hipStream_t s = ctx->stream;
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
-    };
-    if ((e = up(d_ro, j.read_off, m * 8)) || (e = up(d_fo, j.ref_off, m * 8)) ||
-        (e = up(d_coff, coff.data(), m * 8)) || (e = up(d_ptoff, ptoff.data(), P * 8)) ||
-        (e = up(d_proff, proff.data(), P * 8)) || (e = up(d_rl, j.read_len, m * 4)) ||
-        (e = up(d_fl, j.ref_len, m * 4)) || (e = up(d_ab, j.abort_score, m * 4)) ||
-        (e = up(d_gof, group_of, m * 4)) || (e = up(d_neg, group_neg.data(), ng * 4)) ||
-        (e = up(d_pj, pj.data(), pj.size() * 4)) || (e = up(d_prows, prows.data(), P * 4)) ||
-        (e = up(d_pwords, pwords.data(), P * 4)) || (e = up(d_slot, slot.data(), m * 4)) ||
-        (e = up(d_ccap, ccap.data(), m * 4)))
+    if ((e = hipMemcpyAsync(d_blob, h, total, hipMemcpyHostToDevice, s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+    trace_mark(".grp_upload");
+    d_cig_off = (const int64_t *)(d_blob + o_coff);
     GroupLaunch a{};
     a.match_score = cfg.match_score;
     a.mismatch = cfg.mismatch;
@@ -188,30 +197,30 @@ kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *grou
     a.gap_extend = cfg.gap_extend;
     a.clip_penalty = cfg.clip_penalty;
     a.reads = j.d_reads;
-    a.read_off = d_ro;
-    a.read_len = d_rl;
+    a.read_off = (const int64_t *)(d_blob + o_ro);
+    a.read_len = (const int32_t *)(d_blob + o_rl);
     a.refs = j.d_refs;
-    a.ref_off = d_fo;
-    a.ref_len = d_fl;
-    a.abort_score = d_ab;
-    a.group_of = d_gof;
-    a.neg = d_neg;
+    a.ref_off = (const int64_t *)(d_blob + o_fo);
+    a.ref_len = (const int32_t *)(d_blob + o_fl);
+    a.abort_score = (const int32_t *)(d_blob + o_ab);
+    a.group_of = (const int32_t *)(d_blob + o_gof);
+    a.neg = (const int32_t *)(d_blob + o_neg);
     a.group_stop = d_gstop;
-    a.packet_jobs = d_pj;
-    a.packet_rows = d_prows;
-    a.packet_words = d_pwords;
-    a.packet_trace_off = d_ptoff;
-    a.packet_row_off = d_proff;
+    a.packet_jobs = (const int32_t *)(d_blob + o_pj);
+    a.packet_rows = (const int32_t *)(d_blob + o_prows);
+    a.packet_words = (const int32_t *)(d_blob + o_pwords);
+    a.packet_trace_off = (const int64_t *)(d_blob + o_ptoff);
+    a.packet_row_off = (const int64_t *)(d_blob + o_proff);
     a.n_packets = P;
     a.counter = d_counter;
     a.trace = d_trace;
     a.bnd = d_bnd;
     a.rstat = d_rstat;
-    a.job_slot = d_slot;
+    a.job_slot = (const int32_t *)(d_blob + o_slot);
     a.res = d_res;
     a.cigar = d_cig;
-    a.cigar_off = d_coff;
-    a.cigar_cap = d_ccap;
+    a.cigar_off = d_cig_off;
+    a.cigar_cap = (const int32_t *)(d_blob + o_ccap);
     // the keyed maximum shifts scores left by 4: every H <= min(L, W) * match
     // and every candidate >= -max(mismatch, open + extend) must stay in 2^27
     {
@@ -226,7 +235,24 @@ kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *grou
     }
     if ((e = launch_group_sw(a, m, ng, std::min(P, 256 * 16), s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "lane-group launch", e);
-    return fetch_sw_results(ctx, m, d_res, d_cig, coff.data(), false, out, cigar);
+    trace_mark(".grp_launch");
+    return KIRA_OK;
+}
+
+kira_status GroupRun::fetch(kira_sw_result *out, std::vector<uint32_t> &cigar) {
+    const kira_status st =
+        fetch_sw_results(ctx, dm, n, d_res, d_cig, d_cig_off, cig_words, false, out, cigar);
+    trace_mark(".grp_fetch");
+    return st;
+}
+
+kira_status group_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const int32_t *group_of,
+                         const std::vector<int32_t> &group_neg, const kira_sw_cfg &cfg,
+                         kira_sw_result *out, std::vector<uint32_t> &cigar) {
+    GroupRun run(ctx);
+    kira_status st = run.launch(j, group_of, group_neg, cfg, kPinGroup);
+    if (st != KIRA_OK) return st;
+    return run.fetch(out, cigar);
 }
 
 }  // namespace kira
@@ -272,7 +298,7 @@ extern "C" kira_status kira_align_batch_simd(kira_ctx *ctx, int32_t n_batches,
         if (rl[k]) memcpy(reads.data() + ro[k], in[k].read, rl[k]);
         if (wl[k]) memcpy(refs.data() + fo[k], in[k].window, wl[k]);
     }
-    DevBuf d;
+    kira::DevMem d(ctx);
     uint8_t *d_reads, *d_refs;
     hipError_t e;
     hipStream_t s = ctx->stream;

@@ -359,13 +359,28 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
             }
         }
     }
-    // packets: jobs sorted by (band class, read length descending)
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (cls[x] != cls[y]) return cls[x] < cls[y];
-        return read_len[x] > read_len[y];
-    });
+    // packets: jobs sorted by (band class, read length descending), stable
+    std::vector<int32_t> order;
+    {
+        constexpr int32_t kLenMax = (1 << 27) - 1;
+        static_assert(kNumClasses <= 32, "class field of the sort key");
+        bool fits = true;
+        std::vector<uint32_t> key(n);
+        for (int32_t k = 0; k < n; k++) {
+            fits &= read_len[k] <= kLenMax;
+            key[k] = ((uint32_t)cls[k] << 27) | (uint32_t)(kLenMax - std::min(read_len[k], kLenMax));
+        }
+        if (fits) {
+            stable_order_by_key(key.data(), n, order);
+        } else {
+            order.resize(n);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+                if (cls[x] != cls[y]) return cls[x] < cls[y];
+                return read_len[x] > read_len[y];
+            });
+        }
+    }
     L.job_slot.assign(n, 0);
     for (int c = 0; c < kNumClasses; c++) L.class_first[c] = -1;
     for (int32_t s = 0; s < n;) {
@@ -693,47 +708,28 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
 
 namespace kira {
 
-kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, const uint32_t *d_cig,
-                             const int64_t *src_off, bool banded, kira_sw_result *out,
-                             std::vector<uint32_t> &cigar) {
+kira_status fetch_sw_results(kira_ctx *ctx, DevMem &dm, int32_t n, const int32_t *d_res,
+                             const uint32_t *d_cig, const int64_t *d_src_off, int64_t cig_words,
+                             bool banded, kira_sw_result *out, std::vector<uint32_t> &cigar) {
     if (n == 0) return KIRA_OK;
     hipStream_t s = ctx->stream;
-    std::vector<int32_t> res((size_t)n * kSwResWords);
     hipError_t e;
+    int64_t *d_dst = nullptr;
+    uint32_t *d_out = nullptr;
+    if ((e = dm.alloc(&d_dst, n)) || (e = dm.alloc(&d_out, (size_t)std::max<int64_t>(cig_words, 1))))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "CIGAR compaction allocation", e);
+    if ((e = launch_cigar_compact(n, d_res, d_cig, d_src_off, d_dst, d_out, s)))
+        return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR compaction", e);
+    std::vector<int32_t> res((size_t)n * kSwResWords);
     if ((e = hipMemcpyAsync(res.data(), d_res, res.size() * 4, hipMemcpyDeviceToHost, s)) ||
         (e = hipStreamSynchronize(s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "device to host copy", e);
-    std::vector<int64_t> dst(n);
+    trace_mark(".res_sync");
     int64_t total = 0;
     for (int32_t k = 0; k < n; k++) {
         const int32_t *r = &res[(size_t)k * kSwResWords];
         if (r[6] < 0) return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR capacity exceeded");
         if (banded && r[7] != 0) return fail(ctx, KIRA_ERR_RUNTIME, "int16 band overflow");
-        dst[k] = total;
-        total += r[6];
-    }
-    const size_t base = cigar.size();
-    cigar.resize(base + (size_t)total);
-    if (total > 0) {
-        DevMem dm(ctx);
-        int64_t *d_src, *d_dst;
-        uint32_t *d_out;
-        if ((e = dm.alloc(&d_src, n)) || (e = dm.alloc(&d_dst, n)) ||
-            (e = dm.alloc(&d_out, (size_t)total)))
-            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "CIGAR compaction allocation", e);
-        if ((e = hipMemcpyAsync(d_src, src_off, (size_t)n * 8, hipMemcpyHostToDevice, s)) ||
-            (e = hipMemcpyAsync(d_dst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, s)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
-        compact_cigar_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, d_res, d_cig, d_src,
-                                                                         d_dst, d_out);
-        if ((e = hipGetLastError()) ||
-            (e = hipMemcpyAsync(cigar.data() + base, d_out, (size_t)total * 4,
-                                hipMemcpyDeviceToHost, s)) ||
-            (e = hipStreamSynchronize(s)))
-            return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR compaction", e);
-    }
-    for (int32_t k = 0; k < n; k++) {
-        const int32_t *r = &res[(size_t)k * kSwResWords];
         kira_sw_result &o = out[k];
         o.score = r[0];
         o.read_end = r[1];
@@ -743,47 +739,90 @@ kira_status fetch_sw_results(kira_ctx *ctx, int32_t n, const int32_t *d_res, con
         o.read_start = r[5];
         o.n_cigar = r[6];
         o.cells = r[8];
-        o.cigar_off = (int64_t)base + dst[k];
+        o.cigar_off = (int64_t)cigar.size() + total;
+        total += r[6];
     }
+    const size_t base = cigar.size();
+    cigar.resize(base + (size_t)total);
+    if (total > 0 &&
+        ((e = hipMemcpyAsync(cigar.data() + base, d_out, (size_t)total * 4, hipMemcpyDeviceToHost,
+                             s)) ||
+         (e = hipStreamSynchronize(s))))
+        return fail(ctx, KIRA_ERR_RUNTIME, "CIGAR download", e);
     return KIRA_OK;
+}
+
+namespace {
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+kira_status BandedRun::launch(const SwJobsHost &j, const kira_sw_cfg &cfg,
+                              const kira_sw_knobs &kn, int slot) {
+    n = j.n;
+    if (n == 0) return KIRA_OK;
+    if (const char *why = sw_plan_layout(n, j.read_len, j.band, cfg, L, &ctx->tune))
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, why);
+    trace_mark(".bsw_plan");
+    // plan tables, then the job arrays, in one pinned block and one copy
+    const size_t tb = sw_plan_table_bytes(L), sb = sw_plan_slab_bytes(L);
+    const size_t n8 = al256((size_t)n * 8), n4 = al256((size_t)n * 4);
+    const size_t o_ro = tb, o_fo = o_ro + n8, o_rl = o_fo + n8, o_fl = o_rl + n4,
+                 o_off = o_fl + n4, o_ab = o_off + n4, o_band = o_ab + n4,
+                 total = o_band + (j.band ? n4 : 0);
+    void *hp = nullptr;
+    hipError_t e;
+    if ((e = pinned_buf(ctx, slot, total, &hp)))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW staging", e);
+    uint8_t *h = (uint8_t *)hp;
+    sw_plan_pack_tables(L, h);
+    memcpy(h + o_ro, j.read_off, (size_t)n * 8);
+    memcpy(h + o_fo, j.ref_off, (size_t)n * 8);
+    memcpy(h + o_rl, j.read_len, (size_t)n * 4);
+    memcpy(h + o_fl, j.ref_len, (size_t)n * 4);
+    memcpy(h + o_off, j.offset, (size_t)n * 4);
+    memcpy(h + o_ab, j.abort_score, (size_t)n * 4);
+    if (j.band) memcpy(h + o_band, j.band, (size_t)n * 4);
+    uint8_t *d_blob = nullptr, *d_slab = nullptr;
+    if ((e = dm.alloc(&d_blob, total)) || (e = dm.alloc(&d_slab, sb)) ||
+        (e = dm.alloc(&d_res, (size_t)n * kSwResWords)) ||
+        (e = dm.alloc(&d_cig, (size_t)std::max<int64_t>(L.cigar_words, 1))))
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
+    hipStream_t s = ctx->stream;
+    if ((e = hipMemcpyAsync(d_blob, h, total, hipMemcpyHostToDevice, s)))
+        return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
+    const SwPlanDev d = sw_plan_bind(L, d_blob, d_slab);
+    d_cig_off = d.cigar_off;
+    kira_sw_jobs_dev jd{j.d_reads,
+                        (const int64_t *)(d_blob + o_ro),
+                        (const int32_t *)(d_blob + o_rl),
+                        j.d_refs,
+                        (const int64_t *)(d_blob + o_fo),
+                        (const int32_t *)(d_blob + o_fl),
+                        (const int32_t *)(d_blob + o_off),
+                        j.band ? (const int32_t *)(d_blob + o_band) : nullptr,
+                        (const int32_t *)(d_blob + o_ab),
+                        d_res,
+                        d_cig};
+    if (const char *why = sw_plan_launch(L, d, jd, cfg, kn, nullptr, nullptr, s, &e, &ctx->fan))
+        return fail(ctx, e ? KIRA_ERR_RUNTIME : KIRA_ERR_INVALID_ARGUMENT, why, e);
+    trace_mark(".bsw_launch");
+    return KIRA_OK;
+}
+
+kira_status BandedRun::fetch(kira_sw_result *out, std::vector<uint32_t> &cigar) {
+    const kira_status st =
+        fetch_sw_results(ctx, dm, n, d_res, d_cig, d_cig_off, L.cigar_words, true, out, cigar);
+    trace_mark(".bsw_fetch");
+    return st;
 }
 
 kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg &cfg,
                           const kira_sw_knobs &kn, kira_sw_result *out,
                           std::vector<uint32_t> &cigar) {
-    const int32_t n = j.n;
-    if (n == 0) return KIRA_OK;
-    kira_sw_plan *plan = nullptr;
-    kira_status st = kira_banded_sw_plan_create(ctx, n, j.read_len, j.band, &cfg, &plan);
+    BandedRun run(ctx);
+    kira_status st = run.launch(j, cfg, kn, kPinBanded);
     if (st != KIRA_OK) return st;
-    struct PlanGuard {
-        kira_sw_plan *p;
-        ~PlanGuard() { kira_banded_sw_plan_destroy(p); }
-    } pg{plan};
-    DevMem dm(ctx);
-    int64_t *d_ro, *d_fo;
-    int32_t *d_rl, *d_fl, *d_off, *d_band = nullptr, *d_ab, *d_res;
-    uint32_t *d_cig;
-    hipError_t e;
-    hipStream_t s = ctx->stream;
-    if ((e = dm.alloc(&d_ro, n)) || (e = dm.alloc(&d_fo, n)) || (e = dm.alloc(&d_rl, n)) ||
-        (e = dm.alloc(&d_fl, n)) || (e = dm.alloc(&d_off, n)) || (e = dm.alloc(&d_ab, n)) ||
-        (e = dm.alloc(&d_res, (size_t)n * kSwResWords)) ||
-        (e = dm.alloc(&d_cig, (size_t)plan->L.cigar_words)) ||
-        (j.band && (e = dm.alloc(&d_band, n))))
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
-    auto up = [&](void *d, const void *h, size_t bytes) {
-        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
-    };
-    if ((e = up(d_ro, j.read_off, (size_t)n * 8)) || (e = up(d_fo, j.ref_off, (size_t)n * 8)) ||
-        (e = up(d_rl, j.read_len, (size_t)n * 4)) || (e = up(d_fl, j.ref_len, (size_t)n * 4)) ||
-        (e = up(d_off, j.offset, (size_t)n * 4)) || (e = up(d_ab, j.abort_score, (size_t)n * 4)) ||
-        (j.band && (e = up(d_band, j.band, (size_t)n * 4))))
-        return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
-    kira_sw_jobs_dev jd{j.d_reads, d_ro, d_rl, j.d_refs, d_fo, d_fl, d_off,
-                        d_band,    d_ab, d_res, d_cig};
-    if ((st = kira_banded_sw_run_dev(ctx, plan, &jd, &cfg, &kn)) != KIRA_OK) return st;
-    return fetch_sw_results(ctx, n, d_res, d_cig, plan->L.cigar_off.data(), true, out, cigar);
+    return run.fetch(out, cigar);
 }
 
 }  // namespace kira

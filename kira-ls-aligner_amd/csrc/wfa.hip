@@ -906,11 +906,13 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     a.queue = d_queue;
     a.n = n;
     const size_t lds = 4 * (size_t)((m_max + 3) / 4 + 2 + (n_max + 3) / 4 + 2);
+    trace_mark(".wfa_prep");
     hipLaunchKernelGGL(wfa_kernel, dim3((uint32_t)slots), dim3(64), lds, st, a);
     if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA launch", e);
     if ((e = hipMemcpyAsync(res.data(), d_res, 4 * res.size(), hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
         return fail(ctx, KIRA_ERR_RUNTIME, "WFA", e);
+    trace_mark(".wfa_sync");
     int64_t used = 0;
     std::vector<int64_t> dst(n);
     for (int32_t k = 0; k < n; k++) {
