@@ -272,30 +272,48 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
     std::vector<Job> jobs;
     std::vector<kira_fast_aln> &res = res_out;
     try {
-        jobs.reserve(n);
-        for (int32_t k = 0; k < n; k++) {
+        // the chains that pass, in chain order: each range gates its chains
+        // and counts, then places them
+        auto gate = [&](int32_t k, Job &j) -> bool {
             const int32_t L = read_len[k];
-            if (L == 0 || L > kn.wfa_max_read_len || kn.algo == 3) continue;
+            if (L == 0 || L > kn.wfa_max_read_len || kn.algo == 3) return false;
             if (kn.quality_gate) {  // fast_path_worth_attempting (router.rs:169-189)
                 const int32_t cs = chains[k].score;
-                if (cs <= 0) continue;
+                if (cs <= 0) return false;
                 const uint64_t r = std::min<uint64_t>((uint64_t)cs * 100u, UINT32_MAX) / (uint64_t)L;
-                if (r < (uint64_t)std::max(kn.quality_pct, 0)) continue;
+                if (r < (uint64_t)std::max(kn.quality_pct, 0)) return false;
             }
             const int64_t ws = (int64_t)chains[k].ref_start - (int64_t)chains[k].read_start;
             const int32_t cid = (int32_t)chains[k].ref_id;
             const int64_t clen = contig_len[cid];
-            if (ws < 0 || ws >= clen) continue;
+            if (ws < 0 || ws >= clen) return false;
             const int64_t tl = std::min<int64_t>((int64_t)L + band, clen - ws);
-            if (tl < L) continue;
-            Job j{};
+            if (tl < L) return false;
+            j = Job{};
             j.k = k;
             j.contig = cid;
             j.ws = ws;
             j.L = L;
             j.tl = (int32_t)tl;
-            jobs.push_back(j);
-        }
+            return true;
+        };
+        const int64_t kGrain = 8192;
+        const int T = host_threads();
+        const int Tn = (int)std::max<int64_t>(1, std::min<int64_t>(T, ((int64_t)n + kGrain - 1) / kGrain));
+        std::vector<int64_t> cnt((size_t)T + 1, 0);
+        parallel_chunks(n, kGrain, [&](int t, int64_t b, int64_t e) {
+            Job j;
+            int64_t c0 = 0;
+            for (int64_t k = b; k < e; k++) c0 += gate((int32_t)k, j) ? 1 : 0;
+            cnt[(size_t)t + 1] = c0;
+        });
+        for (int t = 0; t < Tn; t++) cnt[(size_t)t + 1] += cnt[(size_t)t];
+        jobs.resize((size_t)cnt[(size_t)Tn]);
+        parallel_chunks(n, kGrain, [&](int t, int64_t b, int64_t e) {
+            int64_t o = cnt[(size_t)t];
+            for (int64_t k = b; k < e; k++)
+                if (gate((int32_t)k, jobs[(size_t)o])) o++;
+        });
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
@@ -375,11 +393,24 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             if ((e = lists.begin(ctx, jobs.size())))
                 return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
             int32_t max_lt = 0;
-            for (int32_t q = 0; q < (int32_t)jobs.size(); q++) {
-                const Job &j = jobs[q];
-                lists.add(read_off[j.k], j.L, text_of(j), j.tl);
-                max_lt = std::max(max_lt, j.L + j.tl);
-            }
+            const int64_t nq = (int64_t)jobs.size();
+            std::vector<int32_t> part_max((size_t)host_threads(), 0);
+            parallel_chunks(nq, 16384, [&](int t, int64_t b, int64_t e) {
+                int32_t mx = 0;
+                for (int64_t q = b; q < e; q++) {
+                    const Job &j = jobs[q];
+                    lists.roff[q] = read_off[j.k];
+                    lists.rlen[q] = j.L;
+                    lists.toff[q] = text_of(j);
+                    lists.tlen[q] = j.tl;
+                    lists.x[q] = 0;
+                    lists.y[q] = 0;
+                    mx = std::max(mx, j.L + j.tl);
+                }
+                part_max[(size_t)t] = mx;
+            });
+            lists.n = (size_t)nq;
+            for (int32_t v : part_max) max_lt = std::max(max_lt, v);
             const int32_t m = (int32_t)lists.n;
             trace_mark(".fp_sp_lists");
             if (m) {

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <memory>
 #include <new>
@@ -213,6 +214,11 @@ struct Driver {
         for (int t = 0; t < T; t++) arenas.emplace_back(new Arena());
         parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
             Arena &A = *arenas[a0 + (size_t)t];
+            // about one record of a few CIGAR ops and MD characters an item
+            A.read.reserve((size_t)(e - b));
+            A.recs.reserve((size_t)(e - b));
+            A.cig.reserve((size_t)(e - b) * 4);
+            A.md.reserve((size_t)(e - b) * 16);
             for (int64_t i = b; i < e; i++) f(i, A);
         });
     }
@@ -461,30 +467,49 @@ kira_status Driver::run() {
         std::vector<int32_t> fk, frl;
         std::vector<int64_t> fro;
         std::vector<kira_chain> fch;
+        // exact copies get their score; the others are listed for the
+        // cascade in candidate order (count per range, then place)
+        const int64_t nc = (int64_t)cand.size();
+        const int64_t kGrain = 8192;
+        const int T = host_threads();
+        const int Tn = (int)std::max<int64_t>(1, std::min<int64_t>(T, (nc + kGrain - 1) / kGrain));
+        std::vector<int64_t> cnt((size_t)T + 1, 0);
         std::vector<uint8_t> is_copy(cand.size(), 0);
-        parallel_chunks((int64_t)cand.size(), 8192, [&](int, int64_t b, int64_t e) {
+        parallel_chunks(nc, kGrain, [&](int t, int64_t b, int64_t e) {
+            int64_t others = 0;
             for (int64_t i = b; i < e; i++) {
-                const Cand &x = cand[i];
+                Cand &x = cand[i];
                 const kira_chain &ch = chains[x.chain];
+                const int32_t L = read_len[x.read];
                 is_copy[i] = exact_candidate(x.read, ch) &&
                              memcmp(oriented(x.read, ch.strand == 1),
-                                    contig(ch.ref_id) + ch.ref_start,
-                                    (size_t)read_len[x.read]) == 0;
+                                    contig(ch.ref_id) + ch.ref_start, (size_t)L) == 0;
+                if (is_copy[i])
+                    x.score = wrap_i32((int64_t)c.cfg.match_score * L);
+                else
+                    others++;
+            }
+            cnt[(size_t)t + 1] = others;
+        });
+        for (int t = 0; t < Tn; t++) cnt[(size_t)t + 1] += cnt[(size_t)t];
+        const int64_t nf = cnt[(size_t)Tn];
+        fk.resize((size_t)nf);
+        fro.resize((size_t)nf);
+        frl.resize((size_t)nf);
+        fch.resize((size_t)nf);
+        parallel_chunks(nc, kGrain, [&](int t, int64_t b, int64_t e) {
+            int64_t o = cnt[(size_t)t];
+            for (int64_t i = b; i < e; i++) {
+                if (is_copy[i]) continue;
+                const Cand &x = cand[i];
+                const kira_chain &ch = chains[x.chain];
+                fk[(size_t)o] = (int32_t)i;
+                fro[(size_t)o] = oriented_off(x.read, ch.strand == 1);
+                frl[(size_t)o] = read_len[x.read];
+                fch[(size_t)o] = ch;
+                o++;
             }
         });
-        for (size_t i = 0; i < cand.size(); i++) {
-            Cand &x = cand[i];
-            const kira_chain &ch = chains[x.chain];
-            const int32_t L = read_len[x.read];
-            if (is_copy[i]) {
-                x.score = wrap_i32((int64_t)c.cfg.match_score * L);
-                continue;
-            }
-            fk.push_back((int32_t)i);
-            fro.push_back(oriented_off(x.read, ch.strand == 1));
-            frl.push_back(L);
-            fch.push_back(ch);
-        }
         kira::trace_mark("tt_prep");
         if (!fk.empty()) {
             kira_fast_stats fst;
@@ -588,13 +613,14 @@ kira_status Driver::run() {
         }
     });
     {
-        size_t tot = 0;
-        for (const auto &v : part) tot += v.size();
-        sels.reserve(tot);
-        for (auto &v : part) {
-            sels.insert(sels.end(), v.begin(), v.end());
-            std::vector<Sel>().swap(v);
-        }
+        std::vector<size_t> at(part.size() + 1, 0);
+        for (size_t t = 0; t < part.size(); t++) at[t + 1] = at[t] + part[t].size();
+        sels.resize(at[part.size()]);
+        parallel_chunks((int64_t)part.size(), 1, [&](int, int64_t b, int64_t e) {
+            for (int64_t t = b; t < e; t++)
+                if (!part[t].empty())
+                    memcpy(sels.data() + at[t], part[t].data(), part[t].size() * sizeof(Sel));
+        });
     }
     st.chains_used = (int64_t)sels.size();
 
@@ -726,38 +752,58 @@ kira_status Driver::run() {
     std::vector<int32_t> abort_of(ns, 0);
     std::vector<int32_t> accepted;  // selected chains that end in a record here
     const int32_t band = std::max(c.cfg.bandwidth, 1);
-    for (int32_t k = 0; k < ns; k++) {
-        const Sel &x = sels[k];
-        if (exact[k]) {
-            st.exact_matches++;
-            outcome[k] = 1;
-            continue;
+    {
+        // in parallel over chain ranges; each range counts into its own stats
+        struct Cnt {
+            int64_t exact = 0, packed = 0, wfa = 0, accept = 0, reject = 0, fallback = 0;
+            int64_t reason[10] = {};
+        };
+        std::vector<Cnt> cnt((size_t)host_threads());
+        parallel_chunks(ns, 8192, [&](int t, int64_t b0, int64_t e0) {
+            Cnt &C = cnt[(size_t)t];
+            for (int64_t k = b0; k < e0; k++) {
+                const Sel &x = sels[k];
+                if (exact[k]) {
+                    C.exact++;
+                    outcome[k] = 1;
+                    continue;
+                }
+                if (x.cached >= 0) {
+                    // the candidate search's verified alignment (:736-747)
+                    if (tt_res[x.cached].kind == 1)
+                        C.packed++;
+                    else
+                        C.wfa++;
+                    __atomic_store_n(&dp_used[x.read], (uint8_t)1, __ATOMIC_RELAXED);
+                    outcome[k] = 1;
+                    continue;
+                }
+                const kira_prefilter_out &o = pf[k];
+                C.reason[o.reason < 9 ? o.reason : 9]++;
+                abort_of[k] = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
+                if (o.result == 0) {
+                    C.accept++;
+                    outcome[k] = 1;
+                    continue;
+                }
+                if (o.result == 1) {
+                    C.reject++;
+                    continue;
+                }
+                C.fallback++;
+                __atomic_store_n(&dp_used[x.read], (uint8_t)1, __ATOMIC_RELAXED);
+                outcome[k] = 2;
+            }
+        });
+        for (const Cnt &C : cnt) {
+            st.exact_matches += C.exact;
+            st.packed_spectral_resolved += C.packed;
+            st.wfa_resolved += C.wfa;
+            st.prefilter_accept += C.accept;
+            st.prefilter_reject += C.reject;
+            st.prefilter_fallback += C.fallback;
+            for (int r = 0; r < 10; r++) st.reason_counts[r] += C.reason[r];
         }
-        if (x.cached >= 0) {
-            // the candidate search's verified alignment (:736-747)
-            if (tt_res[x.cached].kind == 1)
-                st.packed_spectral_resolved++;
-            else
-                st.wfa_resolved++;
-            dp_used[x.read] = 1;
-            outcome[k] = 1;
-            continue;
-        }
-        const kira_prefilter_out &o = pf[k];
-        st.reason_counts[o.reason < 9 ? o.reason : 9]++;
-        abort_of[k] = o.has_metrics ? sat_sub(o.score, c.dp_abort_margin) : INT32_MIN / 4;
-        if (o.result == 0) {
-            st.prefilter_accept++;
-            outcome[k] = 1;
-            continue;
-        }
-        if (o.result == 1) {
-            st.prefilter_reject++;
-            continue;
-        }
-        st.prefilter_fallback++;
-        dp_used[x.read] = 1;
-        outcome[k] = 2;
     }
     // without the GPU dispatcher, a fallback chain tries the fast DP cascade
     // first (try_fast_dp_alignment, :868-882; accept_fast_result, :289-299)
@@ -793,29 +839,68 @@ kira_status Driver::run() {
                 st.wfa_resolved++;
         }
     }
-    for (int32_t k = 0; k < ns; k++) {
-        if (outcome[k] == 0) continue;
-        if (outcome[k] == 1 || fast_of[k] >= 0) {
-            accepted.push_back(k);
-            continue;
-        }
-        const Sel &x = sels[k];
-        const kira_chain &ch = chains[x.chain];
-        const int32_t L = read_len[x.read];
-        const int32_t abort = abort_of[k];
-        const int bucket = L <= 120 ? 0 : (L <= 150 ? 1 : 2);
-        if (bucket < 2 && lanes > 1) {
-            // build_simd_window (:1426-1445)
-            const int32_t flank = band * 2;
-            const int64_t wlen = (int64_t)L + 2 * (int64_t)flank;
-            const int32_t desired = wrap_i32((int64_t)(int32_t)ch.ref_start -
-                                             (int32_t)ch.read_start - flank);
-            if (desired >= 0 && desired + wlen <= contig_len[ch.ref_id]) {
-                simd.push_back(Job{x.read, x.chain, desired, (int32_t)wlen, abort});
-                continue;
+    {
+        // route every chain (0 none, 1 a record here, 2 lane job, 3 scalar
+        // job) in parallel, then place each list's items in chain order
+        std::vector<uint8_t> route(ns);
+        std::vector<Job> job_of(ns);
+        const int T = host_threads();
+        std::vector<std::array<int64_t, 4>> cnt((size_t)T, std::array<int64_t, 4>{});
+        const int64_t kGrain = 8192;
+        const int Tn = (int)std::max<int64_t>(1, std::min<int64_t>(T, (ns + kGrain - 1) / kGrain));
+        parallel_chunks(ns, kGrain, [&](int t, int64_t b0, int64_t e0) {
+            for (int64_t k = b0; k < e0; k++) {
+                uint8_t rt = 0;
+                if (outcome[k] != 0) {
+                    if (outcome[k] == 1 || fast_of[k] >= 0) {
+                        rt = 1;
+                    } else {
+                        const Sel &x = sels[k];
+                        const kira_chain &ch = chains[x.chain];
+                        const int32_t L = read_len[x.read];
+                        const int32_t abort = abort_of[k];
+                        const int bucket = L <= 120 ? 0 : (L <= 150 ? 1 : 2);
+                        rt = 3;
+                        if (bucket < 2 && lanes > 1) {
+                            // build_simd_window (:1426-1445)
+                            const int32_t flank = band * 2;
+                            const int64_t wlen = (int64_t)L + 2 * (int64_t)flank;
+                            const int32_t desired = wrap_i32((int64_t)(int32_t)ch.ref_start -
+                                                             (int32_t)ch.read_start - flank);
+                            if (desired >= 0 && desired + wlen <= contig_len[ch.ref_id]) {
+                                job_of[k] = Job{x.read, x.chain, desired, (int32_t)wlen, abort};
+                                rt = 2;
+                            }
+                        }
+                        if (rt == 3) job_of[k] = clamp_job(x.read, x.chain, abort);
+                    }
+                }
+                route[k] = rt;
+                cnt[(size_t)t][rt]++;
             }
-        }
-        scalar.push_back(clamp_job(x.read, x.chain, abort));
+        });
+        // chunk t's first position in each list
+        std::array<int64_t, 4> tot{};
+        std::vector<std::array<int64_t, 4>> base((size_t)T);
+        for (int t = 0; t < Tn; t++)
+            for (int r = 0; r < 4; r++) {
+                base[(size_t)t][r] = tot[r];
+                tot[r] += cnt[(size_t)t][r];
+            }
+        accepted.resize((size_t)tot[1]);
+        simd.resize((size_t)tot[2]);
+        scalar.resize((size_t)tot[3]);
+        parallel_chunks(ns, kGrain, [&](int t, int64_t b0, int64_t e0) {
+            std::array<int64_t, 4> pos = base[(size_t)t];
+            for (int64_t k = b0; k < e0; k++) {
+                switch (route[k]) {
+                case 1: accepted[(size_t)pos[1]++] = (int32_t)k; break;
+                case 2: simd[(size_t)pos[2]++] = job_of[k]; break;
+                case 3: scalar[(size_t)pos[3]++] = job_of[k]; break;
+                default: break;
+                }
+            }
+        });
     }
     kira::trace_mark("route");
     for (int32_t r = 0; r < n; r++) st.dp_reads += dp_used[r];
