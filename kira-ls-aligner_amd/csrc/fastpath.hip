@@ -228,6 +228,22 @@ struct Job {
     int32_t retry;       // index into the retry list, or -1
 };
 
+// number of positions where a[k] == b[k], k < n (eight at a time: a zero
+// byte of a ^ b is an equal position)
+inline uint32_t count_equal_bytes(const uint8_t *a, const uint8_t *b, uint32_t n) {
+    uint32_t eq = 0, t = 0;
+    for (; t + 8 <= n; t += 8) {
+        uint64_t x, y;
+        memcpy(&x, a + t, 8);
+        memcpy(&y, b + t, 8);
+        const uint64_t z = x ^ y;
+        const uint64_t u = ((z & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | z;
+        eq += (uint32_t)__builtin_popcountll(~u & 0x8080808080808080ull);
+    }
+    for (; t < n; t++) eq += a[t] == b[t] ? 1u : 0u;
+    return eq;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -241,16 +257,19 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                         int64_t reads_bytes, const uint8_t *d_reads_in, const int64_t *read_off,
                         const int32_t *read_len, const kira_chain *chains,
                         const kira_sw_cfg *cfg, const kira_fast_knobs &kn_in,
-                        std::vector<kira_fast_aln> &res_out, std::vector<uint32_t> &cig_out,
+                        PodVec<kira_fast_aln> &res_out, std::vector<uint32_t> &cig_out,
                         std::string &md_out, kira_fast_stats &st_local, bool want_md) {
     memset(&st_local, 0, sizeof st_local);
     try {
-        res_out.assign(n, kira_fast_aln{});
+        res_out.resize((size_t)std::max(n, 0));
         cig_out.clear();
         md_out.clear();
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    parallel_chunks(n, 16384, [&](int, int64_t b, int64_t e) {
+        memset((void *)(res_out.data() + b), 0, (size_t)(e - b) * sizeof(kira_fast_aln));
+    });
     if (n == 0) return KIRA_OK;
     // parse_algo (router.rs:28-37): anything unrecognised is packed; the
     // percentages and floors are u32 knobs
@@ -269,8 +288,8 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
     const bool pen_ok = kira::wfa_pen(c, pen);
     const uint8_t *gh = g->host.data();
     // ---- router gates and windows (host)
-    std::vector<Job> jobs;
-    std::vector<kira_fast_aln> &res = res_out;
+    PodVec<Job> jobs;
+    PodVec<kira_fast_aln> &res = res_out;
     try {
         // the chains that pass, in chain order: each range gates its chains
         // and counts, then places them
@@ -357,15 +376,6 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             n = 0;
             return hipSuccess;
         }
-        void add(int64_t ro, int32_t rl, int64_t to, int32_t tn, int32_t a = 0, int32_t b = 0) {
-            roff[n] = ro;
-            rlen[n] = rl;
-            toff[n] = to;
-            tlen[n] = tn;
-            x[n] = a;
-            y[n] = b;
-            n++;
-        }
     } lists;
     struct DevLists {
         int64_t *roff = nullptr, *toff = nullptr;
@@ -449,12 +459,19 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         if (!cand.empty() && pen_ok) {
             if ((e = lists.begin(ctx, cand.size())))
                 return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
-            for (int32_t q : cand) {
-                const Job &j = jobs[q];
-                lists.add(read_off[j.k], j.L, text_of(j), j.tl,
-                          (int32_t)((int64_t)j.L * kn.myers_pct / 100 + kn.myers_floor));
-            }
             const int32_t m = (int32_t)cand.size();
+            parallel_chunks(m, 16384, [&](int, int64_t b, int64_t e0) {
+                for (int64_t i = b; i < e0; i++) {
+                    const Job &j = jobs[cand[i]];
+                    lists.roff[i] = read_off[j.k];
+                    lists.rlen[i] = j.L;
+                    lists.toff[i] = text_of(j);
+                    lists.tlen[i] = j.tl;
+                    lists.x[i] = (int32_t)((int64_t)j.L * kn.myers_pct / 100 + kn.myers_floor);
+                    lists.y[i] = 0;
+                }
+            });
+            lists.n = (size_t)m;
             DevLists d;
             uint8_t *d_found = nullptr;
             int32_t *d_dist = nullptr, *d_end = nullptr;
@@ -483,10 +500,18 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                            std::vector<uint32_t> &wcig, std::vector<int64_t> &woff) -> kira_status {
             if ((e = lists.begin(ctx, list.size())))
                 return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
-            for (size_t i = 0; i < list.size(); i++) {
-                const Job &j = jobs[list[i]];
-                lists.add(read_off[j.k], j.L, tofs[i], tlens[i], budget[i], tbf[i]);
-            }
+            parallel_chunks((int64_t)list.size(), 16384, [&](int, int64_t b, int64_t e0) {
+                for (int64_t i = b; i < e0; i++) {
+                    const Job &j = jobs[list[i]];
+                    lists.roff[i] = read_off[j.k];
+                    lists.rlen[i] = j.L;
+                    lists.toff[i] = tofs[i];
+                    lists.tlen[i] = tlens[i];
+                    lists.x[i] = budget[i];
+                    lists.y[i] = tbf[i];
+                }
+            });
+            lists.n = list.size();
             DevLists d;
             if ((e = upload(d))) return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path WFA", e);
             kira::WfaDevJobs wj;
@@ -517,7 +542,8 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             std::vector<int32_t> tlens(m), budget(m), tbf(m);
             const int64_t xb = (int64_t)c.match_score + c.mismatch;
             const int64_t ieb = (int64_t)c.match_score + c.gap_extend;
-            for (size_t i = 0; i < m; i++) {
+            parallel_chunks((int64_t)m, 16384, [&](int, int64_t b0, int64_t e0) {
+            for (int64_t i = b0; i < e0; i++) {
                 Job &j = jobs[wfa_jobs[i]];
                 const int64_t mb = ((int64_t)j.L * kn.wfa_budget_pct) / 100 * xb;
                 const int64_t ib = (c.gap_open + ieb * 4) * 3;
@@ -530,6 +556,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 j.ftext = tofs[i];
                 j.ftext_len = tlens[i];
             }
+            });
             kira_status rs = run_wfa(wfa_jobs, tofs, tlens, budget, tbf, wres1, wcig1, woff1);
             if (rs != KIRA_OK) return rs;
             std::vector<int32_t> suspects;
@@ -678,8 +705,12 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                     for (int32_t z = 0; z < j.n_ops; z++) {
                         const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
                         if (op == 0) {
-                            for (uint32_t u = 0; u < len; u++, qi++, ti++)
-                                score += rd[qi] == text[ti] ? c.match_score : -c.mismatch;
+                            // the per-column sum (wrapping like the reference's i32)
+                            const uint32_t eq = count_equal_bytes(rd + qi, text + ti, len);
+                            score = (int32_t)((uint32_t)score + eq * (uint32_t)c.match_score -
+                                              (len - eq) * (uint32_t)c.mismatch);
+                            qi += len;
+                            ti += len;
                         } else if (op == 1 || op == 2) {
                             score -= c.gap_open + c.gap_extend * (int32_t)len;
                             if (op == 1)
@@ -777,7 +808,7 @@ extern "C" kira_status kira_fast_dp_batch(
         if (chains[k].ref_id >= (uint32_t)n_contigs)
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain contig out of range");
     }
-    std::vector<kira_fast_aln> res;
+    kira::PodVec<kira_fast_aln> res;
     std::vector<uint32_t> cig;
     std::string md;
     kira_status rs = kira::fast_dp_run(ctx, g, contig_off, contig_len, n, reads, reads_bytes,

@@ -8,6 +8,8 @@
 #include <atomic>
 #include <condition_variable>
 #include <functional>
+#include <memory>
+#include <type_traits>
 #include <mutex>
 #include <chrono>
 #include <cstdio>
@@ -23,6 +25,31 @@
 #include "kira_hip.h"
 
 namespace kira {
+
+// An allocator whose value-less construct() default-initialises, so that
+// resize() leaves plain-data elements untouched: a large array's pages are
+// then first touched (faulted in) by whoever fills it, in parallel, instead
+// of being zeroed by resize() on one thread.
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <class U>
+    DefaultInitAlloc(const DefaultInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using PodVec = std::vector<T, DefaultInitAlloc<T>>;
 
 // Genome / window text as bit planes: base p lives at bit (p & 31) of word
 // (p >> 5) + kPadFront of each plane.  lo/hi = the two bits of the 2-bit code
@@ -705,7 +732,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                         int64_t reads_bytes, const uint8_t *d_reads, const int64_t *read_off,
                         const int32_t *read_len, const kira_chain *chains,
                         const kira_sw_cfg *cfg, const kira_fast_knobs &kn,
-                        std::vector<kira_fast_aln> &res, std::vector<uint32_t> &cig,
+                        PodVec<kira_fast_aln> &res, std::vector<uint32_t> &cig,
                         std::string &md, kira_fast_stats &stats, bool want_md);
 
 }  // namespace kira

@@ -56,6 +56,32 @@ struct MdText {
             nm++;
         }
     }
+    // n aligned columns of bytes inside both sequences.  Eight equal bytes
+    // that are all upper-case A / C / G / T extend the match run at once
+    // (column() would count each as a match, never as an edit).
+    void columns(const uint8_t *q, const uint8_t *r, size_t n) {
+        size_t t = 0;
+        for (; t + 8 <= n; t += 8) {
+            uint64_t a, b;
+            memcpy(&a, q + t, 8);
+            memcpy(&b, r + t, 8);
+            if (a == b && acgt_bytes(a) == 0x8080808080808080ull) {
+                run += 8;
+                continue;
+            }
+            for (size_t u = t; u < t + 8; u++) column(q[u], r[u]);
+        }
+        for (; t < n; t++) column(q[t], r[t]);
+    }
+    // 0x80 in each byte of x that is 'A', 'C', 'G' or 'T', 0 in the others
+    static uint64_t acgt_bytes(uint64_t x) {
+        auto eq = [](uint64_t v, uint64_t c) {
+            const uint64_t y = v ^ (0x0101010101010101ull * c);
+            const uint64_t t = ((y & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | y;
+            return ~t & 0x8080808080808080ull;
+        };
+        return eq(x, 'A') | eq(x, 'C') | eq(x, 'G') | eq(x, 'T');
+    }
     void deletion(const uint8_t *ref, size_t n) {
         if (n == 0) return;
         flush();
@@ -73,6 +99,12 @@ void replay(const uint8_t *read, size_t read_len, const uint8_t *ref, size_t ref
         const size_t len = cig[k] >> 4;
         switch (cig[k] & 15u) {
         case kOpM:
+            if (q + len <= read_len && r + len <= ref_len) {
+                md.columns(read + q, ref + r, len);
+                q += len;
+                r += len;
+                break;
+            }
             for (size_t t = 0; t < len; t++, q++, r++)
                 md.column(q < read_len ? read[q] : (uint8_t)'N', r < ref_len ? ref[r] : (uint8_t)'N');
             break;
@@ -203,7 +235,7 @@ bool left_normalize(std::vector<uint32_t> &ops, const uint8_t *ref, int64_t ref_
 
 uint32_t ungapped_tags(const uint8_t *read, const uint8_t *ref, int64_t n, std::string &md_out) {
     MdText md(md_out);
-    for (int64_t k = 0; k < n; k++) md.column(read[k], ref[k]);
+    if (n > 0) md.columns(read, ref, (size_t)n);
     md.flush();
     return md.nm;
 }

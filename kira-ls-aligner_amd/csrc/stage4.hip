@@ -22,8 +22,8 @@
 
 struct kira_stage4_result {
     std::vector<int64_t> aln_off;
-    std::vector<kira_stage4_aln> alns;
-    std::vector<uint32_t> cigar;
+    kira::PodVec<kira_stage4_aln> alns;
+    kira::PodVec<uint32_t> cigar;
     std::string md;
     kira_stage4_stats stats{};
     double phase_ms[5] = {};
@@ -170,6 +170,7 @@ struct Job {
     int32_t wlen;
     int32_t abort;
 };
+using JobVec = kira::PodVec<Job>;
 
 struct Driver {
     kira_ctx *ctx;
@@ -292,7 +293,7 @@ struct Driver {
     // Banded SW over device-resident reads and windows (offset: each job's
     // band centre offset), launched on the stream; run.fetch() returns the
     // results and CIGARs in job order.
-    kira_status banded_launch(kira::BandedRun &run, const std::vector<Job> &jobs,
+    kira_status banded_launch(kira::BandedRun &run, const JobVec &jobs,
                               const std::vector<int32_t> &offset, int slot) {
         const int32_t m = (int32_t)jobs.size();
         if (m == 0) return KIRA_OK;
@@ -329,7 +330,7 @@ struct Driver {
 
     // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
     // (align_oriented_chain_with_meta, mod.rs:867-869)
-    std::vector<int32_t> chain_offsets(const std::vector<Job> &jobs) const {
+    std::vector<int32_t> chain_offsets(const JobVec &jobs) const {
         std::vector<int32_t> off(jobs.size());
         for (size_t t = 0; t < jobs.size(); t++) {
             const kira_chain &ch = chains[jobs[t].chain];
@@ -398,7 +399,7 @@ kira_status Driver::run() {
     // chain selection (:549-708, mate guidance off), in two passes: the
     // two-tier candidate search of uncertain short reads (:596-627) runs its
     // fast cascades for the whole batch at once, then each read selects
-    std::vector<Sel> sels;
+    kira::PodVec<Sel> sels;
     std::vector<int32_t> chain_best(n, 0), chain_second(n, 0), min_score_of(n, 0),
         topk_of(n, 1), margin_of(n, INT32_MAX);
     std::vector<uint8_t> has_comp(n, 0);
@@ -460,7 +461,7 @@ kira_status Driver::run() {
     kira::trace_mark("select1");
     // exact copies score A * L (exact_match_exists, mod.rs:818-832); the rest
     // go through the fast cascade together
-    std::vector<kira_fast_aln> tt_res;
+    kira::PodVec<kira_fast_aln> tt_res;
     std::vector<uint32_t> tt_cig;
     std::string tt_md;
     {
@@ -747,7 +748,7 @@ kira_status Driver::run() {
 
     kira::trace_mark("pf_done");
     // outcomes in chain order (:758-934): 0 nothing, 1 a record here, 2 fallback
-    std::vector<Job> simd, scalar;
+    JobVec simd, scalar;
     std::vector<uint8_t> dp_used(n, 0), outcome(ns, 0);
     std::vector<int32_t> abort_of(ns, 0);
     std::vector<int32_t> accepted;  // selected chains that end in a record here
@@ -808,7 +809,7 @@ kira_status Driver::run() {
     // without the GPU dispatcher, a fallback chain tries the fast DP cascade
     // first (try_fast_dp_alignment, :868-882; accept_fast_result, :289-299)
     std::vector<int32_t> fast_of(ns, -1);
-    std::vector<kira_fast_aln> fres;
+    kira::PodVec<kira_fast_aln> fres;
     std::vector<uint32_t> fcig;
     std::string fmd;
     if (c.cpu_mode) {
@@ -843,7 +844,7 @@ kira_status Driver::run() {
         // route every chain (0 none, 1 a record here, 2 lane job, 3 scalar
         // job) in parallel, then place each list's items in chain order
         std::vector<uint8_t> route(ns);
-        std::vector<Job> job_of(ns);
+        JobVec job_of(ns);
         const int T = host_threads();
         std::vector<std::array<int64_t, 4>> cnt((size_t)T, std::array<int64_t, 4>{});
         const int64_t kGrain = 8192;
@@ -979,38 +980,51 @@ kira_status Driver::run() {
     const int32_t nj = (int32_t)simd.size();
     std::vector<int32_t> cert, sv_shift, sv_best;
     if (nj > 0 && !c.cpu_mode) {
-        std::vector<int64_t> ro(nj), ws(nj);
-        std::vector<int32_t> rl(nj), wl(nj), lim(nj);
+        // job arrays side by side in one pinned block, one copy
+        const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t o_ws = al((size_t)nj * 8), o_rl = o_ws + al((size_t)nj * 8),
+                     o_wl = o_rl + al((size_t)nj * 4), o_lim = o_wl + al((size_t)nj * 4),
+                     o_end = o_lim + al((size_t)nj * 4);
+        kira::Pinned<uint8_t> hb;
+        if ((e = hb.resize(ctx, kira::kPinSieve, o_end)))
+            return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned sieve staging", e);
+        int64_t *ro = (int64_t *)hb.data(), *ws = (int64_t *)(hb.data() + o_ws);
+        int32_t *rl = (int32_t *)(hb.data() + o_rl), *wl = (int32_t *)(hb.data() + o_wl),
+                *lim = (int32_t *)(hb.data() + o_lim);
+        std::vector<int32_t> part_max((size_t)host_threads(), 0);
+        parallel_chunks(nj, 16384, [&](int tt, int64_t b0, int64_t e0) {
+            int32_t mx = 0;
+            for (int64_t t = b0; t < e0; t++) {
+                const Job &j = simd[t];
+                const kira_chain &ch = chains[j.chain];
+                ro[t] = oriented_off(j.read, ch.strand == 1);
+                rl[t] = read_len[j.read];
+                ws[t] = contig_off[ch.ref_id] + j.win;
+                wl[t] = j.wlen;
+                // spectral_max_mismatches (router.rs:191-195)
+                lim[t] = std::max<int32_t>((int32_t)((int64_t)rl[t] * c.spectral_pct / 100),
+                                           c.spectral_floor);
+                mx = std::max(mx, rl[t]);
+            }
+            part_max[(size_t)tt] = mx;
+        });
         int32_t max_len = 0;
-        for (int32_t t = 0; t < nj; t++) {
-            const Job &j = simd[t];
-            const kira_chain &ch = chains[j.chain];
-            ro[t] = oriented_off(j.read, ch.strand == 1);
-            rl[t] = read_len[j.read];
-            ws[t] = contig_off[ch.ref_id] + j.win;
-            wl[t] = j.wlen;
-            // spectral_max_mismatches (router.rs:191-195)
-            lim[t] = std::max<int32_t>((int32_t)((int64_t)rl[t] * c.spectral_pct / 100),
-                                       c.spectral_floor);
-            max_len = std::max(max_len, rl[t]);
-        }
+        for (int32_t v : part_max) max_len = std::max(max_len, v);
         DevMem dm(ctx);
-        int64_t *d_ro, *d_ws;
-        int32_t *d_rl, *d_wl, *d_lim, *d_shift, *d_best, *d_second;
+        uint8_t *d_blob;
+        int32_t *d_shift, *d_best, *d_second;
         uint32_t *d_planes;
         uint8_t *d_valid, *d_flags;
-        if ((e = dm.alloc(&d_ro, nj)) || (e = dm.alloc(&d_ws, nj)) || (e = dm.alloc(&d_rl, nj)) ||
-            (e = dm.alloc(&d_wl, nj)) || (e = dm.alloc(&d_lim, nj)) ||
-            (e = dm.alloc(&d_shift, nj)) || (e = dm.alloc(&d_best, nj)) ||
-            (e = dm.alloc(&d_second, nj)) || (e = dm.alloc(&d_planes, (size_t)nj * 24)) ||
-            (e = dm.alloc(&d_valid, nj)) || (e = dm.alloc(&d_flags, nj)))
+        if ((e = dm.alloc(&d_blob, o_end)) || (e = dm.alloc(&d_shift, nj)) ||
+            (e = dm.alloc(&d_best, nj)) || (e = dm.alloc(&d_second, nj)) ||
+            (e = dm.alloc(&d_planes, (size_t)nj * 24)) || (e = dm.alloc(&d_valid, nj)) ||
+            (e = dm.alloc(&d_flags, nj)))
             return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "sieve allocation", e);
-        auto up = [&](void *d, const void *h, size_t bytes) {
-            return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
-        };
-        if ((e = up(d_ro, ro.data(), nj * 8)) || (e = up(d_ws, ws.data(), nj * 8)) ||
-            (e = up(d_rl, rl.data(), nj * 4)) || (e = up(d_wl, wl.data(), nj * 4)) ||
-            (e = up(d_lim, lim.data(), nj * 4)))
+        const int64_t *d_ro = (const int64_t *)d_blob, *d_ws = (const int64_t *)(d_blob + o_ws);
+        const int32_t *d_rl = (const int32_t *)(d_blob + o_rl),
+                      *d_wl = (const int32_t *)(d_blob + o_wl),
+                      *d_lim = (const int32_t *)(d_blob + o_lim);
+        if ((e = hipMemcpyAsync(d_blob, hb.data(), o_end, hipMemcpyHostToDevice, s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
         if ((e = kira::launch_pack_reads_ascii(nj, d_orient, 0, d_ro, d_rl, d_planes, d_valid, s)))
             return fail(ctx, KIRA_ERR_RUNTIME, "read packing", e);
@@ -1188,14 +1202,14 @@ kira_status Driver::run() {
     t0 = Clock::now();
     {
         const int32_t m = (int32_t)lane_order.size();
-        std::vector<kira_sw_result> out(m);
+        kira::PodVec<kira_sw_result> out(m);
         std::vector<uint32_t> cig;
         kira::BandedRun *br = &lane_banded;
         for (int pass = 0; pass < 2; pass++) {
             const std::vector<int32_t> &sel = lane_sel[pass];
             const int32_t ms = (int32_t)sel.size();
             if (ms == 0) continue;
-            std::vector<kira_sw_result> r(ms);
+            kira::PodVec<kira_sw_result> r(ms);
             kira_status kst = pass == 0 ? br->fetch(r.data(), cig) : lane_groups.fetch(r.data(), cig);
             if (kst != KIRA_OK) return kst;
             for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
@@ -1216,7 +1230,7 @@ kira_status Driver::run() {
 
     t0 = Clock::now();
     {
-        std::vector<kira_sw_result> res(scalar.size());
+        kira::PodVec<kira_sw_result> res(scalar.size());
         std::vector<uint32_t> cig;
         kira_status kst = scalar_run.fetch(res.data(), cig);
         if (kst != KIRA_OK) return kst;
@@ -1239,27 +1253,29 @@ kira_status Driver::run() {
     group_by_read(first, perm);
 
     // XS: the two-tier competitor's affine score, else the chain-score proxy
-    // (:1215-1250)
-    for (int32_t r = 0; r < n; r++) {
-        if (first[r + 1] - first[r] != 1) continue;
-        if (has_comp[r]) {
+    // (:1215-1250), on the reads with exactly one record before the
+    // supplementary ones are added
+    auto assign_xs = [&]() {
+        for (int32_t r = 0; r < n; r++) {
+            if (first[r + 1] - first[r] != 1) continue;
+            if (has_comp[r]) {
+                kira_stage4_aln &a = rec(perm[first[r]]);
+                a.has_xs = 1;
+                a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
+                continue;
+            }
+            const int32_t cb = chain_best[r], cs = chain_second[r];
+            if (cb <= 0 || cs <= 0) continue;
             kira_stage4_aln &a = rec(perm[first[r]]);
-            a.has_xs = 1;
-            a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
-            continue;
+            if (a.has_xs) continue;
+            const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
+            if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
+                a.has_xs = 1;
+                a.xs_score = sub;
+            }
         }
-        const int32_t cb = chain_best[r], cs = chain_second[r];
-        if (cb <= 0 || cs <= 0) continue;
-        kira_stage4_aln &a = rec(perm[first[r]]);
-        if (a.has_xs) continue;
-        const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
-        if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
-            a.has_xs = 1;
-            a.xs_score = sub;
-        }
-    }
+    };
 
-    kira::trace_mark("xs");
     // emit_supplementary_alignments (split_read.rs:22-57)
     if (c.supplementary) {
         // picks in the order split_read.rs appends them: job -1 = exact match
@@ -1271,7 +1287,7 @@ kira_status Driver::run() {
         // threads take read ranges; their picks are joined in read order
         struct Part {
             std::vector<Pick> all;
-            std::vector<Job> sj;
+            JobVec sj;
         };
         std::vector<Part> parts((size_t)host_threads());
         parallel_chunks(n, 4096, [&](int t, int64_t rb, int64_t re) {
@@ -1332,7 +1348,7 @@ kira_status Driver::run() {
             }
         });
         std::vector<Pick> all;
-        std::vector<Job> sj;
+        JobVec sj;
         for (Part &P : parts) {
             const int32_t base = (int32_t)sj.size();
             for (Pick x : P.all) {
@@ -1342,13 +1358,19 @@ kira_status Driver::run() {
             sj.insert(sj.end(), P.sj.begin(), P.sj.end());
         }
         kira::trace_mark("supp_picks");
-        std::vector<kira_sw_result> res(sj.size());
+        kira::PodVec<kira_sw_result> res(sj.size());
         std::vector<uint32_t> cig;
         kira::BandedRun supp_run(ctx);
         kira_status kst = banded_launch(supp_run, sj, chain_offsets(sj), kira::kPinBanded);
-        if (kst == KIRA_OK) kst = supp_run.fetch(res.data(), cig);
+        if (kst != KIRA_OK) return kst;
+        // XS while the device aligns the supplementary chains (it reads and
+        // sets fields the picks above do not use)
+        assign_xs();
+        kira::trace_mark("xs");
+        kst = supp_run.fetch(res.data(), cig);
         if (kst != KIRA_OK) return kst;
         kira::trace_mark("supp_sw");
+        const size_t a_supp = arenas.size();
         build((int64_t)all.size(), [&](int64_t i, Arena &A) {
             const Pick &p = all[i];
             const kira_chain &ch = chains[p.chain];
@@ -1364,7 +1386,29 @@ kira_status Driver::run() {
             A.add(p.read, a);
         });
         st.supplementary += (int64_t)all.size();
-        group_by_read(first, perm);
+        // regroup: each read's supplementary records follow its others; the
+        // picks are in read order, so the new records are too
+        std::vector<int64_t> supp_h;
+        supp_h.reserve(all.size());
+        for (size_t ai = a_supp; ai < arenas.size(); ai++)
+            for (size_t k = 0; k < arenas[ai]->recs.size(); k++) supp_h.push_back(rec_handle(ai, k));
+        std::vector<int64_t> sfirst((size_t)n + 1, 0);
+        for (const Pick &x : all) sfirst[(size_t)x.read + 1]++;
+        for (int32_t r = 0; r < n; r++) sfirst[r + 1] += sfirst[r];
+        std::vector<int64_t> first2((size_t)n + 1), perm2(perm.size() + supp_h.size());
+        for (int32_t r = 0; r <= n; r++) first2[r] = first[r] + sfirst[r];
+        parallel_chunks(n, 16384, [&](int, int64_t rb, int64_t re) {
+            for (int64_t r = rb; r < re; r++) {
+                int64_t o = first2[r];
+                for (int64_t k = first[r]; k < first[r + 1]; k++) perm2[o++] = perm[k];
+                for (int64_t k = sfirst[r]; k < sfirst[r + 1]; k++) perm2[o++] = supp_h[k];
+            }
+        });
+        first.swap(first2);
+        perm.swap(perm2);
+    } else {
+        assign_xs();
+        kira::trace_mark("xs");
     }
 
     kira::trace_mark("supp_records");
@@ -1552,7 +1596,7 @@ kira_status kira_stage4_left_normalize(kira_stage4_result *r, const kira_genome 
     try {
         // normalize_alignments (normalize.rs:197-249): every record of every
         // read; NM / MD recomputed for the records that moved
-        std::vector<uint32_t> cig;
+        kira::PodVec<uint32_t> cig;
         std::string md;
         cig.reserve(r->cigar.size());
         md.reserve(r->md.size());

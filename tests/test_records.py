@@ -69,3 +69,38 @@ def test_known_answers():
                   cigar=[(5 << 4) | 0, (3 << 4) | 1, (5 << 4) | 0])
     r, = pkg.build_alignments([(b"ACGTAGGGCGTAC", b"ACGTACGTAC", 0)], [ins_sw])
     assert (r["md"], r["nm"], r["cigar"]) == ("10", 3, "5M3I5M")
+
+
+def test_records_on_long_clean_runs_with_case_and_n():
+    """Long reads that mostly copy an upper-case A/C/G/T window (so whole
+    8-byte groups match), with lower-case stretches, N / IUPAC bytes on either
+    side and the occasional substitution or indel: the records equal the
+    oracle's build_alignment."""
+    pkg = load()
+    rng = random.Random(21)
+    jobs, sws, want = [], [], []
+    for k in range(200):
+        L = rng.randrange(40, 300)
+        flank = rng.randrange(0, 60)
+        g = bytearray(rng.choice(b"ACGT") for _ in range(L + 2 * flank + 10))
+        for _ in range(rng.randrange(0, 4)):  # lower-case and ambiguous stretches
+            a = rng.randrange(len(g))
+            for t in range(a, min(len(g), a + rng.randrange(1, 12))):
+                g[t] = rng.choice([g[t] | 0x20, ord("N"), ord("R"), g[t]])
+        g = bytes(g)
+        read = bytearray(g[flank:flank + L].upper())
+        for _ in range(rng.randrange(0, 5)):
+            i = rng.randrange(len(read))
+            read[i] = rng.choice(b"ACGTNY")
+        if rng.random() < 0.4:
+            i = rng.randrange(1, len(read) - 1)
+            del read[i:i + rng.randrange(1, 4)]
+        read = bytes(read)
+        c = O.cfg(1, 4, 6, 1, rng.choice([16, 32]), 100, rng.choice([0, 5]))
+        sw = O.banded_sw(read, g, flank, c, ABORT_NONE, O.knobs())
+        ws = rng.randrange(0, 10 ** 6)
+        jobs.append((read, g, ws))
+        sws.append(sw)
+        want.append(O.build_alignment(read, g, ws, sw))
+    got = pkg.build_alignments(jobs, sws)
+    assert got == want
