@@ -512,6 +512,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 }
             });
             lists.n = list.size();
+            trace_mark(".wfa_lists");
             DevLists d;
             if ((e = upload(d))) return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path WFA", e);
             kira::WfaDevJobs wj;

@@ -353,27 +353,33 @@ struct SwJobsHost {
 // and gather pack the CIGARs back to back into scratch from dm first, and
 // only the ops cross PCIe; they are appended to `cigar` and out[k].cigar_off
 // indexes it.  Waits for the context's stream.
-kira_status fetch_sw_results(kira_ctx *ctx, DevMem &dm, int32_t n, const int32_t *d_res,
-                             const uint32_t *d_cig, const int64_t *d_src_off, int64_t cig_words,
-                             bool banded, kira_sw_result *out, std::vector<uint32_t> &cigar);
+kira_status fetch_sw_results(kira_ctx *ctx, hipStream_t s, DevMem &dm, int32_t n,
+                             const int32_t *d_res, const uint32_t *d_cig, const int64_t *d_src_off,
+                             int64_t cig_words, bool banded, kira_sw_result *out,
+                             std::vector<uint32_t> &cigar);
 
 // A banded SW batch (banded_sw_with, src/alignment/mod.rs:1370-1650) over
 // device-resident reads and windows, in two steps: launch() lays the plan
 // out on the host, uploads its tables with the job arrays in one copy from
-// pinned buffer `slot` and queues the kernels on the context's stream, then
-// returns; fetch() waits for them and reads the results back.  Device memory
-// comes from the context's block cache and is kept until the object goes.
+// pinned buffer `slot` and queues the kernels on `stream` (the context's
+// stream when null), then returns; fetch() waits for them and reads the
+// results back.  Device memory comes from the context's block cache and is
+// kept until the object goes (which first waits for its stream).
 struct BandedRun {
     kira_ctx *ctx;
     DevMem dm;
     SwPlanLayout L;
+    hipStream_t stream = nullptr;
     int32_t n = 0;
     int32_t *d_res = nullptr;
     uint32_t *d_cig = nullptr;
     const int64_t *d_cig_off = nullptr;
     explicit BandedRun(kira_ctx *c) : ctx(c), dm(c) {}
+    BandedRun(const BandedRun &) = delete;
+    BandedRun &operator=(const BandedRun &) = delete;
+    ~BandedRun();
     kira_status launch(const SwJobsHost &j, const kira_sw_cfg &cfg, const kira_sw_knobs &kn,
-                       int slot);
+                       int slot, hipStream_t s = nullptr);
     kira_status fetch(kira_sw_result *out, std::vector<uint32_t> &cigar);
 };
 kira_status banded_sw_dev(kira_ctx *ctx, const SwJobsHost &j, const kira_sw_cfg &cfg,
@@ -470,7 +476,8 @@ enum PinSlot {
     kPinBanded = 5,   // banded SW job arrays and plan tables
     kPinBanded2 = 6,  // a second banded SW batch in flight with the first
     kPinSieve = 7,    // stage-4 sieve job arrays
-    kPinSlots = 8
+    kPinWfa = 8,      // WFA CIGAR slot tables
+    kPinSlots = 9
 };
 
 // A typed view of pinned staging buffer `slot` (PinSlot) of the context.
@@ -754,6 +761,10 @@ struct kira_ctx {
     kira::SwTuneTable tune;
     // streams the banded SW kernel classes of one plan launch fan out to
     kira::SwFanout fan;
+    // a second stream (null if it could not be made) for work that runs
+    // beside the context stream's, and the event that orders it after it
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr;
 };
 
 struct kira_genome {

@@ -3,6 +3,7 @@
 // (soft clips added), reference span (window start added) and its NM / MD
 // tags, replaying the CIGAR over the read and window with the MdBuilder
 // rules of src/alignment/md.rs:25-204.  Host code only (no device work).
+#include <emmintrin.h>
 #include <string.h>
 
 #include <string>
@@ -61,6 +62,35 @@ struct MdText {
     // (column() would count each as a match, never as an edit).
     void columns(const uint8_t *q, const uint8_t *r, size_t n) {
         size_t t = 0;
+        // sixteen at a time: equal bytes that are each one of A, C, G, T
+        const __m128i cA = _mm_set1_epi8('A'), cC = _mm_set1_epi8('C'),
+                      cG = _mm_set1_epi8('G'), cT = _mm_set1_epi8('T');
+        for (; t + 16 <= n; t += 16) {
+            const __m128i a = _mm_loadu_si128((const __m128i *)(q + t));
+            const __m128i b = _mm_loadu_si128((const __m128i *)(r + t));
+            const __m128i acgt = _mm_or_si128(
+                _mm_or_si128(_mm_cmpeq_epi8(a, cA), _mm_cmpeq_epi8(a, cC)),
+                _mm_or_si128(_mm_cmpeq_epi8(a, cG), _mm_cmpeq_epi8(a, cT)));
+            // bit u: byte u is a plain match; runs of them extend the match
+            // run at once, the other bytes take column()
+            const unsigned m =
+                (unsigned)_mm_movemask_epi8(_mm_and_si128(_mm_cmpeq_epi8(a, b), acgt));
+            if (m == 0xFFFFu) {
+                run += 16;
+                continue;
+            }
+            for (unsigned u = 0; u < 16;) {
+                const unsigned rest = m >> u;
+                if (rest & 1u) {
+                    const unsigned k = (unsigned)__builtin_ctz(~rest);  // <= 16 - u
+                    run += k;
+                    u += k;
+                } else {
+                    column(q[t + u], r[t + u]);
+                    u++;
+                }
+            }
+        }
         for (; t + 8 <= n; t += 8) {
             uint64_t a, b;
             memcpy(&a, q + t, 8);

@@ -103,6 +103,13 @@ kira_status kira_ctx_create(int device, kira_ctx **out) {
     ctx->stream = ctx->own_stream;
     // without its streams a plan launch keeps every class on one stream
     (void)kira::sw_fanout_create(ctx->fan);
+    // without a side stream every launch stays on the context stream
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        ctx->side = nullptr;
+        ctx->side_fork = nullptr;
+    }
     // the no-allocation entry point's staging array: its device address is
     // looked up here, once per device (a failure is reported by that entry
     // point, which needs the address)
@@ -115,6 +122,7 @@ void kira_release_buffers(kira_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (int k = 0; k < 16; k++) {
         if (ctx->buf[k]) (void)hipFree(ctx->buf[k]);
         ctx->buf[k] = nullptr;
@@ -134,6 +142,8 @@ void kira_ctx_destroy(kira_ctx *ctx) {
     if (!ctx) return;
     kira_release_buffers(ctx);
     kira::sw_fanout_destroy(ctx->fan);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

@@ -210,9 +210,18 @@ struct Driver {
     template <class F>
     void build(int64_t n, F &&f) {
         if (n <= 0) return;
+        build_at(reserve_arenas(), n, f);
+    }
+    // Places for one build() call's records after every record so far, to
+    // be filled later by build_at (so records can be made out of order).
+    size_t reserve_arenas() {
         const size_t a0 = arenas.size();
-        const int T = host_threads();
-        for (int t = 0; t < T; t++) arenas.emplace_back(new Arena());
+        for (int t = 0; t < host_threads(); t++) arenas.emplace_back(new Arena());
+        return a0;
+    }
+    template <class F>
+    void build_at(size_t a0, int64_t n, F &&f) {
+        if (n <= 0) return;
         parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
             Arena &A = *arenas[a0 + (size_t)t];
             // about one record of a few CIGAR ops and MD characters an item
@@ -294,23 +303,26 @@ struct Driver {
     // band centre offset), launched on the stream; run.fetch() returns the
     // results and CIGARs in job order.
     kira_status banded_launch(kira::BandedRun &run, const JobVec &jobs,
-                              const std::vector<int32_t> &offset, int slot) {
+                              const kira::PodVec<int32_t> &offset, int slot,
+                              hipStream_t stream = nullptr) {
         const int32_t m = (int32_t)jobs.size();
         if (m == 0) return KIRA_OK;
-        std::vector<int64_t> ro(m), fo(m);
-        std::vector<int32_t> rl(m), fl(m), ab(m);
-        for (int32_t t = 0; t < m; t++) {
-            const Job &j = jobs[t];
-            const kira_chain &ch = chains[j.chain];
-            ro[t] = oriented_off(j.read, ch.strand == 1);
-            rl[t] = read_len[j.read];
-            fo[t] = contig_off[ch.ref_id] + j.win;
-            fl[t] = j.wlen;
-            ab[t] = j.abort;
-        }
+        kira::PodVec<int64_t> ro(m), fo(m);
+        kira::PodVec<int32_t> rl(m), fl(m), ab(m);
+        parallel_chunks(m, 16384, [&](int, int64_t b0, int64_t e0) {
+            for (int64_t t = b0; t < e0; t++) {
+                const Job &j = jobs[t];
+                const kira_chain &ch = chains[j.chain];
+                ro[t] = oriented_off(j.read, ch.strand == 1);
+                rl[t] = read_len[j.read];
+                fo[t] = contig_off[ch.ref_id] + j.win;
+                fl[t] = j.wlen;
+                ab[t] = j.abort;
+            }
+        });
         kira::SwJobsHost sj{m,         d_orient,  ro.data(),     rl.data(), g->ascii,
                             fo.data(), fl.data(), offset.data(), nullptr,   ab.data()};
-        return run.launch(sj, c.cfg, c.knobs, slot);
+        return run.launch(sj, c.cfg, c.knobs, slot, stream);
     }
 
     // Groups the records by read without moving them: read r's records are
@@ -330,13 +342,15 @@ struct Driver {
 
     // chain.ref_start as i32 - win_start as i32 - chain.read_start as i32
     // (align_oriented_chain_with_meta, mod.rs:867-869)
-    std::vector<int32_t> chain_offsets(const JobVec &jobs) const {
-        std::vector<int32_t> off(jobs.size());
-        for (size_t t = 0; t < jobs.size(); t++) {
-            const kira_chain &ch = chains[jobs[t].chain];
-            off[t] = wrap_i32((int64_t)(int32_t)ch.ref_start - (int32_t)(uint32_t)jobs[t].win -
-                              (int32_t)ch.read_start);
-        }
+    kira::PodVec<int32_t> chain_offsets(const JobVec &jobs) const {
+        kira::PodVec<int32_t> off(jobs.size());
+        parallel_chunks((int64_t)jobs.size(), 16384, [&](int, int64_t b0, int64_t e0) {
+            for (int64_t t = b0; t < e0; t++) {
+                const kira_chain &ch = chains[jobs[t].chain];
+                off[t] = wrap_i32((int64_t)(int32_t)ch.ref_start - (int32_t)(uint32_t)jobs[t].win -
+                                  (int32_t)ch.read_start);
+            }
+        });
         return off;
     }
 
@@ -1098,6 +1112,11 @@ kira_status Driver::run() {
     // any group are launched, then the scalar jobs, and the host builds the
     // accepted and certified records while the device runs them
     t0 = Clock::now();
+    // the side stream (scalar jobs, below) starts after what the context
+    // stream holds now, not after the lane groups
+    const bool use_side = ctx->side && !scalar.empty();
+    if (use_side && (e = hipEventRecord(ctx->side_fork, s)) != hipSuccess)
+        return fail(ctx, KIRA_ERR_RUNTIME, "side stream", e);
     std::vector<int32_t> lane_order;  // lane jobs (indices into simd) in group order
     kira::GroupRun lane_groups(ctx);
     kira::BandedRun lane_banded(ctx), scalar_run(ctx);
@@ -1156,21 +1175,23 @@ kira_status Driver::run() {
                 if ((group_of[i] >= 0) == (pass == 1)) sel.push_back(i);
             const int32_t ms = (int32_t)sel.size();
             if (ms == 0) continue;
-            std::vector<int64_t> ro(ms), fo(ms);
-            std::vector<int32_t> srl(ms), sfl(ms), off(ms), ab(ms), gof(ms);
-            for (int32_t t = 0; t < ms; t++) {
-                const Job &j = simd[lane_order[sel[t]]];
-                const kira_chain &ch = chains[j.chain];
-                ro[t] = oriented_off(j.read, ch.strand == 1);
-                srl[t] = read_len[j.read];
-                fo[t] = contig_off[ch.ref_id] + j.win;
-                sfl[t] = j.wlen;
-                int64_t o = (int64_t)ch.ref_start - j.win - (int64_t)ch.read_start;
-                o = std::max<int64_t>(o, -(int64_t)srl[t]);
-                off[t] = (int32_t)std::min<int64_t>(o, (int64_t)j.wlen);
-                ab[t] = j.abort;
-                gof[t] = group_of[sel[t]];
-            }
+            kira::PodVec<int64_t> ro(ms), fo(ms);
+            kira::PodVec<int32_t> srl(ms), sfl(ms), off(ms), ab(ms), gof(ms);
+            parallel_chunks(ms, 16384, [&](int, int64_t b0, int64_t e0) {
+                for (int64_t t = b0; t < e0; t++) {
+                    const Job &j = simd[lane_order[sel[t]]];
+                    const kira_chain &ch = chains[j.chain];
+                    ro[t] = oriented_off(j.read, ch.strand == 1);
+                    srl[t] = read_len[j.read];
+                    fo[t] = contig_off[ch.ref_id] + j.win;
+                    sfl[t] = j.wlen;
+                    int64_t o = (int64_t)ch.ref_start - j.win - (int64_t)ch.read_start;
+                    o = std::max<int64_t>(o, -(int64_t)srl[t]);
+                    off[t] = (int32_t)std::min<int64_t>(o, (int64_t)j.wlen);
+                    ab[t] = j.abort;
+                    gof[t] = group_of[sel[t]];
+                }
+            });
             kira::SwJobsHost sj{ms,        d_orient,   ro.data(),  srl.data(), g->ascii,
                                 fo.data(), sfl.data(), off.data(), nullptr,    ab.data()};
             kira_status kst =
@@ -1181,10 +1202,18 @@ kira_status Driver::run() {
     }
     kira::trace_mark("lane_launch");
     double lane_ms = ms_since(t0);
-    // scalar jobs: align_chain_with_meta (stage4:1196-1205)
+    // scalar jobs: align_chain_with_meta (stage4:1196-1205), on the side
+    // stream beside the lane groups (which leave most of the chip idle)
     t0 = Clock::now();
     {
-        kira_status kst = banded_launch(scalar_run, scalar, chain_offsets(scalar), kira::kPinBanded2);
+        hipStream_t ss = s;
+        if (use_side) {
+            if ((e = hipStreamWaitEvent(ctx->side, ctx->side_fork, 0)) != hipSuccess)
+                return fail(ctx, KIRA_ERR_RUNTIME, "side stream", e);
+            ss = ctx->side;
+        }
+        kira_status kst =
+            banded_launch(scalar_run, scalar, chain_offsets(scalar), kira::kPinBanded2, ss);
         if (kst != KIRA_OK) return kst;
     }
     kira::trace_mark("scalar_launch");
@@ -1199,35 +1228,9 @@ kira_status Driver::run() {
     kira::trace_mark("cert_records");
     R.phase_ms[1] += ms_since(t0);
 
-    t0 = Clock::now();
-    {
-        const int32_t m = (int32_t)lane_order.size();
-        kira::PodVec<kira_sw_result> out(m);
-        std::vector<uint32_t> cig;
-        kira::BandedRun *br = &lane_banded;
-        for (int pass = 0; pass < 2; pass++) {
-            const std::vector<int32_t> &sel = lane_sel[pass];
-            const int32_t ms = (int32_t)sel.size();
-            if (ms == 0) continue;
-            kira::PodVec<kira_sw_result> r(ms);
-            kira_status kst = pass == 0 ? br->fetch(r.data(), cig) : lane_groups.fetch(r.data(), cig);
-            if (kst != KIRA_OK) return kst;
-            for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
-        }
-        kira::trace_mark("lane_sw");
-        build(m, [&](int64_t i, Arena &A) {
-            const Job &j = simd[lane_order[i]];
-            A.add(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
-                                 cig.data() + out[i].cigar_off, A));
-        });
-        for (int32_t i = 0; i < m; i++) {
-            st.dp_simd++;
-            if (out[i].early_abort) st.dp_early_abort++;
-        }
-    }
-    kira::trace_mark("lane_records");
-    R.phase_ms[2] = lane_ms + ms_since(t0);
-
+    // places for the lane-group records, then the scalar ones; the scalar
+    // jobs (on the side stream) usually finish first and are built first
+    const size_t lane_arenas = reserve_arenas(), scalar_arenas = reserve_arenas();
     t0 = Clock::now();
     {
         kira::PodVec<kira_sw_result> res(scalar.size());
@@ -1235,7 +1238,7 @@ kira_status Driver::run() {
         kira_status kst = scalar_run.fetch(res.data(), cig);
         if (kst != KIRA_OK) return kst;
         kira::trace_mark("scalar_sw");
-        build((int64_t)scalar.size(), [&](int64_t t, Arena &A) {
+        build_at(scalar_arenas, (int64_t)scalar.size(), [&](int64_t t, Arena &A) {
             const Job &j = scalar[t];
             A.add(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, res[t],
                                  cig.data() + res[t].cigar_off, A));
@@ -1249,6 +1252,35 @@ kira_status Driver::run() {
     R.phase_ms[3] = scalar_ms + ms_since(t0);
 
     t0 = Clock::now();
+    {
+        const int32_t m = (int32_t)lane_order.size();
+        kira::PodVec<kira_sw_result> out(m);
+        std::vector<uint32_t> cig;
+        for (int pass = 0; pass < 2; pass++) {
+            const std::vector<int32_t> &sel = lane_sel[pass];
+            const int32_t ms = (int32_t)sel.size();
+            if (ms == 0) continue;
+            kira::PodVec<kira_sw_result> r(ms);
+            kira_status kst =
+                pass == 0 ? lane_banded.fetch(r.data(), cig) : lane_groups.fetch(r.data(), cig);
+            if (kst != KIRA_OK) return kst;
+            for (int32_t t = 0; t < ms; t++) out[sel[t]] = r[t];
+        }
+        kira::trace_mark("lane_sw");
+        build_at(lane_arenas, m, [&](int64_t i, Arena &A) {
+            const Job &j = simd[lane_order[i]];
+            A.add(j.read, sw_aln(j.read, chains[j.chain], j.win, j.wlen, out[i],
+                                 cig.data() + out[i].cigar_off, A));
+        });
+        for (int32_t i = 0; i < m; i++) {
+            st.dp_simd++;
+            if (out[i].early_abort) st.dp_early_abort++;
+        }
+    }
+    kira::trace_mark("lane_records");
+    R.phase_ms[2] = lane_ms + ms_since(t0);
+
+    t0 = Clock::now();
     std::vector<int64_t> first, perm;
     group_by_read(first, perm);
 
@@ -1256,24 +1288,26 @@ kira_status Driver::run() {
     // (:1215-1250), on the reads with exactly one record before the
     // supplementary ones are added
     auto assign_xs = [&]() {
-        for (int32_t r = 0; r < n; r++) {
-            if (first[r + 1] - first[r] != 1) continue;
-            if (has_comp[r]) {
+        parallel_chunks(n, 16384, [&](int, int64_t rb, int64_t re) {
+            for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
+                if (first[r + 1] - first[r] != 1) continue;
+                if (has_comp[r]) {
+                    kira_stage4_aln &a = rec(perm[first[r]]);
+                    a.has_xs = 1;
+                    a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
+                    continue;
+                }
+                const int32_t cb = chain_best[r], cs = chain_second[r];
+                if (cb <= 0 || cs <= 0) continue;
                 kira_stage4_aln &a = rec(perm[first[r]]);
-                a.has_xs = 1;
-                a.xs_score = std::min(std::max(comp_of[r], 0), std::max(a.score, 0));
-                continue;
+                if (a.has_xs) continue;
+                const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
+                if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
+                    a.has_xs = 1;
+                    a.xs_score = sub;
+                }
             }
-            const int32_t cb = chain_best[r], cs = chain_second[r];
-            if (cb <= 0 || cs <= 0) continue;
-            kira_stage4_aln &a = rec(perm[first[r]]);
-            if (a.has_xs) continue;
-            const int32_t sub = wrap_i32((int64_t)a.score * cs / cb);
-            if (sub > 0 && (int64_t)cs * 100 >= (int64_t)cb * c.xs_min_ratio_pct) {
-                a.has_xs = 1;
-                a.xs_score = sub;
-            }
-        }
+        });
     };
 
     // emit_supplementary_alignments (split_read.rs:22-57)

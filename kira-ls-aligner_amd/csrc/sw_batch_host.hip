@@ -240,8 +240,8 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
 }
 
 kira_status GroupRun::fetch(kira_sw_result *out, std::vector<uint32_t> &cigar) {
-    const kira_status st =
-        fetch_sw_results(ctx, dm, n, d_res, d_cig, d_cig_off, cig_words, false, out, cigar);
+    const kira_status st = fetch_sw_results(ctx, ctx->stream, dm, n, d_res, d_cig, d_cig_off,
+                                            cig_words, false, out, cigar);
     trace_mark(".grp_fetch");
     return st;
 }

@@ -383,30 +383,43 @@ const char *sw_plan_layout(int32_t n, const int32_t *read_len, const int32_t *ba
     }
     L.job_slot.assign(n, 0);
     for (int c = 0; c < kNumClasses; c++) L.class_first[c] = -1;
-    for (int32_t s = 0; s < n;) {
+    {
+        // packets: each class's jobs in order, class_jobs_per_packet(c) a packet
+        int64_t per_class[kNumClasses] = {}, n_pk = 0;
+        for (int32_t k = 0; k < n; k++) per_class[cls[k]]++;
+        for (int c = 0; c < kNumClasses; c++)
+            n_pk += (per_class[c] + class_jobs_per_packet(c) - 1) / class_jobs_per_packet(c);
+        L.packet_jobs.resize((size_t)n_pk * 64);
+        L.packet_rows.resize((size_t)n_pk);
+        L.packet_rowdw.resize((size_t)n_pk);
+        L.packet_trace_off.resize((size_t)n_pk);
+        L.packet_meta_off.resize((size_t)n_pk);
+    }
+    int32_t pk = 0;
+    for (int32_t s = 0; s < n; pk++) {
         int c = cls[order[s]];
         const int per = class_jobs_per_packet(c), stride = 64 / per;
         int32_t e = s;
         while (e < n && e - s < per && cls[order[e]] == c) e++;
-        int32_t pk = (int32_t)L.packet_rows.size();
         if (L.class_first[c] < 0) L.class_first[c] = pk;
         L.class_count[c]++;
         int32_t rows = read_len[order[s]];
         int32_t rowdw = class_rowdw(c);
+        int32_t *pj = L.packet_jobs.data() + (size_t)pk * 64;
         for (int32_t l = 0; l < 64; l++) {
             int32_t job = (l % stride == 0 && s + l / stride < e) ? order[s + l / stride] : -1;
-            L.packet_jobs.push_back(job);
+            pj[l] = job;
             if (job >= 0) L.job_slot[job] = pk * 64 + l;
         }
-        L.packet_rows.push_back(rows);
-        L.packet_rowdw.push_back(rowdw);
-        L.packet_trace_off.push_back(L.trace_words);
-        L.packet_meta_off.push_back(L.meta_rows);
+        L.packet_rows[pk] = rows;
+        L.packet_rowdw[pk] = rowdw;
+        L.packet_trace_off[pk] = L.trace_words;
+        L.packet_meta_off[pk] = L.meta_rows;
         L.trace_words += (int64_t)rows * rowdw * 64;
         L.meta_rows += (int64_t)rows * 64;
         s = e;
     }
-    L.n_packets = (int32_t)L.packet_rows.size();
+    L.n_packets = pk;
     L.cigar_cap.resize(n);
     L.cigar_off.resize(n);
     for (int32_t k = 0; k < n; k++) {
@@ -708,11 +721,11 @@ kira_status kira_banded_sw_run_dev(kira_ctx *ctx, kira_sw_plan *plan,
 
 namespace kira {
 
-kira_status fetch_sw_results(kira_ctx *ctx, DevMem &dm, int32_t n, const int32_t *d_res,
-                             const uint32_t *d_cig, const int64_t *d_src_off, int64_t cig_words,
-                             bool banded, kira_sw_result *out, std::vector<uint32_t> &cigar) {
+kira_status fetch_sw_results(kira_ctx *ctx, hipStream_t s, DevMem &dm, int32_t n,
+                             const int32_t *d_res, const uint32_t *d_cig, const int64_t *d_src_off,
+                             int64_t cig_words, bool banded, kira_sw_result *out,
+                             std::vector<uint32_t> &cigar) {
     if (n == 0) return KIRA_OK;
-    hipStream_t s = ctx->stream;
     hipError_t e;
     int64_t *d_dst = nullptr;
     uint32_t *d_out = nullptr;
@@ -756,8 +769,13 @@ namespace {
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 }  // namespace
 
+BandedRun::~BandedRun() {
+    if (ctx && stream && stream != ctx->stream) (void)hipStreamSynchronize(stream);
+}
+
 kira_status BandedRun::launch(const SwJobsHost &j, const kira_sw_cfg &cfg,
-                              const kira_sw_knobs &kn, int slot) {
+                              const kira_sw_knobs &kn, int slot, hipStream_t s_in) {
+    stream = s_in ? s_in : ctx->stream;
     n = j.n;
     if (n == 0) return KIRA_OK;
     if (const char *why = sw_plan_layout(n, j.read_len, j.band, cfg, L, &ctx->tune))
@@ -787,7 +805,7 @@ kira_status BandedRun::launch(const SwJobsHost &j, const kira_sw_cfg &cfg,
         (e = dm.alloc(&d_res, (size_t)n * kSwResWords)) ||
         (e = dm.alloc(&d_cig, (size_t)std::max<int64_t>(L.cigar_words, 1))))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "banded SW allocation", e);
-    hipStream_t s = ctx->stream;
+    hipStream_t s = stream;
     if ((e = hipMemcpyAsync(d_blob, h, total, hipMemcpyHostToDevice, s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "host to device copy", e);
     const SwPlanDev d = sw_plan_bind(L, d_blob, d_slab);
@@ -810,8 +828,8 @@ kira_status BandedRun::launch(const SwJobsHost &j, const kira_sw_cfg &cfg,
 }
 
 kira_status BandedRun::fetch(kira_sw_result *out, std::vector<uint32_t> &cigar) {
-    const kira_status st =
-        fetch_sw_results(ctx, dm, n, d_res, d_cig, d_cig_off, L.cigar_words, true, out, cigar);
+    const kira_status st = fetch_sw_results(ctx, stream ? stream : ctx->stream, dm, n, d_res, d_cig,
+                                            d_cig_off, L.cigar_words, true, out, cigar);
     trace_mark(".bsw_fetch");
     return st;
 }

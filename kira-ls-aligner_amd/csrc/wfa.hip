@@ -725,6 +725,9 @@ hipError_t slab_for(kira_ctx *ctx, size_t need, int32_t **p) {
             ctx->cap[kSlabSlot] = 0;
             if (e != hipSuccess) return e;
         }
+        // a quarter more than asked, so batches of slowly growing shapes do
+        // not free and allocate (each hipFree waits for the device) every time
+        need += need / 4;
         hipError_t e = hipMalloc(&ctx->buf[kSlabSlot], need);
         if (e != hipSuccess) {
             ctx->buf[kSlabSlot] = nullptr;
@@ -855,6 +858,7 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    trace_mark(".wfa_loops");
     int64_t val_cap = wfa_slot_values(pen, s_max, m_max, n_max, tbf_max);
     val_cap = (val_cap + 3) & ~(int64_t)3;
     const int64_t slot_ints = val_cap + 12 * (int64_t)(s_max + 1);
