@@ -392,8 +392,10 @@ def main():
     ap.add_argument("--fastpath-only", action="store_true",
                     help="run only the fast DP cascade section (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-budget-gb", type=int, default=96, help="GiB")
+    # 8 requests in flight in a 192 GiB pool (two thirds of the 288 GB of HBM):
+    # 4 slots in 96 GiB gave 0.87-0.91 M reads/s, 8 in 192 GiB 1.0-1.06 M
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=int, default=192, help="GiB")
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--calib-copy-gb", type=float, default=0.0,
                     help="after the timed region, run one plain device copy of this many "

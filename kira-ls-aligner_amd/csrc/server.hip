@@ -105,13 +105,14 @@ struct Request {
     int64_t ticket = 0;
     double t_submit = 0, t_start = 0;
     int32_t nr = 0, ns = 0, nw = 0, max_sieve_len = 0;
-    std::vector<uint8_t> reads;
-    std::vector<int64_t> read_off;
-    std::vector<int32_t> read_len;
-    std::vector<int32_t> s_read, s_len, s_limit;
-    std::vector<int64_t> s_start;
-    std::vector<int64_t> w_roff, w_foff;
-    std::vector<int32_t> w_rlen, w_flen, w_off, w_band, w_abort, w_skip;
+    // copies of the caller's arrays (first touched by the parallel copies)
+    kira::PodVec<uint8_t> reads;
+    kira::PodVec<int64_t> read_off;
+    kira::PodVec<int32_t> read_len;
+    kira::PodVec<int32_t> s_read, s_len, s_limit;
+    kira::PodVec<int64_t> s_start;
+    kira::PodVec<int64_t> w_roff, w_foff;
+    kira::PodVec<int32_t> w_rlen, w_flen, w_off, w_band, w_abort, w_skip;
     kira::SwPlanLayout plan;
     Layout lay{};
     bool laid_out = false;
@@ -122,10 +123,10 @@ struct Request {
 struct Result {
     int32_t status = KIRA_OK;
     std::string err;
-    std::vector<int32_t> shift, best, second;
-    std::vector<uint8_t> flags, skipped;
-    std::vector<kira_sw_result> sw;
-    std::vector<uint32_t> cigar;
+    kira::PodVec<int32_t> shift, best, second;
+    kira::PodVec<uint8_t> flags, skipped;
+    kira::PodVec<kira_sw_result> sw;
+    kira::PodVec<uint32_t> cigar;
     double t_submit = 0, t_start = 0, t_done = 0;
 };
 
@@ -351,14 +352,15 @@ void kira_server::finish(int slot, hipError_t e) {
     } else {
         const Layout &L = r.lay;
         const uint8_t *hout = hpool + r.host_off + L.in_bytes;
-        auto get32 = [&](std::vector<int32_t> &v, size_t off, size_t n) {
+        auto get32 = [&](kira::PodVec<int32_t> &v, size_t off, size_t n) {
             v.resize(n);
-            if (n) memcpy(v.data(), hout + off, n * 4);
+            kira::parallel_memcpy(v.data(), hout + off, n * 4);
         };
         get32(res->shift, L.o_shift, r.ns);
         get32(res->best, L.o_best, r.ns);
         get32(res->second, L.o_second, r.ns);
-        res->flags.assign(hout + L.o_flags, hout + L.o_flags + r.ns);
+        res->flags.resize(r.ns);
+        kira::parallel_memcpy(res->flags.data(), hout + L.o_flags, r.ns);
         const int32_t *rr = (const int32_t *)(hout + L.o_res);
         const int64_t *coff = (const int64_t *)(hout + L.o_coff);
         const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
@@ -565,7 +567,8 @@ kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *t
         if (q->read_off[k] < 0 || q->read_len[k] < 0) return bad("negative read offset / length");
     // reads are re-based to one contiguous copy
     r->read_off.resize(r->nr);
-    r->read_len.assign(q->read_len, q->read_len + r->nr);
+    r->read_len.resize(r->nr);
+    memcpy(r->read_len.data(), q->read_len, (size_t)r->nr * 4);
     int64_t pos = 0;
     for (int32_t k = 0; k < r->nr; k++) {
         r->read_off[k] = pos;
@@ -578,38 +581,79 @@ kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *t
                 memcpy(r->reads.data() + r->read_off[k], q->reads + q->read_off[k],
                        (size_t)q->read_len[k]);
     });
-    r->s_read.assign(q->sieve_read, q->sieve_read + r->ns);
-    r->s_start.assign(q->sieve_win_start, q->sieve_win_start + r->ns);
-    r->s_len.assign(q->sieve_win_len, q->sieve_win_len + r->ns);
+    // the job arrays copied and checked in parallel over job ranges
+    r->s_read.resize(r->ns);
+    r->s_start.resize(r->ns);
+    r->s_len.resize(r->ns);
     r->s_limit.resize(r->ns);
-    for (int32_t k = 0; k < r->ns; k++) {
-        if (r->s_read[k] < 0 || r->s_read[k] >= r->nr) return bad("sieve job read out of range");
-        if (r->s_start[k] < 0 || r->s_len[k] < 0 || r->s_start[k] + r->s_len[k] > G)
-            return bad("sieve window outside the genome");
-        r->s_limit[k] = q->sieve_limit ? q->sieve_limit[k] : s->cfg.sieve_limit;
-        r->max_sieve_len = std::max(r->max_sieve_len, r->read_len[r->s_read[k]]);
+    {
+        const int T = kira::host_threads();
+        std::vector<int32_t> max_len((size_t)T, 0);
+        std::vector<uint8_t> bad_read((size_t)T, 0), bad_win((size_t)T, 0);
+        kira::parallel_chunks(r->ns, 16384, [&](int t, int64_t b, int64_t e) {
+            int32_t mx = 0;
+            for (int64_t k = b; k < e; k++) {
+                const int32_t rd = q->sieve_read[k];
+                const int64_t ws = q->sieve_win_start[k];
+                const int32_t wl = q->sieve_win_len[k];
+                r->s_read[k] = rd;
+                r->s_start[k] = ws;
+                r->s_len[k] = wl;
+                r->s_limit[k] = q->sieve_limit ? q->sieve_limit[k] : s->cfg.sieve_limit;
+                if (rd < 0 || rd >= r->nr) {
+                    bad_read[(size_t)t] = 1;
+                    continue;
+                }
+                if (ws < 0 || wl < 0 || ws + wl > G) bad_win[(size_t)t] = 1;
+                mx = std::max(mx, r->read_len[rd]);
+            }
+            max_len[(size_t)t] = mx;
+        });
+        for (int t = 0; t < T; t++) {
+            if (bad_read[(size_t)t]) return bad("sieve job read out of range");
+            if (bad_win[(size_t)t]) return bad("sieve window outside the genome");
+            r->max_sieve_len = std::max(r->max_sieve_len, max_len[(size_t)t]);
+        }
     }
     r->max_sieve_len = std::min(r->max_sieve_len, kira::kMaxReadBases);
     r->w_roff.resize(r->nw);
     r->w_rlen.resize(r->nw);
     r->w_foff.resize(r->nw);
-    r->w_flen.assign(q->sw_win_len, q->sw_win_len + r->nw);
-    r->w_off.assign(q->sw_offset, q->sw_offset + r->nw);
+    r->w_flen.resize(r->nw);
+    r->w_off.resize(r->nw);
     r->w_band.resize(r->nw);
-    r->w_abort.assign(q->sw_abort, q->sw_abort + r->nw);
+    r->w_abort.resize(r->nw);
     r->w_skip.resize(r->nw);
-    for (int32_t k = 0; k < r->nw; k++) {
-        const int32_t rd = q->sw_read[k];
-        if (rd < 0 || rd >= r->nr) return bad("SW job read out of range");
-        if (q->sw_win_start[k] < 0 || q->sw_win_len[k] < 0 ||
-            q->sw_win_start[k] + q->sw_win_len[k] > G)
-            return bad("SW window outside the genome");
-        r->w_roff[k] = r->read_off[rd];
-        r->w_rlen[k] = r->read_len[rd];
-        r->w_foff[k] = q->sw_win_start[k];
-        r->w_band[k] = q->sw_band ? q->sw_band[k] : s->cfg.cfg.bandwidth;
-        r->w_skip[k] = q->sw_skip_sieve ? q->sw_skip_sieve[k] : -1;
-        if (r->w_skip[k] >= r->ns) return bad("SW skip index out of range");
+    {
+        // 1 read out of range, 2 window outside the genome, 3 skip index
+        const int T = kira::host_threads();
+        std::vector<uint8_t> why((size_t)T, 0);
+        kira::parallel_chunks(r->nw, 16384, [&](int t, int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; k++) {
+                const int32_t rd = q->sw_read[k];
+                if (rd < 0 || rd >= r->nr) {
+                    why[(size_t)t] = why[(size_t)t] ? why[(size_t)t] : 1;
+                    continue;
+                }
+                if (q->sw_win_start[k] < 0 || q->sw_win_len[k] < 0 ||
+                    q->sw_win_start[k] + q->sw_win_len[k] > G)
+                    why[(size_t)t] = why[(size_t)t] ? why[(size_t)t] : 2;
+                r->w_roff[k] = r->read_off[rd];
+                r->w_rlen[k] = r->read_len[rd];
+                r->w_foff[k] = q->sw_win_start[k];
+                r->w_flen[k] = q->sw_win_len[k];
+                r->w_off[k] = q->sw_offset[k];
+                r->w_abort[k] = q->sw_abort[k];
+                r->w_band[k] = q->sw_band ? q->sw_band[k] : s->cfg.cfg.bandwidth;
+                r->w_skip[k] = q->sw_skip_sieve ? q->sw_skip_sieve[k] : -1;
+                if (r->w_skip[k] >= r->ns) why[(size_t)t] = why[(size_t)t] ? why[(size_t)t] : 3;
+            }
+        });
+        for (int t = 0; t < T; t++) {
+            if (why[(size_t)t] == 1) return bad("SW job read out of range");
+            if (why[(size_t)t] == 2) return bad("SW window outside the genome");
+            if (why[(size_t)t] == 3) return bad("SW skip index out of range");
+        }
     }
     r->t_submit = now_s();
     // the job plan and buffer layout on the submitting thread, so the worker
