@@ -483,6 +483,11 @@ typedef struct {
     int32_t sieve_limit;         /* accept limit of sieve jobs without their own */
     int32_t cert_max_mism;       /* certification limit, < 0 disables it */
     int32_t cert_strict;
+    /* coalescing: when a slot frees up, the waiting requests at the front of
+     * the queue, up to this many reads in total (and as many as the free
+     * memory holds), are admitted together as one device batch (their
+     * results are the same as alone); 0 admits one request per slot */
+    int64_t group_max_reads;
 } kira_server_config;
 
 typedef struct {
@@ -530,6 +535,7 @@ typedef struct {
     int64_t submitted, admitted, completed;
     int32_t peak_in_flight;
     int64_t peak_device_bytes;
+    int64_t grouped; /* requests admitted as members of a coalesced group */
 } kira_server_stats;
 
 kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g,

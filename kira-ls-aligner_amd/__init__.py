@@ -162,7 +162,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64),
                 ("host_staging_bytes", C.c_int64), ("use_graph", C.c_int32), ("cfg", SwCfg),
                 ("knobs", SwKnobs), ("sieve_limit", C.c_int32), ("cert_max_mism", C.c_int32),
-                ("cert_strict", C.c_int32)]
+                ("cert_strict", C.c_int32), ("group_max_reads", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -187,7 +187,8 @@ class Response(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
-                ("peak_in_flight", C.c_int32), ("peak_device_bytes", C.c_int64)]
+                ("peak_in_flight", C.c_int32), ("peak_device_bytes", C.c_int64),
+                ("grouped", C.c_int64)]
 
 
 # PrefilterResult, and PrefilterReason::idx() -> as_str() (src/alignment/
@@ -950,11 +951,12 @@ class Server:
                  hbm_budget_bytes: int = 32 << 30, host_staging_bytes: int = 4 << 30,
                  use_graph: bool = False, cfg: Optional[SwCfg] = None,
                  knobs: Optional[SwKnobs] = None, sieve_limit: int = 8,
-                 cert_max_mism: int = -1, cert_strict: bool = False):
+                 cert_max_mism: int = -1, cert_strict: bool = False,
+                 group_max_reads: int = 0):
         self.genome = genome  # keeps the genome alive
         c = ServerConfig(slots, hbm_budget_bytes, host_staging_bytes, int(use_graph),
                          cfg or sw_cfg(), knobs or sw_knobs(), sieve_limit, cert_max_mism,
-                         int(cert_strict))
+                         int(cert_strict), int(group_max_reads))
         h = C.c_void_p()
         st = lib().kira_server_create(ctx.h, genome.h, C.byref(c), C.byref(h))
         if st != KIRA_OK:

@@ -16,10 +16,21 @@
 // a hipGraph.  Results of a request do not depend on what else is in flight:
 // each request owns its memory and every kernel reads only its own inputs
 // and the read-only genome.
+//
+// Coalescing (group_max_reads > 0): when a slot frees up and several
+// requests are waiting, the ones at the front of the queue (up to
+// group_max_reads reads, and as many as the free memory holds) are admitted
+// together as one device batch: their inputs are staged back to back with
+// the read and job indices shifted, one plan covers all their SW jobs, and
+// the outputs are split back per request.  Every job's result depends only
+// on its own inputs, so a request's results are the same alone or grouped;
+// what grouping changes is the size of each launch (a request's long SW
+// classes alone leave most of the chip idle).
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -113,6 +124,14 @@ struct Request {
     kira::PodVec<int64_t> s_start;
     kira::PodVec<int64_t> w_roff, w_foff;
     kira::PodVec<int32_t> w_rlen, w_flen, w_off, w_band, w_abort, w_skip;
+    int64_t reads_bytes = 0;  // total bytes of the reads
+    // a group: requests admitted together as one device batch.  Member k
+    // keeps its own arrays and is staged at bases[k] (first read, read byte,
+    // sieve job, SW job) of the group's; the group itself holds only what
+    // its plan needs (w_rlen, w_band) and the totals.  The members are left
+    // unchanged, so a group that cannot be placed hands them back as they came.
+    std::vector<std::unique_ptr<Request>> members;
+    std::vector<std::array<int64_t, 4>> bases;
     kira::SwPlanLayout plan;
     Layout lay{};
     bool laid_out = false;
@@ -138,6 +157,9 @@ struct kira_server {
     kira_server_config cfg{};
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
+    // per slot: streams the SW classes of long jobs fan out to (not used
+    // under graph capture)
+    std::vector<kira::SwFanout> fans;
     std::vector<std::unique_ptr<Request>> slot_req;  // null = slot free
     uint8_t *dpool = nullptr, *hpool = nullptr;
     uint8_t *hpool_dev = nullptr;  // hpool as the device addresses it
@@ -157,7 +179,11 @@ struct kira_server {
     void run();
     bool layout(Request &r, std::string &why);
     bool admit(std::unique_ptr<Request> &r, int slot, std::string &why);
-    hipError_t enqueue(Request &r, hipStream_t s);
+    std::unique_ptr<Request> make_group(std::vector<std::unique_ptr<Request>> &&members);
+    void stage(const Request &m, const std::array<int64_t, 4> &base, uint8_t *hin,
+               const Layout &L);
+    void fail_request(Request &r, kira_status st, const std::string &why);
+    hipError_t enqueue(Request &r, hipStream_t s, kira::SwFanout *fan);
     void finish(int slot, hipError_t e);
 };
 
@@ -175,7 +201,7 @@ bool kira_server::layout(Request &r, std::string &why) {
         o += align256(std::max<size_t>(bytes, 1));
         return at;
     };
-    L.reads = sec(r.reads.size());
+    L.reads = sec((size_t)r.reads_bytes);
     L.read_off = sec((size_t)r.nr * 8);
     L.read_len = sec((size_t)r.nr * 4);
     L.s_read = sec((size_t)r.ns * 4);
@@ -214,7 +240,7 @@ bool kira_server::layout(Request &r, std::string &why) {
     return true;
 }
 
-hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
+hipError_t kira_server::enqueue(Request &r, hipStream_t s, kira::SwFanout *fan) {
     const Layout &L = r.lay;
     uint8_t *din = dpool + r.dev_off, *dout = din + L.in_bytes, *dscr = dout + L.out_bytes;
     uint8_t *hin = hpool + r.host_off, *hout = hin + L.in_bytes;
@@ -272,7 +298,7 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
         j.cigar = (uint32_t *)(dscr + L.arena);
         const char *why = kira::sw_plan_launch(r.plan, d, j, cfg.cfg, cfg.knobs,
                                                r.ns > 0 ? o_flags : nullptr,
-                                               (const int32_t *)(din + L.w_skip), s, &e);
+                                               (const int32_t *)(din + L.w_skip), s, &e, fan);
         if (why) return e ? e : hipErrorInvalidValue;
         // the ops, packed, straight into the pinned host buffer
         if ((e = kira::launch_cigar_compact(r.nw, j.res, j.cigar, d.cigar_off,
@@ -283,29 +309,94 @@ hipError_t kira_server::enqueue(Request &r, hipStream_t s) {
     return hipMemcpyAsync(hout, dout, L.out_bytes, hipMemcpyDeviceToHost, s);
 }
 
-bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why) {
-    Request &r = *rp;
-    const Layout &L = r.lay;
-    // stage every input in pinned memory: one copy per request
-    uint8_t *hin = hpool + r.host_off;
+void kira_server::stage(const Request &m, const std::array<int64_t, 4> &base, uint8_t *hin,
+                        const Layout &L) {
     auto put = [&](size_t off, const void *src, size_t bytes) {
         kira::parallel_memcpy(hin + off, src, bytes);
     };
-    put(L.reads, r.reads.data(), r.reads.size());
-    put(L.read_off, r.read_off.data(), (size_t)r.nr * 8);
-    put(L.read_len, r.read_len.data(), (size_t)r.nr * 4);
-    put(L.s_read, r.s_read.data(), (size_t)r.ns * 4);
-    put(L.s_start, r.s_start.data(), (size_t)r.ns * 8);
-    put(L.s_len, r.s_len.data(), (size_t)r.ns * 4);
-    put(L.s_limit, r.s_limit.data(), (size_t)r.ns * 4);
-    put(L.w_roff, r.w_roff.data(), (size_t)r.nw * 8);
-    put(L.w_rlen, r.w_rlen.data(), (size_t)r.nw * 4);
-    put(L.w_foff, r.w_foff.data(), (size_t)r.nw * 8);
-    put(L.w_flen, r.w_flen.data(), (size_t)r.nw * 4);
-    put(L.w_off, r.w_off.data(), (size_t)r.nw * 4);
-    put(L.w_band, r.w_band.data(), (size_t)r.nw * 4);
-    put(L.w_abort, r.w_abort.data(), (size_t)r.nw * 4);
-    put(L.w_skip, r.w_skip.data(), (size_t)r.nw * 4);
+    const int64_t rb = base[0], bb = base[1], sb = base[2], wb = base[3];
+    put(L.reads + (size_t)bb, m.reads.data(), (size_t)m.reads_bytes);
+    put(L.read_len + (size_t)rb * 4, m.read_len.data(), (size_t)m.nr * 4);
+    put(L.s_start + (size_t)sb * 8, m.s_start.data(), (size_t)m.ns * 8);
+    put(L.s_len + (size_t)sb * 4, m.s_len.data(), (size_t)m.ns * 4);
+    put(L.s_limit + (size_t)sb * 4, m.s_limit.data(), (size_t)m.ns * 4);
+    put(L.w_rlen + (size_t)wb * 4, m.w_rlen.data(), (size_t)m.nw * 4);
+    put(L.w_foff + (size_t)wb * 8, m.w_foff.data(), (size_t)m.nw * 8);
+    put(L.w_flen + (size_t)wb * 4, m.w_flen.data(), (size_t)m.nw * 4);
+    put(L.w_off + (size_t)wb * 4, m.w_off.data(), (size_t)m.nw * 4);
+    put(L.w_band + (size_t)wb * 4, m.w_band.data(), (size_t)m.nw * 4);
+    put(L.w_abort + (size_t)wb * 4, m.w_abort.data(), (size_t)m.nw * 4);
+    // indices and read offsets shifted by the member's bases
+    int64_t *roff = (int64_t *)(hin + L.read_off) + rb;
+    kira::parallel_chunks(m.nr, 16384, [&](int, int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++) roff[k] = m.read_off[k] + bb;
+    });
+    int32_t *sread = (int32_t *)(hin + L.s_read) + sb;
+    kira::parallel_chunks(m.ns, 16384, [&](int, int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++) sread[k] = m.s_read[k] + (int32_t)rb;
+    });
+    int64_t *wroff = (int64_t *)(hin + L.w_roff) + wb;
+    int32_t *wskip = (int32_t *)(hin + L.w_skip) + wb;
+    kira::parallel_chunks(m.nw, 16384, [&](int, int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++) {
+            wroff[k] = m.w_roff[k] + bb;
+            wskip[k] = m.w_skip[k] >= 0 ? m.w_skip[k] + (int32_t)sb : -1;
+        }
+    });
+}
+
+std::unique_ptr<Request> kira_server::make_group(std::vector<std::unique_ptr<Request>> &&members) {
+    auto g = std::make_unique<Request>();
+    int64_t nw = 0;
+    for (const auto &m : members) nw += m->nw;
+    g->w_rlen.resize((size_t)nw);
+    g->w_band.resize((size_t)nw);
+    g->t_submit = members.front()->t_submit;
+    for (auto &m : members) {
+        g->bases.push_back(std::array<int64_t, 4>{g->nr, g->reads_bytes, g->ns, g->nw});
+        if (m->nw) {
+            memcpy(g->w_rlen.data() + g->nw, m->w_rlen.data(), (size_t)m->nw * 4);
+            memcpy(g->w_band.data() + g->nw, m->w_band.data(), (size_t)m->nw * 4);
+        }
+        g->nr += m->nr;
+        g->reads_bytes += m->reads_bytes;
+        g->ns += m->ns;
+        g->nw += m->nw;
+        g->max_sieve_len = std::max(g->max_sieve_len, m->max_sieve_len);
+        g->t_submit = std::min(g->t_submit, m->t_submit);
+    }
+    g->members = std::move(members);
+    return g;
+}
+
+void kira_server::fail_request(Request &r, kira_status st, const std::string &why) {
+    auto one = [&](const Request &m) {
+        auto res = std::make_unique<Result>();
+        res->status = st;
+        res->err = why;
+        res->t_submit = m.t_submit;
+        res->t_start = r.t_start ? r.t_start : m.t_submit;
+        res->t_done = now_s();
+        results[m.ticket] = std::move(res);
+    };
+    if (r.members.empty())
+        one(r);
+    else
+        for (const auto &m : r.members) one(*m);
+    cv_done.notify_all();
+}
+
+bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why) {
+    Request &r = *rp;
+    const Layout &L = r.lay;
+    // stage every input in pinned memory: one copy per request (per member
+    // of a group, at its bases)
+    uint8_t *hin = hpool + r.host_off;
+    if (r.members.empty()) {
+        stage(r, std::array<int64_t, 4>{}, hin, L);
+    } else {
+        for (size_t k = 0; k < r.members.size(); k++) stage(*r.members[k], r.bases[k], hin, L);
+    }
     kira::sw_plan_pack_tables(r.plan, hin + L.tables);
     hipStream_t s = streams[slot];
     hipError_t e;
@@ -315,7 +406,7 @@ bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why
             why = hipGetErrorString(e);
             return false;
         }
-        hipError_t eq = enqueue(r, s);
+        hipError_t eq = enqueue(r, s, nullptr);
         e = hipStreamEndCapture(s, &graph);
         if (eq || e) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -328,7 +419,7 @@ bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why
             why = hipGetErrorString(e);
             return false;
         }
-    } else if ((e = enqueue(r, s))) {
+    } else if ((e = enqueue(r, s, fans[slot].ok ? &fans[slot] : nullptr))) {
         why = hipGetErrorString(e);
         return false;
     }
@@ -343,42 +434,48 @@ bool kira_server::admit(std::unique_ptr<Request> &rp, int slot, std::string &why
 void kira_server::finish(int slot, hipError_t e) {
     std::unique_ptr<Request> rp = std::move(slot_req[slot]);
     Request &r = *rp;
-    auto res = std::make_unique<Result>();
-    res->t_submit = r.t_submit;
-    res->t_start = r.t_start;
-    if (e != hipSuccess) {
-        res->status = KIRA_ERR_RUNTIME;
-        res->err = hipGetErrorString(e);
-    } else {
-        const Layout &L = r.lay;
-        const uint8_t *hout = hpool + r.host_off + L.in_bytes;
-        auto get32 = [&](kira::PodVec<int32_t> &v, size_t off, size_t n) {
-            v.resize(n);
-            kira::parallel_memcpy(v.data(), hout + off, n * 4);
+    const Layout &L = r.lay;
+    const uint8_t *hout = hpool + r.host_off + L.in_bytes;
+    const int32_t *rr = (const int32_t *)(hout + L.o_res);
+    const int64_t *coff = (const int64_t *)(hout + L.o_coff);
+    const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
+    // the results of sieve jobs [s0, s0 + ns) and SW jobs [w0, w0 + nw) of
+    // the launch (one request, or one member of a group)
+    auto make = [&](const Request &m, int64_t s0, int64_t ns, int64_t w0, int64_t nw) {
+        auto res = std::make_unique<Result>();
+        res->t_submit = m.t_submit;
+        res->t_start = r.t_start;
+        if (e != hipSuccess) {
+            res->status = KIRA_ERR_RUNTIME;
+            res->err = hipGetErrorString(e);
+            res->t_done = now_s();
+            return res;
+        }
+        auto get32 = [&](kira::PodVec<int32_t> &v, size_t off) {
+            v.resize((size_t)ns);
+            kira::parallel_memcpy(v.data(), hout + off + (size_t)s0 * 4, (size_t)ns * 4);
         };
-        get32(res->shift, L.o_shift, r.ns);
-        get32(res->best, L.o_best, r.ns);
-        get32(res->second, L.o_second, r.ns);
-        res->flags.resize(r.ns);
-        kira::parallel_memcpy(res->flags.data(), hout + L.o_flags, r.ns);
-        const int32_t *rr = (const int32_t *)(hout + L.o_res);
-        const int64_t *coff = (const int64_t *)(hout + L.o_coff);
-        const uint32_t *cg = (const uint32_t *)(hout + L.o_cigar);
-        res->sw.resize(r.nw);
-        res->skipped.resize(r.nw);
+        get32(res->shift, L.o_shift);
+        get32(res->best, L.o_best);
+        get32(res->second, L.o_second);
+        res->flags.resize((size_t)ns);
+        kira::parallel_memcpy(res->flags.data(), hout + L.o_flags + s0, (size_t)ns);
+        res->sw.resize((size_t)nw);
+        res->skipped.resize((size_t)nw);
         // the device packed every job's ops in job order at coff[k]: one copy
-        // of the packed words, then the per-job fields in parallel
-        int64_t words = 0;
-        if (r.nw > 0) {
-            const int32_t last = rr[(size_t)(r.nw - 1) * kira::kSwResWords + 6];
-            words = coff[r.nw - 1] + (last > 0 ? last : 0);
+        // of this range's packed words, then the per-job fields in parallel
+        int64_t c0 = 0, words = 0;
+        if (nw > 0) {
+            const int32_t last = rr[(size_t)(w0 + nw - 1) * kira::kSwResWords + 6];
+            c0 = coff[w0];
+            words = coff[w0 + nw - 1] + (last > 0 ? last : 0) - c0;
         }
         res->cigar.resize((size_t)words);
-        kira::parallel_memcpy(res->cigar.data(), cg, (size_t)words * 4);
+        kira::parallel_memcpy(res->cigar.data(), cg + c0, (size_t)words * 4);
         std::vector<int32_t> bad(kira::host_threads(), -1);  // last failing job per thread
-        kira::parallel_chunks(r.nw, 4096, [&](int t, int64_t b, int64_t e) {
-            for (int64_t k = b; k < e; k++) {
-                const int32_t *q = rr + (size_t)k * kira::kSwResWords;
+        kira::parallel_chunks(nw, 4096, [&](int t, int64_t b, int64_t e2) {
+            for (int64_t k = b; k < e2; k++) {
+                const int32_t *q = rr + (size_t)(w0 + k) * kira::kSwResWords;
                 kira_sw_result &o = res->sw[k];
                 o.score = q[0];
                 o.read_end = q[1];
@@ -388,7 +485,7 @@ void kira_server::finish(int slot, hipError_t e) {
                 o.read_start = q[5];
                 o.n_cigar = q[6];
                 o.cells = q[8];
-                o.cigar_off = coff[k];
+                o.cigar_off = coff[w0 + k] - c0;
                 res->skipped[k] = (uint8_t)(q[9] == 1);
                 if (q[6] < 0 || q[7] != 0) {
                     o.n_cigar = 0;
@@ -399,18 +496,30 @@ void kira_server::finish(int slot, hipError_t e) {
         int32_t worst = -1;
         for (int32_t k : bad) worst = std::max(worst, k);
         if (worst >= 0) {
-            const int32_t *q = rr + (size_t)worst * kira::kSwResWords;
+            const int32_t *q = rr + (size_t)(w0 + worst) * kira::kSwResWords;
             res->status = KIRA_ERR_RUNTIME;
             res->err = q[6] < 0 ? "CIGAR capacity exceeded" : "int16 band overflow";
         }
+        res->t_done = now_s();
+        return res;
+    };
+    std::vector<std::pair<int64_t, std::unique_ptr<Result>>> out;
+    if (r.members.empty()) {
+        out.emplace_back(r.ticket, make(r, 0, r.ns, 0, r.nw));
+    } else {
+        for (size_t k = 0; k < r.members.size(); k++) {
+            const Request &m = *r.members[k];
+            out.emplace_back(m.ticket, make(m, r.bases[k][2], m.ns, r.bases[k][3], m.nw));
+        }
     }
     if (r.exec) (void)hipGraphExecDestroy(r.exec);
-    res->t_done = now_s();
     std::lock_guard<std::mutex> lk(mu);
     dalloc.release(r.dev_off, r.dev_bytes);
     halloc.release(r.host_off, r.host_bytes);
-    stats.completed++;
-    results[r.ticket] = std::move(res);
+    for (auto &x : out) {
+        stats.completed++;
+        results[x.first] = std::move(x.second);
+    }
     cv_done.notify_all();
 }
 
@@ -431,6 +540,7 @@ void kira_server::run() {
         std::unique_lock<std::mutex> lk(mu);
         int in_flight = 0;
         for (int k = 0; k < S; k++) in_flight += slot_req[k] ? 1 : 0;
+        bool no_group = false;  // a group that did not fit: its first member goes alone
         while (!pending.empty() && in_flight < S) {
             Request &r = *pending.front();
             std::string why;
@@ -444,34 +554,83 @@ void kira_server::run() {
                 why = "request larger than the host staging pool";
             }
             if (!ok) {
-                auto res = std::make_unique<Result>();
-                res->status = why.find("budget") != std::string::npos ||
-                                      why.find("staging") != std::string::npos
-                                  ? KIRA_ERR_OUT_OF_MEMORY
-                                  : KIRA_ERR_INVALID_ARGUMENT;
-                res->err = why;
-                res->t_submit = res->t_start = r.t_submit;
-                res->t_done = now_s();
-                results[r.ticket] = std::move(res);
+                std::unique_ptr<Request> bad = std::move(pending.front());
                 pending.pop_front();
-                cv_done.notify_all();
+                fail_request(*bad,
+                             why.find("budget") != std::string::npos ||
+                                     why.find("staging") != std::string::npos
+                                 ? KIRA_ERR_OUT_OF_MEMORY
+                                 : KIRA_ERR_INVALID_ARGUMENT,
+                             why);
                 progressed = true;
                 continue;
             }
-            size_t doff, hoff;
-            if (!dalloc.alloc(r.dev_bytes, doff)) break;  // wait for memory to free up
-            if (!halloc.alloc(r.host_bytes, hoff)) {
-                dalloc.release(doff, r.dev_bytes);
-                break;
+            // coalescing: the requests behind it that fit with it (reads and
+            // free memory; the merged layout is about the sum of the parts)
+            size_t n_group = 1;
+            if (cfg.group_max_reads > 0 && !no_group) {
+                int64_t reads = r.nr;
+                size_t dev = r.dev_bytes, host = r.host_bytes;
+                const size_t dfree = dalloc.capacity() - dalloc.used(),
+                             hfree = halloc.capacity() - halloc.used();
+                for (size_t k = 1; k < pending.size(); k++) {
+                    Request &q = *pending[k];
+                    std::string w2;
+                    if (!layout(q, w2) || reads + q.nr > cfg.group_max_reads) break;
+                    const size_t d2 = dev + q.dev_bytes, h2 = host + q.host_bytes;
+                    if (d2 + d2 / 16 + ((size_t)1 << 20) > dfree ||
+                        h2 + h2 / 16 + ((size_t)1 << 20) > hfree)
+                        break;
+                    reads += q.nr;
+                    dev = d2;
+                    host = h2;
+                    n_group = k + 1;
+                }
             }
-            r.dev_off = doff;
-            r.host_off = hoff;
+            std::unique_ptr<Request> rp;
+            if (n_group > 1) {
+                std::vector<std::unique_ptr<Request>> mem;
+                for (size_t k = 0; k < n_group; k++) {
+                    mem.push_back(std::move(pending.front()));
+                    pending.pop_front();
+                }
+                lk.unlock();
+                rp = make_group(std::move(mem));
+                std::string w3;
+                const bool lok = layout(*rp, w3);
+                lk.lock();
+                if (!lok) {
+                    fail_request(*rp, KIRA_ERR_INVALID_ARGUMENT, w3);
+                    progressed = true;
+                    continue;
+                }
+            }
+            Request &a = rp ? *rp : *pending.front();
+            size_t doff, hoff;
+            bool got = dalloc.alloc(a.dev_bytes, doff);
+            if (got && !halloc.alloc(a.host_bytes, hoff)) {
+                dalloc.release(doff, a.dev_bytes);
+                got = false;
+            }
+            if (!got) {
+                if (!rp) break;  // wait for memory to free up
+                // the group does not fit after all: back to the queue, in
+                // order, and the first goes alone
+                for (auto it = rp->members.rbegin(); it != rp->members.rend(); ++it)
+                    pending.push_front(std::move(*it));
+                no_group = true;
+                continue;
+            }
+            a.dev_off = doff;
+            a.host_off = hoff;
             int slot = 0;
             while (slot_req[slot]) slot++;
-            std::unique_ptr<Request> rp = std::move(pending.front());
-            pending.pop_front();
-            const int64_t ticket = rp->ticket;
-            stats.admitted++;
+            if (!rp) {
+                rp = std::move(pending.front());
+                pending.pop_front();
+            }
+            stats.admitted += rp->members.empty() ? 1 : (int64_t)rp->members.size();
+            stats.grouped += (int64_t)rp->members.size();
             stats.peak_device_bytes = std::max<int64_t>(stats.peak_device_bytes, dalloc.used());
             lk.unlock();
             rp->t_start = now_s();
@@ -483,16 +642,10 @@ void kira_server::run() {
                 dalloc.release(raw->dev_off, raw->dev_bytes);
                 halloc.release(raw->host_off, raw->host_bytes);
                 if (raw->exec) (void)hipGraphExecDestroy(raw->exec);
-                auto res = std::make_unique<Result>();
-                res->status = KIRA_ERR_RUNTIME;
-                res->err = why2;
-                res->t_submit = raw->t_submit;
-                res->t_start = raw->t_start;
-                res->t_done = now_s();
-                results[ticket] = std::move(res);
-                cv_done.notify_all();
+                fail_request(*raw, KIRA_ERR_RUNTIME, why2);
             }
             progressed = true;
+            no_group = false;
             in_flight = 0;
             for (int k = 0; k < S; k++) in_flight += slot_req[k] ? 1 : 0;
             stats.peak_in_flight = std::max<int32_t>(stats.peak_in_flight, in_flight);
@@ -528,6 +681,10 @@ kira_status kira_server_create(kira_ctx *ctx, const kira_genome *g, const kira_s
         if (!e) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (!e) s->events.push_back(ev);
     }
+    // without its streams a slot keeps every SW class on its own stream
+    s->fans.resize(c->slots);
+    if (!e)
+        for (auto &f : s->fans) (void)kira::sw_fanout_create(f);
     if (e) {
         for (auto st : s->streams) (void)hipStreamDestroy(st);
         for (auto ev : s->events) (void)hipEventDestroy(ev);
@@ -575,6 +732,7 @@ kira_status kira_server_submit(kira_server *s, const kira_request *q, int64_t *t
         pos += q->read_len[k];
     }
     r->reads.resize((size_t)pos);
+    r->reads_bytes = pos;
     kira::parallel_chunks(r->nr, 2048, [&](int, int64_t b, int64_t e) {
         for (int64_t k = b; k < e; k++)
             if (q->read_len[k])
@@ -730,6 +888,11 @@ void kira_server_destroy(kira_server *s) {
         (void)hipStreamDestroy(st);
     }
     for (auto ev : s->events) (void)hipEventDestroy(ev);
+    for (auto &f : s->fans) {
+        for (hipStream_t st : f.aux)
+            if (st) (void)hipStreamSynchronize(st);
+        kira::sw_fanout_destroy(f);
+    }
     if (s->dpool) (void)hipFree(s->dpool);
     if (s->hpool) (void)hipHostFree(s->hpool);
     delete s;

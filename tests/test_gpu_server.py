@@ -177,3 +177,32 @@ def test_invalid_requests_fail_at_submit(world):
     with pytest.raises(pkg.HipError):
         srv.submit(read, [0], [160], sieve=([1], [0], [300]))
     srv.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_coalesced_requests_give_the_same_results(world, graph):
+    """Waiting requests admitted together as one device batch
+    (group_max_reads): each request's results equal its results alone.  The
+    budget holds about three requests, so some groups do not fit and hand
+    their requests back (the first then runs alone)."""
+    pkg, ctx, genome, gseq = world
+    qs = requests(gseq)
+    cfg = pkg.sw_cfg(1, 4, 6, 1, 32, 100, 5)
+    kw = dict(cfg=cfg, sieve_limit=8, cert_max_mism=4, host_staging_bytes=256 << 20)
+    srv = pkg.Server(ctx, genome, slots=1, hbm_budget_bytes=1 << 30, **kw)
+    alone = [srv.wait(submit(srv, q)) for q in qs]
+    srv.close()
+    for budget in (1 << 30, 96 << 20):
+        print(f"[server] coalesced graph={graph} budget={budget}", flush=True)
+        srv = pkg.Server(ctx, genome, slots=2, hbm_budget_bytes=budget, use_graph=graph,
+                         group_max_reads=700, **kw)
+        tickets = [submit(srv, q) for q in qs]
+        grouped = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        srv.close()
+        assert st["completed"] == len(qs) and st["admitted"] == len(qs), st
+        assert st["peak_device_bytes"] <= budget
+        if budget == 1 << 30:
+            assert st["grouped"] >= 2, st
+        for a, b in zip(alone, grouped):
+            assert strip(a) == strip(b)
