@@ -343,25 +343,6 @@ def sieve_oracle_flags(lay, shifts, best, second, K, cert_max):
 
 
 # --------------------------------------------------------------- main
-# The stage-4 CPU baseline's worker processes: each rebuilds the seeded batch
-# on its first task (the untimed warm-up) and runs the CPU restatement on
-# read ranges.
-_S4 = {}
-
-
-def _s4_run(task):
-    from oracle import stage4 as S
-    n_reads, lo, hi = task
-    if _S4.get("n") != n_reads:
-        from tests.stage4_data import make_batch
-        _S4["b"] = make_batch(21, n_reads, n_contigs=8, contig_len=2_500_000)
-        _S4["n"] = n_reads
-    b = _S4["b"]
-    S.run(b["contigs"], b["reads"][lo:hi], b["chains"][lo:hi], S.default_cfg(),
-          b["repeat_min_occ"][lo:hi])
-    return hi - lo
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,19 +386,7 @@ def main():
         print("bench.py: this build targets one GPU; --gpus must be 1 "
               "(the path shards with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    # the stage-4 CPU baseline's worker processes start (idle) before
-    # anything here touches the GPU
-    args.s4_pool, args.s4_threads = None, len(os.sched_getaffinity(0))
-    if (not args.no_cpu and args.stage4_reads > 0 and not args.sw_only and
-            not args.fastpath_only):
-        import multiprocessing as mp
-        args.s4_pool = mp.get_context("spawn").Pool(args.s4_threads)
-    try:
-        run_bench(args)
-    finally:
-        if args.s4_pool is not None:
-            args.s4_pool.terminate()
-            args.s4_pool.join()
+    run_bench(args)
 
 
 def run_bench(args):
@@ -833,9 +802,10 @@ def bench_fastpath(pkg, ctx, args):
 
 def bench_stage4(pkg, ctx, args):
     """The stage-4 driver (kira_stage4_run) over one batch of single-end reads
-    with their chains: reads/s per call, host-wall phase split, and the CPU
-    restatement on a sample of the same batch (a Python driver over the C
-    oracle, one thread) with a parity check on that sample."""
+    with their chains: reads/s per call and host-wall phase split in the
+    --gpu and CPU-mode configurations; the restated reference CPU path (the
+    C driver, every CPU) over the same batch, with parity of the CPU-mode
+    results on every read and of the --gpu results on the first 2000."""
     from tests.stage4_data import make_batch
     from oracle import stage4 as S
     t0 = time.time()
@@ -881,46 +851,50 @@ def bench_stage4(pkg, ctx, args):
           f"ms/batch, {out['cpu_mode']['reads_per_s']:.0f} reads/s, phases "
           f"{out['cpu_mode']['phase_ms']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
+        # the --gpu configuration against the Python restatement on the first
+        # reads (it has no faster form)
         m = 2000
         sub = {key: b[key][:m] for key in ("reads", "chains", "repeat_min_occ")}
-        t0 = time.perf_counter()
         want, _ = S.run(b["contigs"], sub["reads"], sub["chains"], S.default_cfg(),
                         sub["repeat_min_occ"])
-        cpu_s = time.perf_counter() - t0
         res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], sub["reads"],
                              sub["chains"], cfg, sub["repeat_min_occ"])
-        parity = res.alignments() == want
+        out["parity_vs_oracle_sample"] = res.alignments() == want
         res.close()
-        want_cpu, _ = S.run(b["contigs"], sub["reads"], sub["chains"],
-                            S.default_cfg(cpu_mode=True), sub["repeat_min_occ"])
-        res = pkg.stage4_run(ctx, genome, b["contig_off"], b["contig_len"], sub["reads"],
-                             sub["chains"], cpu_cfg, sub["repeat_min_occ"])
-        out["cpu_mode"]["parity_vs_oracle_sample"] = res.alignments() == want_cpu
+        # the restated reference CPU path: the stage-4 driver in C without the
+        # GPU dispatcher (oracle/kira_stage4_oracle.c, pthreads over reads,
+        # lane groups and jobs) over the whole batch on every CPU of this
+        # process, and a slice of it on one thread; 1 warm-up + median of 5
+        T = len(os.sched_getaffinity(0))
+        ccfg = S.default_cfg(cpu_mode=True)
+        cb = S.CBatch(b["genome"], b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                      b["repeat_min_occ"])
+        t_all, _ = median_of_runs(lambda: S.run_c(cb, ccfg, T, decode=False))
+        m1 = min(n, 10_000)
+        cb1 = S.CBatch(b["genome"], b["contig_off"], b["contig_len"], b["reads"][:m1],
+                       b["chains"][:m1], b["repeat_min_occ"][:m1])
+        t_one, _ = median_of_runs(lambda: S.run_c(cb1, ccfg, 1, decode=False), runs=3)
+        # the device in the same configuration against it, every read
+        want_cpu, _ = S.run_c(cb, ccfg, T)
+        res = pkg.stage4_run(ctx, genome, x, cfg=cpu_cfg)
+        out["cpu_mode"]["parity_vs_c_driver_all_reads"] = res.alignments() == want_cpu
         res.close()
-        out["cpu_baseline"] = {"value": m / cpu_s, "unit": "reads/s", "cores": 1, "kind": "port",
-                               "sample": f"first {m} reads of the batch through the CPU "
-                                         "restatement (Python driver over the C oracle)"}
-        pool = getattr(args, "s4_pool", None)
-        if pool is not None and n >= 1000 * args.s4_threads:
-            # the same restatement in one process per CPU of this process, a
-            # range of 1000 reads each (1 warm-up + median of 5), and one
-            # range on one process
-            T, chunk = args.s4_threads, 1000
-            ranges = [(n, k * chunk, (k + 1) * chunk) for k in range(T)]
-            t_all, _ = median_of_runs(lambda: sum(pool.map(_s4_run, ranges, chunksize=1)))
-            t_one, _ = median_of_runs(lambda: pool.apply(_s4_run, ((n, 0, chunk),)))
-            out["cpu_baseline"] = {
-                "value": T * chunk / t_all, "unit": "reads/s", "cores": T, "kind": "port",
-                "single_thread_value": chunk / t_one,
-                "sample": f"first {T * chunk} reads of the batch through the CPU restatement "
-                          f"(Python driver over the C oracle), {T} worker processes = the CPUs "
-                          f"this process may use, {chunk} reads each; one range on one "
-                          "process; 1 warm-up + median of 5"}
-        out["parity_vs_oracle_sample"] = parity
-        cb = out["cpu_baseline"]
-        print(f"[bench] stage4 cpu baseline {cb['value']:.0f} reads/s on {cb['cores']} "
-              f"processes ({m / cpu_s:.0f} on one, first {m} reads); parity={parity}",
-              file=sys.stderr, flush=True)
+        del want_cpu
+        out["cpu_baseline"] = {
+            "value": n / t_all, "unit": "reads/s", "cores": T, "kind": "port",
+            "label": "restated reference CPU path",
+            "single_thread_value": m1 / t_one,
+            "sample": f"the whole batch ({n} reads) through the stage-4 driver restated in C "
+                      "without the GPU dispatcher (oracle/kira_stage4_oracle.c: fast DP "
+                      "cascade before SW, AVX2 16-lane groups, pthreads over reads, groups "
+                      f"and jobs), {T} threads = the CPUs this process may use; the first "
+                      f"{m1} reads on 1 thread; 1 warm-up + median of 5 (3 on 1 thread)"}
+        cb_ = out["cpu_baseline"]
+        print(f"[bench] stage4 cpu baseline {cb_['value']:.0f} reads/s on {T} threads, "
+              f"{cb_['single_thread_value']:.0f} on 1 (C driver); parity: --gpu "
+              f"{out['parity_vs_oracle_sample']} (first {m} reads vs the Python driver), "
+              f"cpu mode {out['cpu_mode']['parity_vs_c_driver_all_reads']} (all {n} reads "
+              "vs the C driver)", file=sys.stderr, flush=True)
     genome.close()
     out["plain_mem_c1"] = bench_c1(pkg, ctx, args)
     return out

@@ -296,6 +296,39 @@ int ko_fast_dp(const uint8_t *read, size_t read_len, const uint8_t *contig, size
     return r;
 }
 
+struct ko_fast_scratch {
+    scratch s;
+};
+
+ko_fast_scratch *ko_fast_scratch_new(void) {
+    return (ko_fast_scratch *)calloc(1, sizeof(ko_fast_scratch));
+}
+
+void ko_fast_scratch_free(ko_fast_scratch *f) {
+    if (!f) return;
+    scratch_free(&f->s);
+    free(f);
+}
+
+/* The WFA CIGAR is the retry's or the main pass's, both left in s.cig (the
+ * retry swaps the buffers); a certificate's is {L}M. */
+int ko_fast_dp_rec(const uint8_t *read, size_t read_len, const uint8_t *contig,
+                   size_t contig_len, uint32_t ref_start, uint32_t read_start,
+                   int32_t chain_score, const ko_cfg *cfg, const ko_router *router,
+                   ko_fast_scratch *f, ko_fast_out *out, const uint32_t **cigar,
+                   const char **md) {
+    if (!fast_dp_one(read, read_len, contig, contig_len, ref_start, read_start, chain_score, cfg,
+                     router, &f->s, out))
+        return 0;
+    if (out->kind == 1) {
+        grow((void **)&f->s.cig, &f->s.cig_cap, 4 * sizeof(uint32_t));
+        f->s.cig[0] = ((uint32_t)read_len << 4) | KO_M;
+    }
+    *cigar = f->s.cig;
+    *md = f->s.md;
+    return 1;
+}
+
 typedef struct {
     int32_t n;
     const uint8_t *reads;

@@ -19,6 +19,9 @@
 #include "kira_oracle.h"
 
 #include <limits.h>
+#ifdef __AVX2__
+#include <immintrin.h>
+#endif
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -1080,13 +1083,158 @@ static int int8_viable(size_t read_len, const ko_cfg *c) {
            c->gap_extend <= 32;
 }
 
+#ifdef __AVX2__
+/* ko_sw_group's 16-bit form in AVX2 registers: sixteen i16 lanes with
+ * saturating adds, as sw_batch_avx2_inner computes them, over the same cells,
+ * trace bytes (16 per cell, lane k at byte k) and per-lane bookkeeping, so the
+ * results are those of ko_sw_group(bits = 16); a CPU test compares the two
+ * forms.  Lanes n .. 15 repeat lane 0's bytes and are not read back.  The
+ * cost gate (avx2_costs_fit) keeps scores and coordinates inside i16. */
+static void sw_group16_avx2(const ko_batch_input *in, int32_t n, const ko_cfg *cfg,
+                            const ko_knobs *kn, ko_sw_result *res) {
+    enum { LN = 16 };
+    const size_t q = in[0].read_len, r = in[0].window_len, tw = r + 1;
+    const int32_t gof = sat_add32(cfg->gap_open, cfg->gap_extend);
+    const __m256i Z = _mm256_setzero_si256(), NEG = _mm256_set1_epi16(-16384);
+    const __m256i GO = _mm256_set1_epi16((int16_t)(-gof));
+    const __m256i GX = _mm256_set1_epi16((int16_t)(-cfg->gap_extend));
+    const __m256i VM = _mm256_set1_epi16((int16_t)cfg->match_score);
+    const __m256i VX = _mm256_set1_epi16((int16_t)(-cfg->mismatch));
+    const __m256i T1 = _mm256_set1_epi16(TR_DIAG), T2 = _mm256_set1_epi16(TR_E);
+    const __m256i T3 = _mm256_set1_epi16(TR_F), TEE = _mm256_set1_epi16(TR_EEXT);
+    const __m256i TFE = _mm256_set1_epi16(TR_FEXT);
+    const size_t tsize = (q + 1) * tw * LN;
+    uint8_t *trace = (uint8_t *)malloc(tsize);
+    memset(trace, 0, tw * LN); /* row 0 */
+    uint8_t *wt = (uint8_t *)malloc(r * LN + 32);
+    for (size_t j = 0; j < r; j++)
+        for (int k = 0; k < LN; k++) wt[j * LN + k] = in[k < n ? k : 0].window[j];
+    int16_t *buf = (int16_t *)aligned_alloc(32, 4 * tw * LN * sizeof(int16_t));
+    int16_t *prev_h = buf, *prev_e = buf + tw * LN, *cur_h = buf + 2 * tw * LN,
+            *cur_e = buf + 3 * tw * LN;
+    for (size_t j = 0; j < tw; j++) {
+        _mm256_store_si256((__m256i *)(prev_h + j * LN), Z);
+        _mm256_store_si256((__m256i *)(prev_e + j * LN), NEG);
+    }
+    __m256i best = Z, bi = Z, bj = Z, bq = Z, bqj = Z;
+    int32_t done[LN] = {0};
+    size_t rows_done = q;
+    for (size_t i = 1; i <= q; i++) {
+        _mm256_store_si256((__m256i *)cur_h, Z);
+        _mm256_store_si256((__m256i *)cur_e, NEG);
+        memset(trace + i * tw * LN, 0, LN); /* column 0 */
+        uint8_t rb[LN];
+        for (int k = 0; k < LN; k++) rb[k] = in[k < n ? k : 0].read[i - 1];
+        const __m128i rv = _mm_loadu_si128((const __m128i *)rb);
+        const __m256i vi = _mm256_set1_epi16((int16_t)i);
+        const int last = i == q;
+        __m256i hleft = Z, f = NEG, hdiag = Z;
+        uint8_t *trow = trace + i * tw * LN;
+        for (size_t j = 1; j <= r; j++) {
+            const __m128i eq8 = _mm_cmpeq_epi8(_mm_loadu_si128((const __m128i *)(wt + (j - 1) * LN)), rv);
+            const __m256i s = _mm256_blendv_epi8(VX, VM, _mm256_cvtepi8_epi16(eq8));
+            const __m256i hm = _mm256_adds_epi16(hdiag, s);
+            const __m256i hup = _mm256_load_si256((const __m256i *)(prev_h + j * LN));
+            const __m256i efh = _mm256_adds_epi16(hup, GO);
+            const __m256i efe =
+                _mm256_adds_epi16(_mm256_load_si256((const __m256i *)(prev_e + j * LN)), GX);
+            const __m256i e = _mm256_max_epi16(efh, efe);
+            const __m256i ffh = _mm256_adds_epi16(hleft, GO);
+            const __m256i fff = _mm256_adds_epi16(f, GX);
+            f = _mm256_max_epi16(ffh, fff);
+            __m256i h = _mm256_max_epi16(_mm256_max_epi16(hm, e), f);
+            h = _mm256_max_epi16(h, Z);
+            _mm256_store_si256((__m256i *)(cur_h + j * LN), h);
+            _mm256_store_si256((__m256i *)(cur_e + j * LN), e);
+            /* direction: stop if 0, else diagonal, E, F in that order */
+            __m256i tr = T3;
+            tr = _mm256_blendv_epi8(tr, T2, _mm256_cmpeq_epi16(h, e));
+            tr = _mm256_blendv_epi8(tr, T1, _mm256_cmpeq_epi16(h, hm));
+            tr = _mm256_blendv_epi8(tr, Z, _mm256_cmpeq_epi16(h, Z));
+            tr = _mm256_or_si256(tr, _mm256_and_si256(_mm256_cmpgt_epi16(efe, efh), TEE));
+            tr = _mm256_or_si256(tr, _mm256_and_si256(_mm256_cmpgt_epi16(fff, ffh), TFE));
+            _mm_storeu_si128((__m128i *)(trow + j * LN),
+                             _mm_packus_epi16(_mm256_castsi256_si128(tr),
+                                              _mm256_extracti128_si256(tr, 1)));
+            const __m256i vj = _mm256_set1_epi16((int16_t)j);
+            const __m256i gt = _mm256_cmpgt_epi16(h, best);
+            best = _mm256_max_epi16(best, h);
+            bi = _mm256_blendv_epi8(bi, vi, gt);
+            bj = _mm256_blendv_epi8(bj, vj, gt);
+            if (last) {
+                const __m256i gq = _mm256_cmpgt_epi16(h, bq);
+                bq = _mm256_max_epi16(bq, h);
+                bqj = _mm256_blendv_epi8(bqj, vj, gq);
+            }
+            hleft = h;
+            hdiag = hup;
+        }
+        int16_t bl[LN];
+        _mm256_storeu_si256((__m256i *)bl, best);
+        const int32_t remaining = (int32_t)(q - i);
+        int all_done = 1;
+        for (int k = 0; k < n; k++) {
+            if (done[k]) continue;
+            const int32_t ab = in[k].abort_score;
+            if (ab > I32_MIN_8 && bl[k] + remaining * cfg->match_score < ab) done[k] = 1;
+            if (!done[k]) all_done = 0;
+        }
+        if (all_done) {
+            rows_done = i;
+            break;
+        }
+        int16_t *t = prev_h;
+        prev_h = cur_h;
+        cur_h = t;
+        t = prev_e;
+        prev_e = cur_e;
+        cur_e = t;
+    }
+    int16_t vbest[LN], vbi[LN], vbj[LN], vbq[LN], vbqj[LN];
+    _mm256_storeu_si256((__m256i *)vbest, best);
+    _mm256_storeu_si256((__m256i *)vbi, bi);
+    _mm256_storeu_si256((__m256i *)vbj, bj);
+    _mm256_storeu_si256((__m256i *)vbq, bq);
+    _mm256_storeu_si256((__m256i *)vbqj, bqj);
+    for (int k = 0; k < n; k++) {
+        int32_t si, sj, bs;
+        if (vbq[k] > 0 && sat_add32(vbq[k], cfg->clip_penalty) > vbest[k]) {
+            si = (int32_t)q;
+            sj = vbqj[k];
+            bs = vbq[k];
+        } else {
+            si = vbi[k];
+            sj = vbj[k];
+            bs = vbest[k];
+        }
+        int32_t oi, oj;
+        traceback_lane(trace, tw, LN, k, tsize, si, sj, &res[k], &oi, &oj);
+        res[k].ref_start = (uint32_t)oj;
+        res[k].ref_end = (uint32_t)sj;
+        res[k].read_start = oi;
+        res[k].read_end = si;
+        res[k].score = bs;
+        res[k].early_abort = done[k];
+        res[k].cells = (int64_t)(rows_done * r);
+    }
+    (void)kn;
+    free(trace);
+    free(wt);
+    free(buf);
+}
+#endif
+
 static void avx2_chunk(const ko_batch_input *in, int32_t n, const ko_cfg *cfg,
                        const ko_knobs *kn, ko_sw_result *res) {
     if (!avx2_costs_fit(cfg, in[0].read_len, in[0].window_len)) {
         for (int32_t k = 0; k < n; k++) scalar_sw(&in[k], cfg, kn, &res[k]);
         return;
     }
+#ifdef __AVX2__
+    sw_group16_avx2(in, n, cfg, kn, res);
+#else
     ko_sw_group(in, n, cfg, 16, kn, res);
+#endif
 }
 
 /* align_batch_simd SW part — src/alignment/mod.rs:1111-1149 and

@@ -219,6 +219,17 @@ typedef struct {
 int ko_fast_dp(const uint8_t *read, size_t read_len, const uint8_t *contig, size_t contig_len,
                uint32_t ref_start, uint32_t read_start, int32_t chain_score, const ko_cfg *cfg,
                const ko_router *router, ko_fast_out *out);
+/* ko_fast_dp with the record: *cigar (out->n_cigar words) and *md
+ * (NUL-terminated) point into the scratch and stay valid until its next use.
+ * One scratch per thread. */
+typedef struct ko_fast_scratch ko_fast_scratch;
+ko_fast_scratch *ko_fast_scratch_new(void);
+void ko_fast_scratch_free(ko_fast_scratch *s);
+int ko_fast_dp_rec(const uint8_t *read, size_t read_len, const uint8_t *contig,
+                   size_t contig_len, uint32_t ref_start, uint32_t read_start,
+                   int32_t chain_score, const ko_cfg *cfg, const ko_router *router,
+                   ko_fast_scratch *s, ko_fast_out *out, const uint32_t **cigar,
+                   const char **md);
 /* the same over n chains on `threads` pthreads; chain k reads reads +
  * read_off[k] and contig chain_contig[k] of the genome */
 void ko_fast_dp_batch_mt(int32_t n, const uint8_t *reads, const int64_t *read_off,

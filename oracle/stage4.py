@@ -476,3 +476,160 @@ def run(contigs, reads, chains, cfg=None, repeat_min_occ=None):
 
     st["unmapped"] = sum(1 for a in alns if not a)
     return alns, st
+
+
+# ---------------------------------------------------------------- C driver
+# kira_stage4_oracle.c: the same steps with cpu_mode, in C on pthreads (the
+# CPU baseline timed beside the device driver).
+import ctypes as _C
+
+import numpy as _np
+
+_CHAIN = _np.dtype([("ref_id", _np.uint32), ("ref_start", _np.uint32), ("ref_end", _np.uint32),
+                    ("read_start", _np.uint32), ("read_end", _np.uint32),
+                    ("strand", _np.int32), ("score", _np.int32)])
+_SIMD = {"scalar": 0, "avx2": 1, "avxvnni": 2, "neon": 3}
+_ALN_KINDS = {"accepted_ungapped": 0, "dp_aligned": 1}
+_STATS = ("reads", "chains_total", "chains_used", "exact_matches", "prefilter_accept",
+          "prefilter_reject", "prefilter_fallback")
+_STATS_TAIL = ("gpu_spectral_resolved", "dp_simd", "dp_scalar", "dp_early_abort", "dp_reads",
+               "supplementary", "unmapped", "packed_spectral_resolved", "wfa_resolved")
+
+
+class _S4Cfg(_C.Structure):
+    _fields_ = ([("sw", O.Cfg)] + [(k, _C.c_int32) for k in ("adaptive", "max_drift", "dp_xdrop")]
+                + [("min_chain_ratio", _C.c_float)]
+                + [(k, _C.c_int32) for k in ("accept_enable", "accept_only_top1",
+                                              "accept_span_slack")]
+                + [("accept_min_identity", _C.c_float)]
+                + [(k, _C.c_int32) for k in (
+                    "accept_max_mismatches", "accept_require_score_margin", "dp_topk",
+                    "dp_abort_margin", "long_read_threshold", "max_alignments", "tail_ext",
+                    "ungap_min_id", "short_dp_topk", "ambig_div", "xs_min_ratio_pct", "simd",
+                    "supplementary", "twotier", "twotier_k", "ungap_gap_check")]
+                + [("router", F._Router)])
+
+
+class _S4In(_C.Structure):
+    _fields_ = [("n_reads", _C.c_int32), ("genome", _C.c_void_p), ("contig_off", _C.c_void_p),
+                ("contig_len", _C.c_void_p), ("reads", _C.c_void_p), ("read_off", _C.c_void_p),
+                ("read_len", _C.c_void_p), ("repeat_min_occ", _C.c_void_p),
+                ("chain_off", _C.c_void_p), ("chains", _C.c_void_p)]
+
+
+class _S4Rec(_C.Structure):
+    _fields_ = [(k, _C.c_int32) for k in ("read", "kind")] + \
+        [(k, _C.c_uint32) for k in ("ref_id", "ref_start", "ref_end", "read_start", "read_end")] + \
+        [(k, _C.c_int32) for k in ("score", "mapq", "is_rev", "is_supplementary")] + \
+        [("nm", _C.c_uint32)] + \
+        [(k, _C.c_int32) for k in ("has_xs", "xs_score", "n_cigar", "md_len")] + \
+        [("cigar_off", _C.c_int64), ("md_off", _C.c_int64)]
+
+
+class _S4Stats(_C.Structure):
+    _fields_ = [(k, _C.c_int64) for k in _STATS] + [("reason_counts", _C.c_int64 * 10)] + \
+        [(k, _C.c_int64) for k in _STATS_TAIL]
+
+
+class _S4View(_C.Structure):
+    _fields_ = [("n_reads", _C.c_int32), ("aln_off", _C.POINTER(_C.c_int64)),
+                ("alns", _C.POINTER(_S4Rec)), ("cigar", _C.POINTER(_C.c_uint32)),
+                ("md", _C.c_void_p), ("n_alns", _C.c_int64), ("cigar_words", _C.c_int64),
+                ("md_bytes", _C.c_int64), ("stats", _S4Stats)]
+
+
+def _c_lib():
+    L = O.lib()
+    if not getattr(L, "_s4_ready", False):
+        L.ko_stage4_run_mt.restype = _C.c_void_p
+        L.ko_stage4_run_mt.argtypes = [_C.POINTER(_S4In), _C.POINTER(_S4Cfg), _C.c_int]
+        L.ko_stage4_view.argtypes = [_C.c_void_p, _C.POINTER(_S4View)]
+        L.ko_stage4_free.argtypes = [_C.c_void_p]
+        L._s4_ready = True
+    return L
+
+
+def c_cfg(cfg):
+    """A default_cfg dict as the C driver's configuration (cpu_mode is the
+    only mode it has)."""
+    s = _S4Cfg()
+    s.sw = _sw_cfg(cfg)
+    for k in ("adaptive", "max_drift", "dp_xdrop", "min_chain_ratio", "accept_enable",
+              "accept_only_top1", "accept_span_slack", "accept_min_identity",
+              "accept_max_mismatches", "accept_require_score_margin", "dp_topk",
+              "dp_abort_margin", "long_read_threshold", "max_alignments", "tail_ext",
+              "ungap_min_id", "short_dp_topk", "ambig_div", "xs_min_ratio_pct", "supplementary",
+              "twotier", "twotier_k", "ungap_gap_check"):
+        setattr(s, k, type(getattr(s, k))(cfg[k]) if isinstance(getattr(s, k), float)
+                else int(cfg[k]))
+    s.simd = _SIMD[cfg["simd"]]
+    s.router = F._router(F.Router(**cfg["fast"]))
+    return s
+
+
+class CBatch:
+    """A batch as the flat arrays the C driver reads (built once, outside any
+    timing): genome bytes with contig offsets / lengths, reads, per-read
+    chain lists in stage-3 order, repeat_min_occ."""
+
+    def __init__(self, genome, contig_off, contig_len, reads, chains, repeat_min_occ=None):
+        n = len(reads)
+        self.n = n
+        self.g = _np.frombuffer(genome, _np.uint8)
+        self.co = _np.ascontiguousarray(contig_off, _np.int64)
+        self.cl = _np.ascontiguousarray(contig_len, _np.int64)
+        self.rl = _np.array([len(r) for r in reads], _np.int32)
+        self.ro = _np.zeros(n + 1, _np.int64)
+        self.ro[1:] = _np.cumsum(self.rl, dtype=_np.int64)
+        self.rb = _np.frombuffer(b"".join(reads) + b"\0" * 8, _np.uint8)
+        self.coff = _np.zeros(n + 1, _np.int64)
+        self.coff[1:] = _np.cumsum([len(c) for c in chains], dtype=_np.int64)
+        flat = [c for cs in chains for c in cs]
+        self.ch = _np.array(flat, _CHAIN) if flat else _np.zeros(1, _CHAIN)
+        self.rmo = _np.ascontiguousarray(repeat_min_occ if repeat_min_occ is not None
+                                         else [0] * n, _np.int32)
+        P = lambda a: a.ctypes.data_as(_C.c_void_p)
+        self.inp = _S4In(n, P(self.g), P(self.co), P(self.cl), P(self.rb), P(self.ro),
+                         P(self.rl), P(self.rmo), P(self.coff), P(self.ch))
+
+
+def run_c(batch: CBatch, cfg=None, threads=1, decode=True):
+    """The C driver over a CBatch on `threads` pthreads.  Returns (alignments
+    per read as run() gives them, stats dict), or None with decode=False
+    (timing)."""
+    cfg = cfg or default_cfg(cpu_mode=True)
+    if not cfg["cpu_mode"]:
+        raise ValueError("the C driver restates the configuration without the GPU "
+                         "dispatcher (cpu_mode=True)")
+    L = _c_lib()
+    c = c_cfg(cfg)
+    h = L.ko_stage4_run_mt(_C.byref(batch.inp), _C.byref(c), int(threads))
+    try:
+        if not decode:
+            return None
+        v = _S4View()
+        L.ko_stage4_view(h, _C.byref(v))
+        n = v.n_reads
+        off = _np.ctypeslib.as_array(v.aln_off, shape=(n + 1,)).copy()
+        cig = _np.ctypeslib.as_array(v.cigar, shape=(max(v.cigar_words, 1),))[:v.cigar_words] \
+            .copy()
+        md = _C.string_at(v.md, v.md_bytes) if v.md_bytes else b""
+        kinds = {k: name for name, k in _ALN_KINDS.items()}
+        alns = []
+        for r in range(n):
+            recs = []
+            for k in range(int(off[r]), int(off[r + 1])):
+                a = v.alns[k]
+                rec = _record(kinds[a.kind], a.ref_id, a.ref_start, a.ref_end, a.read_start,
+                              a.read_end, O.cigar_str(cig[a.cigar_off:a.cigar_off + a.n_cigar]),
+                              a.score, a.mapq, a.is_rev, a.nm,
+                              md[a.md_off:a.md_off + a.md_len].decode(), bool(a.is_supplementary))
+                rec["xs_score"] = a.xs_score if a.has_xs else None
+                recs.append(rec)
+            alns.append(recs)
+        s = v.stats
+        st = {k: getattr(s, k) for k in _STATS + _STATS_TAIL}
+        st["reason_counts"] = list(s.reason_counts)
+        return alns, st
+    finally:
+        L.ko_stage4_free(h)

@@ -275,3 +275,30 @@ def test_two_tier_reuses_the_verified_alignment():
     # a candidate the cascade cannot align is unknown, not poor: dp_limit 2
     alns, st = S.run([g], [read], [c(12)], cfg)
     assert [x["ref_start"] for x in alns[0]] == [8000, 2000] and st["wfa_resolved"] == 2
+
+
+def test_c_driver_gives_the_records_of_the_python_driver():
+    """kira_stage4_oracle.c (the CPU baseline's driver, on pthreads) against
+    this restatement with cpu_mode, record for record and counter for
+    counter, over configurations that move every branch (lane widths, the
+    two-tier reorder, the cascade's algorithms, the gap gate, top-k, XS),
+    and independent of the thread count."""
+    import pytest
+    from tests.stage4_data import make_batch
+    b = make_batch(11, 600, n_contigs=3, contig_len=200_000)
+    cb = S.CBatch(b["genome"], b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                  b["repeat_min_occ"])
+    variants = [dict(), dict(twotier=False), dict(simd="scalar"), dict(simd="avxvnni"),
+                dict(simd="neon"), dict(supplementary=False), dict(ungap_gap_check=True),
+                dict(fast={"algo": "wfa"}), dict(fast={"algo": "sw"}), dict(short_dp_topk=2),
+                dict(xs_min_ratio_pct=50), dict(dp_topk=3, long_read_threshold=100),
+                dict(bandwidth=16), dict(twotier_k=3, ambig_div=3),
+                dict(fast={"wfa_lead": 8, "quality_gate": True})]
+    for k, v in enumerate(variants):
+        cfg = S.default_cfg(cpu_mode=True, **v)
+        want = S.run(b["contigs"], b["reads"], b["chains"], cfg, b["repeat_min_occ"])
+        got = S.run_c(cb, cfg, threads=1 + k % 4)
+        assert got[1] == want[1], v
+        assert got[0] == want[0], v
+    with pytest.raises(ValueError):
+        S.run_c(cb, S.default_cfg())

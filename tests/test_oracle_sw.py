@@ -288,3 +288,36 @@ def test_int8_matches_truth_on_clean_reads():
         jobs.append(O.Job(r[15:115], r, 0, 0, 100, 0, 100, ABORT_NONE))
     for r in O.sw_group(jobs, c, bits=8):
         assert r["score"] == 100
+
+
+def test_avx2_group_form_equals_the_generic_lane_loop():
+    """align_batch_sw's AVX2 mode runs the 16-lane group in AVX2 registers;
+    the generic lane loop (sw_group, bits=16) must give the same results on
+    groups of 1-16 lanes with N bases, case, indels, unrelated reads, early
+    stops and several cost sets (gap open 0 included)."""
+    rng = random.Random(5)
+
+    def seq(n, alpha=b"ACGT"):
+        return bytes(rng.choice(alpha) for _ in range(n))
+
+    for _ in range(120):
+        L = rng.choice([20, 50, 100, 150])
+        W = L + rng.choice([0, 10, 64, 128])
+        n = rng.randint(1, 16)
+        jobs = []
+        for _ in range(n):
+            w = bytearray(seq(W, b"ACGTN" if rng.random() < 0.2 else b"ACGT"))
+            st = rng.randint(0, W - L)
+            rd = bytearray(w[st:st + L])
+            for _ in range(rng.randint(0, 12)):
+                rd[rng.randrange(L)] = rng.choice(b"ACGTNa")
+            if rng.random() < 0.3:
+                del rd[rng.randrange(1, L - 1)]
+                rd.append(rng.choice(b"ACGT"))
+            if rng.random() < 0.2:
+                rd = bytearray(seq(L))
+            ab = rng.choice([-(2 ** 31) // 4, -(2 ** 31) // 8, rng.randint(-20, L)])
+            jobs.append(O.Job(bytes(rd), bytes(w), 1000, 1000 + st, 1000 + st + L, 0, L, ab))
+        c = O.cfg(rng.choice([1, 2]), rng.choice([3, 4]), rng.choice([0, 4, 6]),
+                  rng.choice([1, 2]), 32, 100, 5)
+        assert O.align_batch_sw(jobs, c, "avx2") == O.sw_group(jobs, c, bits=16)
