@@ -905,7 +905,8 @@ def bench_c1(pkg, ctx, args):
     1 Mbp contig from the reference's test LCG, 10,000 single-end 150 bp reads
     (1% substitutions, 0.1% 1-3 bp indels, truth in the name), one chain per
     read from its truth span, band 32; reads/s per batch, truth recall of the
-    first record, and the CPU restatement on the same reads with parity."""
+    first record, parity with the Python restatement, and the restated
+    reference CPU path (the C driver) on the same reads."""
     from tests.stage4_data import make_c1
     from oracle import stage4 as S
     t0 = time.time()
@@ -934,16 +935,32 @@ def bench_c1(pkg, ctx, args):
            "reads": n, "ms_per_batch": t * 1e3, "reads_per_s": n / t,
            "truth_recall": hit / n}
     if not args.no_cpu:
-        tc, (want, _) = median_of_runs(
-            lambda: S.run(b["contigs"], b["reads"], b["chains"], S.default_cfg(bandwidth=32),
-                          b["repeat_min_occ"]), 3)
+        want, _ = S.run(b["contigs"], b["reads"], b["chains"], S.default_cfg(bandwidth=32),
+                        b["repeat_min_occ"])
         out["parity_vs_oracle"] = got == want
-        out["cpu_baseline"] = {"value": n / tc, "unit": "reads/s", "cores": 1, "kind": "port",
-                               "sample": f"all {n} reads through the stage-4 restatement "
-                                         "(Python driver over the C oracle), 1 warm-up + "
-                                         "median of 3"}
+        # the restated reference CPU path (the C driver without the GPU
+        # dispatcher) on every CPU of this process, with the device in the
+        # same configuration checked against it
+        T = len(os.sched_getaffinity(0))
+        ccfg = S.default_cfg(bandwidth=32, cpu_mode=True)
+        cb = S.CBatch(b["genome"], b["contig_off"], b["contig_len"], b["reads"], b["chains"],
+                      b["repeat_min_occ"])
+        tc, _ = median_of_runs(lambda: S.run_c(cb, ccfg, T, decode=False))
+        tc1, _ = median_of_runs(lambda: S.run_c(cb, ccfg, 1, decode=False))
+        want_cpu, _ = S.run_c(cb, ccfg, T)
+        res = pkg.stage4_run(ctx, genome, x, cfg=pkg.stage4_cfg(bandwidth=32, cpu_mode=True))
+        out["cpu_mode_parity_vs_c_driver"] = res.alignments() == want_cpu
+        res.close()
+        out["cpu_baseline"] = {"value": n / tc, "unit": "reads/s", "cores": T, "kind": "port",
+                               "label": "restated reference CPU path",
+                               "single_thread_value": n / tc1,
+                               "sample": f"all {n} reads through the stage-4 driver restated "
+                                         "in C without the GPU dispatcher "
+                                         f"(oracle/kira_stage4_oracle.c), {T} threads and 1 "
+                                         "thread; 1 warm-up + median of 5"}
     print(f"[bench] C1: {out['ms_per_batch']:.1f} ms per batch, {out['reads_per_s']:.0f} "
-          f"reads/s, recall {out['truth_recall']:.4f}, parity {out.get('parity_vs_oracle')}",
+          f"reads/s, recall {out['truth_recall']:.4f}, parity {out.get('parity_vs_oracle')} "
+          f"/ cpu mode {out.get('cpu_mode_parity_vs_c_driver')}",
           file=sys.stderr, flush=True)
     genome.close()
     return out

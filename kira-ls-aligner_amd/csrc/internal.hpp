@@ -286,6 +286,11 @@ struct GroupLaunch {
     const int32_t *cigar_cap;
     // scores and costs keep value << 4 inside int32 (the keyed maximum)
     int32_t keyed;
+    // the systolic DP (a job's strips on adjacent lanes; the trace kept by
+    // step): tasks (packet, first slot) of 64 / strips jobs
+    int32_t systolic;
+    const int2 *tasks;
+    int32_t n_tasks;
 };
 
 hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_groups, int n_waves,

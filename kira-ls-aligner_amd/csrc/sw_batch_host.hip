@@ -120,7 +120,11 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
         }
     }
     const int32_t P = (m + 63) / 64;
-    // pinned block: per-job arrays, then per-packet tables
+    // the systolic DP takes windows of up to 64 strips (1024 columns)
+    int32_t max_w = 0;
+    for (int32_t t = 0; t < m; t++) max_w = std::max<int32_t>(max_w, j.ref_len[t]);
+    const bool sys = max_w <= 64 * 16;
+    // pinned block: per-job arrays, then per-packet tables, then the tasks
     const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t m8 = al((size_t)m * 8), m4 = al((size_t)m * 4), p8 = al((size_t)P * 8),
                  p4 = al((size_t)P * 4);
@@ -129,7 +133,7 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
                  o_ccap = o_slot + m4, o_neg = o_ccap + m4,
                  o_pj = o_neg + al((size_t)std::max(ng, 1) * 4), o_ptoff = o_pj + P * 256,
                  o_proff = o_ptoff + p8, o_prows = o_proff + p8, o_pwords = o_prows + p4,
-                 total = o_pwords + p4;
+                 o_tasks = o_pwords + p4, total = o_tasks + (sys ? (size_t)P * 64 * 8 : 0);
     void *hp = nullptr;
     hipError_t e;
     if ((e = pinned_buf(ctx, slot, total, &hp)))
@@ -140,6 +144,8 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
     int32_t *slot_of = (int32_t *)(h + o_slot), *ccap = (int32_t *)(h + o_ccap),
             *pj = (int32_t *)(h + o_pj), *prows = (int32_t *)(h + o_prows),
             *pwords = (int32_t *)(h + o_pwords);
+    int2 *tasks = (int2 *)(h + o_tasks);
+    int32_t n_tasks = 0;
     int64_t toff = 0, roff = 0;
     for (int32_t pk = 0; pk < P; pk++) {
         const int32_t s0 = pk * 64, e0 = std::min(m, s0 + 64);
@@ -158,7 +164,14 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
         pwords[pk] = words;
         ptoff[pk] = toff;
         proff[pk] = roff;
-        toff += (int64_t)rows * words * 64;
+        if (sys) {
+            // per slot (rows + G - 1) steps of `words` words; tasks of 64 / G jobs
+            const int32_t G = std::max(words / 2, 1), per = 64 / G;
+            toff += (int64_t)(rows + G - 1) * words * 64;
+            for (int32_t l = 0; l < e0 - s0; l += per) tasks[n_tasks++] = make_int2(pk, l);
+        } else {
+            toff += (int64_t)rows * words * 64;
+        }
         roff += (int64_t)rows * 64;
     }
     int64_t cw = 0;
@@ -183,7 +196,7 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
     if ((e = dm.alloc(&d_blob, total)) || (e = dm.alloc(&d_gstop, std::max(ng, 1))) ||
         (e = dm.alloc(&d_counter, 1)) || (e = dm.alloc(&d_res, (size_t)m * kSwResWords)) ||
         (e = dm.alloc(&d_trace, (size_t)toff)) || (e = dm.alloc(&d_cig, (size_t)cw)) ||
-        (e = dm.alloc(&d_bnd, (size_t)roff)) || (e = dm.alloc(&d_rstat, (size_t)roff)))
+        (e = dm.alloc(&d_bnd, sys ? 1 : (size_t)roff)) || (e = dm.alloc(&d_rstat, (size_t)roff)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "lane-group allocation", e);
     hipStream_t s = ctx->stream;
     if ((e = hipMemcpyAsync(d_blob, h, total, hipMemcpyHostToDevice, s)))
@@ -233,7 +246,13 @@ kira_status GroupRun::launch(const SwJobsHost &j, const int32_t *group_of,
                   cfg.mismatch <= kLim && cfg.gap_open >= 0 && cfg.gap_extend >= 0 &&
                   a.go <= kLim && hmax < (int64_t(1) << 27);
     }
-    if ((e = launch_group_sw(a, m, ng, std::min(P, 256 * 16), s)))
+    a.systolic = sys;
+    a.tasks = (const int2 *)(d_blob + o_tasks);
+    a.n_tasks = n_tasks;
+    // persistent waves on the atomic task counter: the one-lane form at most
+    // 16 a CU, the systolic form (≈90 VGPRs) at most 20 a CU
+    const int n_waves = sys ? std::min(n_tasks, 256 * 20) : std::min(P, 256 * 16);
+    if ((e = launch_group_sw(a, m, ng, n_waves, s)))
         return fail(ctx, KIRA_ERR_RUNTIME, "lane-group launch", e);
     trace_mark(".grp_launch");
     return KIRA_OK;

@@ -73,3 +73,41 @@ def test_align_batch_matches_oracle(pkg, ctx, mode, detected, cfgv):
     for b, (jobs, g) in enumerate(zip(batches, got)):
         want = O.align_batch_sw([O.Job(*j) for j in jobs], c, mode, detected)
         assert g == want, (b, len(jobs))
+
+
+def _jobs_of(rng, n, L, W, A):
+    jobs = []
+    base = bytes(rng.choice(b"ACGT") for _ in range(W + 64))
+    for _ in range(n):
+        st = rng.randrange(0, 32)
+        window = base[st:st + W]
+        p = rng.randrange(0, max(W - L, 0) + 1)
+        read = bytearray(window[p:p + L])
+        while len(read) < L:
+            read.append(rng.choice(b"ACGT"))
+        for _ in range(rng.randrange(0, 6)):
+            read[rng.randrange(L)] = rng.choice(b"ACGTN")
+        abort = rng.choice([AN, NEG, 0, L // 2, L * A + 1])
+        jobs.append((bytes(read), window, 5000, 5000 + p, 5000 + p + L, 0, L, abort))
+    return jobs
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_align_batch_window_layouts_and_empty_windows(pkg, ctx, wide):
+    """Windows up to 1024 columns run the systolic DP (a job's 16-column
+    strips on adjacent lanes), wider ones the one-lane-per-job DP; a call
+    with one window over 1024 columns takes the second for every job.  Both
+    against the oracle, with empty windows and groups of mixed read lengths
+    in one call."""
+    rng = random.Random(77 + wide)
+    cfgv = (1, 4, 6, 1, 32, 100, 5)
+    batches = [_jobs_of(rng, 20, 150, 278, 1), _jobs_of(rng, 5, 40, 0, 1),
+               _jobs_of(rng, 33, 120, 248, 1), _jobs_of(rng, 3, 16, 16, 1),
+               _jobs_of(rng, 7, 150, 1000, 1)]
+    if wide:
+        batches.append(_jobs_of(rng, 4, 100, 1100, 1))
+    got = pkg.align_batch_simd(ctx, batches, pkg.sw_cfg(*cfgv), "avx2", "avx2")
+    c = O.cfg(*cfgv)
+    for b, (jobs, g) in enumerate(zip(batches, got)):
+        want = O.align_batch_sw([O.Job(*j) for j in jobs], c, "avx2", "avx2")
+        assert g == want, b
