@@ -491,13 +491,16 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         }
         st_local.phase_ms[1] = ms_since(t_my);
         trace_mark(".fp_myers");
-        // ---- scored WFA (wfa_score_budget, router.rs:160-166)
+        // ---- scored WFA (wfa_score_budget, router.rs:160-166).  The records
+        // are built while the device runs: the certificates' during the main
+        // pass, the WFA alignments that need no retry during the retry pass,
+        // the retried ones last.
         auto t_wfa = std::chrono::steady_clock::now();
         const int32_t drop = kn.wfa_adaptive > 0 ? kn.wfa_adaptive : 0;
-        auto run_wfa = [&](const std::vector<int32_t> &list, const std::vector<int64_t> &tofs,
-                           const std::vector<int32_t> &tlens, const std::vector<int32_t> &budget,
-                           const std::vector<int32_t> &tbf, std::vector<int32_t> &wres,
-                           std::vector<uint32_t> &wcig, std::vector<int64_t> &woff) -> kira_status {
+        auto launch_wfa = [&](const std::vector<int32_t> &list, const std::vector<int64_t> &tofs,
+                              const std::vector<int32_t> &tlens,
+                              const std::vector<int32_t> &budget, const std::vector<int32_t> &tbf,
+                              kira::WfaRun &run) -> kira_status {
             if ((e = lists.begin(ctx, list.size())))
                 return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "fast path staging", e);
             parallel_chunks((int64_t)list.size(), 16384, [&](int, int64_t b, int64_t e0) {
@@ -529,7 +532,111 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             wj.h_txt_len = lists.tlen;
             wj.h_max_score = lists.x;
             wj.h_tbf = lists.y;
-            return kira::wfa_dev(ctx, wj, pen, drop, true, wres, wcig, woff);
+            return run.launch(wj, pen, drop, true);
+        };
+        // ---- records (host): partial-read check, wfa_result_to_alignment;
+        // per phase (0 certificates, 1 WFA without retry, 2 retried WFA) one
+        // part per thread over the same ranges of jobs
+        const int T = host_threads();
+        struct Part {
+            std::vector<uint32_t> cig;
+            std::string md;
+            int64_t spectral = 0, wfa = 0, declined = 0;
+        };
+        std::vector<Part> parts((size_t)3 * T);
+        const int64_t nj = (int64_t)jobs.size();
+        std::vector<int64_t> bounds(T + 1, nj);
+        for (int t = 0; t <= T; t++) bounds[t] = nj * t / T;
+        auto phase_of = [](const Job &j) -> int {
+            return j.stage == 1 ? 0 : (j.stage == 2 ? (j.retry < 0 ? 1 : 2) : -1);
+        };
+        auto record = [&](int64_t q, Part &P) {
+            const Job &j = jobs[q];
+            kira_fast_aln &a = res[j.k];
+            const uint8_t *rd = reads + read_off[j.k];
+            if (j.stage == 1) {
+                const uint8_t *ref = gh + contig_off[j.contig] + j.ws + j.shift;
+                a.kind = 1;
+                a.ref_start = (uint32_t)(j.ws + j.shift);
+                a.ref_end = a.ref_start + (uint32_t)j.L;
+                a.read_start = 0;
+                a.read_end = (uint32_t)j.L;
+                a.score = (j.L - j.mism) * c.match_score - j.mism * c.mismatch;
+                a.cigar_off = (int64_t)P.cig.size();
+                a.n_cigar = 1;
+                P.cig.push_back((uint32_t)j.L << 4);
+                a.md_off = (int64_t)P.md.size();
+                if (want_md) a.nm = kira::ungapped_tags(rd, ref, j.L, P.md);
+                a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
+                P.spectral++;
+                return;
+            }
+            if (j.stage != 2) return;
+            const uint8_t *text = gh + j.ftext;
+            const uint32_t *ops = j.ops;
+            const int64_t perfect = std::min<int64_t>(
+                std::max<int64_t>((int64_t)c.match_score * j.L, INT32_MIN), INT32_MAX);
+            if ((int64_t)j.loss * 10 > perfect * 3 &&
+                clipping_can_improve(rd, j.L, text, j.ftext_len, j.text_start, ops,
+                                     j.n_ops, c)) {
+                P.declined++;
+                return;
+            }
+            int64_t cref = 0, cq = 0;
+            for (int32_t z = 0; z < j.n_ops; z++) {
+                const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
+                if (op == 0 || op == 2 || op == 3) cref += len;
+                if (op == 0 || op == 1 || op == 4) cq += len;
+            }
+            if (cq != j.L || j.text_start + cref > j.ftext_len || j.text_end > j.ftext_len) {
+                P.declined++;
+                return;
+            }
+            int32_t score = 0;
+            int64_t qi = 0, ti = j.text_start;
+            for (int32_t z = 0; z < j.n_ops; z++) {
+                const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
+                if (op == 0) {
+                    // the per-column sum (wrapping like the reference's i32)
+                    const uint32_t eq = count_equal_bytes(rd + qi, text + ti, len);
+                    score = (int32_t)((uint32_t)score + eq * (uint32_t)c.match_score -
+                                      (len - eq) * (uint32_t)c.mismatch);
+                    qi += len;
+                    ti += len;
+                } else if (op == 1 || op == 2) {
+                    score -= c.gap_open + c.gap_extend * (int32_t)len;
+                    if (op == 1)
+                        qi += len;
+                    else
+                        ti += len;
+                }
+            }
+            const int64_t fstart = j.ftext - contig_off[j.contig];
+            a.kind = 2;
+            a.ref_start = (uint32_t)(fstart + j.text_start);
+            a.ref_end = (uint32_t)(fstart + j.text_end);
+            a.read_start = 0;
+            a.read_end = (uint32_t)j.L;
+            a.score = score;
+            a.cigar_off = (int64_t)P.cig.size();
+            a.n_cigar = j.n_ops;
+            P.cig.insert(P.cig.end(), ops, ops + j.n_ops);
+            a.md_off = (int64_t)P.md.size();
+            if (want_md)
+                a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops,
+                                   P.md);
+            a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
+            P.wfa++;
+        };
+        double rec_ms = 0;
+        auto build = [&](int ph) {
+            const auto t0 = std::chrono::steady_clock::now();
+            parallel_chunks(T, 1, [&](int, int64_t t0_, int64_t t1_) {
+                for (int64_t t = t0_; t < t1_; t++)
+                    for (int64_t q = bounds[t]; q < bounds[t + 1]; q++)
+                        if (phase_of(jobs[q]) == ph) record(q, parts[(size_t)ph * T + t]);
+            });
+            rec_ms += ms_since(t0);
         };
         // the CIGARs of both passes stay alive until the records are built
         std::vector<int32_t> wres1, wres2;
@@ -537,6 +644,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
         std::vector<int64_t> woff1, woff2;
         std::vector<int32_t> retry_jobs, retry_len, retry_free;
         std::vector<int64_t> retry_text;
+        kira::WfaRun main_run(ctx), retry_run(ctx);
         if (!wfa_jobs.empty()) {
             const size_t m = wfa_jobs.size();
             std::vector<int64_t> tofs(m);
@@ -558,7 +666,13 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 j.ftext_len = tlens[i];
             }
             });
-            kira_status rs = run_wfa(wfa_jobs, tofs, tlens, budget, tbf, wres1, wcig1, woff1);
+            kira_status rs = launch_wfa(wfa_jobs, tofs, tlens, budget, tbf, main_run);
+            if (rs != KIRA_OK) return rs;
+        }
+        build(0);  // the certificates, while the main pass runs
+        if (!wfa_jobs.empty()) {
+            const size_t m = wfa_jobs.size();
+            kira_status rs = main_run.finish(wres1, wcig1, woff1);
             if (rs != KIRA_OK) return rs;
             std::vector<int32_t> suspects;
             for (size_t i = 0; i < m; i++) {
@@ -584,6 +698,7 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 j.retry = (int32_t)suspects.size();
                 suspects.push_back(wfa_jobs[i]);
             }
+            trace_mark(".wfa_suspects");
             // the origin probe (mod.rs:389-399): the longest exact read prefix
             // among the starts within the slack
             const size_t ns = suspects.size();
@@ -619,14 +734,21 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                     }
                 }
             });
+            trace_mark(".wfa_probe");
         }
         // ---- the bounded ends-free retry (mod.rs:384-426)
         if (!retry_jobs.empty()) {
             const size_t m = retry_jobs.size();
             std::vector<int32_t> budget(m);
             for (size_t i = 0; i < m; i++) budget[i] = jobs[retry_jobs[i]].first_loss;
-            kira_status rs = run_wfa(retry_jobs, retry_text, retry_len, budget, retry_free, wres2,
-                                     wcig2, woff2);
+            kira_status rs = launch_wfa(retry_jobs, retry_text, retry_len, budget, retry_free,
+                                        retry_run);
+            if (rs != KIRA_OK) return rs;
+        }
+        build(1);  // the WFA alignments that need no retry, while the retry pass runs
+        if (!retry_jobs.empty()) {
+            const size_t m = retry_jobs.size();
+            kira_status rs = retry_run.finish(wres2, wcig2, woff2);
             if (rs != KIRA_OK) return rs;
             for (size_t i = 0; i < m; i++) {
                 Job &j = jobs[retry_jobs[i]];
@@ -642,129 +764,46 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                 j.ftext_len = retry_len[i];
             }
         }
-        st_local.phase_ms[2] = ms_since(t_wfa);
         trace_mark(".fp_wfa");
-        // ---- records (host): partial-read check, wfa_result_to_alignment
-        auto t_rec = std::chrono::steady_clock::now();
-        const int T = host_threads();
-        struct Part {
-            std::vector<uint32_t> cig;
-            std::string md;
-            int64_t spectral = 0, wfa = 0, declined = 0;
-        };
-        std::vector<Part> parts(T);
-        const int64_t nj = (int64_t)jobs.size();
-        std::vector<int64_t> bounds(T + 1, nj);
-        for (int t = 0; t <= T; t++) bounds[t] = nj * t / T;
+        build(2);
+        st_local.phase_ms[2] = ms_since(t_wfa) - rec_ms;
+        // place each part's CIGAR words and MD text after the previous parts'
+        int64_t cig_total = 0, md_total = 0;
+        std::vector<int64_t> cig_base((size_t)3 * T), md_base((size_t)3 * T);
+        for (size_t x = 0; x < parts.size(); x++) {
+            cig_base[x] = cig_total;
+            md_base[x] = md_total;
+            cig_total += (int64_t)parts[x].cig.size();
+            md_total += (int64_t)parts[x].md.size();
+            st_local.spectral += parts[x].spectral;
+            st_local.wfa += parts[x].wfa;
+            st_local.clip_declined += parts[x].declined;
+        }
+        cig_out.resize((size_t)cig_total);
+        md_out.resize((size_t)md_total);
         parallel_chunks(T, 1, [&](int, int64_t t0_, int64_t t1_) {
             for (int64_t t = t0_; t < t1_; t++) {
-                Part &P = parts[t];
                 for (int64_t q = bounds[t]; q < bounds[t + 1]; q++) {
-                    const Job &j = jobs[q];
-                    kira_fast_aln &a = res[j.k];
-                    const uint8_t *rd = reads + read_off[j.k];
-                    if (j.stage == 1) {
-                        const uint8_t *ref = gh + contig_off[j.contig] + j.ws + j.shift;
-                        a.kind = 1;
-                        a.ref_start = (uint32_t)(j.ws + j.shift);
-                        a.ref_end = a.ref_start + (uint32_t)j.L;
-                        a.read_start = 0;
-                        a.read_end = (uint32_t)j.L;
-                        a.score = (j.L - j.mism) * c.match_score - j.mism * c.mismatch;
-                        a.cigar_off = (int64_t)P.cig.size();
-                        a.n_cigar = 1;
-                        P.cig.push_back((uint32_t)j.L << 4);
-                        a.md_off = (int64_t)P.md.size();
-                        if (want_md) a.nm = kira::ungapped_tags(rd, ref, j.L, P.md);
-                        a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
-                        P.spectral++;
-                        continue;
-                    }
-                    if (j.stage != 2) continue;
-                    const uint8_t *text = gh + j.ftext;
-                    const uint32_t *ops = j.ops;
-                    const int64_t perfect = std::min<int64_t>(
-                        std::max<int64_t>((int64_t)c.match_score * j.L, INT32_MIN), INT32_MAX);
-                    if ((int64_t)j.loss * 10 > perfect * 3 &&
-                        clipping_can_improve(rd, j.L, text, j.ftext_len, j.text_start, ops,
-                                             j.n_ops, c)) {
-                        P.declined++;
-                        continue;
-                    }
-                    int64_t cref = 0, cq = 0;
-                    for (int32_t z = 0; z < j.n_ops; z++) {
-                        const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
-                        if (op == 0 || op == 2 || op == 3) cref += len;
-                        if (op == 0 || op == 1 || op == 4) cq += len;
-                    }
-                    if (cq != j.L || j.text_start + cref > j.ftext_len || j.text_end > j.ftext_len) {
-                        P.declined++;
-                        continue;
-                    }
-                    int32_t score = 0;
-                    int64_t qi = 0, ti = j.text_start;
-                    for (int32_t z = 0; z < j.n_ops; z++) {
-                        const uint32_t op = ops[z] & 15u, len = ops[z] >> 4;
-                        if (op == 0) {
-                            // the per-column sum (wrapping like the reference's i32)
-                            const uint32_t eq = count_equal_bytes(rd + qi, text + ti, len);
-                            score = (int32_t)((uint32_t)score + eq * (uint32_t)c.match_score -
-                                              (len - eq) * (uint32_t)c.mismatch);
-                            qi += len;
-                            ti += len;
-                        } else if (op == 1 || op == 2) {
-                            score -= c.gap_open + c.gap_extend * (int32_t)len;
-                            if (op == 1)
-                                qi += len;
-                            else
-                                ti += len;
-                        }
-                    }
-                    const int64_t fstart = j.ftext - contig_off[j.contig];
-                    a.kind = 2;
-                    a.ref_start = (uint32_t)(fstart + j.text_start);
-                    a.ref_end = (uint32_t)(fstart + j.text_end);
-                    a.read_start = 0;
-                    a.read_end = (uint32_t)j.L;
-                    a.score = score;
-                    a.cigar_off = (int64_t)P.cig.size();
-                    a.n_cigar = j.n_ops;
-                    P.cig.insert(P.cig.end(), ops, ops + j.n_ops);
-                    a.md_off = (int64_t)P.md.size();
-                    if (want_md)
-                        a.nm = kira::nm_md(rd, j.L, text, j.ftext_len, j.text_start, ops, j.n_ops,
-                                           P.md);
-                    a.md_len = (int32_t)((int64_t)P.md.size() - a.md_off);
-                    P.wfa++;
+                    kira_fast_aln &a = res[jobs[q].k];
+                    const int ph = phase_of(jobs[q]);
+                    if (a.kind == 0 || ph < 0) continue;
+                    a.cigar_off += cig_base[(size_t)ph * T + t];
+                    a.md_off += md_base[(size_t)ph * T + t];
+                }
+                for (int ph = 0; ph < 3; ph++) {
+                    const Part &P = parts[(size_t)ph * T + t];
+                    if (!P.cig.empty())
+                        memcpy(cig_out.data() + cig_base[(size_t)ph * T + t], P.cig.data(),
+                               4 * P.cig.size());
+                    if (!P.md.empty())
+                        memcpy(&md_out[(size_t)md_base[(size_t)ph * T + t]], P.md.data(),
+                               P.md.size());
                 }
             }
         });
-        // place each part's CIGAR words and MD text after the previous parts'
-        int64_t cig_total = 0, md_total = 0;
-        std::vector<int64_t> cig_base(T), md_base(T);
-        for (int t = 0; t < T; t++) {
-            cig_base[t] = cig_total;
-            md_base[t] = md_total;
-            cig_total += (int64_t)parts[t].cig.size();
-            md_total += (int64_t)parts[t].md.size();
-            st_local.spectral += parts[t].spectral;
-            st_local.wfa += parts[t].wfa;
-            st_local.clip_declined += parts[t].declined;
-        }
-        for (int t = 0; t < T; t++)
-            for (int64_t q = bounds[t]; q < bounds[t + 1]; q++) {
-                kira_fast_aln &a = res[jobs[q].k];
-                if (a.kind == 0) continue;
-                a.cigar_off += cig_base[t];
-                a.md_off += md_base[t];
-            }
-        cig_out.reserve(cig_total);
-        md_out.reserve(md_total);
-        for (int t = 0; t < T; t++) {
-            cig_out.insert(cig_out.end(), parts[t].cig.begin(), parts[t].cig.end());
-            md_out += parts[t].md;
-        }
-        st_local.phase_ms[3] = ms_since(t_rec);
+        // host time of building the records (the first two phases overlap
+        // the WFA kernels)
+        st_local.phase_ms[3] = rec_ms;
         trace_mark(".fp_records");
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");

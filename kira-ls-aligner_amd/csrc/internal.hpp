@@ -726,6 +726,26 @@ constexpr int kWfaRes = 6;  // found, loss, text_start, text_end, n_cigar, statu
 kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32_t drop,
                     bool want_cigar, std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
                     std::vector<int64_t> &cig_off);
+// wfa_dev in two halves, so the host can work while the kernel runs:
+// launch() enqueues the pairs on the context stream and returns; finish()
+// waits and brings the results back.  The job arrays (device and host copies)
+// must stay as they are until launch() returns, the device ones until
+// finish(); no other work may be queued on the context stream in between.
+struct WfaRun {
+    kira_ctx *ctx;
+    DevMem dm;
+    int32_t n = 0;
+    bool want_cigar = false;
+    std::vector<int64_t> cig_off;
+    std::vector<int32_t> cig_cap;
+    int32_t *d_res = nullptr;
+    int64_t *d_coff = nullptr;
+    uint32_t *d_cig = nullptr;
+    explicit WfaRun(kira_ctx *c) : ctx(c), dm(c) {}
+    kira_status launch(const WfaDevJobs &j, const WfaPen &pen, int32_t drop, bool want_cigar);
+    kira_status finish(std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
+                       std::vector<int64_t> &cig_off);
+};
 // bounded_edit_distance for n pairs on device arrays into device found /
 // dist / end; h_pat_len is the host copy of pat_len.  Synchronous.
 kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_t *pat_off,

@@ -1499,24 +1499,45 @@ kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_conti
         (n_reads > 0 && (!reads || !read_off || !read_len || !chain_off)))
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid argument");
     *out = nullptr;
+    kira::StepTrace trace("stage4_call");
     const int64_t glen = (int64_t)g->host.size();
     for (int32_t i = 0; i < n_contigs; i++)
         if (contig_off[i] < 0 || contig_len[i] < 0 || contig_off[i] + contig_len[i] > glen)
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "contig outside the genome");
     if (n_reads > 0 && (chain_off[0] != 0 || (chain_off[n_reads] > 0 && !chains)))
         return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid chain offsets");
-    for (int32_t r = 0; r < n_reads; r++) {
+    // 0 when read r and its chains are well formed, else the error
+    auto check_read = [&](int32_t r) -> const char * {
         if (read_len[r] < 0 || read_off[r] < 0 || chain_off[r + 1] < chain_off[r])
-            return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "invalid read or chain offsets");
+            return "invalid read or chain offsets";
         for (int64_t k = chain_off[r]; k < chain_off[r + 1]; k++) {
             const kira_chain &ch = chains[k];
             if (ch.ref_id >= (uint32_t)n_contigs || contig_len[ch.ref_id] == 0 ||
                 ch.read_start > (uint32_t)read_len[r] || ch.read_end > (uint32_t)read_len[r] ||
                 ch.ref_start > contig_len[ch.ref_id] || ch.ref_end > contig_len[ch.ref_id] ||
                 (ch.strand != 0 && ch.strand != 1))
-                return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "chain outside its read or contig");
+                return "chain outside its read or contig";
         }
+        return nullptr;
+    };
+    // in parallel over reads; the first bad read (by index) names the error
+    std::atomic<int32_t> first_bad{INT32_MAX};
+    try {
+        kira::parallel_chunks(n_reads, 8192, [&](int, int64_t b, int64_t e) {
+            for (int64_t r = b; r < e; r++)
+                if (check_read((int32_t)r)) {
+                    int32_t cur = first_bad.load();
+                    while ((int32_t)r < cur && !first_bad.compare_exchange_weak(cur, (int32_t)r)) {
+                    }
+                    break;
+                }
+        });
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    if (first_bad.load() != INT32_MAX)
+        return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, check_read(first_bad.load()));
+    kira::trace_mark("validate");
     kira_stage4_result *res = new (std::nothrow) kira_stage4_result();
     if (!res) return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "result allocation");
     kira_status st;
@@ -1531,9 +1552,11 @@ kira_status kira_stage4_run(kira_ctx *ctx, const kira_genome *g, int32_t n_conti
         d.rmo = repeat_min_occ;
         d.mem.ctx = ctx;
         st = d.run();
+        kira::trace_mark("run");
     } catch (const std::bad_alloc &) {
         st = fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    kira::trace_mark("teardown");
     if (st != KIRA_OK) {
         delete res;
         return st;

@@ -812,17 +812,9 @@ kira_status myers_dev(kira_ctx *ctx, int32_t n, const uint8_t *pat, const int64_
     return KIRA_OK;
 }
 
-kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32_t drop,
-                    bool want_cigar, std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
-                    std::vector<int64_t> &cig_out) {
-    const int32_t n = j.n;
-    try {
-        res.assign((size_t)n * kWfaRes, 0);
-        cig_out.assign(n, 0);
-        cigar.clear();
-    } catch (const std::bad_alloc &) {
-        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
-    }
+kira_status WfaRun::launch(const WfaDevJobs &j, const WfaPen &pen, int32_t drop, bool want) {
+    n = j.n;
+    want_cigar = want;
     if (n == 0) return KIRA_OK;
     int32_t m_max = 0, n_max = 0, tbf_max = 0, s_max = 0;
     for (int32_t k = 0; k < n; k++) {
@@ -838,8 +830,6 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     }
     // CIGAR slots: every op has length >= 1 (<= m + n ops), and op runs
     // alternate between M and gaps costing >= o + min(ie, de) each
-    std::vector<int64_t> cig_off;
-    std::vector<int32_t> cig_cap;
     int64_t cig_total = 0;
     try {
         cig_off.resize(n);
@@ -872,10 +862,7 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
             return fail(ctx, KIRA_ERR_INVALID_ARGUMENT, "WFA history of one pair above 16 GiB");
         slots = 1;
     }
-    DevMem dm(ctx);
-    int64_t *d_coff = nullptr, *d_dst = nullptr;
-    int32_t *d_res = nullptr, *d_ccap = nullptr, *d_slab = nullptr, *d_queue = nullptr;
-    uint32_t *d_cig = nullptr, *d_cig2 = nullptr;
+    int32_t *d_ccap = nullptr, *d_slab = nullptr, *d_queue = nullptr;
     hipStream_t st = ctx->stream;
     hipError_t e;
     if ((e = dm.alloc(&d_res, (size_t)n * kWfaRes)) || (e = dm.alloc(&d_coff, n)) ||
@@ -913,6 +900,21 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     trace_mark(".wfa_prep");
     hipLaunchKernelGGL(wfa_kernel, dim3((uint32_t)slots), dim3(64), lds, st, a);
     if ((e = hipGetLastError())) return fail(ctx, KIRA_ERR_RUNTIME, "WFA launch", e);
+    return KIRA_OK;
+}
+
+kira_status WfaRun::finish(std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
+                           std::vector<int64_t> &cig_out) {
+    try {
+        res.assign((size_t)n * kWfaRes, 0);
+        cig_out.assign(n, 0);
+        cigar.clear();
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    if (n == 0) return KIRA_OK;
+    hipStream_t st = ctx->stream;
+    hipError_t e;
     if ((e = hipMemcpyAsync(res.data(), d_res, 4 * res.size(), hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
         return fail(ctx, KIRA_ERR_RUNTIME, "WFA", e);
@@ -933,6 +935,8 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
     } catch (const std::bad_alloc &) {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "host allocation");
     }
+    int64_t *d_dst = nullptr;
+    uint32_t *d_cig2 = nullptr;
     if ((e = dm.alloc(&d_dst, n)) || (e = dm.alloc(&d_cig2, used)))
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "WFA allocation", e);
     if ((e = hipMemcpyAsync(d_dst, dst.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st)))
@@ -944,6 +948,15 @@ kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32
         (e = hipStreamSynchronize(st)))
         return fail(ctx, KIRA_ERR_RUNTIME, "WFA", e);
     return KIRA_OK;
+}
+
+kira_status wfa_dev(kira_ctx *ctx, const WfaDevJobs &j, const WfaPen &pen, int32_t drop,
+                    bool want_cigar, std::vector<int32_t> &res, std::vector<uint32_t> &cigar,
+                    std::vector<int64_t> &cig_out) {
+    WfaRun run(ctx);
+    kira_status st = run.launch(j, pen, drop, want_cigar);
+    if (st != KIRA_OK) return st;
+    return run.finish(res, cigar, cig_out);
 }
 
 }  // namespace kira
