@@ -441,16 +441,18 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
                     (e = hipStreamSynchronize(st)))
                     return fail(ctx, KIRA_ERR_RUNTIME, "fast path spectral", e);
                 trace_mark(".fp_sp_sync");
-                for (int32_t q = 0; q < m; q++)
-                    if (so[3 * q]) {
-                        jobs[q].stage = 1;
-                        jobs[q].shift = so[3 * q + 1];
-                        jobs[q].mism = so[3 * q + 2];
-                    }
+                parallel_chunks(m, 16384, [&](int, int64_t b, int64_t e0) {
+                    for (int64_t q = b; q < e0; q++)
+                        if (so[3 * q]) {
+                            jobs[q].stage = 1;
+                            jobs[q].shift = so[3 * q + 1];
+                            jobs[q].mism = so[3 * q + 2];
+                        }
+                });
             }
         }
-        for (int32_t q = 0; q < (int32_t)jobs.size(); q++)
-            if (jobs[q].stage == 0) cand.push_back(q);
+        parallel_filter((int64_t)jobs.size(), 16384,
+                        [&](int64_t q) { return jobs[q].stage == 0; }, cand);
         st_local.phase_ms[0] = ms_since(t_sp);
         trace_mark(".fp_spectral");
         auto t_my = std::chrono::steady_clock::now();
@@ -485,8 +487,10 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             if ((e = hipMemcpyAsync(found.data(), d_found, m, hipMemcpyDeviceToHost, st)) ||
                 (e = hipStreamSynchronize(st)))
                 return fail(ctx, KIRA_ERR_RUNTIME, "fast path Myers", e);
-            for (int32_t i = 0; i < m; i++)
-                if (found[i]) wfa_jobs.push_back(cand[i]);
+            parallel_filter(m, 16384, [&](int64_t i) { return found[i] != 0; }, wfa_jobs);
+            parallel_chunks((int64_t)wfa_jobs.size(), 16384, [&](int, int64_t b, int64_t e0) {
+                for (int64_t i = b; i < e0; i++) wfa_jobs[i] = cand[wfa_jobs[i]];
+            });
             st_local.myers_pass = (int64_t)wfa_jobs.size();
         }
         st_local.phase_ms[1] = ms_since(t_my);
@@ -674,30 +678,41 @@ kira_status fast_dp_run(kira_ctx *ctx, const kira_genome *g, const int64_t *cont
             const size_t m = wfa_jobs.size();
             kira_status rs = main_run.finish(wres1, wcig1, woff1);
             if (rs != KIRA_OK) return rs;
+            std::vector<uint8_t> is_suspect(m, 0);
+            parallel_chunks((int64_t)m, 4096, [&](int, int64_t b0, int64_t e0) {
+                for (int64_t i = b0; i < e0; i++) {
+                    Job &j = jobs[wfa_jobs[i]];
+                    const int32_t *r = &wres1[i * kira::kWfaRes];
+                    j.retry = -1;
+                    if (!r[0]) continue;
+                    j.stage = 2;
+                    j.loss = j.first_loss = r[1];
+                    j.text_start = r[2];
+                    j.text_end = r[3];
+                    j.n_ops = r[4];
+                    j.ops = wcig1.data() + woff1[i];
+                    // suspect origin (mod.rs:373-383): a leading gap, or a
+                    // short leading match followed by more ops
+                    const uint32_t f = j.n_ops ? j.ops[0] : 0;
+                    const bool suspect =
+                        j.n_ops > 0 && ((f & 15u) != 0 || ((f >> 4) < 19 && j.n_ops > 1));
+                    const int64_t lead = std::min<int64_t>(
+                        {(int64_t)std::max(kn.wfa_lead, 0), j.ws, (int64_t)band});
+                    if (!(suspect && lead == 0 && kn.wfa_ends_free == 0)) continue;
+                    if (std::min<int64_t>({(int64_t)band, 32, j.ws}) <= 0) continue;
+                    is_suspect[i] = 1;
+                }
+            });
+            // the suspects in pass order; each job knows its retry slot
             std::vector<int32_t> suspects;
-            for (size_t i = 0; i < m; i++) {
-                Job &j = jobs[wfa_jobs[i]];
-                const int32_t *r = &wres1[i * kira::kWfaRes];
-                j.retry = -1;
-                if (!r[0]) continue;
-                j.stage = 2;
-                j.loss = j.first_loss = r[1];
-                j.text_start = r[2];
-                j.text_end = r[3];
-                j.n_ops = r[4];
-                j.ops = wcig1.data() + woff1[i];
-                // suspect origin (mod.rs:373-383): a leading gap, or a short
-                // leading match followed by more ops
-                const uint32_t f = j.n_ops ? j.ops[0] : 0;
-                const bool suspect =
-                    j.n_ops > 0 && ((f & 15u) != 0 || ((f >> 4) < 19 && j.n_ops > 1));
-                const int64_t lead = std::min<int64_t>({(int64_t)std::max(kn.wfa_lead, 0), j.ws,
-                                                        (int64_t)band});
-                if (!(suspect && lead == 0 && kn.wfa_ends_free == 0)) continue;
-                if (std::min<int64_t>({(int64_t)band, 32, j.ws}) <= 0) continue;
-                j.retry = (int32_t)suspects.size();
-                suspects.push_back(wfa_jobs[i]);
-            }
+            parallel_filter((int64_t)m, 4096, [&](int64_t i) { return is_suspect[i] != 0; },
+                            suspects);
+            parallel_chunks((int64_t)suspects.size(), 4096, [&](int, int64_t b0, int64_t e0) {
+                for (int64_t k = b0; k < e0; k++) {
+                    suspects[k] = wfa_jobs[suspects[k]];
+                    jobs[suspects[k]].retry = (int32_t)k;
+                }
+            });
             trace_mark(".wfa_suspects");
             // the origin probe (mod.rs:389-399): the longest exact read prefix
             // among the starts within the slack

@@ -629,6 +629,26 @@ void parallel_chunks(int64_t n, int64_t grain, F &&f) {
         if (failed[t]) throw std::bad_alloc();
 }
 
+// The indices i in [0, n) with keep(i), in increasing order, into out:
+// counted, then placed, in parallel chunks (keep is called twice per index).
+template <class P>
+void parallel_filter(int64_t n, int64_t grain, P &&keep, std::vector<int32_t> &out) {
+    const int T = host_threads();
+    std::vector<int64_t> cnt((size_t)T + 1, 0);
+    parallel_chunks(n, grain, [&](int t, int64_t b, int64_t e) {
+        int64_t c = 0;
+        for (int64_t i = b; i < e; i++) c += keep(i) ? 1 : 0;
+        cnt[(size_t)t + 1] = c;
+    });
+    for (int t = 0; t < T; t++) cnt[(size_t)t + 1] += cnt[(size_t)t];
+    out.resize((size_t)cnt[(size_t)T]);
+    parallel_chunks(n, grain, [&](int t, int64_t b, int64_t e) {
+        int64_t o = cnt[(size_t)t];
+        for (int64_t i = b; i < e; i++)
+            if (keep(i)) out[(size_t)o++] = (int32_t)i;
+    });
+}
+
 // Wall-clock marks of the steps of one entry-point call, printed to stderr
 // when the environment variable KIRA_TRACE is set (a diagnostic: which host
 // step or synchronisation a call's time goes to).  A StepTrace on the stack

@@ -479,9 +479,9 @@ kira_status Driver::run() {
     std::vector<uint32_t> tt_cig;
     std::string tt_md;
     {
-        std::vector<int32_t> fk, frl;
-        std::vector<int64_t> fro;
-        std::vector<kira_chain> fch;
+        kira::PodVec<int32_t> fk, frl;
+        kira::PodVec<int64_t> fro;
+        kira::PodVec<kira_chain> fch;
         // exact copies get their score; the others are listed for the
         // cascade in candidate order (count per range, then place)
         const int64_t nc = (int64_t)cand.size();
