@@ -186,8 +186,9 @@ __global__ __launch_bounds__(64) void group_dp_kernel(GroupLaunch a) {
 // strip t and computes its row i at step i + t, taking the row's left
 // boundary, read byte and the row statistics of strips < t from lane t - 1
 // (which computed them at the previous step) by a one-lane shift.  The last
-// strip's lane writes the row statistics; the trace is kept per job by step,
-// so the G lanes of a job write adjacent words at each step: word
+// strip's lane writes the row statistics ([row][slot] per packet, as the
+// one-lane form does); the trace is kept per job by step, so the
+// G lanes of a job write adjacent words at each step: word
 // ((slot * (rows + G - 1) + s - 1) * words + 2 t + half) of the packet.
 // A task is (packet, first slot) of 64 / G jobs.
 template <bool KEYED>
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(64) void group_dp_sys_kernel(GroupLaunch a) {
             for (int c = 0; c < kStrip; c++)
                 if (j0 + c <= W) rb[c] = ref[j0 - 1 + c];
         }
-        int2 *rs = a.rstat + a.packet_row_off[p] + l;
+        int2 *rs = a.rstat + a.packet_row_off[p] + (l < 64 ? l : 0);
         if (job >= 0 && nst == 0 && t == 0)  // an empty window: no positive cell in any row
             for (int32_t i = 1; i <= L; i++) rs[(int64_t)(i - 1) * 64] = make_int2(0, 0);
         const bool live = job >= 0 && t < nst;
@@ -286,6 +287,12 @@ __global__ __launch_bounds__(64) void group_dp_sys_kernel(GroupLaunch a) {
     }
 }
 
+// Row i's statistics of the job in slot `lane` of packet p ([row][slot]).
+__host__ __device__ __forceinline__ const int2 *row_stat(const GroupLaunch &a, int32_t p,
+                                                        int32_t lane, int32_t i) {
+    return a.rstat + a.packet_row_off[p] + (int64_t)(i - 1) * 64 + lane;
+}
+
 // First row after which the lane is done (abort bound, mod.rs:2058-2082);
 // L when it never is.  The group stops at the maximum over its lanes.
 __host__ __device__ __forceinline__ int32_t lane_done_row(const GroupLaunch &a, int32_t job,
@@ -293,13 +300,12 @@ __host__ __device__ __forceinline__ int32_t lane_done_row(const GroupLaunch &a, 
     const int32_t L = a.read_len[job];
     const int32_t pl = a.job_slot[job];
     const int32_t p = pl >> 6, lane = pl & 63;
-    const int2 *rs = a.rstat + a.packet_row_off[p];
     const int32_t ab = a.abort_score[job];
     done = false;
     if (ab <= INT32_MIN / 8) return L;
     int32_t best = 0;
     for (int32_t i = 1; i <= L; i++) {
-        best = gmax(best, rs[(int64_t)(i - 1) * 64 + lane].x);
+        best = gmax(best, row_stat(a, p, lane, i)->x);
         if (best + (L - i) * a.match_score < ab) {
             done = true;
             return i;
@@ -308,19 +314,22 @@ __host__ __device__ __forceinline__ int32_t lane_done_row(const GroupLaunch &a, 
     return L;
 }
 
+// One thread per packet slot (n = packets x 64), so a wave reads each row's
+// statistics of 64 adjacent slots at once.
 __global__ void group_stop_kernel(GroupLaunch a, int32_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    const int32_t job = a.packet_jobs[t];
+    if (job < 0) return;
     bool done;
-    int32_t r = lane_done_row(a, (int32_t)t, done);
-    atomicMax(&a.group_stop[a.group_of[t]], r);
+    int32_t r = lane_done_row(a, job, done);
+    atomicMax(&a.group_stop[a.group_of[job]], r);
 }
 
 __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     const int32_t L = a.read_len[job];
     const int32_t pl = a.job_slot[job];
     const int32_t p = pl >> 6, lane = pl & 63;
-    const int2 *rs = a.rstat + a.packet_row_off[p];
     const uint32_t *trace = a.trace + a.packet_trace_off[p];
     const int32_t words = a.packet_words[p];
     const int32_t S = a.group_stop[a.group_of[job]];
@@ -329,7 +338,7 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     (void)dr;
     int32_t best = 0, bi = 0, bj = 0;
     for (int32_t i = 1; i <= S; i++) {
-        int2 v = rs[(int64_t)(i - 1) * 64 + lane];
+        int2 v = *row_stat(a, p, lane, i);
         if (v.x > best) {
             best = v.x;
             bi = i;
@@ -338,7 +347,7 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     }
     int32_t bq = 0, bqj = 0;
     if (S == L && L > 0) {
-        int2 v = rs[(int64_t)(L - 1) * 64 + lane];
+        int2 v = *row_stat(a, p, lane, L);
         bq = v.x;
         bqj = v.y;
     }
@@ -415,9 +424,12 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     res[9] = 0;
 }
 
+// One thread per packet slot, as group_stop_kernel.
 __global__ void group_finish_kernel(GroupLaunch a, int32_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) group_finish_job(a, (int32_t)t);
+    if (t >= n) return;
+    const int32_t job = a.packet_jobs[t];
+    if (job >= 0) group_finish_job(a, job);
 }
 
 __global__ void group_reset_kernel(int32_t *counter, int32_t *group_stop, int32_t n_groups) {
@@ -441,8 +453,9 @@ hipError_t launch_group_sw(const GroupLaunch &a, int32_t n_jobs, int32_t n_group
         group_dp_kernel<true><<<n_waves, 64, 0, s>>>(a);
     else
         group_dp_kernel<false><<<n_waves, 64, 0, s>>>(a);
-    group_stop_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
-    group_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, s>>>(a, n_jobs);
+    const int32_t n_slots = a.n_packets * 64;
+    group_stop_kernel<<<(n_slots + 255) / 256, 256, 0, s>>>(a, n_slots);
+    group_finish_kernel<<<(n_slots + 255) / 256, 256, 0, s>>>(a, n_slots);
     return hipGetLastError();
 }
 
