@@ -294,6 +294,12 @@ __device__ __forceinline__ Desc load_desc(const int4 *descs, int32_t s, int laye
                 __builtin_amdgcn_readfirstlane(v.z), 0};
 }
 
+// The forward pass reads the descriptors of the last max(x, o + ie, o + de)
+// scores; with a look-back below kDescRing they also live in an LDS ring
+// (slot s % kDescRing), so a score step does not wait on a global round trip
+// for them.  The global copies stay for the traceback.
+constexpr int32_t kDescRing = 32;
+
 // Four bytes of an LDS byte string at byte position pos (two aligned words).
 __device__ __forceinline__ uint32_t ld4(const uint32_t *w, int32_t pos) {
     const int32_t q = pos >> 2;
@@ -453,10 +459,19 @@ __device__ int32_t wfa_traceback(const uint32_t *P, int32_t m, const uint32_t *T
 __global__ __launch_bounds__(64) void wfa_kernel(WfaArgs a) {
     extern __shared__ uint32_t lds[];
     __shared__ int32_t job_sh;
+    __shared__ int4 dring[kDescRing * 3];
     const int lane = threadIdx.x;
     int32_t *vals = a.slab + (int64_t)blockIdx.x * a.slot_ints;
     int4 *descs = (int4 *)(vals + a.val_cap);
     const Pen pen = a.pen;
+    const bool ring = max(pen.x, max(pen.o + pen.ie, pen.o + pen.de)) < kDescRing;
+    auto desc_at = [&](int32_t ss, int layer) -> Desc {
+        if (!ring) return load_desc(descs, ss, layer);
+        if (ss < 0) return Desc{0, -1, 0, 0};
+        const int4 v = dring[(ss & (kDescRing - 1)) * 3 + layer];
+        return Desc{__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                    __builtin_amdgcn_readfirstlane(v.z), 0};
+    };
     for (;;) {
         __syncthreads();
         if (lane == 0) job_sh = atomicAdd(a.queue, 1);
@@ -503,11 +518,11 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaArgs a) {
                 dm = Desc{0, tbf, 0, 0};
                 di = dd = Desc{0, -1, 0, 0};
             } else {
-                mx = load_desc(descs, s - pen.x, 0);
-                moi = load_desc(descs, s - pen.o - pen.ie, 0);
-                mod = load_desc(descs, s - pen.o - pen.de, 0);
-                ie = load_desc(descs, s - pen.ie, 1);
-                de = load_desc(descs, s - pen.de, 2);
+                mx = desc_at(s - pen.x, 0);
+                moi = desc_at(s - pen.o - pen.ie, 0);
+                mod = desc_at(s - pen.o - pen.de, 0);
+                ie = desc_at(s - pen.ie, 1);
+                de = desc_at(s - pen.de, 2);
                 int32_t ilo = INT32_MAX, ihi = INT32_MIN, dlo = INT32_MAX, dhi = INT32_MIN;
                 int32_t mlo = INT32_MAX, mhi = INT32_MIN;
                 if (moi.hi >= moi.lo) { ilo = min(ilo, moi.lo - 1); ihi = max(ihi, moi.hi - 1); }
@@ -618,6 +633,12 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaArgs a) {
                 descs[s * 3 + 0] = make_int4(dm.lo, dm.hi, dm.org, 0);
                 descs[s * 3 + 1] = make_int4(di.lo, di.hi, di.org, 0);
                 descs[s * 3 + 2] = make_int4(dd.lo, dd.hi, dd.org, 0);
+                if (ring) {
+                    int4 *r = dring + (s & (kDescRing - 1)) * 3;
+                    r[0] = make_int4(dm.lo, dm.hi, dm.org, 0);
+                    r[1] = make_int4(di.lo, di.hi, di.org, 0);
+                    r[2] = make_int4(dd.lo, dd.hi, dd.org, 0);
+                }
             }
             // check_done (wfa.rs:348-357): the first diagonal from lo
             int32_t fd = INT32_MAX;
