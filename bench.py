@@ -383,8 +383,9 @@ def main():
                          "GiB (the reference kernel for scaling PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build targets one GPU; --gpus must be 1 "
-              "(the path shards with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU; --gpus must be 1 (batches shard across GPUs "
+              "with no exchange, each GPU holding its own copy of the packed genome)",
+              file=sys.stderr)
         sys.exit(2)
     run_bench(args)
 

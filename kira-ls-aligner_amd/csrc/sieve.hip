@@ -452,6 +452,24 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
             ts = ts < hi ? ts : hi;
             tb = tb < b ? tb : b;
         }
+        // a full tile followed by exactly one more shift (nt = 128 t + 1, as
+        // every window of L + 128 bases gives): that shift is word offset 4,
+        // bit 0 of this tile's words, ranked here (tile shift 128) instead of
+        // in a tile of its own
+        const bool tail = rem == 129;
+        if (tail) {
+            uint32_t m = 0;
+#pragma unroll
+            for (int k = 0; k < RW; k++) {
+                uint32_t d = or_xor(rl[k] ^ wl[4 + k], rh[k], wh[4 + k]);
+                if (MASKALL || k == RW - 1) d &= mk[k];
+                m = k == 0 ? __popc(d) : bcnt_acc(d, m);
+            }
+            const uint32_t r = (m << 16) | 128u;
+            const uint32_t hr = tb > r ? tb : r;
+            ts = ts < hr ? ts : hr;
+            tb = tb < r ? tb : r;
+        }
         uint32_t tbm = tb >> 16;
         uint32_t tsm = ts == 0xFFFFFFFFu ? 0xFFFFFFFFu : (ts >> 16);
         uint32_t hi = gbm > tbm ? gbm : tbm;
@@ -462,6 +480,7 @@ __device__ __forceinline__ void sieve_core(const SieveLaunch &a, const uint32_t 
             gbt = T0 + (tb & 0xFFFFu);
         }
         gsm = ns;
+        if (tail) break;
     }
 }
 
