@@ -115,36 +115,41 @@ __global__ void compact_cigar_kernel(int32_t n, const int32_t *__restrict__ res,
 }
 
 // Exclusive prefix sum of the jobs' CIGAR op counts (negative = overflow,
-// counted as 0) in one workgroup: each thread sums a contiguous range, thread
-// 0 scans the 1024 partial sums, each thread writes its range's offsets.
+// counted as 0) in one workgroup of 16 waves: wave w owns jobs
+// [n w / 16, n (w + 1) / 16) and walks them 64 at a time (lane i reads job
+// base + i, so a wave's loads are adjacent), first summing, then, after the
+// waves' sums are scanned, scanning each group of 64 across the lanes with a
+// running carry.
 __global__ __launch_bounds__(1024) void cigar_offsets_kernel(int32_t n,
                                                              const int32_t *__restrict__ res,
                                                              int64_t *__restrict__ off) {
-    __shared__ int64_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = ((int64_t)n + 1023) / 1024, lo = t * per;
-    const int64_t hi = lo + per < n ? lo + per : n;
+    __shared__ int64_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t lo = (int64_t)n * w / 16, hi = (int64_t)n * (w + 1) / 16;
+    auto count = [&](int64_t k) -> int64_t {
+        if (k >= hi) return 0;
+        const int32_t c = res[k * kira::kSwResWords + 6];
+        return c > 0 ? c : 0;
+    };
     int64_t sum = 0;
-    for (int64_t k = lo; k < hi; k++) {
-        const int32_t c = res[k * kira::kSwResWords + 6];
-        sum += c > 0 ? c : 0;
-    }
-    part[t] = sum;
+    for (int64_t base = lo; base < hi; base += 64) sum += count(base + lane);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) wsum[w] = sum;
     __syncthreads();
-    if (t == 0) {
-        int64_t acc = 0;
-        for (int i = 0; i < 1024; i++) {
-            const int64_t v = part[i];
-            part[i] = acc;
-            acc += v;
+    int64_t carry = 0;
+    for (int i = 0; i < w; i++) carry += wsum[i];
+    for (int64_t base = lo; base < hi; base += 64) {
+        const int64_t k = base + lane;
+        const int64_t c = count(k);
+        int64_t x = c;  // inclusive scan of the group across the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
         }
-    }
-    __syncthreads();
-    int64_t acc = part[t];
-    for (int64_t k = lo; k < hi; k++) {
-        off[k] = acc;
-        const int32_t c = res[k * kira::kSwResWords + 6];
-        acc += c > 0 ? c : 0;
+        if (k < hi) off[k] = carry + x - c;
+        carry += __shfl(x, 63, 64);
     }
 }
 
