@@ -175,58 +175,85 @@ __global__ void pack_reads_ascii_kernel(int32_t n, const uint8_t *__restrict__ r
     }
 }
 
-// Same for row-major reads with a stride that is a multiple of 32 and a
-// 16-byte aligned base: each thread fetches its 32 bytes with two 16-byte
-// loads (a 32-byte chunk that starts below the read length lies inside the
-// row, since both are multiples of 32), instead of 32 byte loads.
+// classify() on a 32-byte chunk, four bytes at once: with u = c & 0xDF
+// (upper case), A C G T are 0x41 0x43 0x47 0x54, so code bit 1 is bit 2 of
+// the byte and code bit 0 is bit 1 ^ bit 2; bit 5 marks lower case.  Bases at
+// or past nb are cleared.
+struct ChunkPlanes {
+    uint32_t lo, hi, inv, lc;
+};
+__device__ __forceinline__ ChunkPlanes classify_chunk(uint4 v0, uint4 v1, int32_t nb) {
+    ChunkPlanes o{0, 0, 0, 0};
+    const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint32_t c = w[q], u = c & 0xDFDFDFDFu;
+        const uint32_t z = nz_bytes(u ^ 0x41414141u) & nz_bytes(u ^ 0x43434343u) &
+                           nz_bytes(u ^ 0x47474747u) & nz_bytes(u ^ 0x54545454u);
+        const uint32_t ok = (~z & 0x80808080u) >> 7;  // bit 0 of each ACGT byte
+        o.lo |= gather4(((c >> 1) ^ (c >> 2)) & ok) << (4 * q);
+        o.hi |= gather4((c >> 2) & ok) << (4 * q);
+        o.inv |= gather4(~ok & 0x01010101u) << (4 * q);
+        o.lc |= gather4((c >> 5) & ok) << (4 * q);
+    }
+    const uint32_t m = nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);  // bases past L
+    o.lo &= m;
+    o.hi &= m;
+    o.inv &= m;
+    o.lc &= m;
+    return o;
+}
+
+// Same as pack_reads_ascii_kernel for row-major reads with a stride that is a
+// positive multiple of 32 and a 16-byte aligned base.  Thread (g, k) packs
+// word k of reads g and g + h (h = ceil(n / 2)), fetching each 32-byte chunk
+// with two 16-byte loads.  A chunk that starts below the stride lies inside
+// the row (both are multiples of 32), so all loads of both reads, lengths
+// included, are in flight together (a chunk at or past the stride loads the
+// row's first chunk and is dropped by the mask), and the lengths are applied
+// afterwards as bit masks.
 __global__ void pack_reads_ascii_vec_kernel(int32_t n, const uint8_t *__restrict__ reads,
                                             int32_t stride, const int32_t *__restrict__ lens,
                                             uint32_t *__restrict__ planes,
                                             uint8_t *__restrict__ valid) {
+    const int32_t h = (int32_t)(((int64_t)n + 1) / 2);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n * 8) return;
-    int32_t r = (int32_t)(t >> 3), k = (int32_t)(t & 7);
-    int32_t L = lens[r];
-    // a length past the row width is not read past the row (the read is
-    // marked invalid)
-    const bool over = L > stride;
-    const int32_t Le = over ? stride : L;
-    uint32_t lo = 0, hi = 0, inv = 0, lc = 0;
-    if (k * 32 < Le) {
-        const uint4 *src = (const uint4 *)(reads + (int64_t)r * stride + k * 32);
-        const uint4 v0 = src[0], v1 = src[1];
-        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-        const int nb = Le - k * 32 < 32 ? Le - k * 32 : 32;
-        // classify() on four bytes at once: with u = c & 0xDF (upper case),
-        // A C G T are 0x41 0x43 0x47 0x54, so code bit 1 is bit 2 of the
-        // byte and code bit 0 is bit 1 ^ bit 2; bit 5 marks lower case
+    if (t >= (int64_t)h * 8) return;
+    const int32_t g = (int32_t)(t >> 3), k = (int32_t)(t & 7);
+    const bool two = g + h < n;
+    const int32_t r[2] = {g, two ? g + h : g};
+    const bool in_row = k * 32 < stride;
+    int32_t L[2];
+    uint4 v[2][2];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const uint32_t c = w[q], u = c & 0xDFDFDFDFu;
-            const uint32_t z = nz_bytes(u ^ 0x41414141u) & nz_bytes(u ^ 0x43434343u) &
-                               nz_bytes(u ^ 0x47474747u) & nz_bytes(u ^ 0x54545454u);
-            const uint32_t ok = (~z & 0x80808080u) >> 7;  // bit 0 of each ACGT byte
-            lo |= gather4(((c >> 1) ^ (c >> 2)) & ok) << (4 * q);
-            hi |= gather4((c >> 2) & ok) << (4 * q);
-            inv |= gather4(~ok & 0x01010101u) << (4 * q);
-            lc |= gather4((c >> 5) & ok) << (4 * q);
-        }
-        const uint32_t m = nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);  // bases past L
-        lo &= m;
-        hi &= m;
-        inv &= m;
-        lc &= m;
+    for (int x = 0; x < 2; x++) {
+        L[x] = lens[r[x]];
+        const uint4 *src =
+            (const uint4 *)(reads + (int64_t)r[x] * stride + (in_row ? k * 32 : 0));
+        v[x][0] = src[0];
+        v[x][1] = src[1];
     }
-    uint32_t *dst = planes + (int64_t)r * kReadWords;
-    dst[k] = lo;
-    dst[8 + k] = hi;
-    dst[16 + k] = lc;
-    if (valid) {
-        uint32_t f = inv != 0 ? 1u : 0u;
-        f |= __shfl_xor(f, 1, 8);
-        f |= __shfl_xor(f, 2, 8);
-        f |= __shfl_xor(f, 4, 8);
-        if (k == 0) valid[r] = (L <= kMaxReadBases && !over && f == 0) ? 1 : 0;
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        // a length past the row width is not read past the row (the read is
+        // marked invalid)
+        const bool over = L[x] > stride;
+        const int32_t Le = over ? stride : L[x];
+        const int32_t nb = in_row ? min(max(Le - k * 32, 0), 32) : 0;
+        const ChunkPlanes o = classify_chunk(v[x][0], v[x][1], nb);
+        if (x == 1 && !two) break;
+        uint32_t *dst = planes + (int64_t)r[x] * kReadWords;
+        dst[k] = o.lo;
+        dst[8 + k] = o.hi;
+        dst[16 + k] = o.lc;
+        if (valid) {
+            // 8 consecutive lanes hold one read; reduce their inv flags.
+            uint32_t f = o.inv != 0 ? 1u : 0u;
+            f |= __shfl_xor(f, 1, 8);
+            f |= __shfl_xor(f, 2, 8);
+            f |= __shfl_xor(f, 4, 8);
+            if (k == 0) valid[r[x]] = (L[x] <= kMaxReadBases && !over && f == 0) ? 1 : 0;
+        }
     }
 }
 
@@ -622,8 +649,8 @@ hipError_t launch_pack_reads_ascii(int32_t n, const uint8_t *reads, int32_t stri
                                    const int64_t *offs, const int32_t *lens, uint32_t *planes,
                                    uint8_t *valid, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (!offs && stride % 32 == 0 && ((uintptr_t)reads & 15) == 0)
-        pack_reads_ascii_vec_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(
+    if (!offs && stride > 0 && stride % 32 == 0 && ((uintptr_t)reads & 15) == 0)
+        pack_reads_ascii_vec_kernel<<<grid_for(((int64_t)n + 1) / 2 * 8, 256), 256, 0, s>>>(
             n, reads, stride, lens, planes, valid);
     else
         pack_reads_ascii_kernel<<<grid_for((int64_t)n * 8, 256), 256, 0, s>>>(

@@ -203,21 +203,23 @@ def _expected_planes(read: bytes, L: int):
     return lo + hi + lc, valid
 
 
-@pytest.mark.parametrize("stride", [264, 288])
-def test_read_packing_every_byte_value(pkg, stride):
+@pytest.mark.parametrize("stride,odd", [(264, False), (288, False), (288, True), (160, True),
+                                        (160, False)])
+def test_read_packing_every_byte_value(pkg, stride, odd):
     """All 256 byte values (upper and lower case A/C/G/T, 'n', bytes >= 0x80) through
-    both device packers (stride 288 takes the four-bytes-at-once kernel), plus reads
-    longer than the row, which must come back invalid."""
+    both device packers (strides 288 and 160 take the four-bytes-at-once kernel, which
+    packs two reads a thread; 160 leaves word slots past the row, an odd count a thread
+    with one read), plus reads longer than the row, which must come back invalid."""
     import torch
-    rng = np.random.default_rng(stride)
+    rng = np.random.default_rng(stride + odd)
     alphabet = np.frombuffer(b"ACGTacgtnN", np.uint8)
     reads, lens = [], []
-    for k in range(256):
-        L = int(rng.integers(1, 257))
+    for k in range(256 + odd):
+        L = int(rng.integers(1, min(257, stride + 1)))
         row = rng.choice(alphabet, size=L) if k % 2 else \
             rng.choice(np.frombuffer(b"ACGTacgt", np.uint8), size=L)
         row = row.copy()
-        row[k % L] = k  # byte value k somewhere in read k
+        row[k % L] = k % 256  # byte value k somewhere in read k
         reads.append(bytes(row))
         lens.append(L)
     for L in (stride + 1, stride + 40):  # longer than the row: clamped, invalid
@@ -239,9 +241,9 @@ def test_read_packing_every_byte_value(pkg, stride):
     torch.cuda.synchronize()
     planes = d_planes.cpu().numpy().view(np.uint32)
     valid = d_valid.cpu().numpy()
-    for i in range(256):
+    for i in range(256 + odd):
         want, ok = _expected_planes(reads[i], lens[i])
         assert planes[i].tolist() == want, i
         assert valid[i] == int(ok), i
-    assert valid[256:].tolist() == [0, 0]
+    assert valid[256 + odd:].tolist() == [0, 0]
     ctx.close()
