@@ -222,7 +222,7 @@ struct Driver {
     template <class F>
     void build_at(size_t a0, int64_t n, F &&f) {
         if (n <= 0) return;
-        parallel_chunks(n, 1024, [&](int t, int64_t b, int64_t e) {
+        parallel_chunks(n, 256, [&](int t, int64_t b, int64_t e) {
             Arena &A = *arenas[a0 + (size_t)t];
             // about one record of a few CIGAR ops and MD characters an item
             A.read.reserve((size_t)(e - b));
@@ -391,7 +391,7 @@ kira_status Driver::run() {
         return fail(ctx, KIRA_ERR_OUT_OF_MEMORY, "pinned read staging", e);
     uint8_t comp[256];
     for (int b = 0; b < 256; b++) comp[b] = comp_base((uint8_t)b);
-    parallel_chunks(n, 4096, [&](int, int64_t b, int64_t e) {
+    parallel_chunks(n, 512, [&](int, int64_t b, int64_t e) {
         for (int64_t r = b; r < e; r++) {
             const uint8_t *src = reads + read_off[r];
             uint8_t *f = orient.data() + ooff[r];
@@ -430,7 +430,7 @@ kira_status Driver::run() {
     // candidates (cand_off[r + 1]); then, after the prefix sum, the candidates
     std::vector<int64_t> cand_off(n + 1, 0);
     const int32_t kmax = c.twotier_k >= 2 ? c.twotier_k : 5;
-    parallel_chunks(n, 4096, [&](int, int64_t rb, int64_t re) {
+    parallel_chunks(n, 1024, [&](int, int64_t rb, int64_t re) {
         for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
             const int64_t c0 = chain_off[r], nc = chain_off[r + 1] - c0;
             if (nc == 0) continue;
@@ -462,7 +462,7 @@ kira_status Driver::run() {
     st.chains_total += chain_off[n] - chain_off[0];
     for (int32_t r = 0; r < n; r++) cand_off[r + 1] += cand_off[r];
     cand.resize((size_t)cand_off[n]);
-    parallel_chunks(n, 4096, [&](int, int64_t rb, int64_t re) {
+    parallel_chunks(n, 1024, [&](int, int64_t rb, int64_t re) {
         for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
             int64_t o = cand_off[r];
             if (o == cand_off[r + 1]) continue;
@@ -485,7 +485,7 @@ kira_status Driver::run() {
         // exact copies get their score; the others are listed for the
         // cascade in candidate order (count per range, then place)
         const int64_t nc = (int64_t)cand.size();
-        const int64_t kGrain = 8192;
+        const int64_t kGrain = 1024;
         const int T = host_threads();
         const int Tn = (int)std::max<int64_t>(1, std::min<int64_t>(T, (nc + kGrain - 1) / kGrain));
         std::vector<int64_t> cnt((size_t)T + 1, 0);
@@ -544,7 +544,7 @@ kira_status Driver::run() {
     // each read selects; threads take read ranges and their selections are
     // joined in read order
     std::vector<std::vector<Sel>> part((size_t)host_threads());
-    parallel_chunks(n, 4096, [&](int t, int64_t rb, int64_t re) {
+    parallel_chunks(n, 512, [&](int t, int64_t rb, int64_t re) {
         std::vector<Sel> &out = part[(size_t)t];
         // per-read scratch, reused across reads
         std::vector<int64_t> order, idx;
@@ -665,7 +665,7 @@ kira_status Driver::run() {
         std::vector<int64_t> ex_ro, ex_pos;
         std::vector<int32_t> ex_rl, ex_sel;
         std::vector<uint8_t> is_ex((size_t)ns);
-        parallel_chunks(ns, 16384, [&](int, int64_t b0, int64_t e0) {
+        parallel_chunks(ns, 2048, [&](int, int64_t b0, int64_t e0) {
             for (int64_t k = b0; k < e0; k++) {
                 const Sel &x = sels[k];
                 const kira_chain &ch = chains[x.chain];
@@ -774,7 +774,7 @@ kira_status Driver::run() {
             int64_t reason[10] = {};
         };
         std::vector<Cnt> cnt((size_t)host_threads());
-        parallel_chunks(ns, 8192, [&](int t, int64_t b0, int64_t e0) {
+        parallel_chunks(ns, 1024, [&](int t, int64_t b0, int64_t e0) {
             Cnt &C = cnt[(size_t)t];
             for (int64_t k = b0; k < e0; k++) {
                 const Sel &x = sels[k];
@@ -861,7 +861,7 @@ kira_status Driver::run() {
         JobVec job_of(ns);
         const int T = host_threads();
         std::vector<std::array<int64_t, 4>> cnt((size_t)T, std::array<int64_t, 4>{});
-        const int64_t kGrain = 8192;
+        const int64_t kGrain = 1024;
         const int Tn = (int)std::max<int64_t>(1, std::min<int64_t>(T, (ns + kGrain - 1) / kGrain));
         parallel_chunks(ns, kGrain, [&](int t, int64_t b0, int64_t e0) {
             for (int64_t k = b0; k < e0; k++) {
@@ -1006,7 +1006,7 @@ kira_status Driver::run() {
         int32_t *rl = (int32_t *)(hb.data() + o_rl), *wl = (int32_t *)(hb.data() + o_wl),
                 *lim = (int32_t *)(hb.data() + o_lim);
         std::vector<int32_t> part_max((size_t)host_threads(), 0);
-        parallel_chunks(nj, 16384, [&](int tt, int64_t b0, int64_t e0) {
+        parallel_chunks(nj, 2048, [&](int tt, int64_t b0, int64_t e0) {
             int32_t mx = 0;
             for (int64_t t = b0; t < e0; t++) {
                 const Job &j = simd[t];
@@ -1288,7 +1288,7 @@ kira_status Driver::run() {
     // (:1215-1250), on the reads with exactly one record before the
     // supplementary ones are added
     auto assign_xs = [&]() {
-        parallel_chunks(n, 16384, [&](int, int64_t rb, int64_t re) {
+        parallel_chunks(n, 2048, [&](int, int64_t rb, int64_t re) {
             for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
                 if (first[r + 1] - first[r] != 1) continue;
                 if (has_comp[r]) {
@@ -1324,7 +1324,7 @@ kira_status Driver::run() {
             JobVec sj;
         };
         std::vector<Part> parts((size_t)host_threads());
-        parallel_chunks(n, 4096, [&](int t, int64_t rb, int64_t re) {
+        parallel_chunks(n, 512, [&](int t, int64_t rb, int64_t re) {
             Part &P = parts[(size_t)t];
             std::vector<int64_t> picks;
             for (int32_t r = (int32_t)rb; r < (int32_t)re; r++) {
@@ -1431,7 +1431,7 @@ kira_status Driver::run() {
         for (int32_t r = 0; r < n; r++) sfirst[r + 1] += sfirst[r];
         std::vector<int64_t> first2((size_t)n + 1), perm2(perm.size() + supp_h.size());
         for (int32_t r = 0; r <= n; r++) first2[r] = first[r] + sfirst[r];
-        parallel_chunks(n, 16384, [&](int, int64_t rb, int64_t re) {
+        parallel_chunks(n, 2048, [&](int, int64_t rb, int64_t re) {
             for (int64_t r = rb; r < re; r++) {
                 int64_t o = first2[r];
                 for (int64_t k = first[r]; k < first[r + 1]; k++) perm2[o++] = perm[k];
@@ -1467,7 +1467,7 @@ kira_status Driver::run() {
             }
         });
         R.alns.resize(perm.size());
-        parallel_chunks((int64_t)perm.size(), 16384, [&](int, int64_t b, int64_t e) {
+        parallel_chunks((int64_t)perm.size(), 2048, [&](int, int64_t b, int64_t e) {
             for (int64_t k = b; k < e; k++) {
                 const int64_t h = perm[k];
                 kira_stage4_aln a = rec(h);
