@@ -365,7 +365,11 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     // the systolic layout: this job's words by step (step = row + strip)
     const int32_t G = words >> 1 > 0 ? words >> 1 : 1;
     const uint32_t *jtr = trace + (int64_t)lane * (a.packet_rows[p] + G - 1) * words;
+    // the current op's run length stays in a register and is written once,
+    // when the op changes or the walk ends (an extension written back to
+    // the CIGAR would put a global load and store on every step's path)
     int32_t i = si, j = sj, n = 0, last = -1, state = 0;
+    uint32_t run = 0;
     while (i > 0 && j > 0) {
         int32_t c = j - 1;
         const int32_t st = c / kStrip, half = (c % kStrip) >> 3;
@@ -397,14 +401,16 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
         }
         if (op >= 0) {
             if (op == last) {
-                if (n - 1 < cap) cig[n - 1] += 16u;
+                run++;
             } else {
-                if (n < cap) cig[n] = (1u << 4) | (uint32_t)op;
+                if (last >= 0 && n - 1 < cap) cig[n - 1] = (run << 4) | (uint32_t)last;
                 n++;
                 last = op;
+                run = 1;
             }
         }
     }
+    if (last >= 0 && n - 1 < cap) cig[n - 1] = (run << 4) | (uint32_t)last;
     int32_t m = n < cap ? n : cap;
     for (int32_t x = 0, y = m - 1; x < y; x++, y--) {
         uint32_t tmp = cig[x];
