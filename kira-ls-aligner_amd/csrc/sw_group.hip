@@ -336,12 +336,27 @@ __host__ __device__ void group_finish_job(const GroupLaunch &a, int32_t job) {
     bool done;
     int32_t dr = lane_done_row(a, job, done);
     (void)dr;
+    // best row (the first of the maximum) over rows 1..S: eight rows' loads
+    // are issued before their compares, which keep row order
     int32_t best = 0, bi = 0, bj = 0;
-    for (int32_t i = 1; i <= S; i++) {
-        int2 v = *row_stat(a, p, lane, i);
+    int32_t r0 = 1;
+    for (; r0 + 7 <= S; r0 += 8) {
+        int2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = *row_stat(a, p, lane, r0 + u);
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (v[u].x > best) {
+                best = v[u].x;
+                bi = r0 + u;
+                bj = v[u].y;
+            }
+    }
+    for (; r0 <= S; r0++) {
+        int2 v = *row_stat(a, p, lane, r0);
         if (v.x > best) {
             best = v.x;
-            bi = i;
+            bi = r0;
             bj = v.y;
         }
     }
